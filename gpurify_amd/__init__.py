@@ -1,0 +1,13 @@
+"""gpurify_amd: MI355X (gfx950) quasi-static capacitance engine.
+
+The compute path is libgpurify.so (hand-written HIP kernels for gfx950 behind
+the C ABI of include/gpurify.h).  This package holds its ctypes binding and
+the seeded synthetic layouts used by the tests and the benchmark.  Importing
+a solve without the built library raises; there is no CPU fallback.
+"""
+from ._lib import (  # noqa: F401
+    Context, GpError, default_opts, device_ready, load, mesh_groups, version,
+    GP_PREC_FP64, GP_PREC_MIXED, GP_METHOD_AUTO, GP_METHOD_DENSE, GP_METHOD_FMM,
+    GP_M2L_FP64, GP_M2L_FP32, GP_M2L_BF16, LIB_PATH,
+)
+from . import workloads  # noqa: F401

@@ -1,0 +1,334 @@
+// C ABI of libgpurify (include/gpurify.h).  Every entry point catches all
+// C++ exceptions and turns them into a status code plus a message.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/gpurify.h"
+#include "gp_bridge.h"
+#include "gp_common.h"
+#include "gp_solver.h"
+#include "gp_tree.h"
+
+namespace gp {
+
+size_t &device_bytes_in_use() {
+  static size_t used = 0;
+  return used;
+}
+
+Prof *&active_prof() {
+  static thread_local Prof *p = nullptr;
+  return p;
+}
+
+size_t device_bytes_cap() {
+  const char *e = std::getenv("GPURIFY_MAX_DEVICE_BYTES");
+  if (!e || !*e) return 0;
+  return (size_t)std::strtoull(e, nullptr, 10);
+}
+
+}  // namespace gp
+
+namespace {
+
+thread_local std::string g_thread_error = "";
+
+bool force_no_device() {
+  const char *e = std::getenv("GPURIFY_FORCE_NO_DEVICE");
+  return e && e[0] == '1';
+}
+
+// First device whose arch is gfx950, or -1.
+int find_gfx950(int ordinal, std::string *name, int *cus) {
+  if (force_no_device()) return -1;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return -1;
+  if (ordinal < 0 || ordinal >= count) return -1;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ordinal) != hipSuccess) return -1;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return -1;
+  if (name) *name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+  if (cus) *cus = prop.multiProcessorCount;
+  return ordinal;
+}
+
+struct ActiveProf {
+  explicit ActiveProf(gp_ctx *c) { gp::active_prof() = c ? &c->prof : nullptr; }
+  ~ActiveProf() { gp::active_prof() = nullptr; }
+};
+
+template <typename F>
+int guarded(gp_ctx *ctx, F &&f) {
+  ActiveProf ap(ctx);
+  try {
+    f();
+    if (ctx) ctx->last_error.clear();
+    return GP_OK;
+  } catch (const gp::Error &e) {
+    (ctx ? ctx->last_error : g_thread_error) = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    (ctx ? ctx->last_error : g_thread_error) = "host allocation failed";
+    return GP_ERR_OOM;
+  } catch (const std::exception &e) {
+    (ctx ? ctx->last_error : g_thread_error) = std::string("internal error: ") + e.what();
+    return GP_ERR_INTERNAL;
+  } catch (...) {
+    (ctx ? ctx->last_error : g_thread_error) = "internal error";
+    return GP_ERR_INTERNAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gp_version(void) { return "gpurify-mi355x 0.1.0 (gfx950)"; }
+
+int gp_device_ready(void) { return find_gfx950(0, nullptr, nullptr) >= 0 ? 1 : 0; }
+
+int gp_ctx_create(gp_ctx **out, uint32_t device_ordinal) {
+  if (!out) {
+    g_thread_error = "gp_ctx_create: out is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    std::string name;
+    int cus = 0;
+    int dev = find_gfx950((int)device_ordinal, &name, &cus);
+    if (dev < 0) throw gp::Error(GP_ERR_NO_DEVICE, "no usable gfx950 device");
+    GP_HIP(hipSetDevice(dev));
+    gp_ctx *c = new gp_ctx();
+    c->device = dev;
+    c->device_name = name;
+    c->cu_count = cus;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      gp::hip_check(e, "hipStreamCreateWithFlags");
+    }
+    *out = c;
+  });
+}
+
+void gp_ctx_destroy(gp_ctx *ctx) {
+  if (!ctx) return;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamDestroy(ctx->stream);
+    }
+  }
+  delete ctx;
+}
+
+const char *gp_last_error(const gp_ctx *ctx) {
+  return ctx ? ctx->last_error.c_str() : g_thread_error.c_str();
+}
+
+int gp_device_info(gp_ctx *ctx, char *buf, size_t len) {
+  if (!ctx || !buf || len == 0) return GP_ERR_INVALID_INPUT;
+  std::snprintf(buf, len, "%s, %d CUs, device %d", ctx->device_name.c_str(), ctx->cu_count,
+                ctx->device);
+  return GP_OK;
+}
+
+void gp_cap_opts_default(gp_cap_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->precision = GP_PREC_FP64;
+  o->method = GP_METHOD_AUTO;
+  o->fmm_order = 4;      // bridge.rs:403
+  o->max_level = 4;      // bridge.rs:403
+  o->leaf_target = 64;
+  o->restart = 200;      // cap/fmm_solver.rs:164
+  o->max_restarts = 50;  // cap/fmm_solver.rs:164
+  o->rhs_block = 0;
+  o->tol = 1.0e-8;       // bridge.rs:403
+  o->inner_tol = 1.0e-5;
+  o->max_refine = 8;
+  o->m2l_inner = GP_M2L_FP32;
+  o->m2l_outer = GP_M2L_FP64;
+  o->strict = 0;
+  o->dense_limit = 500;  // bridge.rs:30
+  o->rhs_first = 0;
+  o->rhs_count = 0;
+}
+
+int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,
+                 gp_cap_report *report) {
+  if (!ctx) {
+    g_thread_error = "gp_cap_solve: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    if (!panels || !C) throw gp::Error(GP_ERR_INVALID_INPUT, "null panels or C");
+    gp_cap_opts o;
+    if (opts)
+      o = *opts;
+    else
+      gp_cap_opts_default(&o);
+    GP_HIP(hipSetDevice(ctx->device));
+    gp::PanelGeom g;
+    gp::panel_geometry(*panels, g);
+    gp::cap_solve(ctx, g, o, C, report, nullptr);
+  });
+}
+
+int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, uint32_t nr,
+                 uint32_t precision, const double *X, double *Y) {
+  if (!ctx) {
+    g_thread_error = "gp_fmm_apply: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    if (!panels || !X || !Y || nr == 0 || nr > 256)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "gp_fmm_apply: bad arguments");
+    gp_cap_opts o;
+    if (opts)
+      o = *opts;
+    else
+      gp_cap_opts_default(&o);
+    GP_HIP(hipSetDevice(ctx->device));
+    gp::PanelGeom g;
+    gp::panel_geometry(*panels, g);
+    gp::fmm_apply_host(ctx, g, o, (int)nr, X, Y, precision == GP_PREC_MIXED ? 1 : 0);
+  });
+}
+
+int gp_profile_enable(gp_ctx *ctx, int enable) {
+  if (!ctx) return GP_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    GP_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &r : ctx->prof.recs) {
+      ctx->prof.pool.push_back(r.a);
+      ctx->prof.pool.push_back(r.b);
+    }
+    ctx->prof.recs.clear();
+    ctx->prof.on = enable != 0;
+  });
+}
+
+int gp_profile_report(gp_ctx *ctx, char *json, size_t len) {
+  if (!ctx || !json || len == 0) return GP_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    GP_HIP(hipStreamSynchronize(ctx->stream));
+    struct Agg {
+      long n = 0;
+      double sec = 0, flops = 0, bytes = 0;
+    };
+    std::vector<std::pair<std::string, Agg>> agg;
+    for (auto &r : ctx->prof.recs) {
+      float ms = 0.f;
+      GP_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      auto it = std::find_if(agg.begin(), agg.end(), [&](auto &kv) { return kv.first == r.name; });
+      if (it == agg.end()) {
+        agg.push_back({r.name, Agg()});
+        it = agg.end() - 1;
+      }
+      it->second.n += 1;
+      it->second.sec += ms * 1e-3;
+      it->second.flops += r.flops;
+      it->second.bytes += r.bytes;
+      ctx->prof.pool.push_back(r.a);
+      ctx->prof.pool.push_back(r.b);
+    }
+    ctx->prof.recs.clear();
+    std::string out = "{";
+    char buf[512];
+    for (size_t i = 0; i < agg.size(); ++i) {
+      std::snprintf(buf, sizeof(buf), "%s\"%s\": {\"launches\": %ld, \"seconds\": %.9g, \"flops\": %.9g, \"bytes\": %.9g}",
+                    i ? ", " : "", agg[i].first.c_str(), agg[i].second.n, agg[i].second.sec,
+                    agg[i].second.flops, agg[i].second.bytes);
+      out += buf;
+    }
+    out += "}";
+    if (out.size() + 1 > len) throw gp::Error(GP_ERR_INVALID_INPUT, "profile report buffer too small");
+    std::memcpy(json, out.c_str(), out.size() + 1);
+  });
+}
+
+int gp_mesh_groups(const gp_volume_group *groups, uint32_t n_groups, double xy_scale,
+                   uint32_t refine, uint32_t max_panels, uint32_t *n_panels, double *vertices,
+                   uint32_t *conductor, double *panel_eps) {
+  return guarded(nullptr, [&] {
+    gp::Meshed m;
+    gp::mesh_groups(groups, n_groups, xy_scale, (int)refine, max_panels, m);
+    size_t n = m.cond.size();
+    if (n_panels) *n_panels = (uint32_t)n;
+    if (vertices) std::memcpy(vertices, m.verts.data(), n * 12 * sizeof(double));
+    if (conductor) std::memcpy(conductor, m.cond.data(), n * sizeof(uint32_t));
+    if (panel_eps) std::memcpy(panel_eps, m.eps.data(), n * sizeof(double));
+  });
+}
+
+int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_groups,
+                         double xy_scale, uint32_t refine, uint32_t max_panels,
+                         const gp_cap_opts *opts, uint32_t *n_nets, uint32_t *nets_out,
+                         double *cap_af_out, double *C_out, gp_cap_report *report) {
+  if (!ctx) {
+    g_thread_error = "gp_solve_capacitance: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    gp_cap_opts o;
+    if (opts)
+      o = *opts;
+    else
+      gp_cap_opts_default(&o);
+    GP_HIP(hipSetDevice(ctx->device));
+    gp::Meshed m;
+    gp::mesh_groups(groups, n_groups, xy_scale, (int)refine, max_panels, m);
+    const int nn = (int)m.nets.size(), nc = nn + 1;
+    gp_panels p{(uint32_t)m.cond.size(), (uint32_t)nc, m.verts.data(), m.cond.data()};
+    gp::PanelGeom g;
+    gp::panel_geometry(p, g);
+    // Homogeneity test of bridge.rs:393-396.
+    const double first = m.eps[0];
+    bool homog = true;
+    for (double e : m.eps)
+      if (std::fabs(e - first) > std::max(std::fabs(first), 1.0) * 1.0e-12) homog = false;
+    std::vector<double> C((size_t)nc * nc);
+    if (homog) {
+      gp::cap_solve(ctx, g, o, C.data(), report, nullptr);
+      for (auto &v : C) v *= first;  // bridge.rs:405-407
+    } else {
+      // Conductor-role dielectric solve (bridge.rs:409-430): dense only.
+      gp_cap_opts od = o;
+      uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
+      if (g.n >= (int)limit)
+        throw gp::Error(GP_ERR_PANEL_CAPACITY, "extruded layout needs " + std::to_string(g.n) +
+                                                   " panels; limit is " +
+                                                   std::to_string(limit - 1));
+      od.method = GP_METHOD_DENSE;
+      gp::cap_solve(ctx, g, od, C.data(), report, m.eps.data());
+    }
+    if (n_nets) *n_nets = (uint32_t)nn;
+    for (int i = 0; i < nn; ++i) {
+      double af = C[(size_t)i * nc + i] * 1.0e18;  // bridge.rs:432-441
+      if (!std::isfinite(af) || af < 0.0)
+        throw gp::Error(GP_ERR_NON_FINITE, "quasi-static solver returned a non-finite capacitance for net " +
+                                               std::to_string(m.nets[i]));
+      if (nets_out) nets_out[i] = m.nets[i];
+      if (cap_af_out) cap_af_out[i] = af;
+    }
+    if (C_out) std::memcpy(C_out, C.data(), C.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"

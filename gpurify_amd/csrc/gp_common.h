@@ -1,0 +1,155 @@
+// Internal helpers shared by the host orchestration and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gpurify.h"
+
+namespace gp {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GP_ERR_OOM
+                                                                           : GP_ERR_INTERNAL;
+    throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+#define GP_HIP(x) ::gp::hip_check((x), #x)
+#define GP_LAUNCH_CHECK() ::gp::hip_check(hipGetLastError(), "kernel launch")
+
+// Device-memory accounting: every allocation goes through DBuf so the
+// optional GPURIFY_MAX_DEVICE_BYTES cap (used by tests to force the OOM
+// path from the host) is enforced in one place.
+size_t &device_bytes_in_use();
+size_t device_bytes_cap();
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    size_t bytes = count * sizeof(T);
+    size_t cap = device_bytes_cap();
+    if (cap && device_bytes_in_use() + bytes > cap)
+      throw Error(GP_ERR_OOM, "device allocation of " + std::to_string(bytes) +
+                                  " bytes exceeds GPURIFY_MAX_DEVICE_BYTES");
+    GP_HIP(hipMalloc((void **)&p, bytes));
+    n = count;
+    device_bytes_in_use() += bytes;
+  }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      device_bytes_in_use() -= n * sizeof(T);
+    }
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+  void upload(const std::vector<T> &h, hipStream_t s) {
+    if (h.size() != n) alloc(h.size());
+    if (n) GP_HIP(hipMemcpyAsync(p, h.data(), bytes(), hipMemcpyHostToDevice, s));
+  }
+  void zero(hipStream_t s) {
+    if (n) GP_HIP(hipMemsetAsync(p, 0, bytes(), s));
+  }
+};
+
+template <typename T>
+DBuf<T> to_device(const std::vector<T> &h, hipStream_t s) {
+  DBuf<T> d;
+  d.upload(h, s);
+  return d;
+}
+
+// Kernel-level event profiler.  When enabled on a context, every launch
+// wrapped in a ProfScope records a HIP event pair on the launch stream plus
+// the launch's algorithmic flops and bytes; gp_profile_report() sums them per
+// kernel name.  Disabled, a ProfScope costs one branch.
+struct Prof {
+  bool on = false;
+  struct Rec {
+    const char *name;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    hipEvent_t e;
+    if (!pool.empty()) {
+      e = pool.back();
+      pool.pop_back();
+    } else {
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+    }
+    return e;
+  }
+  ~Prof() {
+    for (auto &r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+Prof *&active_prof();  // per thread, set by the API entry for its context
+
+struct ProfScope {
+  Prof *p;
+  hipStream_t s;
+  size_t idx = 0;
+  ProfScope(const char *name, hipStream_t st, double flops, double bytes) : p(active_prof()), s(st) {
+    if (!p || !p->on) {
+      p = nullptr;
+      return;
+    }
+    Prof::Rec r{name, p->get(), p->get(), flops, bytes};
+    hip_check(hipEventRecord(r.a, s), "hipEventRecord");
+    p->recs.push_back(r);
+    idx = p->recs.size() - 1;
+  }
+  ~ProfScope() {
+    if (p) (void)hipEventRecord(p->recs[idx].b, s);
+  }
+};
+
+// Physical constants (crates/pex/src/quasistatic/constants.rs:13-20).
+constexpr double kEps0 = 8.8541878128e-12;
+constexpr double kPi = 3.14159265358979323846;
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+}  // namespace gp

@@ -1,0 +1,352 @@
+// GMRES building blocks on the device (krylov.rs:50-207), FP64, for a block
+// of independent right-hand sides that iterate in lockstep.
+//
+// Vectors are N x ld row-major (RHS fastest); nr <= ld columns are live.
+// Reductions are two-stage with a fixed order (per-block partials, then one
+// thread per column summing the partials in block order), so every run gives
+// the same bits.  Per-column GMRES state (Hessenberg column, Givens
+// rotations, residual estimate, iteration count) lives in small device
+// arrays updated by one thread per column.
+
+#include "gp_common.h"
+#include "gp_kernels.h"
+
+namespace gp {
+
+constexpr int KR_BLOCKS = 512;
+
+// partial[(v * nblk + b) * ld + r] = sum over rows of block b of X_v[row][r] * Y[row][r]
+__global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ X,
+                                                     size_t strideX,
+                                                     const double *__restrict__ Y, int N, int ld,
+                                                     int nr, double *__restrict__ partial) {
+  __shared__ double red[256];
+  const int v = blockIdx.y, b = blockIdx.x, nblk = gridDim.x;
+  const double *x = X + (size_t)v * strideX;
+  const int rpp = 256 / nr;  // rows per pass
+  const int tid = threadIdx.x;
+  const int r = tid % nr, ro = tid / nr;
+  const int rows_per_block = (N + nblk - 1) / nblk;
+  const int row0 = b * rows_per_block, row1 = min(N, row0 + rows_per_block);
+  double acc = 0.0;
+  if (ro < rpp)
+    for (int row = row0 + ro; row < row1; row += rpp) acc += x[(size_t)row * ld + r] * Y[(size_t)row * ld + r];
+  red[tid] = acc;
+  __syncthreads();
+  if (tid < nr) {
+    double s = 0.0;
+    for (int j = 0; j < rpp; ++j) s += red[j * nr + tid];
+    partial[((size_t)v * nblk + b) * ld + tid] = s;
+  }
+}
+
+// out[v * ld + r] = sum_b partial[(v * nblk + b) * ld + r]  (optionally sqrt)
+__global__ void k_reduce_partial(const double *__restrict__ partial, int nblk, int nvec, int ld,
+                                 int nr, double *__restrict__ out, int take_sqrt) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nvec * nr) return;
+  int v = e / nr, r = e % nr;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[((size_t)v * nblk + b) * ld + r];
+  out[(size_t)v * ld + r] = take_sqrt ? sqrt(s) : s;
+}
+
+// w[row][r] += alpha * sum_{v < nvec} coef[v][r] * V_v[row][r]
+__global__ __launch_bounds__(256) void k_axpy_multi(double *__restrict__ w,
+                                                    const double *__restrict__ V, size_t strideV,
+                                                    const double *__restrict__ coef, int nvec,
+                                                    int N, int ld, int nr, double alpha) {
+  size_t total = (size_t)N * nr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / nr), r = (int)(e % nr);
+    size_t idx = (size_t)row * ld + r;
+    double s = 0.0;
+    for (int v = 0; v < nvec; ++v) s += coef[(size_t)v * ld + r] * V[(size_t)v * strideV + idx];
+    w[idx] += alpha * s;
+  }
+}
+
+// out[row][r] = scale[r] * in[row][r] (0 where mask[r] == 0); mask may be null.
+__global__ __launch_bounds__(256) void k_scale_cols(const double *__restrict__ in,
+                                                    double *__restrict__ out,
+                                                    const double *__restrict__ scale,
+                                                    const int *__restrict__ mask, int N, int ld,
+                                                    int nr) {
+  size_t total = (size_t)N * nr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / nr), r = (int)(e % nr);
+    size_t idx = (size_t)row * ld + r;
+    out[idx] = (mask && !mask[r]) ? 0.0 : scale[r] * in[idx];
+  }
+}
+
+// out = a - b (elementwise over live columns)
+__global__ __launch_bounds__(256) void k_sub(const double *__restrict__ a,
+                                             const double *__restrict__ b,
+                                             double *__restrict__ out, int N, int ld, int nr) {
+  size_t total = (size_t)N * nr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / nr), r = (int)(e % nr);
+    size_t idx = (size_t)row * ld + r;
+    out[idx] = a[idx] - b[idx];
+  }
+}
+
+// z[row][r] = (T)(v[row][r] * (dinv ? dinv[row] : 1)): right Jacobi
+// preconditioner (cap/fmm_solver.rs:133-144) fused with the cast to the
+// operator's working precision.
+template <typename T>
+__global__ __launch_bounds__(256) void k_precond_cast(const double *__restrict__ v,
+                                                      const double *__restrict__ dinv,
+                                                      T *__restrict__ z, int N, int ld, int nr) {
+  size_t total = (size_t)N * nr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / nr), r = (int)(e % nr);
+    size_t idx = (size_t)row * ld + r;
+    double s = dinv ? dinv[row] : 1.0;
+    z[idx] = (T)(v[idx] * s);
+  }
+}
+
+// x += alpha * y over live columns.
+__global__ __launch_bounds__(256) void k_axpy(double *__restrict__ x, const double *__restrict__ y,
+                                              double alpha, int N, int ld, int nr) {
+  size_t total = (size_t)N * nr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / nr), r = (int)(e % nr);
+    size_t idx = (size_t)row * ld + r;
+    x[idx] += alpha * y[idx];
+  }
+}
+
+// Unit-conductor right-hand sides (rhs_unit_conductor, cap/solver.rs:55-63):
+// b[p][r] = 1 where panel p belongs to conductor j0 + r, else 0 (all ld columns).
+__global__ __launch_bounds__(256) void k_unit_rhs(const uint32_t *__restrict__ cond, int j0, int N,
+                                                  int ld, int nr, double *__restrict__ b) {
+  size_t total = (size_t)N * ld;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / ld), r = (int)(e % ld);
+    b[e] = (r < nr && (int)cond[row] == j0 + r) ? 1.0 : 0.0;
+  }
+}
+
+// Givens rotation eliminating b against a (krylov.rs:195-207).
+__device__ inline void givens(double a, double b, double &c, double &s) {
+  if (b == 0.0) {
+    c = 1.0;
+    s = 0.0;
+  } else if (fabs(b) > fabs(a)) {
+    double t = a / b;
+    s = 1.0 / sqrt(1.0 + t * t);
+    c = t * s;
+  } else {
+    double t = b / a;
+    c = 1.0 / sqrt(1.0 + t * t);
+    s = t * c;
+  }
+}
+
+// Cycle start: beta = ||r||, g[0] = beta, active = beta/bnorm > tol.
+// Zeroes g[1..m]; inv_beta scales the first basis vector.
+__global__ void k_gmres_start(int m, int nr, int ld, const double *__restrict__ beta,
+                              const double *__restrict__ bnorm, double tol,
+                              const int *__restrict__ live, int *__restrict__ active,
+                              int *__restrict__ kused, double *__restrict__ g,
+                              double *__restrict__ inv_beta, double *__restrict__ est) {
+  int r = threadIdx.x + blockIdx.x * blockDim.x;
+  if (r >= nr) return;
+  double rel = beta[r] / bnorm[r];
+  est[r] = rel;
+  int a = live[r] && (rel > tol);
+  active[r] = a;
+  kused[r] = 0;
+  g[r] = beta[r];
+  for (int i = 1; i <= m; ++i) g[(size_t)i * ld + r] = 0.0;
+  inv_beta[r] = (a && beta[r] > 0.0) ? 1.0 / beta[r] : 0.0;
+}
+
+// One Arnoldi step's Hessenberg update for every active column: store the
+// new column h[0..k+1] (from Gram-Schmidt), apply the previous rotations,
+// form and apply the new one, update g and the residual estimate
+// (krylov.rs:119-141).  Columns reaching tol stop (kused frozen).
+// Also sets inv_h[r] = 1/h[k+1] (or 1 when h[k+1] <= 1e-14, krylov.rs:112-116).
+__global__ void k_gmres_givens(int k, int m, int nr, int ld, const double *__restrict__ h1,
+                               const double *__restrict__ h2, const double *__restrict__ hn,
+                               double *__restrict__ H, double *__restrict__ cs,
+                               double *__restrict__ sn, double *__restrict__ g,
+                               const double *__restrict__ bnorm, double tol,
+                               int *__restrict__ active, int *__restrict__ kused,
+                               double *__restrict__ est, double *__restrict__ inv_h) {
+  int r = threadIdx.x + blockIdx.x * blockDim.x;
+  if (r >= nr) return;
+  if (!active[r]) {
+    inv_h[r] = 0.0;
+    return;
+  }
+  double hk1 = hn[r];
+  inv_h[r] = hk1 > 1e-14 ? 1.0 / hk1 : 1.0;
+  // H[i][k] stored at ((i * m) + k) * ld + r; Gram-Schmidt coefficients of
+  // both passes summed.
+  for (int i = 0; i <= k; ++i)
+    H[((size_t)i * m + k) * ld + r] = h1[(size_t)i * ld + r] + (h2 ? h2[(size_t)i * ld + r] : 0.0);
+  for (int i = 0; i < k; ++i) {
+    double hi = H[((size_t)i * m + k) * ld + r], hi1 = H[((size_t)(i + 1) * m + k) * ld + r];
+    double c = cs[(size_t)i * ld + r], s = sn[(size_t)i * ld + r];
+    H[((size_t)i * m + k) * ld + r] = c * hi + s * hi1;
+    H[((size_t)(i + 1) * m + k) * ld + r] = -s * hi + c * hi1;
+  }
+  double hkk = H[((size_t)k * m + k) * ld + r];
+  double c, s;
+  givens(hkk, hk1, c, s);
+  cs[(size_t)k * ld + r] = c;
+  sn[(size_t)k * ld + r] = s;
+  H[((size_t)k * m + k) * ld + r] = c * hkk + s * hk1;
+  double gk = g[(size_t)k * ld + r], gk1 = g[(size_t)(k + 1) * ld + r];
+  g[(size_t)k * ld + r] = c * gk + s * gk1;
+  g[(size_t)(k + 1) * ld + r] = -s * gk + c * gk1;
+  kused[r] = k + 1;
+  double e = fabs(g[(size_t)(k + 1) * ld + r]) / bnorm[r];
+  est[r] = e;
+  if (e <= tol) active[r] = 0;
+}
+
+// y = H^-1 g over the kused[r] leading rows (krylov.rs:144-152).
+__global__ void k_gmres_backsolve(int m, int nr, int ld, const double *__restrict__ H,
+                                  const double *__restrict__ g, const int *__restrict__ kused,
+                                  double *__restrict__ Y) {
+  int r = threadIdx.x + blockIdx.x * blockDim.x;
+  if (r >= nr) return;
+  int ku = kused[r];
+  for (int i = ku - 1; i >= 0; --i) {
+    double s = g[(size_t)i * ld + r];
+    for (int j = i + 1; j < ku; ++j) s -= H[((size_t)i * m + j) * ld + r] * Y[(size_t)j * ld + r];
+    Y[(size_t)i * ld + r] = s / H[((size_t)i * m + i) * ld + r];
+  }
+  for (int i = ku; i < m; ++i) Y[(size_t)i * ld + r] = 0.0;
+}
+
+// x[row][r] += dinv[row] * sum_{i < kused[r]} Y[i][r] V_i[row][r]  (krylov.rs:154-170)
+__global__ __launch_bounds__(256) void k_gmres_update_x(double *__restrict__ x,
+                                                        const double *__restrict__ V,
+                                                        size_t strideV,
+                                                        const double *__restrict__ Y,
+                                                        const int *__restrict__ kused, int kmax,
+                                                        const double *__restrict__ dinv, int N,
+                                                        int ld, int nr) {
+  size_t total = (size_t)N * nr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / nr), r = (int)(e % nr);
+    size_t idx = (size_t)row * ld + r;
+    int ku = min(kused[r], kmax);
+    double s = 0.0;
+    for (int i = 0; i < ku; ++i) s += Y[(size_t)i * ld + r] * V[(size_t)i * strideV + idx];
+    x[idx] += (dinv ? dinv[row] : 1.0) * s;
+  }
+}
+
+namespace {
+int grid_for(size_t total) { return (int)std::min<size_t>((total + 255) / 256, 8192); }
+}  // namespace
+
+void kr_dots(const double *X, size_t strideX, int nvec, const double *Y, int N, int ld, int nr,
+             double *partial, double *out, bool take_sqrt, hipStream_t s) {
+  if (nvec <= 0) return;
+  ProfScope ps("gmres_dots", s, 2.0 * nvec * N * nr, 8.0 * (nvec + 1) * N * nr);
+  int nblk = std::min(KR_BLOCKS, std::max(1, (N + 255) / 256));
+  hipLaunchKernelGGL(k_dot_partial, dim3(nblk, nvec), dim3(256), 0, s, X, strideX, Y, N, ld, nr,
+                     partial);
+  GP_LAUNCH_CHECK();
+  int tot = nvec * nr;
+  hipLaunchKernelGGL(k_reduce_partial, dim3((tot + 255) / 256), dim3(256), 0, s, partial, nblk,
+                     nvec, ld, nr, out, take_sqrt ? 1 : 0);
+  GP_LAUNCH_CHECK();
+}
+
+size_t kr_partial_size(int nvec_max, int ld) { return (size_t)KR_BLOCKS * nvec_max * ld; }
+
+void kr_axpy_multi(double *w, const double *V, size_t strideV, const double *coef, int nvec,
+                   int N, int ld, int nr, double alpha, hipStream_t s) {
+  if (nvec <= 0) return;
+  ProfScope ps("gmres_axpy", s, 2.0 * nvec * N * nr, 8.0 * (nvec + 2) * N * nr);
+  hipLaunchKernelGGL(k_axpy_multi, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, w, V, strideV,
+                     coef, nvec, N, ld, nr, alpha);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_scale_cols(const double *in, double *out, const double *scale, const int *mask, int N,
+                   int ld, int nr, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_cols, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, in, out,
+                     scale, mask, N, ld, nr);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_sub(const double *a, const double *b, double *out, int N, int ld, int nr, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, a, b, out, N, ld, nr);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_precond_cast(const double *v, const double *dinv, double *z, int N, int ld, int nr,
+                     hipStream_t s) {
+  hipLaunchKernelGGL((k_precond_cast<double>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, v,
+                     dinv, z, N, ld, nr);
+  GP_LAUNCH_CHECK();
+}
+void kr_precond_cast(const double *v, const double *dinv, float *z, int N, int ld, int nr,
+                     hipStream_t s) {
+  hipLaunchKernelGGL((k_precond_cast<float>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, v,
+                     dinv, z, N, ld, nr);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_gmres_start(int m, int nr, int ld, const double *beta, const double *bnorm, double tol,
+                    const int *live, int *active, int *kused, double *g, double *inv_beta,
+                    double *est, hipStream_t s) {
+  hipLaunchKernelGGL(k_gmres_start, dim3((nr + 255) / 256), dim3(256), 0, s, m, nr, ld, beta, bnorm,
+                     tol, live, active, kused, g, inv_beta, est);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_gmres_givens(int k, int m, int nr, int ld, const double *h1, const double *h2,
+                     const double *hn, double *H, double *cs, double *sn, double *g,
+                     const double *bnorm, double tol, int *active, int *kused, double *est,
+                     double *inv_h, hipStream_t s) {
+  hipLaunchKernelGGL(k_gmres_givens, dim3((nr + 255) / 256), dim3(256), 0, s, k, m, nr, ld, h1, h2,
+                     hn, H, cs, sn, g, bnorm, tol, active, kused, est, inv_h);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_gmres_backsolve(int m, int nr, int ld, const double *H, const double *g, const int *kused,
+                        double *Y, hipStream_t s) {
+  hipLaunchKernelGGL(k_gmres_backsolve, dim3((nr + 255) / 256), dim3(256), 0, s, m, nr, ld, H, g,
+                     kused, Y);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_gmres_update_x(double *x, const double *V, size_t strideV, const double *Y,
+                       const int *kused, int kmax, const double *dinv, int N, int ld, int nr,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_gmres_update_x, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, x, V,
+                     strideV, Y, kused, kmax, dinv, N, ld, nr);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_axpy(double *x, const double *y, double alpha, int N, int ld, int nr, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, x, y, alpha, N, ld,
+                     nr);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_unit_rhs, dim3(grid_for((size_t)N * ld)), dim3(256), 0, s, cond, j0, N, ld,
+                     nr, b);
+  GP_LAUNCH_CHECK();
+}
+
+}  // namespace gp

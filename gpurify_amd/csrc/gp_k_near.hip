@@ -1,0 +1,138 @@
+// Near-field operator apply: the exact-Wilton leaf tiles of the precorrected
+// FMM operator (cap/fmm_solver.rs:116-128) times a block of right-hand sides.
+//
+// Layout: tile of target leaf T is row-major nt x S at nf_off[T]; its S
+// columns are panels cols[col_off[T] ..] (the leaf and its neighbours, runs of
+// consecutive sorted panels).  Vectors are N x ld row-major with the RHS index
+// fastest.  The output y (FP64) is written, not accumulated: the far field is
+// added afterwards by the L2P kernel.
+//
+// Multi-RHS (k_near_spmm): one work-group per (target leaf, 16-column RHS
+// chunk); 64-row target tiles x 32-column source chunks staged in LDS, each
+// thread owns one target row and 4 RHS columns.  Single RHS (k_near_spmv):
+// one wavefront per target row, coalesced over the row's S values.
+
+#include "gp_common.h"
+#include "gp_kernels.h"
+
+namespace gp {
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<float> {
+  using type = float;
+};
+template <>
+struct Acc<double> {
+  using type = double;
+};
+
+constexpr int NF_TM = 64;  // target rows per tile
+constexpr int NF_KS = 32;  // source columns per LDS stage
+constexpr int NF_RC = 16;  // RHS columns per work-group
+constexpr int NF_CPT = 4;  // RHS columns per thread
+
+template <typename TV, typename TX>
+__global__ __launch_bounds__(256) void k_near_spmm(
+    const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
+    const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
+    const TX *__restrict__ x, int ldx, double *__restrict__ y, int ldy, int nr) {
+  using A = typename Acc<TX>::type;
+  __shared__ TV sA[NF_TM][NF_KS + 1];
+  __shared__ TX sB[NF_KS][NF_RC];
+  const int T = blockIdx.x;
+  const int r0 = blockIdx.y * NF_RC;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const int64_t base = nf_off[T];
+  const int tid = threadIdx.x;
+  const int row = tid >> 2, cg = (tid & 3) * NF_CPT;
+  for (int tm = 0; tm < nt; tm += NF_TM) {
+    A acc[NF_CPT];
+#pragma unroll
+    for (int c = 0; c < NF_CPT; ++c) acc[c] = 0;
+    for (int ks = 0; ks < S; ks += NF_KS) {
+      // stage A: NF_TM x NF_KS
+      for (int e = tid; e < NF_TM * NF_KS; e += 256) {
+        int rr = e / NF_KS, ss = e % NF_KS;
+        int t = tm + rr, s = ks + ss;
+        sA[rr][ss] = (t < nt && s < S) ? vals[base + (int64_t)t * S + s] : (TV)0;
+      }
+      // stage B: NF_KS x NF_RC
+      for (int e = tid; e < NF_KS * NF_RC; e += 256) {
+        int ss = e / NF_RC, cc = e % NF_RC;
+        int s = ks + ss, r = r0 + cc;
+        sB[ss][cc] = (s < S && r < nr) ? x[(int64_t)cols[c0 + s] * ldx + r] : (TX)0;
+      }
+      __syncthreads();
+      int kmax = min(NF_KS, S - ks);
+      for (int s = 0; s < kmax; ++s) {
+        A a = (A)sA[row][s];
+#pragma unroll
+        for (int c = 0; c < NF_CPT; ++c) acc[c] += a * (A)sB[s][cg + c];
+      }
+      __syncthreads();
+    }
+    int t = tm + row;
+    if (t < nt) {
+#pragma unroll
+      for (int c = 0; c < NF_CPT; ++c) {
+        int r = r0 + cg + c;
+        if (r < nr) y[(int64_t)(t0 + t) * ldy + r] = (double)acc[c];
+      }
+    }
+  }
+}
+
+// One wavefront per target row (sorted panel i), single RHS.
+template <typename TV, typename TX>
+__global__ __launch_bounds__(256) void k_near_spmv(
+    int n, const int *__restrict__ panel_leaf, const int *__restrict__ leaf_lo,
+    const int64_t *__restrict__ nf_off, const int *__restrict__ col_off,
+    const int *__restrict__ cols, const TV *__restrict__ vals, const TX *__restrict__ x,
+    double *__restrict__ y) {
+  using A = typename Acc<TX>::type;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int T = panel_leaf[i];
+  const int t = i - leaf_lo[T];
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const TV *row = vals + nf_off[T] + (int64_t)t * S;
+  A acc = 0;
+  for (int s = lane; s < S; s += 64) acc += (A)row[s] * (A)x[cols[c0 + s]];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+  if (lane == 0) y[i] = (double)acc;
+}
+
+template <typename TV, typename TX>
+void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int ldx, double *y,
+                         int ldy, int nr, hipStream_t s) {
+  if (nf.n_leaves == 0) return;
+  const double flops = 2.0 * nf.nnz * nr;
+  const double bytes = (double)nf.nnz * sizeof(TV) + (double)nf.n * nr * (sizeof(TX) + sizeof(double));
+  ProfScope ps(sizeof(TV) == 4 ? (nr == 1 ? "near_spmv_f32" : "near_spmm_f32")
+                               : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64"),
+               s, flops, bytes);
+  if (nr == 1 && ldx == 1 && ldy == 1) {
+    hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
+                       nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+  } else {
+    dim3 grid(nf.n_leaves, (nr + NF_RC - 1) / NF_RC);
+    hipLaunchKernelGGL((k_near_spmm<TV, TX>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
+                       nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
+  }
+  GP_LAUNCH_CHECK();
+}
+
+void launch_near_apply(const NearTiles &nf, const double *vals, const double *x, int ldx,
+                       double *y, int ldy, int nr, hipStream_t s) {
+  launch_near_apply_t<double, double>(nf, vals, x, ldx, y, ldy, nr, s);
+}
+void launch_near_apply(const NearTiles &nf, const float *vals, const float *x, int ldx,
+                       double *y, int ldy, int nr, hipStream_t s) {
+  launch_near_apply_t<float, float>(nf, vals, x, ldx, y, ldy, nr, s);
+}
+
+}  // namespace gp

@@ -1,0 +1,95 @@
+// Launch wrappers of the device kernels (defined in gp_k_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gp {
+
+// ---------------------------------------------------------------- assembly
+void launch_dense_assemble(int n, const double *verts, const double *cen, double *P,
+                           hipStream_t s);
+void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_off,
+                          const int *col_off, const int *cols, const double *verts,
+                          const double *cen, double *v64, float *v32, hipStream_t s);
+void launch_self_inv_diag(int n, const double *verts, const double *cen, double si_scale,
+                          double *inv, hipStream_t s);
+
+// ---------------------------------------------------------------- dense LU
+void launch_lu_solve(int n, int nrhs, double *A, int *piv, double *B, double *work, int *info,
+                     hipStream_t s);
+void launch_cap_accumulate(int ncond, int nrhs, int ldx, const int *coff, const int *cidx,
+                           const double *w, const double *X, double *C, hipStream_t s);
+
+// ---------------------------------------------------------------- near field
+struct NearTiles {
+  int n = 0;         // panels
+  long nnz = 0;      // near-field values
+  int n_leaves = 0;  // target leaves
+  const int *leaf_lo = nullptr;     // n_leaves + 1
+  const int64_t *nf_off = nullptr;  // n_leaves + 1 (value offsets)
+  const int *col_off = nullptr;     // n_leaves + 1
+  const int *cols = nullptr;        // source panel per column
+  const int *panel_leaf = nullptr;  // n
+};
+void launch_near_apply(const NearTiles &nf, const double *vals, const double *x, int ldx,
+                       double *y, int ldy, int nr, hipStream_t s);
+void launch_near_apply(const NearTiles &nf, const float *vals, const float *x, int ldx,
+                       double *y, int ldy, int nr, hipStream_t s);
+
+// ---------------------------------------------------------------- far field
+template <typename T>
+struct FmmDev {
+  int n = 0;          // Chebyshev order
+  int L = 0;          // leaf level
+  int n_boxes = 0;
+  int n_targets = 0;  // boxes with a non-empty interaction list
+  long n_pairs = 0;   // M2L (target, source) pairs
+  int n_panels = 0;
+  const int *lvl_off = nullptr;  // HOST array, L + 2 entries
+  const int *leaf_lo = nullptr;  // device, n_leaves + 1
+  const int *parent = nullptr, *octant = nullptr, *child_off = nullptr, *child_idx = nullptr;
+  const int *targets = nullptr, *il_off = nullptr, *il_src = nullptr, *il_oidx = nullptr;
+  const int *box_level = nullptr;
+  const double *inv_half = nullptr;  // per level, scaled units
+  const double *area = nullptr;      // per sorted panel, scaled units
+  const T *w1d = nullptr;            // per sorted panel 3 * n
+  const T *m2m1d = nullptr;          // 24 * n * n
+  const T *G = nullptr;              // 343 * n^3 * n^3
+};
+void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
+                   double *Lc, int ldm, double *y, int ldy, hipStream_t s);
+void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, float *Lc,
+                   int ldm, double *y, int ldy, hipStream_t s);
+
+// ---------------------------------------------------------------- Krylov
+size_t kr_partial_size(int nvec_max, int ld);
+void kr_dots(const double *X, size_t strideX, int nvec, const double *Y, int N, int ld, int nr,
+             double *partial, double *out, bool take_sqrt, hipStream_t s);
+void kr_axpy_multi(double *w, const double *V, size_t strideV, const double *coef, int nvec,
+                   int N, int ld, int nr, double alpha, hipStream_t s);
+void kr_scale_cols(const double *in, double *out, const double *scale, const int *mask, int N,
+                   int ld, int nr, hipStream_t s);
+void kr_axpy(double *x, const double *y, double alpha, int N, int ld, int nr, hipStream_t s);
+void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s);
+void kr_sub(const double *a, const double *b, double *out, int N, int ld, int nr, hipStream_t s);
+void kr_precond_cast(const double *v, const double *dinv, double *z, int N, int ld, int nr,
+                     hipStream_t s);
+void kr_precond_cast(const double *v, const double *dinv, float *z, int N, int ld, int nr,
+                     hipStream_t s);
+void kr_gmres_start(int m, int nr, int ld, const double *beta, const double *bnorm, double tol,
+                    const int *live, int *active, int *kused, double *g, double *inv_beta,
+                    double *est, hipStream_t s);
+void kr_gmres_givens(int k, int m, int nr, int ld, const double *h1, const double *h2,
+                     const double *hn, double *H, double *cs, double *sn, double *g,
+                     const double *bnorm, double tol, int *active, int *kused, double *est,
+                     double *inv_h, hipStream_t s);
+void kr_gmres_backsolve(int m, int nr, int ld, const double *H, const double *g, const int *kused,
+                        double *Y, hipStream_t s);
+void kr_gmres_update_x(double *x, const double *V, size_t strideV, const double *Y,
+                       const int *kused, int kmax, const double *dinv, int N, int ld, int nr,
+                       hipStream_t s);
+
+}  // namespace gp

@@ -1,0 +1,39 @@
+// Host orchestration of the capacitance solve.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpurify.h"
+#include "gp_common.h"
+#include "gp_tree.h"
+
+struct gp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string last_error;
+  std::string device_name;
+  int cu_count = 0;
+  gp::Prof prof;
+};
+
+namespace gp {
+
+// Maxwell C (ncond x ncond, farads, vacuum) of g.  eps_panel, when given,
+// weights each panel's charge by its surrounding permittivity (the bridge's
+// inhomogeneous conductor-role solve, dense only).
+void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, double *C,
+               gp_cap_report *rep, const double *eps_panel);
+
+// Apply the precorrected FMM operator (FP64, no preconditioner) to nr
+// columns X (n x nr, caller order), y = P X, farads^-1 m^-2 units of P.
+// Test entry used to compare the device operator with the oracle's.
+void fmm_apply_host(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, int nr,
+                    const double *X, double *Y, int precision);
+
+}  // namespace gp

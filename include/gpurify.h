@@ -1,0 +1,186 @@
+/*
+ * gpurify.h - C ABI of libgpurify, the MI355X (gfx950) quasi-static
+ * capacitance engine behind the field-solver PEX path.
+ *
+ * The library replaces the Maxwell BEM solve of the reference
+ * (UW-ASIC/GPurify, Rust crate crates/pex).  Every entry point below names
+ * the reference interface it stands in for.  All pointers are plain host
+ * pointers owned by the caller and borrowed for the duration of the call
+ * only; the library owns every device allocation, scoped to a context.
+ * No C++ type or exception crosses this boundary: every function returns a
+ * status code (GP_OK = 0) and gp_last_error() holds the message.
+ *
+ * Units: geometry in metres (panels) or database units scaled by xy_scale
+ * metres/DBU (volume groups), heights and thicknesses in nm, capacitance in
+ * farads (vacuum permittivity unless stated), cap_af in attofarads.
+ * Matrices are row-major.
+ */
+#ifndef GPURIFY_H
+#define GPURIFY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes
+ * Mirror the failure classes of BridgeError (crates/pex/src/bridge.rs:34-56)
+ * plus the device classes the reference has no counterpart for. */
+#define GP_OK 0
+#define GP_ERR_INVALID_INPUT 1   /* bad panels, sizes, non-finite geometry or options */
+#define GP_ERR_NO_DEVICE 2       /* no usable gfx950 device (or GPURIFY_FORCE_NO_DEVICE=1) */
+#define GP_ERR_OOM 3             /* device allocation failed or exceeded GPURIFY_MAX_DEVICE_BYTES */
+#define GP_ERR_NOT_CONVERGED 4   /* GMRES did not reach tol and opts.strict != 0 */
+#define GP_ERR_NON_FINITE 5      /* BridgeError::NonFiniteResult (bridge.rs:54-55, 432-439) */
+#define GP_ERR_INTERNAL 6        /* HIP runtime failure or an unexpected internal state */
+#define GP_ERR_GRID_CAPACITY 7   /* BridgeError::GridCapacity (bridge.rs:46-47, 469-474) */
+#define GP_ERR_PANEL_CAPACITY 8  /* BridgeError::PanelCapacity (bridge.rs:48-49, 381-386, 410-415) */
+#define GP_ERR_SINGULAR 9        /* cap::SolveError::Singular (cap/solver.rs:39-40) */
+#define GP_ERR_EMPTY 10          /* cap::SolveError::Empty (cap/solver.rs:37-38) */
+
+/* ------------------------------------------------------------ context
+ * Stands in for the panic-contained lazy GPU context of the backend crate
+ * (crates/backend/src/gpu.rs:264-284) and gpu_ready() (crates/backend/src/lib.rs:50-59).
+ * A context owns one HIP device, one stream and its device memory.  Calls on
+ * one context are serialized internally; use one context per thread for
+ * concurrency.  gp_ctx_create never aborts: with no usable device it returns
+ * GP_ERR_NO_DEVICE and sets *out to NULL. */
+typedef struct gp_ctx gp_ctx;
+
+int gp_ctx_create(gp_ctx **out, uint32_t device_ordinal);
+void gp_ctx_destroy(gp_ctx *ctx);
+/* Message of the last failing call on ctx (or of the last failing
+ * gp_ctx_create on this thread when ctx is NULL).  Never NULL. */
+const char *gp_last_error(const gp_ctx *ctx);
+/* 1 if a gfx950 device is usable now, else 0 (gpu_ready(), backend/src/lib.rs:50-59). */
+int gp_device_ready(void);
+/* Device name, arch and CU count as text. */
+int gp_device_info(gp_ctx *ctx, char *buf, size_t len);
+/* Library version string (static storage). */
+const char *gp_version(void);
+
+/* ------------------------------------------------------------ panels
+ * The solve-time panel set (PanelSet, integrals/panel.rs:78-119): flat
+ * quadrilateral panels, 4 vertices each, x,y,z per vertex, metres.  A
+ * triangle repeats its last vertex (the repeated vertex adds a zero-length
+ * edge and a zero-area fan triangle, so its integrals are unchanged).
+ * conductor[i] in [0, n_conductors). */
+typedef struct {
+  uint32_t n_panels;
+  uint32_t n_conductors;
+  const double *vertices;    /* n_panels * 12 */
+  const uint32_t *conductor; /* n_panels */
+} gp_panels;
+
+enum { GP_PREC_FP64 = 0, GP_PREC_MIXED = 1 };
+enum { GP_METHOD_AUTO = 0, GP_METHOD_DENSE = 1, GP_METHOD_FMM = 2 };
+enum { GP_M2L_FP64 = 0, GP_M2L_FP32 = 1, GP_M2L_BF16 = 2 };
+
+/* Solver options.  gp_cap_opts_default() fills the reference's own
+ * constants: the dense/FMM rule of bridge.rs:397-404 (dense below 500
+ * panels, else FMM order 4, depth 4, tol 1e-8), GMRES restart 200 and 50
+ * restarts (cap/fmm_solver.rs:164), FP64 throughout. */
+typedef struct {
+  uint32_t precision;    /* GP_PREC_FP64 | GP_PREC_MIXED */
+  uint32_t method;       /* GP_METHOD_AUTO (reference rule) | DENSE | FMM */
+  uint32_t fmm_order;    /* Chebyshev order n (reference: 4) */
+  uint32_t max_level;    /* octree depth; 0 = chosen from leaf_target */
+  uint32_t leaf_target;  /* panel-weighted mean leaf population when max_level == 0 */
+  uint32_t restart;      /* GMRES(m) restart length */
+  uint32_t max_restarts; /* GMRES restart cycles */
+  uint32_t rhs_block;    /* right-hand sides solved together (0 = all) */
+  double tol;            /* relative residual target per right-hand side */
+  double inner_tol;      /* GP_PREC_MIXED: inner GMRES relative tolerance */
+  uint32_t max_refine;   /* GP_PREC_MIXED: outer FP64 refinement steps */
+  uint32_t m2l_inner;    /* GP_PREC_MIXED: GP_M2L_BF16 (default) or GP_M2L_FP32 */
+  uint32_t m2l_outer;    /* GP_PREC_MIXED: M2L precision of the FP64 outer residual */
+  uint32_t strict;       /* 1: non-convergence returns GP_ERR_NOT_CONVERGED */
+  uint32_t dense_limit;  /* AUTO: dense below this many panels (reference 500) */
+  uint32_t rhs_first;    /* solve conductor columns [rhs_first, rhs_first + rhs_count) */
+  uint32_t rhs_count;    /* 0 = all columns; unsolved columns of C are left zero */
+  uint32_t reserved;
+} gp_cap_opts;
+
+void gp_cap_opts_default(gp_cap_opts *opts);
+
+/* Per-solve report (the reference has none; BackendTelemetry,
+ * crates/backend/src/lib.rs:75-84, is the closest). */
+typedef struct {
+  uint32_t n_panels, n_conductors, method_used, precision_used;
+  uint32_t iterations;    /* GMRES iterations (inner, summed over refinement), max over RHS */
+  uint32_t matvecs;       /* operator applications (each covers all RHS of a block) */
+  uint32_t refine_steps;  /* outer FP64 refinement steps (mixed) */
+  uint32_t converged;     /* 1 if every RHS reached tol */
+  uint32_t max_level, n_leaves, n_boxes, fmm_order;
+  uint64_t near_nnz, m2l_pairs;
+  double final_rel_residual; /* max over RHS of ||b - A x|| / ||b|| (FP64 operator) */
+  double seconds_total, seconds_setup, seconds_solve;
+  double seconds_near, seconds_far, seconds_orth; /* time in near-field, far-field, GMRES kernels */
+} gp_cap_report;
+
+/* Maxwell capacitance matrix of a panel set.  Stands in for cap::solve
+ * (crates/pex/src/quasistatic/cap/solver.rs:100-134) and
+ * cap::fmm_solver::solve (cap/fmm_solver.rs:147-172), i.e. for the method
+ * selection and solve of bridge.rs:397-408 before the eps_r scaling.
+ * C: n_conductors^2 doubles, farads, vacuum.  report may be NULL. */
+int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,
+                 gp_cap_report *report);
+
+/* The precorrected FMM operator apply y = P x of FmmCapOperator::apply
+ * (cap/fmm_solver.rs:105-130): exact-Wilton near tiles plus the far field of
+ * the panel charges area * x, built as gp_cap_solve builds it for
+ * GP_METHOD_FMM (opts: fmm_order, max_level / leaf_target).  precision
+ * GP_PREC_FP64 applies the FP64 operator, GP_PREC_MIXED the FP32 one.
+ * X, Y: n_panels x nr row-major (panel-major), SI units (V per C/m^2). */
+int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, uint32_t nr,
+                 uint32_t precision, const double *X, double *Y);
+
+/* Kernel profiler (no reference counterpart; the reference has no timers).
+ * Enabled, every device kernel launch of this context is bracketed by HIP
+ * events on its stream.  gp_profile_report writes a JSON object
+ * {"kernel": {"launches": n, "seconds": s, "flops": f, "bytes": b}, ...}
+ * with the summed event durations and algorithmic flops / bytes since the
+ * last report (or enable), then resets the counters. */
+int gp_profile_enable(gp_ctx *ctx, int enable);
+int gp_profile_report(gp_ctx *ctx, char *json, size_t len);
+
+/* ------------------------------------------------------------ layout bridge
+ * One (net, layer) volume group of bridge.rs:58-65, 131-141: the rectangles
+ * of one net on one layer, extruded from height_nm to height_nm+thickness_nm.
+ * rects: n_rects * 4 int32 (x0, y0, x1, y1) in DBU. */
+typedef struct {
+  uint32_t net;
+  uint32_t n_rects;
+  double height_nm;
+  double thickness_nm;
+  double dielectric_k;
+  const int32_t *rects;
+} gp_volume_group;
+
+/* Panelization of bridge.rs:307-386 + 445-652 (union grid, greedy-merged
+ * top/bottom faces, exposed sidewalls, substrate plate).  Groups must come
+ * in the reference's BTreeMap order, i.e. sorted by (net, layer).
+ * refine >= 1 splits every reference panel refine x refine (1 = the
+ * reference mesh).  max_panels 0 = the reference cap MAX_PANELS = 20000.
+ * Two-call protocol: with vertices == NULL only *n_panels is written. */
+int gp_mesh_groups(const gp_volume_group *groups, uint32_t n_groups, double xy_scale,
+                   uint32_t refine, uint32_t max_panels, uint32_t *n_panels, double *vertices,
+                   uint32_t *conductor, double *panel_eps);
+
+/* solve_capacitance (bridge.rs:307-443): mesh, solve, scale by eps_r
+ * (homogeneous) or by per-panel eps (inhomogeneous, dense only, as the
+ * reference's conductor-role dielectric solve), and return cap_af =
+ * C[i][i] * 1e18 per net in ascending net order.
+ * nets_out / cap_af_out: room for the number of distinct nets.
+ * C_out (optional): (n_nets+1)^2 farads with eps applied, substrate last. */
+int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_groups,
+                         double xy_scale, uint32_t refine, uint32_t max_panels,
+                         const gp_cap_opts *opts, uint32_t *n_nets, uint32_t *nets_out,
+                         double *cap_af_out, double *C_out, gp_cap_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPURIFY_H */

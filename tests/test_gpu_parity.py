@@ -1,0 +1,145 @@
+"""Device parity: every result of libgpurify's HIP path against the oracle
+on the same inputs (called through the C ABI)."""
+import math
+
+import numpy as np
+import pytest
+
+import gpurify_amd as G
+from gpurify_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EPS0 = 8.8541878128e-12
+
+
+def relmax(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_device_is_gfx950(ctx):
+    assert "gfx950" in ctx.info()
+
+
+def test_dense_c1_reference_mesh(ctx):
+    # bridge path at the reference mesh (173 panels -> dense LU, bridge.rs:399-401)
+    groups = W.two_wires()
+    caf, C, rep = ctx.solve_capacitance(groups)
+    caf_o, Co, n = O.solve_capacitance(groups)
+    assert rep["n_panels"] == n == 173 and rep["method_used"] == G.GP_METHOD_DENSE
+    assert relmax(C, Co) < 1e-11
+    for net in caf_o:
+        assert caf[net] == pytest.approx(caf_o[net], rel=1e-11)
+
+
+def test_dense_sphere_and_plates(ctx):
+    v = O.mesh_icosphere((0, 0, 0), 1.0, 2)
+    C, rep = ctx.cap_solve(v, np.zeros(len(v)), 1, G.default_opts(method=G.GP_METHOD_DENSE))
+    Co = O.cap_dense(v, np.zeros(len(v)), 1)
+    assert relmax(C, Co) < 1e-11
+    assert abs(C[0, 0] - 4 * math.pi * EPS0) / (4 * math.pi * EPS0) < 0.03
+    side, gap = 0.2, 0.02
+    v = np.concatenate([O.mesh_plate(0, 0, 0, side, side, 10, 10), O.mesh_plate(0, 0, gap, side, side, 10, 10)])
+    cond = np.repeat([0, 1], 100)
+    C, _ = ctx.cap_solve(v, cond, 2, G.default_opts(method=G.GP_METHOD_DENSE))
+    assert relmax(C, O.cap_dense(v, cond, 2)) < 1e-11
+
+
+@pytest.mark.parametrize("order,level", [(3, 3), (4, 4), (5, 4)])
+def test_fmm_operator_matches_oracle(ctx, order, level):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    n = len(v)
+    rng = np.random.default_rng(order)
+    X = rng.normal(size=(n, 3))
+    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, max_level=level)
+    Y = ctx.fmm_apply(v, X, o)
+    for j in range(3):
+        yo = O.fmm_cap_apply(v, X[:, j], order, level, m2l_mode=1)
+        assert np.linalg.norm(Y[:, j] - yo) / np.linalg.norm(yo) < 1e-12
+        # the reference's CPQR-compressed operator differs by its truncation only
+        yr = O.fmm_cap_apply(v, X[:, j], order, level, m2l_mode=0)
+        assert np.linalg.norm(Y[:, j] - yr) / np.linalg.norm(yr) < 1e-4
+
+
+def test_fmm_operator_fp32_close(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    X = np.random.default_rng(1).normal(size=(len(v), 16))
+    o = G.default_opts(method=G.GP_METHOD_FMM)
+    Y64 = ctx.fmm_apply(v, X, o)
+    Y32 = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
+    assert np.linalg.norm(Y32 - Y64) / np.linalg.norm(Y64) < 1e-5
+
+
+def test_fmm_single_and_block_rhs_agree(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    X = np.random.default_rng(2).normal(size=(len(v), 3))
+    o = G.default_opts(method=G.GP_METHOD_FMM)
+    Yb = ctx.fmm_apply(v, X, o)
+    for j in range(3):
+        Ys = ctx.fmm_apply(v, X[:, j:j + 1], o)
+        assert np.linalg.norm(Ys[:, 0] - Yb[:, j]) / np.linalg.norm(Yb[:, j]) < 1e-13
+
+
+def test_fmm_solve_fp64_matches_oracle(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    o = G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11)
+    C, rep = ctx.cap_solve(v, c, 3, o)
+    Co, it, rs = O.cap_fmm(v, c, 3, order=4, max_level=4, tol=1e-11, m2l_mode=1)
+    assert rep["converged"] == 1 and rep["final_rel_residual"] <= 1e-11
+    assert relmax(C, Co) < 1e-9
+    # versus the reference's own (compressed) operator at its tol 1e-8
+    Cr, _, _ = O.cap_fmm(v, c, 3, order=4, max_level=4, tol=1e-8, m2l_mode=0)
+    assert relmax(C, Cr) < 1e-4
+
+
+def test_fmm_single_rhs_blocks_match(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    Cb, _ = ctx.cap_solve(v, c, 3, G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11))
+    Cs, _ = ctx.cap_solve(v, c, 3, G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11, rhs_block=1))
+    assert relmax(Cs, Cb) < 1e-9
+
+
+def test_mixed_matches_fp64_within_1e6(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    C64, _ = ctx.cap_solve(v, c, 3, G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11))
+    Cm, rep = ctx.cap_solve(v, c, 3, G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11,
+                                                   precision=G.GP_PREC_MIXED))
+    assert rep["converged"] == 1 and rep["refine_steps"] >= 1
+    assert np.max(np.abs(Cm - C64) / np.abs(C64)) < 1e-6
+
+
+def test_bridge_fmm_rule_matches_oracle(ctx):
+    # 1557 panels -> FMM(4,4,1e-8) under the reference rule
+    groups = W.two_wires()
+    caf, C, rep = ctx.solve_capacitance(groups, refine=3)
+    caf_o, Co, n = O.solve_capacitance(groups, refine=3)
+    assert rep["method_used"] == G.GP_METHOD_FMM and n == rep["n_panels"]
+    for net in caf_o:  # dense vs CPQR M2L and GMRES tol 1e-8 on both sides
+        assert caf[net] == pytest.approx(caf_o[net], rel=2e-4)
+
+
+def test_strict_nonconvergence_is_reported(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    o = G.default_opts(method=G.GP_METHOD_FMM, tol=1e-14, restart=2, max_restarts=1, strict=1)
+    with pytest.raises(G.GpError) as ei:
+        ctx.cap_solve(v, c, 3, o)
+    assert ei.value.code == G._lib.GP_ERR_NOT_CONVERGED
+    o.strict = 0
+    C, rep = ctx.cap_solve(v, c, 3, o)
+    assert rep["converged"] == 0 and np.all(np.isfinite(C))
+
+
+def test_invalid_inputs_fail_closed(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires())
+    with pytest.raises(G.GpError) as ei:
+        ctx.cap_solve(v, c, 2)  # conductor id 2 out of range
+    assert ei.value.code == G._lib.GP_ERR_INVALID_INPUT
+    bad = v.copy()
+    bad[5, 0, 0] = np.nan
+    with pytest.raises(G.GpError) as ei:
+        ctx.cap_solve(bad, c, 3)
+    assert ei.value.code == G._lib.GP_ERR_INVALID_INPUT
+    with pytest.raises(G.GpError) as ei:
+        ctx.cap_solve(v, c, 3, G.default_opts(method=G.GP_METHOD_FMM, fmm_order=9))
+    assert ei.value.code == G._lib.GP_ERR_INVALID_INPUT
