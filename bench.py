@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Capacitance-solve benchmark: converged BEM unknowns/s on one MI355X.
+
+Workload: the 64-wire met1 bus (50 um long, widths/spacings uniform
+0.14-0.5 um, seed 2) crossed by 32 met2 wires (seed 3) on the sky130
+heuristic stack, meshed by the bridge panelization refined 4x4
+(211,136 panels), 97 right-hand sides (96 nets + substrate), black-box FMM
+of Chebyshev order 5 on an octree whose depth is chosen for a panel-weighted
+leaf population of 64.
+
+One step = one full Maxwell solve of that layout (panels resident on the
+host -> C on the host): FMM setup (tree, near-field Wilton assembly, M2L
+tables) plus block GMRES over all 97 RHS.  The measured solver is the mixed
+precision one: FP32 near field, BF16 M2L on MFMA with FP32 accumulation,
+FP64 Arnoldi/Givens and FP64 iterative refinement to tol 1e-10.  A step's
+unknowns are N_panels x 97; value = unknowns / second over the timed steps.
+
+Beside it (not in `value`): the plain FP64 device solve of the same
+discretized operator, batched over all RHS and one RHS at a time (sampled),
+the entrywise relative difference of the Maxwell matrix mixed vs FP64
+(gate <= 1e-6), a per-kernel profile of one mixed solve with its roofline,
+and the oracle (a CPU FP64 restatement of the reference algorithm) timed on a
+bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {  # MI355X published peaks
+    "hbm_GBs": 8000.0, "bf16_mfma_TFs": 2500.0, "fp64_TFs": 78.6, "fp32_TFs": 157.3,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=4)
+    ap.add_argument("--order", type=int, default=5)
+    ap.add_argument("--leaf-target", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--plain-rhs", type=int, default=2, help="RHS sampled for the 1-RHS FP64 rate")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is designed, not built)"}))
+        return 2
+    import numpy as np
+
+    import gpurify_amd as G
+    from gpurify_amd import workloads as W
+
+    groups = W.bus_crossing()
+    verts, cond, _ = G.mesh_groups(groups, refine=args.refine, max_panels=10 ** 8)
+    n = len(verts)
+    nc = int(cond.max()) + 1
+    log(f"workload bus64x32 refine={args.refine}: {n} panels, {nc} conductors")
+    common = dict(method=G.GP_METHOD_FMM, fmm_order=args.order, max_level=0,
+                  leaf_target=args.leaf_target, tol=args.tol)
+    o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol,
+                           m2l_inner=G.GP_M2L_BF16, **common)
+    o_64 = G.default_opts(**common)
+    out = {"metric": "converged BEM unknowns/s, mixed vs FP64 GPU @<=1e-6 rel C error",
+           "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+fp32/fp64",
+           "data": "synthetic (seeded sky130 bus layout)"}
+    with G.Context(0) as ctx:
+        log("device:", ctx.info())
+        for w in range(args.warmup):
+            _, rep = ctx.cap_solve(verts, cond, nc, o_mix)
+            log(f"warmup {w}: {rep['seconds_total']:.3f}s it={rep['iterations']}")
+        t0 = time.perf_counter()
+        reps = []
+        for k in range(args.steps):
+            Cm, rep = ctx.cap_solve(verts, cond, nc, o_mix)
+            reps.append(rep)
+            log(f"step {k}: {rep['seconds_total']:.3f}s (setup {rep['seconds_setup']:.3f}) "
+                f"it={rep['iterations']} refine={rep['refine_steps']} res={rep['final_rel_residual']:.1e}")
+        T = time.perf_counter() - t0
+        ok = all(r["converged"] for r in reps)
+        unknowns = n * nc
+        value = unknowns * args.steps / T
+        out.update(value=value, ms_per_step=1e3 * T / args.steps)
+        last = reps[-1]
+        out["config"] = {"workload": f"bus64x32_met1met2_refine{args.refine}", "panels": n,
+                         "rhs": nc, "fmm_order": args.order, "octree_depth": last["max_level"],
+                         "leaf_target": args.leaf_target, "tol": args.tol,
+                         "near_nnz": last["near_nnz"], "m2l_pairs": last["m2l_pairs"],
+                         "iterations": last["iterations"], "refine_steps": last["refine_steps"],
+                         "converged": ok, "global_batch": nc}
+
+        # Profile of one more mixed solve (not in the timed region).
+        ctx.profile(True)
+        ctx.cap_solve(verts, cond, nc, o_mix)
+        prof = ctx.profile_report()
+        ctx.profile(False)
+        kern = sorted(prof.items(), key=lambda kv: -kv[1]["seconds"])
+        out["kernels"] = {k: {"launches": v["launches"], "ms_total": round(1e3 * v["seconds"], 3),
+                              "ms_avg": round(1e3 * v["seconds"] / v["launches"], 4)}
+                          for k, v in kern[:10]}
+        out["roofline"] = roofline(kern)
+
+        if not args.no_fp64:
+            t = time.perf_counter()
+            C64, r64 = ctx.cap_solve(verts, cond, nc, o_64)
+            t64 = time.perf_counter() - t
+            rel = np.abs(Cm - C64) / np.abs(C64)
+            rate64 = unknowns / t64
+            fp64 = {"batched_unknowns_per_s": rate64, "batched_s": t64,
+                    "iterations": r64["iterations"], "converged": bool(r64["converged"])}
+            if args.plain_rhs > 0:
+                op = G.default_opts(rhs_block=1, rhs_count=args.plain_rhs, **common)
+                t = time.perf_counter()
+                _, rp = ctx.cap_solve(verts, cond, nc, op)
+                tp = time.perf_counter() - t
+                fp64.update(plain_1rhs_unknowns_per_s=n * args.plain_rhs / tp,
+                            plain_sample=f"{args.plain_rhs} RHS one at a time, {tp:.2f}s")
+                out["speedup_vs_fp64_plain"] = value / fp64["plain_1rhs_unknowns_per_s"]
+            out["fp64_gpu"] = fp64
+            out["speedup_vs_fp64_batched"] = value / rate64
+            out["max_rel_C_err_vs_fp64"] = float(rel.max())
+            out["max_rel_diag_err_vs_fp64"] = float(np.max(np.abs(np.diag(Cm) - np.diag(C64)) / np.diag(C64)))
+            out["gate_1e-6_pass"] = bool(rel.max() <= 1e-6)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def roofline(kern):
+    """Roofline entry for the kernel with the largest share of device time."""
+    name, v = kern[0]
+    avg = v["seconds"] / v["launches"]
+    if "mfma" in name and "bf16" in name:
+        bound, peak, unit, per = "mfma", PEAKS["bf16_mfma_TFs"], "TFLOP/s", v["flops"]
+    elif "mfma" in name:
+        bound, peak, unit, per = "mfma", PEAKS["fp64_TFs"], "TFLOP/s", v["flops"]
+    elif name.startswith("near_spmm_f32") or name.startswith("m2l_f32"):
+        bound, peak, unit, per = "valu", PEAKS["fp32_TFs"], "TFLOP/s", v["flops"]
+    elif name.endswith("_f64"):
+        bound, peak, unit, per = "valu", PEAKS["fp64_TFs"], "TFLOP/s", v["flops"]
+    else:
+        bound, peak, unit, per = "hbm", PEAKS["hbm_GBs"], "GB/s", v["bytes"]
+    scale = 1e12 if unit == "TFLOP/s" else 1e9
+    achieved = per / v["launches"] / avg / scale
+    return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "avg_launch_ms": 1e3 * avg, "launches": v["launches"],
+            "traffic": None}
+
+
+def cpu_baseline(args):
+    """The oracle (CPU FP64 restatement of the reference FMM solve: CPQR M2L,
+    MGS GMRES) on the same layout at the reference mesh (refine 1) for one
+    RHS, on the host cores."""
+    import numpy as np
+
+    import gpurify_amd as G
+    from gpurify_amd import workloads as W
+    from oracle import oracle as O
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    v, c, _ = G.mesh_groups(W.bus_crossing(), refine=1)
+    n = len(v)
+    nc = int(c.max()) + 1
+    depth = weighted_depth(v, args.leaf_target)
+    t = time.perf_counter()
+    _, it, (ts, tg) = O.cap_fmm_cols(v, c, nc, 0, 1, order=args.order, max_level=depth, tol=args.tol)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "unknowns/s", "cores": threads, "kind": "port",
+            "sample": f"bus64x32 at the reference mesh (refine 1, {n} panels), 1 RHS, FMM order "
+                      f"{args.order} depth {depth}, tol {args.tol}: {dt:.2f}s "
+                      f"(setup {ts:.2f}s, GMRES {tg:.2f}s, {int(it[0])} iterations)"}
+
+
+def weighted_depth(verts, target):
+    """Depth rule of the device solver (panel-weighted leaf population)."""
+    import numpy as np
+    from oracle import oracle as O
+
+    cen, _, _ = O.panel_props(verts)
+    lo, hi = cen.min(0), cen.max(0)
+    ctr = 0.5 * (lo + hi)
+    half = max(0.5 * (hi - lo).max() * (1 + 1e-6), 1e-300)
+    for L in range(2, 21):
+        cells = np.clip(np.floor((cen - (ctr - half)) / (2 * half / 2 ** L)).astype(np.int64), 0,
+                        2 ** L - 1)
+        _, cnt = np.unique(cells, axis=0, return_counts=True)
+        if np.sum(cnt.astype(float) ** 2) / len(cen) <= target:
+            return L
+    return 20
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -157,7 +157,7 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->tol = 1.0e-8;       // bridge.rs:403
   o->inner_tol = 1.0e-5;
   o->max_refine = 8;
-  o->m2l_inner = GP_M2L_FP32;
+  o->m2l_inner = GP_M2L_BF16;
   o->m2l_outer = GP_M2L_FP64;
   o->strict = 0;
   o->dense_limit = 500;  // bridge.rs:30

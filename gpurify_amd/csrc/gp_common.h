@@ -78,9 +78,15 @@ struct DBuf {
     n = 0;
   }
   size_t bytes() const { return n * sizeof(T); }
+  // Completes before returning: callers pass temporaries and locals, and an
+  // asynchronous copy from pageable memory may read the host buffer after
+  // the call returns.
   void upload(const std::vector<T> &h, hipStream_t s) {
     if (h.size() != n) alloc(h.size());
-    if (n) GP_HIP(hipMemcpyAsync(p, h.data(), bytes(), hipMemcpyHostToDevice, s));
+    if (n) {
+      GP_HIP(hipMemcpyAsync(p, h.data(), bytes(), hipMemcpyHostToDevice, s));
+      GP_HIP(hipStreamSynchronize(s));
+    }
   }
   void zero(hipStream_t s) {
     if (n) GP_HIP(hipMemsetAsync(p, 0, bytes(), s));

@@ -1,30 +1,44 @@
 // Black-box Chebyshev FMM passes on the device (fmm/bbfmm.rs:209-448), for a
 // block of right-hand sides, in FP64 or FP32 (template T).
 //
-// Expansions are stored per box as n^3 x ld (Chebyshev node m = a n^2 + b n + c
-// major, RHS fastest).  One work-group handles one box and a chunk of up to
-// 16 RHS columns; each thread owns a fixed set of (node, column) outputs in
-// registers, inputs are staged through LDS.  The far-field potential of
-// panel charges q = area * sigma is accumulated into the FP64 output y, which
-// already holds the near field.
+// Layouts (per box, m = Chebyshev node a n^2 + b n + c, r = RHS column):
+//   multipoles M[box][r][m]  (m fastest, padded to n3p, a multiple of 32),
+//   locals    Lc[box][m][r]  (r fastest).
+// The multipole layout makes both operands of the M2L matrix-core product
+// contiguous along the contraction index.  P2M and M2M also write a BF16
+// copy of the multipoles when the M2L runs on BF16 MFMA.
+//
+// Generic kernels: one work-group per box and 16-column RHS chunk, each
+// thread owns fixed (node, column) outputs in registers, inputs staged in LDS.
+// MFMA M2L: one work-group (4 wavefronts) per target box and all RHS columns
+// (<= 128); the accumulator is a grid of 16x16 tiles (RHS x node), the four
+// wavefronts split the node dimension, and every interaction pair streams its
+// source multipole and its interaction matrix through 16x16x32 BF16 (or
+// 16x16x4 FP64) matrix-core products.
 
 #include "gp_common.h"
 #include "gp_kernels.h"
 
 namespace gp {
 
-constexpr int FM_RC = 16;   // RHS columns per work-group
+constexpr int FM_RC = 16;   // RHS columns per work-group (generic kernels)
 constexpr int FM_OPT = 16;  // max outputs per thread: n^3 * 16 / 256 for n <= 6
 
-// P2M (bbfmm.rs:215-228): M[leaf][m][r] = sum_p S(p, m) area_p z[p][r], with
+template <typename T>
+__device__ inline void store_mult(T *M, __bf16 *Mb, size_t idx, T v) {
+  M[idx] = v;
+  if (Mb) Mb[idx] = (__bf16)(float)v;
+}
+
+// P2M (bbfmm.rs:215-228): M[leaf][r][m] = sum_p S(p, m) area_p z[p][r], with
 // the tensor weights S(p, (a,b,c)) = wx[a] wy[b] wz[c] of the point's
 // 1-D interpolation weights w1d[p][axis][k] (chebyshev.rs:77-94).
 template <typename T>
-__global__ __launch_bounds__(256) void k_p2m(int n, const int *__restrict__ leaf_lo,
+__global__ __launch_bounds__(256) void k_p2m(int n, int n3p, const int *__restrict__ leaf_lo,
                                              int leaf_box0, const T *__restrict__ w1d,
                                              const double *__restrict__ area,
                                              const T *__restrict__ z, int ldz, int nr,
-                                             T *__restrict__ M, int ldm) {
+                                             T *__restrict__ M, __bf16 *__restrict__ Mb, int ldm) {
   const int li = blockIdx.x, r0 = blockIdx.y * FM_RC;
   const int rc = min(FM_RC, nr - r0);
   const int n3 = n * n * n, nout = n3 * rc;
@@ -39,33 +53,35 @@ __global__ __launch_bounds__(256) void k_p2m(int n, const int *__restrict__ leaf
     for (int k = 0; k < FM_OPT; ++k) {
       int o = threadIdx.x + k * 256;
       if (o < nout) {
-        int m = o / rc, r = r0 + o % rc;
+        int r = r0 + o / n3, m = o % n3;
         int a = m / (n * n), b = (m / n) % n, c = m % n;
         T q = ap * z[(size_t)p * ldz + r];
         acc[k] += w[a] * w[n + b] * w[2 * n + c] * q;
       }
     }
   }
-  T *out = M + (size_t)(leaf_box0 + li) * n3 * ldm;
+  const size_t base = (size_t)(leaf_box0 + li) * ldm * n3p;
 #pragma unroll
   for (int k = 0; k < FM_OPT; ++k) {
     int o = threadIdx.x + k * 256;
     if (o < nout) {
-      int m = o / rc, r = r0 + o % rc;
-      out[(size_t)m * ldm + r] = acc[k];
+      int r = r0 + o / n3, m = o % n3;
+      store_mult(M, Mb, base + (size_t)r * n3p + m, acc[k]);
     }
   }
 }
 
 // M2M (bbfmm.rs:230-294): for every box of one level with children,
 // M[parent] = sum over children of (X (x) Y (x) Z)_octant M[child], applied as
-// three 1-D contractions (z, then y, then x) through two LDS buffers.
+// three 1-D contractions (z, then y, then x) through two LDS buffers laid out
+// [m][r] within the chunk.
 template <typename T>
-__global__ __launch_bounds__(256) void k_m2m(int n, int box0, const int *__restrict__ child_off,
+__global__ __launch_bounds__(256) void k_m2m(int n, int n3p, int box0,
+                                             const int *__restrict__ child_off,
                                              const int *__restrict__ child_idx,
                                              const int *__restrict__ octant,
                                              const T *__restrict__ m2m1d, int nr,
-                                             T *__restrict__ M, int ldm) {
+                                             T *__restrict__ M, __bf16 *__restrict__ Mb, int ldm) {
   extern __shared__ unsigned char smem_raw[];
   T *s0 = reinterpret_cast<T *>(smem_raw);
   const int n3 = n * n * n, nn = n * n;
@@ -81,8 +97,11 @@ __global__ __launch_bounds__(256) void k_m2m(int n, int box0, const int *__restr
     const int o = octant[ch];
     const T *mx = m2m1d + (o * 3 + 0) * nn, *my = m2m1d + (o * 3 + 1) * nn,
             *mz = m2m1d + (o * 3 + 2) * nn;
-    const T *wc = M + (size_t)ch * n3 * ldm;
-    for (int e = threadIdx.x; e < nout; e += 256) s0[e] = wc[(size_t)(e / rc) * ldm + r0 + e % rc];
+    const T *wc = M + (size_t)ch * ldm * n3p;
+    for (int e = threadIdx.x; e < nout; e += 256) {
+      int r = e / n3, m = e % n3;
+      s0[m * rc + r] = wc[(size_t)(r0 + r) * n3p + m];
+    }
     __syncthreads();
     // z: s1[a][b][cp] = sum_c mz[cp][c] s0[a][b][c]
     for (int e = threadIdx.x; e < nout; e += 256) {
@@ -102,12 +121,13 @@ __global__ __launch_bounds__(256) void k_m2m(int n, int box0, const int *__restr
       s0[e] = s;
     }
     __syncthreads();
-    // x: acc[ap][bp][cp] += sum_a mx[ap][a] s0[a][bp][cp]
+    // x: acc[ap][bp][cp] += sum_a mx[ap][a] s0[a][bp][cp]; thread output o maps
+    // to (r = o / n3, m = o % n3) so the final store is contiguous in m.
 #pragma unroll
     for (int k = 0; k < FM_OPT; ++k) {
-      int e = threadIdx.x + k * 256;
-      if (e < nout) {
-        int m = e / rc, r = e % rc;
+      int o = threadIdx.x + k * 256;
+      if (o < nout) {
+        int r = o / n3, m = o % n3;
         int ap = m / nn, bc = m % nn;
         T s = 0;
         for (int a = 0; a < n; ++a) s += mx[ap * n + a] * s0[(a * nn + bc) * rc + r];
@@ -116,20 +136,24 @@ __global__ __launch_bounds__(256) void k_m2m(int n, int box0, const int *__restr
     }
     __syncthreads();
   }
-  T *out = M + (size_t)b * n3 * ldm;
+  const size_t base = (size_t)b * ldm * n3p;
 #pragma unroll
   for (int k = 0; k < FM_OPT; ++k) {
-    int e = threadIdx.x + k * 256;
-    if (e < nout) out[(size_t)(e / rc) * ldm + r0 + e % rc] = acc[k];
+    int o = threadIdx.x + k * 256;
+    if (o < nout) {
+      int r = o / n3, m = o % n3;
+      store_mult(M, Mb, base + (size_t)(r0 + r) * n3p + m, acc[k]);
+    }
   }
 }
 
-// M2L (bbfmm.rs:296-335) with the uncompressed Chebyshev interaction matrix
-// G_off[m][l] = 1/|x_m - x_l - 2 off| on the reference box (bbfmm.rs:156-166),
-// scaled by 1/half of the level: Lc[t] = (1/half) sum_(s,off) G_off M[s].
-// targets: box ids with a non-empty interaction list.
+// Generic M2L (bbfmm.rs:296-335) with the uncompressed Chebyshev interaction
+// matrix G_off[m][l] = 1/|x_m - x_l - 2 off| on the reference box
+// (bbfmm.rs:156-166, stored n3p x n3p), scaled by 1/half of the level:
+// Lc[t] = (1/half) sum_(s,off) G_off M[s].  targets: boxes with a non-empty
+// interaction list.
 template <typename T>
-__global__ __launch_bounds__(256) void k_m2l(int n, const int *__restrict__ targets,
+__global__ __launch_bounds__(256) void k_m2l(int n, int n3p, const int *__restrict__ targets,
                                              const int *__restrict__ il_off,
                                              const int *__restrict__ il_src,
                                              const int *__restrict__ il_oidx,
@@ -148,16 +172,19 @@ __global__ __launch_bounds__(256) void k_m2l(int n, const int *__restrict__ targ
 #pragma unroll
   for (int k = 0; k < FM_OPT; ++k) acc[k] = 0;
   for (int q = il_off[t]; q < il_off[t + 1]; ++q) {
-    const T *ws = M + (size_t)il_src[q] * n3 * ldm;
-    const T *g = G + (size_t)il_oidx[q] * n3 * n3;
-    for (int e = threadIdx.x; e < nout; e += 256) s0[e] = ws[(size_t)(e / rc) * ldm + r0 + e % rc];
+    const T *ws = M + (size_t)il_src[q] * ldm * n3p;
+    const T *g = G + (size_t)il_oidx[q] * n3p * n3p;
+    for (int e = threadIdx.x; e < nout; e += 256) {
+      int r = e / n3, m = e % n3;
+      s0[m * rc + r] = ws[(size_t)(r0 + r) * n3p + m];
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < FM_OPT; ++k) {
       int e = threadIdx.x + k * 256;
       if (e < nout) {
         int m = e / rc, r = e % rc;
-        const T *gr = g + (size_t)m * n3;
+        const T *gr = g + (size_t)m * n3p;
         T s = 0;
         for (int l = 0; l < n3; ++l) s += gr[l] * s0[l * rc + r];
         acc[k] += s;
@@ -172,6 +199,138 @@ __global__ __launch_bounds__(256) void k_m2l(int n, const int *__restrict__ targ
     int e = threadIdx.x + k * 256;
     if (e < nout) out[(size_t)(e / rc) * ldm + r0 + e % rc] = acc[k] * ih;
   }
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// MFMA M2L, BF16 inputs, FP32 accumulation.  Requires ldm a multiple of 16,
+// ldm <= 128 and n3p a multiple of 32.  Product per pair: D (RHS x node) +=
+// A (RHS x l) * B (l x node) with A = Mb[s] rows and B[l][m] = G_off[m][l];
+// v_mfma_f32_16x16x32_bf16: lane holds A[i = lane % 16][k = 8 (lane / 16) + 0..7],
+// B[k = 8 (lane / 16) + 0..7][j = lane % 16], D[i = 4 (lane / 16) + v][j = lane % 16].
+template <int MTW>
+__global__ __launch_bounds__(256) void k_m2l_bf16(int n3, int n3p, int nr, int ldm,
+                                                  const int *__restrict__ targets,
+                                                  const int *__restrict__ il_off,
+                                                  const int *__restrict__ il_src,
+                                                  const int *__restrict__ il_oidx,
+                                                  const int *__restrict__ box_level,
+                                                  const double *__restrict__ inv_half,
+                                                  const __bf16 *__restrict__ G,
+                                                  const __bf16 *__restrict__ Mb,
+                                                  float *__restrict__ Lc) {
+  const int t = targets[blockIdx.x];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int RT = (nr + 15) >> 4;
+  const int MT = n3p >> 4;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  f32x4 acc[8][MTW];
+#pragma unroll
+  for (int rt = 0; rt < 8; ++rt)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) acc[rt][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const size_t MS = (size_t)ldm * n3p;
+  for (int q = il_off[t]; q < il_off[t + 1]; ++q) {
+    const __bf16 *Ms = Mb + (size_t)il_src[q] * MS;
+    const __bf16 *Go = G + (size_t)il_oidx[q] * n3p * n3p;
+    for (int k0 = 0; k0 < n3p; k0 += 32) {
+      bf16x8 b[MTW];
+#pragma unroll
+      for (int j = 0; j < MTW; ++j) {
+        int mt = wave * MTW + j;
+        if (mt < MT) b[j] = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * n3p + k0 + lk);
+      }
+#pragma unroll
+      for (int rt = 0; rt < 8; ++rt) {
+        if (rt < RT) {
+          bf16x8 a = *(const bf16x8 *)(Ms + (size_t)(rt * 16 + li) * n3p + k0 + lk);
+#pragma unroll
+          for (int j = 0; j < MTW; ++j)
+            if (wave * MTW + j < MT)
+              acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[rt][j], 0, 0, 0);
+        }
+      }
+    }
+  }
+  const float ih = (float)inv_half[box_level[t]];
+  float *out = Lc + (size_t)t * n3 * ldm;
+#pragma unroll
+  for (int rt = 0; rt < 8; ++rt)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int r = rt * 16 + (lane >> 4) * 4 + v, m = (wave * MTW + j) * 16 + li;
+        if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
+      }
+}
+
+// MFMA M2L in FP64 (v_mfma_f64_16x16x4f64), same tiling.  Each lane loads 4
+// consecutive contraction entries (32 bytes) and feeds them to 4 successive
+// MFMAs: in step j lane group g = lane / 16 supplies contraction index
+// k0 + 4 g + j for both operands, so the product sums every index once.
+template <int MTW>
+__global__ __launch_bounds__(256) void k_m2l_f64(int n3, int n3p, int nr, int ldm,
+                                                 const int *__restrict__ targets,
+                                                 const int *__restrict__ il_off,
+                                                 const int *__restrict__ il_src,
+                                                 const int *__restrict__ il_oidx,
+                                                 const int *__restrict__ box_level,
+                                                 const double *__restrict__ inv_half,
+                                                 const double *__restrict__ G,
+                                                 const double *__restrict__ M,
+                                                 double *__restrict__ Lc) {
+  const int t = targets[blockIdx.x];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int RT = (nr + 15) >> 4;
+  const int MT = n3p >> 4;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  f64x4 acc[8][MTW];
+#pragma unroll
+  for (int rt = 0; rt < 8; ++rt)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) acc[rt][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  const size_t MS = (size_t)ldm * n3p;
+  for (int q = il_off[t]; q < il_off[t + 1]; ++q) {
+    const double *Ms = M + (size_t)il_src[q] * MS;
+    const double *Go = G + (size_t)il_oidx[q] * n3p * n3p;
+    for (int k0 = 0; k0 < n3p; k0 += 16) {
+      f64x4 b[MTW];
+#pragma unroll
+      for (int j = 0; j < MTW; ++j) {
+        int mt = wave * MTW + j;
+        if (mt < MT) b[j] = *(const f64x4 *)(Go + (size_t)(mt * 16 + li) * n3p + k0 + lk);
+      }
+#pragma unroll
+      for (int rt = 0; rt < 8; ++rt) {
+        if (rt < RT) {
+          f64x4 a = *(const f64x4 *)(Ms + (size_t)(rt * 16 + li) * n3p + k0 + lk);
+#pragma unroll
+          for (int j = 0; j < MTW; ++j)
+            if (wave * MTW + j < MT) {
+#pragma unroll
+              for (int kk = 0; kk < 4; ++kk)
+                acc[rt][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], b[j][kk], acc[rt][j], 0, 0, 0);
+            }
+        }
+      }
+    }
+  }
+  // v_mfma_f64_16x16x4f64 interleaves its output rows: D[i][j] sits in lane
+  // j + 16 (i % 4), register i / 4 (unlike the 16x16x32 BF16 form).
+  const double ih = inv_half[box_level[t]];
+  double *out = Lc + (size_t)t * n3 * ldm;
+#pragma unroll
+  for (int rt = 0; rt < 8; ++rt)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int r = rt * 16 + (lane >> 4) + 4 * v, m = (wave * MTW + j) * 16 + li;
+        if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
+      }
 }
 
 // L2L (bbfmm.rs:337-398): for every box of one level,
@@ -256,10 +415,18 @@ __global__ __launch_bounds__(256) void k_l2p(int n, const int *__restrict__ leaf
   }
 }
 
+// Zero fill (instead of a memset node, so the pass can be graph-captured).
 template <typename T>
-void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, T *Lc, int ldm,
-                     double *y, int ldy, hipStream_t s) {
-  const int n = f.n, n3 = n * n * n;
+__global__ __launch_bounds__(256) void k_fill_zero(T *__restrict__ p, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    p[e] = (T)0;
+}
+
+template <typename T>
+void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf16 *Mb, T *Lc,
+                     int ldm, int m2l, double *y, int ldy, hipStream_t s) {
+  const int n = f.n, n3 = n * n * n, n3p = f.n3p;
   const int nrc = (nr + FM_RC - 1) / FM_RC;
   const size_t lds1 = (size_t)n3 * FM_RC * sizeof(T);
   const int L = f.L;
@@ -267,32 +434,76 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, T *L
   const int nleaf = f.lvl_off[L + 1] - leaf0;
   const bool f32 = sizeof(T) == 4;
   const double exp_bytes = (double)n3 * nr * sizeof(T);
-  // P2M
-  ProfScope *ps = new ProfScope(f32 ? "p2m_f32" : "p2m_f64", s, 2.0 * f.n_panels * n3 * nr,
-                                (double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes);
-  hipLaunchKernelGGL((k_p2m<T>), dim3(nleaf, nrc), dim3(256), 0, s, n, f.leaf_lo, leaf0, f.w1d,
-                     f.area, z, ldz, nr, M, ldm);
-  GP_LAUNCH_CHECK();
-  delete ps;
+  const bool want_bf16 = (m2l == GP_M2L_BF16);
+  if (want_bf16 && (!Mb || !f.Gbf || ldm % 16 || ldm > 128 || n3p % 32))
+    throw Error(GP_ERR_INTERNAL, "BF16 M2L needs 16 <= ld <= 128 (multiple of 16) and its tables");
+  __bf16 *mb = want_bf16 ? Mb : nullptr;
+  {
+    ProfScope ps(f32 ? "p2m_f32" : "p2m_f64", s, 2.0 * f.n_panels * n3 * nr,
+                 (double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes);
+    hipLaunchKernelGGL((k_p2m<T>), dim3(nleaf, nrc), dim3(256), 0, s, n, n3p, f.leaf_lo, leaf0,
+                       f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+    GP_LAUNCH_CHECK();
+  }
   // M2M, finest to level 2 (multipoles above level 2 are never used).
   for (int l = L - 1; l >= 2; --l) {
     int nb = f.lvl_off[l + 1] - f.lvl_off[l];
     if (nb == 0) continue;
     int nch = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch, (nb + nch) * exp_bytes);
-    hipLaunchKernelGGL((k_m2m<T>), dim3(nb, nrc), dim3(256), 2 * lds1, s, n, f.lvl_off[l],
-                       f.child_off, f.child_idx, f.octant, f.m2m1d, nr, M, ldm);
+    hipLaunchKernelGGL((k_m2m<T>), dim3(nb, nrc), dim3(256), 2 * lds1, s, n, n3p, f.lvl_off[l],
+                       f.child_off, f.child_idx, f.octant, f.m2m1d, nr, M, mb, ldm);
     GP_LAUNCH_CHECK();
   }
   // M2L over every box of level >= 2 with a non-empty list; boxes without
-  // one get a zero local expansion.
-  GP_HIP(hipMemsetAsync(Lc, 0, (size_t)f.n_boxes * n3 * ldm * sizeof(T), s));
+  // one keep a zero local expansion.
+  {
+    size_t cnt = (size_t)f.n_boxes * n3 * ldm;
+    int blocks = (int)std::min<size_t>((cnt + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_fill_zero<T>), dim3(blocks), dim3(256), 0, s, Lc, cnt);
+    GP_LAUNCH_CHECK();
+  }
   if (f.n_targets > 0) {
-    ProfScope pl(f32 ? "m2l_f32" : "m2l_f64", s, 2.0 * (double)f.n_pairs * n3 * n3 * nr,
-                 (double)(f.n_boxes + f.n_targets) * exp_bytes);
-    hipLaunchKernelGGL((k_m2l<T>), dim3(f.n_targets, nrc), dim3(256), lds1, s, n, f.targets,
-                       f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, f.G, nr, M, Lc,
-                       ldm);
+    const double fl = 2.0 * (double)f.n_pairs * n3 * n3 * nr;
+    const double by = (double)(f.n_boxes + f.n_targets) * exp_bytes;
+    const int MT = n3p / 16;
+    if (want_bf16) {
+      if constexpr (sizeof(T) == 4) {
+        ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * n3 * nr * 2 + f.n_targets * exp_bytes);
+        const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf);
+        if (MT <= 4)
+          hipLaunchKernelGGL((k_m2l_bf16<1>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
+                             Mb, Lc);
+        else if (MT <= 8)
+          hipLaunchKernelGGL((k_m2l_bf16<2>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
+                             Mb, Lc);
+        else
+          hipLaunchKernelGGL((k_m2l_bf16<4>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
+                             Mb, Lc);
+      } else {
+        throw Error(GP_ERR_INTERNAL, "BF16 M2L runs on the FP32 operator only");
+      }
+    } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && ldm % 16 == 0 && ldm <= 128 && MT <= 8) {
+      if constexpr (sizeof(T) == 8) {
+        ProfScope pl("m2l_f64_mfma", s, fl, by);
+        if (MT <= 4)
+          hipLaunchKernelGGL((k_m2l_f64<1>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                             f.G, M, Lc);
+        else
+          hipLaunchKernelGGL((k_m2l_f64<2>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                             f.G, M, Lc);
+      }
+    } else {
+      ProfScope pl(f32 ? "m2l_f32" : "m2l_f64", s, fl, by);
+      hipLaunchKernelGGL((k_m2l<T>), dim3(f.n_targets, nrc), dim3(256), lds1, s, n, n3p,
+                         f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, f.G,
+                         nr, M, Lc, ldm);
+    }
     GP_LAUNCH_CHECK();
   }
   // L2L, level 2 to the leaves.
@@ -312,12 +523,13 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, T *L
 }
 
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
-                   double *Lc, int ldm, double *y, int ldy, hipStream_t s) {
-  run_far_field_t<double>(f, z, ldz, nr, M, Lc, ldm, y, ldy, s);
+                   double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s) {
+  run_far_field_t<double>(f, z, ldz, nr, M, nullptr, Lc, ldm, m2l, y, ldy, s);
 }
-void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, float *Lc,
-                   int ldm, double *y, int ldy, hipStream_t s) {
-  run_far_field_t<float>(f, z, ldz, nr, M, Lc, ldm, y, ldy, s);
+void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,
+                   float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s) {
+  run_far_field_t<float>(f, z, ldz, nr, M, reinterpret_cast<__bf16 *>(Mb), Lc, ldm, m2l, y, ldy,
+                         s);
 }
 
 }  // namespace gp

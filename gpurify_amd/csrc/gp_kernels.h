@@ -55,14 +55,19 @@ struct FmmDev {
   const int *box_level = nullptr;
   const double *inv_half = nullptr;  // per level, scaled units
   const double *area = nullptr;      // per sorted panel, scaled units
+  int n3p = 0;                       // n^3 rounded up to a multiple of 32
   const T *w1d = nullptr;            // per sorted panel 3 * n
   const T *m2m1d = nullptr;          // 24 * n * n
-  const T *G = nullptr;              // 343 * n^3 * n^3
+  const T *G = nullptr;              // 343 * n3p * n3p (zero padded)
+  const void *Gbf = nullptr;         // same, BF16 (FP32 operator only)
 };
+// Far field y += P_far z.  Multipoles M: n_boxes x ldm x n3p; locals Lc:
+// n_boxes x n3 x ldm.  m2l selects the M2L kernel (GP_M2L_*): BF16 needs Mb
+// (BF16 multipoles, same layout), FP64 uses matrix cores when ldm allows.
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
-                   double *Lc, int ldm, double *y, int ldy, hipStream_t s);
-void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, float *Lc,
-                   int ldm, double *y, int ldy, hipStream_t s);
+                   double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
+void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,
+                   float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
 
 // ---------------------------------------------------------------- Krylov
 size_t kr_partial_size(int nvec_max, int ld);

@@ -75,6 +75,15 @@ std::vector<T> cast_vec(const std::vector<double> &v) {
   return std::vector<T>(v.begin(), v.end());
 }
 
+// FP32 -> BF16 bits, round to nearest even (NaN stays NaN).
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
 // Scaled SoA arrays of the panels in the order 'perm' (identity if null).
 void scaled_arrays(const PanelGeom &g, const uint32_t *perm, const double c[3], double L0,
                    std::vector<double> &verts, std::vector<double> &cen,
@@ -172,6 +181,8 @@ struct FmmSystem {
   int64_t nnz = 0;
   long pairs = 0;
   bool have32 = false;
+  int n3p = 0;
+  int m2l_inner = GP_M2L_BF16, m2l_outer = GP_M2L_FP64;
   DBuf<double> verts, cen, area, inv_diag;
   DBuf<uint32_t> cond;
   DBuf<int> leaf_lo, col_off, cols, panel_leaf;
@@ -182,6 +193,7 @@ struct FmmSystem {
   DBuf<double> inv_half;
   DBuf<double> w1d64, m2m64, G64;
   DBuf<float> w1d32, m2m32, G32;
+  DBuf<uint16_t> Gbf;
   DBuf<int> coff, cidx;
   NearTiles nt;
   FmmDev<double> f64;
@@ -189,6 +201,7 @@ struct FmmSystem {
   int ld = 0;
   DBuf<double> z64, M64, Lc64;
   DBuf<float> z32, M32, Lc32;
+  DBuf<uint16_t> Mbf;
   long matvecs = 0;
 
   void setup(const PanelGeom &g, const gp_cap_opts &o, bool need32, hipStream_t stream) {
@@ -197,6 +210,12 @@ struct FmmSystem {
     ncond = g.ncond;
     n = (int)o.fmm_order;
     if (n < 2 || n > 6) throw Error(GP_ERR_INVALID_INPUT, "fmm_order must be in [2, 6]");
+    m2l_inner = (int)o.m2l_inner;
+    m2l_outer = (int)o.m2l_outer;
+    if (m2l_inner != GP_M2L_BF16 && m2l_inner != GP_M2L_FP32)
+      throw Error(GP_ERR_INVALID_INPUT, "m2l_inner must be GP_M2L_BF16 or GP_M2L_FP32");
+    if (m2l_outer != GP_M2L_FP64 && m2l_outer != GP_M2L_FP64_VALU)
+      throw Error(GP_ERR_INVALID_INPUT, "m2l_outer must be GP_M2L_FP64 or GP_M2L_FP64_VALU");
     L = o.max_level ? (int)o.max_level : choose_depth(g, (int)o.leaf_target);
     if (L < 2) L = 2;
     build_tree(g, L, t);
@@ -250,6 +269,7 @@ struct FmmSystem {
 
     // FMM tables.
     const int n3 = n * n * n;
+    n3p = round_up(n3, 32);
     auto nodes = cheb_nodes(n);
     std::vector<double> hw1d((size_t)N * 3 * n);
     const double half_L = t.level_half(L);
@@ -280,13 +300,13 @@ struct FmmSystem {
             }
         }
     // Uncompressed M2L matrices on the reference box, bbfmm.rs:156-166.
-    std::vector<double> hG((size_t)343 * n3 * n3, 0.0);
+    std::vector<double> hG((size_t)343 * n3p * n3p, 0.0);
     for (int o0 = -3; o0 <= 3; ++o0)
       for (int o1 = -3; o1 <= 3; ++o1)
         for (int o2 = -3; o2 <= 3; ++o2) {
           if (std::max(std::abs(o0), std::max(std::abs(o1), std::abs(o2))) <= 1) continue;
           int oi = (o0 + 3) * 49 + (o1 + 3) * 7 + (o2 + 3);
-          double* G = &hG[(size_t)oi * n3 * n3];
+          double *G = &hG[(size_t)oi * n3p * n3p];
           for (int m = 0; m < n3; ++m) {
             int ma = m / (n * n), mb = (m / n) % n, mc = m % n;
             for (int l = 0; l < n3; ++l) {
@@ -295,7 +315,7 @@ struct FmmSystem {
               double dy = nodes[mb] - nodes[lb] - 2.0 * o1;
               double dz = nodes[mc] - nodes[lc] - 2.0 * o2;
               double r = std::sqrt(dx * dx + dy * dy + dz * dz);
-              G[(size_t)m * n3 + l] = r > 0.0 ? 1.0 / r : 0.0;
+              G[(size_t)m * n3p + l] = r > 0.0 ? 1.0 / r : 0.0;
             }
           }
         }
@@ -325,6 +345,9 @@ struct FmmSystem {
       w1d32 = to_device(cast_vec<float>(hw1d), s);
       m2m32 = to_device(cast_vec<float>(hm2m), s);
       G32 = to_device(cast_vec<float>(hG), s);
+      std::vector<uint16_t> hb(hG.size());
+      for (size_t e = 0; e < hG.size(); ++e) hb[e] = bf16_bits((float)hG[e]);
+      Gbf = to_device(hb, s);
     }
     auto fill = [&](auto &f) {
       f.n = n;
@@ -346,6 +369,7 @@ struct FmmSystem {
       f.box_level = box_level.p;
       f.inv_half = inv_half.p;
       f.area = area.p;
+      f.n3p = n3p;
     };
     fill(f64);
     f64.w1d = w1d64.p;
@@ -355,6 +379,7 @@ struct FmmSystem {
     f32.w1d = w1d32.p;
     f32.m2m1d = m2m32.p;
     f32.G = G32.p;
+    f32.Gbf = Gbf.p;
     std::vector<int> hco, hci;
     cond_csr(g, t.perm.data(), hco, hci);
     coff = to_device(hco, s);
@@ -365,16 +390,23 @@ struct FmmSystem {
     if (ld_ == ld) return;
     ld = ld_;
     const size_t vec = (size_t)N * ld;
-    const size_t exp = (size_t)t.n_boxes() * n * n * n * ld;
+    const size_t mexp = (size_t)t.n_boxes() * n3p * ld;   // multipoles, padded
+    const size_t lexp = (size_t)t.n_boxes() * n * n * n * ld;
     z64.alloc(vec);
     z64.zero(s);
-    M64.alloc(exp);
-    Lc64.alloc(exp);
+    M64.alloc(mexp);
+    M64.zero(s);
+    Lc64.alloc(lexp);
     if (have32) {
       z32.alloc(vec);
       z32.zero(s);
-      M32.alloc(exp);
-      Lc32.alloc(exp);
+      M32.alloc(mexp);
+      M32.zero(s);
+      Lc32.alloc(lexp);
+      if (m2l_inner == GP_M2L_BF16 && ld % 16 == 0 && ld <= 128) {
+        Mbf.alloc(mexp);
+        Mbf.zero(s);
+      }
     }
   }
 
@@ -385,12 +417,13 @@ struct FmmSystem {
     if (prec == 0) {
       kr_precond_cast(v, dinv, z64.p, N, ld, nr, s);
       launch_near_apply(nt, nf64.p, z64.p, ld, y, ld, nr, s);
-      run_far_field(f64, z64.p, ld, nr, M64.p, Lc64.p, ld, y, ld, s);
+      run_far_field(f64, z64.p, ld, nr, M64.p, Lc64.p, ld, m2l_outer, y, ld, s);
     } else {
       if (!have32) throw Error(GP_ERR_INTERNAL, "FP32 operator not assembled");
       kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
       launch_near_apply(nt, nf32.p, z32.p, ld, y, ld, nr, s);
-      run_far_field(f32, z32.p, ld, nr, M32.p, Lc32.p, ld, y, ld, s);
+      const int v = (m2l_inner == GP_M2L_BF16 && Mbf.p) ? GP_M2L_BF16 : GP_M2L_FP32;
+      run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, v, y, ld, s);
     }
   }
 };
@@ -446,6 +479,7 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
     DBuf<double> V2((size_t)nc * vec);
     GP_HIP(hipMemcpyAsync(V2.p, V.p, (size_t)vcap * vec * sizeof(double),
                           hipMemcpyDeviceToDevice, s));
+    GP_HIP(hipStreamSynchronize(s));  // the old basis is freed below
     V = std::move(V2);
     partial.alloc(kr_partial_size(nc, ld));
     vcap = nc;

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1459,6 +1460,38 @@ int or_cap_solve_fmm(int n, const double *verts4, const uint32_t *cond, int ncon
                      double *resid_out) {
   PanelSet s = build_set(n, verts4, cond);
   return cap_fmm(s, ncond, order, max_level, tol, m2l_mode, C, iters_out, resid_out);
+}
+
+int or_cap_solve_fmm_cols(int n, const double *verts4, const uint32_t *cond, int ncond,
+                          int order, int max_level, double tol, int m2l_mode, int j0, int nj,
+                          double *C, int *iters_out, double *seconds_out) {
+  // cap/fmm_solver.rs:147-172 for the conductor columns [j0, j0 + nj) only:
+  // operator setup once, then one GMRES per column.
+  auto t0 = std::chrono::steady_clock::now();
+  PanelSet s = build_set(n, verts4, cond);
+  FmmCapOp op;
+  op.init(s, order, max_level, m2l_mode);
+  DiagJacobi pre;
+  pre.inv.assign(n, 1.0);
+  for (long i = 0; i < n; ++i) {
+    double d = 1.0;
+    op.diagonal(i, d);
+    pre.inv[i] = std::fabs(d) > 1e-30 ? 1.0 / d : 1.0;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  std::fill(C, C + (size_t)ncond * ncond, 0.0);
+  for (int j = j0; j < j0 + nj && j < ncond; ++j) {
+    auto b = rhs_unit(s, j);
+    auto r = gmres(op, b.data(), &pre, tol, 200, 50);
+    if (iters_out) iters_out[j - j0] = r.iterations;
+    accumulate_c(s, r.x.data(), j, ncond, C, nullptr);
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  if (seconds_out) {
+    seconds_out[0] = std::chrono::duration<double>(t1 - t0).count();
+    seconds_out[1] = std::chrono::duration<double>(t2 - t1).count();
+  }
+  return 0;
 }
 
 int or_fmm_cap_apply(int n, const double *verts4, int order, int max_level, int m2l_mode,

@@ -52,6 +52,12 @@ int or_cap_solve_fmm(int n, const double *verts4, const uint32_t *cond, int ncon
                      int order, int max_level, double tol, int m2l_mode,
                      double *C, int *iters_out, double *resid_out);
 
+/* As or_cap_solve_fmm for the conductor columns [j0, j0 + nj) only (the
+ * bounded CPU sample).  seconds_out[0] = operator setup, [1] = GMRES. */
+int or_cap_solve_fmm_cols(int n, const double *verts4, const uint32_t *cond, int ncond,
+                          int order, int max_level, double tol, int m2l_mode, int j0, int nj,
+                          double *C, int *iters_out, double *seconds_out);
+
 /* The precorrected operator y = P sigma of cap/fmm_solver.rs:105-130 for a
  * given sigma (one column). */
 int or_fmm_cap_apply(int n, const double *verts4, int order, int max_level, int m2l_mode,

@@ -48,6 +48,8 @@ def lib():
         L.or_cap_solve_dense.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, _d, _d]
         L.or_cap_solve_fmm.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, C.c_int,
                                        C.c_double, C.c_int, _d, _int, _d]
+        L.or_cap_solve_fmm_cols.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, C.c_int,
+                                            C.c_double, C.c_int, C.c_int, C.c_int, _d, _int, _d]
         L.or_fmm_cap_apply.argtypes = [C.c_int, _d, C.c_int, C.c_int, C.c_int, _d, _d]
         L.or_fmm_matvec.argtypes = [C.c_int, _d, _d, C.c_int, C.c_int, C.c_int, C.c_int, _d]
         L.or_dense_p2p.argtypes = [C.c_int, _d, _d, _d]
@@ -138,6 +140,21 @@ def cap_fmm(verts4, cond, ncond, order=4, max_level=4, tol=1e-8, m2l_mode=0):
     if rc:
         raise RuntimeError(f"oracle FMM solve failed ({rc})")
     return Cm, it, rs
+
+
+def cap_fmm_cols(verts4, cond, ncond, j0, nj, order=4, max_level=4, tol=1e-8, m2l_mode=0):
+    """FMM solve of conductor columns [j0, j0+nj): (C, iters, (setup_s, solve_s))."""
+    v = _f64(verts4).reshape(-1, 4, 3)
+    n = v.shape[0]
+    c = np.ascontiguousarray(cond, dtype=np.uint32)
+    Cm = np.zeros((ncond, ncond))
+    it = np.zeros(max(1, nj), dtype=np.int32)
+    sec = np.zeros(2)
+    rc = lib().or_cap_solve_fmm_cols(n, _p(v), _p(c, _u32), ncond, order, max_level, tol,
+                                     m2l_mode, j0, nj, _p(Cm), it.ctypes.data_as(_int), _p(sec))
+    if rc:
+        raise RuntimeError(f"oracle FMM solve failed ({rc})")
+    return Cm, it, (float(sec[0]), float(sec[1]))
 
 
 def fmm_cap_apply(verts4, sigma, order=4, max_level=4, m2l_mode=0):

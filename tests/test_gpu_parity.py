@@ -57,18 +57,35 @@ def test_fmm_operator_matches_oracle(ctx, order, level):
     for j in range(3):
         yo = O.fmm_cap_apply(v, X[:, j], order, level, m2l_mode=1)
         assert np.linalg.norm(Y[:, j] - yo) / np.linalg.norm(yo) < 1e-12
-        # the reference's CPQR-compressed operator differs by its truncation only
+        # the reference's CPQR-compressed operator differs by its truncation
+        # only (~3e-4 at n=3, ~3e-5 at n=5); 1e-3 is the reference's own FMM
+        # tolerance (crates/pex/tests/fmm.rs:46)
         yr = O.fmm_cap_apply(v, X[:, j], order, level, m2l_mode=0)
-        assert np.linalg.norm(Y[:, j] - yr) / np.linalg.norm(yr) < 1e-4
+        assert np.linalg.norm(Y[:, j] - yr) / np.linalg.norm(yr) < 1e-3
 
 
-def test_fmm_operator_fp32_close(ctx):
+def test_fmm_operator_low_precision_close(ctx):
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     X = np.random.default_rng(1).normal(size=(len(v), 16))
-    o = G.default_opts(method=G.GP_METHOD_FMM)
+    o = G.default_opts(method=G.GP_METHOD_FMM, m2l_inner=G.GP_M2L_FP32)
     Y64 = ctx.fmm_apply(v, X, o)
     Y32 = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
     assert np.linalg.norm(Y32 - Y64) / np.linalg.norm(Y64) < 1e-5
+    o.m2l_inner = G.GP_M2L_BF16
+    Yb = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
+    e = np.linalg.norm(Yb - Y64) / np.linalg.norm(Y64)
+    assert 1e-7 < e < 2e-3  # BF16 interaction matrices and multipoles
+
+
+def test_m2l_matrix_cores_match_valu(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    X = np.random.default_rng(5).normal(size=(len(v), 40))
+    for order, level in ((4, 4), (5, 4), (3, 3)):
+        o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, max_level=level)
+        Ym = ctx.fmm_apply(v, X, o)
+        o.m2l_outer = G.GP_M2L_FP64_VALU
+        Yv = ctx.fmm_apply(v, X, o)
+        assert np.linalg.norm(Ym - Yv) / np.linalg.norm(Yv) < 1e-13
 
 
 def test_fmm_single_and_block_rhs_agree(ctx):
