@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--order", type=int, default=5)
     ap.add_argument("--leaf-target", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--inner-tol", type=float, default=1e-2)
     ap.add_argument("--plain-rhs", type=int, default=2, help="RHS sampled for the 1-RHS FP64 rate")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

@@ -155,8 +155,8 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->max_restarts = 50;  // cap/fmm_solver.rs:164
   o->rhs_block = 0;
   o->tol = 1.0e-8;       // bridge.rs:403
-  o->inner_tol = 1.0e-5;
-  o->max_refine = 8;
+  o->inner_tol = 1.0e-2;
+  o->max_refine = 30;
   o->m2l_inner = GP_M2L_BF16;
   o->m2l_outer = GP_M2L_FP64;
   o->strict = 0;
