@@ -42,6 +42,7 @@ GP_M2L_FP64 = 0
 GP_M2L_FP32 = 1
 GP_M2L_BF16 = 2
 GP_M2L_FP64_VALU = 3
+GP_M2L_BF16_VALU = 4
 
 
 class GpPanels(C.Structure):

@@ -267,6 +267,108 @@ __global__ __launch_bounds__(256) void k_m2l_bf16(int n3, int n3p, int nr, int l
       }
 }
 
+// Pipelined BF16 MFMA M2L for n3p = 16 MT (MT = 4 or 8) and ldm <= 128.
+// Per pair the source multipole rows (A operand, RT * 16 rows of n3p) are
+// staged once per work-group in LDS, rows padded by 8 elements so that the
+// 8 lanes of each 128-byte read phase hit distinct banks, and read by all
+// four wavefronts; wavefront w owns node tiles [w MT/4, (w+1) MT/4) and keeps
+// their G_off fragments (B operand) in registers.  The next pair's
+// multipole rows and G_off fragments are prefetched into registers while
+// the current pair runs on the matrix cores.
+template <int MT>
+__global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
+                                                       const int *__restrict__ targets,
+                                                       const int *__restrict__ il_off,
+                                                       const int *__restrict__ il_src,
+                                                       const int *__restrict__ il_oidx,
+                                                       const int *__restrict__ box_level,
+                                                       const double *__restrict__ inv_half,
+                                                       const __bf16 *__restrict__ G,
+                                                       const __bf16 *__restrict__ Mb,
+                                                       float *__restrict__ Lc) {
+  constexpr int NP = 16 * MT;      // n3p
+  constexpr int KS = NP / 32;      // contraction steps of 32
+  constexpr int MTW = MT / 4;      // node tiles per wavefront
+  constexpr int LDA = NP + 8;      // padded LDS row (elements)
+  constexpr int VPR = NP / 8;      // 16-byte vectors per multipole row
+  constexpr int APT = 128 * VPR / 256;  // prefetch vectors per thread (<= 128 rows)
+  __shared__ __attribute__((aligned(16))) __bf16 sA[128 * LDA];
+  const int t = targets[blockIdx.x];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  const int RT = (nr + 15) >> 4;
+  const int AV = RT * 16 * VPR;    // vectors of one pair's A block
+  const size_t MS = (size_t)ldm * NP;
+  f32x4 acc[8][MTW];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int q0 = il_off[t], q1 = il_off[t + 1];
+  bf16x8 apre[APT];
+  bf16x8 bcur[MTW][KS], bnext[MTW][KS];
+  auto load_a = [&](int q) {
+    const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Mb + (size_t)il_src[q] * MS);
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < AV) apre[i] = src[v];
+    }
+  };
+  auto load_b = [&](int q, bf16x8 (&dst)[MTW][KS]) {
+    const __bf16 *Go = G + (size_t)il_oidx[q] * NP * NP;
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int k = 0; k < KS; ++k)
+        dst[j][k] = *(const bf16x8 *)(Go + (size_t)((wave * MTW + j) * 16 + li) * NP + k * 32 + lk);
+  };
+  if (q0 < q1) {
+    load_a(q0);
+    load_b(q0, bcur);
+  }
+  for (int q = q0; q < q1; ++q) {
+    __syncthreads();  // previous pair's LDS reads are done
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < AV) *(bf16x8 *)(sA + (v / VPR) * LDA + (v % VPR) * 8) = apre[i];
+    }
+    __syncthreads();
+    if (q + 1 < q1) {
+      load_a(q + 1);
+      load_b(q + 1, bnext);
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+#pragma unroll
+      for (int rt = 0; rt < 8; ++rt) {
+        if (rt < RT) {
+          const bf16x8 a = *(const bf16x8 *)(sA + (rt * 16 + li) * LDA + k * 32 + lk);
+#pragma unroll
+          for (int j = 0; j < MTW; ++j)
+            acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bcur[j][k], acc[rt][j], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int k = 0; k < KS; ++k) bcur[j][k] = bnext[j][k];
+  }
+  const float ih = (float)inv_half[box_level[t]];
+  float *out = Lc + (size_t)t * n3 * ldm;
+#pragma unroll
+  for (int rt = 0; rt < 8; ++rt)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = rt * 16 + (lane >> 4) * 4 + v, m = (wave * MTW + j) * 16 + li;
+        if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
+      }
+}
+
 // MFMA M2L in FP64 (v_mfma_f64_16x16x4f64), same tiling.  Each lane loads 4
 // consecutive contraction entries (32 bytes) and feeds them to 4 successive
 // MFMAs: in step j lane group g = lane / 16 supplies contraction index
@@ -415,6 +517,313 @@ __global__ __launch_bounds__(256) void k_l2p(int n, const int *__restrict__ leaf
   }
 }
 
+// ---------------------------------------------------------------- tiled kernels
+// Used when the RHS block ldm is a multiple of 16 (<= 128).  Thread t of a
+// 256-thread group: mg = t % 16 (node / point lane), rg = t / 16 (RHS group
+// of RB = ldm / 16 consecutive columns).
+
+constexpr int TP_KP = 16;  // points per LDS stage (P2M)
+constexpr int TL_PT = 32;  // points per tile (L2P)
+constexpr int TL_KM = 32;  // nodes per LDS stage (L2P)
+constexpr int TM_RC = 32;  // RHS columns per work-group (M2M, L2L)
+
+// Tensor weight S(p, m) = wx[a] wy[b] wz[c] of a point's 1-D weights.
+template <typename T>
+__device__ inline T tensor_weight(const T *__restrict__ w, int n, int m) {
+  const int nn = n * n;
+  return w[m / nn] * w[n + (m / n) % n] * w[2 * n + m % n];
+}
+
+// P2M as a per-leaf product M^T[m][r] = sum_p S(p, m) area_p z[p][r]; S and
+// q = area * z are staged for TP_KP points at a time, each thread owns MB
+// nodes (mg + 16 i) x RB columns (rg RB + j) in registers.
+template <typename T, int MB>
+__global__ __launch_bounds__(256) void k_p2m_tiled(int n, int n3p, const int *__restrict__ leaf_lo,
+                                                   int leaf_box0, const T *__restrict__ w1d,
+                                                   const double *__restrict__ area,
+                                                   const T *__restrict__ z, int ldz, int nr,
+                                                   T *__restrict__ M, __bf16 *__restrict__ Mb,
+                                                   int ldm) {
+  __shared__ T sS[TP_KP][16 * MB];
+  __shared__ T sQ[TP_KP][128];
+  const int li = blockIdx.x, tid = threadIdx.x;
+  const int mg = tid & 15, rg = tid >> 4;
+  const int RB = ldm >> 4;
+  const int n3 = n * n * n, NP = 16 * MB;
+  T acc[MB][8];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = 0;
+  const int pb = leaf_lo[li], pe = leaf_lo[li + 1];
+  for (int p0 = pb; p0 < pe; p0 += TP_KP) {
+    const int np = min(TP_KP, pe - p0);
+    for (int e = tid; e < TP_KP * NP; e += 256) {
+      int pp = e / NP, m = e - pp * NP;
+      sS[pp][m] = (pp < np && m < n3) ? tensor_weight(w1d + (size_t)(p0 + pp) * 3 * n, n, m) : (T)0;
+    }
+    for (int e = tid; e < TP_KP * ldm; e += 256) {
+      int pp = e / ldm, r = e - pp * ldm;
+      sQ[pp][r] = (pp < np && r < nr) ? (T)area[p0 + pp] * z[(size_t)(p0 + pp) * ldz + r] : (T)0;
+    }
+    __syncthreads();
+    for (int pp = 0; pp < np; ++pp) {
+      T sv[MB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) sv[i] = sS[pp][mg + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (j < RB) {
+          const T q = sQ[pp][rg * RB + j];
+#pragma unroll
+          for (int i = 0; i < MB; ++i) acc[i][j] += sv[i] * q;
+        }
+    }
+    __syncthreads();
+  }
+  const size_t base = (size_t)(leaf_box0 + li) * ldm * n3p;
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int m = mg + 16 * i, r = rg * RB + j;
+      if (j < RB && m < n3 && r < nr) store_mult(M, Mb, base + (size_t)r * n3p + m, acc[i][j]);
+    }
+}
+
+// L2P as a per-leaf product y[p][r] += sum_m S(p, m) Lc[m][r]: node chunks of
+// Lc and S staged in LDS; each thread owns 2 points x RB columns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict__ leaf_lo,
+                                                   int leaf_box0, const T *__restrict__ w1d, int nr,
+                                                   const T *__restrict__ Lc, int ldm,
+                                                   double *__restrict__ y, int ldy) {
+  __shared__ T sL[TL_KM][128];
+  __shared__ T sS[TL_PT][TL_KM + 1];
+  const int li = blockIdx.x, tid = threadIdx.x;
+  const int pg = tid & 15, rg = tid >> 4;
+  const int RB = ldm >> 4;
+  const int n3 = n * n * n;
+  const T *ll = Lc + (size_t)(leaf_box0 + li) * n3 * ldm;
+  const int pb = leaf_lo[li], pe = leaf_lo[li + 1];
+  for (int p0 = pb; p0 < pe; p0 += TL_PT) {
+    const int np = min(TL_PT, pe - p0);
+    T acc[2][8];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[q][j] = 0;
+    for (int m0 = 0; m0 < n3; m0 += TL_KM) {
+      for (int e = tid; e < TL_KM * ldm; e += 256) {
+        int mm = e / ldm, r = e - mm * ldm;
+        sL[mm][r] = (m0 + mm < n3) ? ll[(size_t)(m0 + mm) * ldm + r] : (T)0;
+      }
+      for (int e = tid; e < TL_PT * TL_KM; e += 256) {
+        int pp = e / TL_KM, mm = e - pp * TL_KM;
+        sS[pp][mm] = (pp < np && m0 + mm < n3)
+                         ? tensor_weight(w1d + (size_t)(p0 + pp) * 3 * n, n, m0 + mm)
+                         : (T)0;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int mm = 0; mm < TL_KM; ++mm) {
+        const T s0 = sS[pg][mm], s1 = sS[pg + 16][mm];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (j < RB) {
+            const T l = sL[mm][rg * RB + j];
+            acc[0][j] += s0 * l;
+            acc[1][j] += s1 * l;
+          }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int pp = pg + 16 * q, r = rg * RB + j;
+        if (j < RB && pp < np && r < nr) y[(size_t)(p0 + pp) * ldy + r] += (double)acc[q][j];
+      }
+  }
+}
+
+// M2M with the order N fixed at compile time: per parent box and 32-column
+// RHS chunk, for each child the z and y contractions of every (r, a) slice
+// run in registers, one LDS exchange, then the x contraction accumulates the
+// parent's (r, bc) outputs in registers.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_m2m_tiled(int n3p, int box0,
+                                                   const int *__restrict__ child_off,
+                                                   const int *__restrict__ child_idx,
+                                                   const int *__restrict__ octant,
+                                                   const T *__restrict__ m2m1d, int nr,
+                                                   T *__restrict__ M, __bf16 *__restrict__ Mb,
+                                                   int ldm) {
+  constexpr int NN = N * N;
+  constexpr int ITEMS = (TM_RC * NN + 255) / 256;
+  __shared__ T t2[TM_RC * N * NN];
+  const int b = box0 + blockIdx.x, r0 = blockIdx.y * TM_RC;
+  const int rc = min(TM_RC, nr - r0);
+  const int tid = threadIdx.x;
+  T acc[ITEMS][N];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k)
+#pragma unroll
+    for (int a = 0; a < N; ++a) acc[k][a] = 0;
+  for (int q = child_off[b]; q < child_off[b + 1]; ++q) {
+    const int ch = child_idx[q], o = octant[ch];
+    const T *mx = m2m1d + (o * 3 + 0) * NN, *my = m2m1d + (o * 3 + 1) * NN,
+            *mz = m2m1d + (o * 3 + 2) * NN;
+    if (tid < rc * N) {
+      const int r = tid / N, a = tid - r * N;
+      const T *src = M + (size_t)ch * ldm * n3p + (size_t)(r0 + r) * n3p + a * NN;
+      T in[NN], u[NN];
+#pragma unroll
+      for (int e = 0; e < NN; ++e) in[e] = src[e];
+      // z: u[b][cp] = sum_c mz[cp][c] in[b][c]
+#pragma unroll
+      for (int bb = 0; bb < N; ++bb)
+#pragma unroll
+        for (int cp = 0; cp < N; ++cp) {
+          T s = 0;
+#pragma unroll
+          for (int c = 0; c < N; ++c) s += mz[cp * N + c] * in[bb * N + c];
+          u[bb * N + cp] = s;
+        }
+      // y: t2[r][a][bp][cp] = sum_b my[bp][b] u[b][cp]
+      T *dst = t2 + ((size_t)r * N + a) * NN;
+#pragma unroll
+      for (int bp = 0; bp < N; ++bp)
+#pragma unroll
+        for (int cp = 0; cp < N; ++cp) {
+          T s = 0;
+#pragma unroll
+          for (int bb = 0; bb < N; ++bb) s += my[bp * N + bb] * u[bb * N + cp];
+          dst[bp * N + cp] = s;
+        }
+    }
+    __syncthreads();
+    // x: acc[(r, bc)][ap] += sum_a mx[ap][a] t2[r][a][bc]
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int item = tid + 256 * k;
+      if (item < rc * NN) {
+        const int r = item / NN, bc = item - r * NN;
+        T tv[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) tv[a] = t2[((size_t)r * N + a) * NN + bc];
+#pragma unroll
+        for (int ap = 0; ap < N; ++ap) {
+          T s = 0;
+#pragma unroll
+          for (int a = 0; a < N; ++a) s += mx[ap * N + a] * tv[a];
+          acc[k][ap] += s;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const size_t base = (size_t)b * ldm * n3p;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int item = tid + 256 * k;
+    if (item < rc * NN) {
+      const int r = item / NN, bc = item - r * NN;
+#pragma unroll
+      for (int ap = 0; ap < N; ++ap)
+        store_mult(M, Mb, base + (size_t)(r0 + r) * n3p + ap * NN + bc, acc[k][ap]);
+    }
+  }
+}
+
+// L2L with N fixed: per child box and 32-column chunk, the transposed 1-D
+// interpolations of the parent's local expansion: z^T and y^T in registers
+// per (r, a) slice, one LDS exchange, x^T per (r, bc), added to the child.
+// (The reference applies x^T first; the three 1-D maps commute.)
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_l2l_tiled(int box0, const int *__restrict__ parent,
+                                                   const int *__restrict__ octant,
+                                                   const T *__restrict__ m2m1d, int nr,
+                                                   T *__restrict__ Lc, int ldm) {
+  constexpr int NN = N * N, N3 = N * N * N;
+  constexpr int ITEMS = (TM_RC * NN + 255) / 256;
+  __shared__ T t2[TM_RC * N * NN];
+  const int ch = box0 + blockIdx.x, r0 = blockIdx.y * TM_RC;
+  const int rc = min(TM_RC, nr - r0);
+  const int tid = threadIdx.x;
+  const int p = parent[ch], o = octant[ch];
+  const T *mx = m2m1d + (o * 3 + 0) * NN, *my = m2m1d + (o * 3 + 1) * NN,
+          *mz = m2m1d + (o * 3 + 2) * NN;
+  const T *lp = Lc + (size_t)p * N3 * ldm;
+  if (tid < rc * N) {
+    const int r = tid % rc, a = tid / rc;
+    T in[NN], u[NN];
+#pragma unroll
+    for (int e = 0; e < NN; ++e) in[e] = lp[(size_t)(a * NN + e) * ldm + r0 + r];
+    // z^T: u[b][cp] = sum_c mz[c][cp] in[b][c]
+#pragma unroll
+    for (int bb = 0; bb < N; ++bb)
+#pragma unroll
+      for (int cp = 0; cp < N; ++cp) {
+        T s = 0;
+#pragma unroll
+        for (int c = 0; c < N; ++c) s += mz[c * N + cp] * in[bb * N + c];
+        u[bb * N + cp] = s;
+      }
+    // y^T: t2[r][a][bp][cp] = sum_b my[b][bp] u[b][cp]
+    T *dst = t2 + ((size_t)r * N + a) * NN;
+#pragma unroll
+    for (int bp = 0; bp < N; ++bp)
+#pragma unroll
+      for (int cp = 0; cp < N; ++cp) {
+        T s = 0;
+#pragma unroll
+        for (int bb = 0; bb < N; ++bb) s += my[bb * N + bp] * u[bb * N + cp];
+        dst[bp * N + cp] = s;
+      }
+  }
+  __syncthreads();
+  T *lc = Lc + (size_t)ch * N3 * ldm;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int item = tid + 256 * k;
+    if (item < rc * NN) {
+      const int r = item % rc, bc = item / rc;
+      T tv[N];
+#pragma unroll
+      for (int a = 0; a < N; ++a) tv[a] = t2[((size_t)r * N + a) * NN + bc];
+#pragma unroll
+      for (int ap = 0; ap < N; ++ap) {
+        T s = 0;
+#pragma unroll
+        for (int a = 0; a < N; ++a) s += mx[a * N + ap] * tv[a];
+        lc[(size_t)(ap * NN + bc) * ldm + r0 + r] += s;
+      }
+    }
+  }
+}
+
+template <typename T, int N>
+void launch_m2m_tiled(int n3p, int box0, int nb, int nrc32, const FmmDev<T> &f, int nr, T *M,
+                      __bf16 *Mb, int ldm, hipStream_t s) {
+  hipLaunchKernelGGL((k_m2m_tiled<T, N>), dim3(nb, nrc32), dim3(256), 0, s, n3p, box0, f.child_off,
+                     f.child_idx, f.octant, f.m2m1d, nr, M, Mb, ldm);
+}
+template <typename T, int N>
+void launch_l2l_tiled(int box0, int nb, int nrc32, const FmmDev<T> &f, int nr, T *Lc, int ldm,
+                      hipStream_t s) {
+  hipLaunchKernelGGL((k_l2l_tiled<T, N>), dim3(nb, nrc32), dim3(256), 0, s, box0, f.parent,
+                     f.octant, f.m2m1d, nr, Lc, ldm);
+}
+
+// In-place rounding to BF16 (round to nearest even) and back.
+__global__ __launch_bounds__(256) void k_round_bf16(float *__restrict__ p, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    p[e] = (float)(__bf16)p[e];
+}
+
 // Zero fill (instead of a memset node, so the pass can be graph-captured).
 template <typename T>
 __global__ __launch_bounds__(256) void k_fill_zero(T *__restrict__ p, size_t n) {
@@ -438,11 +847,30 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   if (want_bf16 && (!Mb || !f.Gbf || ldm % 16 || ldm > 128 || n3p % 32))
     throw Error(GP_ERR_INTERNAL, "BF16 M2L needs 16 <= ld <= 128 (multiple of 16) and its tables");
   __bf16 *mb = want_bf16 ? Mb : nullptr;
+  // Register-tiled kernels for RHS blocks that are a multiple of 16.
+  const bool tiled = (ldm % 16 == 0) && ldm <= 128 && n >= 2 && n <= 6;
+  const int nrc32 = (nr + TM_RC - 1) / TM_RC;
   {
     ProfScope ps(f32 ? "p2m_f32" : "p2m_f64", s, 2.0 * f.n_panels * n3 * nr,
                  (double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes);
-    hipLaunchKernelGGL((k_p2m<T>), dim3(nleaf, nrc), dim3(256), 0, s, n, n3p, f.leaf_lo, leaf0,
-                       f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+    if (tiled) {
+      const int MB = n3p / 16;
+      if (MB <= 2)
+        hipLaunchKernelGGL((k_p2m_tiled<T, 2>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
+                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+      else if (MB <= 4)
+        hipLaunchKernelGGL((k_p2m_tiled<T, 4>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
+                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+      else if (MB <= 8)
+        hipLaunchKernelGGL((k_p2m_tiled<T, 8>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
+                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+      else
+        hipLaunchKernelGGL((k_p2m_tiled<T, 14>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
+                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+    } else {
+      hipLaunchKernelGGL((k_p2m<T>), dim3(nleaf, nrc), dim3(256), 0, s, n, n3p, f.leaf_lo, leaf0,
+                         f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+    }
     GP_LAUNCH_CHECK();
   }
   // M2M, finest to level 2 (multipoles above level 2 are never used).
@@ -451,8 +879,19 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     if (nb == 0) continue;
     int nch = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch, (nb + nch) * exp_bytes);
-    hipLaunchKernelGGL((k_m2m<T>), dim3(nb, nrc), dim3(256), 2 * lds1, s, n, n3p, f.lvl_off[l],
-                       f.child_off, f.child_idx, f.octant, f.m2m1d, nr, M, mb, ldm);
+    if (tiled) {
+      const int b0 = f.lvl_off[l];
+      switch (n) {
+        case 2: launch_m2m_tiled<T, 2>(n3p, b0, nb, nrc32, f, nr, M, mb, ldm, s); break;
+        case 3: launch_m2m_tiled<T, 3>(n3p, b0, nb, nrc32, f, nr, M, mb, ldm, s); break;
+        case 4: launch_m2m_tiled<T, 4>(n3p, b0, nb, nrc32, f, nr, M, mb, ldm, s); break;
+        case 5: launch_m2m_tiled<T, 5>(n3p, b0, nb, nrc32, f, nr, M, mb, ldm, s); break;
+        default: launch_m2m_tiled<T, 6>(n3p, b0, nb, nrc32, f, nr, M, mb, ldm, s); break;
+      }
+    } else {
+      hipLaunchKernelGGL((k_m2m<T>), dim3(nb, nrc), dim3(256), 2 * lds1, s, n, n3p, f.lvl_off[l],
+                         f.child_off, f.child_idx, f.octant, f.m2m1d, nr, M, mb, ldm);
+    }
     GP_LAUNCH_CHECK();
   }
   // M2L over every box of level >= 2 with a non-empty list; boxes without
@@ -463,6 +902,18 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     hipLaunchKernelGGL((k_fill_zero<T>), dim3(blocks), dim3(256), 0, s, Lc, cnt);
     GP_LAUNCH_CHECK();
   }
+  const bool bf16_valu = (m2l == GP_M2L_BF16_VALU);
+  if (bf16_valu) {
+    if constexpr (sizeof(T) == 4) {
+      if (!f.Gr) throw Error(GP_ERR_INTERNAL, "BF16-rounded M2L table missing");
+      size_t cnt = (size_t)f.n_boxes * ldm * n3p;
+      int blocks = (int)std::min<size_t>((cnt + 255) / 256, 8192);
+      hipLaunchKernelGGL(k_round_bf16, dim3(blocks), dim3(256), 0, s, M, cnt);
+      GP_LAUNCH_CHECK();
+    } else {
+      throw Error(GP_ERR_INTERNAL, "BF16 M2L runs on the FP32 operator only");
+    }
+  }
   if (f.n_targets > 0) {
     const double fl = 2.0 * (double)f.n_pairs * n3 * n3 * nr;
     const double by = (double)(f.n_boxes + f.n_targets) * exp_bytes;
@@ -471,7 +922,15 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * n3 * nr * 2 + f.n_targets * exp_bytes);
         const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf);
-        if (MT <= 4)
+        if (MT == 8)
+          hipLaunchKernelGGL((k_m2l_bf16_pipe<8>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
+                             Mb, Lc);
+        else if (MT == 4)
+          hipLaunchKernelGGL((k_m2l_bf16_pipe<4>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
+                             Mb, Lc);
+        else if (MT <= 4)
           hipLaunchKernelGGL((k_m2l_bf16<1>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
                              f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
                              Mb, Lc);
@@ -499,10 +958,10 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                              f.G, M, Lc);
       }
     } else {
-      ProfScope pl(f32 ? "m2l_f32" : "m2l_f64", s, fl, by);
+      ProfScope pl(bf16_valu ? "m2l_bf16_valu" : f32 ? "m2l_f32" : "m2l_f64", s, fl, by);
       hipLaunchKernelGGL((k_m2l<T>), dim3(f.n_targets, nrc), dim3(256), lds1, s, n, n3p,
-                         f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, f.G,
-                         nr, M, Lc, ldm);
+                         f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                         bf16_valu ? f.Gr : f.G, nr, M, Lc, ldm);
     }
     GP_LAUNCH_CHECK();
   }
@@ -511,14 +970,29 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     int nb = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     if (nb == 0) continue;
     ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nr * nb, 2.0 * nb * exp_bytes);
-    hipLaunchKernelGGL((k_l2l<T>), dim3(nb, nrc), dim3(256), 2 * lds1, s, n, f.lvl_off[l + 1],
-                       f.parent, f.octant, f.m2m1d, nr, Lc, ldm);
+    if (tiled) {
+      const int b0 = f.lvl_off[l + 1];
+      switch (n) {
+        case 2: launch_l2l_tiled<T, 2>(b0, nb, nrc32, f, nr, Lc, ldm, s); break;
+        case 3: launch_l2l_tiled<T, 3>(b0, nb, nrc32, f, nr, Lc, ldm, s); break;
+        case 4: launch_l2l_tiled<T, 4>(b0, nb, nrc32, f, nr, Lc, ldm, s); break;
+        case 5: launch_l2l_tiled<T, 5>(b0, nb, nrc32, f, nr, Lc, ldm, s); break;
+        default: launch_l2l_tiled<T, 6>(b0, nb, nrc32, f, nr, Lc, ldm, s); break;
+      }
+    } else {
+      hipLaunchKernelGGL((k_l2l<T>), dim3(nb, nrc), dim3(256), 2 * lds1, s, n, f.lvl_off[l + 1],
+                         f.parent, f.octant, f.m2m1d, nr, Lc, ldm);
+    }
     GP_LAUNCH_CHECK();
   }
   ProfScope pp(f32 ? "l2p_f32" : "l2p_f64", s, 2.0 * f.n_panels * n3 * nr,
                (double)f.n_panels * nr * 16.0 + nleaf * exp_bytes);
-  hipLaunchKernelGGL((k_l2p<T>), dim3(nleaf, nrc), dim3(256), lds1, s, n, f.leaf_lo, leaf0, f.w1d,
-                     nr, Lc, ldm, y, ldy);
+  if (tiled)
+    hipLaunchKernelGGL((k_l2p_tiled<T>), dim3(nleaf), dim3(256), 0, s, n, f.leaf_lo, leaf0, f.w1d,
+                       nr, Lc, ldm, y, ldy);
+  else
+    hipLaunchKernelGGL((k_l2p<T>), dim3(nleaf, nrc), dim3(256), lds1, s, n, f.leaf_lo, leaf0,
+                       f.w1d, nr, Lc, ldm, y, ldy);
   GP_LAUNCH_CHECK();
 }
 

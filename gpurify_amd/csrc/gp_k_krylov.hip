@@ -14,30 +14,42 @@
 namespace gp {
 
 constexpr int KR_BLOCKS = 512;
+constexpr int KR_VB = 8;  // basis vectors per block: Y is read once per 8 dots
 
 // partial[(v * nblk + b) * ld + r] = sum over rows of block b of X_v[row][r] * Y[row][r]
+// for the KR_VB vectors v0 .. v0 + 7 of this block (grid.y = ceil(nvec / 8)).
 __global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ X,
-                                                     size_t strideX,
+                                                     size_t strideX, int nvec,
                                                      const double *__restrict__ Y, int N, int ld,
                                                      int nr, double *__restrict__ partial) {
-  __shared__ double red[256];
-  const int v = blockIdx.y, b = blockIdx.x, nblk = gridDim.x;
-  const double *x = X + (size_t)v * strideX;
+  __shared__ double red[KR_VB][256];
+  const int v0 = blockIdx.y * KR_VB, b = blockIdx.x, nblk = gridDim.x;
+  const int nv = min(KR_VB, nvec - v0);
   const int rpp = 256 / nr;  // rows per pass
   const int tid = threadIdx.x;
   const int r = tid % nr, ro = tid / nr;
   const int rows_per_block = (N + nblk - 1) / nblk;
   const int row0 = b * rows_per_block, row1 = min(N, row0 + rows_per_block);
-  double acc = 0.0;
+  double acc[KR_VB];
+#pragma unroll
+  for (int v = 0; v < KR_VB; ++v) acc[v] = 0.0;
   if (ro < rpp)
-    for (int row = row0 + ro; row < row1; row += rpp) acc += x[(size_t)row * ld + r] * Y[(size_t)row * ld + r];
-  red[tid] = acc;
+    for (int row = row0 + ro; row < row1; row += rpp) {
+      const size_t idx = (size_t)row * ld + r;
+      const double y = Y[idx];
+#pragma unroll
+      for (int v = 0; v < KR_VB; ++v)
+        if (v < nv) acc[v] += X[(size_t)(v0 + v) * strideX + idx] * y;
+    }
+#pragma unroll
+  for (int v = 0; v < KR_VB; ++v) red[v][tid] = acc[v];
   __syncthreads();
-  if (tid < nr) {
-    double s = 0.0;
-    for (int j = 0; j < rpp; ++j) s += red[j * nr + tid];
-    partial[((size_t)v * nblk + b) * ld + tid] = s;
-  }
+  if (tid < nr)
+    for (int v = 0; v < nv; ++v) {
+      double s = 0.0;
+      for (int j = 0; j < rpp; ++j) s += red[v][j * nr + tid];
+      partial[((size_t)(v0 + v) * nblk + b) * ld + tid] = s;
+    }
 }
 
 // out[v * ld + r] = sum_b partial[(v * nblk + b) * ld + r]  (optionally sqrt)
@@ -260,8 +272,8 @@ void kr_dots(const double *X, size_t strideX, int nvec, const double *Y, int N, 
   if (nvec <= 0) return;
   ProfScope ps("gmres_dots", s, 2.0 * nvec * N * nr, 8.0 * (nvec + 1) * N * nr);
   int nblk = std::min(KR_BLOCKS, std::max(1, (N + 255) / 256));
-  hipLaunchKernelGGL(k_dot_partial, dim3(nblk, nvec), dim3(256), 0, s, X, strideX, Y, N, ld, nr,
-                     partial);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0, s, X,
+                     strideX, nvec, Y, N, ld, nr, partial);
   GP_LAUNCH_CHECK();
   int tot = nvec * nr;
   hipLaunchKernelGGL(k_reduce_partial, dim3((tot + 255) / 256), dim3(256), 0, s, partial, nblk,

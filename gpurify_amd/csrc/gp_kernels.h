@@ -60,10 +60,13 @@ struct FmmDev {
   const T *m2m1d = nullptr;          // 24 * n * n
   const T *G = nullptr;              // 343 * n3p * n3p (zero padded)
   const void *Gbf = nullptr;         // same, BF16 (FP32 operator only)
+  const T *Gr = nullptr;             // same, BF16-rounded, widened to T (GP_M2L_BF16_VALU)
 };
 // Far field y += P_far z.  Multipoles M: n_boxes x ldm x n3p; locals Lc:
 // n_boxes x n3 x ldm.  m2l selects the M2L kernel (GP_M2L_*): BF16 needs Mb
-// (BF16 multipoles, same layout), FP64 uses matrix cores when ldm allows.
+// (BF16 multipoles, same layout), FP64 uses matrix cores when ldm allows,
+// BF16_VALU rounds M to BF16 in place after the upward pass and runs the
+// FP32 VALU M2L on the table Gr.
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
                    double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
 void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,

@@ -192,7 +192,7 @@ struct FmmSystem {
   DBuf<int> parent, octant, child_off, child_idx, targets, il_off, il_src, il_oidx, box_level;
   DBuf<double> inv_half;
   DBuf<double> w1d64, m2m64, G64;
-  DBuf<float> w1d32, m2m32, G32;
+  DBuf<float> w1d32, m2m32, G32, Gr32;
   DBuf<uint16_t> Gbf;
   DBuf<int> coff, cidx;
   NearTiles nt;
@@ -212,8 +212,8 @@ struct FmmSystem {
     if (n < 2 || n > 6) throw Error(GP_ERR_INVALID_INPUT, "fmm_order must be in [2, 6]");
     m2l_inner = (int)o.m2l_inner;
     m2l_outer = (int)o.m2l_outer;
-    if (m2l_inner != GP_M2L_BF16 && m2l_inner != GP_M2L_FP32)
-      throw Error(GP_ERR_INVALID_INPUT, "m2l_inner must be GP_M2L_BF16 or GP_M2L_FP32");
+    if (m2l_inner != GP_M2L_BF16 && m2l_inner != GP_M2L_FP32 && m2l_inner != GP_M2L_BF16_VALU)
+      throw Error(GP_ERR_INVALID_INPUT, "m2l_inner must be GP_M2L_BF16, _FP32 or _BF16_VALU");
     if (m2l_outer != GP_M2L_FP64 && m2l_outer != GP_M2L_FP64_VALU)
       throw Error(GP_ERR_INVALID_INPUT, "m2l_outer must be GP_M2L_FP64 or GP_M2L_FP64_VALU");
     L = o.max_level ? (int)o.max_level : choose_depth(g, (int)o.leaf_target);
@@ -348,6 +348,14 @@ struct FmmSystem {
       std::vector<uint16_t> hb(hG.size());
       for (size_t e = 0; e < hG.size(); ++e) hb[e] = bf16_bits((float)hG[e]);
       Gbf = to_device(hb, s);
+      if (m2l_inner == GP_M2L_BF16_VALU) {
+        std::vector<float> hr(hG.size());
+        for (size_t e = 0; e < hG.size(); ++e) {
+          const uint32_t u = (uint32_t)hb[e] << 16;
+          std::memcpy(&hr[e], &u, 4);
+        }
+        Gr32 = to_device(hr, s);
+      }
     }
     auto fill = [&](auto &f) {
       f.n = n;
@@ -380,6 +388,7 @@ struct FmmSystem {
     f32.m2m1d = m2m32.p;
     f32.G = G32.p;
     f32.Gbf = Gbf.p;
+    f32.Gr = Gr32.p;
     std::vector<int> hco, hci;
     cond_csr(g, t.perm.data(), hco, hci);
     coff = to_device(hco, s);
@@ -422,7 +431,9 @@ struct FmmSystem {
       if (!have32) throw Error(GP_ERR_INTERNAL, "FP32 operator not assembled");
       kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
       launch_near_apply(nt, nf32.p, z32.p, ld, y, ld, nr, s);
-      const int v = (m2l_inner == GP_M2L_BF16 && Mbf.p) ? GP_M2L_BF16 : GP_M2L_FP32;
+      const int v = (m2l_inner == GP_M2L_BF16 && Mbf.p) ? GP_M2L_BF16
+                    : m2l_inner == GP_M2L_BF16_VALU    ? GP_M2L_BF16_VALU
+                                                       : GP_M2L_FP32;
       run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, v, y, ld, s);
     }
   }

@@ -78,8 +78,11 @@ enum { GP_PREC_FP64 = 0, GP_PREC_MIXED = 1 };
 enum { GP_METHOD_AUTO = 0, GP_METHOD_DENSE = 1, GP_METHOD_FMM = 2 };
 /* M2L kernels: FP64 on matrix cores (FP64 VALU when the RHS block is not a
  * multiple of 16), FP32 on the VALU, BF16 inputs with FP32 accumulation on
- * matrix cores, FP64 forced onto the VALU. */
-enum { GP_M2L_FP64 = 0, GP_M2L_FP32 = 1, GP_M2L_BF16 = 2, GP_M2L_FP64_VALU = 3 };
+ * matrix cores, FP64 forced onto the VALU, and BF16-rounded operands on the
+ * FP32 VALU (the same rounding as GP_M2L_BF16 through independent kernels,
+ * a cross-check of the matrix-core path; inner operator only). */
+enum { GP_M2L_FP64 = 0, GP_M2L_FP32 = 1, GP_M2L_BF16 = 2, GP_M2L_FP64_VALU = 3,
+       GP_M2L_BF16_VALU = 4 };
 
 /* Solver options.  gp_cap_opts_default() fills the reference's own
  * constants: the dense/FMM rule of bridge.rs:397-404 (dense below 500
@@ -97,7 +100,7 @@ typedef struct {
   double tol;            /* relative residual target per right-hand side */
   double inner_tol;      /* GP_PREC_MIXED: inner GMRES relative tolerance */
   uint32_t max_refine;   /* GP_PREC_MIXED: outer FP64 refinement steps */
-  uint32_t m2l_inner;    /* GP_PREC_MIXED inner operator: GP_M2L_BF16 (default) or GP_M2L_FP32 */
+  uint32_t m2l_inner;    /* GP_PREC_MIXED inner operator: GP_M2L_BF16 (default), _FP32 or _BF16_VALU */
   uint32_t m2l_outer;    /* FP64 operator (FP64 solve, mixed outer residual): GP_M2L_FP64 or _FP64_VALU */
   uint32_t strict;       /* 1: non-convergence returns GP_ERR_NOT_CONVERGED */
   uint32_t dense_limit;  /* AUTO: dense below this many panels (reference 500) */
