@@ -43,6 +43,8 @@ GP_M2L_FP32 = 1
 GP_M2L_BF16 = 2
 GP_M2L_FP64_VALU = 3
 GP_M2L_BF16_VALU = 4
+GP_BASIS_FP32 = 0
+GP_BASIS_FP64 = 1
 
 
 class GpPanels(C.Structure):
@@ -56,7 +58,8 @@ class GpCapOpts(C.Structure):
                 ("max_restarts", C.c_uint32), ("rhs_block", C.c_uint32), ("tol", C.c_double),
                 ("inner_tol", C.c_double), ("max_refine", C.c_uint32), ("m2l_inner", C.c_uint32),
                 ("m2l_outer", C.c_uint32), ("strict", C.c_uint32), ("dense_limit", C.c_uint32),
-                ("rhs_first", C.c_uint32), ("rhs_count", C.c_uint32), ("reserved", C.c_uint32)]
+                ("rhs_first", C.c_uint32), ("rhs_count", C.c_uint32),
+                ("inner_passes", C.c_uint32), ("inner_basis", C.c_uint32)]
 
 
 class GpCapReport(C.Structure):

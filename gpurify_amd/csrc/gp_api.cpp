@@ -163,6 +163,8 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->dense_limit = 500;  // bridge.rs:30
   o->rhs_first = 0;
   o->rhs_count = 0;
+  o->inner_passes = 1;
+  o->inner_basis = GP_BASIS_FP32;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,

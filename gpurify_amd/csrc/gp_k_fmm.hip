@@ -369,6 +369,103 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
       }
 }
 
+// Pipelined FP64 MFMA M2L for n3p = 16 MT (MT = 4 or 8) and ldm <= 128.
+// The contraction runs in steps of 16 over every (pair, k-slab); the source
+// multipole slab (RT * 16 rows x 16) is staged in a double-buffered LDS tile
+// shared by the four wavefronts, each wavefront keeps its node tiles' G_off
+// fragments in registers, and the next step's slab and fragments are loaded
+// while the current step runs on the matrix cores (one barrier per step).
+// Lane layout as in k_m2l_f64 below (permuted contraction index).
+template <int MT>
+__global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
+                                                      const int *__restrict__ targets,
+                                                      const int *__restrict__ il_off,
+                                                      const int *__restrict__ il_src,
+                                                      const int *__restrict__ il_oidx,
+                                                      const int *__restrict__ box_level,
+                                                      const double *__restrict__ inv_half,
+                                                      const double *__restrict__ G,
+                                                      const double *__restrict__ M,
+                                                      double *__restrict__ Lc) {
+  constexpr int NP = 16 * MT;  // n3p
+  constexpr int MTW = MT / 4;  // node tiles per wavefront
+  constexpr int NKS = NP / 16; // k-slabs per pair
+  constexpr int KP = 20;       // padded LDS row (doubles)
+  constexpr int APT = 8;       // slab elements per thread (<= 128 rows x 16)
+  __shared__ __attribute__((aligned(32))) double sA[2][128 * KP];
+  const int t = targets[blockIdx.x];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  const int RT = (nr + 15) >> 4;
+  const int AE = RT * 16 * 16;
+  const size_t MS = (size_t)ldm * NP;
+  f64x4 acc[8][MTW];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  const int q0 = il_off[t], q1 = il_off[t + 1];
+  const int nsteps = (q1 - q0) * NKS;
+  double apre[APT];
+  f64x4 bcur[MTW], bnext[MTW];
+  auto load_step = [&](int st, f64x4 (&bd)[MTW]) {
+    const int q = q0 + st / NKS, k0 = (st % NKS) * 16;
+    const double *Ms = M + (size_t)il_src[q] * MS + k0;
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int e = tid + 256 * i;
+      if (e < AE) apre[i] = Ms[(size_t)(e >> 4) * NP + (e & 15)];
+    }
+    const double *Go = G + (size_t)il_oidx[q] * NP * NP + k0 + lk;
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) bd[j] = *(const f64x4 *)(Go + (size_t)((wave * MTW + j) * 16 + li) * NP);
+  };
+  auto store_a = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int e = tid + 256 * i;
+      if (e < AE) sA[buf][(e >> 4) * KP + (e & 15)] = apre[i];
+    }
+  };
+  if (nsteps > 0) {
+    load_step(0, bcur);
+    store_a(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nsteps) load_step(st + 1, bnext);
+#pragma unroll
+    for (int rt = 0; rt < 8; ++rt) {
+      if (rt < RT) {
+        const f64x4 a = *(const f64x4 *)(&sA[buf][(rt * 16 + li) * KP + lk]);
+#pragma unroll
+        for (int j = 0; j < MTW; ++j)
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+            acc[rt][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], bcur[j][kk], acc[rt][j], 0, 0, 0);
+      }
+    }
+    if (st + 1 < nsteps) {
+      store_a(buf ^ 1);
+#pragma unroll
+      for (int j = 0; j < MTW; ++j) bcur[j] = bnext[j];
+    }
+    __syncthreads();
+  }
+  const double ih = inv_half[box_level[t]];
+  double *out = Lc + (size_t)t * n3 * ldm;
+#pragma unroll
+  for (int rt = 0; rt < 8; ++rt)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = rt * 16 + (lane >> 4) + 4 * v, m = (wave * MTW + j) * 16 + li;
+        if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
+      }
+}
+
 // MFMA M2L in FP64 (v_mfma_f64_16x16x4f64), same tiling.  Each lane loads 4
 // consecutive contraction entries (32 bytes) and feeds them to 4 successive
 // MFMAs: in step j lane group g = lane / 16 supplies contraction index
@@ -948,7 +1045,15 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && ldm % 16 == 0 && ldm <= 128 && MT <= 8) {
       if constexpr (sizeof(T) == 8) {
         ProfScope pl("m2l_f64_mfma", s, fl, by);
-        if (MT <= 4)
+        if (MT == 8)
+          hipLaunchKernelGGL((k_m2l_f64_pipe<8>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                             f.G, M, Lc);
+        else if (MT == 4)
+          hipLaunchKernelGGL((k_m2l_f64_pipe<4>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                             f.G, M, Lc);
+        else if (MT <= 4)
           hipLaunchKernelGGL((k_m2l_f64<1>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
                              f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
                              f.G, M, Lc);

@@ -1,5 +1,7 @@
-// GMRES building blocks on the device (krylov.rs:50-207), FP64, for a block
-// of independent right-hand sides that iterate in lockstep.
+// GMRES building blocks on the device (krylov.rs:50-207) for a block of
+// independent right-hand sides that iterate in lockstep.  Arithmetic and
+// every reduction are FP64; the Krylov basis is stored in FP64 or, for the
+// inner solves of the mixed-precision solver, in FP32 (template TV).
 //
 // Vectors are N x ld row-major (RHS fastest); nr <= ld columns are live.
 // Reductions are two-stage with a fixed order (per-block partials, then one
@@ -18,7 +20,8 @@ constexpr int KR_VB = 8;  // basis vectors per block: Y is read once per 8 dots
 
 // partial[(v * nblk + b) * ld + r] = sum over rows of block b of X_v[row][r] * Y[row][r]
 // for the KR_VB vectors v0 .. v0 + 7 of this block (grid.y = ceil(nvec / 8)).
-__global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ X,
+template <typename TX>
+__global__ __launch_bounds__(256) void k_dot_partial(const TX *__restrict__ X,
                                                      size_t strideX, int nvec,
                                                      const double *__restrict__ Y, int N, int ld,
                                                      int nr, double *__restrict__ partial) {
@@ -39,7 +42,7 @@ __global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ 
       const double y = Y[idx];
 #pragma unroll
       for (int v = 0; v < KR_VB; ++v)
-        if (v < nv) acc[v] += X[(size_t)(v0 + v) * strideX + idx] * y;
+        if (v < nv) acc[v] += (double)X[(size_t)(v0 + v) * strideX + idx] * y;
     }
 #pragma unroll
   for (int v = 0; v < KR_VB; ++v) red[v][tid] = acc[v];
@@ -64,8 +67,9 @@ __global__ void k_reduce_partial(const double *__restrict__ partial, int nblk, i
 }
 
 // w[row][r] += alpha * sum_{v < nvec} coef[v][r] * V_v[row][r]
+template <typename TV>
 __global__ __launch_bounds__(256) void k_axpy_multi(double *__restrict__ w,
-                                                    const double *__restrict__ V, size_t strideV,
+                                                    const TV *__restrict__ V, size_t strideV,
                                                     const double *__restrict__ coef, int nvec,
                                                     int N, int ld, int nr, double alpha) {
   size_t total = (size_t)N * nr;
@@ -74,14 +78,16 @@ __global__ __launch_bounds__(256) void k_axpy_multi(double *__restrict__ w,
     int row = (int)(e / nr), r = (int)(e % nr);
     size_t idx = (size_t)row * ld + r;
     double s = 0.0;
-    for (int v = 0; v < nvec; ++v) s += coef[(size_t)v * ld + r] * V[(size_t)v * strideV + idx];
+    for (int v = 0; v < nvec; ++v)
+      s += coef[(size_t)v * ld + r] * (double)V[(size_t)v * strideV + idx];
     w[idx] += alpha * s;
   }
 }
 
 // out[row][r] = scale[r] * in[row][r] (0 where mask[r] == 0); mask may be null.
+template <typename TO>
 __global__ __launch_bounds__(256) void k_scale_cols(const double *__restrict__ in,
-                                                    double *__restrict__ out,
+                                                    TO *__restrict__ out,
                                                     const double *__restrict__ scale,
                                                     const int *__restrict__ mask, int N, int ld,
                                                     int nr) {
@@ -90,7 +96,7 @@ __global__ __launch_bounds__(256) void k_scale_cols(const double *__restrict__ i
        e += (size_t)gridDim.x * blockDim.x) {
     int row = (int)(e / nr), r = (int)(e % nr);
     size_t idx = (size_t)row * ld + r;
-    out[idx] = (mask && !mask[r]) ? 0.0 : scale[r] * in[idx];
+    out[idx] = (TO)((mask && !mask[r]) ? 0.0 : scale[r] * in[idx]);
   }
 }
 
@@ -110,8 +116,8 @@ __global__ __launch_bounds__(256) void k_sub(const double *__restrict__ a,
 // z[row][r] = (T)(v[row][r] * (dinv ? dinv[row] : 1)): right Jacobi
 // preconditioner (cap/fmm_solver.rs:133-144) fused with the cast to the
 // operator's working precision.
-template <typename T>
-__global__ __launch_bounds__(256) void k_precond_cast(const double *__restrict__ v,
+template <typename TI, typename T>
+__global__ __launch_bounds__(256) void k_precond_cast(const TI *__restrict__ v,
                                                       const double *__restrict__ dinv,
                                                       T *__restrict__ z, int N, int ld, int nr) {
   size_t total = (size_t)N * nr;
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(256) void k_precond_cast(const double *__restrict__
     int row = (int)(e / nr), r = (int)(e % nr);
     size_t idx = (size_t)row * ld + r;
     double s = dinv ? dinv[row] : 1.0;
-    z[idx] = (T)(v[idx] * s);
+    z[idx] = (T)((double)v[idx] * s);
   }
 }
 
@@ -244,8 +250,9 @@ __global__ void k_gmres_backsolve(int m, int nr, int ld, const double *__restric
 }
 
 // x[row][r] += dinv[row] * sum_{i < kused[r]} Y[i][r] V_i[row][r]  (krylov.rs:154-170)
+template <typename TV>
 __global__ __launch_bounds__(256) void k_gmres_update_x(double *__restrict__ x,
-                                                        const double *__restrict__ V,
+                                                        const TV *__restrict__ V,
                                                         size_t strideV,
                                                         const double *__restrict__ Y,
                                                         const int *__restrict__ kused, int kmax,
@@ -258,7 +265,7 @@ __global__ __launch_bounds__(256) void k_gmres_update_x(double *__restrict__ x,
     size_t idx = (size_t)row * ld + r;
     int ku = min(kused[r], kmax);
     double s = 0.0;
-    for (int i = 0; i < ku; ++i) s += Y[(size_t)i * ld + r] * V[(size_t)i * strideV + idx];
+    for (int i = 0; i < ku; ++i) s += Y[(size_t)i * ld + r] * (double)V[(size_t)i * strideV + idx];
     x[idx] += (dinv ? dinv[row] : 1.0) * s;
   }
 }
@@ -267,12 +274,13 @@ namespace {
 int grid_for(size_t total) { return (int)std::min<size_t>((total + 255) / 256, 8192); }
 }  // namespace
 
-void kr_dots(const double *X, size_t strideX, int nvec, const double *Y, int N, int ld, int nr,
+template <typename TX>
+void kr_dots(const TX *X, size_t strideX, int nvec, const double *Y, int N, int ld, int nr,
              double *partial, double *out, bool take_sqrt, hipStream_t s) {
   if (nvec <= 0) return;
-  ProfScope ps("gmres_dots", s, 2.0 * nvec * N * nr, 8.0 * (nvec + 1) * N * nr);
+  ProfScope ps("gmres_dots", s, 2.0 * nvec * N * nr, (sizeof(TX) * nvec + 8.0) * N * nr);
   int nblk = std::min(KR_BLOCKS, std::max(1, (N + 255) / 256));
-  hipLaunchKernelGGL(k_dot_partial, dim3(nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0, s, X,
+  hipLaunchKernelGGL((k_dot_partial<TX>), dim3(nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0, s, X,
                      strideX, nvec, Y, N, ld, nr, partial);
   GP_LAUNCH_CHECK();
   int tot = nvec * nr;
@@ -283,18 +291,20 @@ void kr_dots(const double *X, size_t strideX, int nvec, const double *Y, int N, 
 
 size_t kr_partial_size(int nvec_max, int ld) { return (size_t)KR_BLOCKS * nvec_max * ld; }
 
-void kr_axpy_multi(double *w, const double *V, size_t strideV, const double *coef, int nvec,
+template <typename TV>
+void kr_axpy_multi(double *w, const TV *V, size_t strideV, const double *coef, int nvec,
                    int N, int ld, int nr, double alpha, hipStream_t s) {
   if (nvec <= 0) return;
-  ProfScope ps("gmres_axpy", s, 2.0 * nvec * N * nr, 8.0 * (nvec + 2) * N * nr);
-  hipLaunchKernelGGL(k_axpy_multi, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, w, V, strideV,
+  ProfScope ps("gmres_axpy", s, 2.0 * nvec * N * nr, (sizeof(TV) * nvec + 16.0) * N * nr);
+  hipLaunchKernelGGL((k_axpy_multi<TV>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, w, V, strideV,
                      coef, nvec, N, ld, nr, alpha);
   GP_LAUNCH_CHECK();
 }
 
-void kr_scale_cols(const double *in, double *out, const double *scale, const int *mask, int N,
+template <typename TO>
+void kr_scale_cols(const double *in, TO *out, const double *scale, const int *mask, int N,
                    int ld, int nr, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_cols, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, in, out,
+  hipLaunchKernelGGL((k_scale_cols<TO>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, in, out,
                      scale, mask, N, ld, nr);
   GP_LAUNCH_CHECK();
 }
@@ -304,15 +314,9 @@ void kr_sub(const double *a, const double *b, double *out, int N, int ld, int nr
   GP_LAUNCH_CHECK();
 }
 
-void kr_precond_cast(const double *v, const double *dinv, double *z, int N, int ld, int nr,
-                     hipStream_t s) {
-  hipLaunchKernelGGL((k_precond_cast<double>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, v,
-                     dinv, z, N, ld, nr);
-  GP_LAUNCH_CHECK();
-}
-void kr_precond_cast(const double *v, const double *dinv, float *z, int N, int ld, int nr,
-                     hipStream_t s) {
-  hipLaunchKernelGGL((k_precond_cast<float>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, v,
+template <typename TI, typename T>
+void kr_precond_cast(const TI *v, const double *dinv, T *z, int N, int ld, int nr, hipStream_t s) {
+  hipLaunchKernelGGL((k_precond_cast<TI, T>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, v,
                      dinv, z, N, ld, nr);
   GP_LAUNCH_CHECK();
 }
@@ -341,10 +345,11 @@ void kr_gmres_backsolve(int m, int nr, int ld, const double *H, const double *g,
   GP_LAUNCH_CHECK();
 }
 
-void kr_gmres_update_x(double *x, const double *V, size_t strideV, const double *Y,
+template <typename TV>
+void kr_gmres_update_x(double *x, const TV *V, size_t strideV, const double *Y,
                        const int *kused, int kmax, const double *dinv, int N, int ld, int nr,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_gmres_update_x, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, x, V,
+  hipLaunchKernelGGL((k_gmres_update_x<TV>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, x, V,
                      strideV, Y, kused, kmax, dinv, N, ld, nr);
   GP_LAUNCH_CHECK();
 }
@@ -360,5 +365,24 @@ void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b,
                      nr, b);
   GP_LAUNCH_CHECK();
 }
+
+#define GP_KR_BASIS(TV)                                                                        \
+  template void kr_dots<TV>(const TV *, size_t, int, const double *, int, int, int, double *,      \
+                            double *, bool, hipStream_t);                                          \
+  template void kr_axpy_multi<TV>(double *, const TV *, size_t, const double *, int, int, int, int, \
+                                  double, hipStream_t);                                            \
+  template void kr_scale_cols<TV>(const double *, TV *, const double *, const int *, int, int, int, \
+                                  hipStream_t);                                                    \
+  template void kr_gmres_update_x<TV>(double *, const TV *, size_t, const double *, const int *,   \
+                                      int, const double *, int, int, int, hipStream_t);
+GP_KR_BASIS(double)
+GP_KR_BASIS(float)
+#undef GP_KR_BASIS
+template void kr_precond_cast<double, double>(const double *, const double *, double *, int, int,
+                                              int, hipStream_t);
+template void kr_precond_cast<double, float>(const double *, const double *, float *, int, int,
+                                             int, hipStream_t);
+template void kr_precond_cast<float, float>(const float *, const double *, float *, int, int, int,
+                                            hipStream_t);
 
 }  // namespace gp

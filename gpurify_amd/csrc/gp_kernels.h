@@ -73,20 +73,23 @@ void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, floa
                    float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
 
 // ---------------------------------------------------------------- Krylov
+// Basis types TV / TX / TO: double or float (explicitly instantiated).
 size_t kr_partial_size(int nvec_max, int ld);
-void kr_dots(const double *X, size_t strideX, int nvec, const double *Y, int N, int ld, int nr,
+template <typename TX>
+void kr_dots(const TX *X, size_t strideX, int nvec, const double *Y, int N, int ld, int nr,
              double *partial, double *out, bool take_sqrt, hipStream_t s);
-void kr_axpy_multi(double *w, const double *V, size_t strideV, const double *coef, int nvec,
+template <typename TV>
+void kr_axpy_multi(double *w, const TV *V, size_t strideV, const double *coef, int nvec,
                    int N, int ld, int nr, double alpha, hipStream_t s);
-void kr_scale_cols(const double *in, double *out, const double *scale, const int *mask, int N,
+template <typename TO>
+void kr_scale_cols(const double *in, TO *out, const double *scale, const int *mask, int N,
                    int ld, int nr, hipStream_t s);
 void kr_axpy(double *x, const double *y, double alpha, int N, int ld, int nr, hipStream_t s);
 void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s);
 void kr_sub(const double *a, const double *b, double *out, int N, int ld, int nr, hipStream_t s);
-void kr_precond_cast(const double *v, const double *dinv, double *z, int N, int ld, int nr,
-                     hipStream_t s);
-void kr_precond_cast(const double *v, const double *dinv, float *z, int N, int ld, int nr,
-                     hipStream_t s);
+// TI -> T: (double, double), (double, float), (float, float)
+template <typename TI, typename T>
+void kr_precond_cast(const TI *v, const double *dinv, T *z, int N, int ld, int nr, hipStream_t s);
 void kr_gmres_start(int m, int nr, int ld, const double *beta, const double *bnorm, double tol,
                     const int *live, int *active, int *kused, double *g, double *inv_beta,
                     double *est, hipStream_t s);
@@ -96,7 +99,8 @@ void kr_gmres_givens(int k, int m, int nr, int ld, const double *h1, const doubl
                      double *inv_h, hipStream_t s);
 void kr_gmres_backsolve(int m, int nr, int ld, const double *H, const double *g, const int *kused,
                         double *Y, hipStream_t s);
-void kr_gmres_update_x(double *x, const double *V, size_t strideV, const double *Y,
+template <typename TV>
+void kr_gmres_update_x(double *x, const TV *V, size_t strideV, const double *Y,
                        const int *kused, int kmax, const double *dinv, int N, int ld, int nr,
                        hipStream_t s);
 

@@ -420,11 +420,13 @@ struct FmmSystem {
   }
 
   // y = P' M^-1 v (precond) or y = P' v, in FP64 (prec 0) or FP32 (prec 1).
-  void apply(const double *v, bool precond, int nr, double *y, int prec) {
+  template <typename TV>
+  void apply(const TV *v, bool precond, int nr, double *y, int prec) {
     const double *dinv = precond ? inv_diag.p : nullptr;
     ++matvecs;
     if (prec == 0) {
-      kr_precond_cast(v, dinv, z64.p, N, ld, nr, s);
+      if constexpr (sizeof(TV) != 8) throw Error(GP_ERR_INTERNAL, "FP64 operator on an FP32 basis");
+      else kr_precond_cast(v, dinv, z64.p, N, ld, nr, s);
       launch_near_apply(nt, nf64.p, z64.p, ld, y, ld, nr, s);
       run_far_field(f64, z64.p, ld, nr, M64.p, Lc64.p, ld, m2l_outer, y, ld, s);
     } else {
@@ -462,16 +464,20 @@ struct GmresStats {
 // (device, N x ld) starting from x = 0 (x must be zero on entry), per column
 // exactly the control flow of krylov.rs:50-191: stop a cycle at the residual
 // estimate, check the true residual when the estimate is within 10 tol.
-// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2), the
-// block-friendly equivalent of the reference's modified Gram-Schmidt.
+// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2, passes =
+// 2), the block-friendly equivalent of the reference's modified Gram-Schmidt;
+// the inner solves of the mixed solver (tolerance ~1e-2, corrected by the
+// FP64 refinement) take one pass and keep the basis V in FP32 (TV = float).
+template <typename TV>
 void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, double tol,
-                 int m, int max_restarts, bool true_check, const std::vector<int> &live_in,
-                 GmresStats &st) {
+                 int m, int max_restarts, bool true_check, int passes,
+                 const std::vector<int> &live_in, GmresStats &st) {
   hipStream_t s = sys.s;
   const int N = sys.N, ld = sys.ld;
   const size_t vec = (size_t)N * ld;
   int vcap = std::min(m, 32) + 1;
-  DBuf<double> V(vcap * vec), w(vec), rv(vec), ax(vec);
+  DBuf<TV> V(vcap * vec);
+  DBuf<double> w(vec), rv(vec), ax(vec);
   DBuf<double> partial(kr_partial_size(vcap, ld));
   DBuf<double> h1((size_t)(m + 1) * ld), h2((size_t)(m + 1) * ld), hn(ld);
   DBuf<double> H((size_t)(m + 1) * m * ld), cs((size_t)m * ld), sn((size_t)m * ld),
@@ -487,8 +493,8 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
   auto ensure_V = [&](int nv) {
     if (nv <= vcap) return;
     int nc = std::min(m + 1, std::max(nv, 2 * vcap));
-    DBuf<double> V2((size_t)nc * vec);
-    GP_HIP(hipMemcpyAsync(V2.p, V.p, (size_t)vcap * vec * sizeof(double),
+    DBuf<TV> V2((size_t)nc * vec);
+    GP_HIP(hipMemcpyAsync(V2.p, V.p, (size_t)vcap * vec * sizeof(TV),
                           hipMemcpyDeviceToDevice, s));
     GP_HIP(hipStreamSynchronize(s));  // the old basis is freed below
     V = std::move(V2);
@@ -546,11 +552,13 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
       sys.apply(V.p + (size_t)k * sv, true, nr, w.p, prec);
       kr_dots(V.p, sv, k + 1, w.p, N, ld, nr, partial.p, h1.p, false, s);
       kr_axpy_multi(w.p, V.p, sv, h1.p, k + 1, N, ld, nr, -1.0, s);
-      kr_dots(V.p, sv, k + 1, w.p, N, ld, nr, partial.p, h2.p, false, s);
-      kr_axpy_multi(w.p, V.p, sv, h2.p, k + 1, N, ld, nr, -1.0, s);
+      if (passes > 1) {
+        kr_dots(V.p, sv, k + 1, w.p, N, ld, nr, partial.p, h2.p, false, s);
+        kr_axpy_multi(w.p, V.p, sv, h2.p, k + 1, N, ld, nr, -1.0, s);
+      }
       kr_dots(w.p, 0, 1, w.p, N, ld, nr, partial.p, hn.p, true, s);
-      kr_gmres_givens(k, m, nr, ld, h1.p, h2.p, hn.p, H.p, cs.p, sn.p, g.p, bnorm.p, tol, active.p,
-                      kused.p, est.p, inv_h.p, s);
+      kr_gmres_givens(k, m, nr, ld, h1.p, passes > 1 ? h2.p : nullptr, hn.p, H.p, cs.p, sn.p, g.p,
+                      bnorm.p, tol, active.p, kused.p, est.p, inv_h.p, s);
       kr_scale_cols(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, nullptr, N, ld, nr, s);
       ksteps = k + 1;
       GP_HIP(hipMemcpyAsync(h_active, active.p, nr * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -648,7 +656,7 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
     std::vector<int> live(nr, 1);
     if (!mixed) {
       GmresStats st;
-      gmres_block(sys, 0, b.p, x.p, nr, o.tol, m, mr, true, live, st);
+      gmres_block<double>(sys, 0, b.p, x.p, nr, o.tol, m, mr, true, 2, live, st);
       for (int q = 0; q < nr; ++q) {
         max_iters = std::max(max_iters, st.iters[q]);
         worst = std::max(worst, st.resid[q]);
@@ -665,6 +673,7 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
       std::vector<int> it_sum(nr, 0), done(nr, 0);
       int steps = 0;
       const int max_ref = std::max(1, (int)o.max_refine);
+      const int passes = o.inner_passes >= 2 ? 2 : 1;
       for (;;) {
         kr_dots(r.p, 0, 1, r.p, sys.N, ld, nr, part.p, rn.p, true, s);
         GP_HIP(hipMemcpyAsync(hrn.data(), rn.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -679,7 +688,10 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
         for (int q = 0; q < nr; ++q) lv[q] = !done[q];
         dx.zero(s);
         GmresStats st;
-        gmres_block(sys, 1, r.p, dx.p, nr, o.inner_tol, m, mr, false, lv, st);
+        if (o.inner_basis == GP_BASIS_FP64)
+          gmres_block<double>(sys, 1, r.p, dx.p, nr, o.inner_tol, m, mr, false, passes, lv, st);
+        else
+          gmres_block<float>(sys, 1, r.p, dx.p, nr, o.inner_tol, m, mr, false, passes, lv, st);
         for (int q = 0; q < nr; ++q) it_sum[q] += st.iters[q];
         kr_axpy(x.p, dx.p, 1.0, sys.N, ld, nr, s);
         sys.apply(x.p, false, nr, ax.p, 0);

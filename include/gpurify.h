@@ -106,8 +106,10 @@ typedef struct {
   uint32_t dense_limit;  /* AUTO: dense below this many panels (reference 500) */
   uint32_t rhs_first;    /* solve conductor columns [rhs_first, rhs_first + rhs_count) */
   uint32_t rhs_count;    /* 0 = all columns; unsolved columns of C are left zero */
-  uint32_t reserved;
+  uint32_t inner_passes; /* GP_PREC_MIXED: Gram-Schmidt passes of the inner GMRES, 1 or 2 */
+  uint32_t inner_basis;  /* GP_PREC_MIXED: inner Krylov basis storage, GP_BASIS_FP32 | _FP64 */
 } gp_cap_opts;
+enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 
 void gp_cap_opts_default(gp_cap_opts *opts);
 
