@@ -205,77 +205,26 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// MFMA M2L, BF16 inputs, FP32 accumulation.  Requires ldm a multiple of 16,
-// ldm <= 128 and n3p a multiple of 32.  Product per pair: D (RHS x node) +=
-// A (RHS x l) * B (l x node) with A = Mb[s] rows and B[l][m] = G_off[m][l];
-// v_mfma_f32_16x16x32_bf16: lane holds A[i = lane % 16][k = 8 (lane / 16) + 0..7],
-// B[k = 8 (lane / 16) + 0..7][j = lane % 16], D[i = 4 (lane / 16) + v][j = lane % 16].
-template <int MTW>
-__global__ __launch_bounds__(256) void k_m2l_bf16(int n3, int n3p, int nr, int ldm,
-                                                  const int *__restrict__ targets,
-                                                  const int *__restrict__ il_off,
-                                                  const int *__restrict__ il_src,
-                                                  const int *__restrict__ il_oidx,
-                                                  const int *__restrict__ box_level,
-                                                  const double *__restrict__ inv_half,
-                                                  const __bf16 *__restrict__ G,
-                                                  const __bf16 *__restrict__ Mb,
-                                                  float *__restrict__ Lc) {
-  const int t = targets[blockIdx.x];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int RT = (nr + 15) >> 4;
-  const int MT = n3p >> 4;
-  const int li = lane & 15, lk = (lane >> 4) * 8;
-  f32x4 acc[8][MTW];
-#pragma unroll
-  for (int rt = 0; rt < 8; ++rt)
-#pragma unroll
-    for (int j = 0; j < MTW; ++j) acc[rt][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  const size_t MS = (size_t)ldm * n3p;
-  for (int q = il_off[t]; q < il_off[t + 1]; ++q) {
-    const __bf16 *Ms = Mb + (size_t)il_src[q] * MS;
-    const __bf16 *Go = G + (size_t)il_oidx[q] * n3p * n3p;
-    for (int k0 = 0; k0 < n3p; k0 += 32) {
-      bf16x8 b[MTW];
-#pragma unroll
-      for (int j = 0; j < MTW; ++j) {
-        int mt = wave * MTW + j;
-        if (mt < MT) b[j] = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * n3p + k0 + lk);
-      }
-#pragma unroll
-      for (int rt = 0; rt < 8; ++rt) {
-        if (rt < RT) {
-          bf16x8 a = *(const bf16x8 *)(Ms + (size_t)(rt * 16 + li) * n3p + k0 + lk);
-#pragma unroll
-          for (int j = 0; j < MTW; ++j)
-            if (wave * MTW + j < MT)
-              acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[rt][j], 0, 0, 0);
-        }
-      }
-    }
-  }
-  const float ih = (float)inv_half[box_level[t]];
-  float *out = Lc + (size_t)t * n3 * ldm;
-#pragma unroll
-  for (int rt = 0; rt < 8; ++rt)
-#pragma unroll
-    for (int j = 0; j < MTW; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int r = rt * 16 + (lane >> 4) * 4 + v, m = (wave * MTW + j) * 16 + li;
-        if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
-      }
-}
+// Matrix-core M2L.  Product per pair: D (RHS x node) += A (RHS x l) * B
+// (l x node) with A = the source's multipole rows and B[l][m] = G_off[m][l];
+// both operands are contiguous along the contraction index l.
+//   v_mfma_f32_16x16x32_bf16: lane holds A[i = lane % 16][k = 8 (lane / 16) + 0..7],
+//     B[k = 8 (lane / 16) + 0..7][j = lane % 16], D[i = 4 (lane / 16) + v][j = lane % 16];
+//   v_mfma_f64_16x16x4f64 interleaves its output rows: D[i][j] sits in lane
+//     j + 16 (i % 4), register i / 4.
+// One work-group (four wavefronts) per target box and all RHS columns
+// (RT <= RTM tiles of 16, ldm <= 128); n3p = 16 MT and wavefront w owns node
+// tiles [w MTW, (w + 1) MTW), MTW = ceil(MT / 4).
 
-// Pipelined BF16 MFMA M2L for n3p = 16 MT (MT = 4 or 8) and ldm <= 128.
-// Per pair the source multipole rows (A operand, RT * 16 rows of n3p) are
-// staged once per work-group in LDS, rows padded by 8 elements so that the
-// 8 lanes of each 128-byte read phase hit distinct banks, and read by all
-// four wavefronts; wavefront w owns node tiles [w MT/4, (w+1) MT/4) and keeps
-// their G_off fragments (B operand) in registers.  The next pair's
-// multipole rows and G_off fragments are prefetched into registers while
-// the current pair runs on the matrix cores.
-template <int MT>
+// BF16 inputs, FP32 accumulation.  Per pair the source multipole rows
+// (RT * 16 rows of n3p) are staged once per work-group in LDS, rows padded by
+// 8 elements so that the 8 lanes of each 128-byte read phase hit distinct
+// banks, and read by all four wavefronts as the A operand; each wavefront
+// keeps its G_off fragments (B operand) in registers.  The next pair's
+// multipole rows are prefetched into registers while the current pair runs
+// on the matrix cores, and so are its G_off fragments when they fit
+// (MTW * n3p / 32 <= 8 fragments, orders 2 to 5).
+template <int MT, int RTM>
 __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
                                                        const int *__restrict__ targets,
                                                        const int *__restrict__ il_off,
@@ -288,25 +237,27 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
                                                        float *__restrict__ Lc) {
   constexpr int NP = 16 * MT;      // n3p
   constexpr int KS = NP / 32;      // contraction steps of 32
-  constexpr int MTW = MT / 4;      // node tiles per wavefront
+  constexpr int MTW = (MT + 3) / 4;
   constexpr int LDA = NP + 8;      // padded LDS row (elements)
   constexpr int VPR = NP / 8;      // 16-byte vectors per multipole row
-  constexpr int APT = 128 * VPR / 256;  // prefetch vectors per thread (<= 128 rows)
-  __shared__ __attribute__((aligned(16))) __bf16 sA[128 * LDA];
+  constexpr int APT = (RTM * 16 * VPR + 255) / 256;
+  constexpr bool BPRE = MTW * KS <= 8;
+  __shared__ __attribute__((aligned(16))) __bf16 sA[RTM * 16 * LDA];
   const int t = targets[blockIdx.x];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  const int RT = (nr + 15) >> 4;
+  const int RT = min((nr + 15) >> 4, RTM);
   const int AV = RT * 16 * VPR;    // vectors of one pair's A block
   const size_t MS = (size_t)ldm * NP;
-  f32x4 acc[8][MTW];
+  f32x4 acc[RTM][MTW];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < RTM; ++i)
 #pragma unroll
     for (int j = 0; j < MTW; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const int q0 = il_off[t], q1 = il_off[t + 1];
   bf16x8 apre[APT];
-  bf16x8 bcur[MTW][KS], bnext[MTW][KS];
+  bf16x8 bcur[MTW][KS];
+  bf16x8 bnext[BPRE ? MTW : 1][BPRE ? KS : 1];
   auto load_a = [&](int q) {
     const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Mb + (size_t)il_src[q] * MS);
 #pragma unroll
@@ -318,14 +269,17 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
   auto load_b = [&](int q, bf16x8 (&dst)[MTW][KS]) {
     const __bf16 *Go = G + (size_t)il_oidx[q] * NP * NP;
 #pragma unroll
-    for (int j = 0; j < MTW; ++j)
+    for (int j = 0; j < MTW; ++j) {
+      const int mt = wave * MTW + j;
+      if (mt < MT)
 #pragma unroll
-      for (int k = 0; k < KS; ++k)
-        dst[j][k] = *(const bf16x8 *)(Go + (size_t)((wave * MTW + j) * 16 + li) * NP + k * 32 + lk);
+        for (int k = 0; k < KS; ++k)
+          dst[j][k] = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * NP + k * 32 + lk);
+    }
   };
   if (q0 < q1) {
     load_a(q0);
-    load_b(q0, bcur);
+    if constexpr (BPRE) load_b(q0, bcur);
   }
   for (int q = q0; q < q1; ++q) {
     __syncthreads();  // previous pair's LDS reads are done
@@ -335,31 +289,35 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
       if (v < AV) *(bf16x8 *)(sA + (v / VPR) * LDA + (v % VPR) * 8) = apre[i];
     }
     __syncthreads();
+    if constexpr (!BPRE) load_b(q, bcur);
     if (q + 1 < q1) {
       load_a(q + 1);
-      load_b(q + 1, bnext);
+      if constexpr (BPRE) load_b(q + 1, bnext);
     }
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
 #pragma unroll
-      for (int rt = 0; rt < 8; ++rt) {
+      for (int rt = 0; rt < RTM; ++rt) {
         if (rt < RT) {
           const bf16x8 a = *(const bf16x8 *)(sA + (rt * 16 + li) * LDA + k * 32 + lk);
 #pragma unroll
           for (int j = 0; j < MTW; ++j)
-            acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bcur[j][k], acc[rt][j], 0, 0, 0);
+            if (wave * MTW + j < MT)
+              acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bcur[j][k], acc[rt][j], 0, 0, 0);
         }
       }
     }
+    if constexpr (BPRE) {
 #pragma unroll
-    for (int j = 0; j < MTW; ++j)
+      for (int j = 0; j < MTW; ++j)
 #pragma unroll
-      for (int k = 0; k < KS; ++k) bcur[j][k] = bnext[j][k];
+        for (int k = 0; k < KS; ++k) bcur[j][k] = bnext[j][k];
+    }
   }
   const float ih = (float)inv_half[box_level[t]];
   float *out = Lc + (size_t)t * n3 * ldm;
 #pragma unroll
-  for (int rt = 0; rt < 8; ++rt)
+  for (int rt = 0; rt < RTM; ++rt)
 #pragma unroll
     for (int j = 0; j < MTW; ++j)
 #pragma unroll
@@ -369,14 +327,16 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
       }
 }
 
-// Pipelined FP64 MFMA M2L for n3p = 16 MT (MT = 4 or 8) and ldm <= 128.
-// The contraction runs in steps of 16 over every (pair, k-slab); the source
-// multipole slab (RT * 16 rows x 16) is staged in a double-buffered LDS tile
-// shared by the four wavefronts, each wavefront keeps its node tiles' G_off
-// fragments in registers, and the next step's slab and fragments are loaded
-// while the current step runs on the matrix cores (one barrier per step).
-// Lane layout as in k_m2l_f64 below (permuted contraction index).
-template <int MT>
+// FP64 (v_mfma_f64_16x16x4f64).  The contraction runs in steps of 16 over
+// every (pair, k-slab); the source multipole slab (RT * 16 rows x 16) is
+// staged in a double-buffered LDS tile shared by the four wavefronts, each
+// wavefront keeps its node tiles' G_off fragments in registers, and the next
+// step's slab and fragments are loaded while the current step runs on the
+// matrix cores (one barrier per step).  Each lane loads 4 consecutive
+// contraction entries (32 bytes) of both operands and feeds them to 4
+// successive MFMAs: in MFMA kk lane group g = lane / 16 supplies contraction
+// index k0 + 4 g + kk for both operands, so the product sums every index once.
+template <int MT, int RTM>
 __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
                                                       const int *__restrict__ targets,
                                                       const int *__restrict__ il_off,
@@ -387,21 +347,21 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
                                                       const double *__restrict__ G,
                                                       const double *__restrict__ M,
                                                       double *__restrict__ Lc) {
-  constexpr int NP = 16 * MT;  // n3p
-  constexpr int MTW = MT / 4;  // node tiles per wavefront
-  constexpr int NKS = NP / 16; // k-slabs per pair
-  constexpr int KP = 20;       // padded LDS row (doubles)
-  constexpr int APT = 8;       // slab elements per thread (<= 128 rows x 16)
-  __shared__ __attribute__((aligned(32))) double sA[2][128 * KP];
+  constexpr int NP = 16 * MT;   // n3p
+  constexpr int MTW = (MT + 3) / 4;
+  constexpr int NKS = NP / 16;  // k-slabs per pair
+  constexpr int KP = 20;        // padded LDS row (doubles)
+  constexpr int APT = RTM;      // slab elements per thread (RTM * 16 rows x 16)
+  __shared__ __attribute__((aligned(32))) double sA[2][RTM * 16 * KP];
   const int t = targets[blockIdx.x];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = (lane >> 4) * 4;
-  const int RT = (nr + 15) >> 4;
+  const int RT = min((nr + 15) >> 4, RTM);
   const int AE = RT * 16 * 16;
   const size_t MS = (size_t)ldm * NP;
-  f64x4 acc[8][MTW];
+  f64x4 acc[RTM][MTW];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < RTM; ++i)
 #pragma unroll
     for (int j = 0; j < MTW; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
   const int q0 = il_off[t], q1 = il_off[t + 1];
@@ -418,7 +378,10 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
     }
     const double *Go = G + (size_t)il_oidx[q] * NP * NP + k0 + lk;
 #pragma unroll
-    for (int j = 0; j < MTW; ++j) bd[j] = *(const f64x4 *)(Go + (size_t)((wave * MTW + j) * 16 + li) * NP);
+    for (int j = 0; j < MTW; ++j) {
+      const int mt = wave * MTW + j;
+      if (mt < MT) bd[j] = *(const f64x4 *)(Go + (size_t)(mt * 16 + li) * NP);
+    }
   };
   auto store_a = [&](int buf) {
 #pragma unroll
@@ -436,14 +399,15 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
     const int buf = st & 1;
     if (st + 1 < nsteps) load_step(st + 1, bnext);
 #pragma unroll
-    for (int rt = 0; rt < 8; ++rt) {
+    for (int rt = 0; rt < RTM; ++rt) {
       if (rt < RT) {
         const f64x4 a = *(const f64x4 *)(&sA[buf][(rt * 16 + li) * KP + lk]);
 #pragma unroll
         for (int j = 0; j < MTW; ++j)
+          if (wave * MTW + j < MT)
 #pragma unroll
-          for (int kk = 0; kk < 4; ++kk)
-            acc[rt][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], bcur[j][kk], acc[rt][j], 0, 0, 0);
+            for (int kk = 0; kk < 4; ++kk)
+              acc[rt][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], bcur[j][kk], acc[rt][j], 0, 0, 0);
       }
     }
     if (st + 1 < nsteps) {
@@ -456,7 +420,7 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
   const double ih = inv_half[box_level[t]];
   double *out = Lc + (size_t)t * n3 * ldm;
 #pragma unroll
-  for (int rt = 0; rt < 8; ++rt)
+  for (int rt = 0; rt < RTM; ++rt)
 #pragma unroll
     for (int j = 0; j < MTW; ++j)
 #pragma unroll
@@ -466,71 +430,40 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
       }
 }
 
-// MFMA M2L in FP64 (v_mfma_f64_16x16x4f64), same tiling.  Each lane loads 4
-// consecutive contraction entries (32 bytes) and feeds them to 4 successive
-// MFMAs: in step j lane group g = lane / 16 supplies contraction index
-// k0 + 4 g + j for both operands, so the product sums every index once.
-template <int MTW>
-__global__ __launch_bounds__(256) void k_m2l_f64(int n3, int n3p, int nr, int ldm,
-                                                 const int *__restrict__ targets,
-                                                 const int *__restrict__ il_off,
-                                                 const int *__restrict__ il_src,
-                                                 const int *__restrict__ il_oidx,
-                                                 const int *__restrict__ box_level,
-                                                 const double *__restrict__ inv_half,
-                                                 const double *__restrict__ G,
-                                                 const double *__restrict__ M,
-                                                 double *__restrict__ Lc) {
-  const int t = targets[blockIdx.x];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int RT = (nr + 15) >> 4;
-  const int MT = n3p >> 4;
-  const int li = lane & 15, lk = (lane >> 4) * 4;
-  f64x4 acc[8][MTW];
-#pragma unroll
-  for (int rt = 0; rt < 8; ++rt)
-#pragma unroll
-    for (int j = 0; j < MTW; ++j) acc[rt][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
-  const size_t MS = (size_t)ldm * n3p;
-  for (int q = il_off[t]; q < il_off[t + 1]; ++q) {
-    const double *Ms = M + (size_t)il_src[q] * MS;
-    const double *Go = G + (size_t)il_oidx[q] * n3p * n3p;
-    for (int k0 = 0; k0 < n3p; k0 += 16) {
-      f64x4 b[MTW];
-#pragma unroll
-      for (int j = 0; j < MTW; ++j) {
-        int mt = wave * MTW + j;
-        if (mt < MT) b[j] = *(const f64x4 *)(Go + (size_t)(mt * 16 + li) * n3p + k0 + lk);
-      }
-#pragma unroll
-      for (int rt = 0; rt < 8; ++rt) {
-        if (rt < RT) {
-          f64x4 a = *(const f64x4 *)(Ms + (size_t)(rt * 16 + li) * n3p + k0 + lk);
-#pragma unroll
-          for (int j = 0; j < MTW; ++j)
-            if (wave * MTW + j < MT) {
-#pragma unroll
-              for (int kk = 0; kk < 4; ++kk)
-                acc[rt][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], b[j][kk], acc[rt][j], 0, 0, 0);
-            }
-        }
-      }
-    }
-  }
-  // v_mfma_f64_16x16x4f64 interleaves its output rows: D[i][j] sits in lane
-  // j + 16 (i % 4), register i / 4 (unlike the 16x16x32 BF16 form).
-  const double ih = inv_half[box_level[t]];
-  double *out = Lc + (size_t)t * n3 * ldm;
-#pragma unroll
-  for (int rt = 0; rt < 8; ++rt)
-#pragma unroll
-    for (int j = 0; j < MTW; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int r = rt * 16 + (lane >> 4) + 4 * v, m = (wave * MTW + j) * 16 + li;
-        if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
-      }
+template <int MT, int RTM>
+void launch_m2l_bf16(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb, float *Lc,
+                     hipStream_t s) {
+  hipLaunchKernelGGL((k_m2l_bf16_pipe<MT, RTM>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                     f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                     reinterpret_cast<const __bf16 *>(f.Gbf), Mb, Lc);
 }
+template <int MT, int RTM>
+void launch_m2l_f64(const FmmDev<double> &f, int n3, int nr, int ldm, const double *M, double *Lc,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((k_m2l_f64_pipe<MT, RTM>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                     f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, f.G, M, Lc);
+}
+// Instantiation by (MT, RTM): MT = n3p / 16 in {2, 4, 8, 14} (orders 2-6),
+// RTM in {2, 4, 8} the smallest bound on the RHS tiles.
+#define GP_M2L_DISPATCH(FN, MT_, RT_, ...)                                          \
+  do {                                                                             \
+    const int rtm_ = (RT_) <= 2 ? 2 : (RT_) <= 4 ? 4 : 8;                          \
+    switch ((MT_) * 16 + rtm_) {                                                   \
+      case 2 * 16 + 2: FN<2, 2>(__VA_ARGS__); break;                               \
+      case 2 * 16 + 4: FN<2, 4>(__VA_ARGS__); break;                               \
+      case 2 * 16 + 8: FN<2, 8>(__VA_ARGS__); break;                               \
+      case 4 * 16 + 2: FN<4, 2>(__VA_ARGS__); break;                               \
+      case 4 * 16 + 4: FN<4, 4>(__VA_ARGS__); break;                               \
+      case 4 * 16 + 8: FN<4, 8>(__VA_ARGS__); break;                               \
+      case 8 * 16 + 2: FN<8, 2>(__VA_ARGS__); break;                               \
+      case 8 * 16 + 4: FN<8, 4>(__VA_ARGS__); break;                               \
+      case 8 * 16 + 8: FN<8, 8>(__VA_ARGS__); break;                               \
+      case 14 * 16 + 2: FN<14, 2>(__VA_ARGS__); break;                             \
+      case 14 * 16 + 4: FN<14, 4>(__VA_ARGS__); break;                             \
+      case 14 * 16 + 8: FN<14, 8>(__VA_ARGS__); break;                             \
+      default: throw Error(GP_ERR_INTERNAL, "no matrix-core M2L for this n3p");    \
+    }                                                                              \
+  } while (0)
 
 // L2L (bbfmm.rs:337-398): for every box of one level,
 // Lc[child] += (X^T (x) Y^T (x) Z^T)_octant Lc[parent].
@@ -1014,53 +947,18 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   if (f.n_targets > 0) {
     const double fl = 2.0 * (double)f.n_pairs * n3 * n3 * nr;
     const double by = (double)(f.n_boxes + f.n_targets) * exp_bytes;
-    const int MT = n3p / 16;
+    const int MT = n3p / 16, RT = (nr + 15) / 16;
     if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * n3 * nr * 2 + f.n_targets * exp_bytes);
-        const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf);
-        if (MT == 8)
-          hipLaunchKernelGGL((k_m2l_bf16_pipe<8>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
-                             Mb, Lc);
-        else if (MT == 4)
-          hipLaunchKernelGGL((k_m2l_bf16_pipe<4>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
-                             Mb, Lc);
-        else if (MT <= 4)
-          hipLaunchKernelGGL((k_m2l_bf16<1>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
-                             Mb, Lc);
-        else if (MT <= 8)
-          hipLaunchKernelGGL((k_m2l_bf16<2>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
-                             Mb, Lc);
-        else
-          hipLaunchKernelGGL((k_m2l_bf16<4>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, G,
-                             Mb, Lc);
+        GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, f, n3, nr, ldm, Mb, Lc, s);
       } else {
         throw Error(GP_ERR_INTERNAL, "BF16 M2L runs on the FP32 operator only");
       }
-    } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && ldm % 16 == 0 && ldm <= 128 && MT <= 8) {
+    } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && ldm % 16 == 0 && ldm <= 128) {
       if constexpr (sizeof(T) == 8) {
         ProfScope pl("m2l_f64_mfma", s, fl, by);
-        if (MT == 8)
-          hipLaunchKernelGGL((k_m2l_f64_pipe<8>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
-                             f.G, M, Lc);
-        else if (MT == 4)
-          hipLaunchKernelGGL((k_m2l_f64_pipe<4>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
-                             f.G, M, Lc);
-        else if (MT <= 4)
-          hipLaunchKernelGGL((k_m2l_f64<1>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
-                             f.G, M, Lc);
-        else
-          hipLaunchKernelGGL((k_m2l_f64<2>), dim3(f.n_targets), dim3(256), 0, s, n3, n3p, nr, ldm,
-                             f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
-                             f.G, M, Lc);
+        GP_M2L_DISPATCH(launch_m2l_f64, MT, RT, f, n3, nr, ldm, M, Lc, s);
       }
     } else {
       ProfScope pl(bf16_valu ? "m2l_bf16_valu" : f32 ? "m2l_f32" : "m2l_f64", s, fl, by);

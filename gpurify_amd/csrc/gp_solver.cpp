@@ -21,6 +21,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -34,6 +36,24 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
 }
+
+// Progress lines on stderr when GPURIFY_LOG is set (long solves).
+bool log_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("GPURIFY_LOG");
+    return e && *e && *e != '0';
+  }();
+  return on;
+}
+#define GP_LOG(...)                          \
+  do {                                       \
+    if (log_enabled()) {                     \
+      std::fprintf(stderr, "[gpurify] ");    \
+      std::fprintf(stderr, __VA_ARGS__);     \
+      std::fputc('\n', stderr);              \
+      std::fflush(stderr);                   \
+    }                                        \
+  } while (0)
 
 constexpr double kMinPositive = 2.2250738585072014e-308;
 
@@ -565,6 +585,12 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
       GP_HIP(hipStreamSynchronize(s));
       int any = 0;
       for (int r = 0; r < nr; ++r) any |= h_active[r];
+      if (k % 25 == 24) {
+        int na = 0;
+        for (int r = 0; r < nr; ++r) na += h_active[r];
+        GP_LOG("gmres prec %d cycle %d iteration %d: %d of %d columns active", prec, cycle, k + 1,
+               na, nr);
+      }
       if (!any) break;
     }
     kr_gmres_backsolve(m, nr, ld, H.p, g.p, kused.p, Y.p, s);
@@ -633,13 +659,15 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   const int jlo = std::min((int)o.rhs_first, nc);
   const int jhi = o.rhs_count ? std::min(nc, jlo + (int)o.rhs_count) : nc;
   const int nsolve = std::max(1, jhi - jlo);
+  // Blocks of at most 128 columns: the matrix-core M2L holds one block.
   int blk = o.rhs_block ? (int)o.rhs_block : nsolve;
-  blk = std::max(1, std::min(blk, std::min(nsolve, 256)));
-  int ld = blk == 1 ? 1 : round_up(blk, 16);
-  if (ld > 256) ld = 256;
+  blk = std::max(1, std::min(blk, std::min(nsolve, 128)));
+  const int ld = blk == 1 ? 1 : round_up(blk, 16);
   sys.ensure_work(ld);
   GP_HIP(hipStreamSynchronize(s));
   double t1 = now_s();
+  GP_LOG("setup %.3f s: %d panels, depth %d, %d boxes, near nnz %lld, %lld M2L pairs, ld %d",
+         t1 - t0, sys.N, sys.L, sys.t.n_boxes(), (long long)sys.nnz, (long long)sys.pairs, ld);
 
   const size_t vec = (size_t)sys.N * ld;
   DBuf<double> b(vec), x(vec), dCb((size_t)nc * ld);
@@ -697,6 +725,8 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
         sys.apply(x.p, false, nr, ax.p, 0);
         kr_sub(b.p, ax.p, r.p, sys.N, ld, nr, s);
         ++steps;
+        GP_LOG("refinement step %d: inner iterations up to %d", steps,
+               *std::max_element(st.iters.begin(), st.iters.end()));
       }
       refine_steps = std::max(refine_steps, steps);
       for (int q = 0; q < nr; ++q) {

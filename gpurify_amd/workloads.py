@@ -85,8 +85,48 @@ def crossing_bundle(n=16, width=140, space=140, length=20000, seed=5):
     return sort_groups(groups)
 
 
+def routing_block(size=200000, layers=("met1", "met2", "met3"), wmin=140, wmax=1000,
+                  lmin=1000, lmax=20000, n_rhs=32, seed=6):
+    """Random Manhattan routing over a size x size nm block: on each layer
+    (met1 and met3 along x, met2 along y) tracks of width U[wmin, wmax]
+    separated by U[wmin, wmax] carry segments of length U[lmin, lmax]
+    separated by U[wmin, 4/3 length] gaps, about 30% area density per layer.
+    Every segment is a net; n_rhs segments drawn at random are nets
+    0..n_rhs-1 (the right-hand sides solved), the others follow in
+    generation order.  Returns (groups, density per layer)."""
+    st = sky130_stack()
+    rng = np.random.default_rng(seed)
+    segs, dens = [], {}
+    for li, layer in enumerate(layers):
+        horiz = li % 2 == 0
+        area, pos = 0, 0
+        while True:
+            w = int(rng.integers(wmin, wmax + 1))
+            if pos + w > size:
+                break
+            t = int(rng.integers(0, lmax + 1)) - lmax // 2
+            while True:
+                ln = int(rng.integers(lmin, lmax + 1))
+                gap = int(rng.integers(wmin, (4 * ln) // 3 + 1))
+                a, b = max(t, 0), min(t + ln, size)
+                if a >= size:
+                    break
+                if b - a >= lmin // 2:
+                    r = [a, pos, b, pos + w] if horiz else [pos, a, pos + w, b]
+                    segs.append((layer, r))
+                    area += (b - a) * w
+                t += ln + gap
+            pos += w + int(rng.integers(wmin, wmax + 1))
+        dens[layer] = area / float(size) ** 2
+    pick = rng.choice(len(segs), size=min(n_rhs, len(segs)), replace=False)
+    order = list(pick) + [i for i in range(len(segs)) if i not in set(pick.tolist())]
+    groups = [_group(net, segs[i][0], segs[i][1], st) for net, i in enumerate(order)]
+    return sort_groups(groups), dens
+
+
 WORKLOADS = {
     "two_wires_met1": two_wires,
     "bus64x32": bus_crossing,
     "bundle16x16": crossing_bundle,
+    "routing200": lambda: routing_block()[0],
 }

@@ -46,7 +46,7 @@ def test_dense_sphere_and_plates(ctx):
     assert relmax(C, O.cap_dense(v, cond, 2)) < 1e-11
 
 
-@pytest.mark.parametrize("order,level", [(3, 3), (4, 4), (5, 4)])
+@pytest.mark.parametrize("order,level", [(3, 3), (4, 4), (5, 4), (6, 3)])
 def test_fmm_operator_matches_oracle(ctx, order, level):
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     n = len(v)
@@ -64,7 +64,7 @@ def test_fmm_operator_matches_oracle(ctx, order, level):
         assert np.linalg.norm(Y[:, j] - yr) / np.linalg.norm(yr) < 1e-3
 
 
-@pytest.mark.parametrize("order,ncols", [(4, 16), (4, 40), (5, 97), (3, 20)])
+@pytest.mark.parametrize("order,ncols", [(4, 16), (4, 40), (5, 97), (3, 20), (6, 32), (6, 97)])
 def test_fmm_operator_low_precision_close(ctx, order, ncols):
     # ncols > 16 spreads the columns over several 16-wide MFMA row tiles
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
@@ -92,7 +92,8 @@ def test_fmm_operator_low_precision_close(ctx, order, ncols):
 def test_m2l_matrix_cores_match_valu(ctx):
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     X = np.random.default_rng(5).normal(size=(len(v), 97))
-    for order, level, nc in ((4, 4, 40), (5, 4, 97), (5, 4, 16), (3, 3, 40)):
+    for order, level, nc in ((4, 4, 40), (5, 4, 97), (5, 4, 16), (3, 3, 40), (6, 3, 32),
+                             (6, 3, 97)):
         o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, max_level=level)
         Ym = ctx.fmm_apply(v, X[:, :nc], o)
         o.m2l_outer = G.GP_M2L_FP64_VALU
