@@ -86,6 +86,9 @@ WORKLOADS = {
                    "name": "routing200um_met1met3",
                    "cpu": (lambda: _routing(50000),
                            "routing block 50 x 50 um (same generator, seed 6) at refine 1")},
+    # batch: 4096 independent nets, each with its own substrate plate, 1 RHS
+    "nets4096": {"groups": None, "refine": 1, "order": 4, "rhs": 1, "name": "nets4096_met1met2",
+                 "cpu": None},
 }
 
 
@@ -95,10 +98,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is designed, not built)"}))
         return 2
+    if args.workload == "nets4096":
+        return bench_batch(args)
     import numpy as np
 
     import gpurify_amd as G
-    from gpurify_amd import workloads as W
 
     wl = WORKLOADS[args.workload]
     groups = wl["groups"]()
@@ -192,6 +196,110 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_batch(args):
+    """The batched per-net solve: 4096 nets (seed 4), each meshed at the
+    refinement closest to its log-uniform [200, 20k] panel target, with its
+    own substrate plate; one right-hand side per net (its own column of C).
+    The reference rule per net (dense LU below 500 panels, else FMM), FMM of
+    order 4 at adaptive depth.  value = sum of panels / wall time of one
+    gp_cap_solve_batched call (mixed precision)."""
+    import numpy as np
+
+    import gpurify_amd as G
+    from gpurify_amd import workloads as W
+
+    batch = W.net_batch(n_nets=4096, seed=4)
+    meshes = W.net_batch_meshes(batch, lambda g, r: G.mesh_groups(g, refine=r, max_panels=10 ** 8))
+    probs = [(v, c, int(c.max()) + 1) for v, c, _ in meshes]
+    sizes = np.array([len(v) for v, _, _ in probs])
+    n = int(sizes.sum())
+    log(f"workload nets4096: {len(probs)} nets, {n} panels (min {sizes.min()}, max {sizes.max()}), "
+        f"{int((sizes < 500).sum())} below the dense limit")
+    common = dict(fmm_order=args.order, max_level=0, leaf_target=args.leaf_target, tol=args.tol,
+                  rhs_first=0, rhs_count=1)
+    o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol, **common)
+    o_64 = G.default_opts(**common)
+    out = {"metric": "converged BEM unknowns/s, mixed vs FP64 GPU @<=1e-6 rel C error",
+           "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32/fp64", "data": "synthetic (seeded sky130 layout nets4096)"}
+    with G.Context(0) as ctx:
+        log("device:", ctx.info())
+        for w in range(args.warmup):
+            ctx.cap_solve_batched(probs, o_mix)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            Cm, reps = ctx.cap_solve_batched(probs, o_mix)
+            log(f"step {k}: {time.perf_counter() - t0:.3f}s cumulative")
+        T = time.perf_counter() - t0
+        fm = [r for r in reps if r["method_used"] == G.GP_METHOD_FMM]
+        out.update(value=n * args.steps / T, ms_per_step=1e3 * T / args.steps)
+        out["config"] = {"workload": "nets4096_met1met2", "nets": len(probs), "panels": n,
+                         "rhs_per_net": 1, "fmm_order": args.order, "leaf_target": args.leaf_target,
+                         "tol": args.tol, "dense_nets": len(probs) - len(fm),
+                         "fmm_nets": len(fm), "max_depth": max(r["max_level"] for r in fm),
+                         "near_nnz": fm[0]["near_nnz"], "m2l_pairs": fm[0]["m2l_pairs"],
+                         "iterations_max": max(r["iterations"] for r in fm),
+                         "refine_steps_max": max(r["refine_steps"] for r in fm),
+                         "converged": all(r["converged"] for r in reps), "global_batch": len(probs)}
+        ctx.profile(True)
+        ctx.cap_solve_batched(probs, o_mix)
+        prof = ctx.profile_report()
+        ctx.profile(False)
+        kern = sorted(prof.items(), key=lambda kv: -kv[1]["seconds"])
+        out["kernels"] = {k: {"launches": v["launches"], "ms_total": round(1e3 * v["seconds"], 3),
+                              "ms_avg": round(1e3 * v["seconds"] / v["launches"], 4)}
+                          for k, v in kern[:10]}
+        out["roofline"] = roofline(kern)
+        if not args.no_fp64:
+            t = time.perf_counter()
+            C64, r64 = ctx.cap_solve_batched(probs, o_64)
+            t64 = time.perf_counter() - t
+            rel = np.concatenate([np.abs(a[:, 0] - b[:, 0]) / np.abs(b[:, 0]) for a, b in zip(Cm, C64)])
+            out["fp64_gpu"] = {"batched_unknowns_per_s": n / t64, "batched_s": t64,
+                               "converged": all(r["converged"] for r in r64)}
+            if args.plain_rhs > 0:
+                # the reference's loop: one solve per net, FP64, sampled nets
+                idx = np.linspace(0, len(probs) - 1, 16 * args.plain_rhs).astype(int)
+                t = time.perf_counter()
+                for i in idx:
+                    ctx.cap_solve(*probs[i], o_64)
+                tp = time.perf_counter() - t
+                out["fp64_gpu"].update(plain_1rhs_unknowns_per_s=float(sizes[idx].sum()) / tp,
+                                       plain_sample=f"{len(idx)} nets one by one, {tp:.2f}s")
+                out["speedup_vs_fp64_plain"] = out["value"] / out["fp64_gpu"]["plain_1rhs_unknowns_per_s"]
+            out["speedup_vs_fp64_batched"] = out["value"] / out["fp64_gpu"]["batched_unknowns_per_s"]
+            out["max_rel_C_err_vs_fp64"] = float(rel.max())
+            out["gate_1e-6_pass"] = bool(rel.max() <= 1e-6)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(args, probs)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline_batch(args, probs):
+    """The oracle on a sample of the nets, one by one by the reference's rule
+    (dense LU below 500 panels, else FMM of the bench's order), 1 RHS each."""
+    import numpy as np
+    from oracle import oracle as O
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    idx = np.linspace(0, len(probs) - 1, 24).astype(int)
+    n, t = 0, time.perf_counter()
+    for i in idx:
+        v, c, nc = probs[i]
+        n += len(v)
+        if len(v) < 500:
+            O.cap_dense(v, c, nc)
+        else:
+            O.cap_fmm_cols(v, c, nc, 0, 1, order=args.order, max_level=weighted_depth(v, args.leaf_target),
+                           tol=args.tol)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "unknowns/s", "cores": threads, "kind": "port",
+            "sample": f"{len(idx)} of the 4096 nets (evenly spaced), {n} panels, one by one: {dt:.2f}s"}
 
 
 def roofline(kern):

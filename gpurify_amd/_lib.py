@@ -87,7 +87,8 @@ class GpVolumeGroup(C.Structure):
 # Every symbol include/gpurify.h declares (checked by the CPU test suite).
 EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_device_ready", "gp_device_info",
-    "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_fmm_apply", "gp_mesh_groups",
+    "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_cap_solve_batched",
+    "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report",
 ]
 
@@ -120,6 +121,9 @@ def load():
     L.gp_cap_solve.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(GpCapOpts),
                                C.POINTER(C.c_double), C.POINTER(GpCapReport)]
     L.gp_cap_solve.restype = C.c_int
+    L.gp_cap_solve_batched.argtypes = [vp, C.POINTER(GpPanels), C.c_uint32, C.POINTER(GpCapOpts),
+                                       C.POINTER(C.c_double), C.POINTER(GpCapReport)]
+    L.gp_cap_solve_batched.restype = C.c_int
     L.gp_fmm_apply.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(GpCapOpts), C.c_uint32,
                                C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.gp_fmm_apply.restype = C.c_int
@@ -231,6 +235,26 @@ class Context:
         o = opts if opts is not None else default_opts()
         self._check(self._L.gp_cap_solve(self._h, C.byref(p), C.byref(o), _dptr(Cm), C.byref(rep)))
         return Cm, rep.as_dict()
+
+    def cap_solve_batched(self, problems, opts: GpCapOpts | None = None):
+        """problems: [(verts4, cond, ncond)] -> ([C_p], [report_p])."""
+        keep, arr = [], (GpPanels * len(problems))()
+        for i, (verts4, cond, ncond) in enumerate(problems):
+            v = np.ascontiguousarray(verts4, dtype=np.float64).reshape(-1, 4, 3)
+            c = np.ascontiguousarray(cond, dtype=np.uint32)
+            keep += [v, c]
+            arr[i] = GpPanels(v.shape[0], ncond, _dptr(v), _uptr(c))
+        sizes = [nc * nc for _, _, nc in problems]
+        Cflat = np.zeros(sum(sizes))
+        reps = (GpCapReport * len(problems))()
+        o = opts if opts is not None else default_opts()
+        self._check(self._L.gp_cap_solve_batched(self._h, arr, len(problems), C.byref(o),
+                                                  _dptr(Cflat), reps))
+        out, off = [], 0
+        for (_, _, nc), sz in zip(problems, sizes):
+            out.append(Cflat[off:off + sz].reshape(nc, nc))
+            off += sz
+        return out, [r.as_dict() for r in reps]
 
     def fmm_apply(self, verts4, X, opts: GpCapOpts | None = None, precision=GP_PREC_FP64):
         v = np.ascontiguousarray(verts4, dtype=np.float64).reshape(-1, 4, 3)

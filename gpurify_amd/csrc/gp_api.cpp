@@ -188,6 +188,34 @@ int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
   });
 }
 
+int gp_cap_solve_batched(gp_ctx *ctx, const gp_panels *problems, uint32_t n_problems,
+                         const gp_cap_opts *opts, double *C, gp_cap_report *reports) {
+  if (!ctx) {
+    g_thread_error = "gp_cap_solve_batched: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    if (!problems || !C || n_problems == 0)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "null problems or C, or no problems");
+    gp_cap_opts o;
+    if (opts)
+      o = *opts;
+    else
+      gp_cap_opts_default(&o);
+    GP_HIP(hipSetDevice(ctx->device));
+    std::vector<gp::PanelGeom> gs(n_problems);
+    std::vector<double *> Cp(n_problems);
+    size_t off = 0;
+    for (uint32_t p = 0; p < n_problems; ++p) {
+      gp::panel_geometry(problems[p], gs[p]);
+      Cp[p] = C + off;
+      off += (size_t)gs[p].ncond * gs[p].ncond;
+    }
+    gp::cap_solve_batched(ctx, gs, o, Cp, reports);
+  });
+}
+
 int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, uint32_t nr,
                  uint32_t precision, const double *X, double *Y) {
   if (!ctx) {

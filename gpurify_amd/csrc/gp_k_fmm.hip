@@ -35,7 +35,7 @@ __device__ inline void store_mult(T *M, __bf16 *Mb, size_t idx, T v) {
 // 1-D interpolation weights w1d[p][axis][k] (chebyshev.rs:77-94).
 template <typename T>
 __global__ __launch_bounds__(256) void k_p2m(int n, int n3p, const int *__restrict__ leaf_lo,
-                                             int leaf_box0, const T *__restrict__ w1d,
+                                             const int *__restrict__ leaf_box, int leaf_box0, const T *__restrict__ w1d,
                                              const double *__restrict__ area,
                                              const T *__restrict__ z, int ldz, int nr,
                                              T *__restrict__ M, __bf16 *__restrict__ Mb, int ldm) {
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_p2m(int n, int n3p, const int *__restri
       }
     }
   }
-  const size_t base = (size_t)(leaf_box0 + li) * ldm * n3p;
+  const size_t base = (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * ldm * n3p;
 #pragma unroll
   for (int k = 0; k < FM_OPT; ++k) {
     int o = threadIdx.x + k * 256;
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_l2l(int n, int box0, const int *__restr
 
 // L2P (bbfmm.rs:400-417): y[p][r] += sum_m S(p, m) Lc[leaf][m][r].
 template <typename T>
-__global__ __launch_bounds__(256) void k_l2p(int n, const int *__restrict__ leaf_lo, int leaf_box0,
+__global__ __launch_bounds__(256) void k_l2p(int n, const int *__restrict__ leaf_lo, const int *__restrict__ leaf_box, int leaf_box0,
                                              const T *__restrict__ w1d, int nr,
                                              const T *__restrict__ Lc, int ldm,
                                              double *__restrict__ y, int ldy) {
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(256) void k_l2p(int n, const int *__restrict__ leaf
   const int li = blockIdx.x, r0 = blockIdx.y * FM_RC;
   const int rc = min(FM_RC, nr - r0);
   const int n3 = n * n * n, nn = n * n;
-  const T *ll = Lc + (size_t)(leaf_box0 + li) * n3 * ldm;
+  const T *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * n3 * ldm;
   for (int e = threadIdx.x; e < n3 * rc; e += 256) s0[e] = ll[(size_t)(e / rc) * ldm + r0 + e % rc];
   __syncthreads();
   const int p0 = leaf_lo[li], p1 = leaf_lo[li + 1];
@@ -569,7 +569,7 @@ __device__ inline T tensor_weight(const T *__restrict__ w, int n, int m) {
 // nodes (mg + 16 i) x RB columns (rg RB + j) in registers.
 template <typename T, int MB>
 __global__ __launch_bounds__(256) void k_p2m_tiled(int n, int n3p, const int *__restrict__ leaf_lo,
-                                                   int leaf_box0, const T *__restrict__ w1d,
+                                                   const int *__restrict__ leaf_box, int leaf_box0, const T *__restrict__ w1d,
                                                    const double *__restrict__ area,
                                                    const T *__restrict__ z, int ldz, int nr,
                                                    T *__restrict__ M, __bf16 *__restrict__ Mb,
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(256) void k_p2m_tiled(int n, int n3p, const int *__
     }
     __syncthreads();
   }
-  const size_t base = (size_t)(leaf_box0 + li) * ldm * n3p;
+  const size_t base = (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * ldm * n3p;
 #pragma unroll
   for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(256) void k_p2m_tiled(int n, int n3p, const int *__
 // Lc and S staged in LDS; each thread owns 2 points x RB columns.
 template <typename T>
 __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict__ leaf_lo,
-                                                   int leaf_box0, const T *__restrict__ w1d, int nr,
+                                                   const int *__restrict__ leaf_box, int leaf_box0, const T *__restrict__ w1d, int nr,
                                                    const T *__restrict__ Lc, int ldm,
                                                    double *__restrict__ y, int ldy) {
   __shared__ T sL[TL_KM][128];
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict_
   const int pg = tid & 15, rg = tid >> 4;
   const int RB = ldm >> 4;
   const int n3 = n * n * n;
-  const T *ll = Lc + (size_t)(leaf_box0 + li) * n3 * ldm;
+  const T *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * n3 * ldm;
   const int pb = leaf_lo[li], pe = leaf_lo[li + 1];
   for (int p0 = pb; p0 < pe; p0 += TL_PT) {
     const int np = min(TL_PT, pe - p0);
@@ -870,7 +870,7 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   const size_t lds1 = (size_t)n3 * FM_RC * sizeof(T);
   const int L = f.L;
   const int leaf0 = f.lvl_off[L];
-  const int nleaf = f.lvl_off[L + 1] - leaf0;
+  const int nleaf = f.leaf_box ? f.n_leaves : f.lvl_off[L + 1] - leaf0;
   const bool f32 = sizeof(T) == 4;
   const double exp_bytes = (double)n3 * nr * sizeof(T);
   const bool want_bf16 = (m2l == GP_M2L_BF16);
@@ -887,25 +887,26 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       const int MB = n3p / 16;
       if (MB <= 2)
         hipLaunchKernelGGL((k_p2m_tiled<T, 2>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
-                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+                           f.leaf_box, leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
       else if (MB <= 4)
         hipLaunchKernelGGL((k_p2m_tiled<T, 4>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
-                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+                           f.leaf_box, leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
       else if (MB <= 8)
         hipLaunchKernelGGL((k_p2m_tiled<T, 8>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
-                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+                           f.leaf_box, leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
       else
         hipLaunchKernelGGL((k_p2m_tiled<T, 14>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
-                           leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
+                           f.leaf_box, leaf0, f.w1d, f.area, z, ldz, nr, M, mb, ldm);
     } else {
-      hipLaunchKernelGGL((k_p2m<T>), dim3(nleaf, nrc), dim3(256), 0, s, n, n3p, f.leaf_lo, leaf0,
+      hipLaunchKernelGGL((k_p2m<T>), dim3(nleaf, nrc), dim3(256), 0, s, n, n3p, f.leaf_lo, f.leaf_box, leaf0,
                          f.w1d, f.area, z, ldz, nr, M, mb, ldm);
     }
     GP_LAUNCH_CHECK();
   }
-  // M2M, finest to level 2 (multipoles above level 2 are never used).
+  // M2M, finest to level 2 (multipoles above level 2 are never used), over
+  // the boxes of each level that have children (listed first in the level).
   for (int l = L - 1; l >= 2; --l) {
-    int nb = f.lvl_off[l + 1] - f.lvl_off[l];
+    int nb = f.npar ? f.npar[l] : f.lvl_off[l + 1] - f.lvl_off[l];
     if (nb == 0) continue;
     int nch = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch, (nb + nch) * exp_bytes);
@@ -991,10 +992,10 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   ProfScope pp(f32 ? "l2p_f32" : "l2p_f64", s, 2.0 * f.n_panels * n3 * nr,
                (double)f.n_panels * nr * 16.0 + nleaf * exp_bytes);
   if (tiled)
-    hipLaunchKernelGGL((k_l2p_tiled<T>), dim3(nleaf), dim3(256), 0, s, n, f.leaf_lo, leaf0, f.w1d,
+    hipLaunchKernelGGL((k_l2p_tiled<T>), dim3(nleaf), dim3(256), 0, s, n, f.leaf_lo, f.leaf_box, leaf0, f.w1d,
                        nr, Lc, ldm, y, ldy);
   else
-    hipLaunchKernelGGL((k_l2p<T>), dim3(nleaf, nrc), dim3(256), lds1, s, n, f.leaf_lo, leaf0,
+    hipLaunchKernelGGL((k_l2p<T>), dim3(nleaf, nrc), dim3(256), lds1, s, n, f.leaf_lo, f.leaf_box, leaf0,
                        f.w1d, nr, Lc, ldm, y, ldy);
   GP_LAUNCH_CHECK();
 }

@@ -270,9 +270,175 @@ __global__ __launch_bounds__(256) void k_gmres_update_x(double *__restrict__ x,
   }
 }
 
+// ---------------------------------------------------------------- segmented
+// A batch of independent systems stacked in one vector of N rows (one
+// column): rows [seg_lo[p], seg_lo[p + 1]) belong to system p = seg[row].
+// Per-system state (coefficients, scales, Hessenberg columns) has stride
+// nsys.  Dot products reduce per system over a fixed block map: block b
+// covers rows [blk_lo[b], blk_hi[b]) of system blk_sys[b], and the blocks of
+// system p are [sys_blk[p], sys_blk[p + 1]) (fixed order, same bits every run).
+
+// partial[v * nblk + b] = sum over the rows of block b of X_v[row] * Y[row]
+template <typename TX>
+__global__ __launch_bounds__(256) void k_seg_dot_partial(const TX *__restrict__ X, size_t strideX,
+                                                         int nvec, const double *__restrict__ Y,
+                                                         const int *__restrict__ blk_lo,
+                                                         const int *__restrict__ blk_hi,
+                                                         double *__restrict__ partial) {
+  __shared__ double red[KR_VB][256 / 64];
+  const int b = blockIdx.x, nblk = gridDim.x, v0 = blockIdx.y * KR_VB;
+  const int nv = min(KR_VB, nvec - v0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double acc[KR_VB];
+#pragma unroll
+  for (int v = 0; v < KR_VB; ++v) acc[v] = 0.0;
+  for (int row = blk_lo[b] + tid; row < blk_hi[b]; row += 256) {
+    const double y = Y[row];
+#pragma unroll
+    for (int v = 0; v < KR_VB; ++v)
+      if (v < nv) acc[v] += (double)X[(size_t)(v0 + v) * strideX + row] * y;
+  }
+#pragma unroll
+  for (int v = 0; v < KR_VB; ++v) {
+    double a = acc[v];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (lane == 0) red[v][wave] = a;
+  }
+  __syncthreads();
+  if (tid < nv) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += red[tid][w];
+    partial[(size_t)(v0 + tid) * nblk + b] = t;
+  }
+}
+
+// out[v * nsys + p] = sum of the partials of system p's blocks (optionally sqrt)
+__global__ void k_seg_reduce(const double *__restrict__ partial, int nblk, int nvec,
+                             const int *__restrict__ sys_blk, int nsys, double *__restrict__ out,
+                             int take_sqrt) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nvec * nsys) return;
+  int v = e / nsys, p = e % nsys;
+  double t = 0.0;
+  for (int b = sys_blk[p]; b < sys_blk[p + 1]; ++b) t += partial[(size_t)v * nblk + b];
+  out[e] = take_sqrt ? sqrt(t) : t;
+}
+
+// w[row] += alpha * sum_{v < nvec} coef[v][seg[row]] * V_v[row]
+template <typename TV>
+__global__ __launch_bounds__(256) void k_seg_axpy_multi(double *__restrict__ w,
+                                                        const TV *__restrict__ V, size_t strideV,
+                                                        const double *__restrict__ coef, int nvec,
+                                                        int N, const int *__restrict__ seg, int nsys,
+                                                        double alpha) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < N; row += gridDim.x * blockDim.x) {
+    const int p = seg[row];
+    double t = 0.0;
+    for (int v = 0; v < nvec; ++v) t += coef[(size_t)v * nsys + p] * (double)V[(size_t)v * strideV + row];
+    w[row] += alpha * t;
+  }
+}
+
+// out[row] = scale[seg[row]] * in[row] (0 where mask[seg[row]] == 0)
+template <typename TO>
+__global__ __launch_bounds__(256) void k_seg_scale(const double *__restrict__ in, TO *__restrict__ out,
+                                                   const double *__restrict__ scale,
+                                                   const int *__restrict__ mask, int N,
+                                                   const int *__restrict__ seg) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < N; row += gridDim.x * blockDim.x) {
+    const int p = seg[row];
+    out[row] = (TO)((mask && !mask[p]) ? 0.0 : scale[p] * in[row]);
+  }
+}
+
+// x[row] += dinv[row] * sum_{i < kused[p]} Y[i][p] V_i[row], p = seg[row]
+template <typename TV>
+__global__ __launch_bounds__(256) void k_seg_update_x(double *__restrict__ x,
+                                                      const TV *__restrict__ V, size_t strideV,
+                                                      const double *__restrict__ Y,
+                                                      const int *__restrict__ kused, int kmax,
+                                                      const double *__restrict__ dinv, int N,
+                                                      const int *__restrict__ seg, int nsys) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < N; row += gridDim.x * blockDim.x) {
+    const int p = seg[row];
+    const int ku = min(kused[p], kmax);
+    double t = 0.0;
+    for (int i = 0; i < ku; ++i) t += Y[(size_t)i * nsys + p] * (double)V[(size_t)i * strideV + row];
+    x[row] += (dinv ? dinv[row] : 1.0) * t;
+  }
+}
+
+// b[row] = 1 where the panel's (batch-global) conductor is system p's
+// right-hand-side conductor, else 0.
+__global__ __launch_bounds__(256) void k_seg_unit_rhs(const uint32_t *__restrict__ cond, int N,
+                                                      const int *__restrict__ seg,
+                                                      const int *__restrict__ rhs_cond,
+                                                      double *__restrict__ b) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < N; row += gridDim.x * blockDim.x)
+    b[row] = ((int)cond[row] == rhs_cond[seg[row]]) ? 1.0 : 0.0;
+}
+
 namespace {
 int grid_for(size_t total) { return (int)std::min<size_t>((total + 255) / 256, 8192); }
 }  // namespace
+
+template <typename TX>
+void seg_dots(const SegMap &sm, const TX *X, size_t strideX, int nvec, const double *Y,
+              double *partial, double *out, bool take_sqrt, hipStream_t s) {
+  if (nvec <= 0) return;
+  ProfScope ps("gmres_dots", s, 2.0 * nvec * sm.N, (sizeof(TX) * nvec + 8.0) * sm.N);
+  hipLaunchKernelGGL((k_seg_dot_partial<TX>), dim3(sm.nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0,
+                     s, X, strideX, nvec, Y, sm.blk_lo, sm.blk_hi, partial);
+  GP_LAUNCH_CHECK();
+  const int tot = nvec * sm.nsys;
+  hipLaunchKernelGGL(k_seg_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, partial, sm.nblk, nvec,
+                     sm.sys_blk, sm.nsys, out, take_sqrt ? 1 : 0);
+  GP_LAUNCH_CHECK();
+}
+size_t seg_partial_size(const SegMap &sm, int nvec_max) {
+  return (size_t)sm.nblk * ((nvec_max + KR_VB - 1) / KR_VB) * KR_VB;
+}
+template <typename TV>
+void seg_axpy_multi(const SegMap &sm, double *w, const TV *V, size_t strideV, const double *coef,
+                    int nvec, double alpha, hipStream_t s) {
+  if (nvec <= 0) return;
+  ProfScope ps("gmres_axpy", s, 2.0 * nvec * sm.N, (sizeof(TV) * nvec + 16.0) * sm.N);
+  hipLaunchKernelGGL((k_seg_axpy_multi<TV>), dim3(grid_for(sm.N)), dim3(256), 0, s, w, V, strideV,
+                     coef, nvec, sm.N, sm.seg, sm.nsys, alpha);
+  GP_LAUNCH_CHECK();
+}
+template <typename TO>
+void seg_scale(const SegMap &sm, const double *in, TO *out, const double *scale, const int *mask,
+               hipStream_t s) {
+  hipLaunchKernelGGL((k_seg_scale<TO>), dim3(grid_for(sm.N)), dim3(256), 0, s, in, out, scale, mask,
+                     sm.N, sm.seg);
+  GP_LAUNCH_CHECK();
+}
+template <typename TV>
+void seg_update_x(const SegMap &sm, double *x, const TV *V, size_t strideV, const double *Y,
+                  const int *kused, int kmax, const double *dinv, hipStream_t s) {
+  hipLaunchKernelGGL((k_seg_update_x<TV>), dim3(grid_for(sm.N)), dim3(256), 0, s, x, V, strideV, Y,
+                     kused, kmax, dinv, sm.N, sm.seg, sm.nsys);
+  GP_LAUNCH_CHECK();
+}
+void seg_unit_rhs(const SegMap &sm, const uint32_t *cond, const int *rhs_cond, double *b,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_unit_rhs, dim3(grid_for(sm.N)), dim3(256), 0, s, cond, sm.N, sm.seg,
+                     rhs_cond, b);
+  GP_LAUNCH_CHECK();
+}
+#define GP_SEG_BASIS(TV)                                                                        \
+  template void seg_dots<TV>(const SegMap &, const TV *, size_t, int, const double *, double *,  \
+                             double *, bool, hipStream_t);                                      \
+  template void seg_axpy_multi<TV>(const SegMap &, double *, const TV *, size_t, const double *, \
+                                   int, double, hipStream_t);                                   \
+  template void seg_scale<TV>(const SegMap &, const double *, TV *, const double *, const int *, \
+                              hipStream_t);                                                     \
+  template void seg_update_x<TV>(const SegMap &, double *, const TV *, size_t, const double *,   \
+                                 const int *, int, const double *, hipStream_t);
+GP_SEG_BASIS(double)
+GP_SEG_BASIS(float)
+#undef GP_SEG_BASIS
 
 template <typename TX>
 void kr_dots(const TX *X, size_t strideX, int nvec, const double *Y, int N, int ld, int nr,

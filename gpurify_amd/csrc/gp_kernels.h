@@ -49,6 +49,13 @@ struct FmmDev {
   long n_pairs = 0;   // M2L (target, source) pairs
   int n_panels = 0;
   const int *lvl_off = nullptr;  // HOST array, L + 2 entries
+  // Several independent trees (a batch) share one box numbering: each level
+  // lists its boxes with children first (npar, HOST, L + 1 entries) and the
+  // leaves of every depth are listed in leaf_box (device, n_leaves).  Both
+  // null for one tree whose leaves are the boxes of level L.
+  const int *npar = nullptr;
+  const int *leaf_box = nullptr;
+  int n_leaves = 0;
   const int *leaf_lo = nullptr;  // device, n_leaves + 1
   const int *parent = nullptr, *octant = nullptr, *child_off = nullptr, *child_idx = nullptr;
   const int *targets = nullptr, *il_off = nullptr, *il_src = nullptr, *il_oidx = nullptr;
@@ -99,6 +106,31 @@ void kr_gmres_givens(int k, int m, int nr, int ld, const double *h1, const doubl
                      double *inv_h, hipStream_t s);
 void kr_gmres_backsolve(int m, int nr, int ld, const double *H, const double *g, const int *kused,
                         double *Y, hipStream_t s);
+// Segmented vectors: a batch of independent systems stacked in one column of
+// N rows (device arrays; seg per row, a fixed block map for reductions).
+struct SegMap {
+  int N = 0, nsys = 0, nblk = 0;
+  const int *seg = nullptr;      // N: system of each row
+  const int *blk_lo = nullptr;   // nblk: first row of each reduction block
+  const int *blk_hi = nullptr;   // nblk: end row of each reduction block
+  const int *sys_blk = nullptr;  // nsys + 1: blocks of each system
+};
+size_t seg_partial_size(const SegMap &sm, int nvec_max);
+template <typename TX>
+void seg_dots(const SegMap &sm, const TX *X, size_t strideX, int nvec, const double *Y,
+              double *partial, double *out, bool take_sqrt, hipStream_t s);
+template <typename TV>
+void seg_axpy_multi(const SegMap &sm, double *w, const TV *V, size_t strideV, const double *coef,
+                    int nvec, double alpha, hipStream_t s);
+template <typename TO>
+void seg_scale(const SegMap &sm, const double *in, TO *out, const double *scale, const int *mask,
+               hipStream_t s);
+template <typename TV>
+void seg_update_x(const SegMap &sm, double *x, const TV *V, size_t strideV, const double *Y,
+                  const int *kused, int kmax, const double *dinv, hipStream_t s);
+void seg_unit_rhs(const SegMap &sm, const uint32_t *cond, const int *rhs_cond, double *b,
+                  hipStream_t s);
+
 template <typename TV>
 void kr_gmres_update_x(double *x, const TV *V, size_t strideV, const double *Y,
                        const int *kused, int kmax, const double *dinv, int N, int ld, int nr,

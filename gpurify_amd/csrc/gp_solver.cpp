@@ -195,9 +195,14 @@ void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
 struct FmmSystem {
   hipStream_t s = nullptr;
   int N = 0, n = 0, L = 0, ncond = 0;
-  Tree t;
-  double c[3] = {0, 0, 0}, L0 = 1.0;
-  std::vector<int> lvl_off;
+  // One tree per problem; a batch stacks independent problems (block-diagonal
+  // operator) under one box numbering: per level the boxes with children
+  // first, then the leaves of the problems whose depth is that level.
+  std::vector<Tree> trees;
+  std::vector<double> L0s;                // root half-width per problem, metres
+  std::vector<int> panel_off, cond_off;   // per problem (+1): sorted panel / conductor ranges
+  std::vector<int> lvl_off, npar;
+  int n_leaves = 0;
   int64_t nnz = 0;
   long pairs = 0;
   bool have32 = false;
@@ -205,7 +210,7 @@ struct FmmSystem {
   int m2l_inner = GP_M2L_BF16, m2l_outer = GP_M2L_FP64;
   DBuf<double> verts, cen, area, inv_diag;
   DBuf<uint32_t> cond;
-  DBuf<int> leaf_lo, col_off, cols, panel_leaf;
+  DBuf<int> leaf_lo, leaf_box, col_off, cols, panel_leaf;
   DBuf<int64_t> nf_off;
   DBuf<double> nf64;
   DBuf<float> nf32;
@@ -224,10 +229,17 @@ struct FmmSystem {
   DBuf<uint16_t> Mbf;
   long matvecs = 0;
 
+  const Tree &t() const { return trees[0]; }
+  double L0() const { return L0s[0]; }
+
   void setup(const PanelGeom &g, const gp_cap_opts &o, bool need32, hipStream_t stream) {
+    std::vector<const PanelGeom *> one{&g};
+    setup(one, o, need32, stream);
+  }
+
+  void setup(const std::vector<const PanelGeom *> &probs, const gp_cap_opts &o, bool need32,
+             hipStream_t stream) {
     s = stream;
-    N = g.n;
-    ncond = g.ncond;
     n = (int)o.fmm_order;
     if (n < 2 || n > 6) throw Error(GP_ERR_INVALID_INPUT, "fmm_order must be in [2, 6]");
     m2l_inner = (int)o.m2l_inner;
@@ -236,36 +248,125 @@ struct FmmSystem {
       throw Error(GP_ERR_INVALID_INPUT, "m2l_inner must be GP_M2L_BF16, _FP32 or _BF16_VALU");
     if (m2l_outer != GP_M2L_FP64 && m2l_outer != GP_M2L_FP64_VALU)
       throw Error(GP_ERR_INVALID_INPUT, "m2l_outer must be GP_M2L_FP64 or GP_M2L_FP64_VALU");
-    L = o.max_level ? (int)o.max_level : choose_depth(g, (int)o.leaf_target);
-    if (L < 2) L = 2;
-    build_tree(g, L, t);
-    L0 = t.half;
-    std::memcpy(c, t.center, sizeof(c));
-    lvl_off = t.lvl_off;
-    std::vector<double> hv, hc, ha;
-    std::vector<uint32_t> hcond;
-    scaled_arrays(g, t.perm.data(), c, L0, hv, hc, ha, hcond);
+    const int P = (int)probs.size();
+    trees.assign(P, Tree());
+    L0s.assign(P, 1.0);
+    L = 2;
+    for (int p = 0; p < P; ++p) {
+      const PanelGeom &g = *probs[p];
+      int Lp = o.max_level ? (int)o.max_level : choose_depth(g, (int)o.leaf_target);
+      build_tree(g, std::max(Lp, 2), trees[p]);
+      L0s[p] = trees[p].half;
+      L = std::max(L, trees[p].L);
+    }
+    // Problems in order of depth: the global leaf order (and so the panel
+    // order) lists shallower problems first.
+    std::vector<int> order(P);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return trees[a].L < trees[b].L; });
+    panel_off.assign(P + 1, 0);
+    cond_off.assign(P + 1, 0);
+    {
+      std::vector<int> cnt(P);
+      for (int p = 0; p < P; ++p) cnt[p] = probs[p]->n;
+      int acc = 0;
+      for (int p : order) {
+        panel_off[p] = acc;
+        acc += cnt[p];
+      }
+      N = acc;
+      panel_off[P] = N;  // (per-problem ranges are panel_off[p] .. + n_p)
+      for (int p = 0; p < P; ++p) cond_off[p + 1] = cond_off[p] + probs[p]->ncond;
+      ncond = cond_off[P];
+    }
+    // Global box numbering.
+    std::vector<std::vector<int>> gid(P);
+    for (int p = 0; p < P; ++p) gid[p].assign(trees[p].n_boxes(), -1);
+    lvl_off.assign(L + 2, 0);
+    npar.assign(L + 1, 0);
+    std::vector<int> gp_of, gb_of;  // global box -> (problem, local box)
+    std::vector<int> hleaf_box, hleaf_lo{0};
+    for (int l = 0; l <= L; ++l) {
+      lvl_off[l] = (int)gp_of.size();
+      for (int p : order) {
+        const Tree &t = trees[p];
+        if (t.L <= l) continue;
+        for (int b = t.lvl_off[l]; b < t.lvl_off[l + 1]; ++b) {
+          gid[p][b] = (int)gp_of.size();
+          gp_of.push_back(p);
+          gb_of.push_back(b);
+        }
+      }
+      npar[l] = (int)gp_of.size() - lvl_off[l];
+      for (int p : order) {
+        const Tree &t = trees[p];
+        if (t.L != l) continue;
+        for (int T = 0; T < t.n_leaves(); ++T) {
+          const int b = t.leaf_box(T);
+          gid[p][b] = (int)gp_of.size();
+          gp_of.push_back(p);
+          gb_of.push_back(b);
+          hleaf_box.push_back(gid[p][b]);
+          hleaf_lo.push_back(panel_off[p] + t.leaf_lo[T + 1]);
+        }
+      }
+    }
+    lvl_off[L + 1] = (int)gp_of.size();
+    const int nb = (int)gp_of.size();
+    n_leaves = (int)hleaf_box.size();
+    // Leaf index of each problem's first leaf, for the near-field tiles.
+    std::vector<int> leaf_first(P, 0);
+    {
+      int acc = 0;
+      for (int l = 0; l <= L; ++l)
+        for (int p : order)
+          if (trees[p].L == l) {
+            leaf_first[p] = acc;
+            acc += trees[p].n_leaves();
+          }
+    }
+
+    // Scaled panel arrays, per problem in its own unit frame.
+    std::vector<double> hv((size_t)N * 12), hc((size_t)N * 3), ha(N);
+    std::vector<uint32_t> hcond(N);
+    for (int p = 0; p < P; ++p) {
+      std::vector<double> v, c, a;
+      std::vector<uint32_t> cd;
+      scaled_arrays(*probs[p], trees[p].perm.data(), trees[p].center, L0s[p], v, c, a, cd);
+      const size_t o0 = panel_off[p];
+      std::copy(v.begin(), v.end(), hv.begin() + o0 * 12);
+      std::copy(c.begin(), c.end(), hc.begin() + o0 * 3);
+      std::copy(a.begin(), a.end(), ha.begin() + o0);
+      for (size_t i = 0; i < cd.size(); ++i) hcond[o0 + i] = cd[i] + (uint32_t)cond_off[p];
+    }
     verts = to_device(hv, s);
     cen = to_device(hc, s);
     area = to_device(ha, s);
     cond = to_device(hcond, s);
 
-    // Near-field tiles (cap/fmm_solver.rs:52-91 over bbfmm.rs:189-206).
-    const int nl = t.n_leaves();
-    std::vector<int> hcol_off(nl + 1, 0), hcols, hpl(N);
-    std::vector<int64_t> hnf(nl + 1, 0);
-    for (int T = 0; T < nl; ++T) {
-      for (int q = t.near_off[T]; q < t.near_off[T + 1]; ++q) {
-        int S = t.near_leaf[q];
-        for (int p = t.leaf_lo[S]; p < t.leaf_lo[S + 1]; ++p) hcols.push_back(p);
+    // Near-field tiles (cap/fmm_solver.rs:52-91 over bbfmm.rs:189-206), per
+    // global leaf: its panels against the panels of its near leaves.
+    std::vector<int> hcol_off(n_leaves + 1, 0), hcols, hpl(N);
+    std::vector<int64_t> hnf(n_leaves + 1, 0);
+    for (int T = 0; T < n_leaves; ++T) {
+      const int gbox = hleaf_box[T];
+      const int p = gp_of[gbox];
+      const Tree &t = trees[p];
+      const int lt = gb_of[gbox] - t.lvl_off[t.L];
+      for (int q = t.near_off[lt]; q < t.near_off[lt + 1]; ++q) {
+        const int S = t.near_leaf[q];
+        for (int k = t.leaf_lo[S]; k < t.leaf_lo[S + 1]; ++k) hcols.push_back(panel_off[p] + k);
       }
       hcol_off[T + 1] = (int)hcols.size();
-      int nt_ = t.leaf_lo[T + 1] - t.leaf_lo[T];
+      const int nt_ = hleaf_lo[T + 1] - hleaf_lo[T];
       hnf[T + 1] = hnf[T] + (int64_t)nt_ * (hcol_off[T + 1] - hcol_off[T]);
-      for (int p = t.leaf_lo[T]; p < t.leaf_lo[T + 1]; ++p) hpl[p] = T;
+      for (int k = hleaf_lo[T]; k < hleaf_lo[T + 1]; ++k) hpl[k] = T;
     }
-    nnz = hnf[nl];
-    leaf_lo = to_device(t.leaf_lo, s);
+    (void)leaf_first;
+    nnz = hnf[n_leaves];
+    leaf_lo = to_device(hleaf_lo, s);
+    leaf_box = to_device(hleaf_box, s);
     col_off = to_device(hcol_off, s);
     cols = to_device(hcols, s);
     panel_leaf = to_device(hpl, s);
@@ -273,14 +374,18 @@ struct FmmSystem {
     nf64.alloc((size_t)nnz);
     have32 = need32;
     if (need32) nf32.alloc((size_t)nnz);
-    launch_near_assemble(nl, leaf_lo.p, nf_off.p, col_off.p, cols.p, verts.p, cen.p, nf64.p,
+    launch_near_assemble(n_leaves, leaf_lo.p, nf_off.p, col_off.p, cols.p, verts.p, cen.p, nf64.p,
                          need32 ? nf32.p : nullptr, s);
     inv_diag.alloc(N);
     const double k_e = 1.0 / (4.0 * kPi * kEps0);
-    launch_self_inv_diag(N, verts.p, cen.p, k_e * L0, inv_diag.p, s);
+    for (int p = 0; p < P; ++p) {
+      const int o0 = panel_off[p];
+      launch_self_inv_diag(probs[p]->n, verts.p + (size_t)o0 * 12, cen.p + (size_t)o0 * 3,
+                           k_e * L0s[p], inv_diag.p + o0, s);
+    }
     nt.n = N;
     nt.nnz = (long)nnz;
-    nt.n_leaves = nl;
+    nt.n_leaves = n_leaves;
     nt.leaf_lo = leaf_lo.p;
     nt.nf_off = nf_off.p;
     nt.col_off = col_off.p;
@@ -292,17 +397,22 @@ struct FmmSystem {
     n3p = round_up(n3, 32);
     auto nodes = cheb_nodes(n);
     std::vector<double> hw1d((size_t)N * 3 * n);
-    const double half_L = t.level_half(L);
-    const double origin[3] = {t.center[0] - t.half, t.center[1] - t.half, t.center[2] - t.half};
-    for (int T = 0; T < nl; ++T) {
-      int b = t.leaf_box(T);
-      double bc[3];
-      for (int d = 0; d < 3; ++d) bc[d] = origin[d] + (t.cell[3 * b + d] + 0.5) * 2.0 * half_L;
-      for (int p = t.leaf_lo[T]; p < t.leaf_lo[T + 1]; ++p) {
-        int i = (int)t.perm[p];
-        double a[3] = {g.cx[i], g.cy[i], g.cz[i]};
-        for (int d = 0; d < 3; ++d)
-          interp_weights((a[d] - bc[d]) / half_L, n, nodes.data(), &hw1d[(size_t)p * 3 * n + d * n]);
+    for (int p = 0; p < P; ++p) {
+      const Tree &t = trees[p];
+      const PanelGeom &g = *probs[p];
+      const double half_L = t.level_half(t.L);
+      const double origin[3] = {t.center[0] - t.half, t.center[1] - t.half, t.center[2] - t.half};
+      for (int T = 0; T < t.n_leaves(); ++T) {
+        int b = t.leaf_box(T);
+        double bc[3];
+        for (int d = 0; d < 3; ++d) bc[d] = origin[d] + (t.cell[3 * b + d] + 0.5) * 2.0 * half_L;
+        for (int k = t.leaf_lo[T]; k < t.leaf_lo[T + 1]; ++k) {
+          int i = (int)t.perm[k];
+          double a[3] = {g.cx[i], g.cy[i], g.cz[i]};
+          const size_t row = (size_t)(panel_off[p] + k) * 3 * n;
+          for (int d = 0; d < 3; ++d)
+            interp_weights((a[d] - bc[d]) / half_L, n, nodes.data(), &hw1d[row + d * n]);
+        }
       }
     }
     // M2M 1-D matrices, bbfmm.rs:89-112: mat[m*n + k] = S_n(cc + x_k / 2, x_m).
@@ -339,23 +449,37 @@ struct FmmSystem {
             }
           }
         }
-    int nb = t.n_boxes();
-    std::vector<int> hlevel(nb), htargets;
+    // Box arrays in the global numbering.
+    std::vector<int> hlevel(nb), hparent(nb), hoct(nb), hchild_off(nb + 1, 0), hchild, htargets;
+    std::vector<int> hil_off(nb + 1, 0), hil_src, hil_oidx;
     for (int l = 0; l <= L; ++l)
       for (int b = lvl_off[l]; b < lvl_off[l + 1]; ++b) hlevel[b] = l;
-    for (int b = 0; b < nb; ++b)
-      if (t.il_off[b + 1] > t.il_off[b]) htargets.push_back(b);
-    pairs = (long)t.il_src.size();
+    for (int b = 0; b < nb; ++b) {
+      const int p = gp_of[b], lb = gb_of[b];
+      const Tree &t = trees[p];
+      hparent[b] = t.parent[lb] >= 0 ? gid[p][t.parent[lb]] : -1;
+      hoct[b] = t.octant[lb];
+      for (int q = t.child_off[lb]; q < t.child_off[lb + 1]; ++q) hchild.push_back(gid[p][t.child_idx[q]]);
+      hchild_off[b + 1] = (int)hchild.size();
+      for (int q = t.il_off[lb]; q < t.il_off[lb + 1]; ++q) {
+        hil_src.push_back(gid[p][t.il_src[q]]);
+        hil_oidx.push_back(t.il_oidx[q]);
+      }
+      hil_off[b + 1] = (int)hil_src.size();
+      if (hil_off[b + 1] > hil_off[b]) htargets.push_back(b);
+    }
+    pairs = (long)hil_src.size();
+    // 1/half_l in units of each problem's own root half-width.
     std::vector<double> hih(L + 1);
-    for (int l = 0; l <= L; ++l) hih[l] = (double)(1u << l);  // 1/half_l in units of L0
-    parent = to_device(t.parent, s);
-    octant = to_device(t.octant, s);
-    child_off = to_device(t.child_off, s);
-    child_idx = to_device(t.child_idx, s);
+    for (int l = 0; l <= L; ++l) hih[l] = (double)(1u << l);
+    parent = to_device(hparent, s);
+    octant = to_device(hoct, s);
+    child_off = to_device(hchild_off, s);
+    child_idx = to_device(hchild, s);
     targets = to_device(htargets, s);
-    il_off = to_device(t.il_off, s);
-    il_src = to_device(t.il_src, s);
-    il_oidx = to_device(t.il_oidx, s);
+    il_off = to_device(hil_off, s);
+    il_src = to_device(hil_src, s);
+    il_oidx = to_device(hil_oidx, s);
     box_level = to_device(hlevel, s);
     inv_half = to_device(hih, s);
     w1d64 = to_device(hw1d, s);
@@ -377,6 +501,7 @@ struct FmmSystem {
         Gr32 = to_device(hr, s);
       }
     }
+    const bool batch = P > 1;
     auto fill = [&](auto &f) {
       f.n = n;
       f.L = L;
@@ -385,6 +510,9 @@ struct FmmSystem {
       f.n_pairs = pairs;
       f.n_panels = N;
       f.lvl_off = lvl_off.data();
+      f.npar = batch ? npar.data() : nullptr;
+      f.leaf_box = batch ? leaf_box.p : nullptr;
+      f.n_leaves = n_leaves;
       f.leaf_lo = leaf_lo.p;
       f.parent = parent.p;
       f.octant = octant.p;
@@ -409,8 +537,16 @@ struct FmmSystem {
     f32.G = G32.p;
     f32.Gbf = Gbf.p;
     f32.Gr = Gr32.p;
-    std::vector<int> hco, hci;
-    cond_csr(g, t.perm.data(), hco, hci);
+    // Panels of each (batch-global) conductor, ascending caller order.
+    std::vector<int> hco{0}, hci;
+    for (int p = 0; p < P; ++p) {
+      std::vector<int> co, ci;
+      cond_csr(*probs[p], trees[p].perm.data(), co, ci);
+      for (int c = 0; c < probs[p]->ncond; ++c) {
+        for (int q = co[c]; q < co[c + 1]; ++q) hci.push_back(panel_off[p] + ci[q]);
+        hco.push_back((int)hci.size());
+      }
+    }
     coff = to_device(hco, s);
     cidx = to_device(hci, s);
   }
@@ -419,8 +555,8 @@ struct FmmSystem {
     if (ld_ == ld) return;
     ld = ld_;
     const size_t vec = (size_t)N * ld;
-    const size_t mexp = (size_t)t.n_boxes() * n3p * ld;   // multipoles, padded
-    const size_t lexp = (size_t)t.n_boxes() * n * n * n * ld;
+    const size_t mexp = (size_t)f64.n_boxes * n3p * ld;   // multipoles, padded
+    const size_t lexp = (size_t)f64.n_boxes * n * n * n * ld;
     z64.alloc(vec);
     z64.zero(s);
     M64.alloc(mexp);
@@ -480,46 +616,110 @@ struct GmresStats {
   std::vector<char> conv;
 };
 
-// Restarted GMRES(m) with right Jacobi preconditioning for nr columns of b
-// (device, N x ld) starting from x = 0 (x must be zero on entry), per column
-// exactly the control flow of krylov.rs:50-191: stop a cycle at the residual
-// estimate, check the true residual when the estimate is within 10 tol.
-// Orthogonalisation is classical Gram-Schmidt applied twice (CGS2, passes =
-// 2), the block-friendly equivalent of the reference's modified Gram-Schmidt;
-// the inner solves of the mixed solver (tolerance ~1e-2, corrected by the
-// FP64 refinement) take one pass and keep the basis V in FP32 (TV = float).
-template <typename TV>
-void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, double tol,
-                 int m, int max_restarts, bool true_check, int passes,
-                 const std::vector<int> &live_in, GmresStats &st) {
+// Vector operations of the Krylov drivers over nsys() independent systems.
+// BlockOps: nr right-hand sides of one problem, N x ld row-major, per-system
+// state stride ld.  SegOps: a batch of problems stacked in one column, each
+// system a row segment, per-system state stride nsys.
+struct BlockOps {
+  int N = 0, ld = 0, nr = 0;
+  hipStream_t s = nullptr;
+  int nsys() const { return nr; }
+  int S() const { return ld; }
+  int cols() const { return nr; }
+  size_t vec() const { return (size_t)N * ld; }
+  size_t partial_size(int nv) const { return kr_partial_size(nv, ld); }
+  template <typename TX>
+  void dots(const TX *X, size_t sx, int nv, const double *Y, double *part, double *out,
+            bool sq) const {
+    kr_dots(X, sx, nv, Y, N, ld, nr, part, out, sq, s);
+  }
+  template <typename TV>
+  void axpy_multi(double *w, const TV *V, size_t sv, const double *coef, int nv, double a) const {
+    kr_axpy_multi(w, V, sv, coef, nv, N, ld, nr, a, s);
+  }
+  template <typename TO>
+  void scale(const double *in, TO *out, const double *sc, const int *mask) const {
+    kr_scale_cols(in, out, sc, mask, N, ld, nr, s);
+  }
+  void sub(const double *a, const double *b, double *out) const { kr_sub(a, b, out, N, ld, nr, s); }
+  void axpy(double *x, const double *y, double a) const { kr_axpy(x, y, a, N, ld, nr, s); }
+  template <typename TV>
+  void update_x(double *x, const TV *V, size_t sv, const double *Y, const int *ku, int kmax,
+                const double *dinv) const {
+    kr_gmres_update_x(x, V, sv, Y, ku, kmax, dinv, N, ld, nr, s);
+  }
+};
+
+struct SegOps {
+  SegMap sm;
+  hipStream_t s = nullptr;
+  int nsys() const { return sm.nsys; }
+  int S() const { return sm.nsys; }
+  int cols() const { return 1; }
+  size_t vec() const { return (size_t)sm.N; }
+  size_t partial_size(int nv) const { return seg_partial_size(sm, nv); }
+  template <typename TX>
+  void dots(const TX *X, size_t sx, int nv, const double *Y, double *part, double *out,
+            bool sq) const {
+    seg_dots(sm, X, sx, nv, Y, part, out, sq, s);
+  }
+  template <typename TV>
+  void axpy_multi(double *w, const TV *V, size_t sv, const double *coef, int nv, double a) const {
+    seg_axpy_multi(sm, w, V, sv, coef, nv, a, s);
+  }
+  template <typename TO>
+  void scale(const double *in, TO *out, const double *sc, const int *mask) const {
+    seg_scale(sm, in, out, sc, mask, s);
+  }
+  void sub(const double *a, const double *b, double *out) const { kr_sub(a, b, out, sm.N, 1, 1, s); }
+  void axpy(double *x, const double *y, double a) const { kr_axpy(x, y, a, sm.N, 1, 1, s); }
+  template <typename TV>
+  void update_x(double *x, const TV *V, size_t sv, const double *Y, const int *ku, int kmax,
+                const double *dinv) const {
+    seg_update_x(sm, x, V, sv, Y, ku, kmax, dinv, s);
+  }
+};
+
+// Restarted GMRES(m) with right Jacobi preconditioning for the systems of
+// ops (right-hand sides b, device) starting from x = 0 (x must be zero on
+// entry), per system exactly the control flow of krylov.rs:50-191: stop a
+// cycle at the residual estimate, check the true residual when the estimate
+// is within 10 tol.  Orthogonalisation is classical Gram-Schmidt applied
+// twice (CGS2, passes = 2), the block-friendly equivalent of the reference's
+// modified Gram-Schmidt; the inner solves of the mixed solver (tolerance
+// ~1e-2, corrected by the FP64 refinement) take one pass and keep the basis V
+// in FP32 (TV = float).
+template <typename TV, typename Ops>
+void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double *x, double tol,
+               int m, int max_restarts, bool true_check, int passes,
+               const std::vector<int> &live_in, GmresStats &st) {
   hipStream_t s = sys.s;
-  const int N = sys.N, ld = sys.ld;
-  const size_t vec = (size_t)N * ld;
+  const int nr = ops.nsys(), S = ops.S(), nc = ops.cols();
+  const size_t vec = ops.vec();
   int vcap = std::min(m, 32) + 1;
   DBuf<TV> V(vcap * vec);
   DBuf<double> w(vec), rv(vec), ax(vec);
-  DBuf<double> partial(kr_partial_size(vcap, ld));
-  DBuf<double> h1((size_t)(m + 1) * ld), h2((size_t)(m + 1) * ld), hn(ld);
-  DBuf<double> H((size_t)(m + 1) * m * ld), cs((size_t)m * ld), sn((size_t)m * ld),
-      g((size_t)(m + 1) * ld), Y((size_t)m * ld);
-  DBuf<double> bnorm(ld), beta(ld), inv_beta(ld), est(ld), inv_h(ld);
-  DBuf<int> live(ld), active(ld), kused(ld);
-  Pinned pin(ld * (2 * sizeof(int) + 2 * sizeof(double)));
+  DBuf<double> partial(ops.partial_size(vcap));
+  DBuf<double> h1((size_t)(m + 1) * S), h2((size_t)(m + 1) * S), hn(S);
+  DBuf<double> H((size_t)(m + 1) * m * S), cs((size_t)m * S), sn((size_t)m * S),
+      g((size_t)(m + 1) * S), Y((size_t)m * S);
+  DBuf<double> bnorm(S), beta(S), inv_beta(S), est(S), inv_h(S);
+  DBuf<int> live(S), active(S), kused(S);
+  Pinned pin((size_t)S * (2 * sizeof(int) + 2 * sizeof(double)));
   int *h_active = pin.as<int>();
-  int *h_kused = h_active + ld;
-  double *h_est = reinterpret_cast<double *>(h_kused + ld);
-  double *h_aux = h_est + ld;
+  int *h_kused = h_active + S;
+  double *h_est = reinterpret_cast<double *>(h_kused + S);
+  double *h_aux = h_est + S;
 
   auto ensure_V = [&](int nv) {
     if (nv <= vcap) return;
-    int nc = std::min(m + 1, std::max(nv, 2 * vcap));
-    DBuf<TV> V2((size_t)nc * vec);
-    GP_HIP(hipMemcpyAsync(V2.p, V.p, (size_t)vcap * vec * sizeof(TV),
-                          hipMemcpyDeviceToDevice, s));
+    int ncap = std::min(m + 1, std::max(nv, 2 * vcap));
+    DBuf<TV> V2((size_t)ncap * vec);
+    GP_HIP(hipMemcpyAsync(V2.p, V.p, (size_t)vcap * vec * sizeof(TV), hipMemcpyDeviceToDevice, s));
     GP_HIP(hipStreamSynchronize(s));  // the old basis is freed below
     V = std::move(V2);
-    partial.alloc(kr_partial_size(nc, ld));
-    vcap = nc;
+    partial.alloc(ops.partial_size(ncap));
+    vcap = ncap;
   };
 
   st.iters.assign(nr, 0);
@@ -528,7 +728,7 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
   std::vector<int> done(nr);
   for (int r = 0; r < nr; ++r) done[r] = !live_in[r];
   // bnorm = max(||b||, MIN_POSITIVE)   (krylov.rs:58)
-  kr_dots(b, 0, 1, b, N, ld, nr, partial.p, bnorm.p, true, s);
+  ops.dots(b, 0, 1, b, partial.p, bnorm.p, true);
   std::vector<double> hb(nr);
   GP_HIP(hipMemcpyAsync(hb.data(), bnorm.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
   GP_HIP(hipStreamSynchronize(s));
@@ -541,14 +741,14 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
     if (x_zero) {
       GP_HIP(hipMemcpyAsync(rv.p, b, vec * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
-      sys.apply(x, false, nr, ax.p, prec);
-      kr_sub(b, ax.p, rv.p, N, ld, nr, s);
+      sys.apply(x, false, nc, ax.p, prec);
+      ops.sub(b, ax.p, rv.p);
     }
-    kr_dots(rv.p, 0, 1, rv.p, N, ld, nr, partial.p, beta.p, true, s);
-    std::vector<int> hl(ld, 0);
+    ops.dots(rv.p, 0, 1, rv.p, partial.p, beta.p, true);
+    std::vector<int> hl(S, 0);
     for (int r = 0; r < nr; ++r) hl[r] = !done[r];
-    GP_HIP(hipMemcpyAsync(live.p, hl.data(), ld * sizeof(int), hipMemcpyHostToDevice, s));
-    kr_gmres_start(m, nr, ld, beta.p, bnorm.p, tol, live.p, active.p, kused.p, g.p, inv_beta.p,
+    GP_HIP(hipMemcpyAsync(live.p, hl.data(), S * sizeof(int), hipMemcpyHostToDevice, s));
+    kr_gmres_start(m, nr, S, beta.p, bnorm.p, tol, live.p, active.p, kused.p, g.p, inv_beta.p,
                    est.p, s);
     GP_HIP(hipMemcpyAsync(h_active, active.p, nr * sizeof(int), hipMemcpyDeviceToHost, s));
     GP_HIP(hipMemcpyAsync(h_est, est.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -565,36 +765,33 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
       nact += h_active[r];
     }
     if (nact == 0) break;
-    kr_scale_cols(rv.p, V.p, inv_beta.p, active.p, N, ld, nr, s);
+    ops.scale(rv.p, V.p, inv_beta.p, active.p);
     int ksteps = 0;
     for (int k = 0; k < m; ++k) {
       ensure_V(k + 2);
-      sys.apply(V.p + (size_t)k * sv, true, nr, w.p, prec);
-      kr_dots(V.p, sv, k + 1, w.p, N, ld, nr, partial.p, h1.p, false, s);
-      kr_axpy_multi(w.p, V.p, sv, h1.p, k + 1, N, ld, nr, -1.0, s);
+      sys.apply(V.p + (size_t)k * sv, true, nc, w.p, prec);
+      ops.dots(V.p, sv, k + 1, w.p, partial.p, h1.p, false);
+      ops.axpy_multi(w.p, V.p, sv, h1.p, k + 1, -1.0);
       if (passes > 1) {
-        kr_dots(V.p, sv, k + 1, w.p, N, ld, nr, partial.p, h2.p, false, s);
-        kr_axpy_multi(w.p, V.p, sv, h2.p, k + 1, N, ld, nr, -1.0, s);
+        ops.dots(V.p, sv, k + 1, w.p, partial.p, h2.p, false);
+        ops.axpy_multi(w.p, V.p, sv, h2.p, k + 1, -1.0);
       }
-      kr_dots(w.p, 0, 1, w.p, N, ld, nr, partial.p, hn.p, true, s);
-      kr_gmres_givens(k, m, nr, ld, h1.p, passes > 1 ? h2.p : nullptr, hn.p, H.p, cs.p, sn.p, g.p,
+      ops.dots(w.p, 0, 1, w.p, partial.p, hn.p, true);
+      kr_gmres_givens(k, m, nr, S, h1.p, passes > 1 ? h2.p : nullptr, hn.p, H.p, cs.p, sn.p, g.p,
                       bnorm.p, tol, active.p, kused.p, est.p, inv_h.p, s);
-      kr_scale_cols(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, nullptr, N, ld, nr, s);
+      ops.scale(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, nullptr);
       ksteps = k + 1;
       GP_HIP(hipMemcpyAsync(h_active, active.p, nr * sizeof(int), hipMemcpyDeviceToHost, s));
       GP_HIP(hipStreamSynchronize(s));
-      int any = 0;
-      for (int r = 0; r < nr; ++r) any |= h_active[r];
-      if (k % 25 == 24) {
-        int na = 0;
-        for (int r = 0; r < nr; ++r) na += h_active[r];
-        GP_LOG("gmres prec %d cycle %d iteration %d: %d of %d columns active", prec, cycle, k + 1,
+      int na = 0;
+      for (int r = 0; r < nr; ++r) na += h_active[r];
+      if (k % 25 == 24)
+        GP_LOG("gmres prec %d cycle %d iteration %d: %d of %d systems active", prec, cycle, k + 1,
                na, nr);
-      }
-      if (!any) break;
+      if (!na) break;
     }
-    kr_gmres_backsolve(m, nr, ld, H.p, g.p, kused.p, Y.p, s);
-    kr_gmres_update_x(x, V.p, sv, Y.p, kused.p, ksteps, sys.inv_diag.p, N, ld, nr, s);
+    kr_gmres_backsolve(m, nr, S, H.p, g.p, kused.p, Y.p, s);
+    ops.update_x(x, V.p, sv, Y.p, kused.p, ksteps, sys.inv_diag.p);
     x_zero = false;
     GP_HIP(hipMemcpyAsync(h_kused, kused.p, nr * sizeof(int), hipMemcpyDeviceToHost, s));
     GP_HIP(hipMemcpyAsync(h_est, est.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -607,9 +804,9 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
       }
     if (true_check) {
       if (check) {
-        sys.apply(x, false, nr, ax.p, prec);
-        kr_sub(b, ax.p, rv.p, N, ld, nr, s);
-        kr_dots(rv.p, 0, 1, rv.p, N, ld, nr, partial.p, beta.p, true, s);
+        sys.apply(x, false, nc, ax.p, prec);
+        ops.sub(b, ax.p, rv.p);
+        ops.dots(rv.p, 0, 1, rv.p, partial.p, beta.p, true);
         GP_HIP(hipMemcpyAsync(h_aux, beta.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
         GP_HIP(hipStreamSynchronize(s));
         for (int r = 0; r < nr; ++r)
@@ -634,17 +831,90 @@ void gmres_block(FmmSystem &sys, int prec, const double *b, double *x, int nr, d
     for (int r = 0; r < nr; ++r) all &= (bool)done[r];
     if (all) break;
   }
-  // Columns left unconverged: report their true residual (krylov.rs:185-190).
+  // Systems left unconverged: report their true residual (krylov.rs:185-190).
   bool any_left = false;
   for (int r = 0; r < nr; ++r) any_left |= !done[r];
   if (any_left) {
-    sys.apply(x, false, nr, ax.p, prec);
-    kr_sub(b, ax.p, rv.p, N, ld, nr, s);
-    kr_dots(rv.p, 0, 1, rv.p, N, ld, nr, partial.p, beta.p, true, s);
+    sys.apply(x, false, nc, ax.p, prec);
+    ops.sub(b, ax.p, rv.p);
+    ops.dots(rv.p, 0, 1, rv.p, partial.p, beta.p, true);
     GP_HIP(hipMemcpyAsync(h_aux, beta.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
     GP_HIP(hipStreamSynchronize(s));
     for (int r = 0; r < nr; ++r)
       if (!done[r]) st.resid[r] = h_aux[r] / hb[r];
+  }
+}
+
+// Per-system outcome of one solve of the systems of ops.
+struct SolveStats {
+  std::vector<int> iters;       // GMRES iterations (summed over refinement steps)
+  std::vector<double> resid;    // final relative residual
+  std::vector<char> conv;
+  int refine_steps = 0;
+};
+
+// Solves sys x = b for the live systems of ops: FP64 GMRES, or (mixed) FP64
+// iterative refinement around the FP32 inner GMRES.  x must be zero on entry.
+template <typename Ops>
+void solve_systems(FmmSystem &sys, const Ops &ops, const gp_cap_opts &o, bool mixed,
+                   const double *b, double *x, const std::vector<int> &live, SolveStats &out) {
+  hipStream_t s = sys.s;
+  const int nr = ops.nsys(), S = ops.S(), nc = ops.cols();
+  const size_t vec = ops.vec();
+  const int m = std::max(1, (int)o.restart), mr = std::max(1, (int)o.max_restarts);
+  out.iters.assign(nr, 0);
+  out.resid.assign(nr, 0.0);
+  out.conv.assign(nr, 0);
+  out.refine_steps = 0;
+  if (!mixed) {
+    GmresStats st;
+    gmres_run<double>(sys, ops, 0, b, x, o.tol, m, mr, true, 2, live, st);
+    out.iters = st.iters;
+    out.resid = st.resid;
+    out.conv = st.conv;
+    return;
+  }
+  // FP64 iterative refinement around the FP32 inner GMRES.
+  DBuf<double> r(vec), dx(vec), ax(vec), part(ops.partial_size(1)), rn(S);
+  GP_HIP(hipMemcpyAsync(r.p, b, vec * sizeof(double), hipMemcpyDeviceToDevice, s));
+  std::vector<double> hbn(nr), hrn(nr);
+  ops.dots(b, 0, 1, b, part.p, rn.p, true);
+  GP_HIP(hipMemcpyAsync(hbn.data(), rn.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  for (auto &v : hbn) v = std::max(v, kMinPositive);
+  std::vector<int> done(nr, 0);
+  for (int q = 0; q < nr; ++q) done[q] = !live[q];
+  const int max_ref = std::max(1, (int)o.max_refine);
+  const int passes = o.inner_passes >= 2 ? 2 : 1;
+  for (;;) {
+    ops.dots(r.p, 0, 1, r.p, part.p, rn.p, true);
+    GP_HIP(hipMemcpyAsync(hrn.data(), rn.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int q = 0; q < nr; ++q) {
+      if (live[q] && hrn[q] / hbn[q] <= o.tol) done[q] = 1;
+      all &= (bool)done[q];
+    }
+    if (all || out.refine_steps >= max_ref) break;
+    std::vector<int> lv(nr);
+    for (int q = 0; q < nr; ++q) lv[q] = !done[q];
+    dx.zero(s);
+    GmresStats st;
+    if (o.inner_basis == GP_BASIS_FP64)
+      gmres_run<double>(sys, ops, 1, r.p, dx.p, o.inner_tol, m, mr, false, passes, lv, st);
+    else
+      gmres_run<float>(sys, ops, 1, r.p, dx.p, o.inner_tol, m, mr, false, passes, lv, st);
+    for (int q = 0; q < nr; ++q) out.iters[q] += st.iters[q];
+    ops.axpy(x, dx.p, 1.0);
+    sys.apply(x, false, nc, ax.p, 0);
+    ops.sub(b, ax.p, r.p);
+    ++out.refine_steps;
+    GP_LOG("refinement step %d: inner iterations up to %d", out.refine_steps,
+           *std::max_element(st.iters.begin(), st.iters.end()));
+  }
+  for (int q = 0; q < nr; ++q) {
+    out.resid[q] = live[q] ? hrn[q] / hbn[q] : 0.0;
+    out.conv[q] = live[q] ? (char)done[q] : 1;
   }
 }
 
@@ -667,73 +937,30 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   GP_HIP(hipStreamSynchronize(s));
   double t1 = now_s();
   GP_LOG("setup %.3f s: %d panels, depth %d, %d boxes, near nnz %lld, %lld M2L pairs, ld %d",
-         t1 - t0, sys.N, sys.L, sys.t.n_boxes(), (long long)sys.nnz, (long long)sys.pairs, ld);
+         t1 - t0, sys.N, sys.L, sys.f64.n_boxes, (long long)sys.nnz, (long long)sys.pairs, ld);
 
   const size_t vec = (size_t)sys.N * ld;
   DBuf<double> b(vec), x(vec), dCb((size_t)nc * ld);
-  DBuf<double> r(vec), dx(vec), ax(vec), part(kr_partial_size(1, ld)), rn(ld);
   std::vector<double> hC((size_t)nc * nc, 0.0), hblk((size_t)nc * ld);
   int max_iters = 0, refine_steps = 0;
   double worst = 0.0;
   bool conv_all = true;
-  const int m = std::max(1, (int)o.restart), mr = std::max(1, (int)o.max_restarts);
   for (int j0 = jlo; j0 < jhi; j0 += blk) {
     const int nr = std::min(blk, jhi - j0);
     kr_unit_rhs(sys.cond.p, j0, sys.N, ld, nr, b.p, s);
     x.zero(s);
-    std::vector<int> live(nr, 1);
-    if (!mixed) {
-      GmresStats st;
-      gmres_block<double>(sys, 0, b.p, x.p, nr, o.tol, m, mr, true, 2, live, st);
-      for (int q = 0; q < nr; ++q) {
-        max_iters = std::max(max_iters, st.iters[q]);
-        worst = std::max(worst, st.resid[q]);
-        conv_all &= (bool)st.conv[q];
-      }
-    } else {
-      // FP64 iterative refinement around the FP32 inner GMRES.
-      GP_HIP(hipMemcpyAsync(r.p, b.p, vec * sizeof(double), hipMemcpyDeviceToDevice, s));
-      std::vector<double> hbn(nr), hrn(nr);
-      kr_dots(b.p, 0, 1, b.p, sys.N, ld, nr, part.p, rn.p, true, s);
-      GP_HIP(hipMemcpyAsync(hbn.data(), rn.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
-      GP_HIP(hipStreamSynchronize(s));
-      for (auto &v : hbn) v = std::max(v, kMinPositive);
-      std::vector<int> it_sum(nr, 0), done(nr, 0);
-      int steps = 0;
-      const int max_ref = std::max(1, (int)o.max_refine);
-      const int passes = o.inner_passes >= 2 ? 2 : 1;
-      for (;;) {
-        kr_dots(r.p, 0, 1, r.p, sys.N, ld, nr, part.p, rn.p, true, s);
-        GP_HIP(hipMemcpyAsync(hrn.data(), rn.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
-        GP_HIP(hipStreamSynchronize(s));
-        bool all = true;
-        for (int q = 0; q < nr; ++q) {
-          if (hrn[q] / hbn[q] <= o.tol) done[q] = 1;
-          all &= (bool)done[q];
-        }
-        if (all || steps >= max_ref) break;
-        std::vector<int> lv(nr);
-        for (int q = 0; q < nr; ++q) lv[q] = !done[q];
-        dx.zero(s);
-        GmresStats st;
-        if (o.inner_basis == GP_BASIS_FP64)
-          gmres_block<double>(sys, 1, r.p, dx.p, nr, o.inner_tol, m, mr, false, passes, lv, st);
-        else
-          gmres_block<float>(sys, 1, r.p, dx.p, nr, o.inner_tol, m, mr, false, passes, lv, st);
-        for (int q = 0; q < nr; ++q) it_sum[q] += st.iters[q];
-        kr_axpy(x.p, dx.p, 1.0, sys.N, ld, nr, s);
-        sys.apply(x.p, false, nr, ax.p, 0);
-        kr_sub(b.p, ax.p, r.p, sys.N, ld, nr, s);
-        ++steps;
-        GP_LOG("refinement step %d: inner iterations up to %d", steps,
-               *std::max_element(st.iters.begin(), st.iters.end()));
-      }
-      refine_steps = std::max(refine_steps, steps);
-      for (int q = 0; q < nr; ++q) {
-        max_iters = std::max(max_iters, it_sum[q]);
-        worst = std::max(worst, hrn[q] / hbn[q]);
-        conv_all &= (bool)done[q];
-      }
+    BlockOps ops;
+    ops.N = sys.N;
+    ops.ld = ld;
+    ops.nr = nr;
+    ops.s = s;
+    SolveStats st;
+    solve_systems(sys, ops, o, mixed, b.p, x.p, std::vector<int>(nr, 1), st);
+    refine_steps = std::max(refine_steps, st.refine_steps);
+    for (int q = 0; q < nr; ++q) {
+      max_iters = std::max(max_iters, st.iters[q]);
+      worst = std::max(worst, st.resid[q]);
+      conv_all &= (bool)st.conv[q];
     }
     launch_cap_accumulate(nc, nr, ld, sys.coff.p, sys.cidx.p, sys.area.p, x.p, dCb.p, s);
     GP_HIP(hipMemcpyAsync(hblk.data(), dCb.p, (size_t)nc * nr * sizeof(double),
@@ -742,7 +969,7 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
     for (int i = 0; i < nc; ++i)
       for (int q = 0; q < nr; ++q) hC[(size_t)i * nc + j0 + q] = hblk[(size_t)i * nr + q];
   }
-  const double scale = 4.0 * kPi * kEps0 * sys.L0;
+  const double scale = 4.0 * kPi * kEps0 * sys.L0();
   for (size_t e = 0; e < hC.size(); ++e) C[e] = hC[e] * scale;
   double t2 = now_s();
   if (rep) {
@@ -753,8 +980,8 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
     rep->refine_steps = (uint32_t)refine_steps;
     rep->converged = conv_all ? 1u : 0u;
     rep->max_level = (uint32_t)sys.L;
-    rep->n_leaves = (uint32_t)sys.t.n_leaves();
-    rep->n_boxes = (uint32_t)sys.t.n_boxes();
+    rep->n_leaves = (uint32_t)sys.n_leaves;
+    rep->n_boxes = (uint32_t)sys.f64.n_boxes;
     rep->fmm_order = (uint32_t)sys.n;
     rep->near_nnz = (uint64_t)sys.nnz;
     rep->m2l_pairs = (uint64_t)sys.pairs;
@@ -765,6 +992,112 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   if (!conv_all && o.strict)
     throw Error(GP_ERR_NOT_CONVERGED, "GMRES did not reach tol (worst relative residual " +
                                           std::to_string(worst) + ")");
+}
+
+// A batch of independent FMM problems solved together: one stacked,
+// block-diagonal operator (every kernel launch covers all problems) and one
+// GMRES per problem running in lockstep with per-problem reductions.  Each
+// right-hand-side column j of [rhs_first, rhs_first + rhs_count) is one pass
+// over the batch (problems with fewer conductors sit it out).
+void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
+                     const gp_cap_opts &o, const std::vector<double *> &C, gp_cap_report *reps,
+                     double t_start) {
+  hipStream_t s = ctx->stream;
+  const bool mixed = o.precision == GP_PREC_MIXED;
+  const int P = (int)probs.size();
+  FmmSystem sys;
+  sys.setup(probs, o, mixed, s);
+  sys.ensure_work(1);
+  // Row segments and the reduction block map (at most 2048 rows a block).
+  std::vector<int> hseg(sys.N), blo, bhi, sblk{0};
+  for (int p = 0; p < P; ++p) {
+    const int r0 = sys.panel_off[p], r1 = r0 + probs[p]->n;
+    for (int r = r0; r < r1; ++r) hseg[r] = p;
+    for (int r = r0; r < r1; r += 2048) {
+      blo.push_back(r);
+      bhi.push_back(std::min(r1, r + 2048));
+    }
+    sblk.push_back((int)blo.size());
+  }
+  DBuf<int> dseg = to_device(hseg, s), dblo = to_device(blo, s), dbhi = to_device(bhi, s),
+            dsblk = to_device(sblk, s);
+  SegOps ops;
+  ops.sm.N = sys.N;
+  ops.sm.nsys = P;
+  ops.sm.nblk = (int)blo.size();
+  ops.sm.seg = dseg.p;
+  ops.sm.blk_lo = dblo.p;
+  ops.sm.blk_hi = dbhi.p;
+  ops.sm.sys_blk = dsblk.p;
+  ops.s = s;
+  GP_HIP(hipStreamSynchronize(s));
+  const double t1 = now_s();
+  GP_LOG("batch setup %.3f s: %d problems, %d panels, max depth %d, %d boxes, near nnz %lld, "
+         "%lld M2L pairs", t1 - t_start, P, sys.N, sys.L, sys.f64.n_boxes, (long long)sys.nnz,
+         (long long)sys.pairs);
+
+  int maxc = 0;
+  for (auto *g : probs) maxc = std::max(maxc, g->ncond);
+  const int jlo = std::min((int)o.rhs_first, maxc);
+  const int jhi = o.rhs_count ? std::min(maxc, jlo + (int)o.rhs_count) : maxc;
+  DBuf<double> b(sys.N), x(sys.N), dC(sys.ncond);
+  DBuf<int> drhs(P);
+  std::vector<double> hdC(sys.ncond);
+  std::vector<int> iters(P, 0), refine(P, 0), conv(P, 1);
+  std::vector<double> resid(P, 0.0);
+  for (int p = 0; p < P; ++p)
+    std::fill(C[p], C[p] + (size_t)probs[p]->ncond * probs[p]->ncond, 0.0);
+  for (int j = jlo; j < jhi; ++j) {
+    std::vector<int> hr(P), live(P);
+    for (int p = 0; p < P; ++p) {
+      live[p] = j < probs[p]->ncond;
+      hr[p] = live[p] ? sys.cond_off[p] + j : -1;
+    }
+    GP_HIP(hipMemcpyAsync(drhs.p, hr.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
+    seg_unit_rhs(ops.sm, sys.cond.p, drhs.p, b.p, s);
+    x.zero(s);
+    SolveStats st;
+    solve_systems(sys, ops, o, mixed, b.p, x.p, live, st);
+    launch_cap_accumulate(sys.ncond, 1, 1, sys.coff.p, sys.cidx.p, sys.area.p, x.p, dC.p, s);
+    GP_HIP(hipMemcpyAsync(hdC.data(), dC.p, sys.ncond * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    for (int p = 0; p < P; ++p) {
+      if (!live[p]) continue;
+      const int nc = probs[p]->ncond;
+      const double scale = 4.0 * kPi * kEps0 * sys.L0s[p];
+      for (int i = 0; i < nc; ++i) C[p][(size_t)i * nc + j] = hdC[sys.cond_off[p] + i] * scale;
+      iters[p] = std::max(iters[p], st.iters[p]);
+      refine[p] = std::max(refine[p], st.refine_steps);
+      resid[p] = std::max(resid[p], st.resid[p]);
+      conv[p] &= (int)st.conv[p];
+    }
+  }
+  const double t2 = now_s();
+  if (reps)
+    for (int p = 0; p < P; ++p) {
+      gp_cap_report &r = reps[p];
+      r.method_used = GP_METHOD_FMM;
+      r.precision_used = mixed ? GP_PREC_MIXED : GP_PREC_FP64;
+      r.iterations = (uint32_t)iters[p];
+      r.matvecs = (uint32_t)sys.matvecs;
+      r.refine_steps = (uint32_t)refine[p];
+      r.converged = (uint32_t)conv[p];
+      r.max_level = (uint32_t)sys.trees[p].L;
+      r.n_leaves = (uint32_t)sys.trees[p].n_leaves();
+      r.n_boxes = (uint32_t)sys.trees[p].n_boxes();
+      r.fmm_order = (uint32_t)sys.n;
+      r.near_nnz = (uint64_t)sys.nnz;
+      r.m2l_pairs = (uint64_t)sys.pairs;
+      r.final_rel_residual = resid[p];
+      r.seconds_setup = t1 - t_start;
+      r.seconds_solve = t2 - t1;
+    }
+  if (o.strict)
+    for (int p = 0; p < P; ++p)
+      if (!conv[p])
+        throw Error(GP_ERR_NOT_CONVERGED, "GMRES did not reach tol for problem " +
+                                              std::to_string(p) + " (relative residual " +
+                                              std::to_string(resid[p]) + ")");
 }
 
 }  // namespace
@@ -797,6 +1130,53 @@ void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   if (rep) rep->seconds_total = now_s() - t0;
 }
 
+void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_cap_opts &o,
+                       const std::vector<double *> &C, gp_cap_report *reps) {
+  const double t0 = now_s();
+  if (!(o.tol > 0.0) || !std::isfinite(o.tol)) throw Error(GP_ERR_INVALID_INPUT, "tol must be > 0");
+  if (o.precision != GP_PREC_FP64 && o.precision != GP_PREC_MIXED)
+    throw Error(GP_ERR_INVALID_INPUT, "unknown precision");
+  if (o.method != GP_METHOD_AUTO && o.method != GP_METHOD_DENSE && o.method != GP_METHOD_FMM)
+    throw Error(GP_ERR_INVALID_INPUT, "unknown method");
+  const uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
+  std::vector<const PanelGeom *> fmm;
+  std::vector<double *> Cf;
+  std::vector<int> fidx;
+  const int P = (int)gs.size();
+  if (reps) std::memset(reps, 0, sizeof(gp_cap_report) * P);
+  for (int p = 0; p < P; ++p) {
+    const PanelGeom &g = gs[p];
+    if (reps) {
+      reps[p].n_panels = (uint32_t)g.n;
+      reps[p].n_conductors = (uint32_t)g.ncond;
+    }
+    const bool dense = o.method == GP_METHOD_DENSE ||
+                       (o.method == GP_METHOD_AUTO && (uint32_t)g.n < limit);
+    if (dense) {
+      solve_dense(ctx, g, C[p], reps ? &reps[p] : nullptr, nullptr);
+    } else {
+      fmm.push_back(&g);
+      Cf.push_back(C[p]);
+      fidx.push_back(p);
+    }
+  }
+  if (!fmm.empty()) {
+    std::vector<gp_cap_report> fr(fmm.size());
+    for (size_t q = 0; q < fmm.size(); ++q) fr[q] = reps ? reps[fidx[q]] : gp_cap_report{};
+    solve_fmm_batch(ctx, fmm, o, Cf, fr.data(), now_s());
+    if (reps)
+      for (size_t q = 0; q < fmm.size(); ++q) reps[fidx[q]] = fr[q];
+  }
+  const double t1 = now_s();
+  for (int p = 0; p < P; ++p) {
+    for (size_t e = 0; e < (size_t)gs[p].ncond * gs[p].ncond; ++e)
+      if (!std::isfinite(C[p][e]))
+        throw Error(GP_ERR_NON_FINITE, "solver returned a non-finite capacitance for problem " +
+                                           std::to_string(p));
+    if (reps) reps[p].seconds_total = t1 - t0;
+  }
+}
+
 void fmm_apply_host(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, int nr,
                     const double *X, double *Y, int precision) {
   hipStream_t s = ctx->stream;
@@ -807,16 +1187,16 @@ void fmm_apply_host(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, int n
   const int N = g.n;
   std::vector<double> hx((size_t)N * ld, 0.0), hy((size_t)N * ld);
   for (int p = 0; p < N; ++p)
-    for (int r = 0; r < nr; ++r) hx[(size_t)p * ld + r] = X[(size_t)sys.t.perm[p] * nr + r];
+    for (int r = 0; r < nr; ++r) hx[(size_t)p * ld + r] = X[(size_t)sys.t().perm[p] * nr + r];
   auto dxv = to_device(hx, s);
   DBuf<double> dy((size_t)N * ld);
   sys.apply(dxv.p, false, nr, dy.p, precision);
   GP_HIP(hipMemcpyAsync(hy.data(), dy.p, hy.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   GP_HIP(hipStreamSynchronize(s));
   // back to SI: P = k_e L0 P'
-  const double sc = sys.L0 / (4.0 * kPi * kEps0);
+  const double sc = sys.L0() / (4.0 * kPi * kEps0);
   for (int p = 0; p < N; ++p)
-    for (int r = 0; r < nr; ++r) Y[(size_t)sys.t.perm[p] * nr + r] = hy[(size_t)p * ld + r] * sc;
+    for (int r = 0; r < nr; ++r) Y[(size_t)sys.t().perm[p] * nr + r] = hy[(size_t)p * ld + r] * sc;
 }
 
 }  // namespace gp

@@ -30,6 +30,13 @@ namespace gp {
 void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, double *C,
                gp_cap_report *rep, const double *eps_panel);
 
+// Maxwell C of each independent problem of a batch, C[p] ncond_p x ncond_p.
+// Problems below the dense limit (method AUTO) or all (DENSE) take the dense
+// path one by one; the FMM problems are solved together (stacked operator,
+// lockstep per-problem GMRES).  reps: one per problem, or null.
+void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_cap_opts &opts,
+                       const std::vector<double *> &C, gp_cap_report *reps);
+
 // Apply the precorrected FMM operator (FP64, no preconditioner) to nr
 // columns X (n x nr, caller order), y = P X, farads^-1 m^-2 units of P.
 // Test entry used to compare the device operator with the oracle's.

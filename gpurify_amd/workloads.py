@@ -124,6 +124,53 @@ def routing_block(size=200000, layers=("met1", "met2", "met3"), wmin=140, wmax=1
     return sort_groups(groups), dens
 
 
+def net_batch(n_nets=4096, seed=4, kmax=12, widths=(140, 200, 300, 500), lmin=1000,
+              lmax=10000, pmin=200, pmax=20000):
+    """Independent nets for the batched per-net solve.  Net i: K ~ U{1..kmax}
+    Manhattan segments as a random walk alternating x (met1) and y (met2),
+    widths drawn from `widths` nm, lengths U[lmin, lmax] nm, and a target
+    panel count log-uniform in [pmin, pmax].  Returns [(groups, target)],
+    each net alone as net 0 (its substrate plate is conductor 1)."""
+    st = sky130_stack()
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_nets):
+        k = int(rng.integers(1, kmax + 1))
+        x, y = 0, 0
+        rects = {"met1": [], "met2": []}
+        horiz = bool(rng.integers(0, 2))
+        for _s in range(k):
+            w = int(rng.choice(widths))
+            ln = int(rng.integers(lmin, lmax + 1)) * (1 if rng.integers(0, 2) else -1)
+            if horiz:
+                x1 = x + ln
+                rects["met1"].append([min(x, x1), y - w // 2, max(x, x1), y - w // 2 + w])
+                x = x1
+            else:
+                y1 = y + ln
+                rects["met2"].append([x - w // 2, min(y, y1), x - w // 2 + w, max(y, y1)])
+                y = y1
+            horiz = not horiz
+        target = float(np.exp(rng.uniform(np.log(pmin), np.log(pmax))))
+        groups = [_group(0, layer, rs, st) for layer, rs in rects.items() if rs]
+        out.append((sort_groups(groups), target))
+    return out
+
+
+def net_batch_meshes(batch, mesh):
+    """Mesh every net of net_batch at the refinement r >= 1 whose panel count
+    is closest to its target (mesh = the bridge mesher, mesh(groups, refine)
+    -> (verts, cond, ...)).  Returns [(verts, cond, r)]."""
+    out = []
+    for groups, target in batch:
+        v1, c1 = mesh(groups, 1)[:2]
+        r = max(1, int(round(np.sqrt(target / len(v1)))))
+        if r > 1:
+            v1, c1 = mesh(groups, r)[:2]
+        out.append((v1, c1, r))
+    return out
+
+
 WORKLOADS = {
     "two_wires_met1": two_wires,
     "bus64x32": bus_crossing,

@@ -136,6 +136,19 @@ typedef struct {
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,
                  gp_cap_report *report);
 
+/* The per-net solves of one layout in one call: what the reference runs as
+ * one solve_capacitance (bridge.rs:307-443 -> cap::solve /
+ * cap::fmm_solver::solve) per net.  Each problem is independent (its own
+ * conductors, e.g. one net and its substrate plate); problems below
+ * dense_limit (GP_METHOD_AUTO) take the dense path, all others are solved
+ * together as one stacked block-diagonal FMM operator with one GMRES per
+ * problem in lockstep.  Columns [rhs_first, rhs_first + rhs_count) of each
+ * problem's C are solved (FMM problems; dense ones solve every column).
+ * C: the problems' n_conductors^2 blocks back to back, farads, vacuum.
+ * reports: n_problems entries or NULL (times are those of the whole call). */
+int gp_cap_solve_batched(gp_ctx *ctx, const gp_panels *problems, uint32_t n_problems,
+                         const gp_cap_opts *opts, double *C, gp_cap_report *reports);
+
 /* The precorrected FMM operator apply y = P x of FmmCapOperator::apply
  * (cap/fmm_solver.rs:105-130): exact-Wilton near tiles plus the far field of
  * the panel charges area * x, built as gp_cap_solve builds it for

@@ -1,0 +1,65 @@
+"""Batched per-net solves (gp_cap_solve_batched) on the device: every
+problem of a batch must come out as its own single solve does, in FP64 and
+mixed precision, with the reference's dense/FMM rule per problem."""
+import numpy as np
+import pytest
+
+import gpurify_amd as G
+from gpurify_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(n, pmax, seed=11):
+    batch = W.net_batch(n_nets=n, seed=seed, pmin=150, pmax=pmax)
+    meshes = W.net_batch_meshes(batch, lambda g, r: G.mesh_groups(g, refine=r, max_panels=10 ** 7))
+    return [(v, c, int(c.max()) + 1) for v, c, _ in meshes]
+
+
+@pytest.fixture(scope="module")
+def probs():
+    ps = small_batch(10, 4000)
+    assert sum(len(v) >= 500 for v, _, _ in ps) >= 5 and sum(len(v) < 500 for v, _, _ in ps) >= 1
+    return ps
+
+
+def test_batched_fmm_equals_single_solves(ctx, probs):
+    o = G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11, rhs_count=1, max_level=0,
+                       leaf_target=32)
+    Cs, reps = ctx.cap_solve_batched(probs, o)
+    for (v, c, nc), Cb, rep in zip(probs, Cs, reps):
+        C1, r1 = ctx.cap_solve(v, c, nc, o)
+        assert rep["converged"] == 1 and rep["method_used"] == G.GP_METHOD_FMM
+        assert rep["max_level"] == r1["max_level"] and rep["n_boxes"] == r1["n_boxes"]
+        # same tree, same operator: the solutions agree to the solve tolerance
+        np.testing.assert_allclose(Cb[:, 0], C1[:, 0], rtol=1e-9)
+        assert np.all(Cb[:, 1:] == 0.0)  # columns outside rhs_count stay zero
+
+
+def test_batched_mixed_matches_fp64_within_1e6(ctx, probs):
+    common = dict(method=G.GP_METHOD_FMM, tol=1e-10, rhs_count=1, max_level=0, leaf_target=32)
+    C64, _ = ctx.cap_solve_batched(probs, G.default_opts(**common))
+    Cm, reps = ctx.cap_solve_batched(probs, G.default_opts(precision=G.GP_PREC_MIXED, **common))
+    for a, b, rep in zip(Cm, C64, reps):
+        assert rep["converged"] == 1 and rep["refine_steps"] >= 1
+        assert np.max(np.abs(a[:, 0] - b[:, 0]) / np.abs(b[:, 0])) < 1e-6
+
+
+def test_batched_auto_rule_and_all_columns(ctx, probs):
+    # AUTO: problems below 500 panels go dense (bridge.rs:397-404), as in gp_cap_solve
+    o = G.default_opts(tol=1e-11)
+    Cs, reps = ctx.cap_solve_batched(probs, o)
+    for (v, c, nc), Cb, rep in zip(probs, Cs, reps):
+        want = G.GP_METHOD_DENSE if len(v) < 500 else G.GP_METHOD_FMM
+        assert rep["method_used"] == want
+        C1, _ = ctx.cap_solve(v, c, nc, o)
+        np.testing.assert_allclose(Cb, C1, rtol=1e-8, atol=1e-9 * np.abs(C1).max())
+
+
+def test_batched_rejects_bad_problem(ctx, probs):
+    bad = list(probs[:3])
+    v, c, nc = bad[1]
+    bad[1] = (v, c, nc - 1)  # conductor id out of range
+    with pytest.raises(G.GpError) as ei:
+        ctx.cap_solve_batched(bad)
+    assert ei.value.code == G._lib.GP_ERR_INVALID_INPUT
