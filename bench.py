@@ -89,6 +89,9 @@ WORKLOADS = {
     # batch: 4096 independent nets, each with its own substrate plate, 1 RHS
     "nets4096": {"groups": None, "refine": 1, "order": 4, "rhs": 1, "name": "nets4096_met1met2",
                  "cpu": None},
+    # adjoint sensitivities of the 16 x 16 crossing bundle (W = S = 140 nm)
+    "bundle16": {"groups": None, "refine": 5, "order": 5, "rhs": None, "name": "bundle16x16_met1met2",
+                 "cpu": None},
 }
 
 
@@ -100,6 +103,8 @@ def main():
         return 2
     if args.workload == "nets4096":
         return bench_batch(args)
+    if args.workload == "bundle16":
+        return bench_adjoint(args)
     import numpy as np
 
     import gpurify_amd as G
@@ -275,6 +280,88 @@ def bench_batch(args):
             out["gate_1e-6_pass"] = bool(rel.max() <= 1e-6)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_batch(args, probs)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_adjoint(args):
+    """Sensitivities of the 16 met1 x 16 met2 crossing bundle (W = S = 140 nm,
+    L = 20 um), meshed at refine 5 (101,625 panels, 33 conductors): dC/d(width,
+    spacing, height) of every wire by one forward and one adjoint solve per
+    conductor (mixed precision, FMM order 5) plus the forward-mode integral
+    derivatives; dC/dk = C/k in closed form.  value = (33 forward + 33
+    adjoint) x N unknowns / wall time of one gp_cap_jacobian call.  Beside it:
+    the FP64 Jacobian, and central differences (h = 1e-3 nm, FP64 FMM solves)
+    on a few parameters."""
+    import numpy as np
+
+    import gpurify_amd as G
+    from gpurify_amd import sensitivity as S
+    from gpurify_amd import workloads as W
+
+    v, c, _ = G.mesh_groups(W.crossing_bundle(), refine=args.refine, max_panels=10 ** 8)
+    n, nc = len(v), int(c.max()) + 1
+    params, names = S.wire_params(v, c, list(range(nc - 1)))
+    log(f"workload bundle16 refine={args.refine}: {n} panels, {nc} conductors, {len(params)} parameters")
+    common = dict(method=G.GP_METHOD_FMM, fmm_order=args.order, max_level=0,
+                  leaf_target=args.leaf_target, tol=args.tol)
+    o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol, **common)
+    o_64 = G.default_opts(**common)
+    out = {"metric": "converged BEM unknowns/s, mixed vs FP64 GPU @<=1e-6 rel C error",
+           "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16+fp32/fp64", "data": "synthetic (seeded sky130 layout bundle16)"}
+    with G.Context(0) as ctx:
+        log("device:", ctx.info())
+        for w in range(args.warmup):
+            ctx.cap_jacobian(v, c, nc, params, o_mix)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            Cm, Jm, rep = ctx.cap_jacobian(v, c, nc, params, o_mix)
+            log(f"step {k}: {time.perf_counter() - t0:.3f}s cumulative, it={rep['iterations']}")
+        T = time.perf_counter() - t0
+        out.update(value=2 * nc * n * args.steps / T, ms_per_step=1e3 * T / args.steps)
+        out["config"] = {"workload": f"bundle16x16_met1met2_refine{args.refine}", "panels": n,
+                         "conductors": nc, "parameters": len(params),
+                         "parameter_kinds": list(S.KINDS), "fmm_order": args.order,
+                         "octree_depth": rep["max_level"], "tol": args.tol,
+                         "solves": f"{nc} forward + {nc} adjoint",
+                         "iterations": rep["iterations"], "refine_steps": rep["refine_steps"],
+                         "converged": bool(rep["converged"]), "global_batch": nc}
+        ctx.profile(True)
+        ctx.cap_jacobian(v, c, nc, params, o_mix)
+        prof = ctx.profile_report()
+        ctx.profile(False)
+        kern = sorted(prof.items(), key=lambda kv: -kv[1]["seconds"])
+        out["kernels"] = {k: {"launches": v_["launches"], "ms_total": round(1e3 * v_["seconds"], 3),
+                              "ms_avg": round(1e3 * v_["seconds"] / v_["launches"], 4)}
+                          for k, v_ in kern[:10]}
+        out["roofline"] = roofline(kern)
+        if not args.no_fp64:
+            t = time.perf_counter()
+            C64, J64, r64 = ctx.cap_jacobian(v, c, nc, params, o_64)
+            t64 = time.perf_counter() - t
+            out["fp64_gpu"] = {"batched_unknowns_per_s": 2 * nc * n / t64, "batched_s": t64,
+                               "converged": bool(r64["converged"])}
+            rel = np.abs(Cm - C64) / np.abs(C64)
+            out["max_rel_C_err_vs_fp64"] = float(rel.max())
+            out["gate_1e-6_pass"] = bool(rel.max() <= 1e-6)
+            out["rel_J_err_vs_fp64"] = float(np.linalg.norm(Jm - J64) / np.linalg.norm(J64))
+            out["speedup_vs_fp64_batched"] = out["value"] / out["fp64_gpu"]["batched_unknowns_per_s"]
+            # central differences on a few parameters, FP64 FMM at tol 1e-12
+            pick = [0, 1, 2, 3 * 16, 3 * 16 + 1, 3 * 31 + 2]
+            ofd = G.default_opts(**dict(common, tol=1e-12))
+            h = 1e-12
+            fd_err = {}
+            for p in pick:
+                th = np.zeros(len(params))
+                th[p] = h
+                cp = ctx.cap_solve(S.move(v, params, th), c, nc, ofd)[0]
+                cm = ctx.cap_solve(S.move(v, params, -th), c, nc, ofd)[0]
+                fd = (cp - cm) / (2 * h)
+                fd_err[f"wire{names[p][0]}_{names[p][1]}"] = float(np.linalg.norm(J64[p] - fd) /
+                                                                   np.linalg.norm(fd))
+            out["fd_check_rel_err"] = fd_err
     print(json.dumps(out), flush=True)
     return 0
 

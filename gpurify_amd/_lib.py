@@ -62,6 +62,11 @@ class GpCapOpts(C.Structure):
                 ("inner_passes", C.c_uint32), ("inner_basis", C.c_uint32)]
 
 
+class GpParamMap(C.Structure):
+    _fields_ = [("n_params", C.c_uint32), ("param_off", C.POINTER(C.c_uint32)),
+                ("panel", C.POINTER(C.c_uint32)), ("dverts", C.POINTER(C.c_double))]
+
+
 class GpCapReport(C.Structure):
     _fields_ = [("n_panels", C.c_uint32), ("n_conductors", C.c_uint32),
                 ("method_used", C.c_uint32), ("precision_used", C.c_uint32),
@@ -87,7 +92,7 @@ class GpVolumeGroup(C.Structure):
 # Every symbol include/gpurify.h declares (checked by the CPU test suite).
 EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_device_ready", "gp_device_info",
-    "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_cap_solve_batched",
+    "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_cap_solve_batched", "gp_cap_jacobian",
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report",
 ]
@@ -124,6 +129,10 @@ def load():
     L.gp_cap_solve_batched.argtypes = [vp, C.POINTER(GpPanels), C.c_uint32, C.POINTER(GpCapOpts),
                                        C.POINTER(C.c_double), C.POINTER(GpCapReport)]
     L.gp_cap_solve_batched.restype = C.c_int
+    L.gp_cap_jacobian.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(GpCapOpts),
+                                  C.POINTER(GpParamMap), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(GpCapReport)]
+    L.gp_cap_jacobian.restype = C.c_int
     L.gp_fmm_apply.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(GpCapOpts), C.c_uint32,
                                C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.gp_fmm_apply.restype = C.c_int
@@ -235,6 +244,30 @@ class Context:
         o = opts if opts is not None else default_opts()
         self._check(self._L.gp_cap_solve(self._h, C.byref(p), C.byref(o), _dptr(Cm), C.byref(rep)))
         return Cm, rep.as_dict()
+
+    def cap_jacobian(self, verts4, cond, ncond, params, opts: GpCapOpts | None = None):
+        """C (ncond x ncond) and J = dC/dtheta (n_params x ncond x ncond).
+        params: [(panel indices, dverts (m x 12 or m x 4 x 3), metres per
+        unit theta)]."""
+        v = np.ascontiguousarray(verts4, dtype=np.float64).reshape(-1, 4, 3)
+        c = np.ascontiguousarray(cond, dtype=np.uint32)
+        p = GpPanels(v.shape[0], ncond, _dptr(v), _uptr(c))
+        off = np.zeros(len(params) + 1, dtype=np.uint32)
+        for i, (idx, _) in enumerate(params):
+            off[i + 1] = off[i] + len(idx)
+        pan = np.ascontiguousarray(np.concatenate([np.asarray(i, dtype=np.uint32) for i, _ in params])
+                                   if params else np.zeros(0, np.uint32), dtype=np.uint32)
+        dv = np.ascontiguousarray(np.concatenate([np.asarray(d, dtype=np.float64).reshape(-1, 12)
+                                                  for _, d in params])
+                                  if params else np.zeros((0, 12)), dtype=np.float64)
+        pm = GpParamMap(len(params), _uptr(off), _uptr(pan), _dptr(dv))
+        Cm = np.zeros((ncond, ncond))
+        Jm = np.zeros((max(len(params), 1), ncond, ncond))
+        rep = GpCapReport()
+        o = opts if opts is not None else default_opts()
+        self._check(self._L.gp_cap_jacobian(self._h, C.byref(p), C.byref(o), C.byref(pm), _dptr(Cm),
+                                            _dptr(Jm), C.byref(rep)))
+        return Cm, Jm[:len(params)], rep.as_dict()
 
     def cap_solve_batched(self, problems, opts: GpCapOpts | None = None):
         """problems: [(verts4, cond, ncond)] -> ([C_p], [report_p])."""

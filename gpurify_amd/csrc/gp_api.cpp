@@ -216,6 +216,37 @@ int gp_cap_solve_batched(gp_ctx *ctx, const gp_panels *problems, uint32_t n_prob
   });
 }
 
+int gp_cap_jacobian(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts,
+                    const gp_param_map *params, double *C, double *J, gp_cap_report *report) {
+  if (!ctx) {
+    g_thread_error = "gp_cap_jacobian: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    if (!panels || !C || !params) throw gp::Error(GP_ERR_INVALID_INPUT, "null panels, C or params");
+    if (params->n_params > 0 && (!params->param_off || !J))
+      throw gp::Error(GP_ERR_INVALID_INPUT, "null param_off or J");
+    if (params->n_params > 0 && params->param_off[params->n_params] > 0 &&
+        (!params->panel || !params->dverts))
+      throw gp::Error(GP_ERR_INVALID_INPUT, "null parameter entries");
+    gp_cap_opts o;
+    if (opts)
+      o = *opts;
+    else
+      gp_cap_opts_default(&o);
+    GP_HIP(hipSetDevice(ctx->device));
+    gp::PanelGeom g;
+    gp::panel_geometry(*panels, g);
+    gp::ParamMap pm;
+    pm.n = (int)params->n_params;
+    pm.off = params->param_off;
+    pm.panel = params->panel;
+    pm.dv = params->dverts;
+    gp::cap_jacobian(ctx, g, o, pm, C, J, report);
+  });
+}
+
 int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, uint32_t nr,
                  uint32_t precision, const double *X, double *Y) {
   if (!ctx) {

@@ -147,6 +147,25 @@ __global__ __launch_bounds__(256) void k_cap_accumulate(int ncond, int nrhs, int
   }
 }
 
+// AT = A^T (n x n, row-major), 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void k_transpose(int n, const double *__restrict__ A,
+                                                   double *__restrict__ AT) {
+  __shared__ double t[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int y = ty; y < 32; y += 8)
+    if (by + y < n && bx + tx < n) t[y][tx] = A[(long)(by + y) * n + bx + tx];
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8)
+    if (bx + y < n && by + tx < n) AT[(long)(bx + y) * n + by + tx] = t[tx][y];
+}
+
+void launch_transpose(int n, const double *A, double *AT, hipStream_t s) {
+  dim3 grid((n + 31) / 32, (n + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, n, A, AT);
+  GP_LAUNCH_CHECK();
+}
+
 void launch_lu_solve(int n, int nrhs, double *A, int *piv, double *B, double *work, int *info,
                      hipStream_t s) {
   ProfScope ps("dense_lu_solve", s, 2.0 / 3.0 * n * (double)n * n + 2.0 * n * (double)n * nrhs, 0.0);

@@ -154,6 +154,34 @@ __global__ __launch_bounds__(256) void k_unit_rhs(const uint32_t *__restrict__ c
   }
 }
 
+// Adjoint right-hand sides: b[p][r] = w[p] where panel p belongs to
+// conductor j0 + r (its scaled area), else 0 (all ld columns).
+__global__ __launch_bounds__(256) void k_weight_rhs(const uint32_t *__restrict__ cond,
+                                                    const double *__restrict__ w, int j0, int N,
+                                                    int ld, int nr, double *__restrict__ b) {
+  size_t total = (size_t)N * ld;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / ld), r = (int)(e % ld);
+    b[e] = (r < nr && (int)cond[row] == j0 + r) ? w[row] : 0.0;
+  }
+}
+
+// y[row][r] += w[row] * t[row][r] over live columns, then t = 0 there.
+__global__ __launch_bounds__(256) void k_rows_axpy_clear(double *__restrict__ y,
+                                                         double *__restrict__ t,
+                                                         const double *__restrict__ w, int N, int ld,
+                                                         int nr) {
+  size_t total = (size_t)N * nr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int row = (int)(e / nr), r = (int)(e % nr);
+    size_t idx = (size_t)row * ld + r;
+    y[idx] += w[row] * t[idx];
+    t[idx] = 0.0;
+  }
+}
+
 // Givens rotation eliminating b against a (krylov.rs:195-207).
 __device__ inline void givens(double a, double b, double &c, double &s) {
   if (b == 0.0) {
@@ -523,6 +551,20 @@ void kr_gmres_update_x(double *x, const TV *V, size_t strideV, const double *Y,
 void kr_axpy(double *x, const double *y, double alpha, int N, int ld, int nr, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, x, y, alpha, N, ld,
                      nr);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_weight_rhs(const uint32_t *cond, const double *w, int j0, int N, int ld, int nr, double *b,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_weight_rhs, dim3(grid_for((size_t)N * ld)), dim3(256), 0, s, cond, w, j0, N,
+                     ld, nr, b);
+  GP_LAUNCH_CHECK();
+}
+
+void kr_rows_axpy_clear(double *y, double *t, const double *w, int N, int ld, int nr,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_rows_axpy_clear, dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, y, t, w,
+                     N, ld, nr);
   GP_LAUNCH_CHECK();
 }
 

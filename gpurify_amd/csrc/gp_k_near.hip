@@ -126,6 +126,56 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
   GP_LAUNCH_CHECK();
 }
 
+// Transposed near field y = P_near^T x (the adjoint solve).  The near
+// relation is symmetric, so the rows of leaf S in P_near^T are the columns
+// of S in the tiles of S's near leaves T: block (T, S) of T's row-major tile
+// starts at tr_base and has row stride tr_stride (T's column count).  One
+// work-group per (leaf S, 16-column chunk); consecutive threads take
+// consecutive panels of S, so each tile row is read contiguously.
+template <typename TV, typename TX>
+__global__ __launch_bounds__(256) void k_near_spmm_tr(
+    const int *__restrict__ leaf_lo, const int *__restrict__ tr_off, const int *__restrict__ tr_leaf,
+    const int64_t *__restrict__ tr_base, const int *__restrict__ tr_stride,
+    const TV *__restrict__ vals, const TX *__restrict__ x, int ldx, double *__restrict__ y,
+    int ldy, int nr) {
+  using A = typename Acc<TX>::type;
+  const int S = blockIdx.x, r0 = blockIdx.y * NF_RC;
+  const int rc = min(NF_RC, nr - r0);
+  const int s0 = leaf_lo[S], ns = leaf_lo[S + 1] - s0;
+  for (int e = threadIdx.x; e < ns * rc; e += blockDim.x) {
+    const int s = e % ns, r = r0 + e / ns;
+    A acc = 0;
+    for (int q = tr_off[S]; q < tr_off[S + 1]; ++q) {
+      const int T = tr_leaf[q], t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+      const TV *blk = vals + tr_base[q] + s;
+      const int st = tr_stride[q];
+      for (int t = 0; t < nt; ++t) acc += (A)blk[(int64_t)t * st] * (A)x[(int64_t)(t0 + t) * ldx + r];
+    }
+    y[(int64_t)(s0 + s) * ldy + r] = (double)acc;
+  }
+}
+
+template <typename TV, typename TX>
+void launch_near_apply_tr_t(const NearTiles &nf, const NearTranspose &tr, const TV *vals,
+                            const TX *x, int ldx, double *y, int ldy, int nr, hipStream_t s) {
+  if (nf.n_leaves == 0) return;
+  ProfScope ps(sizeof(TV) == 4 ? "near_tr_f32" : "near_tr_f64", s, 2.0 * nf.nnz * nr,
+               (double)nf.nnz * sizeof(TV) + (double)nf.n * nr * (sizeof(TX) + sizeof(double)));
+  dim3 grid(nf.n_leaves, (nr + NF_RC - 1) / NF_RC);
+  hipLaunchKernelGGL((k_near_spmm_tr<TV, TX>), grid, dim3(256), 0, s, nf.leaf_lo, tr.off, tr.leaf,
+                     tr.base, tr.stride, vals, x, ldx, y, ldy, nr);
+  GP_LAUNCH_CHECK();
+}
+
+void launch_near_apply_tr(const NearTiles &nf, const NearTranspose &tr, const double *vals,
+                          const double *x, int ldx, double *y, int ldy, int nr, hipStream_t s) {
+  launch_near_apply_tr_t<double, double>(nf, tr, vals, x, ldx, y, ldy, nr, s);
+}
+void launch_near_apply_tr(const NearTiles &nf, const NearTranspose &tr, const float *vals,
+                          const float *x, int ldx, double *y, int ldy, int nr, hipStream_t s) {
+  launch_near_apply_tr_t<float, float>(nf, tr, vals, x, ldx, y, ldy, nr, s);
+}
+
 void launch_near_apply(const NearTiles &nf, const double *vals, const double *x, int ldx,
                        double *y, int ldy, int nr, hipStream_t s) {
   launch_near_apply_t<double, double>(nf, vals, x, ldx, y, ldy, nr, s);

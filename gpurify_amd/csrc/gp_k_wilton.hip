@@ -10,20 +10,109 @@
 
 namespace gp {
 
+// Forward-mode dual number: a value and its directional derivatives along
+// ND directions (geometric parameters moving the same panels), for the
+// sensitivities: every operation carries d/dtheta along, the transcendental
+// itself evaluated once for all directions.
+template <int ND>
+struct DualN {
+  double v, d[ND];
+  __device__ DualN(double v_ = 0.0) : v(v_) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) d[i] = 0.0;
+  }
+};
+// a' = f'(a) a' for every direction
+template <int ND>
+__device__ inline DualN<ND> chain(double fv, double fp, const DualN<ND> &a) {
+  DualN<ND> r(fv);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) r.d[i] = fp * a.d[i];
+  return r;
+}
+template <int ND>
+__device__ inline DualN<ND> operator+(const DualN<ND> &a, const DualN<ND> &b) {
+  DualN<ND> r(a.v + b.v);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+template <int ND>
+__device__ inline DualN<ND> operator-(const DualN<ND> &a, const DualN<ND> &b) {
+  DualN<ND> r(a.v - b.v);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+template <int ND>
+__device__ inline DualN<ND> operator-(const DualN<ND> &a) {
+  return chain(-a.v, -1.0, a);
+}
+template <int ND>
+__device__ inline DualN<ND> operator*(const DualN<ND> &a, const DualN<ND> &b) {
+  DualN<ND> r(a.v * b.v);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+template <int ND>
+__device__ inline DualN<ND> operator/(const DualN<ND> &a, const DualN<ND> &b) {
+  const double q = a.v / b.v, ib = 1.0 / b.v;
+  DualN<ND> r(q);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) r.d[i] = (a.d[i] - q * b.d[i]) * ib;
+  return r;
+}
+template <int ND>
+__device__ inline DualN<ND> &operator+=(DualN<ND> &a, const DualN<ND> &b) { return a = a + b; }
+template <int ND>
+__device__ inline DualN<ND> &operator/=(DualN<ND> &a, const DualN<ND> &b) { return a = a / b; }
+template <int ND>
+__device__ inline DualN<ND> sqrt(const DualN<ND> &a) {
+  const double r = ::sqrt(a.v);
+  return chain(r, r > 0.0 ? 0.5 / r : 0.0, a);
+}
+template <int ND>
+__device__ inline DualN<ND> log(const DualN<ND> &a) { return chain(::log(a.v), 1.0 / a.v, a); }
+template <int ND>
+__device__ inline DualN<ND> atan(const DualN<ND> &a) {
+  return chain(::atan(a.v), 1.0 / (1.0 + a.v * a.v), a);
+}
+template <int ND>
+__device__ inline DualN<ND> asinh(const DualN<ND> &a) {
+  return chain(::asinh(a.v), 1.0 / ::sqrt(1.0 + a.v * a.v), a);
+}
+// |x| with derivative 0 at x = 0 (the central difference there).
+template <int ND>
+__device__ inline DualN<ND> fabs(const DualN<ND> &a) {
+  return chain(::fabs(a.v), a.v > 0.0 ? 1.0 : (a.v < 0.0 ? -1.0 : 0.0), a);
+}
+template <int ND>
+__device__ inline double val(const DualN<ND> &x) { return x.v; }
+__device__ inline double val(double x) { return x; }
+// The double forms, visible next to the Dual ones inside this namespace.
+__device__ inline double sqrt(double x) { return ::sqrt(x); }
+__device__ inline double log(double x) { return ::log(x); }
+__device__ inline double atan(double x) { return ::atan(x); }
+__device__ inline double asinh(double x) { return ::asinh(x); }
+__device__ inline double fabs(double x) { return ::fabs(x); }
+
 // Closed-form I(p) = integral over a flat quadrilateral of 1/|p - r'| da'
 // (Wilton et al. 1984), following crates/pex/src/quasistatic/integrals/
 // panel.rs:147-208 term by term: Newell normal (panel.rs:256-269), per edge
 // the in-plane distance, the log term with its asinh fallback
-// (panel.rs:271-285) and the solid-angle atan term.  v: 12 doubles.
-__device__ double wilton(const double *__restrict__ v, double px, double py, double pz) {
-  double vx[4], vy[4], vz[4];
+// (panel.rs:271-285) and the solid-angle atan term.  v: 12 values.  T is
+// double, or Dual for the directional derivative (branches follow values).
+template <typename T>
+__device__ T wilton_t(const T *v, T px, T py, T pz) {
+  T vx[4], vy[4], vz[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     vx[k] = v[3 * k];
     vy[k] = v[3 * k + 1];
     vz[k] = v[3 * k + 2];
   }
-  double nx = 0.0, ny = 0.0, nz = 0.0;
+  T nx = 0.0, ny = 0.0, nz = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     int j = (i + 1) & 3;
@@ -31,53 +120,119 @@ __device__ double wilton(const double *__restrict__ v, double px, double py, dou
     ny += (vz[i] - vz[j]) * (vx[i] + vx[j]);
     nz += (vx[i] - vx[j]) * (vy[i] + vy[j]);
   }
-  double nn = sqrt(nx * nx + ny * ny + nz * nz);
-  if (nn == 0.0) {
+  T nn = sqrt(nx * nx + ny * ny + nz * nz);
+  if (val(nn) == 0.0) {
     nx = ny = nz = 0.0;
   } else {
     nx /= nn;
     ny /= nn;
     nz /= nn;
   }
-  double d = (px - vx[0]) * nx + (py - vy[0]) * ny + (pz - vz[0]) * nz;
-  double p0x = px - nx * d, p0y = py - ny * d, p0z = pz - nz * d;
-  double absd = fabs(d);
-  double total = 0.0;
+  T d = (px - vx[0]) * nx + (py - vy[0]) * ny + (pz - vz[0]) * nz;
+  T p0x = px - nx * d, p0y = py - ny * d, p0z = pz - nz * d;
+  T absd = fabs(d);
+  T total = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     int j = (i + 1) & 3;
-    double ex = vx[j] - vx[i], ey = vy[j] - vy[i], ez = vz[j] - vz[i];
-    double el = sqrt(ex * ex + ey * ey + ez * ez);
-    if (el == 0.0) continue;
-    double lx = ex / el, ly = ey / el, lz = ez / el;
-    double ux = ly * nz - lz * ny, uy = lz * nx - lx * nz, uz = lx * ny - ly * nx;
-    double ax = vx[i] - p0x, ay = vy[i] - p0y, az = vz[i] - p0z;
-    double bx = vx[j] - p0x, by = vy[j] - p0y, bz = vz[j] - p0z;
-    double perp = ax * ux + ay * uy + az * uz;
-    double sm = ax * lx + ay * ly + az * lz;
-    double sp = bx * lx + by * ly + bz * lz;
-    double r0sq = perp * perp + d * d;
-    double rp = sqrt(sp * sp + r0sq);
-    double rm = sqrt(sm * sm + r0sq);
-    double num = rp + sp, den = rm + sm;
-    double logt;
-    if (num > 0.0 && den > 0.0) {
+    T ex = vx[j] - vx[i], ey = vy[j] - vy[i], ez = vz[j] - vz[i];
+    T el = sqrt(ex * ex + ey * ey + ez * ez);
+    if (val(el) == 0.0) continue;
+    T lx = ex / el, ly = ey / el, lz = ez / el;
+    T ux = ly * nz - lz * ny, uy = lz * nx - lx * nz, uz = lx * ny - ly * nx;
+    T ax = vx[i] - p0x, ay = vy[i] - p0y, az = vz[i] - p0z;
+    T bx = vx[j] - p0x, by = vy[j] - p0y, bz = vz[j] - p0z;
+    T perp = ax * ux + ay * uy + az * uz;
+    T sm = ax * lx + ay * ly + az * lz;
+    T sp = bx * lx + by * ly + bz * lz;
+    T r0sq = perp * perp + d * d;
+    T rp = sqrt(sp * sp + r0sq);
+    T rm = sqrt(sm * sm + r0sq);
+    T num = rp + sp, den = rm + sm;
+    T logt;
+    if (val(num) > 0.0 && val(den) > 0.0) {
       logt = perp * log(num / den);
-    } else if (fabs(den) > 0.0) {
-      double r0 = sqrt(r0sq);
-      logt = r0 == 0.0 ? 0.0 : perp * (asinh(sp / r0) - asinh(sm / r0));
+    } else if (::fabs(val(den)) > 0.0) {
+      T r0 = sqrt(r0sq);
+      logt = val(r0) == 0.0 ? T(0.0) : perp * (asinh(sp / r0) - asinh(sm / r0));
     } else {
       logt = 0.0;
     }
-    double att = 0.0;
-    if (absd > 0.0 && perp != 0.0) {
-      double tp = (perp * sp) / (r0sq + absd * rp);
-      double tm = (perp * sm) / (r0sq + absd * rm);
+    T att = 0.0;
+    // perp == 0 gives att == 0 (the reference skips it); its derivative in
+    // perp is not 0 there, so the Dual form always evaluates the term.
+    constexpr bool dual = !std::is_same<T, double>::value;
+    if (val(absd) > 0.0 && (val(perp) != 0.0 || dual)) {
+      T tp = (perp * sp) / (r0sq + absd * rp);
+      T tm = (perp * sm) / (r0sq + absd * rm);
       att = absd * (atan(tp) - atan(tm));
     }
     total += logt - att;
   }
   return total;
+}
+
+__device__ double wilton(const double *__restrict__ v, double px, double py, double pz) {
+  return wilton_t<double>(v, px, py, pz);
+}
+
+// Fan-triangulated area and area-weighted centroid of a panel (panel.rs:29-67),
+// with their directional derivatives.
+template <int ND>
+__device__ void panel_props_dual(const DualN<ND> *v, DualN<ND> c[3], DualN<ND> &area) {
+  using D = DualN<ND>;
+  D asum(0.0), cx(0.0), cy(0.0), cz(0.0);
+  for (int k = 1; k < 3; ++k) {
+    D ax = v[3 * k] - v[0], ay = v[3 * k + 1] - v[1], az = v[3 * k + 2] - v[2];
+    D bx = v[3 * k + 3] - v[0], by = v[3 * k + 4] - v[1], bz = v[3 * k + 5] - v[2];
+    D qx = ay * bz - az * by, qy = az * bx - ax * bz, qz = ax * by - ay * bx;
+    D a = sqrt(qx * qx + qy * qy + qz * qz) * D(0.5);
+    D third(1.0 / 3.0);
+    cx += (v[0] + v[3 * k] + v[3 * k + 3]) * third * a;
+    cy += (v[1] + v[3 * k + 1] + v[3 * k + 4]) * third * a;
+    cz += (v[2] + v[3 * k + 2] + v[3 * k + 5]) * third * a;
+    asum += a;
+  }
+  c[0] = cx / asum;
+  c[1] = cy / asum;
+  c[2] = cz / asum;
+  area = asum;
+}
+
+// 2 x 2 Gauss rule of a flat panel through its bilinear map from [-1, 1]^2:
+// points xg and weights wg (|dx/da x dx/db| at the point), so that
+// sum_g wg / |p - xg| integrates 1/|p - r'| with a relative error below
+// 4e-6 once |p - centroid| exceeds 6 panel diagonals (checked against the
+// closed form); used for the far entries of the sensitivities.
+template <typename T>
+__device__ void gauss4_t(const T *v, T xg[4][3], T wg[4]) {
+  const double g = 0.57735026918962576451;  // 1 / sqrt(3)
+  const double ga[4] = {-g, g, g, -g}, gb[4] = {-g, -g, g, g};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double a = ga[q], b = gb[q];
+    const double N[4] = {0.25 * (1 - a) * (1 - b), 0.25 * (1 + a) * (1 - b),
+                         0.25 * (1 + a) * (1 + b), 0.25 * (1 - a) * (1 + b)};
+    const double Na[4] = {-0.25 * (1 - b), 0.25 * (1 - b), 0.25 * (1 + b), -0.25 * (1 + b)};
+    const double Nb[4] = {-0.25 * (1 - a), -0.25 * (1 + a), 0.25 * (1 + a), 0.25 * (1 - a)};
+    T xa[3], xb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T x = 0.0, ta = 0.0, tb = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        x += T(N[i]) * v[3 * i + d];
+        ta += T(Na[i]) * v[3 * i + d];
+        tb += T(Nb[i]) * v[3 * i + d];
+      }
+      xg[q][d] = x;
+      xa[d] = ta;
+      xb[d] = tb;
+    }
+    T cx = xa[1] * xb[2] - xa[2] * xb[1], cy = xa[2] * xb[0] - xa[0] * xb[2],
+      cz = xa[0] * xb[1] - xa[1] * xb[0];
+    wg[q] = sqrt(cx * cx + cy * cy + cz * cz);
+  }
 }
 
 // P'[k][l] = I_l(c_k), cap/solver.rs:81-97 (row k = collocation at centroid k).
@@ -150,6 +305,391 @@ void launch_self_inv_diag(int n, const double *verts, const double *cen, double 
   hipLaunchKernelGGL(k_self_inv_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, verts, cen,
                      si_scale, inv);
   GP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- sensitivities
+// A group of ND geometric parameters moves the vertices of the same panels
+// moved[0..nm) (device order); direction t has velocities dv + t * dstride
+// (12 per moved panel: d vertex / d theta in scaled units per scaled theta).
+// mv_of[k] is k's index in moved or -1.  With P' sigma_j = e_j and
+// P'^T lambda_i = a_i (scaled areas of conductor i):
+//   dC'_ij/dtheta = sum_(k moved, cond k = i) da_k sigma_kj
+//                   - sum_(k, l) lambda_ki (dP'_kl / dtheta) sigma_lj,
+// dP'_kl nonzero when panel k (its collocation point) or l moves.  Every
+// entry is the forward-mode derivative of the closed-form integral.
+
+__global__ void k_scatter_moved(int nm, const int *__restrict__ moved, int *__restrict__ mv_of,
+                                int set) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nm) mv_of[moved[i]] = set ? i : -1;
+}
+
+// Gauss table of every panel: gq[k][0..11] points, [12..15] weights, [16]
+// squared diagonal (the far-entry test), [17..19] unused.
+__global__ void k_gauss_table(int n, const double *__restrict__ verts, double *__restrict__ gq) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double *v = verts + 12L * k;
+  double xg[4][3], wg[4];
+  gauss4_t<double>(v, xg, wg);
+  double *o = gq + 20L * k;
+  for (int q = 0; q < 4; ++q)
+    for (int d = 0; d < 3; ++d) o[3 * q + d] = xg[q][d];
+  for (int q = 0; q < 4; ++q) o[12 + q] = wg[q];
+  double d1 = 0.0, d2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    d1 += (v[6 + d] - v[d]) * (v[6 + d] - v[d]);
+    d2 += (v[9 + d] - v[3 + d]) * (v[9 + d] - v[3 + d]);
+  }
+  o[16] = fmax(d1, d2);
+}
+
+// dc/dtheta (3 per moved panel and direction), da/dtheta and the
+// derivatives of the panel's Gauss points and weights (16 per direction).
+template <int ND>
+__global__ void k_moved_props(int nm, const int *__restrict__ moved,
+                              const double *__restrict__ verts, const double *__restrict__ dv,
+                              long dstride, double *__restrict__ dcen, double *__restrict__ darea,
+                              double *__restrict__ dgq) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  const int k = moved[i];
+  DualN<ND> v[12];
+  for (int e = 0; e < 12; ++e) {
+    v[e] = DualN<ND>(verts[12L * k + e]);
+    for (int t = 0; t < ND; ++t) v[e].d[t] = dv[t * dstride + 12L * i + e];
+  }
+  DualN<ND> c[3], a;
+  panel_props_dual(v, c, a);
+  DualN<ND> xg[4][3], wg[4];
+  gauss4_t<DualN<ND>>(v, xg, wg);
+  for (int t = 0; t < ND; ++t) {
+    for (int d = 0; d < 3; ++d) dcen[(3L * t) * nm + 3 * i + d] = c[d].d[t];
+    darea[(long)t * nm + i] = a.d[t];
+    double *o = dgq + ((long)t * nm + i) * 16;
+    for (int q = 0; q < 4; ++q) {
+      for (int d = 0; d < 3; ++d) o[3 * q + d] = xg[q][d].d[t];
+      o[12 + q] = wg[q].d[t];
+    }
+  }
+}
+
+// d/dtheta of sum_g w_g / |p - x_g| for the ND directions: dp (observer),
+// dx / dw (source points and weights, null when the source stays).
+template <int ND>
+__device__ inline void far_entry(const double p[3], const double (*dp)[3], const double *gq,
+                                 const double *dgq, long dgs, double out[ND]) {
+#pragma unroll
+  for (int t = 0; t < ND; ++t) out[t] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double ex = p[0] - gq[3 * q], ey = p[1] - gq[3 * q + 1], ez = p[2] - gq[3 * q + 2];
+    const double ir = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+    const double w = gq[12 + q], wir3 = w * ir * ir * ir;
+#pragma unroll
+    for (int t = 0; t < ND; ++t) {
+      double gx = dp ? dp[t][0] : 0.0, gy = dp ? dp[t][1] : 0.0, gz = dp ? dp[t][2] : 0.0;
+      double dw = 0.0;
+      if (dgq) {
+        const double *o = dgq + t * dgs;
+        gx -= o[3 * q];
+        gy -= o[3 * q + 1];
+        gz -= o[3 * q + 2];
+        dw = o[12 + q];
+      }
+      out[t] += dw * ir - wir3 * (ex * gx + ey * gy + ez * gz);
+    }
+  }
+}
+
+// Rows of the moved panels: partial[((chunk * nm + mi) * ND + t) * nc + j] =
+// sum over the panels l of this chunk (2048 a chunk) of dP'_kl sigma[l][j]
+// along direction t, k = moved[mi].  Entries whose source lies beyond
+// sqrt(far2) of its diagonals use the Gauss rule (far2 = 0: every entry
+// exact).  Each thread evaluates 8 entries into LDS; then groups of nc
+// threads (256 / nc of them) contract slices of the chunk against sigma
+// (row-contiguous reads) and the groups' sums are added.
+template <int ND>
+__global__ __launch_bounds__(256) void k_sens_rows(int N, const int *__restrict__ moved,
+                                                   const int *__restrict__ mv_of,
+                                                   const double *__restrict__ verts,
+                                                   const double *__restrict__ cen,
+                                                   const double *__restrict__ dv, long dstride,
+                                                   const double *__restrict__ dcen,
+                                                   const double *__restrict__ gq,
+                                                   const double *__restrict__ dgq, double far2,
+                                                   const double *__restrict__ sigma, int lds,
+                                                   int nc, double *__restrict__ partial) {
+  __shared__ double sdp[ND][2048];
+  __shared__ double sred[ND][256];
+  const int mi = blockIdx.x, chunk = blockIdx.y, nm = gridDim.x;
+  const int k = moved[mi], tid = threadIdx.x;
+  DualN<ND> p[3];
+  double pv[3], pd[ND][3];
+  for (int d = 0; d < 3; ++d) {
+    pv[d] = cen[3 * k + d];
+    p[d] = DualN<ND>(pv[d]);
+    for (int t = 0; t < ND; ++t) p[d].d[t] = pd[t][d] = dcen[(3L * t) * nm + 3 * mi + d];
+  }
+  const int l0 = chunk * 2048, nl = min(N, l0 + 2048) - l0;
+  for (int q = tid; q < 2048; q += 256) {
+    const int l = l0 + q;
+    if (q < nl) {
+      const int ml = mv_of[l];
+      const double cx = cen[3 * l] - pv[0], cy = cen[3 * l + 1] - pv[1], cz = cen[3 * l + 2] - pv[2];
+      if (far2 > 0.0 && cx * cx + cy * cy + cz * cz > far2 * gq[20L * l + 16]) {
+        double f[ND];
+        far_entry<ND>(pv, pd, gq + 20L * l, ml >= 0 ? dgq + 16L * ml : nullptr, 16L * nm, f);
+#pragma unroll
+        for (int t = 0; t < ND; ++t) sdp[t][q] = f[t];
+      } else {
+        DualN<ND> v[12];
+#pragma unroll
+        for (int e = 0; e < 12; ++e) {
+          v[e] = DualN<ND>(verts[12L * l + e]);
+          if (ml >= 0)
+#pragma unroll
+            for (int t = 0; t < ND; ++t) v[e].d[t] = dv[t * dstride + 12L * ml + e];
+        }
+        const DualN<ND> w = wilton_t<DualN<ND>>(v, p[0], p[1], p[2]);
+#pragma unroll
+        for (int t = 0; t < ND; ++t) sdp[t][q] = w.d[t];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < ND; ++t) sdp[t][q] = 0.0;
+    }
+  }
+  __syncthreads();
+  const int ng = max(1, 256 / nc);          // thread groups of nc columns
+  const int g = tid / nc, j = tid % nc;
+  const int span = (nl + ng - 1) / ng;
+  double acc[ND];
+#pragma unroll
+  for (int t = 0; t < ND; ++t) acc[t] = 0.0;
+  if (g < ng && j < nc) {
+    const int q1 = min(nl, (g + 1) * span);
+    for (int q = g * span; q < q1; ++q) {
+      const double sg = sigma[(long)(l0 + q) * lds + j];
+#pragma unroll
+      for (int t = 0; t < ND; ++t) acc[t] += sdp[t][q] * sg;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < ND; ++t) sred[t][tid] = acc[t];
+  __syncthreads();
+  if (tid < nc) {
+#pragma unroll
+    for (int t = 0; t < ND; ++t) {
+      double s = 0.0;
+      for (int gg = 0; gg < ng; ++gg) s += sred[t][gg * nc + tid];
+      partial[(((long)chunk * nm + mi) * ND + t) * nc + tid] = s;
+    }
+  }
+}
+
+// R_t[moved[mi]][j] = sum over chunks of the partials (fixed order); R_t at
+// R + t * N * nc.
+__global__ void k_sens_rows_reduce(int N, int nm, int nchunk, int nd, int nc,
+                                   const int *__restrict__ moved,
+                                   const double *__restrict__ partial, double *__restrict__ R) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nm * nd * nc) return;
+  const int mi = e / (nd * nc), t = (e / nc) % nd, j = e % nc;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += partial[(((long)c * nm + mi) * nd + t) * nc + j];
+  R[(long)t * N * nc + (long)moved[mi] * nc + j] = s;
+}
+
+// Rows of the panels that stay: R_t[k][j] = sum over moved l of dP'_kl sigma[l][j]
+// (fixed collocation point), far entries by the Gauss rule as above.  One
+// thread per row; the moved panels are staged 16 at a time in LDS with their
+// sigma rows, each thread keeps its 16 entries in registers and adds their
+// contraction to its ND * nc accumulators in LDS (dynamic: blockDim * ND * nc
+// doubles after the staging area).
+template <int ND>
+__global__ __launch_bounds__(256) void k_sens_cols(int N, int nm, const int *__restrict__ moved,
+                                                   const int *__restrict__ mv_of,
+                                                   const double *__restrict__ verts,
+                                                   const double *__restrict__ cen,
+                                                   const double *__restrict__ dv, long dstride,
+                                                   const double *__restrict__ gq,
+                                                   const double *__restrict__ dgq, double far2,
+                                                   const double *__restrict__ sigma, int lds,
+                                                   int nc, double *__restrict__ R) {
+  // per staged panel: 12 vertices, 12 ND velocities, 17 Gauss values,
+  // 16 ND Gauss derivatives, 3 centroid, 1 pad
+  constexpr int OV = 0, ODV = 12, OG = 12 * (ND + 1), ODG = OG + 17, OC = ODG + 16 * ND;
+  constexpr int SW = OC + 4;
+  constexpr int MQ = 16;                 // moved panels per stage
+  extern __shared__ double sm[];
+  double *sv = sm;                       // MQ x SW
+  double *ss = sm + MQ * SW;             // MQ x nc sigma rows
+  double *acc = ss + MQ * nc + (long)threadIdx.x * ND * nc;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool row = k < N && mv_of[k] < 0;
+  const double pv[3] = {row ? cen[3 * k] : 0.0, row ? cen[3 * k + 1] : 0.0,
+                        row ? cen[3 * k + 2] : 0.0};
+  const DualN<ND> px(pv[0]), py(pv[1]), pz(pv[2]);
+  for (int j = 0; j < ND * nc; ++j) acc[j] = 0.0;
+  for (int m0 = 0; m0 < nm; m0 += MQ) {
+    const int mn = min(MQ, nm - m0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < mn * SW; e += blockDim.x) {
+      const int q = e / SW, c = e % SW, mq = m0 + q, l = moved[mq];
+      double x = 0.0;
+      if (c < ODV) x = verts[12L * l + c];
+      else if (c < OG) x = dv[((c - ODV) / 12) * dstride + 12L * mq + (c - ODV) % 12];
+      else if (c < ODG) x = gq[20L * l + (c - OG)];
+      else if (c < OC) x = dgq[((long)((c - ODG) / 16) * nm + mq) * 16 + (c - ODG) % 16];
+      else if (c < OC + 3) x = cen[3 * l + (c - OC)];
+      sv[e] = x;
+    }
+    for (int e = threadIdx.x; e < mn * nc; e += blockDim.x)
+      ss[e] = sigma[(long)moved[m0 + e / nc] * lds + e % nc];
+    __syncthreads();
+    if (!row) continue;
+    double dp[ND][MQ];
+#pragma unroll
+    for (int q = 0; q < MQ; ++q) {
+      if (q < mn) {
+        const double *st = sv + q * SW;
+        const double cx = st[OC] - pv[0], cy = st[OC + 1] - pv[1], cz = st[OC + 2] - pv[2];
+        if (far2 > 0.0 && cx * cx + cy * cy + cz * cz > far2 * st[OG + 16]) {
+          double f[ND];
+          far_entry<ND>(pv, nullptr, st + OG, st + ODG, 16, f);
+#pragma unroll
+          for (int t = 0; t < ND; ++t) dp[t][q] = f[t];
+        } else {
+          DualN<ND> v[12];
+#pragma unroll
+          for (int e = 0; e < 12; ++e) {
+            v[e] = DualN<ND>(st[OV + e]);
+#pragma unroll
+            for (int t = 0; t < ND; ++t) v[e].d[t] = st[ODV + 12 * t + e];
+          }
+          const DualN<ND> w = wilton_t<DualN<ND>>(v, px, py, pz);
+#pragma unroll
+          for (int t = 0; t < ND; ++t) dp[t][q] = w.d[t];
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < ND; ++t) dp[t][q] = 0.0;
+      }
+    }
+    for (int j = 0; j < nc; ++j) {
+#pragma unroll
+      for (int t = 0; t < ND; ++t) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < MQ; ++q) s += dp[t][q] * ss[q * nc + j];
+        acc[t * nc + j] += s;
+      }
+    }
+  }
+  if (row)
+    for (int t = 0; t < ND; ++t)
+      for (int j = 0; j < nc; ++j) R[(long)t * N * nc + (long)k * nc + j] = acc[t * nc + j];
+}
+
+// J'[i][j] partial over the rows of block b:
+//   -sum_k lambda[k][i] R[k][j] + sum_(k, cond k = i) da[k] sigma[k][j]
+// (da[k] = 0 for panels that stay), partial[b][i][j].
+__global__ __launch_bounds__(256) void k_sens_contract(int N, int nc, int rows,
+                                                       const double *__restrict__ lambda, int ldl,
+                                                       const double *__restrict__ R,
+                                                       const double *__restrict__ sigma, int lds,
+                                                       const double *__restrict__ da,
+                                                       const uint32_t *__restrict__ cond,
+                                                       int cond0, double *__restrict__ partial) {
+  const int b = blockIdx.x;
+  const int k0 = b * rows, k1 = min(N, k0 + rows);
+  for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
+    const int i = e / nc, j = e % nc;
+    double t = 0.0;
+    for (int k = k0; k < k1; ++k) {
+      t -= lambda[(long)k * ldl + i] * R[(long)k * nc + j];
+      if (da[k] != 0.0 && (int)cond[k] - cond0 == i) t += da[k] * sigma[(long)k * lds + j];
+    }
+    partial[(long)b * nc * nc + e] = t;
+  }
+}
+
+__global__ void k_sens_sum(int nblk, int n, const double *__restrict__ partial,
+                           double *__restrict__ out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double t = 0.0;
+  for (int b = 0; b < nblk; ++b) t += partial[(long)b * n + e];
+  out[e] = t;
+}
+
+__global__ void k_scatter_da(int nm, const int *__restrict__ moved, const double *__restrict__ darea,
+                             double *__restrict__ da, int set) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nm) da[moved[i]] = set ? darea[i] : 0.0;
+}
+
+template <int ND>
+void launch_sensitivity_nd(const SensArgs &a, hipStream_t s) {
+  const int nm = a.nm, nc = a.nc, N = a.N;
+  const int g = (nm + 255) / 256;
+  hipLaunchKernelGGL(k_scatter_moved, dim3(g), dim3(256), 0, s, nm, a.moved, a.mv_of, 1);
+  hipLaunchKernelGGL((k_moved_props<ND>), dim3(g), dim3(256), 0, s, nm, a.moved, a.verts, a.dv,
+                     a.dstride, a.dcen, a.darea, a.dgq);
+  const int nchunk = (N + 2047) / 2048;
+  hipLaunchKernelGGL((k_sens_rows<ND>), dim3(nm, nchunk), dim3(256), 0, s, N, a.moved, a.mv_of,
+                     a.verts, a.cen, a.dv, a.dstride, a.dcen, a.gq, a.dgq, a.far2, a.sigma, a.lds,
+                     nc, a.scratch);
+  hipLaunchKernelGGL(k_sens_rows_reduce, dim3((nm * ND * nc + 255) / 256), dim3(256), 0, s, N, nm,
+                     nchunk, ND, nc, a.moved, a.scratch, a.R);
+  // threads per block such that the staging area and the per-row
+  // accumulators fit 64 KiB of LDS (ND * nc <= 512: at least 16 threads)
+  int tpb = 256;
+  const size_t stage = 16 * (12 * (ND + 1) + 17 + 16 * ND + 4) + 16 * (size_t)nc;
+  while (tpb > 16 && sizeof(double) * (stage + (size_t)tpb * ND * nc) > 65536) tpb >>= 1;
+  const size_t lds_cols = sizeof(double) * (stage + (size_t)tpb * ND * nc);
+  hipLaunchKernelGGL((k_sens_cols<ND>), dim3((N + tpb - 1) / tpb), dim3(tpb), lds_cols, s, N, nm,
+                     a.moved, a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.gq, a.dgq, a.far2,
+                     a.sigma, a.lds, nc, a.R);
+  const int rows = 2048, nblk = (N + rows - 1) / rows;
+  for (int t = 0; t < ND; ++t) {
+    hipLaunchKernelGGL(k_scatter_da, dim3(g), dim3(256), 0, s, nm, a.moved, a.darea + (long)t * nm,
+                       a.da, 1);
+    hipLaunchKernelGGL(k_sens_contract, dim3(nblk), dim3(256), 0, s, N, nc, rows, a.lambda, a.ldl,
+                       a.R + (long)t * N * nc, a.sigma, a.lds, a.da, a.cond, a.cond0, a.scratch);
+    hipLaunchKernelGGL(k_sens_sum, dim3((nc * nc + 255) / 256), dim3(256), 0, s, nblk, nc * nc,
+                       a.scratch, a.J + (long)t * nc * nc);
+    hipLaunchKernelGGL(k_scatter_da, dim3(g), dim3(256), 0, s, nm, a.moved, a.darea, a.da, 0);
+  }
+  hipLaunchKernelGGL(k_scatter_moved, dim3(g), dim3(256), 0, s, nm, a.moved, a.mv_of, 0);
+  GP_LAUNCH_CHECK();
+}
+
+void launch_sensitivity(const SensArgs &a, hipStream_t s) {
+  if (a.nm == 0) {
+    hipLaunchKernelGGL(k_sens_sum, dim3((a.nd * a.nc * a.nc + 255) / 256), dim3(256), 0, s, 0,
+                       a.nd * a.nc * a.nc, a.scratch, a.J);  // zero fill
+    GP_LAUNCH_CHECK();
+    return;
+  }
+  ProfScope ps("sens_wilton_dual", s, 0.0, 0.0);
+  switch (a.nd) {
+    case 1: launch_sensitivity_nd<1>(a, s); break;
+    case 2: launch_sensitivity_nd<2>(a, s); break;
+    default: throw Error(GP_ERR_INTERNAL, "sensitivity groups hold 1 or 2 directions");
+  }
+}
+
+void launch_gauss_table(int n, const double *verts, double *gq, hipStream_t s) {
+  hipLaunchKernelGGL(k_gauss_table, dim3((n + 255) / 256), dim3(256), 0, s, n, verts, gq);
+  GP_LAUNCH_CHECK();
+}
+
+size_t sensitivity_scratch(int N, int nm_max, int nc, int nd) {
+  const size_t rows = (size_t)((N + 2047) / 2048) * nm_max * nd * nc;
+  const size_t con = (size_t)((N + 2047) / 2048) * nc * nc;
+  return std::max(rows, con);
 }
 
 }  // namespace gp

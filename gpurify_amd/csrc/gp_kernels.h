@@ -17,9 +17,38 @@ void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_of
 void launch_self_inv_diag(int n, const double *verts, const double *cen, double si_scale,
                           double *inv, hipStream_t s);
 
+// Geometric sensitivities of C' for a group of nd (1 or 2) parameters that
+// move the same panels (device arrays; scaled units).  dv: direction t at
+// dv + t * dstride.  sigma: N x lds forward solutions (nc columns), lambda:
+// N x ldl adjoint solutions (P'^T lambda_i = scaled areas of conductor
+// cond0 + i).  mv_of (N ints) and da (N doubles) must hold -1 / 0 on entry
+// and are restored.  scratch: sensitivity_scratch(N, nm, nc, nd) doubles;
+// R: nd x N x nc; dcen: 3 nd nm; darea: nd nm; dgq: 16 nd nm.  J: nd x nc x nc.
+// gq: launch_gauss_table of the panels; entries whose source centroid lies
+// beyond sqrt(far2) source diagonals use its 2 x 2 Gauss rule (far2 = 0:
+// every entry by the closed form).
+struct SensArgs {
+  int N = 0, nm = 0, nc = 0, nd = 1, lds = 0, ldl = 0, cond0 = 0;
+  long dstride = 0;
+  double far2 = 0.0;
+  const double *gq = nullptr;
+  double *dgq = nullptr;
+  const int *moved = nullptr;
+  const double *dv = nullptr, *verts = nullptr, *cen = nullptr, *sigma = nullptr,
+               *lambda = nullptr;
+  const uint32_t *cond = nullptr;
+  int *mv_of = nullptr;
+  double *da = nullptr, *dcen = nullptr, *darea = nullptr, *R = nullptr, *scratch = nullptr,
+         *J = nullptr;
+};
+void launch_sensitivity(const SensArgs &a, hipStream_t s);
+size_t sensitivity_scratch(int N, int nm_max, int nc, int nd);
+void launch_gauss_table(int n, const double *verts, double *gq, hipStream_t s);
+
 // ---------------------------------------------------------------- dense LU
 void launch_lu_solve(int n, int nrhs, double *A, int *piv, double *B, double *work, int *info,
                      hipStream_t s);
+void launch_transpose(int n, const double *A, double *AT, hipStream_t s);
 void launch_cap_accumulate(int ncond, int nrhs, int ldx, const int *coff, const int *cidx,
                            const double *w, const double *X, double *C, hipStream_t s);
 
@@ -38,6 +67,16 @@ void launch_near_apply(const NearTiles &nf, const double *vals, const double *x,
                        double *y, int ldy, int nr, hipStream_t s);
 void launch_near_apply(const NearTiles &nf, const float *vals, const float *x, int ldx,
                        double *y, int ldy, int nr, hipStream_t s);
+// Transposed tiles: per leaf S (CSR tr.off), its near leaves T with the
+// value offset and row stride of block (T, S) inside T's tile.
+struct NearTranspose {
+  const int *off = nullptr, *leaf = nullptr, *stride = nullptr;
+  const int64_t *base = nullptr;
+};
+void launch_near_apply_tr(const NearTiles &nf, const NearTranspose &tr, const double *vals,
+                          const double *x, int ldx, double *y, int ldy, int nr, hipStream_t s);
+void launch_near_apply_tr(const NearTiles &nf, const NearTranspose &tr, const float *vals,
+                          const float *x, int ldx, double *y, int ldy, int nr, hipStream_t s);
 
 // ---------------------------------------------------------------- far field
 template <typename T>
@@ -93,6 +132,10 @@ void kr_scale_cols(const double *in, TO *out, const double *scale, const int *ma
                    int ld, int nr, hipStream_t s);
 void kr_axpy(double *x, const double *y, double alpha, int N, int ld, int nr, hipStream_t s);
 void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s);
+void kr_weight_rhs(const uint32_t *cond, const double *w, int j0, int N, int ld, int nr, double *b,
+                   hipStream_t s);
+void kr_rows_axpy_clear(double *y, double *t, const double *w, int N, int ld, int nr,
+                        hipStream_t s);
 void kr_sub(const double *a, const double *b, double *out, int N, int ld, int nr, hipStream_t s);
 // TI -> T: (double, double), (double, float), (float, float)
 template <typename TI, typename T>

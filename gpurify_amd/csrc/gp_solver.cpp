@@ -19,6 +19,7 @@
 #include "gp_solver.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -56,6 +57,12 @@ bool log_enabled() {
   } while (0)
 
 constexpr double kMinPositive = 2.2250738585072014e-308;
+
+// Sensitivity parameters evaluated together when they move the same panels.
+#ifndef GP_SENS_GROUP
+#define GP_SENS_GROUP 2
+#endif
+constexpr int kSensGroup = GP_SENS_GROUP;
 
 // Chebyshev roots (chebyshev.rs:11-17) and 1-D interpolation weights
 // S_n(x, x_m) (chebyshev.rs:40-74).
@@ -228,6 +235,12 @@ struct FmmSystem {
   DBuf<float> z32, M32, Lc32;
   DBuf<uint16_t> Mbf;
   long matvecs = 0;
+  // Transposed operator P'^T (adjoint solves): set want_transpose before setup.
+  bool want_transpose = false, transposed = false;
+  DBuf<int> tr_off, tr_leaf, tr_stride;
+  DBuf<int64_t> tr_base;
+  DBuf<double> dinv_area, inv_area, ytmp;
+  NearTranspose ntr;
 
   const Tree &t() const { return trees[0]; }
   double L0() const { return L0s[0]; }
@@ -349,13 +362,16 @@ struct FmmSystem {
     // global leaf: its panels against the panels of its near leaves.
     std::vector<int> hcol_off(n_leaves + 1, 0), hcols, hpl(N);
     std::vector<int64_t> hnf(n_leaves + 1, 0);
+    std::vector<std::array<int, 3>> trent;  // (S, T, column start of S in T's tile)
     for (int T = 0; T < n_leaves; ++T) {
       const int gbox = hleaf_box[T];
       const int p = gp_of[gbox];
       const Tree &t = trees[p];
       const int lt = gb_of[gbox] - t.lvl_off[t.L];
+      const int cstart = (int)hcols.size();
       for (int q = t.near_off[lt]; q < t.near_off[lt + 1]; ++q) {
         const int S = t.near_leaf[q];
+        if (want_transpose) trent.push_back({leaf_first[p] + S, T, (int)hcols.size() - cstart});
         for (int k = t.leaf_lo[S]; k < t.leaf_lo[S + 1]; ++k) hcols.push_back(panel_off[p] + k);
       }
       hcol_off[T + 1] = (int)hcols.size();
@@ -363,8 +379,30 @@ struct FmmSystem {
       hnf[T + 1] = hnf[T] + (int64_t)nt_ * (hcol_off[T + 1] - hcol_off[T]);
       for (int k = hleaf_lo[T]; k < hleaf_lo[T + 1]; ++k) hpl[k] = T;
     }
-    (void)leaf_first;
     nnz = hnf[n_leaves];
+    if (want_transpose) {
+      std::stable_sort(trent.begin(), trent.end(),
+                       [](const std::array<int, 3> &a, const std::array<int, 3> &b) {
+                         return a[0] < b[0];
+                       });
+      std::vector<int> ho(n_leaves + 1, 0), hl, hs;
+      std::vector<int64_t> hbase;
+      for (auto &e : trent) {
+        ho[e[0] + 1]++;
+        hl.push_back(e[1]);
+        hbase.push_back(hnf[e[1]] + e[2]);
+        hs.push_back(hcol_off[e[1] + 1] - hcol_off[e[1]]);
+      }
+      for (int T = 0; T < n_leaves; ++T) ho[T + 1] += ho[T];
+      tr_off = to_device(ho, s);
+      tr_leaf = to_device(hl, s);
+      tr_base = to_device(hbase, s);
+      tr_stride = to_device(hs, s);
+      ntr.off = tr_off.p;
+      ntr.leaf = tr_leaf.p;
+      ntr.base = tr_base.p;
+      ntr.stride = tr_stride.p;
+    }
     leaf_lo = to_device(hleaf_lo, s);
     leaf_box = to_device(hleaf_box, s);
     col_off = to_device(hcol_off, s);
@@ -382,6 +420,19 @@ struct FmmSystem {
       const int o0 = panel_off[p];
       launch_self_inv_diag(probs[p]->n, verts.p + (size_t)o0 * 12, cen.p + (size_t)o0 * 3,
                            k_e * L0s[p], inv_diag.p + o0, s);
+    }
+    if (want_transpose) {
+      // far-field transpose: P'_far^T x = area * K (x / area), K symmetric
+      std::vector<double> hdi(N);
+      GP_HIP(hipMemcpyAsync(hdi.data(), inv_diag.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+      GP_HIP(hipStreamSynchronize(s));
+      std::vector<double> hia(N);
+      for (int i = 0; i < N; ++i) {
+        hdi[i] /= ha[i];
+        hia[i] = 1.0 / ha[i];
+      }
+      dinv_area = to_device(hdi, s);
+      inv_area = to_device(hia, s);
     }
     nt.n = N;
     nt.nnz = (long)nnz;
@@ -559,6 +610,10 @@ struct FmmSystem {
     const size_t lexp = (size_t)f64.n_boxes * n * n * n * ld;
     z64.alloc(vec);
     z64.zero(s);
+    if (want_transpose) {
+      ytmp.alloc(vec);
+      ytmp.zero(s);
+    }
     M64.alloc(mexp);
     M64.zero(s);
     Lc64.alloc(lexp);
@@ -580,6 +635,10 @@ struct FmmSystem {
   void apply(const TV *v, bool precond, int nr, double *y, int prec) {
     const double *dinv = precond ? inv_diag.p : nullptr;
     ++matvecs;
+    if (transposed) {
+      apply_transposed(v, precond, nr, y, prec);
+      return;
+    }
     if (prec == 0) {
       if constexpr (sizeof(TV) != 8) throw Error(GP_ERR_INTERNAL, "FP64 operator on an FP32 basis");
       else kr_precond_cast(v, dinv, z64.p, N, ld, nr, s);
@@ -594,6 +653,34 @@ struct FmmSystem {
                                                        : GP_M2L_FP32;
       run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, v, y, ld, s);
     }
+  }
+
+  // y = P'^T M^-1 v: transposed near tiles, and the symmetric far kernel K
+  // between area scalings (P'_far = K diag(area)).  The Jacobi preconditioner
+  // is the same diagonal.
+  template <typename TV>
+  void apply_transposed(const TV *v, bool precond, int nr, double *y, int prec) {
+    const double *dinv = precond ? inv_diag.p : nullptr;
+    // dinv / area, or 1 / area without the preconditioner
+    const double *dfar = precond ? dinv_area.p : inv_area.p;
+    if (prec == 0) {
+      if constexpr (sizeof(TV) != 8) throw Error(GP_ERR_INTERNAL, "FP64 operator on an FP32 basis");
+      else {
+        kr_precond_cast(v, dinv, z64.p, N, ld, nr, s);
+        launch_near_apply_tr(nt, ntr, nf64.p, z64.p, ld, y, ld, nr, s);
+        kr_precond_cast(v, dfar, z64.p, N, ld, nr, s);
+      }
+      run_far_field(f64, z64.p, ld, nr, M64.p, Lc64.p, ld, m2l_outer, ytmp.p, ld, s);
+    } else {
+      kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
+      launch_near_apply_tr(nt, ntr, nf32.p, z32.p, ld, y, ld, nr, s);
+      kr_precond_cast(v, dfar, z32.p, N, ld, nr, s);
+      const int m = (m2l_inner == GP_M2L_BF16 && Mbf.p) ? GP_M2L_BF16
+                    : m2l_inner == GP_M2L_BF16_VALU    ? GP_M2L_BF16_VALU
+                                                       : GP_M2L_FP32;
+      run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, m, ytmp.p, ld, s);
+    }
+    kr_rows_axpy_clear(y, ytmp.p, area.p, N, ld, nr, s);
   }
 };
 
@@ -1128,6 +1215,239 @@ void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   for (size_t e = 0; e < (size_t)g.ncond * g.ncond; ++e)
     if (!std::isfinite(C[e])) throw Error(GP_ERR_NON_FINITE, "solver returned a non-finite capacitance");
   if (rep) rep->seconds_total = now_s() - t0;
+}
+
+void cap_jacobian(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, const ParamMap &pm,
+                  double *C, double *J, gp_cap_report *rep) {
+  const double t0 = now_s();
+  hipStream_t s = ctx->stream;
+  if (!(o.tol > 0.0) || !std::isfinite(o.tol)) throw Error(GP_ERR_INVALID_INPUT, "tol must be > 0");
+  if (o.precision != GP_PREC_FP64 && o.precision != GP_PREC_MIXED)
+    throw Error(GP_ERR_INVALID_INPUT, "unknown precision");
+  const int N = g.n, nc = g.ncond;
+  if (nc > 128) throw Error(GP_ERR_INVALID_INPUT, "sensitivities support at most 128 conductors");
+  // Parameter entries: panels in range, no panel twice within a parameter.
+  const int np = pm.n;
+  const uint32_t ne = np ? pm.off[np] : 0;
+  int nm_max = 0;
+  {
+    std::vector<int> seen(N, -1);
+    for (int p = 0; p < np; ++p) {
+      if (pm.off[p + 1] < pm.off[p]) throw Error(GP_ERR_INVALID_INPUT, "param_off must not decrease");
+      nm_max = std::max(nm_max, (int)(pm.off[p + 1] - pm.off[p]));
+      for (uint32_t e = pm.off[p]; e < pm.off[p + 1]; ++e) {
+        const uint32_t k = pm.panel[e];
+        if (k >= (uint32_t)N) throw Error(GP_ERR_INVALID_INPUT, "parameter panel index out of range");
+        if (seen[k] == p) throw Error(GP_ERR_INVALID_INPUT, "panel listed twice in one parameter");
+        seen[k] = p;
+        for (int c = 0; c < 12; ++c)
+          if (!std::isfinite(pm.dv[12 * (size_t)e + c]))
+            throw Error(GP_ERR_INVALID_INPUT, "non-finite vertex velocity");
+      }
+    }
+  }
+  const uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
+  uint32_t method = o.method;
+  if (method == GP_METHOD_AUTO) method = ((uint32_t)N < limit) ? GP_METHOD_DENSE : GP_METHOD_FMM;
+  if (method != GP_METHOD_DENSE && method != GP_METHOD_FMM)
+    throw Error(GP_ERR_INVALID_INPUT, "unknown method");
+  if (rep) {
+    std::memset(rep, 0, sizeof(*rep));
+    rep->n_panels = (uint32_t)N;
+    rep->n_conductors = (uint32_t)nc;
+    rep->method_used = method;
+    rep->precision_used = method == GP_METHOD_FMM ? o.precision : GP_PREC_FP64;
+  }
+  std::vector<uint32_t> hmoved(ne);
+  double L0 = 1.0;
+  std::vector<double> hC((size_t)nc * nc);
+  // Device data the sensitivity kernels read, set by either path.
+  const double *dverts = nullptr, *dcen = nullptr, *dsig = nullptr, *dlam = nullptr;
+  const uint32_t *dcond = nullptr;
+  int lds = 0;
+  // Dense path storage.
+  DBuf<double> dv_, dc_, dB, dA;
+  DBuf<uint32_t> dcond_;
+  // FMM path storage.
+  FmmSystem sys;
+  DBuf<double> xf, xa;
+  if (method == GP_METHOD_DENSE) {
+    double c[3];
+    root_cube(g, c, L0);
+    std::vector<double> hv, hc, ha;
+    std::vector<uint32_t> hcond;
+    scaled_arrays(g, nullptr, c, L0, hv, hc, ha, hcond);
+    std::vector<double> B((size_t)N * nc, 0.0), A((size_t)N * nc, 0.0);
+    for (int i = 0; i < N; ++i) {
+      B[(size_t)i * nc + g.cond[i]] = 1.0;
+      A[(size_t)i * nc + g.cond[i]] = ha[i];
+    }
+    dv_ = to_device(hv, s);
+    dc_ = to_device(hc, s);
+    dcond_ = to_device(hcond, s);
+    dB = to_device(B, s);
+    dA = to_device(A, s);
+    auto d_w = to_device(ha, s);
+    std::vector<int> coff, cidx;
+    cond_csr(g, nullptr, coff, cidx);
+    auto d_coff = to_device(coff, s);
+    auto d_cidx = to_device(cidx, s);
+    DBuf<double> d_P((size_t)N * N), d_PT((size_t)N * N), d_work((size_t)N * nc), d_C((size_t)nc * nc);
+    DBuf<int> d_piv(N), d_info(2);
+    launch_dense_assemble(N, dv_.p, dc_.p, d_P.p, s);
+    launch_transpose(N, d_P.p, d_PT.p, s);
+    launch_lu_solve(N, nc, d_P.p, d_piv.p, dB.p, d_work.p, d_info.p, s);
+    launch_lu_solve(N, nc, d_PT.p, d_piv.p, dA.p, d_work.p, d_info.p + 1, s);
+    launch_cap_accumulate(nc, nc, nc, d_coff.p, d_cidx.p, d_w.p, dB.p, d_C.p, s);
+    int info[2] = {0, 0};
+    GP_HIP(hipMemcpyAsync(info, d_info.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipMemcpyAsync(hC.data(), d_C.p, hC.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    if (info[0] || info[1]) throw Error(GP_ERR_SINGULAR, "potential-coefficient matrix is singular");
+    for (uint32_t e = 0; e < ne; ++e) hmoved[e] = pm.panel[e];
+    dverts = dv_.p;
+    dcen = dc_.p;
+    dcond = dcond_.p;
+    dsig = dB.p;
+    dlam = dA.p;
+    lds = nc;
+    if (rep) rep->converged = 1;
+  } else {
+    const bool mixed = o.precision == GP_PREC_MIXED;
+    sys.want_transpose = true;
+    sys.setup(g, o, mixed, s);
+    L0 = sys.L0();
+    const int ld = nc == 1 ? 1 : round_up(nc, 16);
+    sys.ensure_work(ld);
+    const size_t vec = (size_t)N * ld;
+    DBuf<double> b(vec), dCb((size_t)nc * ld);
+    xf.alloc(vec);
+    xa.alloc(vec);
+    xf.zero(s);
+    xa.zero(s);
+    BlockOps ops;
+    ops.N = N;
+    ops.ld = ld;
+    ops.nr = nc;
+    ops.s = s;
+    const std::vector<int> live(nc, 1);
+    SolveStats sf, sa;
+    kr_unit_rhs(sys.cond.p, 0, N, ld, nc, b.p, s);
+    solve_systems(sys, ops, o, mixed, b.p, xf.p, live, sf);
+    // adjoint: P'^T lambda_i = scaled areas of conductor i
+    sys.transposed = true;
+    kr_weight_rhs(sys.cond.p, sys.area.p, 0, N, ld, nc, b.p, s);
+    solve_systems(sys, ops, o, mixed, b.p, xa.p, live, sa);
+    sys.transposed = false;
+    launch_cap_accumulate(nc, nc, ld, sys.coff.p, sys.cidx.p, sys.area.p, xf.p, dCb.p, s);
+    GP_HIP(hipMemcpyAsync(hC.data(), dCb.p, hC.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    std::vector<int> pos(N);
+    for (int k = 0; k < N; ++k) pos[sys.t().perm[k]] = k;
+    for (uint32_t e = 0; e < ne; ++e) hmoved[e] = (uint32_t)pos[pm.panel[e]];
+    dverts = sys.verts.p;
+    dcen = sys.cen.p;
+    dcond = sys.cond.p;
+    dsig = xf.p;
+    dlam = xa.p;
+    lds = ld;
+    if (rep) {
+      bool conv = true;
+      double worst = 0.0;
+      int its = 0;
+      for (int q = 0; q < nc; ++q) {
+        conv &= sf.conv[q] && sa.conv[q];
+        worst = std::max(worst, std::max(sf.resid[q], sa.resid[q]));
+        its = std::max(its, std::max(sf.iters[q], sa.iters[q]));
+      }
+      rep->converged = conv;
+      rep->final_rel_residual = worst;
+      rep->iterations = (uint32_t)its;
+      rep->refine_steps = (uint32_t)std::max(sf.refine_steps, sa.refine_steps);
+      rep->matvecs = (uint32_t)sys.matvecs;
+      rep->max_level = (uint32_t)sys.L;
+      rep->n_leaves = (uint32_t)sys.n_leaves;
+      rep->n_boxes = (uint32_t)sys.f64.n_boxes;
+      rep->fmm_order = (uint32_t)sys.n;
+      rep->near_nnz = (uint64_t)sys.nnz;
+      rep->m2l_pairs = (uint64_t)sys.pairs;
+    }
+  }
+  const double t1 = now_s();
+  for (size_t e = 0; e < hC.size(); ++e) C[e] = hC[e] * 4.0 * kPi * kEps0 * L0;
+  if (np > 0) {
+    std::vector<int> hm(hmoved.begin(), hmoved.end());
+    std::vector<double> hdv(pm.dv, pm.dv + 12 * (size_t)ne);
+    auto d_moved = to_device(hm, s);
+    auto d_dv = to_device(hdv, s);
+    DBuf<int> mv_of(N);
+    const int nmx = std::max(nm_max, 1);
+    DBuf<double> da(N), R((size_t)2 * N * nc), dcn((size_t)6 * nmx), dar((size_t)2 * nmx),
+        scratch(sensitivity_scratch(N, nmx, nc, 2)), dJ((size_t)np * nc * nc), gq((size_t)20 * N),
+        dgq((size_t)32 * nmx);
+    launch_gauss_table(N, dverts, gq.p, s);
+    // The dense path differentiates every entry in closed form; the FMM path
+    // takes the Gauss rule beyond 6 source diagonals (relative entry error
+    // below 4e-6, far under the FMM's own).
+    const double far2 = method == GP_METHOD_FMM ? 36.0 : 0.0;
+    GP_HIP(hipMemsetAsync(mv_of.p, 0xff, N * sizeof(int), s));
+    da.zero(s);
+    R.zero(s);
+    // Parameters that move the same panels (consecutive, e.g. the width,
+    // spacing and height of one wire) share their integral evaluations.
+    for (int p = 0; p < np;) {
+      const uint32_t o0 = pm.off[p], nm = pm.off[p + 1] - o0;
+      int nd = 1;
+      while (nd < kSensGroup && p + nd < np && pm.off[p + nd + 1] - pm.off[p + nd] == nm &&
+             std::equal(pm.panel + o0, pm.panel + o0 + nm, pm.panel + pm.off[p + nd]))
+        ++nd;
+      SensArgs a;
+      a.N = N;
+      a.nm = (int)nm;
+      a.nc = nc;
+      a.nd = nd;
+      a.lds = lds;
+      a.ldl = lds;
+      a.cond0 = 0;
+      a.moved = d_moved.p + o0;
+      a.dv = d_dv.p + 12 * (size_t)o0;
+      a.dstride = 12L * nm;
+      a.verts = dverts;
+      a.cen = dcen;
+      a.sigma = dsig;
+      a.lambda = dlam;
+      a.cond = dcond;
+      a.mv_of = mv_of.p;
+      a.da = da.p;
+      a.dcen = dcn.p;
+      a.darea = dar.p;
+      a.R = R.p;
+      a.scratch = scratch.p;
+      a.gq = gq.p;
+      a.dgq = dgq.p;
+      a.far2 = far2;
+      a.J = dJ.p + (size_t)p * nc * nc;
+      launch_sensitivity(a, s);
+      p += nd;
+    }
+    std::vector<double> hJ((size_t)np * nc * nc);
+    GP_HIP(hipMemcpyAsync(hJ.data(), dJ.p, hJ.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    // dC/dtheta = 4 pi eps0 dC'/dtheta' (theta' = theta / L0)
+    for (size_t e = 0; e < hJ.size(); ++e) J[e] = hJ[e] * 4.0 * kPi * kEps0;
+  }
+  const double t2 = now_s();
+  for (size_t e = 0; e < hC.size(); ++e)
+    if (!std::isfinite(C[e])) throw Error(GP_ERR_NON_FINITE, "solver returned a non-finite capacitance");
+  for (size_t e = 0; e < (size_t)np * nc * nc; ++e)
+    if (!std::isfinite(J[e])) throw Error(GP_ERR_NON_FINITE, "non-finite sensitivity");
+  if (rep) {
+    rep->seconds_total = t2 - t0;
+    rep->seconds_solve = t1 - t0;
+    rep->seconds_setup = 0.0;
+  }
+  if (rep && !rep->converged && o.strict)
+    throw Error(GP_ERR_NOT_CONVERGED, "GMRES did not reach tol in the forward or adjoint solve");
 }
 
 void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_cap_opts &o,

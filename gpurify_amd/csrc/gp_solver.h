@@ -37,6 +37,20 @@ void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, double 
 void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_cap_opts &opts,
                        const std::vector<double *> &C, gp_cap_report *reps);
 
+// Geometric parameters: parameter p moves the caller's panels
+// panel[off[p] .. off[p+1]) with vertex velocities dv (12 per entry, metres
+// per metre of theta).
+struct ParamMap {
+  int n = 0;
+  const uint32_t *off = nullptr, *panel = nullptr;
+  const double *dv = nullptr;
+};
+// C (farads) and J = dC/dtheta (n x ncond x ncond, F/m) by one forward and
+// one adjoint (transposed) solve per conductor and forward-mode derivatives
+// of the closed-form integrals of every entry a moved panel touches.
+void cap_jacobian(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, const ParamMap &pm,
+                  double *C, double *J, gp_cap_report *rep);
+
 // Apply the precorrected FMM operator (FP64, no preconditioner) to nr
 // columns X (n x nr, caller order), y = P X, farads^-1 m^-2 units of P.
 // Test entry used to compare the device operator with the oracle's.

@@ -149,6 +149,35 @@ int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
 int gp_cap_solve_batched(gp_ctx *ctx, const gp_panels *problems, uint32_t n_problems,
                          const gp_cap_opts *opts, double *C, gp_cap_report *reports);
 
+/* Geometric parameters for sensitivities.  Parameter p moves the panels
+ * panel[param_off[p] .. param_off[p+1]) (caller indices, each at most once
+ * per parameter) with vertex velocities dverts (12 doubles per entry, the
+ * d vertex / d theta of the panel's 4 vertices, metres per metre of theta);
+ * all other panels stay.  A width change stretches the side and cross
+ * faces, a spacing or height change translates a conductor. */
+typedef struct {
+  uint32_t n_params;
+  const uint32_t *param_off;  /* n_params + 1 */
+  const uint32_t *panel;      /* param_off[n_params] entries */
+  const double *dverts;       /* 12 * param_off[n_params] */
+} gp_param_map;
+
+/* C and its Jacobian dC/dtheta (no reference counterpart; the reference
+ * has no adjoint: the forward half of a forward/backward pair modelled on
+ * torch.autograd.Function, whose backward is dL/dtheta_p = sum_ij
+ * dL/dC_ij J[p][i][j]).  One forward solve P sigma_j = e_j and one adjoint
+ * solve P^T lambda_i = a_i (a_i: areas on conductor i) per conductor, then
+ *   dC_ij/dtheta = sum_(k on i) (da_k/dtheta) sigma_kj
+ *                  - sum_(k,l) lambda_ki (dP_kl/dtheta) sigma_lj
+ * with dP_kl the exact forward-mode derivative of the closed-form integral
+ * (collocation point and panel both moving) over every entry a moved panel
+ * touches.  Method per opts (dense below dense_limit, else FMM with the
+ * transposed operator for the adjoint).  C: n_conductors^2 (F, vacuum);
+ * J: n_params * n_conductors^2 (F/m).  The dielectric k scales C (bridge.rs:
+ * 405-407): dC/dk = C / k needs no solve.  report may be NULL. */
+int gp_cap_jacobian(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts,
+                    const gp_param_map *params, double *C, double *J, gp_cap_report *report);
+
 /* The precorrected FMM operator apply y = P x of FmmCapOperator::apply
  * (cap/fmm_solver.rs:105-130): exact-Wilton near tiles plus the far field of
  * the panel charges area * x, built as gp_cap_solve builds it for

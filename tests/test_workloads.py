@@ -1,6 +1,7 @@
 """The seeded synthetic layouts: deterministic, well-formed (no two nets
 overlap on one layer, so no coincident panels), and of the advertised size."""
 import numpy as np
+import pytest
 
 import gpurify_amd as G
 from gpurify_amd import workloads as W
@@ -33,3 +34,21 @@ def test_routing_block_full_size_panel_count():
     v, c, _ = G.mesh_groups(g, refine=3, max_panels=10 ** 9)
     assert 1.8e6 < len(v) < 2.8e6  # ~2M panels
     assert int(c.max()) == len(g)  # the substrate plate is the last conductor
+
+
+def test_wire_params_velocities():
+    from gpurify_amd import sensitivity as S
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=2)
+    params, names = S.wire_params(v, c, [0, 1])
+    assert names[:3] == [(0, "width"), (0, "spacing"), (0, "height")]
+    idx, dv = params[0]
+    assert set(np.unique(c[idx])) == {0}
+    pts = v[idx]
+    y = pts[:, :, 1]
+    # the two side faces move apart by 1/2 each per unit width
+    assert np.allclose(dv[:, :, 1][np.isclose(y, y.max())], 0.5)
+    assert np.allclose(dv[:, :, 1][np.isclose(y, y.min())], -0.5)
+    assert np.all(dv[:, :, [0, 2]] == 0.0)
+    moved = S.move(v, params, np.eye(len(params))[0] * 2e-9)
+    w0 = np.ptp(v[idx][:, :, 1])
+    assert np.ptp(moved[idx][:, :, 1]) == pytest.approx(w0 + 2e-9, rel=1e-9)
