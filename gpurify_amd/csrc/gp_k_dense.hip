@@ -17,11 +17,9 @@ constexpr int kLuThreads = 1024;
 // reference's piv vector), B: n x nrhs row-major (overwritten by the solution
 // of A X = B).  info[0] = 0, or k+1 when column k has no non-zero pivot
 // (LinalgError::Singular, linalg.rs:191-193).
-__global__ __launch_bounds__(kLuThreads) void k_lu_solve(int n, int nrhs, double *__restrict__ A,
-                                                         int *__restrict__ piv,
-                                                         double *__restrict__ B,
-                                                         double *__restrict__ work,
-                                                         int *__restrict__ info) {
+__device__ void lu_solve_block(int n, int nrhs, double *__restrict__ A, int *__restrict__ piv,
+                               double *__restrict__ B, double *__restrict__ work,
+                               int *__restrict__ info) {
   __shared__ double red_v[kLuThreads / 64];
   __shared__ int red_i[kLuThreads / 64];
   __shared__ int s_p;
@@ -125,6 +123,27 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_solve(int n, int nrhs, double
   if (tid == 0) info[0] = 0;
 }
 
+__global__ __launch_bounds__(kLuThreads) void k_lu_solve(int n, int nrhs, double *__restrict__ A,
+                                                         int *__restrict__ piv,
+                                                         double *__restrict__ B,
+                                                         double *__restrict__ work,
+                                                         int *__restrict__ info) {
+  lu_solve_block(n, nrhs, A, piv, B, work, info);
+}
+
+// A batch of independent dense solves, one work-group per problem p:
+// n[p] x n[p] at A + a_off[p], n[p] x nrhs[p] at B + b_off[p] (work alike),
+// pivots at piv + n_off[p], status info[p].
+__global__ __launch_bounds__(kLuThreads) void k_lu_solve_batched(
+    const int *__restrict__ n, const int *__restrict__ nrhs, const long *__restrict__ a_off,
+    const long *__restrict__ b_off, const int *__restrict__ n_off, double *__restrict__ A,
+    int *__restrict__ piv, double *__restrict__ B, double *__restrict__ work,
+    int *__restrict__ info) {
+  const int p = blockIdx.x;
+  lu_solve_block(n[p], nrhs[p], A + a_off[p], piv + n_off[p], B + b_off[p], work + b_off[p],
+                 info + p);
+}
+
 // C'[i][j] = sum over panels k of conductor i of w_k * X[k][j] (accumulate_c,
 // cap/solver.rs:67-71; with per-panel eps weights for the conductor-role
 // dielectric solve, cap/dielectric.rs:139-144).  One block per conductor i;
@@ -147,22 +166,12 @@ __global__ __launch_bounds__(256) void k_cap_accumulate(int ncond, int nrhs, int
   }
 }
 
-// AT = A^T (n x n, row-major), 32 x 32 tiles through LDS.
-__global__ __launch_bounds__(256) void k_transpose(int n, const double *__restrict__ A,
-                                                   double *__restrict__ AT) {
-  __shared__ double t[32][33];
-  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int y = ty; y < 32; y += 8)
-    if (by + y < n && bx + tx < n) t[y][tx] = A[(long)(by + y) * n + bx + tx];
-  __syncthreads();
-  for (int y = ty; y < 32; y += 8)
-    if (bx + y < n && by + tx < n) AT[(long)(bx + y) * n + by + tx] = t[tx][y];
-}
-
-void launch_transpose(int n, const double *A, double *AT, hipStream_t s) {
-  dim3 grid((n + 31) / 32, (n + 31) / 32);
-  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, n, A, AT);
+void launch_dense_batch(const DenseBatch &b, hipStream_t s) {
+  if (b.count == 0) return;
+  launch_dense_assemble_batched(b, s);
+  ProfScope ps("dense_lu_solve", s, 0.0, 0.0);
+  hipLaunchKernelGGL(k_lu_solve_batched, dim3(b.count), dim3(kLuThreads), 0, s, b.n, b.nrhs,
+                     b.a_off, b.b_off, b.n_off, b.A, b.piv, b.B, b.work, b.info);
   GP_LAUNCH_CHECK();
 }
 

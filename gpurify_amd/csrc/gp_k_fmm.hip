@@ -430,6 +430,184 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
       }
 }
 
+// Single right-hand side (ldm = 1) P2M: several leaves per work-group, one
+// thread per (leaf, node), each summing its leaf's panels in order (the
+// same expression and order as k_p2m).
+template <typename T>
+__global__ __launch_bounds__(256) void k_p2m1(int n, int n3p, int tpl, int nleaf,
+                                              const int *__restrict__ leaf_lo,
+                                              const int *__restrict__ leaf_box, int leaf_box0,
+                                              const T *__restrict__ w1d,
+                                              const double *__restrict__ area,
+                                              const T *__restrict__ z, T *__restrict__ M,
+                                              __bf16 *__restrict__ Mb) {
+  const int li = blockIdx.x * (256 / tpl) + threadIdx.x / tpl, m = threadIdx.x % tpl;
+  const int n3 = n * n * n;
+  if (li >= nleaf || m >= n3) return;
+  const int a = m / (n * n), b = (m / n) % n, c = m % n;
+  T acc = 0;
+  for (int p = leaf_lo[li]; p < leaf_lo[li + 1]; ++p) {
+    const T *w = w1d + (size_t)p * 3 * n;
+    const T q = (T)area[p] * z[p];
+    acc += w[a] * w[n + b] * w[2 * n + c] * q;
+  }
+  const int box = leaf_box ? leaf_box[li] : leaf_box0 + li;
+  store_mult(M, Mb, (size_t)box * n3p + m, acc);
+}
+
+// Single right-hand side L2P: one thread per panel, y[p] += sum over nodes of
+// S(p, node) Lc[leaf][node] (the same expression and order as k_l2p).
+template <typename T>
+__global__ __launch_bounds__(256) void k_l2p1(int n, int N, const int *__restrict__ panel_leaf,
+                                              const int *__restrict__ leaf_box, int leaf_box0,
+                                              const T *__restrict__ w1d, const T *__restrict__ Lc,
+                                              double *__restrict__ y) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int li = panel_leaf[p], nn = n * n;
+  const T *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * n * nn;
+  const T *w = w1d + (size_t)p * 3 * n;
+  T acc = 0;
+  for (int a = 0; a < n; ++a) {
+    T sa = 0;
+    for (int bb = 0; bb < n; ++bb) {
+      T sb = 0;
+      for (int c = 0; c < n; ++c) sb += w[2 * n + c] * ll[a * nn + bb * n + c];
+      sa += w[n + bb] * sb;
+    }
+    acc += w[a] * sa;
+  }
+  y[p] += (double)acc;
+}
+
+// Single right-hand side (ldm = 1): M2L grouped by offset.  One launch per
+// M2L offset o covers every (target, source) pair with that offset (at most
+// one per target, so the updates of one launch never collide and the sum
+// order over offsets is fixed): D (node x pair) = G_o (node x l) *
+// [M_s1 .. M_s16] (l x pair) on the matrix cores, 16 pairs per wavefront,
+// then Lc[t] += D[:, pair] / half.  G_o fragments (A operand) come from the
+// L2-resident table, the source multipoles (B operand) from HBM.
+template <int MT>
+__global__ __launch_bounds__(256) void k_m2l1_bf16(int n3, int npairs, const int *__restrict__ tgt,
+                                                   const int *__restrict__ src,
+                                                   const int *__restrict__ box_level,
+                                                   const double *__restrict__ inv_half,
+                                                   const __bf16 *__restrict__ Go,
+                                                   const __bf16 *__restrict__ Mb,
+                                                   float *__restrict__ Lc) {
+  constexpr int NP = 16 * MT, KS = NP / 32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j0 = (blockIdx.x * 4 + wave) * 16;
+  if (j0 >= npairs) return;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  const int jp = min(j0 + li, npairs - 1);  // pair of this lane's B column
+  const __bf16 *Ms = Mb + (size_t)src[jp] * NP;
+  f32x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const bf16x8 b = *(const bf16x8 *)(Ms + k * 32 + lk);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const bf16x8 a = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * NP + k * 32 + lk);
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt], 0, 0, 0);
+    }
+  }
+  // D[i = node][j = pair]: lane holds nodes 4 (lane / 16) + v of pair lane % 16
+  const int j = j0 + li;
+  if (j < npairs) {
+    const int t = tgt[j];
+    const float ih = (float)inv_half[box_level[t]];
+    float *out = Lc + (size_t)t * n3;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = mt * 16 + (lane >> 4) * 4 + v;
+        if (m < n3) out[m] += acc[mt][v] * ih;
+      }
+  }
+}
+
+// FP64 counterpart (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16 k, B[k][j]
+// in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), register i / 4).
+template <int MT>
+__global__ __launch_bounds__(256) void k_m2l1_f64(int n3, int npairs, const int *__restrict__ tgt,
+                                                  const int *__restrict__ src,
+                                                  const int *__restrict__ box_level,
+                                                  const double *__restrict__ inv_half,
+                                                  const double *__restrict__ Go,
+                                                  const double *__restrict__ M,
+                                                  double *__restrict__ Lc) {
+  constexpr int NP = 16 * MT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j0 = (blockIdx.x * 4 + wave) * 16;
+  if (j0 >= npairs) return;
+  const int li = lane & 15, lq = lane >> 4;
+  const int jp = min(j0 + li, npairs - 1);
+  const double *Ms = M + (size_t)src[jp] * NP;
+  f64x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < NP; k0 += 4) {
+    const double b = Ms[k0 + lq];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const double a = Go[(size_t)(mt * 16 + li) * NP + k0 + lq];
+      acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[mt], 0, 0, 0);
+    }
+  }
+  const int j = j0 + li;
+  if (j < npairs) {
+    const int t = tgt[j];
+    const double ih = inv_half[box_level[t]];
+    double *out = Lc + (size_t)t * n3;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = mt * 16 + lq + 4 * v;
+        if (m < n3) out[m] += acc[mt][v] * ih;
+      }
+  }
+}
+
+template <typename T>
+void launch_m2l_by_offset(const FmmDev<T> &f, int n3, const void *Mop, T *Lc, bool bf16,
+                          hipStream_t s) {
+  const int n3p = f.n3p, MT = n3p / 16;
+  for (int o = 0; o < 343; ++o) {
+    const int p0 = f.po_off[o], np = f.po_off[o + 1] - p0;
+    if (np == 0) continue;
+    const dim3 grid((np + 63) / 64);
+    const int *tg = f.po_tgt + p0, *sr = f.po_src + p0;
+    if constexpr (sizeof(T) == 4) {
+      if (!bf16) throw Error(GP_ERR_INTERNAL, "single-RHS M2L by offset runs BF16 or FP64");
+      const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf) + (size_t)o * n3p * n3p;
+      const __bf16 *Mb = reinterpret_cast<const __bf16 *>(Mop);
+      switch (MT) {
+        case 2: hipLaunchKernelGGL((k_m2l1_bf16<2>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
+        case 4: hipLaunchKernelGGL((k_m2l1_bf16<4>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
+        case 8: hipLaunchKernelGGL((k_m2l1_bf16<8>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
+        case 14: hipLaunchKernelGGL((k_m2l1_bf16<14>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
+        default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
+      }
+    } else {
+      const double *G = f.G + (size_t)o * n3p * n3p;
+      const double *M = reinterpret_cast<const double *>(Mop);
+      switch (MT) {
+        case 2: hipLaunchKernelGGL((k_m2l1_f64<2>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+        case 4: hipLaunchKernelGGL((k_m2l1_f64<4>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+        case 8: hipLaunchKernelGGL((k_m2l1_f64<8>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+        case 14: hipLaunchKernelGGL((k_m2l1_f64<14>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+        default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
+      }
+    }
+    GP_LAUNCH_CHECK();
+  }
+}
+
 template <int MT, int RTM>
 void launch_m2l_bf16(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb, float *Lc,
                      hipStream_t s) {
@@ -874,7 +1052,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   const bool f32 = sizeof(T) == 4;
   const double exp_bytes = (double)n3 * nr * sizeof(T);
   const bool want_bf16 = (m2l == GP_M2L_BF16);
-  if (want_bf16 && (!Mb || !f.Gbf || ldm % 16 || ldm > 128 || n3p % 32))
+  const bool by_offset = ldm == 1 && nr == 1 && f.po_off != nullptr;
+  if (want_bf16 && (!Mb || !f.Gbf || n3p % 32 || (!by_offset && (ldm % 16 || ldm > 128))))
     throw Error(GP_ERR_INTERNAL, "BF16 M2L needs 16 <= ld <= 128 (multiple of 16) and its tables");
   __bf16 *mb = want_bf16 ? Mb : nullptr;
   // Register-tiled kernels for RHS blocks that are a multiple of 16.
@@ -883,7 +1062,11 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   {
     ProfScope ps(f32 ? "p2m_f32" : "p2m_f64", s, 2.0 * f.n_panels * n3 * nr,
                  (double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes);
-    if (tiled) {
+    if (ldm == 1 && nr == 1 && ldz == 1) {
+      const int tpl = round_up(n3, 64), lpb = 256 / tpl;
+      hipLaunchKernelGGL((k_p2m1<T>), dim3((nleaf + lpb - 1) / lpb), dim3(256), 0, s, n, n3p, tpl,
+                         nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb);
+    } else if (tiled) {
       const int MB = n3p / 16;
       if (MB <= 2)
         hipLaunchKernelGGL((k_p2m_tiled<T, 2>), dim3(nleaf), dim3(256), 0, s, n, n3p, f.leaf_lo,
@@ -949,7 +1132,13 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     const double fl = 2.0 * (double)f.n_pairs * n3 * n3 * nr;
     const double by = (double)(f.n_boxes + f.n_targets) * exp_bytes;
     const int MT = n3p / 16, RT = (nr + 15) / 16;
-    if (want_bf16) {
+    if (want_bf16 && by_offset) {
+      ProfScope pl("m2l1_bf16_mfma", s, fl, (double)f.n_pairs * (n3p * 2 + n3 * 8));
+      launch_m2l_by_offset(f, n3, Mb, Lc, true, s);
+    } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {
+      ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (n3p * 8 + n3 * 16));
+      launch_m2l_by_offset(f, n3, M, Lc, false, s);
+    } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * n3 * nr * 2 + f.n_targets * exp_bytes);
         GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, f, n3, nr, ldm, Mb, Lc, s);
@@ -991,7 +1180,10 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   }
   ProfScope pp(f32 ? "l2p_f32" : "l2p_f64", s, 2.0 * f.n_panels * n3 * nr,
                (double)f.n_panels * nr * 16.0 + nleaf * exp_bytes);
-  if (tiled)
+  if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf)
+    hipLaunchKernelGGL((k_l2p1<T>), dim3((f.n_panels + 255) / 256), dim3(256), 0, s, n, f.n_panels,
+                       f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y);
+  else if (tiled)
     hipLaunchKernelGGL((k_l2p_tiled<T>), dim3(nleaf), dim3(256), 0, s, n, f.leaf_lo, f.leaf_box, leaf0, f.w1d,
                        nr, Lc, ldm, y, ldy);
   else

@@ -281,6 +281,52 @@ __global__ void k_self_inv_diag(int n, const double *__restrict__ verts,
   inv[i] = fabs(d * si_scale) > 1e-30 ? 1.0 / d : 1.0;
 }
 
+// P'_p[k][l] = I_l(c_k) for every problem p of a batch (blockIdx.y), its
+// panels at verts + 12 n_off[p], cen + 3 n_off[p], matrix at P + a_off[p].
+__global__ __launch_bounds__(256) void k_dense_assemble_batched(const int *__restrict__ n,
+                                                                const int *__restrict__ n_off,
+                                                                const long *__restrict__ a_off,
+                                                                const double *__restrict__ verts,
+                                                                const double *__restrict__ cen,
+                                                                double *__restrict__ P) {
+  const int p = blockIdx.y, np = n[p];
+  const double *v = verts + 12L * n_off[p], *c = cen + 3L * n_off[p];
+  double *Pp = P + a_off[p];
+  const long total = (long)np * np;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(e / np), l = (int)(e % np);
+    Pp[e] = wilton(v + 12L * l, c[3 * k], c[3 * k + 1], c[3 * k + 2]);
+  }
+}
+
+
+// AT = A^T (n x n, row-major), 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void k_transpose(int n, const double *__restrict__ A,
+                                                   double *__restrict__ AT) {
+  __shared__ double t[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int y = ty; y < 32; y += 8)
+    if (by + y < n && bx + tx < n) t[y][tx] = A[(long)(by + y) * n + bx + tx];
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8)
+    if (bx + y < n && by + tx < n) AT[(long)(bx + y) * n + by + tx] = t[tx][y];
+}
+
+void launch_transpose(int n, const double *A, double *AT, hipStream_t s) {
+  dim3 grid((n + 31) / 32, (n + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, n, A, AT);
+  GP_LAUNCH_CHECK();
+}
+
+void launch_dense_assemble_batched(const DenseBatch &b, hipStream_t s) {
+  ProfScope ps("dense_assemble", s, 0.0, 0.0);
+  hipLaunchKernelGGL(k_dense_assemble_batched, dim3(64, b.count), dim3(256), 0, s, b.n, b.n_off,
+                     b.a_off, b.verts, b.cen, b.A);
+  GP_LAUNCH_CHECK();
+}
+
 void launch_dense_assemble(int n, const double *verts, const double *cen, double *P,
                            hipStream_t s) {
   long total = (long)n * n;

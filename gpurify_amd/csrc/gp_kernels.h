@@ -49,6 +49,21 @@ void launch_gauss_table(int n, const double *verts, double *gq, hipStream_t s);
 void launch_lu_solve(int n, int nrhs, double *A, int *piv, double *B, double *work, int *info,
                      hipStream_t s);
 void launch_transpose(int n, const double *A, double *AT, hipStream_t s);
+// A batch of independent dense problems (device arrays): problem p has n[p]
+// panels at verts + 12 n_off[p] / cen + 3 n_off[p], its matrix at
+// A + a_off[p], its n[p] x nrhs[p] right-hand sides at B + b_off[p] (work
+// alike), pivots at piv + n_off[p], status info[p].  Assembles every P'_p and
+// solves P'_p X = B_p in place, one work-group per problem.
+struct DenseBatch {
+  int count = 0;
+  const int *n = nullptr, *nrhs = nullptr, *n_off = nullptr;
+  const long *a_off = nullptr, *b_off = nullptr;
+  const double *verts = nullptr, *cen = nullptr;
+  double *A = nullptr, *B = nullptr, *work = nullptr;
+  int *piv = nullptr, *info = nullptr;
+};
+void launch_dense_assemble_batched(const DenseBatch &b, hipStream_t s);
+void launch_dense_batch(const DenseBatch &b, hipStream_t s);
 void launch_cap_accumulate(int ncond, int nrhs, int ldx, const int *coff, const int *cidx,
                            const double *w, const double *X, double *C, hipStream_t s);
 
@@ -96,6 +111,7 @@ struct FmmDev {
   const int *leaf_box = nullptr;
   int n_leaves = 0;
   const int *leaf_lo = nullptr;  // device, n_leaves + 1
+  const int *panel_leaf = nullptr;  // device, per sorted panel: its leaf index
   const int *parent = nullptr, *octant = nullptr, *child_off = nullptr, *child_idx = nullptr;
   const int *targets = nullptr, *il_off = nullptr, *il_src = nullptr, *il_oidx = nullptr;
   const int *box_level = nullptr;
@@ -107,6 +123,11 @@ struct FmmDev {
   const T *G = nullptr;              // 343 * n3p * n3p (zero padded)
   const void *Gbf = nullptr;         // same, BF16 (FP32 operator only)
   const T *Gr = nullptr;             // same, BF16-rounded, widened to T (GP_M2L_BF16_VALU)
+  // Single right-hand side: the M2L pairs grouped by offset index o, pairs
+  // [po_off[o], po_off[o+1]) (HOST, 344 entries) of device lists po_tgt /
+  // po_src; null when not built.
+  const int *po_off = nullptr;
+  const int *po_tgt = nullptr, *po_src = nullptr;
 };
 // Far field y += P_far z.  Multipoles M: n_boxes x ldm x n3p; locals Lc:
 // n_boxes x n3 x ldm.  m2l selects the M2L kernel (GP_M2L_*): BF16 needs Mb

@@ -20,6 +20,11 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <climits>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -36,6 +41,44 @@ namespace {
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
+}
+
+// Host threads for the set-up loops over problems, leaves and boxes:
+// GPURIFY_HOST_THREADS, else the hardware's, at most 16.
+int host_threads() {
+  static const int nt = [] {
+    const char *e = std::getenv("GPURIFY_HOST_THREADS");
+    int v = e && *e ? std::atoi(e) : (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(16, v));
+  }();
+  return nt;
+}
+
+// f(i) for i in [0, n) on the set-up threads (dynamic chunks of 64); the
+// first exception is rethrown after all threads end.
+template <typename F>
+void parallel_for(int n, F &&f) {
+  const int nt = std::min(host_threads(), (n + 63) / 64);
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::exception_ptr err;
+  std::mutex m;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      try {
+        for (int i0; (i0 = next.fetch_add(64)) < n;)
+          for (int i = i0; i < std::min(n, i0 + 64); ++i) f(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(m);
+        if (!err) err = std::current_exception();
+      }
+    });
+  for (auto &t : th) t.join();
+  if (err) std::rethrow_exception(err);
 }
 
 // Progress lines on stderr when GPURIFY_LOG is set (long solves).
@@ -198,6 +241,93 @@ void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
   }
 }
 
+// The dense problems idx of a batch in two launches (assembly, LU + solve;
+// one work-group per problem), then C per problem on the host in the
+// reference's summation order.
+void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std::vector<int> &idx,
+                       const std::vector<double *> &C, gp_cap_report *reps) {
+  hipStream_t s = ctx->stream;
+  const double t0 = now_s();
+  const int B = (int)idx.size();
+  std::vector<int> hn(B), hrhs(B), hnoff(B + 1, 0);
+  std::vector<long> haoff(B + 1, 0), hboff(B + 1, 0);
+  std::vector<double> L0(B);
+  for (int q = 0; q < B; ++q) {
+    const PanelGeom &g = gs[idx[q]];
+    hn[q] = g.n;
+    hrhs[q] = g.ncond;
+    hnoff[q + 1] = hnoff[q] + g.n;
+    haoff[q + 1] = haoff[q] + (long)g.n * g.n;
+    hboff[q + 1] = hboff[q] + (long)g.n * g.ncond;
+  }
+  std::vector<double> hv((size_t)hnoff[B] * 12), hc((size_t)hnoff[B] * 3), hB(hboff[B], 0.0);
+  std::vector<std::vector<double>> area(B);
+  for (int q = 0; q < B; ++q) {
+    const PanelGeom &g = gs[idx[q]];
+    double c[3];
+    root_cube(g, c, L0[q]);
+    std::vector<double> v, ce;
+    std::vector<uint32_t> cd;
+    scaled_arrays(g, nullptr, c, L0[q], v, ce, area[q], cd);
+    std::copy(v.begin(), v.end(), hv.begin() + (size_t)hnoff[q] * 12);
+    std::copy(ce.begin(), ce.end(), hc.begin() + (size_t)hnoff[q] * 3);
+    for (int i = 0; i < g.n; ++i) hB[hboff[q] + (size_t)i * g.ncond + g.cond[i]] = 1.0;
+  }
+  auto d_n = to_device(hn, s);
+  auto d_r = to_device(hrhs, s);
+  auto d_noff = to_device(hnoff, s);
+  auto d_aoff = to_device(haoff, s);
+  auto d_boff = to_device(hboff, s);
+  auto d_v = to_device(hv, s);
+  auto d_c = to_device(hc, s);
+  auto d_B = to_device(hB, s);
+  DBuf<double> d_A(haoff[B]), d_work(hboff[B]);
+  DBuf<int> d_piv(hnoff[B]), d_info(B);
+  DenseBatch db;
+  db.count = B;
+  db.n = d_n.p;
+  db.nrhs = d_r.p;
+  db.n_off = d_noff.p;
+  db.a_off = d_aoff.p;
+  db.b_off = d_boff.p;
+  db.verts = d_v.p;
+  db.cen = d_c.p;
+  db.A = d_A.p;
+  db.B = d_B.p;
+  db.work = d_work.p;
+  db.piv = d_piv.p;
+  db.info = d_info.p;
+  launch_dense_batch(db, s);
+  std::vector<int> info(B);
+  GP_HIP(hipMemcpyAsync(info.data(), d_info.p, B * sizeof(int), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(hB.data(), d_B.p, hB.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  const double t1 = now_s();
+  for (int q = 0; q < B; ++q) {
+    const PanelGeom &g = gs[idx[q]];
+    if (info[q]) throw Error(GP_ERR_SINGULAR, "potential-coefficient matrix of problem " +
+                                                  std::to_string(idx[q]) + " is singular");
+    const int nc = g.ncond;
+    const double scale = 4.0 * kPi * kEps0 * L0[q];
+    double *Cp = C[idx[q]];
+    std::fill(Cp, Cp + (size_t)nc * nc, 0.0);
+    // accumulate_c (cap/solver.rs:67-71): panels in ascending order
+    std::vector<double> acc((size_t)nc * nc, 0.0);
+    for (int k = 0; k < g.n; ++k)
+      for (int j = 0; j < nc; ++j)
+        acc[(size_t)g.cond[k] * nc + j] += area[q][k] * hB[hboff[q] + (size_t)k * nc + j];
+    for (size_t e = 0; e < acc.size(); ++e) Cp[e] = acc[e] * scale;
+    if (reps) {
+      gp_cap_report &r = reps[idx[q]];
+      r.method_used = GP_METHOD_DENSE;
+      r.precision_used = GP_PREC_FP64;
+      r.converged = 1;
+      r.seconds_setup = 0.0;
+      r.seconds_solve = t1 - t0;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ FMM
 struct FmmSystem {
   hipStream_t s = nullptr;
@@ -237,6 +367,10 @@ struct FmmSystem {
   long matvecs = 0;
   // Transposed operator P'^T (adjoint solves): set want_transpose before setup.
   bool want_transpose = false, transposed = false;
+  // Single-RHS M2L grouped by offset: set want_by_offset before setup.
+  bool want_by_offset = false;
+  std::vector<int> po_off;
+  DBuf<int> po_tgt, po_src;
   DBuf<int> tr_off, tr_leaf, tr_stride;
   DBuf<int64_t> tr_base;
   DBuf<double> dinv_area, inv_area, ytmp;
@@ -265,13 +399,13 @@ struct FmmSystem {
     trees.assign(P, Tree());
     L0s.assign(P, 1.0);
     L = 2;
-    for (int p = 0; p < P; ++p) {
+    parallel_for(P, [&](int p) {
       const PanelGeom &g = *probs[p];
       int Lp = o.max_level ? (int)o.max_level : choose_depth(g, (int)o.leaf_target);
       build_tree(g, std::max(Lp, 2), trees[p]);
       L0s[p] = trees[p].half;
-      L = std::max(L, trees[p].L);
-    }
+    });
+    for (int p = 0; p < P; ++p) L = std::max(L, trees[p].L);
     // Problems in order of depth: the global leaf order (and so the panel
     // order) lists shallower problems first.
     std::vector<int> order(P);
@@ -343,7 +477,7 @@ struct FmmSystem {
     // Scaled panel arrays, per problem in its own unit frame.
     std::vector<double> hv((size_t)N * 12), hc((size_t)N * 3), ha(N);
     std::vector<uint32_t> hcond(N);
-    for (int p = 0; p < P; ++p) {
+    parallel_for(P, [&](int p) {
       std::vector<double> v, c, a;
       std::vector<uint32_t> cd;
       scaled_arrays(*probs[p], trees[p].perm.data(), trees[p].center, L0s[p], v, c, a, cd);
@@ -352,7 +486,7 @@ struct FmmSystem {
       std::copy(c.begin(), c.end(), hc.begin() + o0 * 3);
       std::copy(a.begin(), a.end(), ha.begin() + o0);
       for (size_t i = 0; i < cd.size(); ++i) hcond[o0 + i] = cd[i] + (uint32_t)cond_off[p];
-    }
+    });
     verts = to_device(hv, s);
     cen = to_device(hc, s);
     area = to_device(ha, s);
@@ -363,22 +497,55 @@ struct FmmSystem {
     std::vector<int> hcol_off(n_leaves + 1, 0), hcols, hpl(N);
     std::vector<int64_t> hnf(n_leaves + 1, 0);
     std::vector<std::array<int, 3>> trent;  // (S, T, column start of S in T's tile)
-    for (int T = 0; T < n_leaves; ++T) {
+    auto leaf_of = [&](int T, int &p, int &lt) {
       const int gbox = hleaf_box[T];
-      const int p = gp_of[gbox];
+      p = gp_of[gbox];
+      lt = gb_of[gbox] - trees[p].lvl_off[trees[p].L];
+    };
+    {
+      std::vector<int64_t> ccount(n_leaves);
+      parallel_for(n_leaves, [&](int T) {
+        int p, lt;
+        leaf_of(T, p, lt);
+        const Tree &t = trees[p];
+        int64_t c = 0;
+        for (int q = t.near_off[lt]; q < t.near_off[lt + 1]; ++q)
+          c += t.leaf_lo[t.near_leaf[q] + 1] - t.leaf_lo[t.near_leaf[q]];
+        ccount[T] = c;
+      });
+      int64_t acc = 0;
+      for (int T = 0; T < n_leaves; ++T) {
+        acc += ccount[T];
+        if (acc > INT_MAX) throw Error(GP_ERR_OOM, "near-field column lists exceed 2^31 entries");
+        hcol_off[T + 1] = (int)acc;
+        const int nt_ = hleaf_lo[T + 1] - hleaf_lo[T];
+        hnf[T + 1] = hnf[T] + (int64_t)nt_ * ccount[T];
+      }
+    }
+    hcols.resize(hcol_off[n_leaves]);
+    parallel_for(n_leaves, [&](int T) {
+      int p, lt;
+      leaf_of(T, p, lt);
       const Tree &t = trees[p];
-      const int lt = gb_of[gbox] - t.lvl_off[t.L];
-      const int cstart = (int)hcols.size();
+      int c = hcol_off[T];
       for (int q = t.near_off[lt]; q < t.near_off[lt + 1]; ++q) {
         const int S = t.near_leaf[q];
-        if (want_transpose) trent.push_back({leaf_first[p] + S, T, (int)hcols.size() - cstart});
-        for (int k = t.leaf_lo[S]; k < t.leaf_lo[S + 1]; ++k) hcols.push_back(panel_off[p] + k);
+        for (int k = t.leaf_lo[S]; k < t.leaf_lo[S + 1]; ++k) hcols[c++] = panel_off[p] + k;
       }
-      hcol_off[T + 1] = (int)hcols.size();
-      const int nt_ = hleaf_lo[T + 1] - hleaf_lo[T];
-      hnf[T + 1] = hnf[T] + (int64_t)nt_ * (hcol_off[T + 1] - hcol_off[T]);
       for (int k = hleaf_lo[T]; k < hleaf_lo[T + 1]; ++k) hpl[k] = T;
-    }
+    });
+    if (want_transpose)
+      for (int T = 0; T < n_leaves; ++T) {
+        int p, lt;
+        leaf_of(T, p, lt);
+        const Tree &t = trees[p];
+        int c = 0;
+        for (int q = t.near_off[lt]; q < t.near_off[lt + 1]; ++q) {
+          const int S = t.near_leaf[q];
+          trent.push_back({leaf_first[p] + S, T, c});
+          c += t.leaf_lo[S + 1] - t.leaf_lo[S];
+        }
+      }
     nnz = hnf[n_leaves];
     if (want_transpose) {
       std::stable_sort(trent.begin(), trent.end(),
@@ -448,7 +615,7 @@ struct FmmSystem {
     n3p = round_up(n3, 32);
     auto nodes = cheb_nodes(n);
     std::vector<double> hw1d((size_t)N * 3 * n);
-    for (int p = 0; p < P; ++p) {
+    parallel_for(P, [&](int p) {
       const Tree &t = trees[p];
       const PanelGeom &g = *probs[p];
       const double half_L = t.level_half(t.L);
@@ -465,7 +632,7 @@ struct FmmSystem {
             interp_weights((a[d] - bc[d]) / half_L, n, nodes.data(), &hw1d[row + d * n]);
         }
       }
-    }
+    });
     // M2M 1-D matrices, bbfmm.rs:89-112: mat[m*n + k] = S_n(cc + x_k / 2, x_m).
     std::vector<double> hm2m((size_t)24 * n * n);
     for (int oz = 0; oz < 2; ++oz)
@@ -506,20 +673,44 @@ struct FmmSystem {
     for (int l = 0; l <= L; ++l)
       for (int b = lvl_off[l]; b < lvl_off[l + 1]; ++b) hlevel[b] = l;
     for (int b = 0; b < nb; ++b) {
+      const Tree &t = trees[gp_of[b]];
+      const int lb = gb_of[b];
+      hchild_off[b + 1] = hchild_off[b] + (t.child_off[lb + 1] - t.child_off[lb]);
+      hil_off[b + 1] = hil_off[b] + (t.il_off[lb + 1] - t.il_off[lb]);
+      if (hil_off[b + 1] > hil_off[b]) htargets.push_back(b);
+    }
+    hchild.resize(hchild_off[nb]);
+    hil_src.resize(hil_off[nb]);
+    hil_oidx.resize(hil_off[nb]);
+    parallel_for(nb, [&](int b) {
       const int p = gp_of[b], lb = gb_of[b];
       const Tree &t = trees[p];
       hparent[b] = t.parent[lb] >= 0 ? gid[p][t.parent[lb]] : -1;
       hoct[b] = t.octant[lb];
-      for (int q = t.child_off[lb]; q < t.child_off[lb + 1]; ++q) hchild.push_back(gid[p][t.child_idx[q]]);
-      hchild_off[b + 1] = (int)hchild.size();
-      for (int q = t.il_off[lb]; q < t.il_off[lb + 1]; ++q) {
-        hil_src.push_back(gid[p][t.il_src[q]]);
-        hil_oidx.push_back(t.il_oidx[q]);
+      int c = hchild_off[b];
+      for (int q = t.child_off[lb]; q < t.child_off[lb + 1]; ++q) hchild[c++] = gid[p][t.child_idx[q]];
+      int e = hil_off[b];
+      for (int q = t.il_off[lb]; q < t.il_off[lb + 1]; ++q, ++e) {
+        hil_src[e] = gid[p][t.il_src[q]];
+        hil_oidx[e] = t.il_oidx[q];
       }
-      hil_off[b + 1] = (int)hil_src.size();
-      if (hil_off[b + 1] > hil_off[b]) htargets.push_back(b);
-    }
+    });
     pairs = (long)hil_src.size();
+    if (want_by_offset) {
+      // counting sort of the pairs by offset; targets stay ascending
+      po_off.assign(344, 0);
+      for (int v : hil_oidx) po_off[v + 1]++;
+      for (int o = 0; o < 343; ++o) po_off[o + 1] += po_off[o];
+      std::vector<int> ht(pairs), hs(pairs), fillp(po_off.begin(), po_off.end() - 1);
+      for (int b = 0; b < nb; ++b)
+        for (int q = hil_off[b]; q < hil_off[b + 1]; ++q) {
+          const int e = fillp[hil_oidx[q]]++;
+          ht[e] = b;
+          hs[e] = hil_src[q];
+        }
+      po_tgt = to_device(ht, s);
+      po_src = to_device(hs, s);
+    }
     // 1/half_l in units of each problem's own root half-width.
     std::vector<double> hih(L + 1);
     for (int l = 0; l <= L; ++l) hih[l] = (double)(1u << l);
@@ -565,6 +756,7 @@ struct FmmSystem {
       f.leaf_box = batch ? leaf_box.p : nullptr;
       f.n_leaves = n_leaves;
       f.leaf_lo = leaf_lo.p;
+      f.panel_leaf = panel_leaf.p;
       f.parent = parent.p;
       f.octant = octant.p;
       f.child_off = child_off.p;
@@ -577,6 +769,9 @@ struct FmmSystem {
       f.inv_half = inv_half.p;
       f.area = area.p;
       f.n3p = n3p;
+      f.po_off = want_by_offset ? po_off.data() : nullptr;
+      f.po_tgt = po_tgt.p;
+      f.po_src = po_src.p;
     };
     fill(f64);
     f64.w1d = w1d64.p;
@@ -623,7 +818,7 @@ struct FmmSystem {
       M32.alloc(mexp);
       M32.zero(s);
       Lc32.alloc(lexp);
-      if (m2l_inner == GP_M2L_BF16 && ld % 16 == 0 && ld <= 128) {
+      if (m2l_inner == GP_M2L_BF16 && ((ld % 16 == 0 && ld <= 128) || (ld == 1 && want_by_offset))) {
         Mbf.alloc(mexp);
         Mbf.zero(s);
       }
@@ -1010,8 +1205,6 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   hipStream_t s = ctx->stream;
   double t0 = now_s();
   const bool mixed = o.precision == GP_PREC_MIXED;
-  FmmSystem sys;
-  sys.setup(g, o, mixed, s);
   const int nc = g.ncond;
   const int jlo = std::min((int)o.rhs_first, nc);
   const int jhi = o.rhs_count ? std::min(nc, jlo + (int)o.rhs_count) : nc;
@@ -1020,6 +1213,9 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   int blk = o.rhs_block ? (int)o.rhs_block : nsolve;
   blk = std::max(1, std::min(blk, std::min(nsolve, 128)));
   const int ld = blk == 1 ? 1 : round_up(blk, 16);
+  FmmSystem sys;
+  sys.want_by_offset = ld == 1;
+  sys.setup(g, o, mixed, s);
   sys.ensure_work(ld);
   GP_HIP(hipStreamSynchronize(s));
   double t1 = now_s();
@@ -1093,6 +1289,7 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
   const bool mixed = o.precision == GP_PREC_MIXED;
   const int P = (int)probs.size();
   FmmSystem sys;
+  sys.want_by_offset = true;
   sys.setup(probs, o, mixed, s);
   sys.ensure_work(1);
   // Row segments and the reduction block map (at most 2048 rows a block).
@@ -1461,7 +1658,7 @@ void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_c
   const uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
   std::vector<const PanelGeom *> fmm;
   std::vector<double *> Cf;
-  std::vector<int> fidx;
+  std::vector<int> fidx, dprobs;
   const int P = (int)gs.size();
   if (reps) std::memset(reps, 0, sizeof(gp_cap_report) * P);
   for (int p = 0; p < P; ++p) {
@@ -1473,13 +1670,14 @@ void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_c
     const bool dense = o.method == GP_METHOD_DENSE ||
                        (o.method == GP_METHOD_AUTO && (uint32_t)g.n < limit);
     if (dense) {
-      solve_dense(ctx, g, C[p], reps ? &reps[p] : nullptr, nullptr);
+      dprobs.push_back(p);
     } else {
       fmm.push_back(&g);
       Cf.push_back(C[p]);
       fidx.push_back(p);
     }
   }
+  if (!dprobs.empty()) solve_dense_batch(ctx, gs, dprobs, C, reps);
   if (!fmm.empty()) {
     std::vector<gp_cap_report> fr(fmm.size());
     for (size_t q = 0; q < fmm.size(); ++q) fr[q] = reps ? reps[fidx[q]] : gp_cap_report{};

@@ -124,11 +124,18 @@ def test_fmm_solve_fp64_matches_oracle(ctx):
     assert relmax(C, Cr) < 1e-4
 
 
-def test_fmm_single_rhs_blocks_match(ctx):
+@pytest.mark.parametrize("order", [4, 5])
+def test_fmm_single_rhs_blocks_match(ctx, order):
+    # one RHS at a time runs the single-column kernels (P2M / L2P per panel,
+    # matrix-core M2L grouped by offset, FP64 and BF16)
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
-    Cb, _ = ctx.cap_solve(v, c, 3, G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11))
-    Cs, _ = ctx.cap_solve(v, c, 3, G.default_opts(method=G.GP_METHOD_FMM, tol=1e-11, rhs_block=1))
+    common = dict(method=G.GP_METHOD_FMM, tol=1e-11, fmm_order=order)
+    Cb, _ = ctx.cap_solve(v, c, 3, G.default_opts(**common))
+    Cs, _ = ctx.cap_solve(v, c, 3, G.default_opts(rhs_block=1, **common))
     assert relmax(Cs, Cb) < 1e-9
+    Cm, rep = ctx.cap_solve(v, c, 3, G.default_opts(rhs_block=1, precision=G.GP_PREC_MIXED,
+                                                   **common))
+    assert rep["converged"] == 1 and np.max(np.abs(Cm - Cb) / np.abs(Cb)) < 1e-6
 
 
 @pytest.mark.parametrize("basis,passes", [(0, 1), (0, 2), (1, 1), (1, 2)])
