@@ -480,6 +480,96 @@ __global__ __launch_bounds__(256) void k_l2p1(int n, int N, const int *__restric
   y[p] += (double)acc;
 }
 
+// Single right-hand side M2M: 256 / tpb parent boxes per work-group, one
+// thread per (parent, node); the three 1-D contractions of each child (z,
+// then y, then x, as k_m2m) pass through a per-parent LDS slot.  Every
+// parent loops over 8 child slots so the barriers stay uniform.
+template <typename T>
+__global__ __launch_bounds__(256) void k_m2m1(int n, int n3p, int tpb, int box0, int nb,
+                                              const int *__restrict__ child_off,
+                                              const int *__restrict__ child_idx,
+                                              const int *__restrict__ octant,
+                                              const T *__restrict__ m2m1d, T *__restrict__ M,
+                                              __bf16 *__restrict__ Mb) {
+  extern __shared__ unsigned char smem_raw[];
+  const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
+  const int n3 = n * n * n, nn = n * n;
+  T *s0 = reinterpret_cast<T *>(smem_raw) + slot * 2 * tpb, *s1 = s0 + tpb;
+  const int bi = blockIdx.x * (256 / tpb) + slot;
+  const bool live = bi < nb && m < n3;
+  const int b = box0 + min(bi, nb - 1);
+  const int c0 = child_off[b], nch = bi < nb ? child_off[b + 1] - c0 : 0;
+  const int a = m / nn, bq = (m / n) % n, cq = m % n;
+  T acc = 0;
+  for (int q = 0; q < 8; ++q) {
+    const bool valid = live && q < nch;
+    const int ch = valid ? child_idx[c0 + q] : 0;
+    const int o = valid ? octant[ch] : 0;
+    const T *mx = m2m1d + (o * 3 + 0) * nn, *my = m2m1d + (o * 3 + 1) * nn,
+            *mz = m2m1d + (o * 3 + 2) * nn;
+    if (m < n3) s0[m] = valid ? M[(size_t)ch * n3p + m] : (T)0;
+    __syncthreads();
+    if (m < n3) {  // z: s1[a][b][cp] = sum_c mz[cp][c] s0[a][b][c]
+      T t = 0;
+      for (int c = 0; c < n; ++c) t += mz[cq * n + c] * s0[(a * n + bq) * n + c];
+      s1[m] = t;
+    }
+    __syncthreads();
+    if (m < n3) {  // y: s0[a][bp][cp] = sum_b my[bp][b] s1[a][b][cp]
+      T t = 0;
+      for (int bb = 0; bb < n; ++bb) t += my[bq * n + bb] * s1[(a * n + bb) * n + cq];
+      s0[m] = t;
+    }
+    __syncthreads();
+    if (valid) {  // x: acc[ap][bp][cp] += sum_a mx[ap][a] s0[a][bp][cp]
+      T t = 0;
+      for (int aa = 0; aa < n; ++aa) t += mx[a * n + aa] * s0[aa * nn + bq * n + cq];
+      acc += t;
+    }
+    __syncthreads();
+  }
+  if (live) store_mult(M, Mb, (size_t)b * n3p + m, acc);
+}
+
+// Single right-hand side L2L: one thread per (child, node), the transposed
+// contractions x^T, y^T, z^T of k_l2l through a per-child LDS slot.
+template <typename T>
+__global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
+                                              const int *__restrict__ parent,
+                                              const int *__restrict__ octant,
+                                              const T *__restrict__ m2m1d, T *__restrict__ Lc) {
+  extern __shared__ unsigned char smem_raw[];
+  const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
+  const int n3 = n * n * n, nn = n * n;
+  T *s0 = reinterpret_cast<T *>(smem_raw) + slot * 2 * tpb, *s1 = s0 + tpb;
+  const int bi = blockIdx.x * (256 / tpb) + slot;
+  const bool live = bi < nb && m < n3;
+  const int ch = box0 + min(bi, nb - 1);
+  const int p = parent[ch], o = octant[ch];
+  const T *mx = m2m1d + (o * 3 + 0) * nn, *my = m2m1d + (o * 3 + 1) * nn,
+          *mz = m2m1d + (o * 3 + 2) * nn;
+  const int a = m / nn, bq = (m / n) % n, cq = m % n;
+  if (m < n3) s0[m] = Lc[(size_t)p * n3 + m];
+  __syncthreads();
+  if (m < n3) {  // x^T: s1[ap][b][c] = sum_a mx[a][ap] s0[a][b][c]
+    T t = 0;
+    for (int aa = 0; aa < n; ++aa) t += mx[aa * n + a] * s0[aa * nn + bq * n + cq];
+    s1[m] = t;
+  }
+  __syncthreads();
+  if (m < n3) {  // y^T: s0[a][bp][c] = sum_b my[b][bp] s1[a][b][c]
+    T t = 0;
+    for (int bb = 0; bb < n; ++bb) t += my[bb * n + bq] * s1[(a * n + bb) * n + cq];
+    s0[m] = t;
+  }
+  __syncthreads();
+  if (live) {  // z^T: Lc[child][a][b][cp] += sum_c mz[c][cp] s0[a][b][c]
+    T t = 0;
+    for (int c = 0; c < n; ++c) t += mz[c * n + cq] * s0[(a * n + bq) * n + c];
+    Lc[(size_t)ch * n3 + m] += t;
+  }
+}
+
 // Single right-hand side (ldm = 1): M2L grouped by offset.  One launch per
 // M2L offset o covers every (target, source) pair with that offset (at most
 // one per target, so the updates of one launch never collide and the sum
@@ -514,19 +604,29 @@ __global__ __launch_bounds__(256) void k_m2l1_bf16(int n3, int npairs, const int
       acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt], 0, 0, 0);
     }
   }
-  // D[i = node][j = pair]: lane holds nodes 4 (lane / 16) + v of pair lane % 16
+  // D[i = node][j = pair]: lane holds nodes 4 (lane / 16) + v of pair lane % 16,
+  // 16 contiguous bytes of the target's local expansion
   const int j = j0 + li;
   if (j < npairs) {
     const int t = tgt[j];
     const float ih = (float)inv_half[box_level[t]];
     float *out = Lc + (size_t)t * n3;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MT; ++mt) {
+      const int m0 = mt * 16 + (lane >> 4) * 4;
+      if ((n3 & 3) == 0) {
+        if (m0 < n3) {
+          f32x4 o = *(f32x4 *)(out + m0);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int m = mt * 16 + (lane >> 4) * 4 + v;
-        if (m < n3) out[m] += acc[mt][v] * ih;
+          for (int v = 0; v < 4; ++v) o[v] += acc[mt][v] * ih;
+          *(f32x4 *)(out + m0) = o;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (m0 + v < n3) out[m0 + v] += acc[mt][v] * ih;
       }
+    }
   }
 }
 
@@ -1093,7 +1193,12 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     if (nb == 0) continue;
     int nch = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch, (nb + nch) * exp_bytes);
-    if (tiled) {
+    if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
+      const int tpb = round_up(n3, 64), bpb = 256 / tpb;
+      hipLaunchKernelGGL((k_m2m1<T>), dim3((nb + bpb - 1) / bpb), dim3(256), 512 * sizeof(T), s, n,
+                         n3p, tpb, f.lvl_off[l], nb, f.child_off, f.child_idx, f.octant, f.m2m1d,
+                         M, mb);
+    } else if (tiled) {
       const int b0 = f.lvl_off[l];
       switch (n) {
         case 2: launch_m2m_tiled<T, 2>(n3p, b0, nb, nrc32, f, nr, M, mb, ldm, s); break;
@@ -1163,7 +1268,11 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     int nb = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     if (nb == 0) continue;
     ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nr * nb, 2.0 * nb * exp_bytes);
-    if (tiled) {
+    if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
+      const int tpb = round_up(n3, 64), bpb = 256 / tpb;
+      hipLaunchKernelGGL((k_l2l1<T>), dim3((nb + bpb - 1) / bpb), dim3(256), 512 * sizeof(T), s, n,
+                         tpb, f.lvl_off[l + 1], nb, f.parent, f.octant, f.m2m1d, Lc);
+    } else if (tiled) {
       const int b0 = f.lvl_off[l + 1];
       switch (n) {
         case 2: launch_l2l_tiled<T, 2>(b0, nb, nrc32, f, nr, Lc, ldm, s); break;
