@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one streaming copy kernel of this size first (read N MiB, write "
+                         "N MiB; the calibration of rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     w = WORKLOADS[a.workload]
     a.refine = a.refine or w["refine"]
@@ -101,6 +104,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is designed, not built)"}))
         return 2
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
     if args.workload == "nets4096":
         return bench_batch(args)
     if args.workload == "bundle16":
@@ -160,7 +165,7 @@ def main():
         out["kernels"] = {k: {"launches": v["launches"], "ms_total": round(1e3 * v["seconds"], 3),
                               "ms_avg": round(1e3 * v["seconds"] / v["launches"], 4)}
                           for k, v in kern[:10]}
-        out["roofline"] = roofline(kern)
+        out["roofline"] = roofline(kern, args.workload)
 
         if not args.no_fp64:
             k64 = min(args.fp64_rhs or nrhs, nrhs)
@@ -257,7 +262,7 @@ def bench_batch(args):
         out["kernels"] = {k: {"launches": v["launches"], "ms_total": round(1e3 * v["seconds"], 3),
                               "ms_avg": round(1e3 * v["seconds"] / v["launches"], 4)}
                           for k, v in kern[:10]}
-        out["roofline"] = roofline(kern)
+        out["roofline"] = roofline(kern, "nets4096")
         if not args.no_fp64:
             t = time.perf_counter()
             C64, r64 = ctx.cap_solve_batched(probs, o_64)
@@ -336,7 +341,7 @@ def bench_adjoint(args):
         out["kernels"] = {k: {"launches": v_["launches"], "ms_total": round(1e3 * v_["seconds"], 3),
                               "ms_avg": round(1e3 * v_["seconds"] / v_["launches"], 4)}
                           for k, v_ in kern[:10]}
-        out["roofline"] = roofline(kern)
+        out["roofline"] = roofline(kern, "bundle16")
         if not args.no_fp64:
             t = time.perf_counter()
             C64, J64, r64 = ctx.cap_jacobian(v, c, nc, params, o_64)
@@ -389,7 +394,49 @@ def cpu_baseline_batch(args, probs):
             "sample": f"{len(idx)} of the 4096 nets (evenly spaced), {n} panels, one by one: {dt:.2f}s"}
 
 
-def roofline(kern):
+def calib_copy(mib):
+    """One streaming kernel that reads mib MiB and writes mib MiB (torch's
+    elementwise add), before the workload: under `rocprofv3 --pmc` its
+    FETCH_SIZE / WRITE_SIZE calibrate the counters against known bytes."""
+    import torch
+
+    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.add(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    log(f"calibration copy: {mib} MiB read, {mib} MiB written")
+    del a, b
+    torch.cuda.empty_cache()
+
+
+# device-profiler names of the dominant kernels -> their HIP symbols
+PROFILE_SYMBOLS = {"m2l_bf16_mfma": "k_m2l_bf16_pipe", "m2l1_bf16_mfma": "k_m2l1_bf16",
+                   "m2l_f64_mfma": "k_m2l_f64_pipe", "m2l1_f64_mfma": "k_m2l1_f64",
+                   "near_spmv_f32": "k_near_spmv", "near_spmm_f32": "k_near_spmm"}
+
+
+def measured_traffic(name, workload):
+    """HBM bytes per launch of a kernel on a workload, from the committed
+    rocprofv3 --pmc summaries (lines `traffic_bytes_per_launch: <symbol>
+    <workload> <bytes>`), or None when none covers it."""
+    sym = PROFILE_SYMBOLS.get(name)
+    pdir = os.path.join(ROOT, "profiles")
+    if not sym or not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, fn)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch:" \
+                        and parts[1] == sym and parts[2] == workload:
+                    return float(parts[3])
+    return None
+
+
+def roofline(kern, workload=None):
     """Roofline entry for the kernel with the largest share of device time."""
     name, v = kern[0]
     avg = v["seconds"] / v["launches"]
@@ -407,7 +454,7 @@ def roofline(kern):
     achieved = per / v["launches"] / avg / scale
     return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "avg_launch_ms": 1e3 * avg, "launches": v["launches"],
-            "traffic": None}
+            "traffic": measured_traffic(name, workload)}
 
 
 def cpu_baseline(args):

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_m2m(int n, int n3p, int box0,
 // Lc[t] = (1/half) sum_(s,off) G_off M[s].  targets: boxes with a non-empty
 // interaction list.
 template <typename T>
-__global__ __launch_bounds__(256) void k_m2l(int n, int n3p, const int *__restrict__ targets,
+__global__ __launch_bounds__(256) void k_m2l(int n3, int n3p, const int *__restrict__ targets,
                                              const int *__restrict__ il_off,
                                              const int *__restrict__ il_src,
                                              const int *__restrict__ il_oidx,
@@ -164,7 +164,6 @@ __global__ __launch_bounds__(256) void k_m2l(int n, int n3p, const int *__restri
                                              int ldm) {
   extern __shared__ unsigned char smem_raw[];
   T *s0 = reinterpret_cast<T *>(smem_raw);
-  const int n3 = n * n * n;
   const int t = targets[blockIdx.x], r0 = blockIdx.y * FM_RC;
   const int rc = min(FM_RC, nr - r0);
   const int nout = n3 * rc;
@@ -1140,9 +1139,81 @@ __global__ __launch_bounds__(256) void k_fill_zero(T *__restrict__ p, size_t n) 
     p[e] = (T)0;
 }
 
+// Low-rank M2L, projection: Mt[b][r] = U^T M[b][r] (U: n3 x kr, row-major)
+// for the rows q -> (box0 + q / nr, q % nr), CK_ROWS rows per work-group
+// staged in LDS; each thread owns one of the kr outputs of CK_ROWS * kr / 256
+// rows.  A BF16 copy goes to Mtb when set.
+constexpr int CK_ROWS = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void k_m2l_compress(int n3, int n3p, int kr, int box0, int nrow,
+                                                      int nr, int ldm, const T *__restrict__ U,
+                                                      const T *__restrict__ M,
+                                                      T *__restrict__ Mt,
+                                                      __bf16 *__restrict__ Mtb) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sm = reinterpret_cast<T *>(smem_raw);  // CK_ROWS x (n3 + 1)
+  const int q0 = blockIdx.x * CK_ROWS, ls = n3 + 1;
+  for (int e = threadIdx.x; e < CK_ROWS * n3; e += 256) {
+    const int i = e / n3, m = e % n3, q = q0 + i;
+    sm[i * ls + m] = q < nrow ? M[((size_t)(box0 + q / nr) * ldm + q % nr) * n3p + m] : (T)0;
+  }
+  __syncthreads();
+  const int G = 256 / kr, j = threadIdx.x % kr, i0 = threadIdx.x / kr;
+  if (i0 >= G) return;
+  constexpr int NA = CK_ROWS / 2;  // rows per thread for kr <= 128
+  T acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = 0;
+  for (int m = 0; m < n3; ++m) {
+    const T u = U[(size_t)m * kr + j];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int i = i0 + a * G;
+      if (i < CK_ROWS) acc[a] += u * sm[i * ls + m];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    const int i = i0 + a * G, q = q0 + i;
+    if (i < CK_ROWS && q < nrow) {
+      const size_t idx = ((size_t)(box0 + q / nr) * ldm + q % nr) * kr + j;
+      Mt[idx] = acc[a];
+      if (Mtb) Mtb[idx] = (__bf16)(float)acc[a];
+    }
+  }
+}
+
+// Low-rank M2L, expansion: Lc[b][m][r] = sum_j U[m][j] Lt[b][j][r] for the
+// columns c -> (targets[c / nr], c % nr), 16 columns per work-group staged in
+// LDS (kr x 16); thread (m0, i) writes nodes m0, m0 + 16, ... of column i.
+template <typename T>
+__global__ __launch_bounds__(256) void k_m2l_expand(int n3, int kr, int ncol, int nr, int ldm,
+                                                    const int *__restrict__ targets,
+                                                    const T *__restrict__ U,
+                                                    const T *__restrict__ Lt, T *__restrict__ Lc) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sm = reinterpret_cast<T *>(smem_raw);
+  const int c0 = blockIdx.x * 16;
+  for (int e = threadIdx.x; e < kr * 16; e += 256) {
+    const int j = e / 16, i = e % 16, c = c0 + i;
+    sm[e] = c < ncol ? Lt[((size_t)targets[c / nr] * kr + j) * ldm + c % nr] : (T)0;
+  }
+  __syncthreads();
+  const int i = threadIdx.x % 16, m0 = threadIdx.x / 16, c = c0 + i;
+  if (c >= ncol) return;
+  const int b = targets[c / nr], r = c % nr;
+  for (int m = m0; m < n3; m += 16) {
+    const T *um = U + (size_t)m * kr;
+    T acc = 0;
+    for (int j = 0; j < kr; ++j) acc += um[j] * sm[j * 16 + i];
+    Lc[((size_t)b * n3 + m) * ldm + r] = acc;
+  }
+}
+
 template <typename T>
 void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf16 *Mb, T *Lc,
-                     int ldm, int m2l, double *y, int ldy, hipStream_t s) {
+                     int ldm, int m2l, double *y, int ldy, hipStream_t s, T *Mt, __bf16 *Mtb,
+                     T *Lt) {
   const int n = f.n, n3 = n * n * n, n3p = f.n3p;
   const int nrc = (nr + FM_RC - 1) / FM_RC;
   const size_t lds1 = (size_t)n3 * FM_RC * sizeof(T);
@@ -1153,9 +1224,16 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   const double exp_bytes = (double)n3 * nr * sizeof(T);
   const bool want_bf16 = (m2l == GP_M2L_BF16);
   const bool by_offset = ldm == 1 && nr == 1 && f.po_off != nullptr;
-  if (want_bf16 && (!Mb || !f.Gbf || n3p % 32 || (!by_offset && (ldm % 16 || ldm > 128))))
+  const bool comp = f.kr > 0;
+  if (comp && (!f32 || !Mt || !Lt || !f.U || (want_bf16 && !Mtb) || f.kr % 32 || f.kr > 128))
+    throw Error(GP_ERR_INTERNAL, "low-rank M2L needs the FP32 operator, kr in {32, 64, 128} "
+                                 "and its work buffers");
+  if (want_bf16 && ((comp ? !f.Gcbf : (!Mb || !f.Gbf || n3p % 32)) ||
+                    (!by_offset && (ldm % 16 || ldm > 128))))
     throw Error(GP_ERR_INTERNAL, "BF16 M2L needs 16 <= ld <= 128 (multiple of 16) and its tables");
-  __bf16 *mb = want_bf16 ? Mb : nullptr;
+  // BF16 multipoles from the upward pass (the low-rank M2L takes its BF16
+  // operands from the projection instead)
+  __bf16 *mb = (want_bf16 && !comp) ? Mb : nullptr;
   // Register-tiled kernels for RHS blocks that are a multiple of 16.
   const bool tiled = (ldm % 16 == 0) && ldm <= 128 && n >= 2 && n <= 6;
   const int nrc32 = (nr + TM_RC - 1) / TM_RC;
@@ -1214,54 +1292,97 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     GP_LAUNCH_CHECK();
   }
   // M2L over every box of level >= 2 with a non-empty list; boxes without
-  // one keep a zero local expansion.
+  // one keep a zero local expansion.  A low-rank operator (f.kr > 0) runs the
+  // M2L on the projected multipoles Mt and locals Lt (kr values per box and
+  // RHS column) with the kr x kr tables, then expands Lt into Lc.
+  FmmDev<T> fm = f;
+  int m3 = n3;
+  T *Mo = M, *Lo = Lc;
+  __bf16 *mbo = mb;
+  if (comp) {
+    fm.n3p = f.kr;
+    fm.G = f.Gc;
+    fm.Gbf = f.Gcbf;
+    fm.Gr = f.Gcr;
+    m3 = f.kr;
+    Mo = Mt;
+    Lo = Lt;
+    mbo = want_bf16 ? Mtb : nullptr;
+    const int b0 = f.lvl_off[2], nrow = (f.n_boxes - b0) * nr;
+    if (nrow > 0) {
+      ProfScope pc("m2l_compress_f32", s, 2.0 * nrow * n3 * f.kr,
+                   (double)nrow * (n3 + f.kr) * sizeof(T));
+      hipLaunchKernelGGL((k_m2l_compress<T>), dim3((nrow + CK_ROWS - 1) / CK_ROWS), dim3(256),
+                         CK_ROWS * (n3 + 1) * sizeof(T), s, n3, n3p, f.kr, b0, nrow, nr, ldm, f.U,
+                         M, Mt, mbo);
+      GP_LAUNCH_CHECK();
+    }
+  }
+  const int m3p = fm.n3p;
   {
     size_t cnt = (size_t)f.n_boxes * n3 * ldm;
     int blocks = (int)std::min<size_t>((cnt + 255) / 256, 8192);
     hipLaunchKernelGGL((k_fill_zero<T>), dim3(blocks), dim3(256), 0, s, Lc, cnt);
     GP_LAUNCH_CHECK();
+    if (comp) {
+      cnt = (size_t)f.n_boxes * m3 * ldm;
+      blocks = (int)std::min<size_t>((cnt + 255) / 256, 8192);
+      hipLaunchKernelGGL((k_fill_zero<T>), dim3(blocks), dim3(256), 0, s, Lt, cnt);
+      GP_LAUNCH_CHECK();
+    }
   }
   const bool bf16_valu = (m2l == GP_M2L_BF16_VALU);
   if (bf16_valu) {
     if constexpr (sizeof(T) == 4) {
-      if (!f.Gr) throw Error(GP_ERR_INTERNAL, "BF16-rounded M2L table missing");
-      size_t cnt = (size_t)f.n_boxes * ldm * n3p;
+      if (!fm.Gr) throw Error(GP_ERR_INTERNAL, "BF16-rounded M2L table missing");
+      size_t cnt = (size_t)f.n_boxes * ldm * m3p;
       int blocks = (int)std::min<size_t>((cnt + 255) / 256, 8192);
-      hipLaunchKernelGGL(k_round_bf16, dim3(blocks), dim3(256), 0, s, M, cnt);
+      hipLaunchKernelGGL(k_round_bf16, dim3(blocks), dim3(256), 0, s, Mo, cnt);
       GP_LAUNCH_CHECK();
     } else {
       throw Error(GP_ERR_INTERNAL, "BF16 M2L runs on the FP32 operator only");
     }
   }
   if (f.n_targets > 0) {
-    const double fl = 2.0 * (double)f.n_pairs * n3 * n3 * nr;
-    const double by = (double)(f.n_boxes + f.n_targets) * exp_bytes;
-    const int MT = n3p / 16, RT = (nr + 15) / 16;
+    const double fl = 2.0 * (double)f.n_pairs * m3 * m3 * nr;
+    const double mbytes = (double)m3 * nr * sizeof(T);
+    const double by = (double)(f.n_boxes + f.n_targets) * mbytes;
+    const int MT = m3p / 16, RT = (nr + 15) / 16;
     if (want_bf16 && by_offset) {
-      ProfScope pl("m2l1_bf16_mfma", s, fl, (double)f.n_pairs * (n3p * 2 + n3 * 8));
-      launch_m2l_by_offset(f, n3, Mb, Lc, true, s);
+      ProfScope pl("m2l1_bf16_mfma", s, fl, (double)f.n_pairs * (m3p * 2 + m3 * 8));
+      launch_m2l_by_offset(fm, m3, mbo, Lo, true, s);
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {
-      ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (n3p * 8 + n3 * 16));
-      launch_m2l_by_offset(f, n3, M, Lc, false, s);
+      ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (m3p * 8 + m3 * 16));
+      launch_m2l_by_offset(fm, m3, Mo, Lo, false, s);
     } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
-        ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * n3 * nr * 2 + f.n_targets * exp_bytes);
-        GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, f, n3, nr, ldm, Mb, Lc, s);
+        ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * m3 * nr * 2 + f.n_targets * mbytes);
+        GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, fm, m3, nr, ldm, mbo, Lo, s);
       } else {
         throw Error(GP_ERR_INTERNAL, "BF16 M2L runs on the FP32 operator only");
       }
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && ldm % 16 == 0 && ldm <= 128) {
       if constexpr (sizeof(T) == 8) {
         ProfScope pl("m2l_f64_mfma", s, fl, by);
-        GP_M2L_DISPATCH(launch_m2l_f64, MT, RT, f, n3, nr, ldm, M, Lc, s);
+        GP_M2L_DISPATCH(launch_m2l_f64, MT, RT, fm, m3, nr, ldm, Mo, Lo, s);
       }
     } else {
       ProfScope pl(bf16_valu ? "m2l_bf16_valu" : f32 ? "m2l_f32" : "m2l_f64", s, fl, by);
-      hipLaunchKernelGGL((k_m2l<T>), dim3(f.n_targets, nrc), dim3(256), lds1, s, n, n3p,
-                         f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
-                         bf16_valu ? f.Gr : f.G, nr, M, Lc, ldm);
+      hipLaunchKernelGGL((k_m2l<T>), dim3(f.n_targets, nrc), dim3(256),
+                         (size_t)m3 * FM_RC * sizeof(T), s, m3, m3p, f.targets, f.il_off, f.il_src,
+                         f.il_oidx, f.box_level, f.inv_half, bf16_valu ? fm.Gr : fm.G, nr, Mo, Lo,
+                         ldm);
     }
     GP_LAUNCH_CHECK();
+    if (comp) {
+      const int ncol = f.n_targets * nr;
+      ProfScope pe("m2l_expand_f32", s, 2.0 * ncol * n3 * f.kr,
+                   (double)ncol * (n3 + f.kr) * sizeof(T));
+      hipLaunchKernelGGL((k_m2l_expand<T>), dim3((ncol + 15) / 16), dim3(256),
+                         (size_t)f.kr * 16 * sizeof(T), s, n3, f.kr, ncol, nr, ldm, f.targets, f.U,
+                         Lt, Lc);
+      GP_LAUNCH_CHECK();
+    }
   }
   // L2L, level 2 to the leaves.
   for (int l = 2; l < L; ++l) {
@@ -1303,12 +1424,14 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
 
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
                    double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s) {
-  run_far_field_t<double>(f, z, ldz, nr, M, nullptr, Lc, ldm, m2l, y, ldy, s);
+  run_far_field_t<double>(f, z, ldz, nr, M, nullptr, Lc, ldm, m2l, y, ldy, s, nullptr, nullptr,
+                          nullptr);
 }
 void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,
-                   float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s) {
+                   float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s, float *Mt,
+                   void *Mtb, float *Lt) {
   run_far_field_t<float>(f, z, ldz, nr, M, reinterpret_cast<__bf16 *>(Mb), Lc, ldm, m2l, y, ldy,
-                         s);
+                         s, Mt, reinterpret_cast<__bf16 *>(Mtb), Lt);
 }
 
 }  // namespace gp

@@ -123,6 +123,11 @@ struct FmmDev {
   const T *G = nullptr;              // 343 * n3p * n3p (zero padded)
   const void *Gbf = nullptr;         // same, BF16 (FP32 operator only)
   const T *Gr = nullptr;             // same, BF16-rounded, widened to T (GP_M2L_BF16_VALU)
+  // Low-rank M2L (FP32 operator only): rank kr (0 = off), U: n3 x kr
+  // (row-major), tables Gc / Gcbf / Gcr as G / Gbf / Gr with kr x kr blocks.
+  int kr = 0;
+  const T *U = nullptr, *Gc = nullptr, *Gcr = nullptr;
+  const void *Gcbf = nullptr;
   // Single right-hand side: the M2L pairs grouped by offset index o, pairs
   // [po_off[o], po_off[o+1]) (HOST, 344 entries) of device lists po_tgt /
   // po_src; null when not built.
@@ -133,11 +138,14 @@ struct FmmDev {
 // n_boxes x n3 x ldm.  m2l selects the M2L kernel (GP_M2L_*): BF16 needs Mb
 // (BF16 multipoles, same layout), FP64 uses matrix cores when ldm allows,
 // BF16_VALU rounds M to BF16 in place after the upward pass and runs the
-// FP32 VALU M2L on the table Gr.
+// FP32 VALU M2L on the table Gr.  A low-rank operator (f.kr > 0) needs the
+// projected multipoles Mt (n_boxes x ldm x kr, BF16 copy Mtb for the BF16
+// M2L) and locals Lt (n_boxes x kr x ldm); Mb is then unused.
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
                    double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
 void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,
-                   float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
+                   float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s,
+                   float *Mt = nullptr, void *Mtb = nullptr, float *Lt = nullptr);
 
 // ---------------------------------------------------------------- Krylov
 // Basis types TV / TX / TO: double or float (explicitly instantiated).

@@ -30,9 +30,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <numeric>
 
 #include "gp_kernels.h"
+#include "gp_m2l_svd.h"
 
 namespace gp {
 
@@ -329,6 +332,25 @@ void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std:
 }
 
 // ------------------------------------------------------------------ FMM
+// Low-rank M2L tables per (order, rank), computed once per process: they
+// depend on the Chebyshev order only (reference box, unit half-width).
+struct LowRank {
+  std::vector<double> U, Gc;
+};
+std::shared_ptr<const LowRank> low_rank_tables(int n, int n3p, int kr,
+                                               const std::vector<double> &hG) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, std::shared_ptr<const LowRank>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto &e = cache[{n, kr}];
+  if (!e) {
+    auto lr = std::make_shared<LowRank>();
+    m2l_low_rank(n * n * n, n3p, hG.data(), kr, host_threads(), lr->U, lr->Gc);
+    e = lr;
+  }
+  return e;
+}
+
 struct FmmSystem {
   hipStream_t s = nullptr;
   int N = 0, n = 0, L = 0, ncond = 0;
@@ -356,14 +378,19 @@ struct FmmSystem {
   DBuf<double> w1d64, m2m64, G64;
   DBuf<float> w1d32, m2m32, G32, Gr32;
   DBuf<uint16_t> Gbf;
+  // Low-rank inner M2L: rank kr (0 = off), projection U and kr x kr tables.
+  int kr = 0;
+  DBuf<float> U32, Gc32, Gcr32;
+  DBuf<uint16_t> Gcbf;
   DBuf<int> coff, cidx;
   NearTiles nt;
   FmmDev<double> f64;
   FmmDev<float> f32;
   int ld = 0;
   DBuf<double> z64, M64, Lc64;
-  DBuf<float> z32, M32, Lc32;
-  DBuf<uint16_t> Mbf;
+  DBuf<float> z32, M32, Lc32, Mt32, Lt32;
+  DBuf<uint16_t> Mbf, Mtbf;
+  bool bf16_ok = false;  // the BF16 M2L runs at this ld
   long matvecs = 0;
   // Transposed operator P'^T (adjoint solves): set want_transpose before setup.
   bool want_transpose = false, transposed = false;
@@ -742,6 +769,26 @@ struct FmmSystem {
         }
         Gr32 = to_device(hr, s);
       }
+      if (o.m2l_rank > 0) {
+        kr = o.m2l_rank <= 32 ? 32 : o.m2l_rank <= 64 ? 64 : 128;
+        if (kr >= n3p) kr = 0;
+      }
+      if (kr) {
+        auto lr = low_rank_tables(n, n3p, kr, hG);
+        U32 = to_device(cast_vec<float>(lr->U), s);
+        Gc32 = to_device(cast_vec<float>(lr->Gc), s);
+        std::vector<uint16_t> hcb(lr->Gc.size());
+        for (size_t e = 0; e < hcb.size(); ++e) hcb[e] = bf16_bits((float)lr->Gc[e]);
+        Gcbf = to_device(hcb, s);
+        if (m2l_inner == GP_M2L_BF16_VALU) {
+          std::vector<float> hr(hcb.size());
+          for (size_t e = 0; e < hcb.size(); ++e) {
+            const uint32_t u = (uint32_t)hcb[e] << 16;
+            std::memcpy(&hr[e], &u, 4);
+          }
+          Gcr32 = to_device(hr, s);
+        }
+      }
     }
     const bool batch = P > 1;
     auto fill = [&](auto &f) {
@@ -783,6 +830,11 @@ struct FmmSystem {
     f32.G = G32.p;
     f32.Gbf = Gbf.p;
     f32.Gr = Gr32.p;
+    f32.kr = kr;
+    f32.U = U32.p;
+    f32.Gc = Gc32.p;
+    f32.Gcbf = Gcbf.p;
+    f32.Gcr = Gcr32.p;
     // Panels of each (batch-global) conductor, ascending caller order.
     std::vector<int> hco{0}, hci;
     for (int p = 0; p < P; ++p) {
@@ -818,11 +870,27 @@ struct FmmSystem {
       M32.alloc(mexp);
       M32.zero(s);
       Lc32.alloc(lexp);
-      if (m2l_inner == GP_M2L_BF16 && ((ld % 16 == 0 && ld <= 128) || (ld == 1 && want_by_offset))) {
+      bf16_ok = m2l_inner == GP_M2L_BF16 &&
+                ((ld % 16 == 0 && ld <= 128) || (ld == 1 && want_by_offset));
+      if (kr) {
+        const size_t texp = (size_t)f64.n_boxes * kr * ld;
+        Mt32.alloc(texp);
+        Mt32.zero(s);
+        Lt32.alloc(texp);
+        if (bf16_ok) {
+          Mtbf.alloc(texp);
+          Mtbf.zero(s);
+        }
+      } else if (bf16_ok) {
         Mbf.alloc(mexp);
         Mbf.zero(s);
       }
     }
+  }
+
+  // M2L kernel of the FP32 operator at the current ld
+  int inner_m2l() const {
+    return bf16_ok ? GP_M2L_BF16 : m2l_inner == GP_M2L_BF16_VALU ? GP_M2L_BF16_VALU : GP_M2L_FP32;
   }
 
   // y = P' M^-1 v (precond) or y = P' v, in FP64 (prec 0) or FP32 (prec 1).
@@ -843,10 +911,8 @@ struct FmmSystem {
       if (!have32) throw Error(GP_ERR_INTERNAL, "FP32 operator not assembled");
       kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
       launch_near_apply(nt, nf32.p, z32.p, ld, y, ld, nr, s);
-      const int v = (m2l_inner == GP_M2L_BF16 && Mbf.p) ? GP_M2L_BF16
-                    : m2l_inner == GP_M2L_BF16_VALU    ? GP_M2L_BF16_VALU
-                                                       : GP_M2L_FP32;
-      run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, v, y, ld, s);
+      run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, inner_m2l(), y, ld, s, Mt32.p,
+                    Mtbf.p, Lt32.p);
     }
   }
 
@@ -870,10 +936,8 @@ struct FmmSystem {
       kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
       launch_near_apply_tr(nt, ntr, nf32.p, z32.p, ld, y, ld, nr, s);
       kr_precond_cast(v, dfar, z32.p, N, ld, nr, s);
-      const int m = (m2l_inner == GP_M2L_BF16 && Mbf.p) ? GP_M2L_BF16
-                    : m2l_inner == GP_M2L_BF16_VALU    ? GP_M2L_BF16_VALU
-                                                       : GP_M2L_FP32;
-      run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, m, ytmp.p, ld, s);
+      run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, inner_m2l(), ytmp.p, ld, s,
+                    Mt32.p, Mtbf.p, Lt32.p);
     }
     kr_rows_axpy_clear(y, ytmp.p, area.p, N, ld, nr, s);
   }
@@ -1699,6 +1763,7 @@ void fmm_apply_host(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, int n
                     const double *X, double *Y, int precision) {
   hipStream_t s = ctx->stream;
   FmmSystem sys;
+  sys.want_by_offset = nr == 1;
   sys.setup(g, o, precision == 1, s);
   int ld = nr == 1 ? 1 : round_up(nr, 16);
   sys.ensure_work(ld);

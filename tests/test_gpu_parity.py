@@ -69,7 +69,8 @@ def test_fmm_operator_low_precision_close(ctx, order, ncols):
     # ncols > 16 spreads the columns over several 16-wide MFMA row tiles
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     X = np.random.default_rng(order).normal(size=(len(v), ncols))
-    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, m2l_inner=G.GP_M2L_FP32)
+    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, m2l_inner=G.GP_M2L_FP32,
+                       m2l_rank=0)
     Y64 = ctx.fmm_apply(v, X, o)
     Y32 = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
     e32 = np.linalg.norm(Y32 - Y64, axis=0) / np.linalg.norm(Y64, axis=0)
@@ -87,6 +88,45 @@ def test_fmm_operator_low_precision_close(ctx, order, ncols):
     Yv = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
     ex = np.linalg.norm(Yb - Yv, axis=0) / nrm
     assert ex.max() < 2e-5 and ex.max() < 0.05 * e.min()
+
+
+@pytest.mark.parametrize("order,ncols", [(5, 97), (6, 32), (5, 1), (6, 1), (5, 16)])
+def test_fmm_low_rank_operator(ctx, order, ncols):
+    # rank-64 projection of the M2L matrices (inner operator): the far-field
+    # matrices keep <= 7.7e-5 of their Frobenius norm (orders 5 and 6)
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    X = np.random.default_rng(order + ncols).normal(size=(len(v), ncols))
+    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, m2l_inner=G.GP_M2L_FP32,
+                       m2l_rank=0)
+    Y64 = ctx.fmm_apply(v, X, o)
+    nrm = np.linalg.norm(Y64, axis=0)
+    Yf = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
+    ef = np.linalg.norm(Yf - Y64, axis=0) / nrm
+    o.m2l_rank = 64
+    Yl = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
+    el = np.linalg.norm(Yl - Y64, axis=0) / nrm
+    # the projection shows above FP32 rounding, and stays below its bound
+    assert el.max() < 1e-4 and el.min() > 3 * ef.max()
+    # BF16 on matrix cores vs the same BF16-rounded low-rank operands on the
+    # FP32 VALU (single RHS: the by-offset kernels)
+    o.m2l_inner = G.GP_M2L_BF16
+    Yb = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
+    eb = np.linalg.norm(Yb - Y64, axis=0) / nrm
+    assert eb.max() < 5e-3
+    o.m2l_inner = G.GP_M2L_BF16_VALU
+    Yv = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
+    ex = np.linalg.norm(Yb - Yv, axis=0) / nrm
+    assert ex.max() < 2e-5 and ex.max() < 0.05 * eb.min()
+
+
+@pytest.mark.parametrize("order,rank", [(5, 64), (6, 64), (6, 128), (5, 32)])
+def test_mixed_low_rank_matches_fp64(ctx, order, rank):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    common = dict(method=G.GP_METHOD_FMM, tol=1e-11, fmm_order=order)
+    C64, _ = ctx.cap_solve(v, c, 3, G.default_opts(**common))
+    Cm, rep = ctx.cap_solve(v, c, 3, G.default_opts(precision=G.GP_PREC_MIXED, m2l_rank=rank,
+                                                   **common))
+    assert rep["converged"] == 1 and np.max(np.abs(Cm - C64) / np.abs(C64)) < 1e-6
 
 
 def test_m2l_matrix_cores_match_valu(ctx):
