@@ -9,8 +9,11 @@
 //
 // Multi-RHS (k_near_spmm): one work-group per (target leaf, 16-column RHS
 // chunk); 64-row target tiles x 32-column source chunks staged in LDS, each
-// thread owns one target row and 4 RHS columns.  Single RHS (k_near_spmv):
-// one wavefront per target row, coalesced over the row's S values.
+// thread owns one target row and 4 RHS columns.  Single RHS
+// (k_near_spmv_leaf): one work-group per target leaf gathers the leaf's S
+// source values into LDS once, then each wavefront streams whole tile rows
+// with 16-byte loads; k_near_spmv (one wavefront per row, x gathered per row)
+// covers tiles wider than the LDS budget.
 
 #include "gp_common.h"
 #include "gp_kernels.h"
@@ -106,6 +109,62 @@ __global__ __launch_bounds__(256) void k_near_spmv(
   if (lane == 0) y[i] = (double)acc;
 }
 
+// 16-byte vector of values
+template <typename TV>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+
+// One work-group (WPB wavefronts) per target leaf, single RHS: xs = x[cols]
+// in LDS, then wavefront w takes rows w, w + WPB, ...; a row is an unaligned
+// head (< VN values), 16-byte aligned vectors, and a tail.
+template <typename TV, typename TX, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf(
+    const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
+    const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
+    const TX *__restrict__ x, double *__restrict__ y) {
+  using A = typename Acc<TX>::type;
+  using V = typename Vec16<TV>::type;
+  constexpr int VN = Vec16<TV>::n;
+  extern __shared__ unsigned char smem_raw[];
+  A *xs = reinterpret_cast<A *>(smem_raw);
+  const int T = blockIdx.x;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const int64_t base = nf_off[T];
+  for (int s = threadIdx.x; s < S; s += 64 * WPB) xs[s] = (A)x[cols[c0 + s]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int t = threadIdx.x >> 6; t < nt; t += WPB) {
+    const int64_t st = base + (int64_t)t * S;
+    const TV *row = vals + st;
+    const int head = min((int)((VN - (st & (VN - 1))) & (VN - 1)), S);
+    const int nv = (S - head) / VN;
+    A acc = 0;
+    if (lane < head) acc += (A)row[lane] * xs[lane];
+    const V *rv = reinterpret_cast<const V *>(row + head);
+#pragma unroll 4
+    for (int q = lane; q < nv; q += 64) {
+      const V v = rv[q];
+      const A *xq = xs + head + q * VN;
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc += (A)v[k] * xq[k];
+    }
+    const int tl = head + nv * VN + lane;
+    if (tl < S) acc += (A)row[tl] * xs[tl];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if (lane == 0) y[t0 + t] = (double)acc;
+  }
+}
+
 template <typename TV, typename TX>
 void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int ldx, double *y,
                          int ldy, int nr, hipStream_t s) {
@@ -115,7 +174,12 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
   ProfScope ps(sizeof(TV) == 4 ? (nr == 1 ? "near_spmv_f32" : "near_spmm_f32")
                                : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64"),
                s, flops, bytes);
-  if (nr == 1 && ldx == 1 && ldy == 1) {
+  using A = typename Acc<TX>::type;
+  const size_t xs_bytes = (size_t)nf.max_cols * sizeof(A);
+  if (nr == 1 && ldx == 1 && ldy == 1 && xs_bytes <= 48 * 1024) {
+    hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 4>), dim3(nf.n_leaves), dim3(256), xs_bytes, s,
+                       nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+  } else if (nr == 1 && ldx == 1 && ldy == 1) {
     hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
                        nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
   } else {

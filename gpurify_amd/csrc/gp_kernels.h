@@ -77,6 +77,7 @@ struct NearTiles {
   const int *col_off = nullptr;     // n_leaves + 1
   const int *cols = nullptr;        // source panel per column
   const int *panel_leaf = nullptr;  // n
+  int max_cols = 0;                  // widest tile (source columns of one leaf)
 };
 void launch_near_apply(const NearTiles &nf, const double *vals, const double *x, int ldx,
                        double *y, int ldy, int nr, hipStream_t s);

@@ -631,6 +631,9 @@ struct FmmSystem {
     nt.n = N;
     nt.nnz = (long)nnz;
     nt.n_leaves = n_leaves;
+    nt.max_cols = 0;
+    for (int T = 0; T < n_leaves; ++T)
+      nt.max_cols = std::max(nt.max_cols, hcol_off[T + 1] - hcol_off[T]);
     nt.leaf_lo = leaf_lo.p;
     nt.nf_off = nf_off.p;
     nt.col_off = col_off.p;
