@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--leaf-target", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--inner-tol", type=float, default=1e-2)
+    ap.add_argument("--m2l-rank", type=int, default=-1,
+                    help="rank of the inner M2L (0 = full; -1 = the library default)")
     ap.add_argument("--plain-rhs", type=int, default=2, help="RHS sampled for the 1-RHS FP64 rate")
     ap.add_argument("--fp64-rhs", type=int, default=0, help="RHS of the FP64 comparison (0 = all)")
     ap.add_argument("--no-fp64", action="store_true")
@@ -125,6 +127,8 @@ def main():
                   leaf_target=args.leaf_target, tol=args.tol, rhs_first=0, rhs_count=nrhs)
     o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol,
                            m2l_inner=G.GP_M2L_BF16, **common)
+    if args.m2l_rank >= 0:
+        o_mix.m2l_rank = args.m2l_rank
     o_64 = G.default_opts(**common)
     out = {"metric": "converged BEM unknowns/s, mixed vs FP64 GPU @<=1e-6 rel C error",
            "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -152,6 +156,7 @@ def main():
                          "conductors": nc, "rhs": nrhs, "fmm_order": args.order,
                          "octree_depth": last["max_level"],
                          "leaf_target": args.leaf_target, "tol": args.tol,
+                         "m2l_rank": int(o_mix.m2l_rank),
                          "near_nnz": last["near_nnz"], "m2l_pairs": last["m2l_pairs"],
                          "iterations": last["iterations"], "refine_steps": last["refine_steps"],
                          "converged": ok, "global_batch": nrhs}
@@ -230,6 +235,8 @@ def bench_batch(args):
     common = dict(fmm_order=args.order, max_level=0, leaf_target=args.leaf_target, tol=args.tol,
                   rhs_first=0, rhs_count=1)
     o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol, **common)
+    if args.m2l_rank >= 0:
+        o_mix.m2l_rank = args.m2l_rank
     o_64 = G.default_opts(**common)
     out = {"metric": "converged BEM unknowns/s, mixed vs FP64 GPU @<=1e-6 rel C error",
            "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -311,6 +318,8 @@ def bench_adjoint(args):
     common = dict(method=G.GP_METHOD_FMM, fmm_order=args.order, max_level=0,
                   leaf_target=args.leaf_target, tol=args.tol)
     o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol, **common)
+    if args.m2l_rank >= 0:
+        o_mix.m2l_rank = args.m2l_rank
     o_64 = G.default_opts(**common)
     out = {"metric": "converged BEM unknowns/s, mixed vs FP64 GPU @<=1e-6 rel C error",
            "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -1139,74 +1139,120 @@ __global__ __launch_bounds__(256) void k_fill_zero(T *__restrict__ p, size_t n) 
     p[e] = (T)0;
 }
 
-// Low-rank M2L, projection: Mt[b][r] = U^T M[b][r] (U: n3 x kr, row-major)
-// for the rows q -> (box0 + q / nr, q % nr), CK_ROWS rows per work-group
-// staged in LDS; each thread owns one of the kr outputs of CK_ROWS * kr / 256
-// rows.  A BF16 copy goes to Mtb when set.
-constexpr int CK_ROWS = 16;
-template <typename T>
-__global__ __launch_bounds__(256) void k_m2l_compress(int n3, int n3p, int kr, int box0, int nrow,
-                                                      int nr, int ldm, const T *__restrict__ U,
-                                                      const T *__restrict__ M,
-                                                      T *__restrict__ Mt,
+// Low-rank M2L, projection: Mt[q] = U^T M[q] (U: n3 x KR, row-major) for the
+// multipole rows q -> (box0 + q / nr, q % nr): a register-tiled GEMM, 64 rows
+// x KR columns per work-group, the contraction over n3 in LDS stages of 32;
+// thread (ty, tx) owns rows 4 ty .. 4 ty + 3 and columns tx + 16 c.  A BF16
+// copy goes to Mtb when set.
+template <int KR>
+__global__ __launch_bounds__(256) void k_m2l_compress(int n3, int n3p, int box0, int nrow, int nr,
+                                                      int ldm, const float *__restrict__ U,
+                                                      const float *__restrict__ M,
+                                                      float *__restrict__ Mt,
                                                       __bf16 *__restrict__ Mtb) {
-  extern __shared__ unsigned char smem_raw[];
-  T *sm = reinterpret_cast<T *>(smem_raw);  // CK_ROWS x (n3 + 1)
-  const int q0 = blockIdx.x * CK_ROWS, ls = n3 + 1;
-  for (int e = threadIdx.x; e < CK_ROWS * n3; e += 256) {
-    const int i = e / n3, m = e % n3, q = q0 + i;
-    sm[i * ls + m] = q < nrow ? M[((size_t)(box0 + q / nr) * ldm + q % nr) * n3p + m] : (T)0;
-  }
-  __syncthreads();
-  const int G = 256 / kr, j = threadIdx.x % kr, i0 = threadIdx.x / kr;
-  if (i0 >= G) return;
-  constexpr int NA = CK_ROWS / 2;  // rows per thread for kr <= 128
-  T acc[NA];
+  constexpr int TC = KR / 16;
+  __shared__ __align__(16) float sM[32][64 + 4];  // [k][row]
+  __shared__ float sU[32][KR];                    // [k][col]
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int q0 = blockIdx.x * 64;
+  float acc[4][TC];
 #pragma unroll
-  for (int a = 0; a < NA; ++a) acc[a] = 0;
-  for (int m = 0; m < n3; ++m) {
-    const T u = U[(size_t)m * kr + j];
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      const int i = i0 + a * G;
-      if (i < CK_ROWS) acc[a] += u * sm[i * ls + m];
+    for (int c = 0; c < TC; ++c) acc[i][c] = 0.f;
+  for (int k0 = 0; k0 < n3; k0 += 32) {
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+      const int i = e >> 5, k = e & 31, q = q0 + i, m = k0 + k;
+      sM[k][i] = (q < nrow && m < n3) ? M[((size_t)(box0 + q / nr) * ldm + q % nr) * n3p + m] : 0.f;
     }
+    for (int e = threadIdx.x; e < 32 * KR; e += 256) {
+      const int k = e / KR, j = e % KR, m = k0 + k;
+      sU[k][j] = m < n3 ? U[(size_t)m * KR + j] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 32; ++k) {
+      const float4 a = *reinterpret_cast<const float4 *>(&sM[k][ty * 4]);
+      float b[TC];
+#pragma unroll
+      for (int c = 0; c < TC; ++c) b[c] = sU[k][tx + 16 * c];
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {
+        acc[0][c] += a.x * b[c];
+        acc[1][c] += a.y * b[c];
+        acc[2][c] += a.z * b[c];
+        acc[3][c] += a.w * b[c];
+      }
+    }
+    __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < NA; ++a) {
-    const int i = i0 + a * G, q = q0 + i;
-    if (i < CK_ROWS && q < nrow) {
-      const size_t idx = ((size_t)(box0 + q / nr) * ldm + q % nr) * kr + j;
-      Mt[idx] = acc[a];
-      if (Mtb) Mtb[idx] = (__bf16)(float)acc[a];
+  for (int i = 0; i < 4; ++i) {
+    const int q = q0 + ty * 4 + i;
+    if (q >= nrow) continue;
+    const size_t rb = ((size_t)(box0 + q / nr) * ldm + q % nr) * KR;
+#pragma unroll
+    for (int c = 0; c < TC; ++c) {
+      Mt[rb + tx + 16 * c] = acc[i][c];
+      if (Mtb) Mtb[rb + tx + 16 * c] = (__bf16)acc[i][c];
     }
   }
 }
 
 // Low-rank M2L, expansion: Lc[b][m][r] = sum_j U[m][j] Lt[b][j][r] for the
-// columns c -> (targets[c / nr], c % nr), 16 columns per work-group staged in
-// LDS (kr x 16); thread (m0, i) writes nodes m0, m0 + 16, ... of column i.
-template <typename T>
+// columns c -> (b, r) = (targets[c / nr], c % nr): a register-tiled GEMM,
+// 64 nodes x 64 columns per work-group, the contraction over kr in LDS stages
+// of 32; thread (ty, tx) owns nodes 4 ty .. 4 ty + 3 and columns tx + 16 c.
 __global__ __launch_bounds__(256) void k_m2l_expand(int n3, int kr, int ncol, int nr, int ldm,
                                                     const int *__restrict__ targets,
-                                                    const T *__restrict__ U,
-                                                    const T *__restrict__ Lt, T *__restrict__ Lc) {
-  extern __shared__ unsigned char smem_raw[];
-  T *sm = reinterpret_cast<T *>(smem_raw);
-  const int c0 = blockIdx.x * 16;
-  for (int e = threadIdx.x; e < kr * 16; e += 256) {
-    const int j = e / 16, i = e % 16, c = c0 + i;
-    sm[e] = c < ncol ? Lt[((size_t)targets[c / nr] * kr + j) * ldm + c % nr] : (T)0;
+                                                    const float *__restrict__ U,
+                                                    const float *__restrict__ Lt,
+                                                    float *__restrict__ Lc) {
+  __shared__ __align__(16) float sU[32][64 + 4];  // [k][node]
+  __shared__ float sB[32][64];                    // [k][column]
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int c0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[i][c] = 0.f;
+  for (int k0 = 0; k0 < kr; k0 += 32) {
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+      const int i = e >> 5, k = e & 31, m = m0 + i;
+      sU[k][i] = m < n3 ? U[(size_t)m * kr + k0 + k] : 0.f;
+    }
+    for (int e = threadIdx.x; e < 32 * 64; e += 256) {
+      const int k = e >> 6, cc = e & 63, c = c0 + cc;
+      sB[k][cc] = c < ncol ? Lt[((size_t)targets[c / nr] * kr + k0 + k) * ldm + c % nr] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 32; ++k) {
+      const float4 a = *reinterpret_cast<const float4 *>(&sU[k][ty * 4]);
+      float b[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b[c] = sB[k][tx + 16 * c];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        acc[0][c] += a.x * b[c];
+        acc[1][c] += a.y * b[c];
+        acc[2][c] += a.z * b[c];
+        acc[3][c] += a.w * b[c];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const int i = threadIdx.x % 16, m0 = threadIdx.x / 16, c = c0 + i;
-  if (c >= ncol) return;
-  const int b = targets[c / nr], r = c % nr;
-  for (int m = m0; m < n3; m += 16) {
-    const T *um = U + (size_t)m * kr;
-    T acc = 0;
-    for (int j = 0; j < kr; ++j) acc += um[j] * sm[j * 16 + i];
-    Lc[((size_t)b * n3 + m) * ldm + r] = acc;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int cg = c0 + tx + 16 * c;
+    if (cg >= ncol) continue;
+    const int b = targets[cg / nr], r = cg % nr;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + ty * 4 + i;
+      if (m < n3) Lc[((size_t)b * n3 + m) * ldm + r] = acc[i][c];
+    }
   }
 }
 
@@ -1312,10 +1358,19 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     if (nrow > 0) {
       ProfScope pc("m2l_compress_f32", s, 2.0 * nrow * n3 * f.kr,
                    (double)nrow * (n3 + f.kr) * sizeof(T));
-      hipLaunchKernelGGL((k_m2l_compress<T>), dim3((nrow + CK_ROWS - 1) / CK_ROWS), dim3(256),
-                         CK_ROWS * (n3 + 1) * sizeof(T), s, n3, n3p, f.kr, b0, nrow, nr, ldm, f.U,
-                         M, Mt, mbo);
-      GP_LAUNCH_CHECK();
+      if constexpr (sizeof(T) == 4) {
+        const dim3 grid((nrow + 63) / 64);
+        if (f.kr == 32)
+          hipLaunchKernelGGL((k_m2l_compress<32>), grid, dim3(256), 0, s, n3, n3p, b0, nrow, nr,
+                             ldm, f.U, M, Mt, mbo);
+        else if (f.kr == 64)
+          hipLaunchKernelGGL((k_m2l_compress<64>), grid, dim3(256), 0, s, n3, n3p, b0, nrow, nr,
+                             ldm, f.U, M, Mt, mbo);
+        else
+          hipLaunchKernelGGL((k_m2l_compress<128>), grid, dim3(256), 0, s, n3, n3p, b0, nrow, nr,
+                             ldm, f.U, M, Mt, mbo);
+        GP_LAUNCH_CHECK();
+      }
     }
   }
   const int m3p = fm.n3p;
@@ -1378,10 +1433,11 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       const int ncol = f.n_targets * nr;
       ProfScope pe("m2l_expand_f32", s, 2.0 * ncol * n3 * f.kr,
                    (double)ncol * (n3 + f.kr) * sizeof(T));
-      hipLaunchKernelGGL((k_m2l_expand<T>), dim3((ncol + 15) / 16), dim3(256),
-                         (size_t)f.kr * 16 * sizeof(T), s, n3, f.kr, ncol, nr, ldm, f.targets, f.U,
-                         Lt, Lc);
-      GP_LAUNCH_CHECK();
+      if constexpr (sizeof(T) == 4) {
+        hipLaunchKernelGGL(k_m2l_expand, dim3((ncol + 63) / 64, (n3 + 63) / 64), dim3(256), 0, s,
+                           n3, f.kr, ncol, nr, ldm, f.targets, f.U, Lt, Lc);
+        GP_LAUNCH_CHECK();
+      }
     }
   }
   // L2L, level 2 to the leaves.

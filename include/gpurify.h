@@ -108,10 +108,11 @@ typedef struct {
   uint32_t rhs_count;    /* 0 = all columns; unsolved columns of C are left zero */
   uint32_t inner_passes; /* GP_PREC_MIXED: Gram-Schmidt passes of the inner GMRES, 1 or 2 */
   uint32_t inner_basis;  /* GP_PREC_MIXED: inner Krylov basis storage, GP_BASIS_FP32 | _FP64 */
-  uint32_t m2l_rank;     /* GP_PREC_MIXED inner M2L rank: 0 = full n^3; r > 0 projects the
-                            interaction matrices on their leading singular vectors (r rounded
-                            up to 32, 64 or 128; no projection when that reaches n^3 rounded
-                            up to 32).  Default 64.  The FP64 operator is never projected. */
+  uint32_t m2l_rank;     /* GP_PREC_MIXED inner M2L rank: 0 = full n^3 (default); r > 0
+                            projects the interaction matrices on their leading singular
+                            vectors (r rounded up to 32, 64 or 128; no projection when that
+                            reaches n^3 rounded up to 32).  Fewer M2L flops, more refinement
+                            steps.  The FP64 operator is never projected. */
 } gp_cap_opts;
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 
