@@ -409,7 +409,8 @@ def calib_copy(mib):
     FETCH_SIZE / WRITE_SIZE calibrate the counters against known bytes."""
     import torch
 
-    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    # uninitialised input: the add is the only elementwise kernel launched
+    a = torch.empty(mib << 18, dtype=torch.float32, device="cuda:0")
     b = torch.empty_like(a)
     torch.cuda.synchronize()
     torch.add(a, 1.0, out=b)
