@@ -158,4 +158,20 @@ constexpr double kPi = 3.14159265358979323846;
 
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+// The work-groups of a launch are dealt to the 8 XCDs of the MI355X round
+// robin (hardware block b on XCD b % 8), each XCD with its own L2.  xcd_block
+// renumbers them so that each XCD runs runs of `chunk` consecutive logical
+// blocks (XCD x takes runs x, x + 8, ...): neighbouring work items, which read
+// the same inputs (sibling boxes' source multipoles, one leaf's tile for each
+// RHS chunk), run at the same time on one XCD and share its L2, while the
+// XCDs still interleave along the whole range.  chunk 0 keeps the hardware
+// order.  A bijection of [0, nb) (the last partial round keeps its order).
+__device__ inline int xcd_block(int b, int nb, int chunk) {
+  if (chunk <= 0) return b;
+  const int span = 8 * chunk, full = nb / span * span;
+  if (b >= full) return b;
+  const int x = b & 7, i = b >> 3;
+  return ((i / chunk) * 8 + x) * chunk + i % chunk;
+}
+
 }  // namespace gp

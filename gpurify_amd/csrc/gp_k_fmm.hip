@@ -22,6 +22,7 @@
 namespace gp {
 
 constexpr int FM_RC = 16;   // RHS columns per work-group (generic kernels)
+constexpr int XCD_RUN_M2L = 16;  // consecutive targets per XCD run (two sibling groups)
 constexpr int FM_OPT = 16;  // max outputs per thread: n^3 * 16 / 256 for n <= 6
 
 template <typename T>
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
   constexpr int APT = (RTM * 16 * VPR + 255) / 256;
   constexpr bool BPRE = MTW * KS <= 8;
   __shared__ __attribute__((aligned(16))) __bf16 sA[RTM * 16 * LDA];
-  const int t = targets[blockIdx.x];
+  const int t = targets[xcd_block(blockIdx.x, gridDim.x, XCD_RUN_M2L)];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = (lane >> 4) * 8;
   const int RT = min((nr + 15) >> 4, RTM);
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
   constexpr int KP = 20;        // padded LDS row (doubles)
   constexpr int APT = RTM;      // slab elements per thread (RTM * 16 rows x 16)
   __shared__ __attribute__((aligned(32))) double sA[2][RTM * 16 * KP];
-  const int t = targets[blockIdx.x];
+  const int t = targets[xcd_block(blockIdx.x, gridDim.x, XCD_RUN_M2L)];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = (lane >> 4) * 4;
   const int RT = min((nr + 15) >> 4, RTM);
@@ -569,6 +570,102 @@ __global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
   }
 }
 
+// Single right-hand side (ldm = 1), target tiles: one wavefront per tile of
+// 16 targets of one octant, whose interaction lists share most offsets.  Per
+// entry (an offset o of the tile's union), D (node x target) += G_o (node x l)
+// * [M_s1 .. M_s16] (l x target) on the matrix cores, zero columns for
+// targets without that offset; the next entry's source multipoles (and, when
+// they fit in registers, its G_o fragments) load during the current products.
+// Each target's local expansion is complete at the end and written once,
+// scaled by 1/half of its level: no read-modify-write of the locals.
+template <int MT>
+__global__ __launch_bounds__(256) void k_m2l1t_bf16(int n3, int n_tiles,
+                                                    const int *__restrict__ tile_tgt,
+                                                    const int *__restrict__ tile_eoff,
+                                                    const int *__restrict__ ent_oidx,
+                                                    const int *__restrict__ ent_src,
+                                                    const int *__restrict__ box_level,
+                                                    const double *__restrict__ inv_half,
+                                                    const __bf16 *__restrict__ G,
+                                                    const __bf16 *__restrict__ Mb,
+                                                    float *__restrict__ Lc) {
+  constexpr int NP = 16 * MT, KS = NP / 32;
+  constexpr bool APRE = MT * KS <= 8;  // G_o fragments of the next entry in registers
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= n_tiles) return;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
+  f32x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero = {};
+  bf16x8 b[KS], bn[KS];
+  bf16x8 a[APRE ? MT : 1][APRE ? KS : 1], an[APRE ? MT : 1][APRE ? KS : 1];
+  auto load_b = [&](int e, bf16x8 (&B)[KS]) {
+    const int sb = ent_src[(size_t)e * 16 + li];
+    const __bf16 *Ms = Mb + (size_t)(sb < 0 ? 0 : sb) * NP;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) B[k] = sb >= 0 ? *(const bf16x8 *)(Ms + k * 32 + lk) : zero;
+  };
+  auto load_a = [&](int e, bf16x8 (&A)[APRE ? MT : 1][APRE ? KS : 1]) {
+    const __bf16 *Go = G + (size_t)ent_oidx[e] * NP * NP;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int k = 0; k < KS; ++k)
+        if constexpr (APRE) A[mt][k] = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * NP + k * 32 + lk);
+  };
+  if (e0 < e1) {
+    load_b(e0, b);
+    if constexpr (APRE) load_a(e0, a);
+  }
+  for (int e = e0; e < e1; ++e) {
+    const __bf16 *Go = G + (size_t)ent_oidx[e] * NP * NP;
+    if (e + 1 < e1) {
+      load_b(e + 1, bn);
+      if constexpr (APRE) load_a(e + 1, an);
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        bf16x8 g;
+        if constexpr (APRE) g = a[mt][k];
+        else g = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * NP + k * 32 + lk);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(g, b[k], acc[mt], 0, 0, 0);
+      }
+    if (e + 1 < e1) {
+#pragma unroll
+      for (int k = 0; k < KS; ++k) b[k] = bn[k];
+      if constexpr (APRE) {
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int k = 0; k < KS; ++k) a[mt][k] = an[mt][k];
+      }
+    }
+  }
+  // D[i = node][j = target]: lane holds nodes 4 (lane / 16) + v of target lane % 16
+  const int t = tile_tgt[tile * 16 + li];
+  if (t < 0) return;
+  const float ih = (float)inv_half[box_level[t]];
+  float *out = Lc + (size_t)t * n3;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int m0 = mt * 16 + (lane >> 4) * 4;
+    if ((n3 & 3) == 0) {
+      if (m0 < n3)
+        *(f32x4 *)(out + m0) = (f32x4){acc[mt][0] * ih, acc[mt][1] * ih, acc[mt][2] * ih,
+                                       acc[mt][3] * ih};
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (m0 + v < n3) out[m0 + v] = acc[mt][v] * ih;
+    }
+  }
+}
+
 // Single right-hand side (ldm = 1): M2L grouped by offset.  One launch per
 // M2L offset o covers every (target, source) pair with that offset (at most
 // one per target, so the updates of one launch never collide and the sum
@@ -705,6 +802,20 @@ void launch_m2l_by_offset(const FmmDev<T> &f, int n3, const void *Mop, T *Lc, bo
     }
     GP_LAUNCH_CHECK();
   }
+}
+
+void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *Lc,
+                      hipStream_t s) {
+  const dim3 grid((f.n_tiles + 3) / 4);
+  const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf);
+  switch (f.n3p / 16) {
+    case 2: hipLaunchKernelGGL((k_m2l1t_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+    case 4: hipLaunchKernelGGL((k_m2l1t_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+    case 8: hipLaunchKernelGGL((k_m2l1t_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+    case 14: hipLaunchKernelGGL((k_m2l1t_bf16<14>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+    default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
+  }
+  GP_LAUNCH_CHECK();
 }
 
 template <int MT, int RTM>
@@ -1403,7 +1514,12 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     const double mbytes = (double)m3 * nr * sizeof(T);
     const double by = (double)(f.n_boxes + f.n_targets) * mbytes;
     const int MT = m3p / 16, RT = (nr + 15) / 16;
-    if (want_bf16 && by_offset) {
+    if (want_bf16 && by_offset && f.tile_tgt) {
+      if constexpr (sizeof(T) == 4) {
+        ProfScope pl("m2l1t_bf16_mfma", s, fl, (double)f.n_pairs * m3p * 2 + f.n_targets * mbytes);
+        launch_m2l_tiles(fm, m3, mbo, Lo, s);
+      }
+    } else if (want_bf16 && by_offset) {
       ProfScope pl("m2l1_bf16_mfma", s, fl, (double)f.n_pairs * (m3p * 2 + m3 * 8));
       launch_m2l_by_offset(fm, m3, mbo, Lo, true, s);
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {

@@ -35,6 +35,7 @@ constexpr int NF_TM = 64;  // target rows per tile
 constexpr int NF_KS = 32;  // source columns per LDS stage
 constexpr int NF_RC = 16;  // RHS columns per work-group
 constexpr int NF_CPT = 4;  // RHS columns per thread
+constexpr int XCD_RUN_NEAR = 16;  // consecutive (leaf, RHS chunk) blocks per XCD run
 
 template <typename TV, typename TX>
 __global__ __launch_bounds__(256) void k_near_spmm(
@@ -44,8 +45,12 @@ __global__ __launch_bounds__(256) void k_near_spmm(
   using A = typename Acc<TX>::type;
   __shared__ TV sA[NF_TM][NF_KS + 1];
   __shared__ TX sB[NF_KS][NF_RC];
-  const int T = blockIdx.x;
-  const int r0 = blockIdx.y * NF_RC;
+  // logical block = (leaf, RHS chunk), chunks fastest: the chunks of one
+  // leaf run together on one XCD and read its tile from that L2
+  const int nch = (nr + NF_RC - 1) / NF_RC;
+  const int lb = xcd_block(blockIdx.x, gridDim.x, XCD_RUN_NEAR);
+  const int T = lb / nch;
+  const int r0 = (lb % nch) * NF_RC;
   const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
   const int c0 = col_off[T], S = col_off[T + 1] - c0;
   const int64_t base = nf_off[T];
@@ -107,6 +112,75 @@ __global__ __launch_bounds__(256) void k_near_spmv(
   for (int s = lane; s < S; s += 64) acc += (A)row[s] * (A)x[cols[c0 + s]];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
   if (lane == 0) y[i] = (double)acc;
+}
+
+// Multi-RHS, register-tiled: one work-group per (target leaf, RHS chunk of
+// 16 CPT columns); 64-row x 32-source tiles of the leaf's values (stored
+// source-major in LDS) and 32 x 16 CPT slabs of x; thread (ty, tx) owns rows
+// 4 ty .. 4 ty + 3 and columns CPT tx .. CPT tx + CPT - 1, read as vectors.
+template <typename T, int W>
+struct VecN {
+  typedef T type __attribute__((ext_vector_type(W)));
+};
+template <typename TV, typename TX, int CPT>
+__global__ __launch_bounds__(256) void k_near_spmm_rt(
+    const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
+    const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
+    const TX *__restrict__ x, int ldx, double *__restrict__ y, int ldy, int nr) {
+  using A = typename Acc<TX>::type;
+  using V4 = typename VecN<A, 4>::type;
+  using VC = typename VecN<A, CPT>::type;
+  constexpr int RC = 16 * CPT;
+  __shared__ __attribute__((aligned(32))) A sA[32][64 + 4];  // [source][row]
+  __shared__ __attribute__((aligned(32))) A sB[32][RC];      // [source][column]
+  const int nch = (nr + RC - 1) / RC;
+  const int lb = xcd_block(blockIdx.x, gridDim.x, XCD_RUN_NEAR);
+  const int T = lb / nch, r0 = (lb % nch) * RC;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const int64_t base = nf_off[T];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int tm = 0; tm < nt; tm += 64) {
+    A acc[4][CPT];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) acc[i][c] = 0;
+    for (int ks = 0; ks < S; ks += 32) {
+      for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+        const int rr = e >> 5, ss = e & 31, t = tm + rr, s = ks + ss;
+        sA[ss][rr] = (t < nt && s < S) ? (A)vals[base + (int64_t)t * S + s] : (A)0;
+      }
+      for (int e = threadIdx.x; e < 32 * RC; e += 256) {
+        const int ss = e / RC, cc = e % RC, s = ks + ss, r = r0 + cc;
+        sB[ss][cc] = (s < S && r < nr) ? (A)x[(int64_t)cols[c0 + s] * ldx + r] : (A)0;
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int s = 0; s < 32; ++s) {
+        const V4 a = *reinterpret_cast<const V4 *>(&sA[s][4 * ty]);
+        const VC b = *reinterpret_cast<const VC *>(&sB[s][CPT * tx]);
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          acc[0][c] += a[0] * b[c];
+          acc[1][c] += a[1] * b[c];
+          acc[2][c] += a[2] * b[c];
+          acc[3][c] += a[3] * b[c];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = tm + 4 * ty + i;
+      if (t >= nt) continue;
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        const int r = r0 + CPT * tx + c;
+        if (r < nr) y[(int64_t)(t0 + t) * ldy + r] = (double)acc[i][c];
+      }
+    }
+  }
 }
 
 // 16-byte vector of values
@@ -182,8 +256,16 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
   } else if (nr == 1 && ldx == 1 && ldy == 1) {
     hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
                        nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+  } else if (nr > 32) {
+    const dim3 grid(nf.n_leaves * ((nr + 63) / 64));
+    hipLaunchKernelGGL((k_near_spmm_rt<TV, TX, 4>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
+                       nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
+  } else if (nr > 16) {
+    const dim3 grid(nf.n_leaves * ((nr + 31) / 32));
+    hipLaunchKernelGGL((k_near_spmm_rt<TV, TX, 2>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
+                       nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
   } else {
-    dim3 grid(nf.n_leaves, (nr + NF_RC - 1) / NF_RC);
+    const dim3 grid(nf.n_leaves * ((nr + NF_RC - 1) / NF_RC));
     hipLaunchKernelGGL((k_near_spmm<TV, TX>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
                        nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
   }

@@ -134,6 +134,13 @@ struct FmmDev {
   // po_src; null when not built.
   const int *po_off = nullptr;
   const int *po_tgt = nullptr, *po_src = nullptr;
+  // Single right-hand side, target tiles (BF16 M2L): tile T holds 16 targets
+  // of one octant (tile_tgt, -1 padded) and the entries [tile_eoff[T],
+  // tile_eoff[T + 1]): the union of their M2L offsets ascending (ent_oidx),
+  // each with the source box of every target (ent_src, 16 per entry, -1 for
+  // none).  Null when not built.
+  int n_tiles = 0;
+  const int *tile_tgt = nullptr, *tile_eoff = nullptr, *ent_oidx = nullptr, *ent_src = nullptr;
 };
 // Far field y += P_far z.  Multipoles M: n_boxes x ldm x n3p; locals Lc:
 // n_boxes x n3 x ldm.  m2l selects the M2L kernel (GP_M2L_*): BF16 needs Mb

@@ -398,6 +398,8 @@ struct FmmSystem {
   bool want_by_offset = false;
   std::vector<int> po_off;
   DBuf<int> po_tgt, po_src;
+  int n_tiles = 0;
+  DBuf<int> tile_tgt, tile_eoff, ent_oidx, ent_src;
   DBuf<int> tr_off, tr_leaf, tr_stride;
   DBuf<int64_t> tr_base;
   DBuf<double> dinv_area, inv_area, ytmp;
@@ -740,6 +742,61 @@ struct FmmSystem {
         }
       po_tgt = to_device(ht, s);
       po_src = to_device(hs, s);
+      // Target tiles for the BF16 M2L: 16 targets of one octant (box order
+      // within it; a tile never spans octants), the union of their offsets.
+      std::vector<int> ord(htargets);
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hoct[a] < hoct[b]; });
+      std::vector<int> tfirst, tcount;
+      for (size_t i = 0; i < ord.size();) {
+        size_t j = i;
+        while (j < ord.size() && hoct[ord[j]] == hoct[ord[i]]) ++j;
+        for (size_t k = i; k < j; k += 16) {
+          tfirst.push_back((int)k);
+          tcount.push_back((int)std::min<size_t>(16, j - k));
+        }
+        i = j;
+      }
+      n_tiles = (int)tfirst.size();
+      std::vector<int> htt((size_t)n_tiles * 16, -1), heoff(n_tiles + 1, 0);
+      parallel_for(n_tiles, [&](int T) {
+        bool seen[343] = {};
+        int cnt = 0;
+        for (int j = 0; j < tcount[T]; ++j) {
+          const int b = ord[tfirst[T] + j];
+          htt[(size_t)T * 16 + j] = b;
+          for (int q = hil_off[b]; q < hil_off[b + 1]; ++q)
+            if (!seen[hil_oidx[q]]) {
+              seen[hil_oidx[q]] = true;
+              ++cnt;
+            }
+        }
+        heoff[T + 1] = cnt;
+      });
+      for (int T = 0; T < n_tiles; ++T) heoff[T + 1] += heoff[T];
+      std::vector<int> heo(heoff[n_tiles]), hes((size_t)heoff[n_tiles] * 16, -1);
+      parallel_for(n_tiles, [&](int T) {
+        int slot[343];
+        std::fill(slot, slot + 343, -1);
+        for (int j = 0; j < tcount[T]; ++j) {
+          const int b = ord[tfirst[T] + j];
+          for (int q = hil_off[b]; q < hil_off[b + 1]; ++q) slot[hil_oidx[q]] = 0;
+        }
+        int e = heoff[T];
+        for (int o = 0; o < 343; ++o)
+          if (slot[o] == 0) {
+            slot[o] = e;
+            heo[e++] = o;
+          }
+        for (int j = 0; j < tcount[T]; ++j) {
+          const int b = ord[tfirst[T] + j];
+          for (int q = hil_off[b]; q < hil_off[b + 1]; ++q)
+            hes[(size_t)slot[hil_oidx[q]] * 16 + j] = hil_src[q];
+        }
+      });
+      tile_tgt = to_device(htt, s);
+      tile_eoff = to_device(heoff, s);
+      ent_oidx = to_device(heo, s);
+      ent_src = to_device(hes, s);
     }
     // 1/half_l in units of each problem's own root half-width.
     std::vector<double> hih(L + 1);
@@ -822,6 +879,11 @@ struct FmmSystem {
       f.po_off = want_by_offset ? po_off.data() : nullptr;
       f.po_tgt = po_tgt.p;
       f.po_src = po_src.p;
+      f.n_tiles = n_tiles;
+      f.tile_tgt = want_by_offset ? tile_tgt.p : nullptr;
+      f.tile_eoff = tile_eoff.p;
+      f.ent_oidx = ent_oidx.p;
+      f.ent_src = ent_src.p;
     };
     fill(f64);
     f64.w1d = w1d64.p;

@@ -90,6 +90,25 @@ def test_fmm_operator_low_precision_close(ctx, order, ncols):
     assert ex.max() < 2e-5 and ex.max() < 0.05 * e.min()
 
 
+@pytest.mark.parametrize("order", [3, 4, 5, 6])
+def test_single_rhs_bf16_m2l_matches_valu(ctx, order):
+    # one RHS: the target-tiled matrix-core M2L against the same BF16-rounded
+    # operands on the FP32 VALU, on a mesh whose octree has partial
+    # interaction lists (tiles with absent sources)
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    x = np.random.default_rng(order).normal(size=(len(v), 1))
+    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order)
+    Y64 = ctx.fmm_apply(v, x, o)
+    o.m2l_inner = G.GP_M2L_BF16
+    Yb = ctx.fmm_apply(v, x, o, precision=G.GP_PREC_MIXED)
+    o.m2l_inner = G.GP_M2L_BF16_VALU
+    Yv = ctx.fmm_apply(v, x, o, precision=G.GP_PREC_MIXED)
+    nrm = np.linalg.norm(Y64)
+    eb = np.linalg.norm(Yb - Y64) / nrm
+    assert 1e-7 < eb < 5e-3
+    assert np.linalg.norm(Yb - Yv) / nrm < min(2e-5, 0.05 * eb)
+
+
 @pytest.mark.parametrize("order,ncols", [(5, 97), (6, 32), (5, 1), (6, 1), (5, 16)])
 def test_fmm_low_rank_operator(ctx, order, ncols):
     # rank-64 projection of the M2L matrices (inner operator): the far-field
@@ -144,14 +163,21 @@ def test_m2l_matrix_cores_match_valu(ctx):
         assert err.max() < 1e-13, (order, nc)
 
 
-def test_fmm_single_and_block_rhs_agree(ctx):
+@pytest.mark.parametrize("ncols", [3, 16, 20, 40, 97])
+def test_fmm_single_and_block_rhs_agree(ctx, ncols):
+    # block kernels (near-field tiles of 16 / 32 / 64 RHS columns, far field
+    # over RHS blocks) against the single-column kernels, FP64 and FP32
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
-    X = np.random.default_rng(2).normal(size=(len(v), 3))
-    o = G.default_opts(method=G.GP_METHOD_FMM)
+    X = np.random.default_rng(ncols).normal(size=(len(v), ncols))
+    o = G.default_opts(method=G.GP_METHOD_FMM, m2l_inner=G.GP_M2L_FP32)
     Yb = ctx.fmm_apply(v, X, o)
-    for j in range(3):
+    Yb32 = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
+    for j in sorted({0, ncols // 2, ncols - 1}):
         Ys = ctx.fmm_apply(v, X[:, j:j + 1], o)
-        assert np.linalg.norm(Ys[:, 0] - Yb[:, j]) / np.linalg.norm(Yb[:, j]) < 1e-13
+        nrm = np.linalg.norm(Yb[:, j])
+        assert np.linalg.norm(Ys[:, 0] - Yb[:, j]) / nrm < 1e-13
+        Ys32 = ctx.fmm_apply(v, X[:, j:j + 1], o, precision=G.GP_PREC_MIXED)
+        assert np.linalg.norm(Ys32[:, 0] - Yb32[:, j]) / nrm < 2e-6
 
 
 def test_fmm_solve_fp64_matches_oracle(ctx):
