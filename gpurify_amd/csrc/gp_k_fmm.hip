@@ -666,6 +666,102 @@ __global__ __launch_bounds__(256) void k_m2l1t_bf16(int n3, int n_tiles,
   }
 }
 
+// The same over tiles of 64 targets (one per wavefront quarter): the four
+// wavefronts walk one entry list, G_o is staged once per entry in LDS (double
+// buffered, rows padded by 8 BF16) by the whole work-group, each wavefront
+// streams the source multipoles of its 16 targets.  One barrier per entry.
+template <int MT>
+__global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles,
+                                                      const int *__restrict__ tile_tgt,
+                                                      const int *__restrict__ tile_eoff,
+                                                      const int *__restrict__ ent_oidx,
+                                                      const int *__restrict__ ent_src,
+                                                      const int *__restrict__ box_level,
+                                                      const double *__restrict__ inv_half,
+                                                      const __bf16 *__restrict__ G,
+                                                      const __bf16 *__restrict__ Mb,
+                                                      float *__restrict__ Lc) {
+  constexpr int NP = 16 * MT, KS = NP / 32, LDG = NP + 8;
+  constexpr int VPR = NP / 8;                  // 16-byte vectors per G_o row
+  constexpr int GPT = (NP * VPR + 255) / 256;  // G_o vectors per thread
+  __shared__ __attribute__((aligned(16))) __bf16 sG[2][NP * LDG];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tile = blockIdx.x;
+  const int li = lane & 15, lk = (lane >> 4) * 8, col = wave * 16 + li;
+  const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
+  f32x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero = {};
+  bf16x8 b[KS], bn[KS], gp[GPT];
+  auto load_b = [&](int e, bf16x8 (&B)[KS]) {
+    const int sb = ent_src[(size_t)e * 64 + col];
+    const __bf16 *Ms = Mb + (size_t)(sb < 0 ? 0 : sb) * NP;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) B[k] = sb >= 0 ? *(const bf16x8 *)(Ms + k * 32 + lk) : zero;
+  };
+  auto load_g = [&](int e) {
+    const bf16x8 *Go = reinterpret_cast<const bf16x8 *>(G + (size_t)ent_oidx[e] * NP * NP);
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < NP * VPR) gp[i] = Go[v];
+    }
+  };
+  auto store_g = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < NP * VPR) *(bf16x8 *)(&sG[buf][(v / VPR) * LDG + (v % VPR) * 8]) = gp[i];
+    }
+  };
+  if (e0 < e1) {
+    load_g(e0);
+    load_b(e0, b);
+    store_g(0);
+  }
+  __syncthreads();
+  for (int e = e0; e < e1; ++e) {
+    const int buf = (e - e0) & 1;
+    const bool more = e + 1 < e1;
+    if (more) {
+      load_g(e + 1);
+      load_b(e + 1, bn);
+    }
+    const __bf16 *sg = sG[buf];
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const bf16x8 g = *(const bf16x8 *)(sg + (mt * 16 + li) * LDG + k * 32 + lk);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(g, b[k], acc[mt], 0, 0, 0);
+      }
+    if (more) {
+      store_g(buf ^ 1);
+#pragma unroll
+      for (int k = 0; k < KS; ++k) b[k] = bn[k];
+    }
+    __syncthreads();
+  }
+  const int t = tile_tgt[tile * 64 + col];
+  if (t < 0) return;
+  const float ih = (float)inv_half[box_level[t]];
+  float *out = Lc + (size_t)t * n3;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int m0 = mt * 16 + (lane >> 4) * 4;
+    if ((n3 & 3) == 0) {
+      if (m0 < n3)
+        *(f32x4 *)(out + m0) = (f32x4){acc[mt][0] * ih, acc[mt][1] * ih, acc[mt][2] * ih,
+                                       acc[mt][3] * ih};
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (m0 + v < n3) out[m0 + v] = acc[mt][v] * ih;
+    }
+  }
+}
+
 // Single right-hand side (ldm = 1): M2L grouped by offset.  One launch per
 // M2L offset o covers every (target, source) pair with that offset (at most
 // one per target, so the updates of one launch never collide and the sum
@@ -806,8 +902,20 @@ void launch_m2l_by_offset(const FmmDev<T> &f, int n3, const void *Mop, T *Lc, bo
 
 void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *Lc,
                       hipStream_t s) {
-  const dim3 grid((f.n_tiles + 3) / 4);
   const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf);
+  if (f.tile_size == 64) {
+    const dim3 grid(f.n_tiles);
+    switch (f.n3p / 16) {
+      case 2: hipLaunchKernelGGL((k_m2l1t64_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+      case 4: hipLaunchKernelGGL((k_m2l1t64_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+      case 8: hipLaunchKernelGGL((k_m2l1t64_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+      default: throw Error(GP_ERR_INTERNAL, "no 64-target M2L tiles for this n3p");
+    }
+    GP_LAUNCH_CHECK();
+    return;
+  }
+  if (f.tile_size != 16) throw Error(GP_ERR_INTERNAL, "M2L target tiles of 16 or 64");
+  const dim3 grid((f.n_tiles + 3) / 4);
   switch (f.n3p / 16) {
     case 2: hipLaunchKernelGGL((k_m2l1t_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
     case 4: hipLaunchKernelGGL((k_m2l1t_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;

@@ -139,7 +139,7 @@ struct FmmDev {
   // tile_eoff[T + 1]): the union of their M2L offsets ascending (ent_oidx),
   // each with the source box of every target (ent_src, 16 per entry, -1 for
   // none).  Null when not built.
-  int n_tiles = 0;
+  int n_tiles = 0, tile_size = 16;  // targets per tile: 16, or 64 (n3p <= 128)
   const int *tile_tgt = nullptr, *tile_eoff = nullptr, *ent_oidx = nullptr, *ent_src = nullptr;
 };
 // Far field y += P_far z.  Multipoles M: n_boxes x ldm x n3p; locals Lc:

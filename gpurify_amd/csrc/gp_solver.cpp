@@ -398,7 +398,7 @@ struct FmmSystem {
   bool want_by_offset = false;
   std::vector<int> po_off;
   DBuf<int> po_tgt, po_src;
-  int n_tiles = 0;
+  int n_tiles = 0, tile_size = 16;
   DBuf<int> tile_tgt, tile_eoff, ent_oidx, ent_src;
   DBuf<int> tr_off, tr_leaf, tr_stride;
   DBuf<int64_t> tr_base;
@@ -645,6 +645,11 @@ struct FmmSystem {
     // FMM tables.
     const int n3 = n * n * n;
     n3p = round_up(n3, 32);
+    kr = 0;  // low-rank inner M2L: rank 32, 64 or 128 below n3p
+    if (need32 && o.m2l_rank > 0) {
+      kr = o.m2l_rank <= 32 ? 32 : o.m2l_rank <= 64 ? 64 : 128;
+      if (kr >= n3p) kr = 0;
+    }
     auto nodes = cheb_nodes(n);
     std::vector<double> hw1d((size_t)N * 3 * n);
     parallel_for(P, [&](int p) {
@@ -742,28 +747,35 @@ struct FmmSystem {
         }
       po_tgt = to_device(ht, s);
       po_src = to_device(hs, s);
-      // Target tiles for the BF16 M2L: 16 targets of one octant (box order
-      // within it; a tile never spans octants), the union of their offsets.
+      // Target tiles for the BF16 M2L: TS targets of one octant and one list
+      // length (box order within; a tile never spans octants), the union of
+      // their offsets.  TS = 64 where the work-group stages G_o in LDS.
+      const int mp = kr ? kr : n3p;
+      tile_size = mp <= 128 ? 64 : 16;
+      const int TS = tile_size;
       std::vector<int> ord(htargets);
-      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hoct[a] < hoct[b]; });
+      auto len = [&](int b) { return hil_off[b + 1] - hil_off[b]; };
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return hoct[a] != hoct[b] ? hoct[a] < hoct[b] : len(a) > len(b);
+      });
       std::vector<int> tfirst, tcount;
       for (size_t i = 0; i < ord.size();) {
         size_t j = i;
         while (j < ord.size() && hoct[ord[j]] == hoct[ord[i]]) ++j;
-        for (size_t k = i; k < j; k += 16) {
+        for (size_t k = i; k < j; k += TS) {
           tfirst.push_back((int)k);
-          tcount.push_back((int)std::min<size_t>(16, j - k));
+          tcount.push_back((int)std::min<size_t>(TS, j - k));
         }
         i = j;
       }
       n_tiles = (int)tfirst.size();
-      std::vector<int> htt((size_t)n_tiles * 16, -1), heoff(n_tiles + 1, 0);
+      std::vector<int> htt((size_t)n_tiles * TS, -1), heoff(n_tiles + 1, 0);
       parallel_for(n_tiles, [&](int T) {
         bool seen[343] = {};
         int cnt = 0;
         for (int j = 0; j < tcount[T]; ++j) {
           const int b = ord[tfirst[T] + j];
-          htt[(size_t)T * 16 + j] = b;
+          htt[(size_t)T * TS + j] = b;
           for (int q = hil_off[b]; q < hil_off[b + 1]; ++q)
             if (!seen[hil_oidx[q]]) {
               seen[hil_oidx[q]] = true;
@@ -773,7 +785,7 @@ struct FmmSystem {
         heoff[T + 1] = cnt;
       });
       for (int T = 0; T < n_tiles; ++T) heoff[T + 1] += heoff[T];
-      std::vector<int> heo(heoff[n_tiles]), hes((size_t)heoff[n_tiles] * 16, -1);
+      std::vector<int> heo(heoff[n_tiles]), hes((size_t)heoff[n_tiles] * TS, -1);
       parallel_for(n_tiles, [&](int T) {
         int slot[343];
         std::fill(slot, slot + 343, -1);
@@ -790,7 +802,7 @@ struct FmmSystem {
         for (int j = 0; j < tcount[T]; ++j) {
           const int b = ord[tfirst[T] + j];
           for (int q = hil_off[b]; q < hil_off[b + 1]; ++q)
-            hes[(size_t)slot[hil_oidx[q]] * 16 + j] = hil_src[q];
+            hes[(size_t)slot[hil_oidx[q]] * TS + j] = hil_src[q];
         }
       });
       tile_tgt = to_device(htt, s);
@@ -828,10 +840,6 @@ struct FmmSystem {
           std::memcpy(&hr[e], &u, 4);
         }
         Gr32 = to_device(hr, s);
-      }
-      if (o.m2l_rank > 0) {
-        kr = o.m2l_rank <= 32 ? 32 : o.m2l_rank <= 64 ? 64 : 128;
-        if (kr >= n3p) kr = 0;
       }
       if (kr) {
         auto lr = low_rank_tables(n, n3p, kr, hG);
@@ -880,6 +888,7 @@ struct FmmSystem {
       f.po_tgt = po_tgt.p;
       f.po_src = po_src.p;
       f.n_tiles = n_tiles;
+      f.tile_size = tile_size;
       f.tile_tgt = want_by_offset ? tile_tgt.p : nullptr;
       f.tile_eoff = tile_eoff.p;
       f.ent_oidx = ent_oidx.p;
