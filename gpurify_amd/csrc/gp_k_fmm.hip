@@ -762,6 +762,96 @@ __global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles,
   }
 }
 
+// FP64 form of the 64-target tiles (orders <= 4: G_o of 64 x 64 doubles, two
+// LDS buffers of 33 KB): 16x16x4 FP64 matrix-core products, G_o rows padded
+// by 2 doubles in LDS.
+template <int MT>
+__global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles,
+                                                     const int *__restrict__ tile_tgt,
+                                                     const int *__restrict__ tile_eoff,
+                                                     const int *__restrict__ ent_oidx,
+                                                     const int *__restrict__ ent_src,
+                                                     const int *__restrict__ box_level,
+                                                     const double *__restrict__ inv_half,
+                                                     const double *__restrict__ G,
+                                                     const double *__restrict__ M,
+                                                     double *__restrict__ Lc) {
+  constexpr int NP = 16 * MT, K4 = NP / 4, LDG = NP + 2;
+  constexpr int VPR = NP / 2;                  // double2 vectors per G_o row
+  constexpr int GPT = (NP * VPR + 255) / 256;  // G_o vectors per thread
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  __shared__ __attribute__((aligned(16))) double sG[2][NP * LDG];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tile = blockIdx.x;
+  const int li = lane & 15, lq = lane >> 4, col = wave * 16 + li;
+  const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
+  f64x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  double b[K4], bn[K4];
+  f64x2 gp[GPT];
+  auto load_b = [&](int e, double (&B)[K4]) {
+    const int sb = ent_src[(size_t)e * 64 + col];
+    const double *Ms = M + (size_t)(sb < 0 ? 0 : sb) * NP;
+#pragma unroll
+    for (int k = 0; k < K4; ++k) B[k] = sb >= 0 ? Ms[4 * k + lq] : 0.0;
+  };
+  auto load_g = [&](int e) {
+    const f64x2 *Go = reinterpret_cast<const f64x2 *>(G + (size_t)ent_oidx[e] * NP * NP);
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < NP * VPR) gp[i] = Go[v];
+    }
+  };
+  auto store_g = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < NP * VPR) *(f64x2 *)(&sG[buf][(v / VPR) * LDG + (v % VPR) * 2]) = gp[i];
+    }
+  };
+  if (e0 < e1) {
+    load_g(e0);
+    load_b(e0, b);
+    store_g(0);
+  }
+  __syncthreads();
+  for (int e = e0; e < e1; ++e) {
+    const int buf = (e - e0) & 1;
+    const bool more = e + 1 < e1;
+    if (more) {
+      load_g(e + 1);
+      load_b(e + 1, bn);
+    }
+    const double *sg = sG[buf];
+#pragma unroll
+    for (int k = 0; k < K4; ++k)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sg[(mt * 16 + li) * LDG + 4 * k + lq], b[k],
+                                                       acc[mt], 0, 0, 0);
+    if (more) {
+      store_g(buf ^ 1);
+#pragma unroll
+      for (int k = 0; k < K4; ++k) b[k] = bn[k];
+    }
+    __syncthreads();
+  }
+  // D[i][j]: lane holds nodes lq + 4 v of target column li
+  const int t = tile_tgt[tile * 64 + col];
+  if (t < 0) return;
+  const double ih = inv_half[box_level[t]];
+  double *out = Lc + (size_t)t * n3;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = mt * 16 + lq + 4 * v;
+      if (m < n3) out[m] = acc[mt][v] * ih;
+    }
+}
+
 // Single right-hand side (ldm = 1): M2L grouped by offset.  One launch per
 // M2L offset o covers every (target, source) pair with that offset (at most
 // one per target, so the updates of one launch never collide and the sum
@@ -922,6 +1012,17 @@ void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *L
     case 8: hipLaunchKernelGGL((k_m2l1t_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
     case 14: hipLaunchKernelGGL((k_m2l1t_bf16<14>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
     default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
+  }
+  GP_LAUNCH_CHECK();
+}
+
+void launch_m2l_tiles_f64(const FmmDev<double> &f, int n3, const double *M, double *Lc,
+                          hipStream_t s) {
+  const dim3 grid(f.n_tiles);
+  switch (f.n3p / 16) {
+    case 2: hipLaunchKernelGGL((k_m2l1t64_f64<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
+    case 4: hipLaunchKernelGGL((k_m2l1t64_f64<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
+    default: throw Error(GP_ERR_INTERNAL, "no FP64 M2L tiles for this n3p");
   }
   GP_LAUNCH_CHECK();
 }
@@ -1488,7 +1589,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   const bool f32 = sizeof(T) == 4;
   const double exp_bytes = (double)n3 * nr * sizeof(T);
   const bool want_bf16 = (m2l == GP_M2L_BF16);
-  const bool by_offset = ldm == 1 && nr == 1 && f.po_off != nullptr;
+  // single right-hand side kernels: target tiles, or pairs grouped by offset
+  const bool by_offset = ldm == 1 && nr == 1 && (f.po_off != nullptr || f.tile_tgt != nullptr);
   const bool comp = f.kr > 0;
   if (comp && (!f32 || !Mt || !Lt || !f.U || (want_bf16 && !Mtb) || f.kr % 32 || f.kr > 128))
     throw Error(GP_ERR_INTERNAL, "low-rank M2L needs the FP32 operator, kr in {32, 64, 128} "
@@ -1630,7 +1732,14 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     } else if (want_bf16 && by_offset) {
       ProfScope pl("m2l1_bf16_mfma", s, fl, (double)f.n_pairs * (m3p * 2 + m3 * 8));
       launch_m2l_by_offset(fm, m3, mbo, Lo, true, s);
+    } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset && f.tile_tgt &&
+               f.tile_size == 64 && m3p <= 64) {
+      if constexpr (sizeof(T) == 8) {
+        ProfScope pl("m2l1t_f64_mfma", s, fl, (double)f.n_pairs * m3p * 8 + f.n_targets * mbytes);
+        launch_m2l_tiles_f64(fm, m3, Mo, Lo, s);
+      }
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {
+      if (!f.po_off) throw Error(GP_ERR_INTERNAL, "single-RHS M2L pairs by offset not built");
       ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (m3p * 8 + m3 * 16));
       launch_m2l_by_offset(fm, m3, Mo, Lo, false, s);
     } else if (want_bf16) {

@@ -425,6 +425,12 @@ struct FmmSystem {
     if (m2l_outer != GP_M2L_FP64 && m2l_outer != GP_M2L_FP64_VALU)
       throw Error(GP_ERR_INVALID_INPUT, "m2l_outer must be GP_M2L_FP64 or GP_M2L_FP64_VALU");
     const int P = (int)probs.size();
+    double tph = now_s();
+    auto phase = [&](const char *what) {
+      const double t = now_s();
+      GP_LOG("setup phase %-12s %.3f s", what, t - tph);
+      tph = t;
+    };
     trees.assign(P, Tree());
     L0s.assign(P, 1.0);
     L = 2;
@@ -456,6 +462,7 @@ struct FmmSystem {
       for (int p = 0; p < P; ++p) cond_off[p + 1] = cond_off[p] + probs[p]->ncond;
       ncond = cond_off[P];
     }
+    phase("trees");
     // Global box numbering.
     std::vector<std::vector<int>> gid(P);
     for (int p = 0; p < P; ++p) gid[p].assign(trees[p].n_boxes(), -1);
@@ -503,6 +510,7 @@ struct FmmSystem {
           }
     }
 
+    phase("numbering");
     // Scaled panel arrays, per problem in its own unit frame.
     std::vector<double> hv((size_t)N * 12), hc((size_t)N * 3), ha(N);
     std::vector<uint32_t> hcond(N);
@@ -521,6 +529,7 @@ struct FmmSystem {
     area = to_device(ha, s);
     cond = to_device(hcond, s);
 
+    phase("panels");
     // Near-field tiles (cap/fmm_solver.rs:52-91 over bbfmm.rs:189-206), per
     // global leaf: its panels against the panels of its near leaves.
     std::vector<int> hcol_off(n_leaves + 1, 0), hcols, hpl(N);
@@ -642,6 +651,7 @@ struct FmmSystem {
     nt.cols = cols.p;
     nt.panel_leaf = panel_leaf.p;
 
+    phase("near");
     // FMM tables.
     const int n3 = n * n * n;
     n3p = round_up(n3, 32);
@@ -704,6 +714,7 @@ struct FmmSystem {
             }
           }
         }
+    phase("tables");
     // Box arrays in the global numbering.
     std::vector<int> hlevel(nb), hparent(nb), hoct(nb), hchild_off(nb + 1, 0), hchild, htargets;
     std::vector<int> hil_off(nb + 1, 0), hil_src, hil_oidx;
@@ -732,9 +743,11 @@ struct FmmSystem {
         hil_oidx[e] = t.il_oidx[q];
       }
     });
+    phase("boxes");
     pairs = (long)hil_src.size();
-    if (want_by_offset) {
-      // counting sort of the pairs by offset; targets stay ascending
+    if (want_by_offset && n3p > 64) {
+      // FP64 single-RHS M2L beyond the 64-target tiles: the pairs grouped by
+      // offset (counting sort; targets stay ascending)
       po_off.assign(344, 0);
       for (int v : hil_oidx) po_off[v + 1]++;
       for (int o = 0; o < 343; ++o) po_off[o + 1] += po_off[o];
@@ -747,9 +760,12 @@ struct FmmSystem {
         }
       po_tgt = to_device(ht, s);
       po_src = to_device(hs, s);
-      // Target tiles for the BF16 M2L: TS targets of one octant and one list
-      // length (box order within; a tile never spans octants), the union of
-      // their offsets.  TS = 64 where the work-group stages G_o in LDS.
+    }
+    if (want_by_offset) {
+      // Target tiles for the single-RHS M2L (BF16, and FP64 up to n3p = 64):
+      // TS targets of one octant, longest lists first (box order within; a
+      // tile never spans octants), the union of their offsets.  TS = 64 where
+      // the work-group stages G_o in LDS.
       const int mp = kr ? kr : n3p;
       tile_size = mp <= 128 ? 64 : 16;
       const int TS = tile_size;
@@ -810,6 +826,7 @@ struct FmmSystem {
       ent_oidx = to_device(heo, s);
       ent_src = to_device(hes, s);
     }
+    phase("m2l lists");
     // 1/half_l in units of each problem's own root half-width.
     std::vector<double> hih(L + 1);
     for (int l = 0; l <= L; ++l) hih[l] = (double)(1u << l);
@@ -858,6 +875,7 @@ struct FmmSystem {
         }
       }
     }
+    phase("uploads");
     const bool batch = P > 1;
     auto fill = [&](auto &f) {
       f.n = n;
@@ -884,7 +902,7 @@ struct FmmSystem {
       f.inv_half = inv_half.p;
       f.area = area.p;
       f.n3p = n3p;
-      f.po_off = want_by_offset ? po_off.data() : nullptr;
+      f.po_off = (want_by_offset && !po_off.empty()) ? po_off.data() : nullptr;
       f.po_tgt = po_tgt.p;
       f.po_src = po_src.p;
       f.n_tiles = n_tiles;

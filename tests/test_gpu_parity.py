@@ -163,13 +163,15 @@ def test_m2l_matrix_cores_match_valu(ctx):
         assert err.max() < 1e-13, (order, nc)
 
 
-@pytest.mark.parametrize("ncols", [3, 16, 20, 40, 97])
-def test_fmm_single_and_block_rhs_agree(ctx, ncols):
+@pytest.mark.parametrize("order,ncols", [(4, 3), (4, 16), (4, 20), (4, 40), (4, 97), (3, 3),
+                                         (5, 3), (6, 3)])
+def test_fmm_single_and_block_rhs_agree(ctx, order, ncols):
     # block kernels (near-field tiles of 16 / 32 / 64 RHS columns, far field
-    # over RHS blocks) against the single-column kernels, FP64 and FP32
+    # over RHS blocks) against the single-column kernels (FP64 M2L over
+    # 64-target tiles at orders <= 4, grouped by offset above), FP64 and FP32
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     X = np.random.default_rng(ncols).normal(size=(len(v), ncols))
-    o = G.default_opts(method=G.GP_METHOD_FMM, m2l_inner=G.GP_M2L_FP32)
+    o = G.default_opts(method=G.GP_METHOD_FMM, m2l_inner=G.GP_M2L_FP32, fmm_order=order)
     Yb = ctx.fmm_apply(v, X, o)
     Yb32 = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
     for j in sorted({0, ncols // 2, ncols - 1}):
