@@ -171,6 +171,7 @@ def main():
                               "ms_avg": round(1e3 * v["seconds"] / v["launches"], 4)}
                           for k, v in kern[:10]}
         out["roofline"] = roofline(kern, args.workload)
+        out["autotune"] = autotune_choices(ctx)
 
         if not args.no_fp64:
             k64 = min(args.fp64_rhs or nrhs, nrhs)
@@ -270,6 +271,7 @@ def bench_batch(args):
                               "ms_avg": round(1e3 * v["seconds"] / v["launches"], 4)}
                           for k, v in kern[:10]}
         out["roofline"] = roofline(kern, "nets4096")
+        out["autotune"] = autotune_choices(ctx)
         if not args.no_fp64:
             t = time.perf_counter()
             C64, r64 = ctx.cap_solve_batched(probs, o_64)
@@ -351,6 +353,7 @@ def bench_adjoint(args):
                               "ms_avg": round(1e3 * v_["seconds"] / v_["launches"], 4)}
                           for k, v_ in kern[:10]}
         out["roofline"] = roofline(kern, "bundle16")
+        out["autotune"] = autotune_choices(ctx)
         if not args.no_fp64:
             t = time.perf_counter()
             C64, J64, r64 = ctx.cap_jacobian(v, c, nc, params, o_64)
@@ -444,6 +447,12 @@ def measured_traffic(name, workload):
                         and parts[1] == sym and parts[2] == workload:
                     return float(parts[3])
     return None
+
+
+def autotune_choices(ctx):
+    """The context's autotuned kernel variants: kernel/key -> [variant, ms of each]."""
+    return {f"{e['kernel']}/{'x'.join(str(k) for k in e['key'])}": [e["variant"], e["ms"]]
+            for e in ctx.tune_report()}
 
 
 def roofline(kern, workload=None):

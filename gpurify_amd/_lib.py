@@ -95,7 +95,8 @@ EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_device_ready", "gp_device_info",
     "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_cap_solve_batched", "gp_cap_jacobian",
     "gp_fmm_apply", "gp_mesh_groups",
-    "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report",
+    "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
+    "gp_tune_report",
 ]
 
 _lib = None
@@ -151,6 +152,10 @@ def load():
     L.gp_profile_enable.restype = C.c_int
     L.gp_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.gp_profile_report.restype = C.c_int
+    L.gp_tune_force.argtypes = [vp, C.c_int]
+    L.gp_tune_force.restype = C.c_int
+    L.gp_tune_report.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.gp_tune_report.restype = C.c_int
     _lib = L
     return L
 
@@ -228,6 +233,16 @@ class Context:
         import json
         buf = C.create_string_buffer(1 << 16)
         self._check(self._L.gp_profile_report(self._h, buf, 1 << 16))
+        return json.loads(buf.value.decode())
+
+    def tune_force(self, variant: int):
+        """Pin every tunable kernel to `variant` (< 0: autotune again)."""
+        self._check(self._L.gp_tune_force(self._h, int(variant)))
+
+    def tune_report(self) -> list:
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self._L.gp_tune_report(self._h, buf, 1 << 16))
         return json.loads(buf.value.decode())
 
     def info(self) -> str:

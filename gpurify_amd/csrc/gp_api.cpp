@@ -28,6 +28,11 @@ Prof *&active_prof() {
   return p;
 }
 
+Tuner *&active_tuner() {
+  static thread_local Tuner *t = nullptr;
+  return t;
+}
+
 size_t device_bytes_cap() {
   const char *e = std::getenv("GPURIFY_MAX_DEVICE_BYTES");
   if (!e || !*e) return 0;
@@ -60,8 +65,14 @@ int find_gfx950(int ordinal, std::string *name, int *cus) {
 }
 
 struct ActiveProf {
-  explicit ActiveProf(gp_ctx *c) { gp::active_prof() = c ? &c->prof : nullptr; }
-  ~ActiveProf() { gp::active_prof() = nullptr; }
+  explicit ActiveProf(gp_ctx *c) {
+    gp::active_prof() = c ? &c->prof : nullptr;
+    gp::active_tuner() = c ? &c->tuner : nullptr;
+  }
+  ~ActiveProf() {
+    gp::active_prof() = nullptr;
+    gp::active_tuner() = nullptr;
+  }
 };
 
 template <typename F>
@@ -281,6 +292,39 @@ int gp_profile_enable(gp_ctx *ctx, int enable) {
     }
     ctx->prof.recs.clear();
     ctx->prof.on = enable != 0;
+  });
+}
+
+int gp_tune_force(gp_ctx *ctx, int variant) {
+  if (!ctx) return GP_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    ctx->tuner.force = variant < 0 ? -1 : variant;
+    ctx->tuner.table.clear();
+  });
+}
+
+int gp_tune_report(gp_ctx *ctx, char *json, size_t len) {
+  if (!ctx || !json || len == 0) return GP_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    std::string out = "[";
+    char buf[256];
+    for (size_t i = 0; i < ctx->tuner.table.size(); ++i) {
+      const auto &e = ctx->tuner.table[i];
+      std::snprintf(buf, sizeof(buf), "%s{\"kernel\": \"%s\", \"key\": [%lld, %lld, %lld], \"variant\": %d, \"ms\": [",
+                    i ? ", " : "", e.kernel.c_str(), (long long)e.k0, (long long)e.k1,
+                    (long long)e.k2, e.variant);
+      out += buf;
+      for (size_t v = 0; v < e.ms.size(); ++v) {
+        std::snprintf(buf, sizeof(buf), "%s%.4f", v ? ", " : "", e.ms[v]);
+        out += buf;
+      }
+      out += "]}";
+    }
+    out += "]";
+    if (out.size() + 1 > len) throw gp::Error(GP_ERR_INVALID_INPUT, "tuning report buffer too small");
+    std::memcpy(json, out.c_str(), out.size() + 1);
   });
 }
 

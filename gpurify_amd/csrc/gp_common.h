@@ -152,6 +152,70 @@ struct ProfScope {
   }
 };
 
+// Kernel-variant autotuner, one per context.  The first launch of a tunable
+// kernel with a given shape key runs each of its variants on the real
+// operands (every variant writes the same outputs) and times them with HIP
+// events on the launch stream, after one untimed warm-up launch; the fastest
+// then serves every later launch with that key (and is launched once more so
+// the outputs are its own).  A stream under graph capture launches the
+// default variant 0 of an untuned key.  force >= 0 pins every tunable kernel
+// to that variant (clamped to its count), for tests.
+struct Tuner {
+  int force = -1;
+  struct Entry {
+    std::string kernel;
+    int64_t k0, k1, k2;
+    int variant;
+    std::vector<float> ms;
+  };
+  std::vector<Entry> table;
+};
+Tuner *&active_tuner();  // per thread, set by the API entry for its context
+
+template <typename F>
+void tuned_launch(const char *kernel, int64_t k0, int64_t k1, int64_t k2, int nvar, hipStream_t s,
+                  F &&launch) {
+  Tuner *t = active_tuner();
+  if (!t || nvar <= 1) {
+    launch(0);
+    return;
+  }
+  if (t->force >= 0) {
+    launch(t->force < nvar ? t->force : nvar - 1);
+    return;
+  }
+  for (const auto &e : t->table)
+    if (e.kernel == kernel && e.k0 == k0 && e.k1 == k1 && e.k2 == k2) {
+      launch(e.variant);
+      return;
+    }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+  if (cs != hipStreamCaptureStatusNone) {
+    launch(0);
+    return;
+  }
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "hipEventCreate");
+  hip_check(hipEventCreate(&b), "hipEventCreate");
+  std::vector<float> ms(nvar, 0.f);
+  launch(0);  // warm-up: first touch of the operands
+  for (int v = 0; v < nvar; ++v) {
+    hip_check(hipEventRecord(a, s), "hipEventRecord");
+    launch(v);
+    hip_check(hipEventRecord(b, s), "hipEventRecord");
+    hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms[v], a, b), "hipEventElapsedTime");
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  int best = 0;
+  for (int v = 1; v < nvar; ++v)
+    if (ms[v] < ms[best]) best = v;
+  if (best != nvar - 1) launch(best);
+  t->table.push_back({kernel, k0, k1, k2, best, ms});
+}
+
 // Physical constants (crates/pex/src/quasistatic/constants.rs:13-20).
 constexpr double kEps0 = 8.8541878128e-12;
 constexpr double kPi = 3.14159265358979323846;

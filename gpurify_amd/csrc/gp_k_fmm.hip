@@ -22,7 +22,6 @@
 namespace gp {
 
 constexpr int FM_RC = 16;   // RHS columns per work-group (generic kernels)
-constexpr int XCD_RUN_M2L = 16;  // consecutive targets per XCD run (two sibling groups)
 constexpr int FM_OPT = 16;  // max outputs per thread: n^3 * 16 / 256 for n <= 6
 
 template <typename T>
@@ -225,7 +224,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // on the matrix cores, and so are its G_off fragments when they fit
 // (MTW * n3p / 32 <= 8 fragments, orders 2 to 5).
 template <int MT, int RTM>
-__global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
+__global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm, int xrun,
                                                        const int *__restrict__ targets,
                                                        const int *__restrict__ il_off,
                                                        const int *__restrict__ il_src,
@@ -243,7 +242,7 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
   constexpr int APT = (RTM * 16 * VPR + 255) / 256;
   constexpr bool BPRE = MTW * KS <= 8;
   __shared__ __attribute__((aligned(16))) __bf16 sA[RTM * 16 * LDA];
-  const int t = targets[xcd_block(blockIdx.x, gridDim.x, XCD_RUN_M2L)];
+  const int t = targets[xcd_block(blockIdx.x, gridDim.x, xrun)];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = (lane >> 4) * 8;
   const int RT = min((nr + 15) >> 4, RTM);
@@ -337,7 +336,7 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm,
 // successive MFMAs: in MFMA kk lane group g = lane / 16 supplies contraction
 // index k0 + 4 g + kk for both operands, so the product sums every index once.
 template <int MT, int RTM>
-__global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
+__global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm, int xrun,
                                                       const int *__restrict__ targets,
                                                       const int *__restrict__ il_off,
                                                       const int *__restrict__ il_src,
@@ -353,7 +352,7 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm,
   constexpr int KP = 20;        // padded LDS row (doubles)
   constexpr int APT = RTM;      // slab elements per thread (RTM * 16 rows x 16)
   __shared__ __attribute__((aligned(32))) double sA[2][RTM * 16 * KP];
-  const int t = targets[xcd_block(blockIdx.x, gridDim.x, XCD_RUN_M2L)];
+  const int t = targets[xcd_block(blockIdx.x, gridDim.x, xrun)];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = (lane >> 4) * 4;
   const int RT = min((nr + 15) >> 4, RTM);
@@ -671,7 +670,7 @@ __global__ __launch_bounds__(256) void k_m2l1t_bf16(int n3, int n_tiles,
 // buffered, rows padded by 8 BF16) by the whole work-group, each wavefront
 // streams the source multipoles of its 16 targets.  One barrier per entry.
 template <int MT>
-__global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles,
+__global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles, int xrun,
                                                       const int *__restrict__ tile_tgt,
                                                       const int *__restrict__ tile_eoff,
                                                       const int *__restrict__ ent_oidx,
@@ -686,7 +685,7 @@ __global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles,
   constexpr int GPT = (NP * VPR + 255) / 256;  // G_o vectors per thread
   __shared__ __attribute__((aligned(16))) __bf16 sG[2][NP * LDG];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(blockIdx.x, gridDim.x, xrun);
   const int li = lane & 15, lk = (lane >> 4) * 8, col = wave * 16 + li;
   const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
   f32x4 acc[MT];
@@ -766,7 +765,7 @@ __global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles,
 // LDS buffers of 33 KB): 16x16x4 FP64 matrix-core products, G_o rows padded
 // by 2 doubles in LDS.
 template <int MT>
-__global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles,
+__global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles, int xrun,
                                                      const int *__restrict__ tile_tgt,
                                                      const int *__restrict__ tile_eoff,
                                                      const int *__restrict__ ent_oidx,
@@ -782,7 +781,7 @@ __global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles,
   typedef double f64x2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) double sG[2][NP * LDG];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(blockIdx.x, gridDim.x, xrun);
   const int li = lane & 15, lq = lane >> 4, col = wave * 16 + li;
   const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
   f64x4 acc[MT];
@@ -990,15 +989,15 @@ void launch_m2l_by_offset(const FmmDev<T> &f, int n3, const void *Mop, T *Lc, bo
   }
 }
 
-void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *Lc,
+void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *Lc, int xrun,
                       hipStream_t s) {
   const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf);
   if (f.tile_size == 64) {
     const dim3 grid(f.n_tiles);
     switch (f.n3p / 16) {
-      case 2: hipLaunchKernelGGL((k_m2l1t64_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
-      case 4: hipLaunchKernelGGL((k_m2l1t64_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
-      case 8: hipLaunchKernelGGL((k_m2l1t64_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+      case 2: hipLaunchKernelGGL((k_m2l1t64_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+      case 4: hipLaunchKernelGGL((k_m2l1t64_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+      case 8: hipLaunchKernelGGL((k_m2l1t64_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
       default: throw Error(GP_ERR_INTERNAL, "no 64-target M2L tiles for this n3p");
     }
     GP_LAUNCH_CHECK();
@@ -1017,11 +1016,11 @@ void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *L
 }
 
 void launch_m2l_tiles_f64(const FmmDev<double> &f, int n3, const double *M, double *Lc,
-                          hipStream_t s) {
+                          int xrun, hipStream_t s) {
   const dim3 grid(f.n_tiles);
   switch (f.n3p / 16) {
-    case 2: hipLaunchKernelGGL((k_m2l1t64_f64<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
-    case 4: hipLaunchKernelGGL((k_m2l1t64_f64<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
+    case 2: hipLaunchKernelGGL((k_m2l1t64_f64<2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
+    case 4: hipLaunchKernelGGL((k_m2l1t64_f64<4>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
     default: throw Error(GP_ERR_INTERNAL, "no FP64 M2L tiles for this n3p");
   }
   GP_LAUNCH_CHECK();
@@ -1029,15 +1028,17 @@ void launch_m2l_tiles_f64(const FmmDev<double> &f, int n3, const double *M, doub
 
 template <int MT, int RTM>
 void launch_m2l_bf16(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb, float *Lc,
-                     hipStream_t s) {
+                     int xrun, hipStream_t s) {
   hipLaunchKernelGGL((k_m2l_bf16_pipe<MT, RTM>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                     xrun,
                      f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
                      reinterpret_cast<const __bf16 *>(f.Gbf), Mb, Lc);
 }
 template <int MT, int RTM>
 void launch_m2l_f64(const FmmDev<double> &f, int n3, int nr, int ldm, const double *M, double *Lc,
-                    hipStream_t s) {
+                    int xrun, hipStream_t s) {
   hipLaunchKernelGGL((k_m2l_f64_pipe<MT, RTM>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
+                     xrun,
                      f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half, f.G, M, Lc);
 }
 // Instantiation by (MT, RTM): MT = n3p / 16 in {2, 4, 8, 14} (orders 2-6),
@@ -1727,7 +1728,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     if (want_bf16 && by_offset && f.tile_tgt) {
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l1t_bf16_mfma", s, fl, (double)f.n_pairs * m3p * 2 + f.n_targets * mbytes);
-        launch_m2l_tiles(fm, m3, mbo, Lo, s);
+        tuned_launch("m2l1t_bf16", f.n_tiles, m3p, f.tile_size, 2, s,
+                     [&](int v) { launch_m2l_tiles(fm, m3, mbo, Lo, v ? 16 : 0, s); });
       }
     } else if (want_bf16 && by_offset) {
       ProfScope pl("m2l1_bf16_mfma", s, fl, (double)f.n_pairs * (m3p * 2 + m3 * 8));
@@ -1736,7 +1738,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                f.tile_size == 64 && m3p <= 64) {
       if constexpr (sizeof(T) == 8) {
         ProfScope pl("m2l1t_f64_mfma", s, fl, (double)f.n_pairs * m3p * 8 + f.n_targets * mbytes);
-        launch_m2l_tiles_f64(fm, m3, Mo, Lo, s);
+        tuned_launch("m2l1t_f64", f.n_tiles, m3p, f.tile_size, 2, s,
+                     [&](int v) { launch_m2l_tiles_f64(fm, m3, Mo, Lo, v ? 16 : 0, s); });
       }
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {
       if (!f.po_off) throw Error(GP_ERR_INTERNAL, "single-RHS M2L pairs by offset not built");
@@ -1745,14 +1748,21 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * m3 * nr * 2 + f.n_targets * mbytes);
-        GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, fm, m3, nr, ldm, mbo, Lo, s);
+        // work-group order: XCD runs of 16 targets, the hardware order, runs of 64
+        tuned_launch("m2l_bf16_pipe", f.n_targets, MT * 100 + RT, ldm, 3, s, [&](int v) {
+          const int xr = v == 0 ? 16 : v == 1 ? 0 : 64;
+          GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, fm, m3, nr, ldm, mbo, Lo, xr, s);
+        });
       } else {
         throw Error(GP_ERR_INTERNAL, "BF16 M2L runs on the FP32 operator only");
       }
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && ldm % 16 == 0 && ldm <= 128) {
       if constexpr (sizeof(T) == 8) {
         ProfScope pl("m2l_f64_mfma", s, fl, by);
-        GP_M2L_DISPATCH(launch_m2l_f64, MT, RT, fm, m3, nr, ldm, Mo, Lo, s);
+        tuned_launch("m2l_f64_pipe", f.n_targets, MT * 100 + RT, ldm, 3, s, [&](int v) {
+          const int xr = v == 0 ? 16 : v == 1 ? 0 : 64;
+          GP_M2L_DISPATCH(launch_m2l_f64, MT, RT, fm, m3, nr, ldm, Mo, Lo, xr, s);
+        });
       }
     } else {
       ProfScope pl(bf16_valu ? "m2l_bf16_valu" : f32 ? "m2l_f32" : "m2l_f64", s, fl, by);

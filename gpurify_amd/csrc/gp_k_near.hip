@@ -250,24 +250,54 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
                s, flops, bytes);
   using A = typename Acc<TX>::type;
   const size_t xs_bytes = (size_t)nf.max_cols * sizeof(A);
-  if (nr == 1 && ldx == 1 && ldy == 1 && xs_bytes <= 48 * 1024) {
-    hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 4>), dim3(nf.n_leaves), dim3(256), xs_bytes, s,
-                       nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
-  } else if (nr == 1 && ldx == 1 && ldy == 1) {
+  // kernel variants, fastest per shape picked by the context's autotuner
+  auto spmv_row = [&] {
     hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
                        nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
-  } else if (nr > 32) {
-    const dim3 grid(nf.n_leaves * ((nr + 63) / 64));
-    hipLaunchKernelGGL((k_near_spmm_rt<TV, TX, 4>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
-                       nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
-  } else if (nr > 16) {
-    const dim3 grid(nf.n_leaves * ((nr + 31) / 32));
-    hipLaunchKernelGGL((k_near_spmm_rt<TV, TX, 2>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
-                       nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
-  } else {
+  };
+  auto spmm16 = [&] {
     const dim3 grid(nf.n_leaves * ((nr + NF_RC - 1) / NF_RC));
     hipLaunchKernelGGL((k_near_spmm<TV, TX>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
                        nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
+  };
+  auto spmm_rt = [&](int cpt) {
+    if (cpt == 4)
+      hipLaunchKernelGGL((k_near_spmm_rt<TV, TX, 4>), dim3(nf.n_leaves * ((nr + 63) / 64)),
+                         dim3(256), 0, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, ldx,
+                         y, ldy, nr);
+    else
+      hipLaunchKernelGGL((k_near_spmm_rt<TV, TX, 2>), dim3(nf.n_leaves * ((nr + 31) / 32)),
+                         dim3(256), 0, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, ldx,
+                         y, ldy, nr);
+  };
+  const char *tname = sizeof(TV) == 4 ? (nr == 1 ? "near_spmv_f32" : "near_spmm_f32")
+                                      : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64");
+  if (nr == 1 && ldx == 1 && ldy == 1) {
+    // leaf-staged with 4 or 8 wavefronts a leaf, or one wavefront per row
+    const bool leaf_ok = xs_bytes <= 48 * 1024;
+    tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 3 : 1, s, [&](int v) {
+      if (!leaf_ok || v == 2)
+        spmv_row();
+      else if (v == 0)
+        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 4>), dim3(nf.n_leaves), dim3(256), xs_bytes,
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+      else
+        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 8>), dim3(nf.n_leaves), dim3(512), xs_bytes,
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+    });
+  } else if (nr > 16) {
+    // register tiles of 64 or 32 RHS columns, or 16-column work-groups
+    const int c0 = nr > 32 ? 4 : 2;
+    tuned_launch(tname, nf.n_leaves, nr, ldx, 3, s, [&](int v) {
+      if (v == 0)
+        spmm_rt(c0);
+      else if (v == 1)
+        spmm_rt(6 - c0);
+      else
+        spmm16();
+    });
+  } else {
+    spmm16();
   }
   GP_LAUNCH_CHECK();
 }

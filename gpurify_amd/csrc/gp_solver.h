@@ -20,6 +20,7 @@ struct gp_ctx {
   std::string device_name;
   int cu_count = 0;
   gp::Prof prof;
+  gp::Tuner tuner;
 };
 
 namespace gp {

@@ -201,6 +201,16 @@ int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
 int gp_profile_enable(gp_ctx *ctx, int enable);
 int gp_profile_report(gp_ctx *ctx, char *json, size_t len);
 
+/* Kernel-variant autotuner (no reference counterpart; the context's
+ * autotune cache).  Tunable kernels (near-field products, M2L work-group
+ * orders) time their variants on the first launch of each shape and keep the
+ * fastest for the context's lifetime.  gp_tune_force(ctx, v) pins every
+ * tunable kernel to variant v (clamped to its count) and clears the cache;
+ * v < 0 returns to tuning.  gp_tune_report writes a JSON list
+ * [{"kernel": name, "key": [k0, k1, k2], "variant": v, "ms": [t0, ...]}, ...]. */
+int gp_tune_force(gp_ctx *ctx, int variant);
+int gp_tune_report(gp_ctx *ctx, char *json, size_t len);
+
 /* ------------------------------------------------------------ layout bridge
  * One (net, layer) volume group of bridge.rs:58-65, 131-141: the rectangles
  * of one net on one layer, extruded from height_nm to height_nm+thickness_nm.
