@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--inner-tol", type=float, default=1e-2)
     ap.add_argument("--m2l-rank", type=int, default=-1,
                     help="rank of the inner M2L (0 = full; -1 = the library default)")
+    ap.add_argument("--m2l-inner", choices=["bf16", "fp32"], default="bf16",
+                    help="inner M2L: BF16 matrix cores or the FP32 VALU kernel")
     ap.add_argument("--plain-rhs", type=int, default=2, help="RHS sampled for the 1-RHS FP64 rate")
     ap.add_argument("--fp64-rhs", type=int, default=0, help="RHS of the FP64 comparison (0 = all)")
     ap.add_argument("--no-fp64", action="store_true")
@@ -126,7 +128,8 @@ def main():
     common = dict(method=G.GP_METHOD_FMM, fmm_order=args.order, max_level=0,
                   leaf_target=args.leaf_target, tol=args.tol, rhs_first=0, rhs_count=nrhs)
     o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol,
-                           m2l_inner=G.GP_M2L_BF16, **common)
+                           m2l_inner=G.GP_M2L_BF16 if args.m2l_inner == "bf16" else G.GP_M2L_FP32,
+                           **common)
     if args.m2l_rank >= 0:
         o_mix.m2l_rank = args.m2l_rank
     o_64 = G.default_opts(**common)
@@ -465,7 +468,7 @@ def roofline(kern, workload=None):
         bound, peak, unit, per = "mfma", PEAKS["fp64_TFs"], "TFLOP/s", v["flops"]
     elif name.startswith("near_spmm_f32") or name.startswith("m2l_f32"):
         bound, peak, unit, per = "valu", PEAKS["fp32_TFs"], "TFLOP/s", v["flops"]
-    elif name.endswith("_f64"):
+    elif name.endswith("_f64") or name.startswith("sens_"):
         bound, peak, unit, per = "valu", PEAKS["fp64_TFs"], "TFLOP/s", v["flops"]
     else:
         bound, peak, unit, per = "hbm", PEAKS["hbm_GBs"], "GB/s", v["bytes"]

@@ -60,7 +60,7 @@ class GpCapOpts(C.Structure):
                 ("m2l_outer", C.c_uint32), ("strict", C.c_uint32), ("dense_limit", C.c_uint32),
                 ("rhs_first", C.c_uint32), ("rhs_count", C.c_uint32),
                 ("inner_passes", C.c_uint32), ("inner_basis", C.c_uint32),
-                ("m2l_rank", C.c_uint32)]
+                ("m2l_rank", C.c_uint32), ("step_graphs", C.c_uint32)]
 
 
 class GpParamMap(C.Structure):

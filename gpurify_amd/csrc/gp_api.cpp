@@ -177,6 +177,7 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->inner_passes = 1;
   o->inner_basis = GP_BASIS_FP32;
   o->m2l_rank = 0;
+  o->step_graphs = 1;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,

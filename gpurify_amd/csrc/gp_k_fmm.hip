@@ -569,6 +569,137 @@ __global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
   }
 }
 
+// Sibling pairs: one work-group per (pair of sibling targets, RTM RHS tiles
+// from tile rt0 = RTM blockIdx.y).  The pair's entries are the union of the
+// two interaction lists ordered by source box, each with the source's offset
+// for either target (-1: not in that list).  Every source block is staged in
+// LDS once and feeds the matrix-core products of both targets (two
+// accumulator sets), so a source shared by the siblings is read once: the
+// per-target kernel reads a source block from HBM for every pair.  The next
+// entry's source rows and both G_off fragment sets are prefetched into
+// registers during the current products.
+template <int MT, int RTM>
+__global__ __launch_bounds__(256) void k_m2l_bf16_pair(int n3, int nr, int ldm, int xrun,
+                                                       const int *__restrict__ pair_tgt,
+                                                       const int *__restrict__ pair_off,
+                                                       const int *__restrict__ pe_src,
+                                                       const int *__restrict__ pe_o1,
+                                                       const int *__restrict__ pe_o2,
+                                                       const int *__restrict__ box_level,
+                                                       const double *__restrict__ inv_half,
+                                                       const __bf16 *__restrict__ G,
+                                                       const __bf16 *__restrict__ Mb,
+                                                       float *__restrict__ Lc) {
+  constexpr int NP = 16 * MT, KS = NP / 32, MTW = (MT + 3) / 4, LDA = NP + 8, VPR = NP / 8;
+  constexpr int APT = (RTM * 16 * VPR + 255) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 sA[RTM * 16 * LDA];
+  const int grp = xcd_block(blockIdx.x, gridDim.x, xrun);
+  const int rt0 = blockIdx.y * RTM;
+  const int RT = min((nr + 15) >> 4, rt0 + RTM) - rt0;
+  const int t1 = pair_tgt[2 * grp], t2 = pair_tgt[2 * grp + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  const int AV = RT * 16 * VPR;
+  const size_t MS = (size_t)ldm * NP;
+  f32x4 acc1[RTM][MTW], acc2[RTM][MTW];
+#pragma unroll
+  for (int i = 0; i < RTM; ++i)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) acc1[i][j] = acc2[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int q0 = pair_off[grp], q1 = pair_off[grp + 1];
+  bf16x8 apre[APT];
+  bf16x8 b1[MTW][KS], b2[MTW][KS], n1[MTW][KS], n2[MTW][KS];
+  auto load_a = [&](int q) {
+    const bf16x8 *src =
+        reinterpret_cast<const bf16x8 *>(Mb + (size_t)pe_src[q] * MS + (size_t)rt0 * 16 * NP);
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < AV) apre[i] = src[v];
+    }
+  };
+  auto load_b = [&](int o, bf16x8 (&dst)[MTW][KS]) {
+    if (o < 0) return;
+    const __bf16 *Go = G + (size_t)o * NP * NP;
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) {
+      const int mt = wave * MTW + j;
+      if (mt < MT)
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+          dst[j][k] = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * NP + k * 32 + lk);
+    }
+  };
+  int o1 = -1, o2 = -1;
+  if (q0 < q1) {
+    load_a(q0);
+    o1 = pe_o1[q0];
+    o2 = pe_o2[q0];
+    load_b(o1, b1);
+    load_b(o2, b2);
+  }
+  for (int q = q0; q < q1; ++q) {
+    __syncthreads();  // previous entry's LDS reads are done
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < AV) *(bf16x8 *)(sA + (v / VPR) * LDA + (v % VPR) * 8) = apre[i];
+    }
+    __syncthreads();
+    int p1 = -1, p2 = -1;
+    if (q + 1 < q1) {
+      load_a(q + 1);
+      p1 = pe_o1[q + 1];
+      p2 = pe_o2[q + 1];
+      load_b(p1, n1);
+      load_b(p2, n2);
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+#pragma unroll
+      for (int rt = 0; rt < RTM; ++rt) {
+        if (rt < RT) {
+          const bf16x8 a = *(const bf16x8 *)(sA + (rt * 16 + li) * LDA + k * 32 + lk);
+#pragma unroll
+          for (int j = 0; j < MTW; ++j)
+            if (wave * MTW + j < MT) {
+              if (o1 >= 0)
+                acc1[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1[j][k], acc1[rt][j], 0, 0, 0);
+              if (o2 >= 0)
+                acc2[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b2[j][k], acc2[rt][j], 0, 0, 0);
+            }
+        }
+      }
+    }
+    o1 = p1;
+    o2 = p2;
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int k = 0; k < KS; ++k) {
+        b1[j][k] = n1[j][k];
+        b2[j][k] = n2[j][k];
+      }
+  }
+#pragma unroll
+  for (int s2 = 0; s2 < 2; ++s2) {
+    const int t = s2 ? t2 : t1;
+    if (t < 0) continue;
+    const float ih = (float)inv_half[box_level[t]];
+    float *out = Lc + (size_t)t * n3 * ldm;
+#pragma unroll
+    for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+      for (int j = 0; j < MTW; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = (rt0 + rt) * 16 + (lane >> 4) * 4 + v, m = (wave * MTW + j) * 16 + li;
+          if (rt < RT && r < nr && m < n3)
+            out[(size_t)m * ldm + r] = (s2 ? acc2[rt][j][v] : acc1[rt][j][v]) * ih;
+        }
+  }
+}
+
 // Single right-hand side (ldm = 1), target tiles: one wavefront per tile of
 // 16 targets of one octant, whose interaction lists share most offsets.  Per
 // entry (an offset o of the tile's union), D (node x target) += G_o (node x l)
@@ -1034,6 +1165,20 @@ void launch_m2l_bf16(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf
                      f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
                      reinterpret_cast<const __bf16 *>(f.Gbf), Mb, Lc);
 }
+void launch_m2l_pairs(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
+                      float *Lc, int xrun, hipStream_t s) {
+  constexpr int RTM = 4;
+  const dim3 grid(f.n_pairs2, ((nr + 15) / 16 + RTM - 1) / RTM);
+  const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf);
+  switch (f.n3p / 16) {
+    case 2: hipLaunchKernelGGL((k_m2l_bf16_pair<2, RTM>), grid, dim3(256), 0, s, n3, nr, ldm, xrun, f.pair_tgt, f.pair_off, f.pe_src, f.pe_o1, f.pe_o2, f.box_level, f.inv_half, G, Mb, Lc); break;
+    case 4: hipLaunchKernelGGL((k_m2l_bf16_pair<4, RTM>), grid, dim3(256), 0, s, n3, nr, ldm, xrun, f.pair_tgt, f.pair_off, f.pe_src, f.pe_o1, f.pe_o2, f.box_level, f.inv_half, G, Mb, Lc); break;
+    case 8: hipLaunchKernelGGL((k_m2l_bf16_pair<8, RTM>), grid, dim3(256), 0, s, n3, nr, ldm, xrun, f.pair_tgt, f.pair_off, f.pe_src, f.pe_o1, f.pe_o2, f.box_level, f.inv_half, G, Mb, Lc); break;
+    default: throw Error(GP_ERR_INTERNAL, "no sibling-pair M2L for this n3p");
+  }
+  GP_LAUNCH_CHECK();
+}
+
 template <int MT, int RTM>
 void launch_m2l_f64(const FmmDev<double> &f, int n3, int nr, int ldm, const double *M, double *Lc,
                     int xrun, hipStream_t s) {
@@ -1748,10 +1893,16 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * m3 * nr * 2 + f.n_targets * mbytes);
-        // work-group order: XCD runs of 16 targets, the hardware order, runs of 64
-        tuned_launch("m2l_bf16_pipe", f.n_targets, MT * 100 + RT, ldm, 3, s, [&](int v) {
-          const int xr = v == 0 ? 16 : v == 1 ? 0 : 64;
-          GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, fm, m3, nr, ldm, mbo, Lo, xr, s);
+        // sibling pairs or one target a work-group; XCD runs of 16 or 64, or
+        // the hardware order
+        const bool pairs = f.pair_tgt && m3p <= 128;
+        tuned_launch("m2l_bf16_pipe", f.n_targets, MT * 100 + RT, ldm, pairs ? 5 : 3, s, [&](int v) {
+          if (pairs && v >= 3) {
+            launch_m2l_pairs(fm, m3, nr, ldm, mbo, Lo, v == 3 ? 16 : 0, s);
+          } else {
+            const int xr = v == 0 ? 16 : v == 1 ? 0 : 64;
+            GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, fm, m3, nr, ldm, mbo, Lo, xr, s);
+          }
         });
       } else {
         throw Error(GP_ERR_INTERNAL, "BF16 M2L runs on the FP32 operator only");

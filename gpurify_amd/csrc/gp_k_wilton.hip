@@ -719,7 +719,11 @@ void launch_sensitivity(const SensArgs &a, hipStream_t s) {
     GP_LAUNCH_CHECK();
     return;
   }
-  ProfScope ps("sens_wilton_dual", s, 0.0, 0.0);
+  // algorithmic flops: 2 nm N dual entries (row and column of each moved
+  // panel) at the 2x2 Gauss rule's ~60 real flops, times (1 + 2 nd) for the
+  // dual arithmetic; the closed-form near entries cost more, so this is a
+  // lower bound
+  ProfScope ps("sens_wilton_dual", s, 2.0 * a.nm * (double)a.N * 60.0 * (1 + 2 * a.nd), 0.0);
   switch (a.nd) {
     case 1: launch_sensitivity_nd<1>(a, s); break;
     case 2: launch_sensitivity_nd<2>(a, s); break;

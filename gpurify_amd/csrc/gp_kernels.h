@@ -139,6 +139,12 @@ struct FmmDev {
   // tile_eoff[T + 1]): the union of their M2L offsets ascending (ent_oidx),
   // each with the source box of every target (ent_src, 16 per entry, -1 for
   // none).  Null when not built.
+  // Multi-RHS sibling pairs (BF16 M2L): pair g = targets pair_tgt[2g], [2g+1]
+  // (-1: none), entries [pair_off[g], pair_off[g + 1]) by source box with the
+  // offsets pe_o1 / pe_o2 of the source for each target (-1: not listed).
+  int n_pairs2 = 0;
+  const int *pair_tgt = nullptr, *pair_off = nullptr, *pe_src = nullptr, *pe_o1 = nullptr,
+            *pe_o2 = nullptr;
   int n_tiles = 0, tile_size = 16;  // targets per tile: 16, or 64 (n3p <= 128)
   const int *tile_tgt = nullptr, *tile_eoff = nullptr, *ent_oidx = nullptr, *ent_src = nullptr;
 };

@@ -400,6 +400,8 @@ struct FmmSystem {
   DBuf<int> po_tgt, po_src;
   int n_tiles = 0, tile_size = 16;
   DBuf<int> tile_tgt, tile_eoff, ent_oidx, ent_src;
+  int n_pairs2 = 0;
+  DBuf<int> pair_tgt, pair_off, pe_src, pe_o1, pe_o2;
   DBuf<int> tr_off, tr_leaf, tr_stride;
   DBuf<int64_t> tr_base;
   DBuf<double> dinv_area, inv_area, ytmp;
@@ -745,6 +747,57 @@ struct FmmSystem {
     });
     phase("boxes");
     pairs = (long)hil_src.size();
+    if (need32 && m2l_inner == GP_M2L_BF16 && !want_by_offset) {
+      // Sibling pairs for the multi-RHS BF16 M2L: consecutive targets with
+      // one parent, the union of their lists ordered by source box.
+      std::vector<int> gfirst;
+      for (size_t i = 0; i < htargets.size();) {
+        gfirst.push_back((int)i);
+        i += (i + 1 < htargets.size() && hparent[htargets[i + 1]] == hparent[htargets[i]]) ? 2 : 1;
+      }
+      n_pairs2 = (int)gfirst.size();
+      gfirst.push_back((int)htargets.size());
+      std::vector<std::vector<std::array<int, 3>>> ents(n_pairs2);
+      parallel_for(n_pairs2, [&](int gi) {
+        auto &e = ents[gi];
+        for (int i = gfirst[gi]; i < gfirst[gi + 1]; ++i) {
+          const int b = htargets[i], side = i - gfirst[gi];
+          for (int q = hil_off[b]; q < hil_off[b + 1]; ++q)
+            e.push_back({hil_src[q], side ? -1 : hil_oidx[q], side ? hil_oidx[q] : -1});
+        }
+        std::sort(e.begin(), e.end());
+        size_t w = 0;
+        for (size_t r = 0; r < e.size(); ++r) {
+          if (w > 0 && e[w - 1][0] == e[r][0]) {
+            if (e[r][1] >= 0) e[w - 1][1] = e[r][1];
+            if (e[r][2] >= 0) e[w - 1][2] = e[r][2];
+          } else {
+            e[w++] = e[r];
+          }
+        }
+        e.resize(w);
+      });
+      std::vector<int> hpt((size_t)2 * n_pairs2, -1), hpo(n_pairs2 + 1, 0);
+      for (int gi = 0; gi < n_pairs2; ++gi) {
+        for (int i = gfirst[gi]; i < gfirst[gi + 1]; ++i) hpt[2 * gi + (i - gfirst[gi])] = htargets[i];
+        hpo[gi + 1] = hpo[gi] + (int)ents[gi].size();
+      }
+      std::vector<int> hs(hpo[n_pairs2]), h1(hpo[n_pairs2]), h2(hpo[n_pairs2]);
+      parallel_for(n_pairs2, [&](int gi) {
+        int q = hpo[gi];
+        for (const auto &e : ents[gi]) {
+          hs[q] = e[0];
+          h1[q] = e[1];
+          h2[q] = e[2];
+          ++q;
+        }
+      });
+      pair_tgt = to_device(hpt, s);
+      pair_off = to_device(hpo, s);
+      pe_src = to_device(hs, s);
+      pe_o1 = to_device(h1, s);
+      pe_o2 = to_device(h2, s);
+    }
     if (want_by_offset && n3p > 64) {
       // FP64 single-RHS M2L beyond the 64-target tiles: the pairs grouped by
       // offset (counting sort; targets stay ascending)
@@ -907,6 +960,12 @@ struct FmmSystem {
       f.po_src = po_src.p;
       f.n_tiles = n_tiles;
       f.tile_size = tile_size;
+      f.n_pairs2 = n_pairs2;
+      f.pair_tgt = n_pairs2 ? pair_tgt.p : nullptr;
+      f.pair_off = pair_off.p;
+      f.pe_src = pe_src.p;
+      f.pe_o1 = pe_o1.p;
+      f.pe_o2 = pe_o2.p;
       f.tile_tgt = want_by_offset ? tile_tgt.p : nullptr;
       f.tile_eoff = tile_eoff.p;
       f.ent_oidx = ent_oidx.p;
@@ -1118,6 +1177,56 @@ struct SegOps {
   }
 };
 
+// GMRES workspace, kept across the solves of one system (the refinement
+// steps of the mixed solver), with the captured Arnoldi steps that replay on
+// it: graph k holds step k of a cycle (operator apply, Gram-Schmidt passes,
+// norm, Givens update, the scaled new basis vector and the copy of the
+// active flags to the host), valid for the current basis allocation and the
+// scalars it was captured with.
+template <typename TV>
+struct GmresWork {
+  int m = 0, S = 0, vcap = 0;
+  size_t vec = 0;
+  DBuf<TV> V;
+  DBuf<double> w, rv, ax, partial, h1, h2, hn, H, cs, sn, g, Y, bnorm, beta, inv_beta, est, inv_h;
+  DBuf<int> live, active, kused;
+  std::unique_ptr<Pinned> pin;
+  std::vector<hipGraphExec_t> graphs;
+  double g_tol = -1.0;
+  int g_passes = 0, g_prec = -1, g_nc = 0, g_nr = 0;
+  void clear_graphs() {
+    for (auto e : graphs)
+      if (e) (void)hipGraphExecDestroy(e);
+    graphs.clear();
+  }
+  ~GmresWork() { clear_graphs(); }
+  template <typename Ops>
+  void prepare(int m_, int S_, size_t vec_, const Ops &ops) {
+    if (m_ == m && S_ == S && vec_ == vec) return;
+    clear_graphs();
+    m = m_;
+    S = S_;
+    vec = vec_;
+    vcap = std::min(m, 32) + 1;
+    V.alloc(vcap * vec);
+    w.alloc(vec);
+    rv.alloc(vec);
+    ax.alloc(vec);
+    partial.alloc(ops.partial_size(vcap));
+    h1.alloc((size_t)(m + 1) * S);
+    h2.alloc((size_t)(m + 1) * S);
+    hn.alloc(S);
+    H.alloc((size_t)(m + 1) * m * S);
+    cs.alloc((size_t)m * S);
+    sn.alloc((size_t)m * S);
+    g.alloc((size_t)(m + 1) * S);
+    Y.alloc((size_t)m * S);
+    for (auto *b : {&bnorm, &beta, &inv_beta, &est, &inv_h}) b->alloc(S);
+    for (auto *b : {&live, &active, &kused}) b->alloc(S);
+    pin = std::make_unique<Pinned>((size_t)S * (2 * sizeof(int) + 2 * sizeof(double)));
+  }
+};
+
 // Restarted GMRES(m) with right Jacobi preconditioning for the systems of
 // ops (right-hand sides b, device) starting from x = 0 (x must be zero on
 // entry), per system exactly the control flow of krylov.rs:50-191: stop a
@@ -1126,28 +1235,42 @@ struct SegOps {
 // twice (CGS2, passes = 2), the block-friendly equivalent of the reference's
 // modified Gram-Schmidt; the inner solves of the mixed solver (tolerance
 // ~1e-2, corrected by the FP64 refinement) take one pass and keep the basis V
-// in FP32 (TV = float).
+// in FP32 (TV = float).  work (optional) keeps the workspace across calls;
+// with capture set, each Arnoldi step k runs as a HIP graph captured on its
+// first use and replayed afterwards (never while the kernel profiler runs).
 template <typename TV, typename Ops>
 void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double *x, double tol,
                int m, int max_restarts, bool true_check, int passes,
-               const std::vector<int> &live_in, GmresStats &st) {
+               const std::vector<int> &live_in, GmresStats &st, GmresWork<TV> *work = nullptr,
+               bool capture = false) {
   hipStream_t s = sys.s;
   const int nr = ops.nsys(), S = ops.S(), nc = ops.cols();
   const size_t vec = ops.vec();
-  int vcap = std::min(m, 32) + 1;
-  DBuf<TV> V(vcap * vec);
-  DBuf<double> w(vec), rv(vec), ax(vec);
-  DBuf<double> partial(ops.partial_size(vcap));
-  DBuf<double> h1((size_t)(m + 1) * S), h2((size_t)(m + 1) * S), hn(S);
-  DBuf<double> H((size_t)(m + 1) * m * S), cs((size_t)m * S), sn((size_t)m * S),
-      g((size_t)(m + 1) * S), Y((size_t)m * S);
-  DBuf<double> bnorm(S), beta(S), inv_beta(S), est(S), inv_h(S);
-  DBuf<int> live(S), active(S), kused(S);
-  Pinned pin((size_t)S * (2 * sizeof(int) + 2 * sizeof(double)));
-  int *h_active = pin.as<int>();
+  GmresWork<TV> local;
+  GmresWork<TV> &W = work ? *work : local;
+  W.prepare(m, S, vec, ops);
+  int &vcap = W.vcap;
+  DBuf<TV> &V = W.V;
+  DBuf<double> &w = W.w, &rv = W.rv, &ax = W.ax, &partial = W.partial, &h1 = W.h1, &h2 = W.h2,
+               &hn = W.hn, &H = W.H, &cs = W.cs, &sn = W.sn, &g = W.g, &Y = W.Y,
+               &bnorm = W.bnorm, &beta = W.beta, &inv_beta = W.inv_beta, &est = W.est,
+               &inv_h = W.inv_h;
+  DBuf<int> &live = W.live, &active = W.active, &kused = W.kused;
+  int *h_active = W.pin->template as<int>();
   int *h_kused = h_active + S;
   double *h_est = reinterpret_cast<double *>(h_kused + S);
   double *h_aux = h_est + S;
+  const Prof *prof = active_prof();
+  const bool graphs = capture && !(prof && prof->on);
+  if (graphs && (W.g_tol != tol || W.g_passes != passes || W.g_prec != prec || W.g_nc != nc ||
+                 W.g_nr != nr)) {
+    W.clear_graphs();
+    W.g_tol = tol;
+    W.g_passes = passes;
+    W.g_prec = prec;
+    W.g_nc = nc;
+    W.g_nr = nr;
+  }
 
   auto ensure_V = [&](int nv) {
     if (nv <= vcap) return;
@@ -1158,6 +1281,7 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
     V = std::move(V2);
     partial.alloc(ops.partial_size(ncap));
     vcap = ncap;
+    W.clear_graphs();  // captured with the old basis
   };
 
   st.iters.assign(nr, 0);
@@ -1207,19 +1331,44 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
     int ksteps = 0;
     for (int k = 0; k < m; ++k) {
       ensure_V(k + 2);
-      sys.apply(V.p + (size_t)k * sv, true, nc, w.p, prec);
-      ops.dots(V.p, sv, k + 1, w.p, partial.p, h1.p, false);
-      ops.axpy_multi(w.p, V.p, sv, h1.p, k + 1, -1.0);
-      if (passes > 1) {
-        ops.dots(V.p, sv, k + 1, w.p, partial.p, h2.p, false);
-        ops.axpy_multi(w.p, V.p, sv, h2.p, k + 1, -1.0);
+      auto step = [&] {
+        sys.apply(V.p + (size_t)k * sv, true, nc, w.p, prec);
+        ops.dots(V.p, sv, k + 1, w.p, partial.p, h1.p, false);
+        ops.axpy_multi(w.p, V.p, sv, h1.p, k + 1, -1.0);
+        if (passes > 1) {
+          ops.dots(V.p, sv, k + 1, w.p, partial.p, h2.p, false);
+          ops.axpy_multi(w.p, V.p, sv, h2.p, k + 1, -1.0);
+        }
+        ops.dots(w.p, 0, 1, w.p, partial.p, hn.p, true);
+        kr_gmres_givens(k, m, nr, S, h1.p, passes > 1 ? h2.p : nullptr, hn.p, H.p, cs.p, sn.p,
+                        g.p, bnorm.p, tol, active.p, kused.p, est.p, inv_h.p, s);
+        ops.scale(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, nullptr);
+        GP_HIP(hipMemcpyAsync(h_active, active.p, nr * sizeof(int), hipMemcpyDeviceToHost, s));
+      };
+      if (graphs) {
+        if ((int)W.graphs.size() <= k) W.graphs.resize(k + 1, nullptr);
+        if (!W.graphs[k]) {
+          hipGraph_t gr = nullptr;
+          GP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          try {
+            step();
+          } catch (...) {
+            (void)hipStreamEndCapture(s, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+          }
+          GP_HIP(hipStreamEndCapture(s, &gr));
+          hipGraphExec_t ex = nullptr;
+          const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+          GP_HIP(e);
+          W.graphs[k] = ex;
+        }
+        GP_HIP(hipGraphLaunch(W.graphs[k], s));
+      } else {
+        step();
       }
-      ops.dots(w.p, 0, 1, w.p, partial.p, hn.p, true);
-      kr_gmres_givens(k, m, nr, S, h1.p, passes > 1 ? h2.p : nullptr, hn.p, H.p, cs.p, sn.p, g.p,
-                      bnorm.p, tol, active.p, kused.p, est.p, inv_h.p, s);
-      ops.scale(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, nullptr);
       ksteps = k + 1;
-      GP_HIP(hipMemcpyAsync(h_active, active.p, nr * sizeof(int), hipMemcpyDeviceToHost, s));
       GP_HIP(hipStreamSynchronize(s));
       int na = 0;
       for (int r = 0; r < nr; ++r) na += h_active[r];
@@ -1324,6 +1473,11 @@ void solve_systems(FmmSystem &sys, const Ops &ops, const gp_cap_opts &o, bool mi
   for (int q = 0; q < nr; ++q) done[q] = !live[q];
   const int max_ref = std::max(1, (int)o.max_refine);
   const int passes = o.inner_passes >= 2 ? 2 : 1;
+  // inner workspace and its captured Arnoldi steps, shared by the refinement
+  // steps; capture starts with the second step (the first one also settles
+  // the autotuned kernel variants)
+  GmresWork<float> work32;
+  GmresWork<double> work64;
   for (;;) {
     ops.dots(r.p, 0, 1, r.p, part.p, rn.p, true);
     GP_HIP(hipMemcpyAsync(hrn.data(), rn.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1338,10 +1492,13 @@ void solve_systems(FmmSystem &sys, const Ops &ops, const gp_cap_opts &o, bool mi
     for (int q = 0; q < nr; ++q) lv[q] = !done[q];
     dx.zero(s);
     GmresStats st;
+    const bool capture = o.step_graphs != 0 && out.refine_steps >= 1;
     if (o.inner_basis == GP_BASIS_FP64)
-      gmres_run<double>(sys, ops, 1, r.p, dx.p, o.inner_tol, m, mr, false, passes, lv, st);
+      gmres_run<double>(sys, ops, 1, r.p, dx.p, o.inner_tol, m, mr, false, passes, lv, st,
+                        &work64, capture);
     else
-      gmres_run<float>(sys, ops, 1, r.p, dx.p, o.inner_tol, m, mr, false, passes, lv, st);
+      gmres_run<float>(sys, ops, 1, r.p, dx.p, o.inner_tol, m, mr, false, passes, lv, st, &work32,
+                       capture);
     for (int q = 0; q < nr; ++q) out.iters[q] += st.iters[q];
     ops.axpy(x, dx.p, 1.0);
     sys.apply(x, false, nc, ax.p, 0);

@@ -113,6 +113,10 @@ typedef struct {
                             vectors (r rounded up to 32, 64 or 128; no projection when that
                             reaches n^3 rounded up to 32).  Fewer M2L flops, more refinement
                             steps.  The FP64 operator is never projected. */
+  uint32_t step_graphs;  /* GP_PREC_MIXED: 1 (default) = from the second refinement step on,
+                            each inner GMRES step (operator apply, orthogonalisation, Givens
+                            update) runs as a HIP graph captured on first use and replayed;
+                            0 = kernels launched one by one.  Off while profiling. */
 } gp_cap_opts;
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 

@@ -52,6 +52,7 @@ def test_opts_struct_size_matches_header():
     G._lib.load().gp_cap_opts_default(ctypes.cast(ctypes.byref(p), ctypes.POINTER(G._lib.GpCapOpts)))
     assert p.canary == 0x5A5A5A5A5A5A5A5A
     assert p.o.inner_basis == G.GP_BASIS_FP32 and p.o.inner_passes == 1 and p.o.m2l_rank == 0
+    assert p.o.step_graphs == 1
     # the last field is written, so the library's struct is not shorter
     text = open(os.path.join(ROOT, "include", "gpurify.h")).read()
     body = re.search(r"typedef struct \{([^{}]*)\} gp_cap_opts;", text).group(1)

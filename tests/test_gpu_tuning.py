@@ -24,7 +24,7 @@ def test_every_variant_applies_the_same_operator(ctx, order):
     o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, m2l_inner=G.GP_M2L_BF16)
     ref = {}
     try:
-        for var in (0, 1, 2):
+        for var in (0, 1, 2, 3, 4):  # 3, 4: sibling-pair M2L (others clamp)
             ctx.tune_force(var)
             out = {
                 "fp64_1": ctx.fmm_apply(v, x1, o),
@@ -38,7 +38,8 @@ def test_every_variant_applies_the_same_operator(ctx, order):
                 continue
             for k, y in out.items():
                 # near-field variants differ in summation order only; the M2L
-                # variants only in which work-group takes which target
+                # variants in which work-group takes which target, and the
+                # sibling pairs in the order of the sources
                 bound = 1e-13 if k.startswith("fp64") else 1e-6
                 assert rel(y, ref[k]) < bound, (var, k)
     finally:
