@@ -1722,6 +1722,53 @@ __global__ __launch_bounds__(256) void k_m2l_expand(int n3, int kr, int ncol, in
   }
 }
 
+// One thread per (panel, axis): x = the centroid's coordinate in the leaf
+// box's [-1, 1] frame, w[m] = (2/n) (1/2 + sum_{k=1}^{n-1} T_k(x_m) T_k(x))
+// (chebyshev.rs:49-84), the Chebyshev polynomials by their recurrence.
+__global__ __launch_bounds__(256) void k_interp_w1d(int N, int n,
+                                                    const int *__restrict__ panel_leaf,
+                                                    const double *__restrict__ geo,
+                                                    const double *__restrict__ cen,
+                                                    const double *__restrict__ nodes,
+                                                    double *__restrict__ w64,
+                                                    float *__restrict__ w32) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * 3) return;
+  const int i = e / 3, d = e % 3;
+  const double *g = geo + (size_t)panel_leaf[i] * 4;
+  const double x = (cen[(size_t)i * 3 + d] - g[d]) / g[3];
+  double tx[8];
+  tx[0] = 1.0;
+  tx[1] = x;
+  for (int k = 2; k < n; ++k) tx[k] = 2.0 * x * tx[k - 1] - tx[k - 2];
+  const double inv_n = 1.0 / n;
+  for (int m = 0; m < n; ++m) {
+    const double xm = nodes[m];
+    double sum = 0.5, tkm1 = 1.0, tk = xm;
+    for (int k = 1; k < n; ++k) {
+      if (k > 1) {
+        const double t = 2.0 * xm * tk - tkm1;
+        tkm1 = tk;
+        tk = t;
+      }
+      sum += tk * tx[k];
+    }
+    const double w = 2.0 * inv_n * sum;
+    const size_t o = (size_t)e * n + m;
+    w64[o] = w;
+    if (w32) w32[o] = (float)w;
+  }
+}
+
+void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, const double *cen,
+                       const double *nodes, double *w64, float *w32, hipStream_t s) {
+  if (N == 0) return;
+  if (n < 2 || n > 8) throw Error(GP_ERR_INTERNAL, "interpolation order out of range");
+  hipLaunchKernelGGL(k_interp_w1d, dim3((N * 3 + 255) / 256), dim3(256), 0, s, N, n, panel_leaf,
+                     geo, cen, nodes, w64, w32);
+  GP_LAUNCH_CHECK();
+}
+
 template <typename T>
 void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf16 *Mb, T *Lc,
                      int ldm, int m2l, double *y, int ldy, hipStream_t s, T *Mt, __bf16 *Mtb,

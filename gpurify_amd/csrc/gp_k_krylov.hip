@@ -55,15 +55,19 @@ __global__ __launch_bounds__(256) void k_dot_partial(const TX *__restrict__ X,
     }
 }
 
-// out[v * ld + r] = sum_b partial[(v * nblk + b) * ld + r]  (optionally sqrt)
-__global__ void k_reduce_partial(const double *__restrict__ partial, int nblk, int nvec, int ld,
-                                 int nr, double *__restrict__ out, int take_sqrt) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// out[v * ld + r] = sum_b partial[(v * nblk + b) * ld + r]  (optionally sqrt):
+// one wavefront per output, lane l summing blocks l, l + 64, ... in order,
+// then a fixed shuffle tree, so every run gives the same bits.
+__global__ __launch_bounds__(256) void k_reduce_partial(const double *__restrict__ partial,
+                                                        int nblk, int nvec, int ld, int nr,
+                                                        double *__restrict__ out, int take_sqrt) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (e >= nvec * nr) return;
-  int v = e / nr, r = e % nr;
+  const int v = e / nr, r = e % nr;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[((size_t)v * nblk + b) * ld + r];
-  out[(size_t)v * ld + r] = take_sqrt ? sqrt(s) : s;
+  for (int b = lane; b < nblk; b += 64) s += partial[((size_t)v * nblk + b) * ld + r];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if (lane == 0) out[(size_t)v * ld + r] = take_sqrt ? sqrt(s) : s;
 }
 
 // w[row][r] += alpha * sum_{v < nvec} coef[v][r] * V_v[row][r]
@@ -478,8 +482,8 @@ void kr_dots(const TX *X, size_t strideX, int nvec, const double *Y, int N, int 
                      strideX, nvec, Y, N, ld, nr, partial);
   GP_LAUNCH_CHECK();
   int tot = nvec * nr;
-  hipLaunchKernelGGL(k_reduce_partial, dim3((tot + 255) / 256), dim3(256), 0, s, partial, nblk,
-                     nvec, ld, nr, out, take_sqrt ? 1 : 0);
+  hipLaunchKernelGGL(k_reduce_partial, dim3((tot + 3) / 4), dim3(256), 0, s, partial, nblk, nvec,
+                     ld, nr, out, take_sqrt ? 1 : 0);
   GP_LAUNCH_CHECK();
 }
 

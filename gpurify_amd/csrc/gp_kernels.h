@@ -148,6 +148,12 @@ struct FmmDev {
   int n_tiles = 0, tile_size = 16;  // targets per tile: 16, or 64 (n3p <= 128)
   const int *tile_tgt = nullptr, *tile_eoff = nullptr, *ent_oidx = nullptr, *ent_src = nullptr;
 };
+// Chebyshev interpolation weights of every sorted panel's centroid in its
+// leaf box: w1d[i][d][m] for axis d and node m (3 n per panel), FP64 and, when
+// w32 is set, FP32.  geo: per leaf the box centre (3) and half-width in the
+// problem's unit frame; cen: scaled centroids; nodes: the n Chebyshev nodes.
+void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, const double *cen,
+                       const double *nodes, double *w64, float *w32, hipStream_t s);
 // Far field y += P_far z.  Multipoles M: n_boxes x ldm x n3p; locals Lc:
 // n_boxes x n3 x ldm.  m2l selects the M2L kernel (GP_M2L_*): BF16 needs Mb
 // (BF16 multipoles, same layout), FP64 uses matrix cores when ldm allows,

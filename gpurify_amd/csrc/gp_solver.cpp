@@ -663,24 +663,17 @@ struct FmmSystem {
       if (kr >= n3p) kr = 0;
     }
     auto nodes = cheb_nodes(n);
-    std::vector<double> hw1d((size_t)N * 3 * n);
-    parallel_for(P, [&](int p) {
+    // Leaf boxes in each problem's unit frame (centre and half-width, exact
+    // powers of two), for the interpolation weights computed on the device.
+    std::vector<double> hgeo((size_t)n_leaves * 4);
+    parallel_for(n_leaves, [&](int T) {
+      int p, lt;
+      leaf_of(T, p, lt);
       const Tree &t = trees[p];
-      const PanelGeom &g = *probs[p];
-      const double half_L = t.level_half(t.L);
-      const double origin[3] = {t.center[0] - t.half, t.center[1] - t.half, t.center[2] - t.half};
-      for (int T = 0; T < t.n_leaves(); ++T) {
-        int b = t.leaf_box(T);
-        double bc[3];
-        for (int d = 0; d < 3; ++d) bc[d] = origin[d] + (t.cell[3 * b + d] + 0.5) * 2.0 * half_L;
-        for (int k = t.leaf_lo[T]; k < t.leaf_lo[T + 1]; ++k) {
-          int i = (int)t.perm[k];
-          double a[3] = {g.cx[i], g.cy[i], g.cz[i]};
-          const size_t row = (size_t)(panel_off[p] + k) * 3 * n;
-          for (int d = 0; d < 3; ++d)
-            interp_weights((a[d] - bc[d]) / half_L, n, nodes.data(), &hw1d[row + d * n]);
-        }
-      }
+      const int b = t.leaf_box(lt);
+      const double hs = t.level_half(t.L) / t.half;
+      for (int d = 0; d < 3; ++d) hgeo[(size_t)T * 4 + d] = -1.0 + (t.cell[3 * b + d] + 0.5) * 2.0 * hs;
+      hgeo[(size_t)T * 4 + 3] = hs;
     });
     // M2M 1-D matrices, bbfmm.rs:89-112: mat[m*n + k] = S_n(cc + x_k / 2, x_m).
     std::vector<double> hm2m((size_t)24 * n * n);
@@ -893,11 +886,17 @@ struct FmmSystem {
     il_oidx = to_device(hil_oidx, s);
     box_level = to_device(hlevel, s);
     inv_half = to_device(hih, s);
-    w1d64 = to_device(hw1d, s);
+    {
+      DBuf<double> dgeo = to_device(hgeo, s), dnodes = to_device(nodes, s);
+      w1d64.alloc((size_t)N * 3 * n);
+      if (need32) w1d32.alloc((size_t)N * 3 * n);
+      launch_interp_w1d(N, n, panel_leaf.p, dgeo.p, cen.p, dnodes.p, w1d64.p,
+                        need32 ? w1d32.p : nullptr, s);
+      GP_HIP(hipStreamSynchronize(s));  // dgeo and dnodes are freed below
+    }
     m2m64 = to_device(hm2m, s);
     G64 = to_device(hG, s);
     if (need32) {
-      w1d32 = to_device(cast_vec<float>(hw1d), s);
       m2m32 = to_device(cast_vec<float>(hm2m), s);
       G32 = to_device(cast_vec<float>(hG), s);
       std::vector<uint16_t> hb(hG.size());
