@@ -429,6 +429,42 @@ __global__ __launch_bounds__(256) void k_m2l_f64_pipe(int n3, int nr, int ldm, i
       }
 }
 
+// Single right-hand side L2P for n^3 <= 64: one wavefront per leaf, lane m
+// holding local coefficient m (one coalesced load); per panel of the leaf the
+// lanes form w_x[a] w_y[b] w_z[c] L[m] and a shuffle tree sums them (the
+// panel rows of the leaf are read as broadcasts, four panels in flight).
+template <typename T>
+__global__ __launch_bounds__(256) void k_l2p1w(int n, int nleaf, const int *__restrict__ leaf_lo,
+                                               const int *__restrict__ leaf_box, int leaf_box0,
+                                               const T *__restrict__ w1d,
+                                               const T *__restrict__ Lc, double *__restrict__ y) {
+  const int li = blockIdx.x * 4 + (threadIdx.x >> 6), m = threadIdx.x & 63;
+  if (li >= nleaf) return;
+  const int n3 = n * n * n;
+  const int mm = m < n3 ? m : 0;  // idle lanes read node 0
+  const int a = mm / (n * n), b = (mm / n) % n, c = mm % n;
+  const int box = leaf_box ? leaf_box[li] : leaf_box0 + li;
+  const T l = m < n3 ? Lc[(size_t)box * n3 + m] : (T)0;
+  const int p0 = leaf_lo[li], p1 = leaf_lo[li + 1];
+  for (int p = p0; p < p1; p += 4) {
+    T t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T *w = w1d + (size_t)min(p + u, p1 - 1) * 3 * n;  // in range even past the leaf
+      const T v = w[a] * w[n + b] * w[2 * n + c] * l;
+      t[u] = (p + u < p1 && m < n3) ? v : (T)0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] += __shfl_xor(t[u], off);
+    if (m == 0)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (p + u < p1) y[p + u] += (double)t[u];
+  }
+}
+
 // Single right-hand side (ldm = 1) P2M: several leaves per work-group, one
 // thread per (leaf, node), each summing its leaf's panels in order (the
 // same expression and order as k_p2m).
@@ -445,6 +481,7 @@ __global__ __launch_bounds__(256) void k_p2m1(int n, int n3p, int tpl, int nleaf
   if (li >= nleaf || m >= n3) return;
   const int a = m / (n * n), b = (m / n) % n, c = m % n;
   T acc = 0;
+#pragma unroll 4
   for (int p = leaf_lo[li]; p < leaf_lo[li + 1]; ++p) {
     const T *w = w1d + (size_t)p * 3 * n;
     const T q = (T)area[p] * z[p];
@@ -480,9 +517,10 @@ __global__ __launch_bounds__(256) void k_l2p1(int n, int N, const int *__restric
 }
 
 // Single right-hand side M2M: 256 / tpb parent boxes per work-group, one
-// thread per (parent, node); the three 1-D contractions of each child (z,
-// then y, then x, as k_m2m) pass through a per-parent LDS slot.  Every
-// parent loops over 8 child slots so the barriers stay uniform.
+// thread per (parent, node).  All children of a parent are loaded into its
+// LDS slot at once (8 loads in flight), then the three 1-D contractions (z,
+// then y, then x, as k_m2m) run for every child between three barriers, the
+// x pass summing over the children in order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_m2m1(int n, int n3p, int tpb, int box0, int nb,
                                               const int *__restrict__ child_off,
@@ -493,40 +531,54 @@ __global__ __launch_bounds__(256) void k_m2m1(int n, int n3p, int tpb, int box0,
   extern __shared__ unsigned char smem_raw[];
   const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
   const int n3 = n * n * n, nn = n * n;
-  T *s0 = reinterpret_cast<T *>(smem_raw) + slot * 2 * tpb, *s1 = s0 + tpb;
+  T *s0 = reinterpret_cast<T *>(smem_raw) + (size_t)slot * 16 * tpb, *s1 = s0 + 8 * tpb;
   const int bi = blockIdx.x * (256 / tpb) + slot;
   const bool live = bi < nb && m < n3;
   const int b = box0 + min(bi, nb - 1);
   const int c0 = child_off[b], nch = bi < nb ? child_off[b + 1] - c0 : 0;
   const int a = m / nn, bq = (m / n) % n, cq = m % n;
-  T acc = 0;
+  int oc[8];
+#pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const bool valid = live && q < nch;
+    const bool valid = q < nch;
     const int ch = valid ? child_idx[c0 + q] : 0;
-    const int o = valid ? octant[ch] : 0;
-    const T *mx = m2m1d + (o * 3 + 0) * nn, *my = m2m1d + (o * 3 + 1) * nn,
-            *mz = m2m1d + (o * 3 + 2) * nn;
-    if (m < n3) s0[m] = valid ? M[(size_t)ch * n3p + m] : (T)0;
-    __syncthreads();
-    if (m < n3) {  // z: s1[a][b][cp] = sum_c mz[cp][c] s0[a][b][c]
-      T t = 0;
-      for (int c = 0; c < n; ++c) t += mz[cq * n + c] * s0[(a * n + bq) * n + c];
-      s1[m] = t;
-    }
-    __syncthreads();
-    if (m < n3) {  // y: s0[a][bp][cp] = sum_b my[bp][b] s1[a][b][cp]
-      T t = 0;
-      for (int bb = 0; bb < n; ++bb) t += my[bq * n + bb] * s1[(a * n + bb) * n + cq];
-      s0[m] = t;
-    }
-    __syncthreads();
-    if (valid) {  // x: acc[ap][bp][cp] += sum_a mx[ap][a] s0[a][bp][cp]
-      T t = 0;
-      for (int aa = 0; aa < n; ++aa) t += mx[a * n + aa] * s0[aa * nn + bq * n + cq];
-      acc += t;
-    }
-    __syncthreads();
+    oc[q] = valid ? octant[ch] : 0;
+    if (m < n3) s0[q * tpb + m] = valid ? M[(size_t)ch * n3p + m] : (T)0;
   }
+  __syncthreads();
+  if (m < n3)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < nch) {  // z: s1[a][b][cp] = sum_c mz[cp][c] s0[a][b][c]
+        const T *mz = m2m1d + (oc[q] * 3 + 2) * nn;
+        const T *x0 = s0 + q * tpb + (a * n + bq) * n;
+        T t = 0;
+        for (int c = 0; c < n; ++c) t += mz[cq * n + c] * x0[c];
+        s1[q * tpb + m] = t;
+      }
+  __syncthreads();
+  if (m < n3)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < nch) {  // y: s0[a][bp][cp] = sum_b my[bp][b] s1[a][b][cp]
+        const T *my = m2m1d + (oc[q] * 3 + 1) * nn;
+        const T *x1 = s1 + q * tpb + a * nn + cq;
+        T t = 0;
+        for (int bb = 0; bb < n; ++bb) t += my[bq * n + bb] * x1[bb * n];
+        s0[q * tpb + m] = t;
+      }
+  __syncthreads();
+  T acc = 0;
+  if (live)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < nch) {  // x: acc[ap][bp][cp] += sum_a mx[ap][a] s0[a][bp][cp]
+        const T *mx = m2m1d + (oc[q] * 3 + 0) * nn;
+        const T *x2 = s0 + q * tpb + bq * n + cq;
+        T t = 0;
+        for (int aa = 0; aa < n; ++aa) t += mx[a * n + aa] * x2[aa * nn];
+        acc += t;
+      }
   if (live) store_mult(M, Mb, (size_t)b * n3p + m, acc);
 }
 
@@ -548,6 +600,7 @@ __global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
   const T *mx = m2m1d + (o * 3 + 0) * nn, *my = m2m1d + (o * 3 + 1) * nn,
           *mz = m2m1d + (o * 3 + 2) * nn;
   const int a = m / nn, bq = (m / n) % n, cq = m % n;
+  const T own = live ? Lc[(size_t)ch * n3 + m] : (T)0;  // the child's M2L part, loaded early
   if (m < n3) s0[m] = Lc[(size_t)p * n3 + m];
   __syncthreads();
   if (m < n3) {  // x^T: s1[ap][b][c] = sum_a mx[a][ap] s0[a][b][c]
@@ -565,7 +618,7 @@ __global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
   if (live) {  // z^T: Lc[child][a][b][cp] += sum_c mz[c][cp] s0[a][b][c]
     T t = 0;
     for (int c = 0; c < n; ++c) t += mz[c * n + cq] * s0[(a * n + bq) * n + c];
-    Lc[(size_t)ch * n3 + m] += t;
+    Lc[(size_t)ch * n3 + m] = own + t;
   }
 }
 
@@ -1833,7 +1886,7 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch, (nb + nch) * exp_bytes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
       const int tpb = round_up(n3, 64), bpb = 256 / tpb;
-      hipLaunchKernelGGL((k_m2m1<T>), dim3((nb + bpb - 1) / bpb), dim3(256), 512 * sizeof(T), s, n,
+      hipLaunchKernelGGL((k_m2m1<T>), dim3((nb + bpb - 1) / bpb), dim3(256), 4096 * sizeof(T), s, n,
                          n3p, tpb, f.lvl_off[l], nb, f.child_off, f.child_idx, f.octant, f.m2m1d,
                          M, mb);
     } else if (tiled) {
@@ -2007,7 +2060,12 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   }
   ProfScope pp(f32 ? "l2p_f32" : "l2p_f64", s, 2.0 * f.n_panels * n3 * nr,
                (double)f.n_panels * nr * 16.0 + nleaf * exp_bytes);
-  if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf)
+  // (L2P accumulates into y, so it is not a tuner candidate: the tuner
+  // launches every variant on the same outputs)
+  if (ldm == 1 && nr == 1 && ldy == 1 && n3 <= 64)
+    hipLaunchKernelGGL((k_l2p1w<T>), dim3((nleaf + 3) / 4), dim3(256), 0, s, n, nleaf, f.leaf_lo,
+                       f.leaf_box, leaf0, f.w1d, Lc, y);
+  else if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf)
     hipLaunchKernelGGL((k_l2p1<T>), dim3((f.n_panels + 255) / 256), dim3(256), 0, s, n, f.n_panels,
                        f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y);
   else if (tiled)
