@@ -1418,7 +1418,9 @@ __global__ __launch_bounds__(256) void k_p2m_tiled(int n, int n3p, const int *__
 }
 
 // L2P as a per-leaf product y[p][r] += sum_m S(p, m) Lc[m][r]: node chunks of
-// Lc and S staged in LDS; each thread owns 2 points x RB columns.
+// Lc and S staged in LDS; thread (pg, rc) owns points pg, pg + 16 and the RB
+// columns rc + 16 j, so the 16 lanes of a point update 16 consecutive
+// columns of y.
 template <typename T>
 __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict__ leaf_lo,
                                                    const int *__restrict__ leaf_box, int leaf_box0, const T *__restrict__ w1d, int nr,
@@ -1427,7 +1429,7 @@ __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict_
   __shared__ T sL[TL_KM][128];
   __shared__ T sS[TL_PT][TL_KM + 1];
   const int li = blockIdx.x, tid = threadIdx.x;
-  const int pg = tid & 15, rg = tid >> 4;
+  const int pg = tid >> 4, rc = tid & 15;
   const int RB = ldm >> 4;
   const int n3 = n * n * n;
   const T *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * n3 * ldm;
@@ -1457,7 +1459,7 @@ __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict_
 #pragma unroll
         for (int j = 0; j < 8; ++j)
           if (j < RB) {
-            const T l = sL[mm][rg * RB + j];
+            const T l = sL[mm][rc + 16 * j];
             acc[0][j] += s0 * l;
             acc[1][j] += s1 * l;
           }
@@ -1468,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict_
     for (int q = 0; q < 2; ++q)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int pp = pg + 16 * q, r = rg * RB + j;
+        const int pp = pg + 16 * q, r = rc + 16 * j;
         if (j < RB && pp < np && r < nr) y[(size_t)(p0 + pp) * ldy + r] += (double)acc[q][j];
       }
   }
