@@ -222,8 +222,9 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // keeps its G_off fragments (B operand) in registers.  The next pair's
 // multipole rows are prefetched into registers while the current pair runs
 // on the matrix cores, and so are its G_off fragments when they fit
-// (MTW * n3p / 32 <= 8 fragments, orders 2 to 5).
-template <int MT, int RTM>
+// (MTW * n3p / 32 <= 8 fragments, orders 2 to 5).  DEPTH 2 keeps the source
+// rows of the next two pairs in flight (two register buffers).
+template <int MT, int RTM, int DEPTH = 1>
 __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm, int xrun,
                                                        const int *__restrict__ targets,
                                                        const int *__restrict__ il_off,
@@ -245,7 +246,8 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm, 
   const int t = targets[xcd_block(blockIdx.x, gridDim.x, xrun)];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  const int RT = min((nr + 15) >> 4, RTM);
+  const int rt0 = blockIdx.y * RTM;  // RHS tiles [rt0, rt0 + RT) of this work-group
+  const int RT = min((nr + 15) >> 4, rt0 + RTM) - rt0;
   const int AV = RT * 16 * VPR;    // vectors of one pair's A block
   const size_t MS = (size_t)ldm * NP;
   f32x4 acc[RTM][MTW];
@@ -254,15 +256,16 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm, 
 #pragma unroll
     for (int j = 0; j < MTW; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const int q0 = il_off[t], q1 = il_off[t + 1];
-  bf16x8 apre[APT];
+  bf16x8 apre[APT], apre2[DEPTH > 1 ? APT : 1];
   bf16x8 bcur[MTW][KS];
   bf16x8 bnext[BPRE ? MTW : 1][BPRE ? KS : 1];
-  auto load_a = [&](int q) {
-    const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Mb + (size_t)il_src[q] * MS);
+  auto load_a = [&](int q, bf16x8 (&dst)[APT]) {
+    const bf16x8 *src =
+        reinterpret_cast<const bf16x8 *>(Mb + (size_t)il_src[q] * MS + (size_t)rt0 * 16 * NP);
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
       const int v = tid + 256 * i;
-      if (v < AV) apre[i] = src[v];
+      if (v < AV) dst[i] = src[v];
     }
   };
   auto load_b = [&](int q, bf16x8 (&dst)[MTW][KS]) {
@@ -277,22 +280,25 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm, 
     }
   };
   if (q0 < q1) {
-    load_a(q0);
+    load_a(q0, apre);
+    if constexpr (DEPTH > 1)
+      if (q0 + 1 < q1) load_a(q0 + 1, apre2);
     if constexpr (BPRE) load_b(q0, bcur);
   }
-  for (int q = q0; q < q1; ++q) {
+  // one pair: its source rows (ab) into LDS, the rows DEPTH pairs ahead into
+  // ab, the next G_off fragments, the matrix-core products
+  auto step = [&](int q, bf16x8 (&ab)[APT]) {
     __syncthreads();  // previous pair's LDS reads are done
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
       const int v = tid + 256 * i;
-      if (v < AV) *(bf16x8 *)(sA + (v / VPR) * LDA + (v % VPR) * 8) = apre[i];
+      if (v < AV) *(bf16x8 *)(sA + (v / VPR) * LDA + (v % VPR) * 8) = ab[i];
     }
     __syncthreads();
     if constexpr (!BPRE) load_b(q, bcur);
-    if (q + 1 < q1) {
-      load_a(q + 1);
-      if constexpr (BPRE) load_b(q + 1, bnext);
-    }
+    if (q + DEPTH < q1) load_a(q + DEPTH, ab);
+    if constexpr (BPRE)
+      if (q + 1 < q1) load_b(q + 1, bnext);
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
 #pragma unroll
@@ -312,6 +318,14 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm, 
 #pragma unroll
         for (int k = 0; k < KS; ++k) bcur[j][k] = bnext[j][k];
     }
+  };
+  if constexpr (DEPTH > 1) {
+    for (int q = q0; q < q1; q += 2) {
+      step(q, apre);
+      if (q + 1 < q1) step(q + 1, apre2);
+    }
+  } else {
+    for (int q = q0; q < q1; ++q) step(q, apre);
   }
   const float ih = (float)inv_half[box_level[t]];
   float *out = Lc + (size_t)t * n3 * ldm;
@@ -321,7 +335,7 @@ __global__ __launch_bounds__(256) void k_m2l_bf16_pipe(int n3, int nr, int ldm, 
     for (int j = 0; j < MTW; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = rt * 16 + (lane >> 4) * 4 + v, m = (wave * MTW + j) * 16 + li;
+        const int r = (rt0 + rt) * 16 + (lane >> 4) * 4 + v, m = (wave * MTW + j) * 16 + li;
         if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
       }
 }
@@ -1213,8 +1227,9 @@ void launch_m2l_tiles_f64(const FmmDev<double> &f, int n3, const double *M, doub
 template <int MT, int RTM>
 void launch_m2l_bf16(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb, float *Lc,
                      int xrun, hipStream_t s) {
-  hipLaunchKernelGGL((k_m2l_bf16_pipe<MT, RTM>), dim3(f.n_targets), dim3(256), 0, s, n3, nr, ldm,
-                     xrun,
+  hipLaunchKernelGGL((k_m2l_bf16_pipe<MT, RTM>),
+                     dim3(f.n_targets, ((nr + 15) / 16 + RTM - 1) / RTM), dim3(256), 0, s, n3, nr,
+                     ldm, xrun,
                      f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
                      reinterpret_cast<const __bf16 *>(f.Gbf), Mb, Lc);
 }
@@ -1232,6 +1247,14 @@ void launch_m2l_pairs(const FmmDev<float> &f, int n3, int nr, int ldm, const __b
   GP_LAUNCH_CHECK();
 }
 
+template <int MT, int RTM>
+void launch_m2l_bf16_d2(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
+                        float *Lc, int xrun, hipStream_t s) {
+  hipLaunchKernelGGL((k_m2l_bf16_pipe<MT, RTM, 2>),
+                     dim3(f.n_targets, ((nr + 15) / 16 + RTM - 1) / RTM), dim3(256), 0, s, n3, nr,
+                     ldm, xrun, f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                     reinterpret_cast<const __bf16 *>(f.Gbf), Mb, Lc);
+}
 template <int MT, int RTM>
 void launch_m2l_f64(const FmmDev<double> &f, int n3, int nr, int ldm, const double *M, double *Lc,
                     int xrun, hipStream_t s) {
@@ -1995,12 +2018,23 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * m3 * nr * 2 + f.n_targets * mbytes);
-        // sibling pairs or one target a work-group; XCD runs of 16 or 64, or
-        // the hardware order
+        // one target a work-group (XCD runs of 16 or 64, or the hardware
+        // order), the RHS tiles split over two work-groups per target (half
+        // the registers and LDS, twice the G_off loads), or sibling pairs
         const bool pairs = f.pair_tgt && m3p <= 128;
-        tuned_launch("m2l_bf16_pipe", f.n_targets, MT * 100 + RT, ldm, pairs ? 5 : 3, s, [&](int v) {
-          if (pairs && v >= 3) {
-            launch_m2l_pairs(fm, m3, nr, ldm, mbo, Lo, v == 3 ? 16 : 0, s);
+        const bool split = RT > 4;
+        std::vector<int> var = {0, 1, 2, 6};
+        if (split) var.push_back(3);
+        if (pairs) var.push_back(4), var.push_back(5);
+        tuned_launch("m2l_bf16_pipe", f.n_targets, MT * 100 + RT, ldm, (int)var.size(), s,
+                     [&](int vi) {
+          const int v = var[vi];
+          if (v == 6) {  // source rows two pairs ahead
+            GP_M2L_DISPATCH(launch_m2l_bf16_d2, MT, RT, fm, m3, nr, ldm, mbo, Lo, 16, s);
+          } else if (v >= 4) {
+            launch_m2l_pairs(fm, m3, nr, ldm, mbo, Lo, v == 4 ? 16 : 0, s);
+          } else if (v == 3) {
+            GP_M2L_DISPATCH(launch_m2l_bf16, MT, 4, fm, m3, nr, ldm, mbo, Lo, 16, s);
           } else {
             const int xr = v == 0 ? 16 : v == 1 ? 0 : 64;
             GP_M2L_DISPATCH(launch_m2l_bf16, MT, RT, fm, m3, nr, ldm, mbo, Lo, xr, s);
