@@ -1049,68 +1049,15 @@ __global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles, int xr
     }
 }
 
-// Single right-hand side (ldm = 1): M2L grouped by offset.  One launch per
-// M2L offset o covers every (target, source) pair with that offset (at most
-// one per target, so the updates of one launch never collide and the sum
-// order over offsets is fixed): D (node x pair) = G_o (node x l) *
-// [M_s1 .. M_s16] (l x pair) on the matrix cores, 16 pairs per wavefront,
-// then Lc[t] += D[:, pair] / half.  G_o fragments (A operand) come from the
-// L2-resident table, the source multipoles (B operand) from HBM.
-template <int MT>
-__global__ __launch_bounds__(256) void k_m2l1_bf16(int n3, int npairs, const int *__restrict__ tgt,
-                                                   const int *__restrict__ src,
-                                                   const int *__restrict__ box_level,
-                                                   const double *__restrict__ inv_half,
-                                                   const __bf16 *__restrict__ Go,
-                                                   const __bf16 *__restrict__ Mb,
-                                                   float *__restrict__ Lc) {
-  constexpr int NP = 16 * MT, KS = NP / 32;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j0 = (blockIdx.x * 4 + wave) * 16;
-  if (j0 >= npairs) return;
-  const int li = lane & 15, lk = (lane >> 4) * 8;
-  const int jp = min(j0 + li, npairs - 1);  // pair of this lane's B column
-  const __bf16 *Ms = Mb + (size_t)src[jp] * NP;
-  f32x4 acc[MT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int k = 0; k < KS; ++k) {
-    const bf16x8 b = *(const bf16x8 *)(Ms + k * 32 + lk);
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const bf16x8 a = *(const bf16x8 *)(Go + (size_t)(mt * 16 + li) * NP + k * 32 + lk);
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt], 0, 0, 0);
-    }
-  }
-  // D[i = node][j = pair]: lane holds nodes 4 (lane / 16) + v of pair lane % 16,
-  // 16 contiguous bytes of the target's local expansion
-  const int j = j0 + li;
-  if (j < npairs) {
-    const int t = tgt[j];
-    const float ih = (float)inv_half[box_level[t]];
-    float *out = Lc + (size_t)t * n3;
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const int m0 = mt * 16 + (lane >> 4) * 4;
-      if ((n3 & 3) == 0) {
-        if (m0 < n3) {
-          f32x4 o = *(f32x4 *)(out + m0);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) o[v] += acc[mt][v] * ih;
-          *(f32x4 *)(out + m0) = o;
-        }
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (m0 + v < n3) out[m0 + v] += acc[mt][v] * ih;
-      }
-    }
-  }
-}
 
-// FP64 counterpart (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16 k, B[k][j]
-// in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), register i / 4).
+// Single right-hand side (ldm = 1), FP64 beyond the 64-target tiles: M2L
+// grouped by offset.  One launch per M2L offset o covers every (target,
+// source) pair with that offset (at most one per target, so the updates of one
+// launch never collide and the sum order over offsets is fixed): D (node x
+// pair) = G_o (node x l) * [M_s1 .. M_s16] (l x pair) on the FP64 matrix
+// cores (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16 k, B[k][j] in lane
+// j + 16 k, D[i][j] in lane j + 16 (i % 4), register i / 4), 16 pairs per
+// wavefront, then Lc[t] += D[:, pair] / half.
 template <int MT>
 __global__ __launch_bounds__(256) void k_m2l1_f64(int n3, int npairs, const int *__restrict__ tgt,
                                                   const int *__restrict__ src,
@@ -1152,8 +1099,7 @@ __global__ __launch_bounds__(256) void k_m2l1_f64(int n3, int npairs, const int 
   }
 }
 
-template <typename T>
-void launch_m2l_by_offset(const FmmDev<T> &f, int n3, const void *Mop, T *Lc, bool bf16,
+void launch_m2l_by_offset(const FmmDev<double> &f, int n3, const double *M, double *Lc,
                           hipStream_t s) {
   const int n3p = f.n3p, MT = n3p / 16;
   for (int o = 0; o < 343; ++o) {
@@ -1161,27 +1107,13 @@ void launch_m2l_by_offset(const FmmDev<T> &f, int n3, const void *Mop, T *Lc, bo
     if (np == 0) continue;
     const dim3 grid((np + 63) / 64);
     const int *tg = f.po_tgt + p0, *sr = f.po_src + p0;
-    if constexpr (sizeof(T) == 4) {
-      if (!bf16) throw Error(GP_ERR_INTERNAL, "single-RHS M2L by offset runs BF16 or FP64");
-      const __bf16 *G = reinterpret_cast<const __bf16 *>(f.Gbf) + (size_t)o * n3p * n3p;
-      const __bf16 *Mb = reinterpret_cast<const __bf16 *>(Mop);
-      switch (MT) {
-        case 2: hipLaunchKernelGGL((k_m2l1_bf16<2>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
-        case 4: hipLaunchKernelGGL((k_m2l1_bf16<4>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
-        case 8: hipLaunchKernelGGL((k_m2l1_bf16<8>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
-        case 14: hipLaunchKernelGGL((k_m2l1_bf16<14>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, Mb, Lc); break;
-        default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
-      }
-    } else {
-      const double *G = f.G + (size_t)o * n3p * n3p;
-      const double *M = reinterpret_cast<const double *>(Mop);
-      switch (MT) {
-        case 2: hipLaunchKernelGGL((k_m2l1_f64<2>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
-        case 4: hipLaunchKernelGGL((k_m2l1_f64<4>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
-        case 8: hipLaunchKernelGGL((k_m2l1_f64<8>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
-        case 14: hipLaunchKernelGGL((k_m2l1_f64<14>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
-        default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
-      }
+    const double *G = f.G + (size_t)o * n3p * n3p;
+    switch (MT) {
+      case 2: hipLaunchKernelGGL((k_m2l1_f64<2>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+      case 4: hipLaunchKernelGGL((k_m2l1_f64<4>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+      case 8: hipLaunchKernelGGL((k_m2l1_f64<8>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+      case 14: hipLaunchKernelGGL((k_m2l1_f64<14>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+      default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
     }
     GP_LAUNCH_CHECK();
   }
@@ -2002,8 +1934,7 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                      [&](int v) { launch_m2l_tiles(fm, m3, mbo, Lo, v ? 16 : 0, s); });
       }
     } else if (want_bf16 && by_offset) {
-      ProfScope pl("m2l1_bf16_mfma", s, fl, (double)f.n_pairs * (m3p * 2 + m3 * 8));
-      launch_m2l_by_offset(fm, m3, mbo, Lo, true, s);
+      throw Error(GP_ERR_INTERNAL, "single-RHS BF16 M2L needs its target tiles");
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset && f.tile_tgt &&
                f.tile_size == 64 && m3p <= 64) {
       if constexpr (sizeof(T) == 8) {
@@ -2013,8 +1944,10 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       }
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {
       if (!f.po_off) throw Error(GP_ERR_INTERNAL, "single-RHS M2L pairs by offset not built");
-      ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (m3p * 8 + m3 * 16));
-      launch_m2l_by_offset(fm, m3, Mo, Lo, false, s);
+      if constexpr (sizeof(T) == 8) {
+        ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (m3p * 8 + m3 * 16));
+        launch_m2l_by_offset(fm, m3, Mo, Lo, s);
+      }
     } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
         ProfScope pl("m2l_bf16_mfma", s, fl, (double)f.n_boxes * m3 * nr * 2 + f.n_targets * mbytes);
