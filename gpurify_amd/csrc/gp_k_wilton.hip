@@ -462,10 +462,11 @@ __global__ __launch_bounds__(256) void k_sens_rows(int N, const int *__restrict_
                                                    const double *__restrict__ cen,
                                                    const double *__restrict__ dv, long dstride,
                                                    const double *__restrict__ dcen,
+                                                   const double *__restrict__ darea,
                                                    const double *__restrict__ gq,
                                                    const double *__restrict__ dgq, double far2,
-                                                   const double *__restrict__ sigma, int lds,
-                                                   int nc, double *__restrict__ partial) {
+                                                   double far1, const double *__restrict__ sigma,
+                                                   int lds, int nc, double *__restrict__ partial) {
   __shared__ double sdp[ND][2048];
   __shared__ double sred[ND][256];
   const int mi = blockIdx.x, chunk = blockIdx.y, nm = gridDim.x;
@@ -483,7 +484,24 @@ __global__ __launch_bounds__(256) void k_sens_rows(int N, const int *__restrict_
     if (q < nl) {
       const int ml = mv_of[l];
       const double cx = cen[3 * l] - pv[0], cy = cen[3 * l + 1] - pv[1], cz = cen[3 * l + 2] - pv[2];
-      if (far2 > 0.0 && cx * cx + cy * cy + cz * cz > far2 * gq[20L * l + 16]) {
+      const double r2 = cx * cx + cy * cy + cz * cz;
+      if (far1 > 0.0 && r2 > far1 * gq[20L * l + 16]) {
+        // centroid rule A_l / |p - c_l| and its derivative
+        const double *w = gq + 20L * l + 12;
+        const double A = w[0] + w[1] + w[2] + w[3], ir = 1.0 / sqrt(r2), air3 = A * ir * ir * ir;
+#pragma unroll
+        for (int t = 0; t < ND; ++t) {
+          double gx = pd[t][0], gy = pd[t][1], gz = pd[t][2], dA = 0.0;
+          if (ml >= 0) {
+            const double *dc = dcen + (3L * t) * nm + 3 * ml;
+            gx -= dc[0];
+            gy -= dc[1];
+            gz -= dc[2];
+            dA = darea[(long)t * nm + ml];
+          }
+          sdp[t][q] = dA * ir + air3 * (cx * gx + cy * gy + cz * gz);
+        }
+      } else if (far2 > 0.0 && r2 > far2 * gq[20L * l + 16]) {
         double f[ND];
         far_entry<ND>(pv, pd, gq + 20L * l, ml >= 0 ? dgq + 16L * ml : nullptr, 16L * nm, f);
 #pragma unroll
@@ -559,14 +577,18 @@ __global__ __launch_bounds__(256) void k_sens_cols(int N, int nm, const int *__r
                                                    const double *__restrict__ verts,
                                                    const double *__restrict__ cen,
                                                    const double *__restrict__ dv, long dstride,
+                                                   const double *__restrict__ dcen,
+                                                   const double *__restrict__ darea,
                                                    const double *__restrict__ gq,
                                                    const double *__restrict__ dgq, double far2,
-                                                   const double *__restrict__ sigma, int lds,
-                                                   int nc, double *__restrict__ R) {
+                                                   double far1, const double *__restrict__ sigma,
+                                                   int lds, int nc, double *__restrict__ R) {
   // per staged panel: 12 vertices, 12 ND velocities, 17 Gauss values,
-  // 16 ND Gauss derivatives, 3 centroid, 1 pad
+  // 16 ND Gauss derivatives, 3 centroid, 1 pad, 3 ND centroid and ND area
+  // derivatives
   constexpr int OV = 0, ODV = 12, OG = 12 * (ND + 1), ODG = OG + 17, OC = ODG + 16 * ND;
-  constexpr int SW = OC + 4;
+  constexpr int ODC = OC + 4, ODA = ODC + 3 * ND;
+  constexpr int SW = ODA + ND;
   constexpr int MQ = 16;                 // moved panels per stage
   extern __shared__ double sm[];
   double *sv = sm;                       // MQ x SW
@@ -589,6 +611,8 @@ __global__ __launch_bounds__(256) void k_sens_cols(int N, int nm, const int *__r
       else if (c < ODG) x = gq[20L * l + (c - OG)];
       else if (c < OC) x = dgq[((long)((c - ODG) / 16) * nm + mq) * 16 + (c - ODG) % 16];
       else if (c < OC + 3) x = cen[3 * l + (c - OC)];
+      else if (c >= ODC && c < ODA) x = dcen[(3L * ((c - ODC) / 3)) * nm + 3 * mq + (c - ODC) % 3];
+      else if (c >= ODA) x = darea[(long)(c - ODA) * nm + mq];
       sv[e] = x;
     }
     for (int e = threadIdx.x; e < mn * nc; e += blockDim.x)
@@ -601,7 +625,16 @@ __global__ __launch_bounds__(256) void k_sens_cols(int N, int nm, const int *__r
       if (q < mn) {
         const double *st = sv + q * SW;
         const double cx = st[OC] - pv[0], cy = st[OC + 1] - pv[1], cz = st[OC + 2] - pv[2];
-        if (far2 > 0.0 && cx * cx + cy * cy + cz * cz > far2 * st[OG + 16]) {
+        const double r2 = cx * cx + cy * cy + cz * cz;
+        if (far1 > 0.0 && r2 > far1 * st[OG + 16]) {
+          // centroid rule A_l / |p - c_l|, the source moving
+          const double A = st[OG + 12] + st[OG + 13] + st[OG + 14] + st[OG + 15];
+          const double ir = 1.0 / sqrt(r2), air3 = A * ir * ir * ir;
+#pragma unroll
+          for (int t = 0; t < ND; ++t)
+            dp[t][q] = st[ODA + t] * ir - air3 * (cx * st[ODC + 3 * t] + cy * st[ODC + 3 * t + 1] +
+                                                  cz * st[ODC + 3 * t + 2]);
+        } else if (far2 > 0.0 && r2 > far2 * st[OG + 16]) {
           double f[ND];
           far_entry<ND>(pv, nullptr, st + OG, st + ODG, 16, f);
 #pragma unroll
@@ -685,19 +718,19 @@ void launch_sensitivity_nd(const SensArgs &a, hipStream_t s) {
                      a.dstride, a.dcen, a.darea, a.dgq);
   const int nchunk = (N + 2047) / 2048;
   hipLaunchKernelGGL((k_sens_rows<ND>), dim3(nm, nchunk), dim3(256), 0, s, N, a.moved, a.mv_of,
-                     a.verts, a.cen, a.dv, a.dstride, a.dcen, a.gq, a.dgq, a.far2, a.sigma, a.lds,
-                     nc, a.scratch);
+                     a.verts, a.cen, a.dv, a.dstride, a.dcen, a.darea, a.gq, a.dgq, a.far2, a.far1,
+                     a.sigma, a.lds, nc, a.scratch);
   hipLaunchKernelGGL(k_sens_rows_reduce, dim3((nm * ND * nc + 255) / 256), dim3(256), 0, s, N, nm,
                      nchunk, ND, nc, a.moved, a.scratch, a.R);
   // threads per block such that the staging area and the per-row
   // accumulators fit 64 KiB of LDS (ND * nc <= 512: at least 16 threads)
   int tpb = 256;
-  const size_t stage = 16 * (12 * (ND + 1) + 17 + 16 * ND + 4) + 16 * (size_t)nc;
+  const size_t stage = 16 * (12 * (ND + 1) + 17 + 16 * ND + 4 + 4 * ND) + 16 * (size_t)nc;
   while (tpb > 16 && sizeof(double) * (stage + (size_t)tpb * ND * nc) > 65536) tpb >>= 1;
   const size_t lds_cols = sizeof(double) * (stage + (size_t)tpb * ND * nc);
   hipLaunchKernelGGL((k_sens_cols<ND>), dim3((N + tpb - 1) / tpb), dim3(tpb), lds_cols, s, N, nm,
-                     a.moved, a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.gq, a.dgq, a.far2,
-                     a.sigma, a.lds, nc, a.R);
+                     a.moved, a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.dcen, a.darea, a.gq,
+                     a.dgq, a.far2, a.far1, a.sigma, a.lds, nc, a.R);
   const int rows = 2048, nblk = (N + rows - 1) / rows;
   for (int t = 0; t < ND; ++t) {
     hipLaunchKernelGGL(k_scatter_da, dim3(g), dim3(256), 0, s, nm, a.moved, a.darea + (long)t * nm,

@@ -25,12 +25,13 @@ void launch_self_inv_diag(int n, const double *verts, const double *cen, double 
 // and are restored.  scratch: sensitivity_scratch(N, nm, nc, nd) doubles;
 // R: nd x N x nc; dcen: 3 nd nm; darea: nd nm; dgq: 16 nd nm.  J: nd x nc x nc.
 // gq: launch_gauss_table of the panels; entries whose source centroid lies
-// beyond sqrt(far2) source diagonals use its 2 x 2 Gauss rule (far2 = 0:
+// beyond sqrt(far2) source diagonals use its 2 x 2 Gauss rule, beyond
+// sqrt(far1) diagonals the centroid rule A / |p - c| (0: not used; both 0:
 // every entry by the closed form).
 struct SensArgs {
   int N = 0, nm = 0, nc = 0, nd = 1, lds = 0, ldl = 0, cond0 = 0;
   long dstride = 0;
-  double far2 = 0.0;
+  double far2 = 0.0, far1 = 0.0;
   const double *gq = nullptr;
   double *dgq = nullptr;
   const int *moved = nullptr;

@@ -1897,8 +1897,10 @@ void cap_jacobian(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, const P
     launch_gauss_table(N, dverts, gq.p, s);
     // The dense path differentiates every entry in closed form; the FMM path
     // takes the Gauss rule beyond 6 source diagonals (relative entry error
-    // below 4e-6, far under the FMM's own).
+    // below 4e-6) and the centroid rule beyond 30 (below 8e-5), both under
+    // the FMM's own far-field error.
     const double far2 = method == GP_METHOD_FMM ? 36.0 : 0.0;
+    const double far1 = method == GP_METHOD_FMM ? 900.0 : 0.0;
     GP_HIP(hipMemsetAsync(mv_of.p, 0xff, N * sizeof(int), s));
     da.zero(s);
     R.zero(s);
@@ -1935,6 +1937,7 @@ void cap_jacobian(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, const P
       a.gq = gq.p;
       a.dgq = dgq.p;
       a.far2 = far2;
+      a.far1 = far1;
       a.J = dJ.p + (size_t)p * nc * nc;
       launch_sensitivity(a, s);
       p += nd;
