@@ -215,15 +215,6 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf(
   const int c0 = col_off[T], S = col_off[T + 1] - c0;
   const int64_t base = nf_off[T];
   const int lane = threadIdx.x & 63;
-  // the first vector of this wavefront's first row does not depend on x:
-  // its load overlaps the gather below
-  V pre = {};
-  {
-    const int t = threadIdx.x >> 6;
-    const int64_t st = base + (int64_t)t * S;
-    const int head = min((int)((VN - (st & (VN - 1))) & (VN - 1)), S);
-    if (t < nt && lane < (S - head) / VN) pre = reinterpret_cast<const V *>(vals + st + head)[lane];
-  }
   for (int s = threadIdx.x; s < S; s += 64 * WPB) xs[s] = (A)x[cols[c0 + s]];
   __syncthreads();
   for (int t = threadIdx.x >> 6; t < nt; t += WPB) {
@@ -236,7 +227,7 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf(
     const V *rv = reinterpret_cast<const V *>(row + head);
 #pragma unroll 4
     for (int q = lane; q < nv; q += 64) {
-      const V v = (t < WPB && q < 64) ? pre : rv[q];
+      const V v = rv[q];
       const A *xq = xs + head + q * VN;
 #pragma unroll
       for (int k = 0; k < VN; ++k) acc += (A)v[k] * xq[k];
