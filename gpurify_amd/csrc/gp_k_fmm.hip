@@ -636,6 +636,131 @@ __global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
   }
 }
 
+// Split-BF16 products for the low-rank M2L: every projected multipole and
+// every kr x kr table entry is a BF16 pair hi + lo (lo = the rounding
+// residual of hi), and the matrix cores form hi*hi + hi*lo + lo*hi with FP32
+// accumulation: products to ~2^-16 relative (the dropped lo*lo term and the
+// pair's mantissa) instead of BF16's 2^-8, at three BF16 operations.  The
+// projected basis has mixed signs and a wide range, where plain BF16 rounding
+// slows the refinement.  Otherwise as k_m2l_bf16_pipe (one pair of
+// prefetch).
+template <int MT, int RTM>
+__global__ __launch_bounds__(256) void k_m2l_bf16x3(int n3, int nr, int ldm, int xrun,
+                                                    const int *__restrict__ targets,
+                                                    const int *__restrict__ il_off,
+                                                    const int *__restrict__ il_src,
+                                                    const int *__restrict__ il_oidx,
+                                                    const int *__restrict__ box_level,
+                                                    const double *__restrict__ inv_half,
+                                                    const __bf16 *__restrict__ G,
+                                                    const __bf16 *__restrict__ Gl,
+                                                    const __bf16 *__restrict__ Mb,
+                                                    const __bf16 *__restrict__ Mbl,
+                                                    float *__restrict__ Lc) {
+  constexpr int NP = 16 * MT, KS = NP / 32, MTW = (MT + 3) / 4, LDA = NP + 8, VPR = NP / 8;
+  constexpr int APT = (RTM * 16 * VPR + 255) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 sA[RTM * 16 * LDA], sAl[RTM * 16 * LDA];
+  const int t = targets[xcd_block(blockIdx.x, gridDim.x, xrun)];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  const int rt0 = blockIdx.y * RTM;
+  const int RT = min((nr + 15) >> 4, rt0 + RTM) - rt0;
+  const int AV = RT * 16 * VPR;
+  const size_t MS = (size_t)ldm * NP;
+  f32x4 acc[RTM][MTW];
+#pragma unroll
+  for (int i = 0; i < RTM; ++i)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int q0 = il_off[t], q1 = il_off[t + 1];
+  bf16x8 ah[APT], al[APT];
+  bf16x8 bh[MTW][KS], bl[MTW][KS], nh[MTW][KS], nl[MTW][KS];
+  auto load_a = [&](int q) {
+    const size_t off = (size_t)il_src[q] * MS + (size_t)rt0 * 16 * NP;
+    const bf16x8 *sh = reinterpret_cast<const bf16x8 *>(Mb + off);
+    const bf16x8 *sl = reinterpret_cast<const bf16x8 *>(Mbl + off);
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < AV) {
+        ah[i] = sh[v];
+        al[i] = sl[v];
+      }
+    }
+  };
+  auto load_b = [&](int q, bf16x8 (&dh)[MTW][KS], bf16x8 (&dl)[MTW][KS]) {
+    const size_t off = (size_t)il_oidx[q] * NP * NP;
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) {
+      const int mt = wave * MTW + j;
+      if (mt < MT)
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+          const size_t e = off + (size_t)(mt * 16 + li) * NP + k * 32 + lk;
+          dh[j][k] = *(const bf16x8 *)(G + e);
+          dl[j][k] = *(const bf16x8 *)(Gl + e);
+        }
+    }
+  };
+  if (q0 < q1) {
+    load_a(q0);
+    load_b(q0, bh, bl);
+  }
+  for (int q = q0; q < q1; ++q) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < AV) {
+        const int o = (v / VPR) * LDA + (v % VPR) * 8;
+        *(bf16x8 *)(sA + o) = ah[i];
+        *(bf16x8 *)(sAl + o) = al[i];
+      }
+    }
+    __syncthreads();
+    if (q + 1 < q1) {
+      load_a(q + 1);
+      load_b(q + 1, nh, nl);
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+#pragma unroll
+      for (int rt = 0; rt < RTM; ++rt) {
+        if (rt < RT) {
+          const int o = (rt * 16 + li) * LDA + k * 32 + lk;
+          const bf16x8 a = *(const bf16x8 *)(sA + o), a2 = *(const bf16x8 *)(sAl + o);
+#pragma unroll
+          for (int j = 0; j < MTW; ++j)
+            if (wave * MTW + j < MT) {
+              f32x4 c = acc[rt][j];
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bh[j][k], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[j][k], c, 0, 0, 0);
+              acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[j][k], c, 0, 0, 0);
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int k = 0; k < KS; ++k) {
+        bh[j][k] = nh[j][k];
+        bl[j][k] = nl[j][k];
+      }
+  }
+  const float ih = (float)inv_half[box_level[t]];
+  float *out = Lc + (size_t)t * n3 * ldm;
+#pragma unroll
+  for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+    for (int j = 0; j < MTW; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = (rt0 + rt) * 16 + (lane >> 4) * 4 + v, m = (wave * MTW + j) * 16 + li;
+        if (rt < RT && r < nr && m < n3) out[(size_t)m * ldm + r] = acc[rt][j][v] * ih;
+      }
+}
+
 // Sibling pairs: one work-group per (pair of sibling targets, RTM RHS tiles
 // from tile rt0 = RTM blockIdx.y).  The pair's entries are the union of the
 // two interaction lists ordered by source box, each with the source's offset
@@ -1180,6 +1305,15 @@ void launch_m2l_pairs(const FmmDev<float> &f, int n3, int nr, int ldm, const __b
 }
 
 template <int MT, int RTM>
+void launch_m2l_bf16x3(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
+                       const __bf16 *Mbl, float *Lc, int xrun, hipStream_t s) {
+  hipLaunchKernelGGL((k_m2l_bf16x3<MT, RTM>),
+                     dim3(f.n_targets, ((nr + 15) / 16 + RTM - 1) / RTM), dim3(256), 0, s, n3, nr,
+                     ldm, xrun, f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                     reinterpret_cast<const __bf16 *>(f.Gbf),
+                     reinterpret_cast<const __bf16 *>(f.Gcbf_lo), Mb, Mbl, Lc);
+}
+template <int MT, int RTM>
 void launch_m2l_bf16_d2(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
                         float *Lc, int xrun, hipStream_t s) {
   hipLaunchKernelGGL((k_m2l_bf16_pipe<MT, RTM, 2>),
@@ -1625,7 +1759,8 @@ __global__ __launch_bounds__(256) void k_m2l_compress(int n3, int n3p, int box0,
                                                       int ldm, const float *__restrict__ U,
                                                       const float *__restrict__ M,
                                                       float *__restrict__ Mt,
-                                                      __bf16 *__restrict__ Mtb) {
+                                                      __bf16 *__restrict__ Mtb,
+                                                      __bf16 *__restrict__ Mtbl) {
   constexpr int TC = KR / 16;
   __shared__ __align__(16) float sM[32][64 + 4];  // [k][row]
   __shared__ float sU[32][KR];                    // [k][col]
@@ -1669,8 +1804,13 @@ __global__ __launch_bounds__(256) void k_m2l_compress(int n3, int n3p, int box0,
     const size_t rb = ((size_t)(box0 + q / nr) * ldm + q % nr) * KR;
 #pragma unroll
     for (int c = 0; c < TC; ++c) {
-      Mt[rb + tx + 16 * c] = acc[i][c];
-      if (Mtb) Mtb[rb + tx + 16 * c] = (__bf16)acc[i][c];
+      const float v = acc[i][c];
+      Mt[rb + tx + 16 * c] = v;
+      if (Mtb) {
+        const __bf16 h = (__bf16)v;
+        Mtb[rb + tx + 16 * c] = h;
+        if (Mtbl) Mtbl[rb + tx + 16 * c] = (__bf16)(v - (float)h);
+      }
     }
   }
 }
@@ -1782,7 +1922,7 @@ void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, c
 template <typename T>
 void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf16 *Mb, T *Lc,
                      int ldm, int m2l, double *y, int ldy, hipStream_t s, T *Mt, __bf16 *Mtb,
-                     T *Lt) {
+                     T *Lt, __bf16 *Mtbl) {
   const int n = f.n, n3 = n * n * n, n3p = f.n3p;
   const int nrc = (nr + FM_RC - 1) / FM_RC;
   const size_t lds1 = (size_t)n3 * FM_RC * sizeof(T);
@@ -1869,6 +2009,9 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   int m3 = n3;
   T *Mo = M, *Lo = Lc;
   __bf16 *mbo = mb;
+  // split-BF16 products for a block of RHS on the projected operator
+  const bool split = comp && want_bf16 && !by_offset && Mtbl && f.Gcbf_lo;
+  __bf16 *mbl = split ? Mtbl : nullptr;
   if (comp) {
     fm.n3p = f.kr;
     fm.G = f.Gc;
@@ -1886,13 +2029,13 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
         const dim3 grid((nrow + 63) / 64);
         if (f.kr == 32)
           hipLaunchKernelGGL((k_m2l_compress<32>), grid, dim3(256), 0, s, n3, n3p, b0, nrow, nr,
-                             ldm, f.U, M, Mt, mbo);
+                             ldm, f.U, M, Mt, mbo, mbl);
         else if (f.kr == 64)
           hipLaunchKernelGGL((k_m2l_compress<64>), grid, dim3(256), 0, s, n3, n3p, b0, nrow, nr,
-                             ldm, f.U, M, Mt, mbo);
+                             ldm, f.U, M, Mt, mbo, mbl);
         else
           hipLaunchKernelGGL((k_m2l_compress<128>), grid, dim3(256), 0, s, n3, n3p, b0, nrow, nr,
-                             ldm, f.U, M, Mt, mbo);
+                             ldm, f.U, M, Mt, mbo, mbl);
         GP_LAUNCH_CHECK();
       }
     }
@@ -1947,6 +2090,13 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       if constexpr (sizeof(T) == 8) {
         ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (m3p * 8 + m3 * 16));
         launch_m2l_by_offset(fm, m3, Mo, Lo, s);
+      }
+    } else if (split) {
+      if constexpr (sizeof(T) == 4) {
+        // the algorithm's matrix-core work: three BF16 products per projected pair
+        ProfScope pl("m2l_bf16x3_mfma", s, 3.0 * fl,
+                     (double)f.n_boxes * m3 * nr * 4 + f.n_targets * mbytes);
+        GP_M2L_DISPATCH(launch_m2l_bf16x3, MT, RT, fm, m3, nr, ldm, mbo, mbl, Lo, 16, s);
       }
     } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
@@ -2049,13 +2199,14 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
                    double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s) {
   run_far_field_t<double>(f, z, ldz, nr, M, nullptr, Lc, ldm, m2l, y, ldy, s, nullptr, nullptr,
-                          nullptr);
+                          nullptr, nullptr);
 }
 void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,
                    float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s, float *Mt,
-                   void *Mtb, float *Lt) {
+                   void *Mtb, float *Lt, void *Mtbl) {
   run_far_field_t<float>(f, z, ldz, nr, M, reinterpret_cast<__bf16 *>(Mb), Lc, ldm, m2l, y, ldy,
-                         s, Mt, reinterpret_cast<__bf16 *>(Mtb), Lt);
+                         s, Mt, reinterpret_cast<__bf16 *>(Mtb), Lt,
+                         reinterpret_cast<__bf16 *>(Mtbl));
 }
 
 }  // namespace gp

@@ -129,7 +129,7 @@ struct FmmDev {
   // (row-major), tables Gc / Gcbf / Gcr as G / Gbf / Gr with kr x kr blocks.
   int kr = 0;
   const T *U = nullptr, *Gc = nullptr, *Gcr = nullptr;
-  const void *Gcbf = nullptr;
+  const void *Gcbf = nullptr, *Gcbf_lo = nullptr;  // BF16 pair hi + lo of Gc
   // Single right-hand side: the M2L pairs grouped by offset index o, pairs
   // [po_off[o], po_off[o+1]) (HOST, 344 entries) of device lists po_tgt /
   // po_src; null when not built.
@@ -161,12 +161,14 @@ void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, c
 // BF16_VALU rounds M to BF16 in place after the upward pass and runs the
 // FP32 VALU M2L on the table Gr.  A low-rank operator (f.kr > 0) needs the
 // projected multipoles Mt (n_boxes x ldm x kr, BF16 copy Mtb for the BF16
-// M2L) and locals Lt (n_boxes x kr x ldm); Mb is then unused.
+// M2L, with Mtbl its rounding residual for the split-BF16 products of a
+// block of RHS) and locals Lt (n_boxes x kr x ldm); Mb is then unused.
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
                    double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
 void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,
                    float *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s,
-                   float *Mt = nullptr, void *Mtb = nullptr, float *Lt = nullptr);
+                   float *Mt = nullptr, void *Mtb = nullptr, float *Lt = nullptr,
+                   void *Mtbl = nullptr);
 
 // ---------------------------------------------------------------- Krylov
 // Basis types TV / TX / TO: double or float (explicitly instantiated).

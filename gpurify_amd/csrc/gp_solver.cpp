@@ -381,7 +381,7 @@ struct FmmSystem {
   // Low-rank inner M2L: rank kr (0 = off), projection U and kr x kr tables.
   int kr = 0;
   DBuf<float> U32, Gc32, Gcr32;
-  DBuf<uint16_t> Gcbf;
+  DBuf<uint16_t> Gcbf, Gcbf_lo;
   DBuf<int> coff, cidx;
   NearTiles nt;
   FmmDev<double> f64;
@@ -389,7 +389,7 @@ struct FmmSystem {
   int ld = 0;
   DBuf<double> z64, M64, Lc64;
   DBuf<float> z32, M32, Lc32, Mt32, Lt32;
-  DBuf<uint16_t> Mbf, Mtbf;
+  DBuf<uint16_t> Mbf, Mtbf, Mtbf_lo;
   bool bf16_ok = false;  // the BF16 M2L runs at this ld
   long matvecs = 0;
   // Transposed operator P'^T (adjoint solves): set want_transpose before setup.
@@ -657,8 +657,9 @@ struct FmmSystem {
     // FMM tables.
     const int n3 = n * n * n;
     n3p = round_up(n3, 32);
-    kr = 0;  // low-rank inner M2L: rank 32, 64 or 128 below n3p
-    if (need32 && o.m2l_rank > 0) {
+    kr = 0;  // low-rank inner M2L: rank 32, 64 or 128 below n3p (blocks of RHS only:
+             // single-RHS solves run plain BF16 target tiles, too coarse for it)
+    if (need32 && o.m2l_rank > 0 && !want_by_offset) {
       kr = o.m2l_rank <= 32 ? 32 : o.m2l_rank <= 64 ? 64 : 128;
       if (kr >= n3p) kr = 0;
     }
@@ -914,9 +915,17 @@ struct FmmSystem {
         auto lr = low_rank_tables(n, n3p, kr, hG);
         U32 = to_device(cast_vec<float>(lr->U), s);
         Gc32 = to_device(cast_vec<float>(lr->Gc), s);
-        std::vector<uint16_t> hcb(lr->Gc.size());
-        for (size_t e = 0; e < hcb.size(); ++e) hcb[e] = bf16_bits((float)lr->Gc[e]);
+        std::vector<uint16_t> hcb(lr->Gc.size()), hcl(lr->Gc.size());
+        for (size_t e = 0; e < hcb.size(); ++e) {
+          const float v = (float)lr->Gc[e];
+          hcb[e] = bf16_bits(v);
+          const uint32_t u = (uint32_t)hcb[e] << 16;
+          float h;
+          std::memcpy(&h, &u, 4);
+          hcl[e] = bf16_bits(v - h);  // the pair hi + lo for the split-BF16 products
+        }
         Gcbf = to_device(hcb, s);
+        Gcbf_lo = to_device(hcl, s);
         if (m2l_inner == GP_M2L_BF16_VALU) {
           std::vector<float> hr(hcb.size());
           for (size_t e = 0; e < hcb.size(); ++e) {
@@ -984,6 +993,7 @@ struct FmmSystem {
     f32.U = U32.p;
     f32.Gc = Gc32.p;
     f32.Gcbf = Gcbf.p;
+    f32.Gcbf_lo = Gcbf_lo.p;
     f32.Gcr = Gcr32.p;
     // Panels of each (batch-global) conductor, ascending caller order.
     std::vector<int> hco{0}, hci;
@@ -1030,6 +1040,10 @@ struct FmmSystem {
         if (bf16_ok) {
           Mtbf.alloc(texp);
           Mtbf.zero(s);
+          if (ld > 1) {  // blocks of RHS take the split-BF16 products
+            Mtbf_lo.alloc(texp);
+            Mtbf_lo.zero(s);
+          }
         }
       } else if (bf16_ok) {
         Mbf.alloc(mexp);
@@ -1062,7 +1076,7 @@ struct FmmSystem {
       kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
       launch_near_apply(nt, nf32.p, z32.p, ld, y, ld, nr, s);
       run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, inner_m2l(), y, ld, s, Mt32.p,
-                    Mtbf.p, Lt32.p);
+                    Mtbf.p, Lt32.p, Mtbf_lo.p);
     }
   }
 
@@ -1087,7 +1101,7 @@ struct FmmSystem {
       launch_near_apply_tr(nt, ntr, nf32.p, z32.p, ld, y, ld, nr, s);
       kr_precond_cast(v, dfar, z32.p, N, ld, nr, s);
       run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, inner_m2l(), ytmp.p, ld, s,
-                    Mt32.p, Mtbf.p, Lt32.p);
+                    Mt32.p, Mtbf.p, Lt32.p, Mtbf_lo.p);
     }
     kr_rows_axpy_clear(y, ytmp.p, area.p, N, ld, nr, s);
   }

@@ -49,7 +49,7 @@ def test_every_variant_applies_the_same_operator(ctx, order):
 def test_autotuner_records_its_choices(ctx):
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     X = np.random.default_rng(3).normal(size=(len(v), 97))
-    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=5, m2l_inner=G.GP_M2L_BF16)
+    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=5, m2l_inner=G.GP_M2L_BF16, m2l_rank=0)
     ctx.tune_force(0)
     Y0 = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
     ctx.tune_force(-1)  # clears the cache: the next launches tune
