@@ -1872,6 +1872,183 @@ __global__ __launch_bounds__(256) void k_m2l_expand(int n3, int kr, int ncol, in
   }
 }
 
+// The same projection and expansion on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32: lane holds A[i = lane % 16][k = lane / 16],
+// B[k = lane / 16][j = lane % 16], D[i = 4 (lane / 16) + v][j = lane % 16]).
+// Both are HBM streams (FP32 expansions in, projected ones out, or back), so
+// one wavefront takes one block of rows at a time with all its loads issued
+// ahead, and U sits in LDS for the whole work-group.
+
+// Projection: per wavefront 16 multipole rows q (column j of the product),
+// D_ct = U_ct^T M^T over the n3 nodes in NS steps of 16.  Tile ct's output row
+// i = 4 qq + v is projected column (KR / 4) qq + 4 ct + v, so lane (j, qq) ends
+// with KR / 4 consecutive columns of row j (16-byte stores).  The k order
+// inside a step is 4 qq + t, matching one 16-byte load of M per lane.
+template <int KR, int NS>
+__global__ __launch_bounds__(256) void k_m2l_compress_mfma(int n3, int box0, int nrow, int nr,
+                                                           int ldm, const float *__restrict__ U,
+                                                           const float *__restrict__ M,
+                                                           float *__restrict__ Mt,
+                                                           __bf16 *__restrict__ Mtb,
+                                                           __bf16 *__restrict__ Mtbl) {
+  constexpr int CT = KR / 16, QW = KR / 4, N3P = NS * 16, LDA = N3P + 4;
+  extern __shared__ __align__(16) float sA[];  // [CT][16][LDA]: U_ct^T
+  for (int e = threadIdx.x; e < CT * 16 * N3P; e += 256) {
+    const int k = e % N3P, ci = e / N3P, ct = ci >> 4, i = ci & 15;
+    const int col = QW * (i >> 2) + 4 * ct + (i & 3);
+    sA[ci * LDA + k] = k < n3 ? U[(size_t)k * KR + col] : 0.f;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, j = lane & 15, qq = lane >> 4;
+  const int ngroups = (nrow + 15) >> 4;
+  const int stride = gridDim.x * 4;
+  int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // row base of multipole row q (the caller's M rows are n3p = N3P wide)
+  auto row_ptr = [&](int q) -> const float * {
+    return M + ((size_t)(box0 + q / nr) * ldm + q % nr) * N3P;
+  };
+  f32x4 bn[NS];
+  auto load = [&](int gg) {
+    const int q = gg * 16 + j;
+    const bool ok = gg < ngroups && q < nrow;
+    const float *p = ok ? row_ptr(q) : M;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int k0 = 16 * s + 4 * qq;
+      f32x4 v = {0.f, 0.f, 0.f, 0.f};
+      if (ok && k0 < n3) {
+        v = *reinterpret_cast<const f32x4 *>(p + k0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (k0 + t >= n3) v[t] = 0.f;  // node padding may hold anything
+      }
+      bn[s] = v;
+    }
+  };
+  load(g);
+  for (; g < ngroups; g += stride) {
+    f32x4 b[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) b[s] = bn[s];
+    load(g + stride);  // the next block's rows stream in under this one's products
+    f32x4 acc[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const f32x4 a = *reinterpret_cast<const f32x4 *>(&sA[(ct * 16 + j) * LDA + 16 * s + 4 * qq]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b[s][t], acc[ct], 0, 0, 0);
+      }
+    }
+    // lane (qq, v) of tile ct holds row j, projected column QW qq + 4 ct + v
+    const int q = g * 16 + j;
+    if (q < nrow) {
+      const size_t rb = ((size_t)(box0 + q / nr) * ldm + q % nr) * KR + QW * qq;
+      if (Mtb) {
+        __bf16 h[QW], l[QW];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float x = acc[ct][v];
+            const __bf16 hi = (__bf16)x;
+            h[4 * ct + v] = hi;
+            l[4 * ct + v] = (__bf16)(x - (float)hi);
+          }
+#pragma unroll
+        for (int c = 0; c < QW; c += 8) {
+          bf16x8 vh, vl;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) vh[e] = h[c + e], vl[e] = l[c + e];
+          *reinterpret_cast<bf16x8 *>(Mtb + rb + c) = vh;
+          if (Mtbl) *reinterpret_cast<bf16x8 *>(Mtbl + rb + c) = vl;
+        }
+      } else {
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) *reinterpret_cast<f32x4 *>(Mt + rb + 4 * ct) = acc[ct];
+      }
+    }
+  }
+}
+
+// Expansion: per wavefront one box b and 64 columns r0 .. r0 + 63 of it,
+// Lc[b][m][r] = sum_k U[m][k] Lt[b][k][r] for m < n3 in chunks of 64 nodes
+// (four 16-node tiles; U in LDS for the nodes rounded up to 16), every
+// column r < ldm written (zero for r >= nr).
+// Tile v holds columns r0 + 4 j + v, so lane (j, qq) stores 16 bytes per
+// node row; the k order inside a step of 16 is 4 qq + t, matching the lane's
+// four rows of Lt.  Boxes without an interaction list have Lt = 0 and get
+// Lc = 0, so the caller needs no zero fill of Lc.
+template <int KR>
+__global__ __launch_bounds__(256) void k_m2l_expand_mfma(int n3, int nmc, int n_boxes, int ngr,
+                                                         int nr, int ldm,
+                                                         const float *__restrict__ U,
+                                                         const float *__restrict__ Lt,
+                                                         float *__restrict__ Lc) {
+  constexpr int KS = KR / 16, LDU = KR + 4;
+  const int m16 = (n3 + 15) & ~15;  // node rows held in LDS
+  extern __shared__ __align__(16) float sU[];  // [m16][LDU]
+  for (int e = threadIdx.x; e < m16 * KR; e += 256) {
+    const int m = e / KR, k = e % KR;
+    sU[m * LDU + k] = m < n3 ? U[(size_t)m * KR + k] : 0.f;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, j = lane & 15, qq = lane >> 4;
+  const int nitems = n_boxes * ngr;
+  for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += gridDim.x * 4) {
+    const int b = w / ngr, r = (w % ngr) * 64 + 4 * j;
+    const bool rok = r < ldm;
+    f32x4 bv[KS][4];
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        bv[s][t] = rok ? *reinterpret_cast<const f32x4 *>(
+                             Lt + ((size_t)b * KR + 16 * s + 4 * qq + t) * ldm + r)
+                       : f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int mc = 0; mc < nmc; ++mc) {
+      f32x4 acc[4][4];  // [node tile][column tile v]
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[mt][v] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        f32x4 a[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+          a[mt] = mc * 64 + mt * 16 < m16
+                      ? *reinterpret_cast<const f32x4 *>(&sU[(mc * 64 + mt * 16 + j) * LDU + 16 * s + 4 * qq])
+                      : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              acc[mt][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt][t], bv[s][t][v], acc[mt][v], 0, 0, 0);
+      }
+      if (rok) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+          for (int vv = 0; vv < 4; ++vv) {
+            const int m = mc * 64 + mt * 16 + 4 * qq + vv;
+            if (m >= n3) continue;
+            f32x4 o;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) o[v] = r + v < nr ? acc[mt][v][vv] : 0.f;
+            *reinterpret_cast<f32x4 *>(Lc + ((size_t)b * n3 + m) * ldm + r) = o;
+          }
+      }
+    }
+  }
+}
+
 // One thread per (panel, axis): x = the centroid's coordinate in the leaf
 // box's [-1, 1] frame, w[m] = (2/n) (1/2 + sum_{k=1}^{n-1} T_k(x_m) T_k(x))
 // (chebyshev.rs:49-84), the Chebyshev polynomials by their recurrence.
@@ -2011,6 +2188,12 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   __bf16 *mbo = mb;
   // split-BF16 products for a block of RHS on the projected operator
   const bool split = comp && want_bf16 && !by_offset && Mtbl && f.Gcbf_lo;
+  // projection and expansion on the FP32 matrix cores: rank 32 or 64, node
+  // counts padded to 32, 64, 128 or 224, 16-byte column groups
+  const int pns = n3p / 16;
+  const bool proj_mfma = comp && sizeof(T) == 4 && (f.kr == 32 || f.kr == 64) &&
+                         n3p % 16 == 0 && (pns == 2 || pns == 4 || pns == 8 || pns == 14) &&
+                         ldm % 4 == 0;
   __bf16 *mbl = split ? Mtbl : nullptr;
   if (comp) {
     fm.n3p = f.kr;
@@ -2027,6 +2210,23 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                    (double)nrow * (n3 + f.kr) * sizeof(T));
       if constexpr (sizeof(T) == 4) {
         const dim3 grid((nrow + 63) / 64);
+        if (proj_mfma) {
+          const int ng = (nrow + 15) / 16;
+          const dim3 gm((unsigned)std::min(ng / 4 + 1, 2048));
+          const size_t lds = (size_t)f.kr * (n3p + 4) * sizeof(float);
+#define GP_COMPRESS_MFMA(KR, NS)                                                              \
+  hipLaunchKernelGGL((k_m2l_compress_mfma<KR, NS>), gm, dim3(256), lds, s, n3, b0, nrow, nr, \
+                     ldm, f.U, M, Mt, mbo, mbl)
+          const int ns = n3p / 16;
+          if (f.kr == 32) {
+            if (ns == 2) GP_COMPRESS_MFMA(32, 2); else if (ns == 4) GP_COMPRESS_MFMA(32, 4);
+            else if (ns == 8) GP_COMPRESS_MFMA(32, 8); else GP_COMPRESS_MFMA(32, 14);
+          } else {
+            if (ns == 2) GP_COMPRESS_MFMA(64, 2); else if (ns == 4) GP_COMPRESS_MFMA(64, 4);
+            else if (ns == 8) GP_COMPRESS_MFMA(64, 8); else GP_COMPRESS_MFMA(64, 14);
+          }
+#undef GP_COMPRESS_MFMA
+        } else
         if (f.kr == 32)
           hipLaunchKernelGGL((k_m2l_compress<32>), grid, dim3(256), 0, s, n3, n3p, b0, nrow, nr,
                              ldm, f.U, M, Mt, mbo, mbl);
@@ -2044,8 +2244,10 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   {
     size_t cnt = (size_t)f.n_boxes * n3 * ldm;
     int blocks = (int)std::min<size_t>((cnt + 255) / 256, 8192);
-    hipLaunchKernelGGL((k_fill_zero<T>), dim3(blocks), dim3(256), 0, s, Lc, cnt);
-    GP_LAUNCH_CHECK();
+    if (!proj_mfma) {  // the matrix-core expansion writes every box's locals
+      hipLaunchKernelGGL((k_fill_zero<T>), dim3(blocks), dim3(256), 0, s, Lc, cnt);
+      GP_LAUNCH_CHECK();
+    }
     if (comp) {
       cnt = (size_t)f.n_boxes * m3 * ldm;
       blocks = (int)std::min<size_t>((cnt + 255) / 256, 8192);
@@ -2147,8 +2349,21 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       ProfScope pe("m2l_expand_f32", s, 2.0 * ncol * n3 * f.kr,
                    (double)ncol * (n3 + f.kr) * sizeof(T));
       if constexpr (sizeof(T) == 4) {
-        hipLaunchKernelGGL(k_m2l_expand, dim3((ncol + 63) / 64, (n3 + 63) / 64), dim3(256), 0, s,
-                           n3, f.kr, ncol, nr, ldm, f.targets, f.U, Lt, Lc);
+        if (proj_mfma) {
+          const int ngr = (ldm + 63) / 64, nmc = (n3 + 63) / 64;
+          const int items = f.n_boxes * ngr;
+          const dim3 gm((unsigned)std::min(items / 4 + 1, 2048));
+          const size_t lds = (size_t)((n3 + 15) & ~15) * (f.kr + 4) * sizeof(float);
+          if (f.kr == 32)
+            hipLaunchKernelGGL((k_m2l_expand_mfma<32>), gm, dim3(256), lds, s, n3, nmc,
+                               f.n_boxes, ngr, nr, ldm, f.U, Lt, Lc);
+          else
+            hipLaunchKernelGGL((k_m2l_expand_mfma<64>), gm, dim3(256), lds, s, n3, nmc,
+                               f.n_boxes, ngr, nr, ldm, f.U, Lt, Lc);
+        } else {
+          hipLaunchKernelGGL(k_m2l_expand, dim3((ncol + 63) / 64, (n3 + 63) / 64), dim3(256), 0,
+                             s, n3, f.kr, ncol, nr, ldm, f.targets, f.U, Lt, Lc);
+        }
         GP_LAUNCH_CHECK();
       }
     }
