@@ -1565,6 +1565,64 @@ __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict_
   }
 }
 
+// L2P with one wavefront per leaf (FP32 locals, column groups of 4):
+// y[p][r] += sum_m S(p, m) Lc[m][r].  Lane (ph, cq) owns columns 4 cq ..
+// 4 cq + 3 and points ph, ph + 2, ... of a chunk of 16; the leaf's locals
+// stream from memory one 16-byte group per node, N node rows issued at a time,
+// and the chunk's tensor weights S sit in LDS (read as broadcasts).  The
+// weights come from the points' 1-D weights, staged first.
+template <int N>
+__global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo,
+                                                 const int *__restrict__ leaf_box, int leaf_box0,
+                                                 const float *__restrict__ w1d, int nr,
+                                                 const float *__restrict__ Lc, int ldm,
+                                                 double *__restrict__ y, int ldy) {
+  constexpr int N3 = N * N * N, PC = 16, NW = PC * 3 * N;
+  __shared__ float sW[NW];
+  __shared__ float sS[PC][N3];
+  const int li = blockIdx.x, lane = threadIdx.x, cq = lane & 31, ph = lane >> 5;
+  const int r = 4 * cq;
+  const bool rok = r < ldm;
+  const float *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * N3 * ldm + (rok ? r : 0);
+  const int pb = leaf_lo[li], pe = leaf_lo[li + 1];
+  for (int p0 = pb; p0 < pe; p0 += PC) {
+    const int np = min(PC, pe - p0);
+    __syncthreads();  // the previous chunk's weights are no longer read
+    for (int e = lane; e < NW; e += 64)
+      sW[e] = e < np * 3 * N ? w1d[(size_t)p0 * 3 * N + e] : 0.f;
+    __syncthreads();
+    for (int e = lane; e < PC * N3; e += 64) {
+      const int pp = e / N3, m = e - pp * N3;
+      const float *w = sW + pp * 3 * N;
+      sS[pp][m] = w[m / (N * N)] * w[N + (m / N) % N] * w[2 * N + m % N];
+    }
+    __syncthreads();
+    f32x4 acc[PC / 2];
+#pragma unroll
+    for (int i = 0; i < PC / 2; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (rok) {
+      for (int m = 0; m < N3; m += N) {
+        f32x4 L[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) L[c] = *reinterpret_cast<const f32x4 *>(ll + (size_t)(m + c) * ldm);
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+#pragma unroll
+          for (int i = 0; i < PC / 2; ++i) acc[i] += sS[ph + 2 * i][m + c] * L[c];
+      }
+#pragma unroll
+      for (int i = 0; i < PC / 2; ++i) {
+        const int pp = ph + 2 * i;
+        if (pp >= np) continue;
+        double *yr = y + (size_t)(p0 + pp) * ldy + r;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (r + v < nr) yr[v] += (double)acc[i][v];
+      }
+    }
+  }
+}
+
 // M2M with the order N fixed at compile time: per parent box and 32-column
 // RHS chunk, for each child the z and y contractions of every (r, a) slice
 // run in registers, one LDS exchange, then the x contraction accumulates the
@@ -2402,7 +2460,20 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   else if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf)
     hipLaunchKernelGGL((k_l2p1<T>), dim3((f.n_panels + 255) / 256), dim3(256), 0, s, n, f.n_panels,
                        f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y);
-  else if (tiled)
+  else if (tiled && sizeof(T) == 4 && n >= 3 && n <= 6) {
+    if constexpr (sizeof(T) == 4) {
+#define GP_L2P_WAVE(N)                                                                          \
+  hipLaunchKernelGGL((k_l2p_wave<N>), dim3(nleaf), dim3(64), 0, s, f.leaf_lo, f.leaf_box, leaf0, \
+                     f.w1d, nr, Lc, ldm, y, ldy)
+      switch (n) {
+        case 3: GP_L2P_WAVE(3); break;
+        case 4: GP_L2P_WAVE(4); break;
+        case 5: GP_L2P_WAVE(5); break;
+        default: GP_L2P_WAVE(6); break;
+      }
+#undef GP_L2P_WAVE
+    }
+  } else if (tiled)
     hipLaunchKernelGGL((k_l2p_tiled<T>), dim3(nleaf), dim3(256), 0, s, n, f.leaf_lo, f.leaf_box, leaf0, f.w1d,
                        nr, Lc, ldm, y, ldy);
   else

@@ -15,6 +15,8 @@
 // with 16-byte loads; k_near_spmv (one wavefront per row, x gathered per row)
 // covers tiles wider than the LDS budget.
 
+#include <type_traits>
+
 #include "gp_common.h"
 #include "gp_kernels.h"
 
@@ -183,6 +185,102 @@ __global__ __launch_bounds__(256) void k_near_spmm_rt(
   }
 }
 
+// Multi-RHS on the FP32 matrix cores (v_mfma_f32_16x16x4f32: lane holds
+// A[i = lane % 16][k = lane / 16], B[k = lane / 16][j = lane % 16],
+// D[4 (lane / 16) + v][lane % 16]).  One wavefront per (target leaf, 64-column
+// RHS chunk), four per work-group, no LDS and no barriers: D (rows x columns)
+// = tile (rows x sources) * x[cols] (sources x columns) in 16-row tiles, RTM
+// per pass and only as many as the leaf has rows (leaves average a few panels, so fixed 64-row
+// tiles would mostly multiply padding).  Per step of 16 sources, k-slot q of
+// MFMA t is source 16 step + 4 q + t; tile v holds columns r0 + 4 j + v, so
+// one 16-byte load of an x row feeds four tiles.  The source indices are
+// loaded two steps ahead (one per lane, spread by shuffles) and the x rows
+// and values one step ahead of the products.
+typedef float nf_f32x4 __attribute__((ext_vector_type(4)));
+template <int RTM>  // row tiles per pass (16 RTM rows)
+__global__ __launch_bounds__(256) void k_near_spmm_mfma(
+    int nitems, const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
+    const int *__restrict__ col_off, const int *__restrict__ cols, const float *__restrict__ vals,
+    const float *__restrict__ x, int ldx, double *__restrict__ y, int ldy, int nr) {
+  const int lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
+  const int nch = (nr + 63) >> 6;
+  const int w = xcd_block(blockIdx.x, gridDim.x, XCD_RUN_NEAR / 4) * 4 + (threadIdx.x >> 6);
+  if (w >= nitems) return;  // whole wavefront; the kernel has no barriers
+  const int T = w / nch, r0 = (w % nch) * 64;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const float *tile = vals + nf_off[T];
+  const int rc = r0 + 4 * j;
+  const bool cok = rc < ldx;
+  const int nsteps = (S + 15) >> 4;
+  for (int tm = 0; tm < nt; tm += 16 * RTM) {
+    const int rtn = min(RTM, (nt - tm + 15) >> 4);
+    nf_f32x4 acc[RTM][4];
+#pragma unroll
+    for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[rt][v] = nf_f32x4{0.f, 0.f, 0.f, 0.f};
+    auto load_col = [&](int st) {
+      const int s = 16 * st + (lane & 15);
+      return (st < nsteps && s < S) ? cols[c0 + s] : -1;
+    };
+    // x rows and values of step st; src holds that step's source indices
+    // (one per lane, lane l: source 16 st + l % 16)
+    auto load_step = [&](int st, int src, nf_f32x4 (&xb)[4], float (&av)[RTM][4]) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int sidx = __shfl(src, 4 * kq + u);
+        xb[u] = (cok && sidx >= 0) ? *reinterpret_cast<const nf_f32x4 *>(x + (int64_t)sidx * ldx + rc)
+                                   : nf_f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+      const int s = 16 * st + 4 * kq;
+#pragma unroll
+      for (int rt = 0; rt < RTM; ++rt) {
+        const int row = tm + 16 * rt + j;
+        const bool rok = rt < rtn && row < nt;
+        const float *vr = tile + (int64_t)row * S + s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) av[rt][u] = (rok && s + u < S) ? vr[u] : 0.f;
+      }
+    };
+    nf_f32x4 xb[4], xn[4];
+    float av[RTM][4], an[RTM][4];
+    int cnext = load_col(1);
+    load_step(0, load_col(0), xb, av);
+    for (int st = 0; st < nsteps; ++st) {
+      const int cnn = load_col(st + 2);
+      if (st + 1 < nsteps) load_step(st + 1, cnext, xn, an);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int rt = 0; rt < RTM; ++rt)
+          if (rt < rtn)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              acc[rt][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][u], xb[u][v], acc[rt][v], 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xb[u] = xn[u];
+#pragma unroll
+        for (int rt = 0; rt < RTM; ++rt) av[rt][u] = an[rt][u];
+      }
+      cnext = cnn;
+    }
+    // lane (j, kq) of tile (rt, v): row tm + 16 rt + 4 kq + vv, column rc + v
+#pragma unroll
+    for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv) {
+        const int t = tm + 16 * rt + 4 * kq + vv;
+        if (rt >= rtn || t >= nt) continue;
+        double *yr = y + (int64_t)(t0 + t) * ldy + rc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (rc + v < nr) yr[v] = (double)acc[rt][v][vv];
+      }
+  }
+}
+
 // 16-byte vector of values
 template <typename TV>
 struct Vec16;
@@ -286,10 +384,23 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
                            s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
     });
   } else if (nr > 16) {
-    // register tiles of 64 or 32 RHS columns, or 16-column work-groups
+    // FP32 matrix cores (64-column chunks, 16-row tiles), register tiles of
+    // 64 or 32 RHS columns, or 16-column work-groups
     const int c0 = nr > 32 ? 4 : 2;
-    tuned_launch(tname, nf.n_leaves, nr, ldx, 3, s, [&](int v) {
-      if (v == 0)
+    const bool mfma = std::is_same<TV, float>::value && std::is_same<TX, float>::value &&
+                      ldx % 4 == 0;
+    tuned_launch(tname, nf.n_leaves, nr, ldx, mfma ? 5 : 3, s, [&](int v) {
+      if (v >= 3) {  // 32 or 64 rows a pass
+        if constexpr (std::is_same<TV, float>::value && std::is_same<TX, float>::value) {
+          const int items = nf.n_leaves * ((nr + 63) / 64);
+          if (v == 3)
+            hipLaunchKernelGGL((k_near_spmm_mfma<2>), dim3((items + 3) / 4), dim3(256), 0, s, items,
+                               nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
+          else
+            hipLaunchKernelGGL((k_near_spmm_mfma<4>), dim3((items + 3) / 4), dim3(256), 0, s, items,
+                               nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
+        }
+      } else if (v == 0)
         spmm_rt(c0);
       else if (v == 1)
         spmm_rt(6 - c0);
