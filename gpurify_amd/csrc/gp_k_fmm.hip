@@ -16,6 +16,8 @@
 // source multipole and its interaction matrix through 16x16x32 BF16 (or
 // 16x16x4 FP64) matrix-core products.
 
+#include <type_traits>
+
 #include "gp_common.h"
 #include "gp_kernels.h"
 
@@ -642,9 +644,11 @@ __global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
 // accumulation: products to ~2^-16 relative (the dropped lo*lo term and the
 // pair's mantissa) instead of BF16's 2^-8, at three BF16 operations.  The
 // projected basis has mixed signs and a wide range, where plain BF16 rounding
-// slows the refinement.  Otherwise as k_m2l_bf16_pipe (one pair of
-// prefetch).
-template <int MT, int RTM>
+// slows the refinement.  Otherwise as k_m2l_bf16_pipe, with DEPTH pairs of
+// prefetch: DEPTH = 2 keeps two register sets, the source rows of pair q + 2
+// issued as soon as pair q's rows are in LDS and its G_off fragments right
+// after pair q's products, so twice the bytes are in flight per work-group.
+template <int MT, int RTM, int DEPTH = 1>
 __global__ __launch_bounds__(256) void k_m2l_bf16x3(int n3, int nr, int ldm, int xrun,
                                                     const int *__restrict__ targets,
                                                     const int *__restrict__ il_off,
@@ -673,80 +677,164 @@ __global__ __launch_bounds__(256) void k_m2l_bf16x3(int n3, int nr, int ldm, int
 #pragma unroll
     for (int j = 0; j < MTW; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const int q0 = il_off[t], q1 = il_off[t + 1];
-  bf16x8 ah[APT], al[APT];
-  bf16x8 bh[MTW][KS], bl[MTW][KS], nh[MTW][KS], nl[MTW][KS];
-  auto load_a = [&](int q) {
-    const size_t off = (size_t)il_src[q] * MS + (size_t)rt0 * 16 * NP;
-    const bf16x8 *sh = reinterpret_cast<const bf16x8 *>(Mb + off);
-    const bf16x8 *sl = reinterpret_cast<const bf16x8 *>(Mbl + off);
-#pragma unroll
-    for (int i = 0; i < APT; ++i) {
-      const int v = tid + 256 * i;
-      if (v < AV) {
-        ah[i] = sh[v];
-        al[i] = sl[v];
+  if constexpr (DEPTH == 1) {
+    bf16x8 ah[APT], al[APT];
+    bf16x8 bh[MTW][KS], bl[MTW][KS], nh[MTW][KS], nl[MTW][KS];
+    auto load_a = [&](int q) {
+      const size_t off = (size_t)il_src[q] * MS + (size_t)rt0 * 16 * NP;
+      const bf16x8 *sh = reinterpret_cast<const bf16x8 *>(Mb + off);
+      const bf16x8 *sl = reinterpret_cast<const bf16x8 *>(Mbl + off);
+  #pragma unroll
+      for (int i = 0; i < APT; ++i) {
+        const int v = tid + 256 * i;
+        if (v < AV) {
+          ah[i] = sh[v];
+          al[i] = sl[v];
+        }
       }
+    };
+    auto load_b = [&](int q, bf16x8 (&dh)[MTW][KS], bf16x8 (&dl)[MTW][KS]) {
+      const size_t off = (size_t)il_oidx[q] * NP * NP;
+  #pragma unroll
+      for (int j = 0; j < MTW; ++j) {
+        const int mt = wave * MTW + j;
+        if (mt < MT)
+  #pragma unroll
+          for (int k = 0; k < KS; ++k) {
+            const size_t e = off + (size_t)(mt * 16 + li) * NP + k * 32 + lk;
+            dh[j][k] = *(const bf16x8 *)(G + e);
+            dl[j][k] = *(const bf16x8 *)(Gl + e);
+          }
+      }
+    };
+    if (q0 < q1) {
+      load_a(q0);
+      load_b(q0, bh, bl);
     }
-  };
-  auto load_b = [&](int q, bf16x8 (&dh)[MTW][KS], bf16x8 (&dl)[MTW][KS]) {
-    const size_t off = (size_t)il_oidx[q] * NP * NP;
-#pragma unroll
-    for (int j = 0; j < MTW; ++j) {
-      const int mt = wave * MTW + j;
-      if (mt < MT)
-#pragma unroll
+    for (int q = q0; q < q1; ++q) {
+      __syncthreads();
+  #pragma unroll
+      for (int i = 0; i < APT; ++i) {
+        const int v = tid + 256 * i;
+        if (v < AV) {
+          const int o = (v / VPR) * LDA + (v % VPR) * 8;
+          *(bf16x8 *)(sA + o) = ah[i];
+          *(bf16x8 *)(sAl + o) = al[i];
+        }
+      }
+      __syncthreads();
+      if (q + 1 < q1) {
+        load_a(q + 1);
+        load_b(q + 1, nh, nl);
+      }
+  #pragma unroll
+      for (int k = 0; k < KS; ++k) {
+  #pragma unroll
+        for (int rt = 0; rt < RTM; ++rt) {
+          if (rt < RT) {
+            const int o = (rt * 16 + li) * LDA + k * 32 + lk;
+            const bf16x8 a = *(const bf16x8 *)(sA + o), a2 = *(const bf16x8 *)(sAl + o);
+  #pragma unroll
+            for (int j = 0; j < MTW; ++j)
+              if (wave * MTW + j < MT) {
+                f32x4 c = acc[rt][j];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bh[j][k], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[j][k], c, 0, 0, 0);
+                acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[j][k], c, 0, 0, 0);
+              }
+          }
+        }
+      }
+  #pragma unroll
+      for (int j = 0; j < MTW; ++j)
+  #pragma unroll
         for (int k = 0; k < KS; ++k) {
-          const size_t e = off + (size_t)(mt * 16 + li) * NP + k * 32 + lk;
-          dh[j][k] = *(const bf16x8 *)(G + e);
-          dl[j][k] = *(const bf16x8 *)(Gl + e);
+          bh[j][k] = nh[j][k];
+          bl[j][k] = nl[j][k];
         }
     }
-  };
-  if (q0 < q1) {
-    load_a(q0);
-    load_b(q0, bh, bl);
-  }
-  for (int q = q0; q < q1; ++q) {
-    __syncthreads();
+  } else {
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    bf16x8 ah[2][APT], al[2][APT];
+    bf16x8 bh[2][MTW][KS], bl[2][MTW][KS];
+    auto load_a = [&](int q, auto bc) {
+      constexpr int b = decltype(bc)::value;
+      const size_t off = (size_t)il_src[q] * MS + (size_t)rt0 * 16 * NP;
+      const bf16x8 *sh = reinterpret_cast<const bf16x8 *>(Mb + off);
+      const bf16x8 *sl = reinterpret_cast<const bf16x8 *>(Mbl + off);
 #pragma unroll
-    for (int i = 0; i < APT; ++i) {
-      const int v = tid + 256 * i;
-      if (v < AV) {
-        const int o = (v / VPR) * LDA + (v % VPR) * 8;
-        *(bf16x8 *)(sA + o) = ah[i];
-        *(bf16x8 *)(sAl + o) = al[i];
-      }
-    }
-    __syncthreads();
-    if (q + 1 < q1) {
-      load_a(q + 1);
-      load_b(q + 1, nh, nl);
-    }
-#pragma unroll
-    for (int k = 0; k < KS; ++k) {
-#pragma unroll
-      for (int rt = 0; rt < RTM; ++rt) {
-        if (rt < RT) {
-          const int o = (rt * 16 + li) * LDA + k * 32 + lk;
-          const bf16x8 a = *(const bf16x8 *)(sA + o), a2 = *(const bf16x8 *)(sAl + o);
-#pragma unroll
-          for (int j = 0; j < MTW; ++j)
-            if (wave * MTW + j < MT) {
-              f32x4 c = acc[rt][j];
-              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bh[j][k], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[j][k], c, 0, 0, 0);
-              acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[j][k], c, 0, 0, 0);
-            }
+      for (int i = 0; i < APT; ++i) {
+        const int v = tid + 256 * i;
+        if (v < AV) {
+          ah[b][i] = sh[v];
+          al[b][i] = sl[v];
         }
       }
-    }
+    };
+    auto load_b = [&](int q, auto bc) {
+      constexpr int b = decltype(bc)::value;
+      const size_t off = (size_t)il_oidx[q] * NP * NP;
 #pragma unroll
-    for (int j = 0; j < MTW; ++j)
+      for (int j = 0; j < MTW; ++j) {
+        const int mt = wave * MTW + j;
+        if (mt < MT)
+#pragma unroll
+          for (int k = 0; k < KS; ++k) {
+            const size_t e = off + (size_t)(mt * 16 + li) * NP + k * 32 + lk;
+            bh[b][j][k] = *(const bf16x8 *)(G + e);
+            bl[b][j][k] = *(const bf16x8 *)(Gl + e);
+          }
+      }
+    };
+    auto step = [&](int q, auto bc) {
+      constexpr int b = decltype(bc)::value;
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < APT; ++i) {
+        const int v = tid + 256 * i;
+        if (v < AV) {
+          const int o = (v / VPR) * LDA + (v % VPR) * 8;
+          *(bf16x8 *)(sA + o) = ah[b][i];
+          *(bf16x8 *)(sAl + o) = al[b][i];
+        }
+      }
+      __syncthreads();
+      if (q + 2 < q1) load_a(q + 2, bc);
 #pragma unroll
       for (int k = 0; k < KS; ++k) {
-        bh[j][k] = nh[j][k];
-        bl[j][k] = nl[j][k];
+#pragma unroll
+        for (int rt = 0; rt < RTM; ++rt) {
+          if (rt < RT) {
+            const int o = (rt * 16 + li) * LDA + k * 32 + lk;
+            const bf16x8 a = *(const bf16x8 *)(sA + o), a2 = *(const bf16x8 *)(sAl + o);
+#pragma unroll
+            for (int j = 0; j < MTW; ++j)
+              if (wave * MTW + j < MT) {
+                f32x4 c = acc[rt][j];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, bh[b][j][k], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[b][j][k], c, 0, 0, 0);
+                acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[b][j][k], c, 0, 0, 0);
+              }
+          }
+        }
       }
+      if (q + 2 < q1) load_b(q + 2, bc);
+    };
+    if (q0 < q1) {
+      load_a(q0, I0{});
+      load_b(q0, I0{});
+    }
+    if (q0 + 1 < q1) {
+      load_a(q0 + 1, I1{});
+      load_b(q0 + 1, I1{});
+    }
+    int q = q0;
+    for (; q + 1 < q1; q += 2) {
+      step(q, I0{});
+      step(q + 1, I1{});
+    }
+    if (q < q1) step(q, I0{});
   }
   const float ih = (float)inv_half[box_level[t]];
   float *out = Lc + (size_t)t * n3 * ldm;
@@ -1308,6 +1396,15 @@ template <int MT, int RTM>
 void launch_m2l_bf16x3(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
                        const __bf16 *Mbl, float *Lc, int xrun, hipStream_t s) {
   hipLaunchKernelGGL((k_m2l_bf16x3<MT, RTM>),
+                     dim3(f.n_targets, ((nr + 15) / 16 + RTM - 1) / RTM), dim3(256), 0, s, n3, nr,
+                     ldm, xrun, f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                     reinterpret_cast<const __bf16 *>(f.Gbf),
+                     reinterpret_cast<const __bf16 *>(f.Gcbf_lo), Mb, Mbl, Lc);
+}
+template <int MT, int RTM>
+void launch_m2l_bf16x3_d2(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
+                          const __bf16 *Mbl, float *Lc, int xrun, hipStream_t s) {
+  hipLaunchKernelGGL((k_m2l_bf16x3<MT, RTM, 2>),
                      dim3(f.n_targets, ((nr + 15) / 16 + RTM - 1) / RTM), dim3(256), 0, s, n3, nr,
                      ldm, xrun, f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
                      reinterpret_cast<const __bf16 *>(f.Gbf),
@@ -2356,7 +2453,13 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
         // the algorithm's matrix-core work: three BF16 products per projected pair
         ProfScope pl("m2l_bf16x3_mfma", s, 3.0 * fl,
                      (double)f.n_boxes * m3 * nr * 4 + f.n_targets * mbytes);
-        GP_M2L_DISPATCH(launch_m2l_bf16x3, MT, RT, fm, m3, nr, ldm, mbo, mbl, Lo, 16, s);
+        // one or two pairs of prefetch
+        tuned_launch("m2l_bf16x3", f.n_targets, MT * 100 + RT, ldm, 2, s, [&](int v) {
+          if (v == 0)
+            GP_M2L_DISPATCH(launch_m2l_bf16x3, MT, RT, fm, m3, nr, ldm, mbo, mbl, Lo, 16, s);
+          else
+            GP_M2L_DISPATCH(launch_m2l_bf16x3_d2, MT, RT, fm, m3, nr, ldm, mbo, mbl, Lo, 16, s);
+        });
       }
     } else if (want_bf16) {
       if constexpr (sizeof(T) == 4) {
