@@ -176,7 +176,7 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->rhs_count = 0;
   o->inner_passes = 1;
   o->inner_basis = GP_BASIS_FP32;
-  o->m2l_rank = 64;
+  o->m2l_rank = 32;
   o->step_graphs = 1;
 }
 

@@ -109,9 +109,9 @@ typedef struct {
   uint32_t inner_passes; /* GP_PREC_MIXED: Gram-Schmidt passes of the inner GMRES, 1 or 2 */
   uint32_t inner_basis;  /* GP_PREC_MIXED: inner Krylov basis storage, GP_BASIS_FP32 | _FP64 */
   uint32_t m2l_rank;     /* GP_PREC_MIXED inner M2L of a block of RHS: 0 = full n^3; r > 0
-                            (default 64) projects the interaction matrices on their leading
+                            (default 32) projects the interaction matrices on their leading
                             singular vectors (r rounded up to 32, 64 or 128; no projection
-                            when that reaches n^3 rounded up to 32, i.e. below order 5) and
+                            when that reaches n^3 rounded up to 32, i.e. below order 4) and
                             runs split-BF16 products (hi*hi + hi*lo + lo*hi).  Single-RHS
                             solves and the FP64 operator are never projected. */
   uint32_t step_graphs;  /* GP_PREC_MIXED: 1 (default) = from the second refinement step on,

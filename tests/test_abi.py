@@ -39,7 +39,7 @@ def test_version_and_defaults():
     assert (o.fmm_order, o.max_level, o.tol, o.restart, o.max_restarts, o.dense_limit) == \
         (4, 4, 1e-8, 200, 50, 500)
     assert o.precision == G.GP_PREC_FP64 and o.method == G.GP_METHOD_AUTO
-    assert (o.inner_passes, o.inner_basis, o.m2l_rank) == (1, G.GP_BASIS_FP32, 64)
+    assert (o.inner_passes, o.inner_basis, o.m2l_rank) == (1, G.GP_BASIS_FP32, 32)
 
 
 def test_opts_struct_size_matches_header():
@@ -51,7 +51,7 @@ def test_opts_struct_size_matches_header():
     p.canary = 0x5A5A5A5A5A5A5A5A
     G._lib.load().gp_cap_opts_default(ctypes.cast(ctypes.byref(p), ctypes.POINTER(G._lib.GpCapOpts)))
     assert p.canary == 0x5A5A5A5A5A5A5A5A
-    assert p.o.inner_basis == G.GP_BASIS_FP32 and p.o.inner_passes == 1 and p.o.m2l_rank == 64
+    assert p.o.inner_basis == G.GP_BASIS_FP32 and p.o.inner_passes == 1 and p.o.m2l_rank == 32
     assert p.o.step_graphs == 1
     # the last field is written, so the library's struct is not shorter
     text = open(os.path.join(ROOT, "include", "gpurify.h")).read()
