@@ -448,13 +448,93 @@ __device__ inline void far_entry(const double p[3], const double (*dp)[3], const
   }
 }
 
+// Sensitivity entries d/dtheta P'(p, panel l) along the ND directions: the
+// observer p = pv moves with pd (null: it stays), the source panel l moves
+// when ml >= 0 (its derivatives at index ml of the moved-panel arrays).
+// Beyond sqrt(far1) of l's diagonal the centroid rule A_l / |p - c_l|,
+// beyond sqrt(far2) the 2x2 Gauss rule (sens_far_entry, which gives 0
+// inside sqrt(far2)); inside, the dual closed form (sens_near_entry).  The
+// two live in separate kernels: the closed form in dual numbers needs far
+// more registers than the rules, and only a few entries per row take it.
+__device__ inline bool sens_is_near(int l, const double pv[3], const double *__restrict__ cen,
+                                    const double *__restrict__ gq, double far2, double far1) {
+  const double cx = cen[3L * l] - pv[0], cy = cen[3L * l + 1] - pv[1], cz = cen[3L * l + 2] - pv[2];
+  const double r2 = cx * cx + cy * cy + cz * cz, d2 = gq[20L * l + 16];
+  return !((far1 > 0.0 && r2 > far1 * d2) || (far2 > 0.0 && r2 > far2 * d2));
+}
+
+template <int ND>
+__device__ inline void sens_far_entry(int l, int ml, int nm, const double pv[3],
+                                      const double (*pd)[3], const double *__restrict__ cen,
+                                      const double *__restrict__ dcen,
+                                      const double *__restrict__ darea,
+                                      const double *__restrict__ gq,
+                                      const double *__restrict__ dgq, double far2, double far1,
+                                      double out[ND]) {
+  const double cx = cen[3L * l] - pv[0], cy = cen[3L * l + 1] - pv[1], cz = cen[3L * l + 2] - pv[2];
+  const double r2 = cx * cx + cy * cy + cz * cz;
+  const double *gl = gq + 20L * l;
+  if (far1 > 0.0 && r2 > far1 * gl[16]) {
+    const double A = gl[12] + gl[13] + gl[14] + gl[15], ir = 1.0 / sqrt(r2), air3 = A * ir * ir * ir;
+#pragma unroll
+    for (int t = 0; t < ND; ++t) {
+      double gx = pd ? pd[t][0] : 0.0, gy = pd ? pd[t][1] : 0.0, gz = pd ? pd[t][2] : 0.0, dA = 0.0;
+      if (ml >= 0) {
+        const double *dc = dcen + (3L * t) * nm + 3L * ml;
+        gx -= dc[0];
+        gy -= dc[1];
+        gz -= dc[2];
+        dA = darea[(long)t * nm + ml];
+      }
+      out[t] = dA * ir + air3 * (cx * gx + cy * gy + cz * gz);
+    }
+  } else if (far2 > 0.0 && r2 > far2 * gl[16]) {
+    far_entry<ND>(pv, pd, gl, ml >= 0 ? dgq + 16L * ml : nullptr, 16L * nm, out);
+  } else {
+#pragma unroll
+    for (int t = 0; t < ND; ++t) out[t] = 0.0;
+  }
+}
+
+template <int ND>
+__device__ inline void sens_near_entry(int l, int ml, const double pv[3], const double (*pd)[3],
+                                       const double *__restrict__ verts,
+                                       const double *__restrict__ dv, long dstride,
+                                       double out[ND]) {
+  DualN<ND> v[12], P[3];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    v[e] = DualN<ND>(verts[12L * l + e]);
+    if (ml >= 0)
+#pragma unroll
+      for (int tt = 0; tt < ND; ++tt) v[e].d[tt] = dv[tt * dstride + 12L * ml + e];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    P[d] = DualN<ND>(pv[d]);
+    if (pd)
+#pragma unroll
+      for (int tt = 0; tt < ND; ++tt) P[d].d[tt] = pd[tt][d];
+  }
+  const DualN<ND> w = wilton_t<DualN<ND>>(v, P[0], P[1], P[2]);
+#pragma unroll
+  for (int tt = 0; tt < ND; ++tt) out[tt] = w.d[tt];
+}
+
+// Both far-rule contractions are tiles of SR_ROWS observer rows against a
+// run of source panels, in sub-chunks of SR_SC sources: the entries go to
+// LDS (each thread: one source, every other row of the tile, so the
+// source's data is read once per 8 entries), then thread groups of nc
+// columns (256 / nc of them) contract slices of the sub-chunk against the
+// sources' sigma rows into SR_ROWS x ND register accumulators, so a sigma
+// row is read once per tile instead of once per row.  The groups' sums are
+// added through LDS at the end, one row at a time.
+constexpr int SR_ROWS = 16, SR_SC = 128;
+
 // Rows of the moved panels: partial[((chunk * nm + mi) * ND + t) * nc + j] =
 // sum over the panels l of this chunk (2048 a chunk) of dP'_kl sigma[l][j]
-// along direction t, k = moved[mi].  Entries whose source lies beyond
-// sqrt(far2) of its diagonals use the Gauss rule (far2 = 0: every entry
-// exact).  Each thread evaluates 8 entries into LDS; then groups of nc
-// threads (256 / nc of them) contract slices of the chunk against sigma
-// (row-contiguous reads) and the groups' sums are added.
+// along direction t, k = moved[mi] (the observer and, when l is moved, the
+// source move).  Work-group (tile of SR_ROWS moved rows, chunk).
 template <int ND>
 __global__ __launch_bounds__(256) void k_sens_rows(int N, const int *__restrict__ moved,
                                                    const int *__restrict__ mv_of,
@@ -465,90 +545,190 @@ __global__ __launch_bounds__(256) void k_sens_rows(int N, const int *__restrict_
                                                    const double *__restrict__ darea,
                                                    const double *__restrict__ gq,
                                                    const double *__restrict__ dgq, double far2,
-                                                   double far1, const double *__restrict__ sigma,
-                                                   int lds, int nc, double *__restrict__ partial) {
-  __shared__ double sdp[ND][2048];
+                                                   double far1, int nm,
+                                                   const double *__restrict__ sigma, int lds,
+                                                   int nc, double *__restrict__ partial) {
+  __shared__ double sdp[ND][SR_ROWS][SR_SC];
   __shared__ double sred[ND][256];
-  const int mi = blockIdx.x, chunk = blockIdx.y, nm = gridDim.x;
-  const int k = moved[mi], tid = threadIdx.x;
-  DualN<ND> p[3];
-  double pv[3], pd[ND][3];
-  for (int d = 0; d < 3; ++d) {
-    pv[d] = cen[3 * k + d];
-    p[d] = DualN<ND>(pv[d]);
-    for (int t = 0; t < ND; ++t) p[d].d[t] = pd[t][d] = dcen[(3L * t) * nm + 3 * mi + d];
+  __shared__ double sp[SR_ROWS][3 + 3 * ND];  // observers: centroid, ND derivatives
+  const int m0 = blockIdx.x * SR_ROWS, chunk = blockIdx.y, tid = threadIdx.x;
+  const int mr = min(SR_ROWS, nm - m0);
+  for (int e = tid; e < SR_ROWS * (3 + 3 * ND); e += 256) {
+    const int r = e / (3 + 3 * ND), c = e % (3 + 3 * ND), mi = m0 + r;
+    double x = 0.0;
+    if (r < mr) {
+      if (c < 3) x = cen[3L * moved[mi] + c];
+      else x = dcen[(3L * ((c - 3) / 3)) * nm + 3L * mi + (c - 3) % 3];
+    }
+    sp[r][c] = x;
   }
   const int l0 = chunk * 2048, nl = min(N, l0 + 2048) - l0;
-  for (int q = tid; q < 2048; q += 256) {
-    const int l = l0 + q;
-    if (q < nl) {
-      const int ml = mv_of[l];
-      const double cx = cen[3 * l] - pv[0], cy = cen[3 * l + 1] - pv[1], cz = cen[3 * l + 2] - pv[2];
-      const double r2 = cx * cx + cy * cy + cz * cz;
-      if (far1 > 0.0 && r2 > far1 * gq[20L * l + 16]) {
-        // centroid rule A_l / |p - c_l| and its derivative
-        const double *w = gq + 20L * l + 12;
-        const double A = w[0] + w[1] + w[2] + w[3], ir = 1.0 / sqrt(r2), air3 = A * ir * ir * ir;
+  const int ng = max(1, 256 / nc), g = tid / nc, j = tid % nc;
+  double acc[SR_ROWS][ND];
 #pragma unroll
-        for (int t = 0; t < ND; ++t) {
-          double gx = pd[t][0], gy = pd[t][1], gz = pd[t][2], dA = 0.0;
-          if (ml >= 0) {
-            const double *dc = dcen + (3L * t) * nm + 3 * ml;
-            gx -= dc[0];
-            gy -= dc[1];
-            gz -= dc[2];
-            dA = darea[(long)t * nm + ml];
-          }
-          sdp[t][q] = dA * ir + air3 * (cx * gx + cy * gy + cz * gz);
-        }
-      } else if (far2 > 0.0 && r2 > far2 * gq[20L * l + 16]) {
+  for (int r = 0; r < SR_ROWS; ++r)
+#pragma unroll
+    for (int t = 0; t < ND; ++t) acc[r][t] = 0.0;
+  const int q = tid % SR_SC, rb = tid / SR_SC;  // entry phase: source q, rows rb + 2 i
+  for (int s0 = 0; s0 < nl; s0 += SR_SC) {
+    const int ns = min(SR_SC, nl - s0);
+    __syncthreads();  // observers staged / previous sub-chunk contracted
+    {
+      const int l = l0 + s0 + q;
+      const int ml = q < ns ? mv_of[l] : -1;
+#pragma unroll 1
+      for (int r = rb; r < SR_ROWS; r += 256 / SR_SC) {
         double f[ND];
-        far_entry<ND>(pv, pd, gq + 20L * l, ml >= 0 ? dgq + 16L * ml : nullptr, 16L * nm, f);
+        if (q < ns && r < mr) {
+          double pd[ND][3];
 #pragma unroll
-        for (int t = 0; t < ND; ++t) sdp[t][q] = f[t];
-      } else {
-        DualN<ND> v[12];
+          for (int t = 0; t < ND; ++t)
 #pragma unroll
-        for (int e = 0; e < 12; ++e) {
-          v[e] = DualN<ND>(verts[12L * l + e]);
-          if (ml >= 0)
+            for (int d = 0; d < 3; ++d) pd[t][d] = sp[r][3 + 3 * t + d];
+          sens_far_entry<ND>(l, ml, nm, &sp[r][0], pd, cen, dcen, darea, gq, dgq, far2, far1, f);
+        } else {
 #pragma unroll
-            for (int t = 0; t < ND; ++t) v[e].d[t] = dv[t * dstride + 12L * ml + e];
+          for (int t = 0; t < ND; ++t) f[t] = 0.0;
         }
-        const DualN<ND> w = wilton_t<DualN<ND>>(v, p[0], p[1], p[2]);
 #pragma unroll
-        for (int t = 0; t < ND; ++t) sdp[t][q] = w.d[t];
+        for (int t = 0; t < ND; ++t) sdp[t][r][q] = f[t];
       }
-    } else {
+    }
+    __syncthreads();
+    if (g < ng) {
+      const int span = (ns + ng - 1) / ng, q1 = min(ns, (g + 1) * span);
+      for (int qq = g * span; qq < q1; ++qq) {
+        const double sg = sigma[(long)(l0 + s0 + qq) * lds + j];
 #pragma unroll
-      for (int t = 0; t < ND; ++t) sdp[t][q] = 0.0;
+        for (int r = 0; r < SR_ROWS; ++r)
+#pragma unroll
+          for (int t = 0; t < ND; ++t) acc[r][t] += sdp[t][r][qq] * sg;
+      }
     }
   }
-  __syncthreads();
-  const int ng = max(1, 256 / nc);          // thread groups of nc columns
-  const int g = tid / nc, j = tid % nc;
-  const int span = (nl + ng - 1) / ng;
+#pragma unroll
+  for (int r = 0; r < SR_ROWS; ++r) {
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < ND; ++t) sred[t][tid] = g < ng ? acc[r][t] : 0.0;
+    __syncthreads();
+    if (tid < ND * nc && r < mr) {
+      const int t = tid / nc, jj = tid % nc;
+      double s = 0.0;
+      for (int gg = 0; gg < ng; ++gg) s += sred[t][gg * nc + jj];
+      partial[(((long)chunk * nm + m0 + r) * ND + t) * nc + jj] = s;
+    }
+  }
+}
+
+// The closed-form entries the far-rule kernels leave out (source within
+// sqrt(far2) of its diagonal; every entry when far2 = 0), added to their
+// outputs afterwards.  Work-group: one observer row against a run of
+// sources (rows of moved panels: a chunk of 2048 panels, the partial of
+// (chunk, row); rows that stay: every moved panel, R directly).  Rounds of
+// 2048 sources, 8 tests per thread issued together; the hits are compacted
+// in a fixed order (test slot, then thread), their dual closed forms go to
+// LDS (the kernel's registers are sized for those, so one round per chunk
+// keeps the latency-bound scan short), then groups of nc threads contract
+// the hits against their sigma rows.  Deterministic.
+constexpr int SN_T = 8;  // tests per thread per round
+template <int ND>
+__global__ __launch_bounds__(256) void k_sens_near(int N, int nm, int rows_moved,
+                                                   const int *__restrict__ moved,
+                                                   const int *__restrict__ mv_of,
+                                                   const double *__restrict__ verts,
+                                                   const double *__restrict__ cen,
+                                                   const double *__restrict__ dv, long dstride,
+                                                   const double *__restrict__ dcen,
+                                                   const double *__restrict__ gq, double far2,
+                                                   double far1, const double *__restrict__ sigma,
+                                                   int lds, int nc, double *__restrict__ out) {
+  constexpr int RS = 256 * SN_T;
+  __shared__ double sdp[ND][RS];
+  __shared__ int shl[RS];
+  __shared__ int swc[SN_T][4];
+  __shared__ double sred[ND][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int k, mi = -1, s_lo, s_hi, chunk = 0;
+  if (rows_moved) {
+    mi = blockIdx.x;
+    k = moved[mi];
+    chunk = blockIdx.y;
+    s_lo = chunk * 2048;
+    s_hi = min(N, s_lo + 2048);
+  } else {
+    k = blockIdx.x;
+    if (mv_of[k] >= 0) return;  // the whole work-group: a moved row is k_sens_rows'
+    s_lo = 0;
+    s_hi = nm;
+  }
+  const double pv[3] = {cen[3L * k], cen[3L * k + 1], cen[3L * k + 2]};
+  double pd[ND][3];
+#pragma unroll
+  for (int t = 0; t < ND; ++t)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pd[t][d] = rows_moved ? dcen[(3L * t) * nm + 3L * mi + d] : 0.0;
+  const int ng = max(1, 256 / nc), g = tid / nc, j = tid % nc;
   double acc[ND];
 #pragma unroll
   for (int t = 0; t < ND; ++t) acc[t] = 0.0;
-  if (g < ng && j < nc) {
-    const int q1 = min(nl, (g + 1) * span);
-    for (int q = g * span; q < q1; ++q) {
-      const double sg = sigma[(long)(l0 + q) * lds + j];
+  for (int b0 = s_lo; b0 < s_hi; b0 += RS) {
+    int cand[SN_T];
+    unsigned hit = 0;
 #pragma unroll
-      for (int t = 0; t < ND; ++t) acc[t] += sdp[t][q] * sg;
+    for (int i = 0; i < SN_T; ++i) {
+      const int s = b0 + i * 256 + tid;
+      cand[i] = s < s_hi ? (rows_moved ? s : moved[s]) : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < SN_T; ++i)
+      if (cand[i] >= 0 && sens_is_near(cand[i], pv, cen, gq, far2, far1)) hit |= 1u << i;
+    __syncthreads();  // the previous round's hits are contracted
+#pragma unroll
+    for (int i = 0; i < SN_T; ++i) {
+      const unsigned long long m = __ballot((hit >> i) & 1u);
+      if (lane == 0) swc[i][wave] = __popcll(m);
+    }
+    __syncthreads();
+    int nh = 0;
+#pragma unroll
+    for (int i = 0; i < SN_T; ++i) {
+      const unsigned long long m = __ballot((hit >> i) & 1u);
+      int base = nh;
+      for (int w = 0; w < wave; ++w) base += swc[i][w];
+      nh += swc[i][0] + swc[i][1] + swc[i][2] + swc[i][3];
+      if ((hit >> i) & 1u) {
+        const int h = base + __popcll(m & ((1ull << lane) - 1ull));
+        const int l = cand[i], s = b0 + i * 256 + tid;
+        const int ml = rows_moved ? mv_of[l] : s;
+        double f[ND];
+        sens_near_entry<ND>(l, ml, pv, rows_moved ? pd : nullptr, verts, dv, dstride, f);
+#pragma unroll
+        for (int t = 0; t < ND; ++t) sdp[t][h] = f[t];
+        shl[h] = l;
+      }
+    }
+    __syncthreads();
+    if (g < ng) {
+      const int span = (nh + ng - 1) / ng, h1 = min(nh, (g + 1) * span);
+      for (int h = g * span; h < h1; ++h) {
+        const double sg = sigma[(long)shl[h] * lds + j];
+#pragma unroll
+        for (int t = 0; t < ND; ++t) acc[t] += sdp[t][h] * sg;
+      }
     }
   }
 #pragma unroll
-  for (int t = 0; t < ND; ++t) sred[t][tid] = acc[t];
+  for (int t = 0; t < ND; ++t) sred[t][tid] = g < ng ? acc[t] : 0.0;
   __syncthreads();
-  if (tid < nc) {
-#pragma unroll
-    for (int t = 0; t < ND; ++t) {
-      double s = 0.0;
-      for (int gg = 0; gg < ng; ++gg) s += sred[t][gg * nc + tid];
-      partial[(((long)chunk * nm + mi) * ND + t) * nc + tid] = s;
-    }
+  if (tid < ND * nc) {
+    const int t = tid / nc, jj = tid % nc;
+    double sum = 0.0;
+    for (int gg = 0; gg < ng; ++gg) sum += sred[t][gg * nc + jj];
+    if (rows_moved)
+      out[(((long)chunk * nm + mi) * ND + t) * nc + jj] += sum;
+    else
+      out[(long)t * N * nc + (long)k * nc + jj] += sum;
   }
 }
 
@@ -565,12 +745,10 @@ __global__ void k_sens_rows_reduce(int N, int nm, int nchunk, int nd, int nc,
   R[(long)t * N * nc + (long)moved[mi] * nc + j] = s;
 }
 
-// Rows of the panels that stay: R_t[k][j] = sum over moved l of dP'_kl sigma[l][j]
-// (fixed collocation point), far entries by the Gauss rule as above.  One
-// thread per row; the moved panels are staged 16 at a time in LDS with their
-// sigma rows, each thread keeps its 16 entries in registers and adds their
-// contraction to its ND * nc accumulators in LDS (dynamic: blockDim * ND * nc
-// doubles after the staging area).
+// Rows of the panels that stay: R_t[k][j] = sum over moved l of dP'_kl
+// sigma[l][j] (fixed collocation point, moving source).  Work-group: a tile
+// of SR_ROWS consecutive panels against every moved panel; rows of moved
+// panels are skipped (k_sens_rows writes them).
 template <int ND>
 __global__ __launch_bounds__(256) void k_sens_cols(int N, int nm, const int *__restrict__ moved,
                                                    const int *__restrict__ mv_of,
@@ -583,92 +761,71 @@ __global__ __launch_bounds__(256) void k_sens_cols(int N, int nm, const int *__r
                                                    const double *__restrict__ dgq, double far2,
                                                    double far1, const double *__restrict__ sigma,
                                                    int lds, int nc, double *__restrict__ R) {
-  // per staged panel: 12 vertices, 12 ND velocities, 17 Gauss values,
-  // 16 ND Gauss derivatives, 3 centroid, 1 pad, 3 ND centroid and ND area
-  // derivatives
-  constexpr int OV = 0, ODV = 12, OG = 12 * (ND + 1), ODG = OG + 17, OC = ODG + 16 * ND;
-  constexpr int ODC = OC + 4, ODA = ODC + 3 * ND;
-  constexpr int SW = ODA + ND;
-  constexpr int MQ = 16;                 // moved panels per stage
-  extern __shared__ double sm[];
-  double *sv = sm;                       // MQ x SW
-  double *ss = sm + MQ * SW;             // MQ x nc sigma rows
-  double *acc = ss + MQ * nc + (long)threadIdx.x * ND * nc;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool row = k < N && mv_of[k] < 0;
-  const double pv[3] = {row ? cen[3 * k] : 0.0, row ? cen[3 * k + 1] : 0.0,
-                        row ? cen[3 * k + 2] : 0.0};
-  const DualN<ND> px(pv[0]), py(pv[1]), pz(pv[2]);
-  for (int j = 0; j < ND * nc; ++j) acc[j] = 0.0;
-  for (int m0 = 0; m0 < nm; m0 += MQ) {
-    const int mn = min(MQ, nm - m0);
+  __shared__ double sdp[ND][SR_ROWS][SR_SC];
+  __shared__ double sred[ND][256];
+  __shared__ double sp[SR_ROWS][3];
+  __shared__ int srow[SR_ROWS];
+  const int k0 = blockIdx.x * SR_ROWS, tid = threadIdx.x;
+  if (tid < SR_ROWS) {
+    const int k = k0 + tid;
+    srow[tid] = k < N && mv_of[k] < 0;
+  }
+  for (int e = tid; e < SR_ROWS * 3; e += 256) {
+    const int r = e / 3, k = k0 + r;
+    sp[r][e % 3] = k < N ? cen[3L * k + e % 3] : 0.0;
+  }
+  const int ng = max(1, 256 / nc), g = tid / nc, j = tid % nc;
+  double acc[SR_ROWS][ND];
+#pragma unroll
+  for (int r = 0; r < SR_ROWS; ++r)
+#pragma unroll
+    for (int t = 0; t < ND; ++t) acc[r][t] = 0.0;
+  const int q = tid % SR_SC, rb = tid / SR_SC;
+  for (int s0 = 0; s0 < nm; s0 += SR_SC) {
+    const int ns = min(SR_SC, nm - s0);
     __syncthreads();
-    for (int e = threadIdx.x; e < mn * SW; e += blockDim.x) {
-      const int q = e / SW, c = e % SW, mq = m0 + q, l = moved[mq];
-      double x = 0.0;
-      if (c < ODV) x = verts[12L * l + c];
-      else if (c < OG) x = dv[((c - ODV) / 12) * dstride + 12L * mq + (c - ODV) % 12];
-      else if (c < ODG) x = gq[20L * l + (c - OG)];
-      else if (c < OC) x = dgq[((long)((c - ODG) / 16) * nm + mq) * 16 + (c - ODG) % 16];
-      else if (c < OC + 3) x = cen[3 * l + (c - OC)];
-      else if (c >= ODC && c < ODA) x = dcen[(3L * ((c - ODC) / 3)) * nm + 3 * mq + (c - ODC) % 3];
-      else if (c >= ODA) x = darea[(long)(c - ODA) * nm + mq];
-      sv[e] = x;
-    }
-    for (int e = threadIdx.x; e < mn * nc; e += blockDim.x)
-      ss[e] = sigma[(long)moved[m0 + e / nc] * lds + e % nc];
-    __syncthreads();
-    if (!row) continue;
-    double dp[ND][MQ];
-#pragma unroll
-    for (int q = 0; q < MQ; ++q) {
-      if (q < mn) {
-        const double *st = sv + q * SW;
-        const double cx = st[OC] - pv[0], cy = st[OC + 1] - pv[1], cz = st[OC + 2] - pv[2];
-        const double r2 = cx * cx + cy * cy + cz * cz;
-        if (far1 > 0.0 && r2 > far1 * st[OG + 16]) {
-          // centroid rule A_l / |p - c_l|, the source moving
-          const double A = st[OG + 12] + st[OG + 13] + st[OG + 14] + st[OG + 15];
-          const double ir = 1.0 / sqrt(r2), air3 = A * ir * ir * ir;
-#pragma unroll
-          for (int t = 0; t < ND; ++t)
-            dp[t][q] = st[ODA + t] * ir - air3 * (cx * st[ODC + 3 * t] + cy * st[ODC + 3 * t + 1] +
-                                                  cz * st[ODC + 3 * t + 2]);
-        } else if (far2 > 0.0 && r2 > far2 * st[OG + 16]) {
-          double f[ND];
-          far_entry<ND>(pv, nullptr, st + OG, st + ODG, 16, f);
-#pragma unroll
-          for (int t = 0; t < ND; ++t) dp[t][q] = f[t];
+    {
+      const int mq = s0 + q;
+      const int l = q < ns ? moved[mq] : 0;
+#pragma unroll 1
+      for (int r = rb; r < SR_ROWS; r += 256 / SR_SC) {
+        double f[ND];
+        if (q < ns && srow[r]) {
+          sens_far_entry<ND>(l, mq, nm, &sp[r][0], nullptr, cen, dcen, darea, gq, dgq, far2, far1,
+                             f);
         } else {
-          DualN<ND> v[12];
 #pragma unroll
-          for (int e = 0; e < 12; ++e) {
-            v[e] = DualN<ND>(st[OV + e]);
-#pragma unroll
-            for (int t = 0; t < ND; ++t) v[e].d[t] = st[ODV + 12 * t + e];
-          }
-          const DualN<ND> w = wilton_t<DualN<ND>>(v, px, py, pz);
-#pragma unroll
-          for (int t = 0; t < ND; ++t) dp[t][q] = w.d[t];
+          for (int t = 0; t < ND; ++t) f[t] = 0.0;
         }
-      } else {
 #pragma unroll
-        for (int t = 0; t < ND; ++t) dp[t][q] = 0.0;
+        for (int t = 0; t < ND; ++t) sdp[t][r][q] = f[t];
       }
     }
-    for (int j = 0; j < nc; ++j) {
+    __syncthreads();
+    if (g < ng) {
+      const int span = (ns + ng - 1) / ng, q1 = min(ns, (g + 1) * span);
+      for (int qq = g * span; qq < q1; ++qq) {
+        const double sg = sigma[(long)moved[s0 + qq] * lds + j];
 #pragma unroll
-      for (int t = 0; t < ND; ++t) {
-        double s = 0.0;
+        for (int r = 0; r < SR_ROWS; ++r)
 #pragma unroll
-        for (int q = 0; q < MQ; ++q) s += dp[t][q] * ss[q * nc + j];
-        acc[t * nc + j] += s;
+          for (int t = 0; t < ND; ++t) acc[r][t] += sdp[t][r][qq] * sg;
       }
     }
   }
-  if (row)
-    for (int t = 0; t < ND; ++t)
-      for (int j = 0; j < nc; ++j) R[(long)t * N * nc + (long)k * nc + j] = acc[t * nc + j];
+#pragma unroll
+  for (int r = 0; r < SR_ROWS; ++r) {
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < ND; ++t) sred[t][tid] = g < ng ? acc[r][t] : 0.0;
+    __syncthreads();
+    if (tid < ND * nc && srow[r]) {
+      const int t = tid / nc, jj = tid % nc;
+      double s = 0.0;
+      for (int gg = 0; gg < ng; ++gg) s += sred[t][gg * nc + jj];
+      R[(long)t * N * nc + (long)(k0 + r) * nc + jj] = s;
+    }
+  }
 }
 
 // J'[i][j] partial over the rows of block b:
@@ -717,20 +874,20 @@ void launch_sensitivity_nd(const SensArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((k_moved_props<ND>), dim3(g), dim3(256), 0, s, nm, a.moved, a.verts, a.dv,
                      a.dstride, a.dcen, a.darea, a.dgq);
   const int nchunk = (N + 2047) / 2048;
-  hipLaunchKernelGGL((k_sens_rows<ND>), dim3(nm, nchunk), dim3(256), 0, s, N, a.moved, a.mv_of,
-                     a.verts, a.cen, a.dv, a.dstride, a.dcen, a.darea, a.gq, a.dgq, a.far2, a.far1,
-                     a.sigma, a.lds, nc, a.scratch);
+  hipLaunchKernelGGL((k_sens_rows<ND>), dim3((nm + SR_ROWS - 1) / SR_ROWS, nchunk), dim3(256), 0, s,
+                     N, a.moved, a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.dcen, a.darea, a.gq,
+                     a.dgq, a.far2, a.far1, nm, a.sigma, a.lds, nc, a.scratch);
+  hipLaunchKernelGGL((k_sens_near<ND>), dim3(nm, nchunk), dim3(256), 0, s, N, nm, 1, a.moved,
+                     a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.dcen, a.gq, a.far2, a.far1, a.sigma,
+                     a.lds, nc, a.scratch);
   hipLaunchKernelGGL(k_sens_rows_reduce, dim3((nm * ND * nc + 255) / 256), dim3(256), 0, s, N, nm,
                      nchunk, ND, nc, a.moved, a.scratch, a.R);
-  // threads per block such that the staging area and the per-row
-  // accumulators fit 64 KiB of LDS (ND * nc <= 512: at least 16 threads)
-  int tpb = 256;
-  const size_t stage = 16 * (12 * (ND + 1) + 17 + 16 * ND + 4 + 4 * ND) + 16 * (size_t)nc;
-  while (tpb > 16 && sizeof(double) * (stage + (size_t)tpb * ND * nc) > 65536) tpb >>= 1;
-  const size_t lds_cols = sizeof(double) * (stage + (size_t)tpb * ND * nc);
-  hipLaunchKernelGGL((k_sens_cols<ND>), dim3((N + tpb - 1) / tpb), dim3(tpb), lds_cols, s, N, nm,
+  hipLaunchKernelGGL((k_sens_cols<ND>), dim3((N + SR_ROWS - 1) / SR_ROWS), dim3(256), 0, s, N, nm,
                      a.moved, a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.dcen, a.darea, a.gq,
                      a.dgq, a.far2, a.far1, a.sigma, a.lds, nc, a.R);
+  hipLaunchKernelGGL((k_sens_near<ND>), dim3(N), dim3(256), 0, s, N, nm, 0, a.moved, a.mv_of,
+                     a.verts, a.cen, a.dv, a.dstride, a.dcen, a.gq, a.far2, a.far1, a.sigma, a.lds, nc,
+                     a.R);
   const int rows = 2048, nblk = (N + rows - 1) / rows;
   for (int t = 0; t < ND; ++t) {
     hipLaunchKernelGGL(k_scatter_da, dim3(g), dim3(256), 0, s, nm, a.moved, a.darea + (long)t * nm,
