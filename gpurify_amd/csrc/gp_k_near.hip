@@ -442,15 +442,107 @@ __global__ __launch_bounds__(256) void k_near_spmm_tr(
   }
 }
 
+// Transposed near field on the FP32 matrix cores: one wavefront per (leaf
+// S, 64-column RHS chunk), D (rows of S x columns) = sum over S's near leaves
+// T of block(T, S)^T (rows of S x rows of T) * x[rows of T], in 16-row tiles
+// of S (RTM per pass, only as many as S has).  Per step of 16 rows of T,
+// k-slot q of MFMA u is T's row 16 step + 4 q + u; tile v holds columns
+// r0 + 4 j + v (one 16-byte x load feeds four tiles).  A block row (fixed t)
+// is contiguous along S's panels, so the 16 lanes of a k-slot read 16
+// consecutive values.
+template <int RTM>
+__global__ __launch_bounds__(256) void k_near_spmm_tr_mfma(
+    int nitems, const int *__restrict__ leaf_lo, const int *__restrict__ tr_off,
+    const int *__restrict__ tr_leaf, const int64_t *__restrict__ tr_base,
+    const int *__restrict__ tr_stride, const float *__restrict__ vals,
+    const float *__restrict__ x, int ldx, double *__restrict__ y, int ldy, int nr) {
+  const int lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
+  const int nch = (nr + 63) >> 6;
+  const int w = xcd_block(blockIdx.x, gridDim.x, XCD_RUN_NEAR / 4) * 4 + (threadIdx.x >> 6);
+  if (w >= nitems) return;  // whole wavefront; the kernel has no barriers
+  const int S = w / nch, r0 = (w % nch) * 64;
+  const int s0 = leaf_lo[S], ns = leaf_lo[S + 1] - s0;
+  const int rc = r0 + 4 * j;
+  const bool cok = rc < ldx;
+  const int q0 = tr_off[S], q1 = tr_off[S + 1];
+  for (int tm = 0; tm < ns; tm += 16 * RTM) {
+    const int rtn = min(RTM, (ns - tm + 15) >> 4);
+    nf_f32x4 acc[RTM][4];
+#pragma unroll
+    for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[rt][v] = nf_f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int q = q0; q < q1; ++q) {
+      const int T = tr_leaf[q], t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+      const float *blk = vals + tr_base[q];
+      const int st = tr_stride[q];
+      for (int kb = 0; kb < nt; kb += 16) {
+        nf_f32x4 xb[4];
+        float av[RTM][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int tt = kb + 4 * kq + u;
+          xb[u] = (cok && tt < nt) ? *reinterpret_cast<const nf_f32x4 *>(x + (int64_t)(t0 + tt) * ldx + rc)
+                                   : nf_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int rt = 0; rt < RTM; ++rt) {
+            const int sr = tm + 16 * rt + j;
+            av[rt][u] = (rt < rtn && sr < ns && tt < nt) ? blk[(int64_t)tt * st + sr] : 0.f;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int rt = 0; rt < RTM; ++rt)
+            if (rt < rtn)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                acc[rt][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][u], xb[u][v], acc[rt][v], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv) {
+        const int sr = tm + 16 * rt + 4 * kq + vv;
+        if (rt >= rtn || sr >= ns) continue;
+        double *yr = y + (int64_t)(s0 + sr) * ldy + rc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (rc + v < nr) yr[v] = (double)acc[rt][v][vv];
+      }
+  }
+}
+
 template <typename TV, typename TX>
 void launch_near_apply_tr_t(const NearTiles &nf, const NearTranspose &tr, const TV *vals,
                             const TX *x, int ldx, double *y, int ldy, int nr, hipStream_t s) {
   if (nf.n_leaves == 0) return;
   ProfScope ps(sizeof(TV) == 4 ? "near_tr_f32" : "near_tr_f64", s, 2.0 * nf.nnz * nr,
                (double)nf.nnz * sizeof(TV) + (double)nf.n * nr * (sizeof(TX) + sizeof(double)));
-  dim3 grid(nf.n_leaves, (nr + NF_RC - 1) / NF_RC);
-  hipLaunchKernelGGL((k_near_spmm_tr<TV, TX>), grid, dim3(256), 0, s, nf.leaf_lo, tr.off, tr.leaf,
-                     tr.base, tr.stride, vals, x, ldx, y, ldy, nr);
+  auto valu = [&] {
+    dim3 grid(nf.n_leaves, (nr + NF_RC - 1) / NF_RC);
+    hipLaunchKernelGGL((k_near_spmm_tr<TV, TX>), grid, dim3(256), 0, s, nf.leaf_lo, tr.off,
+                       tr.leaf, tr.base, tr.stride, vals, x, ldx, y, ldy, nr);
+  };
+  if constexpr (std::is_same<TV, float>::value && std::is_same<TX, float>::value) {
+    if (nr > 16 && ldx % 4 == 0) {
+      // FP32 matrix cores (32 rows of S a pass), or the VALU kernel
+      tuned_launch("near_tr_f32", nf.n_leaves, nr, ldx, 2, s, [&](int v) {
+        if (v == 0) {
+          const int items = nf.n_leaves * ((nr + 63) / 64);
+          hipLaunchKernelGGL((k_near_spmm_tr_mfma<2>), dim3((items + 3) / 4), dim3(256), 0, s,
+                             items, nf.leaf_lo, tr.off, tr.leaf, tr.base, tr.stride, vals, x, ldx,
+                             y, ldy, nr);
+        } else {
+          valu();
+        }
+      });
+      GP_LAUNCH_CHECK();
+      return;
+    }
+  }
+  valu();
   GP_LAUNCH_CHECK();
 }
 

@@ -828,6 +828,8 @@ __global__ __launch_bounds__(256) void k_sens_cols(int N, int nm, const int *__r
   }
 }
 
+constexpr int SC_ROWS = 256;  // rows per work-group of k_sens_contract
+
 // J'[i][j] partial over the rows of block b:
 //   -sum_k lambda[k][i] R[k][j] + sum_(k, cond k = i) da[k] sigma[k][j]
 // (da[k] = 0 for panels that stay), partial[b][i][j].
@@ -888,7 +890,7 @@ void launch_sensitivity_nd(const SensArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((k_sens_near<ND>), dim3(N), dim3(256), 0, s, N, nm, 0, a.moved, a.mv_of,
                      a.verts, a.cen, a.dv, a.dstride, a.dcen, a.gq, a.far2, a.far1, a.sigma, a.lds, nc,
                      a.R);
-  const int rows = 2048, nblk = (N + rows - 1) / rows;
+  const int rows = SC_ROWS, nblk = (N + rows - 1) / rows;  // enough work-groups to fill the chip
   for (int t = 0; t < ND; ++t) {
     hipLaunchKernelGGL(k_scatter_da, dim3(g), dim3(256), 0, s, nm, a.moved, a.darea + (long)t * nm,
                        a.da, 1);
@@ -928,7 +930,7 @@ void launch_gauss_table(int n, const double *verts, double *gq, hipStream_t s) {
 
 size_t sensitivity_scratch(int N, int nm_max, int nc, int nd) {
   const size_t rows = (size_t)((N + 2047) / 2048) * nm_max * nd * nc;
-  const size_t con = (size_t)((N + 2047) / 2048) * nc * nc;
+  const size_t con = (size_t)((N + SC_ROWS - 1) / SC_ROWS) * nc * nc;
   return std::max(rows, con);
 }
 
