@@ -631,6 +631,61 @@ __global__ __launch_bounds__(256) void k_sens_rows(int N, const int *__restrict_
 // LDS (the kernel's registers are sized for those, so one round per chunk
 // keeps the latency-bound scan short), then groups of nc threads contract
 // the hits against their sigma rows.  Deterministic.
+// Per chunk c of 2048 sources (source s: panel s, or moved[s] with
+// `moved`): the bounding box of the centroids and the largest squared
+// radius within which an entry takes the closed form (min of the active far
+// thresholds times the squared diagonal; +inf when neither is active), box[8 c
+// + 0..6] = min xyz, max xyz, radius^2.  An observer farther than that from
+// the box has no closed-form entry in the chunk.
+__global__ __launch_bounds__(256) void k_sens_boxes(int n_src, const int *__restrict__ moved,
+                                                    const double *__restrict__ cen,
+                                                    const double *__restrict__ gq, double far2,
+                                                    double far1, double *__restrict__ box) {
+  __shared__ double sr[7][256];
+  const int c = blockIdx.x, tid = threadIdx.x;
+  double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+  for (int s = c * 2048 + tid; s < min(n_src, (c + 1) * 2048); s += 256) {
+    const int l = moved ? moved[s] : s;
+    const double d2 = gq[20L * l + 16];
+    double r2 = INFINITY;
+    if (far1 > 0.0) r2 = fmin(r2, far1 * d2);
+    if (far2 > 0.0) r2 = fmin(r2, far2 * d2);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v[d] = fmin(v[d], cen[3L * l + d]);
+      v[3 + d] = fmax(v[3 + d], cen[3L * l + d]);
+    }
+    v[6] = fmax(v[6], r2);
+  }
+#pragma unroll
+  for (int e = 0; e < 7; ++e) sr[e][tid] = v[e];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        sr[d][tid] = fmin(sr[d][tid], sr[d][tid + h]);
+        sr[3 + d][tid] = fmax(sr[3 + d][tid], sr[3 + d][tid + h]);
+      }
+      sr[6][tid] = fmax(sr[6][tid], sr[6][tid + h]);
+    }
+    __syncthreads();
+  }
+  if (tid < 7) box[8L * c + tid] = sr[tid][0];
+}
+
+// true when the observer p is farther than the chunk's closed-form radius
+// from its box
+__device__ inline bool sens_box_far(const double *__restrict__ b, const double pv[3]) {
+  double d2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double e = fmax(fmax(b[d] - pv[d], pv[d] - b[3 + d]), 0.0);
+    d2 += e * e;
+  }
+  return d2 > b[6];
+}
+
 constexpr int SN_T = 8;  // tests per thread per round
 template <int ND>
 __global__ __launch_bounds__(256) void k_sens_near(int N, int nm, int rows_moved,
@@ -642,7 +697,8 @@ __global__ __launch_bounds__(256) void k_sens_near(int N, int nm, int rows_moved
                                                    const double *__restrict__ dcen,
                                                    const double *__restrict__ gq, double far2,
                                                    double far1, const double *__restrict__ sigma,
-                                                   int lds, int nc, double *__restrict__ out) {
+                                                   int lds, int nc, const double *__restrict__ box,
+                                                   double *__restrict__ out) {
   constexpr int RS = 256 * SN_T;
   __shared__ double sdp[ND][RS];
   __shared__ int shl[RS];
@@ -663,6 +719,7 @@ __global__ __launch_bounds__(256) void k_sens_near(int N, int nm, int rows_moved
     s_hi = nm;
   }
   const double pv[3] = {cen[3L * k], cen[3L * k + 1], cen[3L * k + 2]};
+  if (rows_moved && sens_box_far(box + 8L * chunk, pv)) return;  // no closed-form entry here
   double pd[ND][3];
 #pragma unroll
   for (int t = 0; t < ND; ++t)
@@ -673,6 +730,7 @@ __global__ __launch_bounds__(256) void k_sens_near(int N, int nm, int rows_moved
 #pragma unroll
   for (int t = 0; t < ND; ++t) acc[t] = 0.0;
   for (int b0 = s_lo; b0 < s_hi; b0 += RS) {
+    if (!rows_moved && sens_box_far(box + 8L * (b0 / RS), pv)) continue;  // uniform
     int cand[SN_T];
     unsigned hit = 0;
 #pragma unroll
@@ -876,12 +934,21 @@ void launch_sensitivity_nd(const SensArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((k_moved_props<ND>), dim3(g), dim3(256), 0, s, nm, a.moved, a.verts, a.dv,
                      a.dstride, a.dcen, a.darea, a.dgq);
   const int nchunk = (N + 2047) / 2048;
+  // chunk boxes for the closed-form scans, after the partials / contraction
+  // blocks in the scratch buffer
+  const size_t used = std::max((size_t)nchunk * nm * ND * nc,
+                               (size_t)((N + SC_ROWS - 1) / SC_ROWS) * nc * nc);
+  double *box_rows = a.scratch + used, *box_cols = box_rows + 8L * nchunk;
+  hipLaunchKernelGGL(k_sens_boxes, dim3(nchunk), dim3(256), 0, s, N, nullptr, a.cen, a.gq, a.far2,
+                     a.far1, box_rows);
+  hipLaunchKernelGGL(k_sens_boxes, dim3((nm + 2047) / 2048), dim3(256), 0, s, nm, a.moved, a.cen,
+                     a.gq, a.far2, a.far1, box_cols);
   hipLaunchKernelGGL((k_sens_rows<ND>), dim3((nm + SR_ROWS - 1) / SR_ROWS, nchunk), dim3(256), 0, s,
                      N, a.moved, a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.dcen, a.darea, a.gq,
                      a.dgq, a.far2, a.far1, nm, a.sigma, a.lds, nc, a.scratch);
   hipLaunchKernelGGL((k_sens_near<ND>), dim3(nm, nchunk), dim3(256), 0, s, N, nm, 1, a.moved,
                      a.mv_of, a.verts, a.cen, a.dv, a.dstride, a.dcen, a.gq, a.far2, a.far1, a.sigma,
-                     a.lds, nc, a.scratch);
+                     a.lds, nc, box_rows, a.scratch);
   hipLaunchKernelGGL(k_sens_rows_reduce, dim3((nm * ND * nc + 255) / 256), dim3(256), 0, s, N, nm,
                      nchunk, ND, nc, a.moved, a.scratch, a.R);
   hipLaunchKernelGGL((k_sens_cols<ND>), dim3((N + SR_ROWS - 1) / SR_ROWS), dim3(256), 0, s, N, nm,
@@ -889,7 +956,7 @@ void launch_sensitivity_nd(const SensArgs &a, hipStream_t s) {
                      a.dgq, a.far2, a.far1, a.sigma, a.lds, nc, a.R);
   hipLaunchKernelGGL((k_sens_near<ND>), dim3(N), dim3(256), 0, s, N, nm, 0, a.moved, a.mv_of,
                      a.verts, a.cen, a.dv, a.dstride, a.dcen, a.gq, a.far2, a.far1, a.sigma, a.lds, nc,
-                     a.R);
+                     box_cols, a.R);
   const int rows = SC_ROWS, nblk = (N + rows - 1) / rows;  // enough work-groups to fill the chip
   for (int t = 0; t < ND; ++t) {
     hipLaunchKernelGGL(k_scatter_da, dim3(g), dim3(256), 0, s, nm, a.moved, a.darea + (long)t * nm,
@@ -931,7 +998,8 @@ void launch_gauss_table(int n, const double *verts, double *gq, hipStream_t s) {
 size_t sensitivity_scratch(int N, int nm_max, int nc, int nd) {
   const size_t rows = (size_t)((N + 2047) / 2048) * nm_max * nd * nc;
   const size_t con = (size_t)((N + SC_ROWS - 1) / SC_ROWS) * nc * nc;
-  return std::max(rows, con);
+  const size_t boxes = 8 * (size_t)((N + 2047) / 2048 + (nm_max + 2047) / 2048);
+  return std::max(rows, con) + boxes;
 }
 
 }  // namespace gp
