@@ -10,6 +10,8 @@
 // rotations, residual estimate, iteration count) lives in small device
 // arrays updated by one thread per column.
 
+#include <type_traits>
+
 #include "gp_common.h"
 #include "gp_kernels.h"
 
@@ -55,6 +57,64 @@ __global__ __launch_bounds__(256) void k_dot_partial(const TX *__restrict__ X,
     }
 }
 
+// The same dots for an FP32 basis with ld % 4 == 0: thread (ro, cq) takes the
+// four columns 4 cq .. 4 cq + 3 (16-byte loads of the basis, two of y) of
+// rows row0 + ro, + rpp, ... with rpp = 256 / (ld / 4), two rows an
+// iteration; the column groups' sums go through LDS one column at a time.
+__global__ __launch_bounds__(256) void k_dot_partial4(const float *__restrict__ X,
+                                                      size_t strideX, int nvec,
+                                                      const double *__restrict__ Y, int N, int ld,
+                                                      double *__restrict__ partial) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  __shared__ double red[KR_VB][256];
+  const int v0 = blockIdx.y * KR_VB, b = blockIdx.x, nblk = gridDim.x;
+  const int nv = min(KR_VB, nvec - v0);
+  const int qpr = ld >> 2, rpp = 256 / qpr;
+  const int tid = threadIdx.x, cq = tid % qpr, ro = tid / qpr;
+  const int rows_per_block = (N + nblk - 1) / nblk;
+  const int row0 = b * rows_per_block, row1 = min(N, row0 + rows_per_block);
+  double acc[KR_VB][4];
+#pragma unroll
+  for (int v = 0; v < KR_VB; ++v)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[v][c] = 0.0;
+  if (ro < rpp) {
+    auto step = [&](int row) {
+      const size_t idx = (size_t)row * ld + 4 * cq;
+      const d2 ya = *reinterpret_cast<const d2 *>(Y + idx), yb = *reinterpret_cast<const d2 *>(Y + idx + 2);
+#pragma unroll
+      for (int v = 0; v < KR_VB; ++v)
+        if (v < nv) {
+          const f4 x = *reinterpret_cast<const f4 *>(X + (size_t)(v0 + v) * strideX + idx);
+          acc[v][0] += (double)x[0] * ya[0];
+          acc[v][1] += (double)x[1] * ya[1];
+          acc[v][2] += (double)x[2] * yb[0];
+          acc[v][3] += (double)x[3] * yb[1];
+        }
+    };
+    int row = row0 + ro;
+    for (; row + rpp < row1; row += 2 * rpp) {
+      step(row);
+      step(row + rpp);
+    }
+    if (row < row1) step(row);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < KR_VB; ++v) red[v][tid] = ro < rpp ? acc[v][c] : 0.0;
+    __syncthreads();
+    if (tid < qpr)
+      for (int v = 0; v < nv; ++v) {
+        double s = 0.0;
+        for (int j = 0; j < rpp; ++j) s += red[v][j * qpr + tid];
+        partial[((size_t)(v0 + v) * nblk + b) * ld + 4 * tid + c] = s;
+      }
+  }
+}
+
 // out[v * ld + r] = sum_b partial[(v * nblk + b) * ld + r]  (optionally sqrt):
 // one wavefront per output, lane l summing blocks l, l + 64, ... in order,
 // then a fixed shuffle tree, so every run gives the same bits.
@@ -85,6 +145,42 @@ __global__ __launch_bounds__(256) void k_axpy_multi(double *__restrict__ w,
     for (int v = 0; v < nvec; ++v)
       s += coef[(size_t)v * ld + r] * (double)V[(size_t)v * strideV + idx];
     w[idx] += alpha * s;
+  }
+}
+
+// The same update for an FP32 basis with ld % 4 == 0: one thread per
+// (row, group of 4 columns), 16-byte basis loads, the vectors' loads issued
+// four at a time; columns r >= nr keep their values.
+__global__ __launch_bounds__(256) void k_axpy_multi4(double *__restrict__ w,
+                                                     const float *__restrict__ V, size_t strideV,
+                                                     const double *__restrict__ coef, int nvec,
+                                                     int N, int ld, int nr, double alpha) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const int qpr = ld >> 2;
+  const size_t total = (size_t)N * qpr;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int row = (int)(e / qpr), r = 4 * (int)(e % qpr);
+    const size_t idx = (size_t)row * ld + r;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+    for (int v = 0; v < nvec; ++v) {
+      const f4 x = *reinterpret_cast<const f4 *>(V + (size_t)v * strideV + idx);
+      const d2 ca = *reinterpret_cast<const d2 *>(coef + (size_t)v * ld + r);
+      const d2 cb = *reinterpret_cast<const d2 *>(coef + (size_t)v * ld + r + 2);
+      s0 += ca[0] * (double)x[0];
+      s1 += ca[1] * (double)x[1];
+      s2 += cb[0] * (double)x[2];
+      s3 += cb[1] * (double)x[3];
+    }
+    d2 wa = *reinterpret_cast<d2 *>(w + idx), wb = *reinterpret_cast<d2 *>(w + idx + 2);
+    if (r < nr) wa[0] += alpha * s0;
+    if (r + 1 < nr) wa[1] += alpha * s1;
+    if (r + 2 < nr) wb[0] += alpha * s2;
+    if (r + 3 < nr) wb[1] += alpha * s3;
+    *reinterpret_cast<d2 *>(w + idx) = wa;
+    *reinterpret_cast<d2 *>(w + idx + 2) = wb;
   }
 }
 
@@ -478,8 +574,12 @@ void kr_dots(const TX *X, size_t strideX, int nvec, const double *Y, int N, int 
   if (nvec <= 0) return;
   ProfScope ps("gmres_dots", s, 2.0 * nvec * N * nr, (sizeof(TX) * nvec + 8.0) * N * nr);
   int nblk = std::min(KR_BLOCKS, std::max(1, (N + 255) / 256));
-  hipLaunchKernelGGL((k_dot_partial<TX>), dim3(nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0, s, X,
-                     strideX, nvec, Y, N, ld, nr, partial);
+  if (std::is_same<TX, float>::value && ld % 4 == 0 && ld >= 16 && ld <= 1024 && strideX % 4 == 0)
+    hipLaunchKernelGGL(k_dot_partial4, dim3(nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0, s,
+                       reinterpret_cast<const float *>(X), strideX, nvec, Y, N, ld, partial);
+  else
+    hipLaunchKernelGGL((k_dot_partial<TX>), dim3(nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0, s,
+                       X, strideX, nvec, Y, N, ld, nr, partial);
   GP_LAUNCH_CHECK();
   int tot = nvec * nr;
   hipLaunchKernelGGL(k_reduce_partial, dim3((tot + 3) / 4), dim3(256), 0, s, partial, nblk, nvec,
@@ -494,8 +594,12 @@ void kr_axpy_multi(double *w, const TV *V, size_t strideV, const double *coef, i
                    int N, int ld, int nr, double alpha, hipStream_t s) {
   if (nvec <= 0) return;
   ProfScope ps("gmres_axpy", s, 2.0 * nvec * N * nr, (sizeof(TV) * nvec + 16.0) * N * nr);
-  hipLaunchKernelGGL((k_axpy_multi<TV>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, w, V, strideV,
-                     coef, nvec, N, ld, nr, alpha);
+  if (std::is_same<TV, float>::value && ld % 4 == 0 && ld >= 16 && strideV % 4 == 0)
+    hipLaunchKernelGGL(k_axpy_multi4, dim3(grid_for((size_t)N * (ld / 4))), dim3(256), 0, s, w,
+                       reinterpret_cast<const float *>(V), strideV, coef, nvec, N, ld, nr, alpha);
+  else
+    hipLaunchKernelGGL((k_axpy_multi<TV>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, w, V,
+                       strideV, coef, nvec, N, ld, nr, alpha);
   GP_LAUNCH_CHECK();
 }
 
