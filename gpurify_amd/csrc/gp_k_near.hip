@@ -371,16 +371,23 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
   const char *tname = sizeof(TV) == 4 ? (nr == 1 ? "near_spmv_f32" : "near_spmm_f32")
                                       : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64");
   if (nr == 1 && ldx == 1 && ldy == 1) {
-    // leaf-staged with 4 or 8 wavefronts a leaf, or one wavefront per row
+    // leaf-staged with 4, 8, 2 or 1 wavefronts a leaf (fewer: more leaves in
+    // flight per CU for leaves of a few panels), or one wavefront per row
     const bool leaf_ok = xs_bytes <= 48 * 1024;
-    tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 3 : 1, s, [&](int v) {
+    tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 5 : 1, s, [&](int v) {
       if (!leaf_ok || v == 2)
         spmv_row();
       else if (v == 0)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 4>), dim3(nf.n_leaves), dim3(256), xs_bytes,
                            s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
-      else
+      else if (v == 1)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 8>), dim3(nf.n_leaves), dim3(512), xs_bytes,
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+      else if (v == 3)
+        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 2>), dim3(nf.n_leaves), dim3(128), xs_bytes,
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+      else
+        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 1>), dim3(nf.n_leaves), dim3(64), xs_bytes,
                            s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
     });
   } else if (nr > 16) {
