@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_near_spmm_rt(
 // loaded two steps ahead (one per lane, spread by shuffles) and the x rows
 // and values one step ahead of the products.
 typedef float nf_f32x4 __attribute__((ext_vector_type(4)));
-template <int RTM>  // row tiles per pass (16 RTM rows)
+template <int RTM, int DEPTH = 1>  // row tiles per pass (16 RTM rows), steps of prefetch
 __global__ __launch_bounds__(256) void k_near_spmm_mfma(
     int nitems, const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
     const int *__restrict__ col_off, const int *__restrict__ cols, const float *__restrict__ vals,
@@ -243,13 +243,7 @@ __global__ __launch_bounds__(256) void k_near_spmm_mfma(
         for (int u = 0; u < 4; ++u) av[rt][u] = (rok && s + u < S) ? vr[u] : 0.f;
       }
     };
-    nf_f32x4 xb[4], xn[4];
-    float av[RTM][4], an[RTM][4];
-    int cnext = load_col(1);
-    load_step(0, load_col(0), xb, av);
-    for (int st = 0; st < nsteps; ++st) {
-      const int cnn = load_col(st + 2);
-      if (st + 1 < nsteps) load_step(st + 1, cnext, xn, an);
+    auto mma = [&](const nf_f32x4 (&xb)[4], const float (&av)[RTM][4]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -258,13 +252,43 @@ __global__ __launch_bounds__(256) void k_near_spmm_mfma(
 #pragma unroll
             for (int v = 0; v < 4; ++v)
               acc[rt][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][u], xb[u][v], acc[rt][v], 0, 0, 0);
+    };
+    if constexpr (DEPTH == 1) {
+      nf_f32x4 xb[4], xn[4];
+      float av[RTM][4], an[RTM][4];
+      int cnext = load_col(1);
+      load_step(0, load_col(0), xb, av);
+      for (int st = 0; st < nsteps; ++st) {
+        const int cnn = load_col(st + 2);
+        if (st + 1 < nsteps) load_step(st + 1, cnext, xn, an);
+        mma(xb, av);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        xb[u] = xn[u];
+        for (int u = 0; u < 4; ++u) {
+          xb[u] = xn[u];
 #pragma unroll
-        for (int rt = 0; rt < RTM; ++rt) av[rt][u] = an[rt][u];
+          for (int rt = 0; rt < RTM; ++rt) av[rt][u] = an[rt][u];
+        }
+        cnext = cnn;
       }
-      cnext = cnn;
+    } else {
+      // two register sets: the x rows and values of step st + 2 load into
+      // the set step st has just used; source indices three steps ahead
+      nf_f32x4 x0[4], x1[4];
+      float a0[RTM][4], a1[RTM][4];
+      int c2 = load_col(2);
+      load_step(0, load_col(0), x0, a0);
+      if (1 < nsteps) load_step(1, load_col(1), x1, a1);
+      int st = 0;
+      for (; st + 1 < nsteps; st += 2) {
+        const int c3 = load_col(st + 3);
+        mma(x0, a0);
+        if (st + 2 < nsteps) load_step(st + 2, c2, x0, a0);
+        const int c4 = load_col(st + 4);
+        mma(x1, a1);
+        if (st + 3 < nsteps) load_step(st + 3, c3, x1, a1);
+        c2 = c4;
+      }
+      if (st < nsteps) mma(x0, a0);
     }
     // lane (j, kq) of tile (rt, v): row tm + 16 rt + 4 kq + vv, column rc + v
 #pragma unroll
@@ -396,16 +420,20 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
     const int c0 = nr > 32 ? 4 : 2;
     const bool mfma = std::is_same<TV, float>::value && std::is_same<TX, float>::value &&
                       ldx % 4 == 0;
-    tuned_launch(tname, nf.n_leaves, nr, ldx, mfma ? 5 : 3, s, [&](int v) {
-      if (v >= 3) {  // 32 or 64 rows a pass
+    tuned_launch(tname, nf.n_leaves, nr, ldx, mfma ? 6 : 3, s, [&](int v) {
+      if (v >= 3) {  // 32 or 64 rows a pass; 32 rows with two steps of prefetch
         if constexpr (std::is_same<TV, float>::value && std::is_same<TX, float>::value) {
           const int items = nf.n_leaves * ((nr + 63) / 64);
           if (v == 3)
             hipLaunchKernelGGL((k_near_spmm_mfma<2>), dim3((items + 3) / 4), dim3(256), 0, s, items,
                                nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
-          else
+          else if (v == 4)
             hipLaunchKernelGGL((k_near_spmm_mfma<4>), dim3((items + 3) / 4), dim3(256), 0, s, items,
                                nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, ldx, y, ldy, nr);
+          else
+            hipLaunchKernelGGL((k_near_spmm_mfma<2, 2>), dim3((items + 3) / 4), dim3(256), 0, s,
+                               items, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, ldx, y,
+                               ldy, nr);
         }
       } else if (v == 0)
         spmm_rt(c0);
