@@ -532,6 +532,39 @@ __global__ __launch_bounds__(256) void k_l2p1(int n, int N, const int *__restric
   y[p] += (double)acc;
 }
 
+// Single right-hand side L2P, order 4, FP32: one thread per panel, its
+// leaf's 64 local coefficients as 16 aligned 16-byte loads (the panels of a
+// leaf are neighbouring threads and share those lines), the tensor-product
+// sum sum_a wx[a] sum_b wy[b] sum_c wz[c] L[a][b][c] in registers.
+__global__ __launch_bounds__(256) void k_l2p1v4(int N, const int *__restrict__ panel_leaf,
+                                                const int *__restrict__ leaf_box, int leaf_box0,
+                                                const float *__restrict__ w1d,
+                                                const float *__restrict__ Lc,
+                                                double *__restrict__ y) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int li = panel_leaf[p];
+  const f4 *l4 = reinterpret_cast<const f4 *>(Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * 64);
+  const f4 *w4 = reinterpret_cast<const f4 *>(w1d + (size_t)p * 12);
+  f4 L[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) L[e] = l4[e];
+  const f4 wx = w4[0], wy = w4[1], wz = w4[2];
+  float acc = 0.f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    float sa = 0.f;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const f4 l = L[a * 4 + b];
+      sa += wy[b] * (wz[0] * l[0] + wz[1] * l[1] + wz[2] * l[2] + wz[3] * l[3]);
+    }
+    acc += wx[a] * sa;
+  }
+  y[p] += (double)acc;
+}
+
 // Single right-hand side M2M: 256 / tpb parent boxes per work-group, one
 // thread per (parent, node).  All children of a parent are loaded into its
 // LDS slot at once (8 loads in flight), then the three 1-D contractions (z,
@@ -2557,7 +2590,11 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                (double)f.n_panels * nr * 16.0 + nleaf * exp_bytes);
   // (L2P accumulates into y, so it is not a tuner candidate: the tuner
   // launches every variant on the same outputs)
-  if (ldm == 1 && nr == 1 && ldy == 1 && n3 <= 64)
+  if (ldm == 1 && nr == 1 && ldy == 1 && n == 4 && sizeof(T) == 4 && f.panel_leaf) {
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL(k_l2p1v4, dim3((f.n_panels + 255) / 256), dim3(256), 0, s, f.n_panels,
+                         f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y);
+  } else if (ldm == 1 && nr == 1 && ldy == 1 && n3 <= 64)
     hipLaunchKernelGGL((k_l2p1w<T>), dim3((nleaf + 3) / 4), dim3(256), 0, s, n, nleaf, f.leaf_lo,
                        f.leaf_box, leaf0, f.w1d, Lc, y);
   else if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf)
