@@ -1861,6 +1861,18 @@ __global__ __launch_bounds__(256) void k_l2l_tiled(int box0, const int *__restri
   const T *mx = m2m1d + (o * 3 + 0) * NN, *my = m2m1d + (o * 3 + 1) * NN,
           *mz = m2m1d + (o * 3 + 2) * NN;
   const T *lp = Lc + (size_t)p * N3 * ldm;
+  T *lc = Lc + (size_t)ch * N3 * ldm;
+  // the child's current locals, read before the parent's passes so the
+  // loads overlap them (phase 2 adds to these)
+  T cur[ITEMS][N];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int item = tid + 256 * k;
+    const int r = item % rc, bc = item / rc;
+#pragma unroll
+    for (int ap = 0; ap < N; ++ap)
+      cur[k][ap] = item < rc * NN ? lc[(size_t)(ap * NN + bc) * ldm + r0 + r] : (T)0;
+  }
   if (tid < rc * N) {
     const int r = tid % rc, a = tid / rc;
     T in[NN], u[NN];
@@ -1889,7 +1901,6 @@ __global__ __launch_bounds__(256) void k_l2l_tiled(int box0, const int *__restri
       }
   }
   __syncthreads();
-  T *lc = Lc + (size_t)ch * N3 * ldm;
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const int item = tid + 256 * k;
@@ -1903,7 +1914,7 @@ __global__ __launch_bounds__(256) void k_l2l_tiled(int box0, const int *__restri
         T s = 0;
 #pragma unroll
         for (int a = 0; a < N; ++a) s += mx[a * N + ap] * tv[a];
-        lc[(size_t)(ap * NN + bc) * ldm + r0 + r] += s;
+        lc[(size_t)(ap * NN + bc) * ldm + r0 + r] = cur[k][ap] + s;
       }
     }
   }
