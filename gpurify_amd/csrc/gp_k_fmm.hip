@@ -1696,12 +1696,14 @@ __global__ __launch_bounds__(256) void k_l2p_tiled(int n, const int *__restrict_
 }
 
 // L2P with one wavefront per leaf (FP32 locals, column groups of 4):
-// y[p][r] += sum_m S(p, m) Lc[m][r].  Lane (ph, cq) owns columns 4 cq ..
-// 4 cq + 3 and points ph, ph + 2, ... of a chunk of 16; the leaf's locals
-// stream from memory one 16-byte group per node, N node rows issued at a time,
-// and the chunk's tensor weights S sit in LDS (read as broadcasts).  The
-// weights come from the points' 1-D weights, staged first.
-template <int N>
+// y[p][r] += sum_m S(p, m) Lc[m][r].  The wavefront's lanes are qn = ld / 4
+// column groups x pg = 64 / qn point groups: lane (ph, cq) owns columns
+// 4 cq .. 4 cq + 3 and points ph, ph + pg, ... of a chunk of 16 (32 columns:
+// 8 groups x 8 point groups, so no lane idles for narrow blocks); the leaf's
+// locals stream from memory one 16-byte group per node, N node rows issued
+// at a time, and the chunk's tensor weights sit in LDS (read as
+// broadcasts).  The weights come from the points' 1-D weights, staged first.
+template <int N, int PPL>  // PPL = ceil(16 / pg) points per lane and chunk
 __global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo,
                                                  const int *__restrict__ leaf_box, int leaf_box0,
                                                  const float *__restrict__ w1d, int nr,
@@ -1710,10 +1712,12 @@ __global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo
   constexpr int N3 = N * N * N, PC = 16, NW = PC * 3 * N;
   __shared__ float sW[NW];
   __shared__ float sS[PC][N3];
-  const int li = blockIdx.x, lane = threadIdx.x, cq = lane & 31, ph = lane >> 5;
+  const int li = blockIdx.x, lane = threadIdx.x;
+  const int qn = ldm >> 2, pg = 64 / qn;  // caller: ldm % 4 == 0, 16 <= ldm <= 128
+  const int cq = lane % qn, ph = lane / qn;
   const int r = 4 * cq;
-  const bool rok = r < ldm;
-  const float *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * N3 * ldm + (rok ? r : 0);
+  const bool rok = ph < pg;
+  const float *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * N3 * ldm + r;
   const int pb = leaf_lo[li], pe = leaf_lo[li + 1];
   for (int p0 = pb; p0 < pe; p0 += PC) {
     const int np = min(PC, pe - p0);
@@ -1727,10 +1731,15 @@ __global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo
       sS[pp][m] = w[m / (N * N)] * w[N + (m / N) % N] * w[2 * N + m % N];
     }
     __syncthreads();
-    f32x4 acc[PC / 2];
+    f32x4 acc[PPL];
+    const float *srow[PPL];  // the lane's weight rows (clamped; extra rows are never stored)
 #pragma unroll
-    for (int i = 0; i < PC / 2; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < PPL; ++i) {
+      acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+      srow[i] = &sS[min(ph + pg * i, PC - 1)][0];
+    }
     if (rok) {
+#pragma unroll 1
       for (int m = 0; m < N3; m += N) {
         f32x4 L[N];
 #pragma unroll
@@ -1738,11 +1747,11 @@ __global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo
 #pragma unroll
         for (int c = 0; c < N; ++c)
 #pragma unroll
-          for (int i = 0; i < PC / 2; ++i) acc[i] += sS[ph + 2 * i][m + c] * L[c];
+          for (int i = 0; i < PPL; ++i) acc[i] += srow[i][m + c] * L[c];
       }
 #pragma unroll
-      for (int i = 0; i < PC / 2; ++i) {
-        const int pp = ph + 2 * i;
+      for (int i = 0; i < PPL; ++i) {
+        const int pp = ph + pg * i;
         if (pp >= np) continue;
         double *yr = y + (size_t)(p0 + pp) * ldy + r;
 #pragma unroll
@@ -2611,17 +2620,27 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   else if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf)
     hipLaunchKernelGGL((k_l2p1<T>), dim3((f.n_panels + 255) / 256), dim3(256), 0, s, n, f.n_panels,
                        f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y);
-  else if (tiled && sizeof(T) == 4 && n >= 3 && n <= 6) {
+  else if (tiled && sizeof(T) == 4 && n >= 3 && n <= 6 && ldm % 4 == 0 && ldm >= 16) {
     if constexpr (sizeof(T) == 4) {
-#define GP_L2P_WAVE(N)                                                                          \
-  hipLaunchKernelGGL((k_l2p_wave<N>), dim3(nleaf), dim3(64), 0, s, f.leaf_lo, f.leaf_box, leaf0, \
+      // points per lane: 16 / (64 / (ldm / 4)) rounded up
+      const int pg = 64 / (ldm / 4), ppl = (16 + pg - 1) / pg;
+#define GP_L2P_WAVE(N, P)                                                                         \
+  hipLaunchKernelGGL((k_l2p_wave<N, P>), dim3(nleaf), dim3(64), 0, s, f.leaf_lo, f.leaf_box, leaf0, \
                      f.w1d, nr, Lc, ldm, y, ldy)
+#define GP_L2P_WAVE_N(N)                      \
+  do {                                        \
+    if (ppl <= 1) GP_L2P_WAVE(N, 1);          \
+    else if (ppl <= 2) GP_L2P_WAVE(N, 2);     \
+    else if (ppl <= 4) GP_L2P_WAVE(N, 4);     \
+    else GP_L2P_WAVE(N, 8);                   \
+  } while (0)
       switch (n) {
-        case 3: GP_L2P_WAVE(3); break;
-        case 4: GP_L2P_WAVE(4); break;
-        case 5: GP_L2P_WAVE(5); break;
-        default: GP_L2P_WAVE(6); break;
+        case 3: GP_L2P_WAVE_N(3); break;
+        case 4: GP_L2P_WAVE_N(4); break;
+        case 5: GP_L2P_WAVE_N(5); break;
+        default: GP_L2P_WAVE_N(6); break;
       }
+#undef GP_L2P_WAVE_N
 #undef GP_L2P_WAVE
     }
   } else if (tiled)
