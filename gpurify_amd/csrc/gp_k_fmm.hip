@@ -1591,6 +1591,7 @@ __global__ __launch_bounds__(256) void k_p2m_tiled(int n, int n3p, const int *__
                                                    int ldm) {
   __shared__ T sS[TP_KP][16 * MB];
   __shared__ T sQ[TP_KP][128];
+  __shared__ T sW[TP_KP * 18];  // the stage's 1-D weights (3 n <= 18 per point)
   const int li = blockIdx.x, tid = threadIdx.x;
   const int mg = tid & 15, rg = tid >> 4;
   const int RB = ldm >> 4;
@@ -1603,9 +1604,11 @@ __global__ __launch_bounds__(256) void k_p2m_tiled(int n, int n3p, const int *__
   const int pb = leaf_lo[li], pe = leaf_lo[li + 1];
   for (int p0 = pb; p0 < pe; p0 += TP_KP) {
     const int np = min(TP_KP, pe - p0);
+    for (int e = tid; e < np * 3 * n; e += 256) sW[e] = w1d[(size_t)p0 * 3 * n + e];
+    __syncthreads();
     for (int e = tid; e < TP_KP * NP; e += 256) {
       int pp = e / NP, m = e - pp * NP;
-      sS[pp][m] = (pp < np && m < n3) ? tensor_weight(w1d + (size_t)(p0 + pp) * 3 * n, n, m) : (T)0;
+      sS[pp][m] = (pp < np && m < n3) ? tensor_weight(sW + pp * 3 * n, n, m) : (T)0;
     }
     for (int e = tid; e < TP_KP * ldm; e += 256) {
       int pp = e / ldm, r = e - pp * ldm;
