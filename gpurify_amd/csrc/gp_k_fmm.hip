@@ -678,9 +678,10 @@ __global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
 // pair's mantissa) instead of BF16's 2^-8, at three BF16 operations.  The
 // projected basis has mixed signs and a wide range, where plain BF16 rounding
 // slows the refinement.  Otherwise as k_m2l_bf16_pipe, with DEPTH pairs of
-// prefetch: DEPTH = 2 keeps two register sets, the source rows of pair q + 2
-// issued as soon as pair q's rows are in LDS and its G_off fragments right
-// after pair q's products, so twice the bytes are in flight per work-group.
+// prefetch: DEPTH = 2 or 4 keeps that many register sets, the source rows of
+// pair q + DEPTH issued as soon as pair q's rows are in LDS and its G_off
+// fragments right after pair q's products, so DEPTH times the bytes are in
+// flight per work-group.
 template <int MT, int RTM, int DEPTH = 1>
 __global__ __launch_bounds__(256) void k_m2l_bf16x3(int n3, int nr, int ldm, int xrun,
                                                     const int *__restrict__ targets,
@@ -789,8 +790,10 @@ __global__ __launch_bounds__(256) void k_m2l_bf16x3(int n3, int nr, int ldm, int
   } else {
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
-    bf16x8 ah[2][APT], al[2][APT];
-    bf16x8 bh[2][MTW][KS], bl[2][MTW][KS];
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    bf16x8 ah[DEPTH][APT], al[DEPTH][APT];
+    bf16x8 bh[DEPTH][MTW][KS], bl[DEPTH][MTW][KS];
     auto load_a = [&](int q, auto bc) {
       constexpr int b = decltype(bc)::value;
       const size_t off = (size_t)il_src[q] * MS + (size_t)rt0 * 16 * NP;
@@ -833,7 +836,7 @@ __global__ __launch_bounds__(256) void k_m2l_bf16x3(int n3, int nr, int ldm, int
         }
       }
       __syncthreads();
-      if (q + 2 < q1) load_a(q + 2, bc);
+      if (q + DEPTH < q1) load_a(q + DEPTH, bc);
 #pragma unroll
       for (int k = 0; k < KS; ++k) {
 #pragma unroll
@@ -852,22 +855,34 @@ __global__ __launch_bounds__(256) void k_m2l_bf16x3(int n3, int nr, int ldm, int
           }
         }
       }
-      if (q + 2 < q1) load_b(q + 2, bc);
+      if (q + DEPTH < q1) load_b(q + DEPTH, bc);
     };
-    if (q0 < q1) {
-      load_a(q0, I0{});
-      load_b(q0, I0{});
-    }
-    if (q0 + 1 < q1) {
-      load_a(q0 + 1, I1{});
-      load_b(q0 + 1, I1{});
+    auto prime = [&](int q, auto bc) {
+      if (q < q1) {
+        load_a(q, bc);
+        load_b(q, bc);
+      }
+    };
+    prime(q0, I0{});
+    prime(q0 + 1, I1{});
+    if constexpr (DEPTH == 4) {
+      prime(q0 + 2, I2{});
+      prime(q0 + 3, I3{});
     }
     int q = q0;
-    for (; q + 1 < q1; q += 2) {
+    for (; q + DEPTH - 1 < q1; q += DEPTH) {
       step(q, I0{});
       step(q + 1, I1{});
+      if constexpr (DEPTH == 4) {
+        step(q + 2, I2{});
+        step(q + 3, I3{});
+      }
     }
+    // the last pairs, in the register sets they were loaded into
     if (q < q1) step(q, I0{});
+    if (q + 1 < q1) step(q + 1, I1{});
+    if constexpr (DEPTH == 4)
+      if (q + 2 < q1) step(q + 2, I2{});
   }
   const float ih = (float)inv_half[box_level[t]];
   float *out = Lc + (size_t)t * n3 * ldm;
@@ -1438,6 +1453,15 @@ template <int MT, int RTM>
 void launch_m2l_bf16x3_d2(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
                           const __bf16 *Mbl, float *Lc, int xrun, hipStream_t s) {
   hipLaunchKernelGGL((k_m2l_bf16x3<MT, RTM, 2>),
+                     dim3(f.n_targets, ((nr + 15) / 16 + RTM - 1) / RTM), dim3(256), 0, s, n3, nr,
+                     ldm, xrun, f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
+                     reinterpret_cast<const __bf16 *>(f.Gbf),
+                     reinterpret_cast<const __bf16 *>(f.Gcbf_lo), Mb, Mbl, Lc);
+}
+template <int MT, int RTM>
+void launch_m2l_bf16x3_d4(const FmmDev<float> &f, int n3, int nr, int ldm, const __bf16 *Mb,
+                          const __bf16 *Mbl, float *Lc, int xrun, hipStream_t s) {
+  hipLaunchKernelGGL((k_m2l_bf16x3<MT, RTM, 4>),
                      dim3(f.n_targets, ((nr + 15) / 16 + RTM - 1) / RTM), dim3(256), 0, s, n3, nr,
                      ldm, xrun, f.targets, f.il_off, f.il_src, f.il_oidx, f.box_level, f.inv_half,
                      reinterpret_cast<const __bf16 *>(f.Gbf),
@@ -2509,12 +2533,15 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
         // the algorithm's matrix-core work: three BF16 products per projected pair
         ProfScope pl("m2l_bf16x3_mfma", s, 3.0 * fl,
                      (double)f.n_boxes * m3 * nr * 4 + f.n_targets * mbytes);
-        // one or two pairs of prefetch
-        tuned_launch("m2l_bf16x3", f.n_targets, MT * 100 + RT, ldm, 2, s, [&](int v) {
+        // one, two or four pairs of prefetch (four: rank 32 and 64 only, the
+        // register sets of larger tables do not fit)
+        tuned_launch("m2l_bf16x3", f.n_targets, MT * 100 + RT, ldm, MT <= 4 ? 3 : 2, s, [&](int v) {
           if (v == 0)
             GP_M2L_DISPATCH(launch_m2l_bf16x3, MT, RT, fm, m3, nr, ldm, mbo, mbl, Lo, 16, s);
-          else
+          else if (v == 1)
             GP_M2L_DISPATCH(launch_m2l_bf16x3_d2, MT, RT, fm, m3, nr, ldm, mbo, mbl, Lo, 16, s);
+          else
+            GP_M2L_DISPATCH(launch_m2l_bf16x3_d4, MT, RT, fm, m3, nr, ldm, mbo, mbl, Lo, 16, s);
         });
       }
     } else if (want_bf16) {
