@@ -1766,7 +1766,9 @@ __global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo
       srow[i] = &sS[min(ph + pg * i, PC - 1)][0];
     }
     if (rok) {
-#pragma unroll 1
+      // two node groups in flight for narrow blocks (few points per lane)
+      constexpr int UNR = PPL <= 4 ? 2 : 1;
+#pragma unroll UNR
       for (int m = 0; m < N3; m += N) {
         f32x4 L[N];
 #pragma unroll
