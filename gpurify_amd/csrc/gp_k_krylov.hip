@@ -148,32 +148,45 @@ __global__ __launch_bounds__(256) void k_axpy_multi(double *__restrict__ w,
   }
 }
 
-// The same update for an FP32 basis with ld % 4 == 0: one thread per
-// (row, group of 4 columns), 16-byte basis loads, the vectors' loads issued
-// four at a time; columns r >= nr keep their values.
+// The same update for an FP32 basis with ld % 4 == 0 and ld <= 128: one
+// thread per (row, group of 4 columns), 16-byte basis loads issued four
+// vectors at a time; the coefficients go through LDS 16 vectors at a time
+// (one 16-byte basis load per vector and thread instead of three loads);
+// columns r >= nr keep their values.
 __global__ __launch_bounds__(256) void k_axpy_multi4(double *__restrict__ w,
                                                      const float *__restrict__ V, size_t strideV,
                                                      const double *__restrict__ coef, int nvec,
                                                      int N, int ld, int nr, double alpha) {
   typedef float f4 __attribute__((ext_vector_type(4)));
   typedef double d2 __attribute__((ext_vector_type(2)));
+  constexpr int VC = 16;
+  __shared__ __attribute__((aligned(16))) double sc[VC * 128];
   const int qpr = ld >> 2;
   const size_t total = (size_t)N * qpr;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int row = (int)(e / qpr), r = 4 * (int)(e % qpr);
-    const size_t idx = (size_t)row * ld + r;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = e < total;
+  const int row = ok ? (int)(e / qpr) : 0, r = ok ? 4 * (int)(e % qpr) : 0;
+  const size_t idx = (size_t)row * ld + r;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int v0 = 0; v0 < nvec; v0 += VC) {
+    const int vn = min(VC, nvec - v0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < vn * ld; i += 256) sc[i] = coef[(size_t)v0 * ld + i];
+    __syncthreads();
+    if (ok) {
 #pragma unroll 4
-    for (int v = 0; v < nvec; ++v) {
-      const f4 x = *reinterpret_cast<const f4 *>(V + (size_t)v * strideV + idx);
-      const d2 ca = *reinterpret_cast<const d2 *>(coef + (size_t)v * ld + r);
-      const d2 cb = *reinterpret_cast<const d2 *>(coef + (size_t)v * ld + r + 2);
-      s0 += ca[0] * (double)x[0];
-      s1 += ca[1] * (double)x[1];
-      s2 += cb[0] * (double)x[2];
-      s3 += cb[1] * (double)x[3];
+      for (int u = 0; u < vn; ++u) {
+        const f4 x = *reinterpret_cast<const f4 *>(V + (size_t)(v0 + u) * strideV + idx);
+        const d2 ca = *reinterpret_cast<const d2 *>(sc + u * ld + r);
+        const d2 cb = *reinterpret_cast<const d2 *>(sc + u * ld + r + 2);
+        s0 += ca[0] * (double)x[0];
+        s1 += ca[1] * (double)x[1];
+        s2 += cb[0] * (double)x[2];
+        s3 += cb[1] * (double)x[3];
+      }
     }
+  }
+  if (ok) {
     d2 wa = *reinterpret_cast<d2 *>(w + idx), wb = *reinterpret_cast<d2 *>(w + idx + 2);
     if (r < nr) wa[0] += alpha * s0;
     if (r + 1 < nr) wa[1] += alpha * s1;
@@ -594,9 +607,10 @@ void kr_axpy_multi(double *w, const TV *V, size_t strideV, const double *coef, i
                    int N, int ld, int nr, double alpha, hipStream_t s) {
   if (nvec <= 0) return;
   ProfScope ps("gmres_axpy", s, 2.0 * nvec * N * nr, (sizeof(TV) * nvec + 16.0) * N * nr);
-  if (std::is_same<TV, float>::value && ld % 4 == 0 && ld >= 16 && strideV % 4 == 0)
-    hipLaunchKernelGGL(k_axpy_multi4, dim3(grid_for((size_t)N * (ld / 4))), dim3(256), 0, s, w,
-                       reinterpret_cast<const float *>(V), strideV, coef, nvec, N, ld, nr, alpha);
+  if (std::is_same<TV, float>::value && ld % 4 == 0 && ld >= 16 && ld <= 128 && strideV % 4 == 0)
+    hipLaunchKernelGGL(k_axpy_multi4, dim3((unsigned)(((size_t)N * (ld / 4) + 255) / 256)), dim3(256),
+                       0, s, w, reinterpret_cast<const float *>(V), strideV, coef, nvec, N, ld, nr,
+                       alpha);
   else
     hipLaunchKernelGGL((k_axpy_multi<TV>), dim3(grid_for((size_t)N * nr)), dim3(256), 0, s, w, V,
                        strideV, coef, nvec, N, ld, nr, alpha);
