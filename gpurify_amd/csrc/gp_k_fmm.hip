@@ -570,13 +570,14 @@ __global__ __launch_bounds__(256) void k_l2p1v4(int N, const int *__restrict__ p
 // LDS slot at once (8 loads in flight), then the three 1-D contractions (z,
 // then y, then x, as k_m2m) run for every child between three barriers, the
 // x pass summing over the children in order.
-template <typename T>
-__global__ __launch_bounds__(256) void k_m2m1(int n, int n3p, int tpb, int box0, int nb,
+template <typename T, int NF = 0>  // NF > 0: the order fixed at compile time (loops unrolled)
+__global__ __launch_bounds__(256) void k_m2m1(int n_rt, int n3p, int tpb, int box0, int nb,
                                               const int *__restrict__ child_off,
                                               const int *__restrict__ child_idx,
                                               const int *__restrict__ octant,
                                               const T *__restrict__ m2m1d, T *__restrict__ M,
                                               __bf16 *__restrict__ Mb) {
+  const int n = NF > 0 ? NF : n_rt;
   extern __shared__ unsigned char smem_raw[];
   const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
   const int n3 = n * n * n, nn = n * n;
@@ -633,11 +634,12 @@ __global__ __launch_bounds__(256) void k_m2m1(int n, int n3p, int tpb, int box0,
 
 // Single right-hand side L2L: one thread per (child, node), the transposed
 // contractions x^T, y^T, z^T of k_l2l through a per-child LDS slot.
-template <typename T>
-__global__ __launch_bounds__(256) void k_l2l1(int n, int tpb, int box0, int nb,
+template <typename T, int NF = 0>  // NF > 0: the order fixed at compile time (loops unrolled)
+__global__ __launch_bounds__(256) void k_l2l1(int n_rt, int tpb, int box0, int nb,
                                               const int *__restrict__ parent,
                                               const int *__restrict__ octant,
                                               const T *__restrict__ m2m1d, T *__restrict__ Lc) {
+  const int n = NF > 0 ? NF : n_rt;
   extern __shared__ unsigned char smem_raw[];
   const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
   const int n3 = n * n * n, nn = n * n;
@@ -2397,9 +2399,18 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch, (nb + nch) * exp_bytes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
       const int tpb = round_up(n3, 64), bpb = 256 / tpb;
-      hipLaunchKernelGGL((k_m2m1<T>), dim3((nb + bpb - 1) / bpb), dim3(256), 4096 * sizeof(T), s, n,
-                         n3p, tpb, f.lvl_off[l], nb, f.child_off, f.child_idx, f.octant, f.m2m1d,
-                         M, mb);
+#define GP_M2M1(NF)                                                                             \
+  hipLaunchKernelGGL((k_m2m1<T, NF>), dim3((nb + bpb - 1) / bpb), dim3(256), 4096 * sizeof(T), s, \
+                     n, n3p, tpb, f.lvl_off[l], nb, f.child_off, f.child_idx, f.octant, f.m2m1d,   \
+                     M, mb)
+      switch (n) {
+        case 3: GP_M2M1(3); break;
+        case 4: GP_M2M1(4); break;
+        case 5: GP_M2M1(5); break;
+        case 6: GP_M2M1(6); break;
+        default: GP_M2M1(0); break;
+      }
+#undef GP_M2M1
     } else if (tiled) {
       const int b0 = f.lvl_off[l];
       switch (n) {
@@ -2621,8 +2632,17 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nr * nb, 2.0 * nb * exp_bytes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
       const int tpb = round_up(n3, 64), bpb = 256 / tpb;
-      hipLaunchKernelGGL((k_l2l1<T>), dim3((nb + bpb - 1) / bpb), dim3(256), 512 * sizeof(T), s, n,
-                         tpb, f.lvl_off[l + 1], nb, f.parent, f.octant, f.m2m1d, Lc);
+#define GP_L2L1(NF)                                                                            \
+  hipLaunchKernelGGL((k_l2l1<T, NF>), dim3((nb + bpb - 1) / bpb), dim3(256), 512 * sizeof(T), s, \
+                     n, tpb, f.lvl_off[l + 1], nb, f.parent, f.octant, f.m2m1d, Lc)
+      switch (n) {
+        case 3: GP_L2L1(3); break;
+        case 4: GP_L2L1(4); break;
+        case 5: GP_L2L1(5); break;
+        case 6: GP_L2L1(6); break;
+        default: GP_L2L1(0); break;
+      }
+#undef GP_L2L1
     } else if (tiled) {
       const int b0 = f.lvl_off[l + 1];
       switch (n) {
