@@ -484,14 +484,15 @@ __global__ __launch_bounds__(256) void k_l2p1w(int n, int nleaf, const int *__re
 // Single right-hand side (ldm = 1) P2M: several leaves per work-group, one
 // thread per (leaf, node), each summing its leaf's panels in order (the
 // same expression and order as k_p2m).
-template <typename T>
-__global__ __launch_bounds__(256) void k_p2m1(int n, int n3p, int tpl, int nleaf,
+template <typename T, int NF = 0>  // NF > 0: the order fixed at compile time
+__global__ __launch_bounds__(256) void k_p2m1(int n_rt, int n3p, int tpl, int nleaf,
                                               const int *__restrict__ leaf_lo,
                                               const int *__restrict__ leaf_box, int leaf_box0,
                                               const T *__restrict__ w1d,
                                               const double *__restrict__ area,
                                               const T *__restrict__ z, T *__restrict__ M,
                                               __bf16 *__restrict__ Mb) {
+  const int n = NF > 0 ? NF : n_rt;
   const int li = blockIdx.x * (256 / tpl) + threadIdx.x / tpl, m = threadIdx.x % tpl;
   const int n3 = n * n * n;
   if (li >= nleaf || m >= n3) return;
@@ -2368,8 +2369,17 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                  (double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes);
     if (ldm == 1 && nr == 1 && ldz == 1) {
       const int tpl = round_up(n3, 64), lpb = 256 / tpl;
-      hipLaunchKernelGGL((k_p2m1<T>), dim3((nleaf + lpb - 1) / lpb), dim3(256), 0, s, n, n3p, tpl,
-                         nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb);
+      switch (n) {
+#define GP_P2M1(NF)                                                                              \
+  hipLaunchKernelGGL((k_p2m1<T, NF>), dim3((nleaf + lpb - 1) / lpb), dim3(256), 0, s, n, n3p, tpl, \
+                     nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb)
+        case 3: GP_P2M1(3); break;
+        case 4: GP_P2M1(4); break;
+        case 5: GP_P2M1(5); break;
+        case 6: GP_P2M1(6); break;
+        default: GP_P2M1(0); break;
+#undef GP_P2M1
+      }
     } else if (tiled) {
       const int MB = n3p / 16;
       if (MB <= 2)
