@@ -81,13 +81,14 @@ struct DBuf {
   // Completes before returning: callers pass temporaries and locals, and an
   // asynchronous copy from pageable memory may read the host buffer after
   // the call returns.
-  void upload(const std::vector<T> &h, hipStream_t s) {
-    if (h.size() != n) alloc(h.size());
+  void upload(const T *h, size_t count, hipStream_t s) {
+    if (count != n) alloc(count);
     if (n) {
-      GP_HIP(hipMemcpyAsync(p, h.data(), bytes(), hipMemcpyHostToDevice, s));
+      GP_HIP(hipMemcpyAsync(p, h, bytes(), hipMemcpyHostToDevice, s));
       GP_HIP(hipStreamSynchronize(s));
     }
   }
+  void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
   void zero(hipStream_t s) {
     if (n) GP_HIP(hipMemsetAsync(p, 0, bytes(), s));
   }
@@ -97,6 +98,13 @@ template <typename T>
 DBuf<T> to_device(const std::vector<T> &h, hipStream_t s) {
   DBuf<T> d;
   d.upload(h, s);
+  return d;
+}
+
+template <typename T>
+DBuf<T> to_device(const T *h, size_t count, hipStream_t s) {
+  DBuf<T> d;
+  d.upload(h, count, s);
   return d;
 }
 

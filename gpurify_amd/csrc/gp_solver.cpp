@@ -23,6 +23,7 @@
 #include <atomic>
 #include <climits>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <chrono>
@@ -57,11 +58,12 @@ int host_threads() {
   return nt;
 }
 
-// f(i) for i in [0, n) on the set-up threads (dynamic chunks of 64); the
+// f(i) for i in [0, n) on the set-up threads (dynamic chunks of `grain`:
+// 64 for leaves and boxes, 1 for problems of very different sizes); the
 // first exception is rethrown after all threads end.
 template <typename F>
-void parallel_for(int n, F &&f) {
-  const int nt = std::min(host_threads(), (n + 63) / 64);
+void parallel_for(int n, F &&f, int grain = 64) {
+  const int nt = std::min(host_threads(), (n + grain - 1) / grain);
   if (nt <= 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
@@ -73,8 +75,8 @@ void parallel_for(int n, F &&f) {
   for (int t = 0; t < nt; ++t)
     th.emplace_back([&] {
       try {
-        for (int i0; (i0 = next.fetch_add(64)) < n;)
-          for (int i = i0; i < std::min(n, i0 + 64); ++i) f(i);
+        for (int i0; (i0 = next.fetch_add(grain)) < n;)
+          for (int i = i0; i < std::min(n, i0 + grain); ++i) f(i);
       } catch (...) {
         std::lock_guard<std::mutex> lk(m);
         if (!err) err = std::current_exception();
@@ -158,15 +160,10 @@ uint16_t bf16_bits(float f) {
 }
 
 // Scaled SoA arrays of the panels in the order 'perm' (identity if null).
-void scaled_arrays(const PanelGeom &g, const uint32_t *perm, const double c[3], double L0,
-                   std::vector<double> &verts, std::vector<double> &cen,
-                   std::vector<double> &area, std::vector<uint32_t> &cond) {
-  int n = g.n;
-  verts.resize((size_t)n * 12);
-  cen.resize((size_t)n * 3);
-  area.resize(n);
-  cond.resize(n);
-  double inv = 1.0 / L0;
+void scale_into(const PanelGeom &g, const uint32_t *perm, const double c[3], double L0,
+                double *verts, double *cen, double *area, uint32_t *cond, uint32_t cond0) {
+  const int n = g.n;
+  const double inv = 1.0 / L0;
   for (int p = 0; p < n; ++p) {
     int i = perm ? (int)perm[p] : p;
     for (int k = 0; k < 4; ++k)
@@ -176,8 +173,19 @@ void scaled_arrays(const PanelGeom &g, const uint32_t *perm, const double c[3], 
     cen[3 * p + 1] = (g.cy[i] - c[1]) * inv;
     cen[3 * p + 2] = (g.cz[i] - c[2]) * inv;
     area[p] = g.area[i] * inv * inv;
-    cond[p] = g.cond[i];
+    cond[p] = g.cond[i] + cond0;
   }
+}
+
+void scaled_arrays(const PanelGeom &g, const uint32_t *perm, const double c[3], double L0,
+                   std::vector<double> &verts, std::vector<double> &cen,
+                   std::vector<double> &area, std::vector<uint32_t> &cond) {
+  int n = g.n;
+  verts.resize((size_t)n * 12);
+  cen.resize((size_t)n * 3);
+  area.resize(n);
+  cond.resize(n);
+  scale_into(g, perm, c, L0, verts.data(), cen.data(), area.data(), cond.data(), 0);
 }
 
 // Panels of each conductor as positions in the device order, listed by
@@ -436,12 +444,22 @@ struct FmmSystem {
     trees.assign(P, Tree());
     L0s.assign(P, 1.0);
     L = 2;
-    parallel_for(P, [&](int p) {
-      const PanelGeom &g = *probs[p];
-      int Lp = o.max_level ? (int)o.max_level : choose_depth(g, (int)o.leaf_target);
-      build_tree(g, std::max(Lp, 2), trees[p]);
-      L0s[p] = trees[p].half;
-    });
+    // Per-problem loops take problems one at a time, largest first, so the
+    // threads finish together on batches whose sizes span two decades.
+    std::vector<int> big_first(P);
+    std::iota(big_first.begin(), big_first.end(), 0);
+    std::stable_sort(big_first.begin(), big_first.end(),
+                     [&](int a, int b) { return probs[a]->n > probs[b]->n; });
+    parallel_for(
+        P,
+        [&](int i) {
+          const int p = big_first[i];
+          const PanelGeom &g = *probs[p];
+          int Lp = o.max_level ? (int)o.max_level : choose_depth(g, (int)o.leaf_target);
+          build_tree(g, std::max(Lp, 2), trees[p]);
+          L0s[p] = trees[p].half;
+        },
+        1);
     for (int p = 0; p < P; ++p) L = std::max(L, trees[p].L);
     // Problems in order of depth: the global leaf order (and so the panel
     // order) lists shallower problems first.
@@ -514,22 +532,26 @@ struct FmmSystem {
 
     phase("numbering");
     // Scaled panel arrays, per problem in its own unit frame.
-    std::vector<double> hv((size_t)N * 12), hc((size_t)N * 3), ha(N);
-    std::vector<uint32_t> hcond(N);
-    parallel_for(P, [&](int p) {
-      std::vector<double> v, c, a;
-      std::vector<uint32_t> cd;
-      scaled_arrays(*probs[p], trees[p].perm.data(), trees[p].center, L0s[p], v, c, a, cd);
-      const size_t o0 = panel_off[p];
-      std::copy(v.begin(), v.end(), hv.begin() + o0 * 12);
-      std::copy(c.begin(), c.end(), hc.begin() + o0 * 3);
-      std::copy(a.begin(), a.end(), ha.begin() + o0);
-      for (size_t i = 0; i < cd.size(); ++i) hcond[o0 + i] = cd[i] + (uint32_t)cond_off[p];
-    });
-    verts = to_device(hv, s);
-    cen = to_device(hc, s);
-    area = to_device(ha, s);
-    cond = to_device(hcond, s);
+    // (written in place by the problems' threads: no zero fill, no copies)
+    std::unique_ptr<double[]> hv(new double[(size_t)N * 12]), hc(new double[(size_t)N * 3]),
+        ha(new double[N]);
+    std::unique_ptr<uint32_t[]> hcond(new uint32_t[N]);
+    parallel_for(
+        P,
+        [&](int i) {
+          const int p = big_first[i];
+          const size_t o0 = panel_off[p];
+          scale_into(*probs[p], trees[p].perm.data(), trees[p].center, L0s[p],
+                     hv.get() + o0 * 12, hc.get() + o0 * 3, ha.get() + o0, hcond.get() + o0,
+                     (uint32_t)cond_off[p]);
+        },
+        1);
+    verts = to_device(hv.get(), (size_t)N * 12, s);
+    cen = to_device(hc.get(), (size_t)N * 3, s);
+    area = to_device(ha.get(), (size_t)N, s);
+    cond = to_device(hcond.get(), (size_t)N, s);
+    hv.reset();
+    hc.reset();
 
     phase("panels");
     // Near-field tiles (cap/fmm_solver.rs:52-91 over bbfmm.rs:189-206), per
@@ -716,12 +738,16 @@ struct FmmSystem {
     std::vector<int> hil_off(nb + 1, 0), hil_src, hil_oidx;
     for (int l = 0; l <= L; ++l)
       for (int b = lvl_off[l]; b < lvl_off[l + 1]; ++b) hlevel[b] = l;
-    for (int b = 0; b < nb; ++b) {
+    parallel_for(nb, [&](int b) {
       const Tree &t = trees[gp_of[b]];
       const int lb = gb_of[b];
-      hchild_off[b + 1] = hchild_off[b] + (t.child_off[lb + 1] - t.child_off[lb]);
-      hil_off[b + 1] = hil_off[b] + (t.il_off[lb + 1] - t.il_off[lb]);
-      if (hil_off[b + 1] > hil_off[b]) htargets.push_back(b);
+      hchild_off[b + 1] = t.child_off[lb + 1] - t.child_off[lb];
+      hil_off[b + 1] = t.il_off[lb + 1] - t.il_off[lb];
+    });
+    for (int b = 0; b < nb; ++b) {
+      if (hil_off[b + 1] > 0) htargets.push_back(b);
+      hchild_off[b + 1] += hchild_off[b];
+      hil_off[b + 1] += hil_off[b];
     }
     hchild.resize(hchild_off[nb]);
     hil_src.resize(hil_off[nb]);
@@ -816,11 +842,19 @@ struct FmmSystem {
       const int mp = kr ? kr : n3p;
       tile_size = mp <= 128 ? 64 : 16;
       const int TS = tile_size;
-      std::vector<int> ord(htargets);
-      auto len = [&](int b) { return hil_off[b + 1] - hil_off[b]; };
-      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-        return hoct[a] != hoct[b] ? hoct[a] < hoct[b] : len(a) > len(b);
-      });
+      // (a stable counting sort on (octant, longest list first): the order of
+      // a stable comparison sort, in linear time over millions of targets)
+      std::vector<int> ord(htargets.size());
+      {
+        auto len = [&](int b) { return hil_off[b + 1] - hil_off[b]; };
+        int maxlen = 0;
+        for (int b : htargets) maxlen = std::max(maxlen, len(b));
+        auto key = [&](int b) { return hoct[b] * (maxlen + 1) + (maxlen - len(b)); };
+        std::vector<int> start((size_t)8 * (maxlen + 1) + 1, 0);
+        for (int b : htargets) start[key(b) + 1]++;
+        for (size_t k = 0; k + 1 < start.size(); ++k) start[k + 1] += start[k];
+        for (int b : htargets) ord[start[key(b)]++] = b;
+      }
       std::vector<int> tfirst, tcount;
       for (size_t i = 0; i < ord.size();) {
         size_t j = i;
