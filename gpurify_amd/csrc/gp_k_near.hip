@@ -361,6 +361,97 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf(
   }
 }
 
+// k_near_spmv_leaf with the values issued early: each wavefront loads the
+// first 16-byte vector (and the unaligned head and tail) of its first R rows
+// before the x gather, so the value loads overlap the column -> x -> LDS
+// chain instead of following it.  Same sums in the same order.
+template <typename TV, typename TX, int WPB, int R>
+__global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf_pf(
+    const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
+    const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
+    const TX *__restrict__ x, double *__restrict__ y) {
+  using A = typename Acc<TX>::type;
+  using V = typename Vec16<TV>::type;
+  constexpr int VN = Vec16<TV>::n;
+  extern __shared__ unsigned char smem_raw[];
+  A *xs = reinterpret_cast<A *>(smem_raw);
+  const int T = blockIdx.x;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const int64_t base = nf_off[T];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  V pv[R];
+  TV ph[R], pt[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = w + r * WPB;
+#pragma unroll
+    for (int k = 0; k < VN; ++k) pv[r][k] = 0;
+    ph[r] = 0;
+    pt[r] = 0;
+    if (t < nt) {
+      const int64_t st = base + (int64_t)t * S;
+      const TV *row = vals + st;
+      const int head = min((int)((VN - (st & (VN - 1))) & (VN - 1)), S);
+      const int nv = (S - head) / VN;
+      if (lane < head) ph[r] = row[lane];
+      if (lane < nv) pv[r] = reinterpret_cast<const V *>(row + head)[lane];
+      const int tl = head + nv * VN + lane;
+      if (tl < S) pt[r] = row[tl];
+    }
+  }
+  for (int s = threadIdx.x; s < S; s += 64 * WPB) xs[s] = (A)x[cols[c0 + s]];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int t = w + r * WPB;
+    if (t < nt) {
+      const int64_t st = base + (int64_t)t * S;
+      const TV *row = vals + st;
+      const int head = min((int)((VN - (st & (VN - 1))) & (VN - 1)), S);
+      const int nv = (S - head) / VN;
+      A acc = 0;
+      if (lane < head) acc += (A)ph[r] * xs[lane];
+      if (lane < nv) {
+        const A *xq = xs + head + lane * VN;
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc += (A)pv[r][k] * xq[k];
+      }
+      const V *rv = reinterpret_cast<const V *>(row + head);
+      for (int q = lane + 64; q < nv; q += 64) {
+        const V v = rv[q];
+        const A *xq = xs + head + q * VN;
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc += (A)v[k] * xq[k];
+      }
+      const int tl = head + nv * VN + lane;
+      if (tl < S) acc += (A)pt[r] * xs[tl];
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+      if (lane == 0) y[t0 + t] = (double)acc;
+    }
+  }
+  for (int t = w + R * WPB; t < nt; t += WPB) {
+    const int64_t st = base + (int64_t)t * S;
+    const TV *row = vals + st;
+    const int head = min((int)((VN - (st & (VN - 1))) & (VN - 1)), S);
+    const int nv = (S - head) / VN;
+    A acc = 0;
+    if (lane < head) acc += (A)row[lane] * xs[lane];
+    const V *rv = reinterpret_cast<const V *>(row + head);
+#pragma unroll 4
+    for (int q = lane; q < nv; q += 64) {
+      const V v = rv[q];
+      const A *xq = xs + head + q * VN;
+#pragma unroll
+      for (int k = 0; k < VN; ++k) acc += (A)v[k] * xq[k];
+    }
+    const int tl = head + nv * VN + lane;
+    if (tl < S) acc += (A)row[tl] * xs[tl];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if (lane == 0) y[t0 + t] = (double)acc;
+  }
+}
+
 template <typename TV, typename TX>
 void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int ldx, double *y,
                          int ldy, int nr, hipStream_t s) {
@@ -396,9 +487,10 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
                                       : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64");
   if (nr == 1 && ldx == 1 && ldy == 1) {
     // leaf-staged with 4, 8, 2 or 1 wavefronts a leaf (fewer: more leaves in
-    // flight per CU for leaves of a few panels), or one wavefront per row
+    // flight per CU for leaves of a few panels), or one wavefront per row;
+    // 5-7: leaf-staged with the first rows' values issued before the gather
     const bool leaf_ok = xs_bytes <= 48 * 1024;
-    tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 5 : 1, s, [&](int v) {
+    tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 8 : 1, s, [&](int v) {
       if (!leaf_ok || v == 2)
         spmv_row();
       else if (v == 0)
@@ -410,9 +502,18 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
       else if (v == 3)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 2>), dim3(nf.n_leaves), dim3(128), xs_bytes,
                            s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
-      else
+      else if (v == 4)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 1>), dim3(nf.n_leaves), dim3(64), xs_bytes,
                            s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+      else if (v == 5)
+        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 2>), dim3(nf.n_leaves), dim3(128),
+                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+      else if (v == 6)
+        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 1, 4>), dim3(nf.n_leaves), dim3(64),
+                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+      else
+        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 4, 1>), dim3(nf.n_leaves), dim3(256),
+                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
     });
   } else if (nr > 16) {
     // FP32 matrix cores (64-column chunks, 16-row tiles), register tiles of
