@@ -508,6 +508,43 @@ __global__ __launch_bounds__(256) void k_p2m1(int n_rt, int n3p, int tpl, int nl
   store_mult(M, Mb, (size_t)box * n3p + m, acc);
 }
 
+// Single-RHS P2M with the leaf's panels staged: one work-group of
+// round_up(n3, 64) threads per leaf loads the 1-D weights and charges of up to
+// 64 panels at a time as one contiguous block into LDS, then thread m sums
+// them for node m (the expression and order of k_p2m1).  k_p2m1 issues one
+// round of dependent loads per panel; leaves of a few panels are all latency.
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void k_p2m1s(int n3p, int nleaf, const int *__restrict__ leaf_lo,
+                                               const int *__restrict__ leaf_box, int leaf_box0,
+                                               const T *__restrict__ w1d,
+                                               const double *__restrict__ area,
+                                               const T *__restrict__ z, T *__restrict__ M,
+                                               __bf16 *__restrict__ Mb) {
+  constexpr int n = NF, n3 = n * n * n, W3 = 3 * n, CH = 64;
+  __shared__ T sw[CH * W3];
+  __shared__ T sq[CH];
+  const int li = blockIdx.x, m = threadIdx.x;
+  const int lo = leaf_lo[li], hi = leaf_lo[li + 1];
+  const int a = m / (n * n), b = (m / n) % n, c = m % n;
+  T acc = 0;
+  for (int p0 = lo; p0 < hi; p0 += CH) {
+    const int cnt = min(CH, hi - p0);
+    __syncthreads();  // the previous chunk has been read
+    for (int e = m; e < cnt * W3; e += blockDim.x) sw[e] = w1d[(size_t)p0 * W3 + e];
+    for (int j = m; j < cnt; j += blockDim.x) sq[j] = (T)area[p0 + j] * z[p0 + j];
+    __syncthreads();
+    if (m < n3)
+      for (int j = 0; j < cnt; ++j) {
+        const T *w = sw + j * W3;
+        acc += w[a] * w[n + b] * w[2 * n + c] * sq[j];
+      }
+  }
+  if (m < n3) {
+    const int box = leaf_box ? leaf_box[li] : leaf_box0 + li;
+    store_mult(M, Mb, (size_t)box * n3p + m, acc);
+  }
+}
+
 // Single right-hand side L2P: one thread per panel, y[p] += sum over nodes of
 // S(p, node) Lc[leaf][node] (the same expression and order as k_l2p).
 template <typename T>
@@ -2369,17 +2406,26 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                  (double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes);
     if (ldm == 1 && nr == 1 && ldz == 1) {
       const int tpl = round_up(n3, 64), lpb = 256 / tpl;
-      switch (n) {
+      // one thread per (leaf, node), or (variant 1) one work-group per leaf
+      // with its panels staged in LDS
+      tuned_launch(f32 ? "p2m1_f32" : "p2m1_f64", nleaf, n, 1, n >= 3 && n <= 6 ? 2 : 1, s,
+                   [&](int v) {
+        switch (n) {
 #define GP_P2M1(NF)                                                                              \
-  hipLaunchKernelGGL((k_p2m1<T, NF>), dim3((nleaf + lpb - 1) / lpb), dim3(256), 0, s, n, n3p, tpl, \
-                     nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb)
-        case 3: GP_P2M1(3); break;
-        case 4: GP_P2M1(4); break;
-        case 5: GP_P2M1(5); break;
-        case 6: GP_P2M1(6); break;
-        default: GP_P2M1(0); break;
+  if (v == 1 && NF > 0)                                                                          \
+    hipLaunchKernelGGL((k_p2m1s<T, (NF > 0 ? NF : 3)>), dim3(nleaf), dim3(tpl), 0, s, n3p, nleaf, \
+                       f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb);                    \
+  else                                                                                           \
+    hipLaunchKernelGGL((k_p2m1<T, NF>), dim3((nleaf + lpb - 1) / lpb), dim3(256), 0, s, n, n3p,  \
+                       tpl, nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb)
+          case 3: GP_P2M1(3); break;
+          case 4: GP_P2M1(4); break;
+          case 5: GP_P2M1(5); break;
+          case 6: GP_P2M1(6); break;
+          default: GP_P2M1(0); break;
 #undef GP_P2M1
-      }
+        }
+      });
     } else if (tiled) {
       const int MB = n3p / 16;
       if (MB <= 2)

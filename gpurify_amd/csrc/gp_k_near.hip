@@ -509,10 +509,10 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
         hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 2>), dim3(nf.n_leaves), dim3(128),
                            xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
       else if (v == 6)
-        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 1, 4>), dim3(nf.n_leaves), dim3(64),
+        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 3>), dim3(nf.n_leaves), dim3(128),
                            xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
       else
-        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 4, 1>), dim3(nf.n_leaves), dim3(256),
+        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 4, 2>), dim3(nf.n_leaves), dim3(256),
                            xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
     });
   } else if (nr > 16) {
