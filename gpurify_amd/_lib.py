@@ -96,7 +96,7 @@ EXPORTS = [
     "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_cap_solve_batched", "gp_cap_jacobian",
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
-    "gp_tune_report",
+    "gp_tune_report", "gp_qui_parse",
 ]
 
 _lib = None
@@ -152,6 +152,10 @@ def load():
     L.gp_profile_enable.restype = C.c_int
     L.gp_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.gp_profile_report.restype = C.c_int
+    L.gp_qui_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_double, C.POINTER(C.c_uint32),
+                               C.POINTER(C.c_uint32), C.POINTER(C.c_size_t), C.POINTER(C.c_double),
+                               C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_char_p]
+    L.gp_qui_parse.restype = C.c_int
     L.gp_tune_force.argtypes = [vp, C.c_int]
     L.gp_tune_force.restype = C.c_int
     L.gp_tune_report.argtypes = [vp, C.c_char_p, C.c_size_t]
@@ -370,3 +374,27 @@ def mesh_groups(groups, xy_scale=1e-9, refine=1, max_panels=0):
         raise GpError(rc, L.gp_last_error(None).decode())
     del keep
     return v, c, e
+
+
+def parse_qui(text, scale=1.0):
+    """FastCap quickif geometry (cap/geometry.rs:56-135) through the product
+    parser: dict with vertices (n,4,3) (a triangle repeats its last vertex),
+    conductor (n,) uint32, n_vertices (n,) uint8 and names (list of str).
+    Raises GpError (GP_ERR_INVALID_INPUT, "line <n>: ...") on a bad line."""
+    L = load()
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    n, nc, nl = C.c_uint32(0), C.c_uint32(0), C.c_size_t(0)
+    rc = L.gp_qui_parse(raw, len(raw), scale, C.byref(n), C.byref(nc), C.byref(nl), None, None,
+                        None, None)
+    if rc != GP_OK:
+        raise GpError(rc, L.gp_last_error(None).decode())
+    v = np.zeros((n.value, 4, 3))
+    c = np.zeros(n.value, dtype=np.uint32)
+    nv = np.zeros(n.value, dtype=np.uint8)
+    names = C.create_string_buffer(max(1, nl.value))
+    rc = L.gp_qui_parse(raw, len(raw), scale, C.byref(n), C.byref(nc), C.byref(nl), _dptr(v),
+                        _uptr(c), nv.ctypes.data_as(C.POINTER(C.c_uint8)), names)
+    if rc != GP_OK:
+        raise GpError(rc, L.gp_last_error(None).decode())
+    out = names.raw[:nl.value].split(b"\0")[:-1] if nl.value else []
+    return {"vertices": v, "conductor": c, "n_vertices": nv, "names": [x.decode() for x in out]}

@@ -13,6 +13,7 @@
 #include "../../include/gpurify.h"
 #include "gp_bridge.h"
 #include "gp_common.h"
+#include "gp_fastcap.h"
 #include "gp_solver.h"
 #include "gp_tree.h"
 
@@ -435,6 +436,31 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
       if (cap_af_out) cap_af_out[i] = af;
     }
     if (C_out) std::memcpy(C_out, C.data(), C.size() * sizeof(double));
+  });
+}
+
+
+int gp_qui_parse(const char *text, size_t text_len, double scale, uint32_t *n_panels,
+                 uint32_t *n_conductors, size_t *names_len, double *vertices,
+                 uint32_t *conductor, uint8_t *n_vertices, char *names) {
+  return guarded(nullptr, [&] {
+    if (!text && text_len) throw gp::Error(GP_ERR_INVALID_INPUT, "gp_qui_parse: text is NULL");
+    gp::QuiGeometry geo;
+    gp::parse_qui(text, text_len, scale, geo);
+    const size_t n = geo.cond.size();
+    size_t nl = 0;
+    for (const auto &s : geo.names) nl += s.size() + 1;
+    if (n_panels) *n_panels = (uint32_t)n;
+    if (n_conductors) *n_conductors = (uint32_t)geo.names.size();
+    if (names_len) *names_len = nl;
+    if (vertices) std::memcpy(vertices, geo.verts.data(), n * 12 * sizeof(double));
+    if (conductor) std::memcpy(conductor, geo.cond.data(), n * sizeof(uint32_t));
+    if (n_vertices) std::memcpy(n_vertices, geo.n_vertices.data(), n);
+    if (names)
+      for (const auto &s : geo.names) {
+        std::memcpy(names, s.c_str(), s.size() + 1);
+        names += s.size() + 1;
+      }
   });
 }
 

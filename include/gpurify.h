@@ -250,6 +250,23 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
                          const gp_cap_opts *opts, uint32_t *n_nets, uint32_t *nets_out,
                          double *cap_af_out, double *C_out, gp_cap_report *report);
 
+/* ------------------------------------------------------------ FastCap input
+ * FastCap "quickif" geometry text (.qui), as cap::geometry::parse
+ * (crates/pex/src/quasistatic/cap/geometry.rs:56-135) reads it: Q (quad)
+ * and T (triangle) panel lines, N name lines, comment lines starting with
+ * 0, * or %; conductor numbers interned to 0-based indices in first-seen
+ * order; a C include or any other line is GP_ERR_INVALID_INPUT with
+ * "line <n>: ..." in gp_last_error(NULL).  Coordinates are multiplied by
+ * scale.  The panels come in the gp_panels layout (a triangle repeats its
+ * last vertex), ready for gp_cap_solve.
+ * Two-call protocol: with vertices == NULL only *n_panels, *n_conductors and
+ * *names_len (bytes of the names, each NUL-terminated, in conductor order)
+ * are written; the second call fills every non-NULL output (names needs
+ * *names_len bytes, n_vertices gets 3 or 4 per panel). */
+int gp_qui_parse(const char *text, size_t text_len, double scale, uint32_t *n_panels,
+                 uint32_t *n_conductors, size_t *names_len, double *vertices,
+                 uint32_t *conductor, uint8_t *n_vertices, char *names);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,48 @@
+"""A FastCap quickif (.qui) file through the device solve: panels read by the
+product parser (gp_qui_parse, the grammar of cap/geometry.rs:56-135) and
+solved by gp_cap_solve give the oracle's capacitance on the same panels.
+The file mixes quads and triangles: a triangle repeats its last vertex,
+which adds a zero-length edge and a zero-area fan triangle to the
+integrals (panel.rs:29-47, 147-208), so it is solved as the triangle."""
+import numpy as np
+import pytest
+
+import gpurify_amd as G
+from gpurify_amd import workloads as W
+from gpurify_amd.fastcap import write_qui
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_qui_file_solves_like_the_oracle(ctx):
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=1)
+    # split the substrate plate's quads into triangle pairs (same surface)
+    plate = np.flatnonzero(c == c.max())
+    tris, tc = [], []
+    for k in plate:
+        a, b, cc, d = v[k]
+        tris += [[a, b, cc, cc], [a, cc, d, d]]
+        tc += [c[k], c[k]]
+    keep = np.flatnonzero(c != c.max())
+    v2 = np.concatenate([v[keep], np.array(tris)])
+    c2 = np.concatenate([c[keep], np.array(tc, dtype=np.uint32)])
+    g = G.parse_qui(write_qui(v2, c2, names=["a", "b", "gnd"]))
+    assert g["names"] == ["a", "b", "gnd"] and int((g["n_vertices"] == 3).sum()) == len(tris)
+    n = len(g["conductor"])
+    print(f"qui: {n} panels ({len(tris)} triangles), dense solve", flush=True)
+    o = G.default_opts(method=G.GP_METHOD_DENSE)
+    Cd, rep = ctx.cap_solve(g["vertices"], g["conductor"], 3, o)
+    Co = O.cap_dense(g["vertices"], g["conductor"], 3)
+    assert np.abs(Cd - Co).max() / np.abs(Co).max() < 1e-11
+    # the triangle plate gives the quad plate's capacitance to discretisation
+    # accuracy (same surface, different collocation points)
+    Cq = O.cap_dense(v, c, 3)
+    assert np.abs(Cd - Cq).max() / np.abs(Cq).max() < 0.05
+    print("qui: FMM solve", flush=True)
+    of = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=4, max_level=3, tol=1e-10)
+    Cf, rf = ctx.cap_solve(g["vertices"], g["conductor"], 3, of)
+    Cfo, _, _ = O.cap_fmm(g["vertices"], g["conductor"], 3, order=4, max_level=3, tol=1e-10,
+                          m2l_mode=1)
+    assert rf["converged"] == 1
+    assert np.abs(Cf - Cfo).max() / np.abs(Cfo).max() < 1e-7
