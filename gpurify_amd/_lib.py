@@ -96,7 +96,7 @@ EXPORTS = [
     "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_cap_solve_batched", "gp_cap_jacobian",
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
-    "gp_tune_report", "gp_qui_parse",
+    "gp_tune_report", "gp_qui_parse", "gp_solve_resistance",
 ]
 
 _lib = None
@@ -156,6 +156,10 @@ def load():
                                C.POINTER(C.c_uint32), C.POINTER(C.c_size_t), C.POINTER(C.c_double),
                                C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_char_p]
     L.gp_qui_parse.restype = C.c_int
+    L.gp_solve_resistance.argtypes = [C.POINTER(GpResPolygon), C.c_uint32, C.c_double,
+                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_double)]
+    L.gp_solve_resistance.restype = C.c_int
     L.gp_tune_force.argtypes = [vp, C.c_int]
     L.gp_tune_force.restype = C.c_int
     L.gp_tune_report.argtypes = [vp, C.c_char_p, C.c_size_t]
@@ -398,3 +402,43 @@ def parse_qui(text, scale=1.0):
         raise GpError(rc, L.gp_last_error(None).decode())
     out = names.raw[:nl.value].split(b"\0")[:-1] if nl.value else []
     return {"vertices": v, "conductor": c, "n_vertices": nv, "names": [x.decode() for x in out]}
+
+
+class GpResPolygon(C.Structure):
+    _fields_ = [("net", C.c_uint32), ("n_vertices", C.c_uint32), ("xy", C.POINTER(C.c_int32)),
+                ("has_pex", C.c_uint32), ("volume_conductor", C.c_uint32),
+                ("sheet_res_ohm_sq", C.c_double), ("via_res_ohm", C.c_double),
+                ("thickness_nm", C.c_double), ("height_nm", C.c_double)]
+
+
+def solve_resistance(polys, dbu_nm=1.0):
+    """r_ohm per net of extract_quasistatic's resistance half (host C++,
+    gp_solve_resistance).  polys: dicts with net, xy ((n,2) DBU ring), and
+    optional has_pex (1), volume_conductor (1), sheet_res (0), via_res (0),
+    thickness_nm, height_nm.  Returns {net: r_ohm}; raises GpError."""
+    L = load()
+    keep, arr = [], (GpResPolygon * max(1, len(polys)))()
+    for i, q in enumerate(polys):
+        xy = np.ascontiguousarray(np.asarray(q["xy"], dtype=np.int32).reshape(-1))
+        keep.append(xy)
+        a = arr[i]
+        a.net = int(q["net"])
+        a.n_vertices = len(xy) // 2
+        a.xy = xy.ctypes.data_as(C.POINTER(C.c_int32))
+        a.has_pex = int(q.get("has_pex", 1))
+        a.volume_conductor = int(q.get("volume_conductor", 1))
+        a.sheet_res_ohm_sq = float(q.get("sheet_res", 0.0))
+        a.via_res_ohm = float(q.get("via_res", 0.0))
+        a.thickness_nm = float(q.get("thickness_nm", 0.0))
+        a.height_nm = float(q.get("height_nm", 0.0))
+    n = C.c_uint32(0)
+    rc = L.gp_solve_resistance(arr, len(polys), dbu_nm, C.byref(n), None, None)
+    if rc != GP_OK:
+        raise GpError(rc, L.gp_last_error(None).decode())
+    nets = np.zeros(n.value, dtype=np.uint32)
+    r = np.zeros(n.value)
+    rc = L.gp_solve_resistance(arr, len(polys), dbu_nm, C.byref(n), _uptr(nets), _dptr(r))
+    if rc != GP_OK:
+        raise GpError(rc, L.gp_last_error(None).decode())
+    del keep
+    return {int(k): float(v) for k, v in zip(nets, r)}

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 
@@ -14,6 +15,7 @@
 #include "gp_bridge.h"
 #include "gp_common.h"
 #include "gp_fastcap.h"
+#include "gp_resistance.h"
 #include "gp_solver.h"
 #include "gp_tree.h"
 
@@ -461,6 +463,21 @@ int gp_qui_parse(const char *text, size_t text_len, double scale, uint32_t *n_pa
         std::memcpy(names, s.c_str(), s.size() + 1);
         names += s.size() + 1;
       }
+  });
+}
+
+int gp_solve_resistance(const gp_res_polygon *polys, uint32_t n_polys, double dbu_nm,
+                        uint32_t *n_nets, uint32_t *nets_out, double *r_ohm_out) {
+  return guarded(nullptr, [&] {
+    std::map<uint32_t, double> r;
+    gp::solve_resistance(polys, n_polys, dbu_nm, r);
+    if (n_nets) *n_nets = (uint32_t)r.size();
+    size_t i = 0;
+    for (const auto &e : r) {
+      if (nets_out) nets_out[i] = e.first;
+      if (r_ohm_out) r_ohm_out[i] = e.second;
+      ++i;
+    }
   });
 }
 

@@ -267,6 +267,40 @@ int gp_qui_parse(const char *text, size_t text_len, double scale, uint32_t *n_pa
                  uint32_t *n_conductors, size_t *names_len, double *vertices,
                  uint32_t *conductor, uint8_t *n_vertices, char *names);
 
+/* ------------------------------------------------------------ DC resistance
+ * One layout polygon for the resistance half of extract_quasistatic
+ * (bridge.rs:97-129, 180-299).  xy: the ring of n_vertices points (x, y
+ * int32 DBU, first point not repeated), as GeometryStore holds it.  has_pex:
+ * the polygon's layer has deck.pex parameters (else it is skipped, as
+ * bridge.rs:99-101); sheet_res_ohm_sq / via_res_ohm are those parameters;
+ * volume_conductor is is_volume_conductor(params) (bridge.rs:170-177);
+ * thickness_nm / height_nm are the layer's ProcessStack values. */
+typedef struct {
+  uint32_t net;
+  uint32_t n_vertices;
+  const int32_t *xy;
+  uint32_t has_pex;
+  uint32_t volume_conductor;
+  double sheet_res_ohm_sq;
+  double via_res_ohm;
+  double thickness_nm;
+  double height_nm;
+} gp_res_polygon;
+
+/* r_ohm per net (NetParasitics.r_ohm, analytical/mod.rs:353-358) of
+ * extract_quasistatic: the polygons' via resistances in polygon order, then
+ * the port resistance of every sheet-resistance polygon, which the
+ * reference obtains from solve_resistance -> henry::solve_with(Direct) at
+ * f = 0 (bridge.rs:180-299, henry/solver.rs:336-584) and this library
+ * evaluates in closed form, operation for operation (each polygon is an
+ * isolated one-filament segment, so the nodal system is diagonal).  Nets
+ * come in ascending order; only nets the reference's map would hold appear.
+ * Errors: GP_ERR_INVALID_INPUT (the reference's UnsupportedGeometry /
+ * InvalidProcessValue / InvalidDatabaseUnit messages), GP_ERR_NON_FINITE.
+ * Two-call protocol: with nets_out == NULL only *n_nets is written. */
+int gp_solve_resistance(const gp_res_polygon *polys, uint32_t n_polys, double dbu_nm,
+                        uint32_t *n_nets, uint32_t *nets_out, double *r_ohm_out);
+
 #ifdef __cplusplus
 }
 #endif
