@@ -68,6 +68,11 @@ class GpParamMap(C.Structure):
                 ("panel", C.POINTER(C.c_uint32)), ("dverts", C.POINTER(C.c_double))]
 
 
+class GpPanelRole(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("conductor", C.c_uint32), ("eps_surrounding", C.c_double),
+                ("eps_out", C.c_double), ("eps_in", C.c_double), ("reference", C.c_double * 3)]
+
+
 class GpCapReport(C.Structure):
     _fields_ = [("n_panels", C.c_uint32), ("n_conductors", C.c_uint32),
                 ("method_used", C.c_uint32), ("precision_used", C.c_uint32),
@@ -96,7 +101,7 @@ EXPORTS = [
     "gp_version", "gp_cap_opts_default", "gp_cap_solve", "gp_cap_solve_batched", "gp_cap_jacobian",
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
-    "gp_tune_report", "gp_qui_parse", "gp_solve_resistance",
+    "gp_tune_report", "gp_qui_parse", "gp_solve_resistance", "gp_cap_solve_dielectric",
 ]
 
 _lib = None
@@ -152,6 +157,9 @@ def load():
     L.gp_profile_enable.restype = C.c_int
     L.gp_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.gp_profile_report.restype = C.c_int
+    L.gp_cap_solve_dielectric.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(GpPanelRole),
+                                          C.POINTER(C.c_double), C.POINTER(GpCapReport)]
+    L.gp_cap_solve_dielectric.restype = C.c_int
     L.gp_qui_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_double, C.POINTER(C.c_uint32),
                                C.POINTER(C.c_uint32), C.POINTER(C.c_size_t), C.POINTER(C.c_double),
                                C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_char_p]
@@ -267,6 +275,31 @@ class Context:
         rep = GpCapReport()
         o = opts if opts is not None else default_opts()
         self._check(self._L.gp_cap_solve(self._h, C.byref(p), C.byref(o), _dptr(Cm), C.byref(rep)))
+        return Cm, rep.as_dict()
+
+    def cap_solve_dielectric(self, verts4, role, cond, eps, eps_out, eps_in, ref, ncond):
+        """cap::solve_dielectric on the device (dense FP64): C (ncond x ncond,
+        F, permittivities applied).  role[k] 0 = conductor (cond[k], eps[k]
+        surrounding), 1 = dielectric interface (eps_out[k], eps_in[k], ref[k]
+        a point on the inner side)."""
+        v = np.ascontiguousarray(verts4, dtype=np.float64).reshape(-1, 4, 3)
+        n = v.shape[0]
+        c = np.ascontiguousarray(np.where(np.asarray(role) == 0, cond, 0), dtype=np.uint32)
+        roles = (GpPanelRole * max(1, n))()
+        ref = np.asarray(ref, dtype=np.float64).reshape(-1, 3)
+        for k in range(n):
+            r = roles[k]
+            r.kind = int(role[k])
+            r.conductor = int(c[k])
+            r.eps_surrounding = float(eps[k])
+            r.eps_out = float(eps_out[k])
+            r.eps_in = float(eps_in[k])
+            r.reference[:] = [float(x) for x in ref[k]]
+        p = GpPanels(n, ncond, _dptr(v), _uptr(c))
+        Cm = np.zeros((ncond, ncond))
+        rep = GpCapReport()
+        self._check(self._L.gp_cap_solve_dielectric(self._h, C.byref(p), roles, _dptr(Cm),
+                                                     C.byref(rep)))
         return Cm, rep.as_dict()
 
     def cap_jacobian(self, verts4, cond, ncond, params, opts: GpCapOpts | None = None):

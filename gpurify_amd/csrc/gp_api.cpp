@@ -204,6 +204,24 @@ int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
   });
 }
 
+int gp_cap_solve_dielectric(gp_ctx *ctx, const gp_panels *panels, const gp_panel_role *roles,
+                            double *C, gp_cap_report *report) {
+  if (!ctx) {
+    g_thread_error = "gp_cap_solve_dielectric: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    if (!panels || !roles || !C) throw gp::Error(GP_ERR_INVALID_INPUT, "null panels, roles or C");
+    if (panels->n_panels == 0) throw gp::Error(GP_ERR_EMPTY, "empty problem");
+    if (report) *report = gp_cap_report{};
+    GP_HIP(hipSetDevice(ctx->device));
+    gp::PanelGeom g;
+    gp::panel_geometry(*panels, g);
+    gp::solve_dense_roles(ctx, g, roles, C, report);
+  });
+}
+
 int gp_cap_solve_batched(gp_ctx *ctx, const gp_panels *problems, uint32_t n_problems,
                          const gp_cap_opts *opts, double *C, gp_cap_report *reports) {
   if (!ctx) {

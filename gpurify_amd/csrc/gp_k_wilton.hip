@@ -247,6 +247,59 @@ __global__ __launch_bounds__(256) void k_dense_assemble(int n, const double *__r
   }
 }
 
+// Dielectric-interface rows of the mixed system (cap/dielectric.rs:91-101),
+// in the unit frame of the conductor rows: row k = drow[i] of the dense
+// matrix becomes -F(l, c_k, n_k) for l != k and diag[i] on the diagonal,
+// F being the normal-field integral over panel l (integrals/panel.rs:
+// 232-253, scale-free).  F: the degree-5 seven-point symmetric triangle rule
+// (barycentric points and weights of the published rule) on the fan
+// triangles of the quad; a triangle's repeated last vertex adds a
+// zero-area fan triangle, which is skipped.  One work-group per row.
+__constant__ double c_tri7[7][4] = {
+    {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0.225},
+    {0.059715871789770, 0.470142064105115, 0.470142064105115, 0.132394152788506},
+    {0.470142064105115, 0.059715871789770, 0.470142064105115, 0.132394152788506},
+    {0.470142064105115, 0.470142064105115, 0.059715871789770, 0.132394152788506},
+    {0.797426985353087, 0.101286507323456, 0.101286507323456, 0.125939180544827},
+    {0.101286507323456, 0.797426985353087, 0.101286507323456, 0.125939180544827},
+    {0.101286507323456, 0.101286507323456, 0.797426985353087, 0.125939180544827}};
+
+__device__ double normal_field(const double *__restrict__ q, double px, double py, double pz,
+                               double nx, double ny, double nz) {
+  double acc = 0.0;
+  for (int t = 1; t <= 2; ++t) {
+    const double *a = q, *b = q + 3 * t, *c = q + 3 * t + 3;
+    const double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+    const double vx = c[0] - a[0], vy = c[1] - a[1], vz = c[2] - a[2];
+    const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+    const double ar = sqrt(cx * cx + cy * cy + cz * cz) * 0.5;
+    if (ar == 0.0) continue;
+    for (int g = 0; g < 7; ++g) {
+      const double b0 = c_tri7[g][0], b1 = c_tri7[g][1], b2 = c_tri7[g][2];
+      const double dx = a[0] * b0 + b[0] * b1 + c[0] * b2 - px;
+      const double dy = a[1] * b0 + b[1] * b1 + c[1] * b2 - py;
+      const double dz = a[2] * b0 + b[2] * b1 + c[2] * b2 - pz;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > 0.0) acc += c_tri7[g][3] * ar * (dx * nx + dy * ny + dz * nz) / (r2 * sqrt(r2));
+    }
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_dielectric_rows(int n, const int *__restrict__ drow,
+                                                         const double *__restrict__ nrm,
+                                                         const double *__restrict__ diag,
+                                                         const double *__restrict__ verts,
+                                                         const double *__restrict__ cen,
+                                                         double *__restrict__ P) {
+  const int i = blockIdx.x, k = drow[i];
+  const double px = cen[3 * k], py = cen[3 * k + 1], pz = cen[3 * k + 2];
+  const double nx = nrm[3 * i], ny = nrm[3 * i + 1], nz = nrm[3 * i + 2];
+  double *row = P + (size_t)k * n;
+  for (int l = threadIdx.x; l < n; l += blockDim.x)
+    row[l] = l == k ? diag[i] : -normal_field(verts + 12L * l, px, py, pz, nx, ny, nz);
+}
+
 // Near-field tile of target leaf T: rows = T's panels, columns = the panels of
 // T and its neighbour leaves (cap/fmm_solver.rs:52-91), row-major at nf_off[T].
 // Writes the FP64 values and, when v32 != nullptr, their FP32 rounding.
@@ -333,6 +386,15 @@ void launch_dense_assemble(int n, const double *verts, const double *cen, double
   int blocks = (int)std::min<long>((total + 255) / 256, 65536);
   ProfScope ps("dense_assemble", s, 0.0, 8.0 * total);
   hipLaunchKernelGGL(k_dense_assemble, dim3(blocks), dim3(256), 0, s, n, verts, cen, P);
+  GP_LAUNCH_CHECK();
+}
+
+void launch_dielectric_rows(int n, int nd, const int *drow, const double *nrm, const double *diag,
+                            const double *verts, const double *cen, double *P, hipStream_t s) {
+  if (nd == 0) return;
+  ProfScope ps("dielectric_rows", s, 0.0, 8.0 * (double)nd * n);
+  hipLaunchKernelGGL(k_dielectric_rows, dim3(nd), dim3(256), 0, s, n, drow, nrm, diag, verts, cen,
+                     P);
   GP_LAUNCH_CHECK();
 }
 

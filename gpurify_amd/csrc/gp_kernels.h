@@ -9,6 +9,11 @@
 namespace gp {
 
 // ---------------------------------------------------------------- assembly
+// Dielectric-interface rows drow[0..nd) of a dense system (oriented normals
+// nrm, 3 per row; diagonal values diag), over the rows launch_dense_assemble
+// wrote.
+void launch_dielectric_rows(int n, int nd, const int *drow, const double *nrm, const double *diag,
+                            const double *verts, const double *cen, double *P, hipStream_t s);
 void launch_dense_assemble(int n, const double *verts, const double *cen, double *P,
                            hipStream_t s);
 void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_off,

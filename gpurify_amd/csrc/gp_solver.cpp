@@ -1746,6 +1746,104 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
 
 }  // namespace
 
+// cap::solve_dielectric (cap/dielectric.rs:47-151): conductor rows of
+// potential coefficients, dielectric-interface rows of normal fields with the
+// jump term on the diagonal, one LU, one solve per conductor, and the charge
+// of each conductor's panels weighted by the permittivity around them.  In
+// the unit frame (sigma' = k_e L0 sigma) the interface rows are scaled by L0:
+// -F(l, c_k, n_k) off the diagonal (scale-free) and
+// 2 pi (eps_out + eps_in) / (eps_out - eps_in) on it.
+void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles, double *C,
+                       gp_cap_report *rep) {
+  hipStream_t s = ctx->stream;
+  double t0 = now_s();
+  const int n = g.n, nc = g.ncond;
+  double c[3], L0;
+  root_cube(g, c, L0);
+  std::vector<double> hv, hc, ha;
+  std::vector<uint32_t> hcond;
+  scaled_arrays(g, nullptr, c, L0, hv, hc, ha, hcond);
+  std::vector<double> w(n, 0.0), B((size_t)n * nc, 0.0), hn, hd;
+  std::vector<int> drow, coff(nc + 1, 0), cidx;
+  for (int k = 0; k < n; ++k) {
+    const gp_panel_role &r = roles[k];
+    if (r.kind == GP_ROLE_CONDUCTOR) {
+      if (r.conductor >= (uint32_t)nc)
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " conductor out of range");
+      if (!std::isfinite(r.eps_surrounding))
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " has a non-finite eps");
+      w[k] = ha[k] * r.eps_surrounding;
+      B[(size_t)k * nc + r.conductor] = 1.0;
+      coff[r.conductor + 1]++;
+    } else if (r.kind == GP_ROLE_DIELECTRIC) {
+      if (!(std::isfinite(r.eps_out) && std::isfinite(r.eps_in)) || r.eps_out == r.eps_in)
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) +
+                                              ": interface permittivities must be finite and differ");
+      // Panel::normal (panel.rs:50-55) toward the outer side (dielectric.rs:39-46)
+      const double *q = &g.verts[(size_t)k * 12];
+      const double ux = q[3] - q[0], uy = q[4] - q[1], uz = q[5] - q[2];
+      const double vx = q[6] - q[0], vy = q[7] - q[1], vz = q[8] - q[2];
+      double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+      const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+      if (!(len > 0.0)) throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " has no normal");
+      nx /= len, ny /= len, nz /= len;
+      if (nx * (g.cx[k] - r.reference[0]) + ny * (g.cy[k] - r.reference[1]) +
+              nz * (g.cz[k] - r.reference[2]) < 0.0)
+        nx = -nx, ny = -ny, nz = -nz;
+      drow.push_back(k);
+      hn.insert(hn.end(), {nx, ny, nz});
+      hd.push_back(2.0 * kPi * (r.eps_out + r.eps_in) / (r.eps_out - r.eps_in));
+    } else {
+      throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " has an unknown role");
+    }
+  }
+  // conductor panels by conductor, ascending panel index (the summation order)
+  for (int j = 0; j < nc; ++j) coff[j + 1] += coff[j];
+  cidx.resize(coff[nc]);
+  {
+    std::vector<int> fill(coff.begin(), coff.end() - 1);
+    for (int k = 0; k < n; ++k)
+      if (roles[k].kind == GP_ROLE_CONDUCTOR) cidx[fill[roles[k].conductor]++] = k;
+  }
+  auto d_v = to_device(hv, s);
+  auto d_c = to_device(hc, s);
+  auto d_w = to_device(w, s);
+  auto d_B = to_device(B, s);
+  auto d_coff = to_device(coff, s);
+  auto d_cidx = to_device(cidx, s);
+  DBuf<double> d_P((size_t)n * n), d_work((size_t)n * nc), d_C((size_t)nc * nc);
+  DBuf<int> d_piv(n), d_info(1);
+  launch_dense_assemble(n, d_v.p, d_c.p, d_P.p, s);
+  if (!drow.empty()) {
+    auto d_dr = to_device(drow, s);
+    auto d_n = to_device(hn, s);
+    auto d_d = to_device(hd, s);
+    launch_dielectric_rows(n, (int)drow.size(), d_dr.p, d_n.p, d_d.p, d_v.p, d_c.p, d_P.p, s);
+    GP_HIP(hipStreamSynchronize(s));
+  }
+  GP_HIP(hipStreamSynchronize(s));
+  double t1 = now_s();
+  launch_lu_solve(n, nc, d_P.p, d_piv.p, d_B.p, d_work.p, d_info.p, s);
+  launch_cap_accumulate(nc, nc, nc, d_coff.p, d_cidx.p, d_w.p, d_B.p, d_C.p, s);
+  int info = 0;
+  GP_HIP(hipMemcpyAsync(&info, d_info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  std::vector<double> hC((size_t)nc * nc);
+  GP_HIP(hipMemcpyAsync(hC.data(), d_C.p, hC.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  if (info != 0) throw Error(GP_ERR_SINGULAR, "dielectric system is singular");
+  const double scale = 4.0 * kPi * kEps0 * L0;
+  for (size_t e = 0; e < hC.size(); ++e) C[e] = hC[e] * scale;
+  double t2 = now_s();
+  if (rep) {
+    rep->method_used = GP_METHOD_DENSE;
+    rep->precision_used = GP_PREC_FP64;
+    rep->converged = 1;
+    rep->seconds_setup = t1 - t0;
+    rep->seconds_solve = t2 - t1;
+  }
+}
+
+
 void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
                gp_cap_report *rep, const double *eps_panel) {
   double t0 = now_s();

@@ -28,6 +28,9 @@ namespace gp {
 // Maxwell C (ncond x ncond, farads, vacuum) of g.  eps_panel, when given,
 // weights each panel's charge by its surrounding permittivity (the bridge's
 // inhomogeneous conductor-role solve, dense only).
+// cap::solve_dielectric (cap/dielectric.rs:47-151), dense FP64 on the device.
+void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles, double *C,
+                       gp_cap_report *rep);
 void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, double *C,
                gp_cap_report *rep, const double *eps_panel);
 

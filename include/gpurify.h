@@ -250,6 +250,31 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
                          const gp_cap_opts *opts, uint32_t *n_nets, uint32_t *nets_out,
                          double *cap_af_out, double *C_out, gp_cap_report *report);
 
+/* ------------------------------------------------------------ dielectrics
+ * Panel roles of cap::dielectric::Problem (cap/dielectric.rs:10-14):
+ * a conductor panel (conductor id, eps_surrounding: the relative
+ * permittivity of the medium around it, weighting its charge), or a
+ * dielectric-interface panel (eps_out / eps_in: relative permittivities on
+ * the outer / inner side; reference: a point on the inner side, which
+ * orients the panel's normal outward, dielectric.rs:39-46). */
+enum { GP_ROLE_CONDUCTOR = 0, GP_ROLE_DIELECTRIC = 1 };
+typedef struct {
+  uint32_t kind;
+  uint32_t conductor;
+  double eps_surrounding;
+  double eps_out;
+  double eps_in;
+  double reference[3];
+} gp_panel_role;
+
+/* cap::solve_dielectric (cap/dielectric.rs:47-151): conductor rows of
+ * potential coefficients, interface rows of normal fields (7-point rule,
+ * integrals/panel.rs:213-253) with the jump (eps_out + eps_in) /
+ * (2 eps0 (eps_out - eps_in)) on the diagonal, dense FP64 LU on the device.
+ * roles: one per panel.  C: n_conductors^2 farads, permittivities applied. */
+int gp_cap_solve_dielectric(gp_ctx *ctx, const gp_panels *panels, const gp_panel_role *roles,
+                            double *C, gp_cap_report *report);
+
 /* ------------------------------------------------------------ FastCap input
  * FastCap "quickif" geometry text (.qui), as cap::geometry::parse
  * (crates/pex/src/quasistatic/cap/geometry.rs:56-135) reads it: Q (quad)

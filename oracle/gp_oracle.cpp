@@ -1401,6 +1401,52 @@ PanelSet set_from(const std::vector<Pan> &pans, const std::vector<uint32_t> &con
 }  // namespace
 
 // ================================================================ C ABI
+// cap/dielectric.rs:47-151: mixed conductor / dielectric-interface rows.
+// role[k] = 0 conductor (cond[k], eps[k] = eps_surrounding) or 1 dielectric
+// (eps_out[k], eps_in[k], reference point ref[3k..3k+2]).  C: ncond^2 F.
+int solve_dielectric(int n, const double *verts4, const int *role, const uint32_t *cond,
+                     const double *eps, const double *eps_out, const double *eps_in,
+                     const double *ref, int ncond, double *C) {
+  if (n == 0) return -6;
+  PanelSet s = build_set(n, verts4, cond);
+  const double eps0 = EPS0, k_e = ONE_OVER_4PI_EPS0;
+  // Panel::normal (panel.rs:50-55), flipped toward the outer side for
+  // dielectric panels (oriented_normal, dielectric.rs:39-46)
+  std::vector<V3> nrm(n);
+  for (int k = 0; k < n; ++k) {
+    const V3 *v = s.pans[k].v;
+    V3 nk = normalized(cross(v[1] - v[0], v[2] - v[0]));
+    if (role[k] == 1) {
+      V3 r{ref[3 * k], ref[3 * k + 1], ref[3 * k + 2]};
+      if (dot(nk, s.c[k] - r) < 0.0) nk = nk * -1.0;
+    }
+    nrm[k] = nk;
+  }
+  std::vector<double> A((size_t)n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int k = 0; k < n; ++k) {
+    double *row = &A[(size_t)k * n];
+    if (role[k] == 0) {
+      for (int l = 0; l < n; ++l) row[l] = k_e * potential_integral(s.pans[l].v, 4, s.c[k]);
+    } else {
+      for (int l = 0; l < n; ++l)
+        if (l != k) row[l] = -k_e * field_integral(s.pans[l].v, 4, s.c[k], nrm[k]);
+      row[k] = (eps_out[k] + eps_in[k]) / (2.0 * eps0 * (eps_out[k] - eps_in[k]));
+    }
+  }
+  LU lu;
+  if (!lu.factor(A.data(), n)) return -1;
+  std::fill(C, C + (size_t)ncond * ncond, 0.0);
+  std::vector<double> b(n), sig(n);
+  for (int j = 0; j < ncond; ++j) {
+    for (int k = 0; k < n; ++k) b[k] = (role[k] == 0 && (int)cond[k] == j) ? 1.0 : 0.0;
+    lu.solve(b.data(), sig.data());
+    for (int k = 0; k < n; ++k)
+      if (role[k] == 0) C[(size_t)cond[k] * ncond + j] += eps[k] * sig[k] * s.area[k];
+  }
+  return 0;
+}
+
 extern "C" {
 
 double or_potential_integral(const double *verts, int nv, const double *p) {
@@ -1684,6 +1730,12 @@ int or_get_threads(void) {
 #else
   return 1;
 #endif
+}
+
+int or_solve_dielectric(int n, const double *verts4, const int *role, const uint32_t *cond,
+                        const double *eps, const double *eps_out, const double *eps_in,
+                        const double *ref, int ncond, double *C) {
+  return solve_dielectric(n, verts4, role, cond, eps, eps_out, eps_in, ref, ncond, C);
 }
 
 }  // extern "C"

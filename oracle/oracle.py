@@ -46,6 +46,7 @@ def lib():
         L.or_assemble_p.argtypes = [C.c_int, _d, _d]
         L.or_lu_solve.argtypes = [C.c_int, _d, C.c_int, _d]
         L.or_cap_solve_dense.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, _d, _d]
+        L.or_solve_dielectric.argtypes = [C.c_int, _d, _int, _u32, _d, _d, _d, _d, C.c_int, _d]
         L.or_cap_solve_fmm.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, C.c_int,
                                        C.c_double, C.c_int, _d, _int, _d]
         L.or_cap_solve_fmm_cols.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, C.c_int,
@@ -126,6 +127,24 @@ def cap_dense(verts4, cond, ncond, method=0, want_p=False):
     if rc:
         raise RuntimeError(f"oracle dense solve failed ({rc})")
     return (Cm, P) if want_p else Cm
+
+
+def solve_dielectric(verts4, role, cond, eps, eps_out, eps_in, ref, ncond):
+    """cap/dielectric.rs:47-151 (C in farads, permittivities relative): role 0
+    conductor (cond, eps = eps_surrounding), 1 dielectric interface (eps_out,
+    eps_in, ref = a point on the inner side, (n, 3))."""
+    v = _f64(verts4).reshape(-1, 4, 3)
+    n = v.shape[0]
+    r = np.ascontiguousarray(role, dtype=np.int32)
+    c = np.ascontiguousarray(cond, dtype=np.uint32)
+    e, eo, ei = _f64(eps), _f64(eps_out), _f64(eps_in)
+    rf = _f64(ref).reshape(-1, 3)
+    Cm = np.zeros((ncond, ncond))
+    rc = lib().or_solve_dielectric(n, _p(v), r.ctypes.data_as(_int), _p(c, _u32), _p(e), _p(eo),
+                                   _p(ei), _p(rf), ncond, _p(Cm))
+    if rc:
+        raise RuntimeError(f"oracle dielectric solve failed ({rc})")
+    return Cm
 
 
 def cap_fmm(verts4, cond, ncond, order=4, max_level=4, tol=1e-8, m2l_mode=0):

@@ -1,0 +1,76 @@
+"""cap::solve_dielectric on the device (gp_cap_solve_dielectric: interface
+rows by k_dielectric_rows, dense FP64 LU) against the oracle restatement of
+cap/dielectric.rs:47-151 on the same panels."""
+import numpy as np
+import pytest
+
+import gpurify_amd as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EPS0 = 8.8541878128e-12
+
+
+def coated_sphere(subdiv, a=1e-6, b=2e-6, er=4.0):
+    vc = O.mesh_icosphere((0, 0, 0), a, subdiv)
+    vd = O.mesh_icosphere((0, 0, 0), b, subdiv)
+    n1, n2 = len(vc), len(vd)
+    v = np.concatenate([vc, vd])
+    role = np.r_[np.zeros(n1), np.ones(n2)].astype(int)
+    cond = np.zeros(n1 + n2, dtype=np.uint32)
+    eps = np.r_[np.full(n1, er), np.zeros(n2)]
+    eo = np.r_[np.zeros(n1), np.ones(n2)]
+    ei = np.r_[np.zeros(n1), np.full(n2, er)]
+    return v, role, cond, eps, eo, ei, np.zeros((n1 + n2, 3))
+
+
+def test_coated_sphere_matches_the_oracle(ctx):
+    prob = coated_sphere(2)
+    print(f"coated sphere: {len(prob[0])} panels", flush=True)
+    Cg, rep = ctx.cap_solve_dielectric(*prob, 1)
+    Co = O.solve_dielectric(*prob, 1)
+    assert abs(Cg[0, 0] - Co[0, 0]) <= 1e-10 * Co[0, 0]
+    exact = 4 * np.pi * EPS0 / ((1 / 4.0) * (1 / 1e-6 - 1 / 2e-6) + 1 / 2e-6)
+    assert abs(Cg[0, 0] - exact) / exact < 0.05
+
+
+def test_two_conductors_in_a_dielectric_box_match_the_oracle(ctx):
+    # two spheres inside a dielectric cube (eps 3.9 inside, 1 outside), the
+    # cube's faces as interface panels with their reference point inside
+    s1 = O.mesh_icosphere((-1.5e-6, 0, 0), 1e-6, 1)
+    s2 = O.mesh_icosphere((1.5e-6, 0, 0), 0.7e-6, 1)
+    faces = []
+    L, m = 3.5e-6, 4
+    g = np.linspace(-L, L, m + 1)
+    for ax in range(3):
+        for side in (-L, L):
+            for i in range(m):
+                for j in range(m):
+                    q = np.zeros((4, 3))
+                    a0, a1 = [(ax + 1) % 3, (ax + 2) % 3]
+                    for k, (u, w) in enumerate([(g[i], g[j]), (g[i + 1], g[j]),
+                                                (g[i + 1], g[j + 1]), (g[i], g[j + 1])]):
+                        q[k, ax], q[k, a0], q[k, a1] = side, u, w
+                    faces.append(q)
+    box = np.array(faces)
+    v = np.concatenate([s1, s2, box])
+    n1, n2, nb = len(s1), len(s2), len(box)
+    role = np.r_[np.zeros(n1 + n2), np.ones(nb)].astype(int)
+    cond = np.r_[np.zeros(n1), np.ones(n2), np.zeros(nb)].astype(np.uint32)
+    eps = np.r_[np.full(n1 + n2, 3.9), np.zeros(nb)]
+    eo = np.r_[np.zeros(n1 + n2), np.ones(nb)]
+    ei = np.r_[np.zeros(n1 + n2), np.full(nb, 3.9)]
+    ref = np.zeros((len(v), 3))
+    print(f"dielectric box: {len(v)} panels", flush=True)
+    Cg, _ = ctx.cap_solve_dielectric(v, role, cond, eps, eo, ei, ref, 2)
+    Co = O.solve_dielectric(v, role, cond, eps, eo, ei, ref, 2)
+    assert np.abs(Cg - Co).max() <= 1e-10 * np.abs(Co).max()
+    assert Cg[0, 1] < 0 and Cg[0, 0] > 0 and abs(Cg[0, 1] - Cg[1, 0]) < 0.02 * abs(Cg[0, 1])
+
+
+def test_interface_with_equal_permittivities_is_rejected(ctx):
+    v, role, cond, eps, eo, ei, ref = coated_sphere(1)
+    ei = np.where(role == 1, 1.0, ei)
+    with pytest.raises(G.GpError, match="INVALID_INPUT"):
+        ctx.cap_solve_dielectric(v, role, cond, eps, eo, ei, ref, 1)
