@@ -159,3 +159,59 @@ def test_triangles_round_trip():
     assert np.array_equal(g["vertices"], v) and np.array_equal(g["conductor"], c)
     assert np.array_equal(g["n_vertices"] == 3, np.arange(20) % 3 == 0)
     assert g["names"] == [f"cond{i}" for i in range(1, 5)]
+
+
+# ---------------------------------------------------------------- list files
+def _coated_sphere_files(tmp_path, tr=(0.0, 0.0, 0.0)):
+    from oracle import oracle as O
+    (tmp_path / "sphere.qui").write_text(write_qui(O.mesh_icosphere((0, 0, 0), 1.0, 2),
+                                                   np.zeros(320, np.uint32)))
+    (tmp_path / "shell.qui").write_text(write_qui(O.mesh_icosphere((0, 0, 0), 2.0, 2),
+                                                  np.zeros(320, np.uint32)))
+    t = " ".join(repr(x) for x in tr)
+    lst = tmp_path / "coated.lst"
+    lst.write_text(f"* coated sphere (crates/pex/tests/dielectric.rs:8-45)\n"
+                   f"C sphere.qui 4.0 {t}\n\n% shell: vacuum outside, eps 4 inside\n"
+                   f"D shell.qui 1.0 4.0 0 0 0 {t} +\n")
+    return str(lst)
+
+
+def test_list_file_roles_and_translation(tmp_path):
+    # listfile.rs:56-109: C panels are conductors of eps outerperm, numbered
+    # "<file>:<local>" per C line; D panels interfaces with the reference
+    # point translated with them; a trailing +/- flag is ignored
+    from gpurify_amd.fastcap import parse_lst
+    g = parse_lst(_coated_sphere_files(tmp_path, tr=(1.0, -2.0, 0.5)))
+    assert g["names"] == ["sphere.qui:0"]
+    assert (g["role"] == 0).sum() == 320 and (g["role"] == 1).sum() == 320
+    assert np.all(g["eps"][g["role"] == 0] == 4.0)
+    assert np.all(g["eps_out"][g["role"] == 1] == 1.0) and np.all(g["eps_in"][g["role"] == 1] == 4.0)
+    assert np.allclose(g["ref"][g["role"] == 1], [1.0, -2.0, 0.5])
+    assert np.allclose(g["vertices"][g["role"] == 0].mean(axis=(0, 1)), [1.0, -2.0, 0.5], atol=1e-12)
+
+
+def test_list_file_coated_sphere_solves_to_the_closed_form(tmp_path):
+    from gpurify_amd.fastcap import parse_lst
+    from oracle import oracle as O
+    g = parse_lst(_coated_sphere_files(tmp_path))
+    C = O.solve_dielectric(g["vertices"], g["role"], g["conductor"], g["eps"], g["eps_out"],
+                           g["eps_in"], g["ref"], 1)
+    eps0 = 8.8541878128e-12
+    exact = 4 * np.pi * eps0 / ((1 / 4.0) * (1 / 1.0 - 1 / 2.0) + 1 / 2.0)
+    assert abs(C[0, 0] - exact) / exact < 0.05
+
+
+@pytest.mark.parametrize("body,msg", [
+    ("X a.qui 1\n", "line 1: unrecognized list line: X a.qui 1"),
+    ("C\n", "line 1: C: missing file"),
+    ("C sphere.qui 4.0 0 0\n", "line 1: expected number for ztrans"),
+    ("D shell.qui 1.0 0x4 0 0 0 0 0 0\n", "line 1: expected number for inner permittivity"),
+    ("C missing.qui 1 0 0 0\n", "cannot read"),
+])
+def test_list_file_errors(tmp_path, body, msg):
+    from gpurify_amd.fastcap import ListFileError, parse_lst
+    _coated_sphere_files(tmp_path)
+    p = tmp_path / "bad.lst"
+    p.write_text(body)
+    with pytest.raises(ListFileError, match=msg):
+        parse_lst(str(p))

@@ -74,3 +74,20 @@ def test_interface_with_equal_permittivities_is_rejected(ctx):
     ei = np.where(role == 1, 1.0, ei)
     with pytest.raises(G.GpError, match="INVALID_INPUT"):
         ctx.cap_solve_dielectric(v, role, cond, eps, eo, ei, ref, 1)
+
+
+def test_list_file_problem_on_the_device(ctx, tmp_path):
+    """A FastCap list file (C conductor file + D interface file, translated)
+    read by parse_lst (cap/listfile.rs:35-115) and solved on the device."""
+    from gpurify_amd.fastcap import parse_lst, write_qui
+    (tmp_path / "s.qui").write_text(write_qui(O.mesh_icosphere((0, 0, 0), 1e-6, 2),
+                                              np.zeros(320, np.uint32)))
+    (tmp_path / "b.qui").write_text(write_qui(O.mesh_icosphere((0, 0, 0), 2e-6, 2),
+                                              np.zeros(320, np.uint32)))
+    (tmp_path / "p.lst").write_text("C s.qui 4.0 1e-6 0 0\nD b.qui 1.0 4.0 0 0 0 1e-6 0 0\n")
+    g = parse_lst(str(tmp_path / "p.lst"))
+    args = (g["vertices"], g["role"], g["conductor"], g["eps"], g["eps_out"], g["eps_in"], g["ref"], 1)
+    print(f"list file: {len(g['role'])} panels", flush=True)
+    Cg, _ = ctx.cap_solve_dielectric(*args)
+    Co = O.solve_dielectric(*args)
+    assert abs(Cg[0, 0] - Co[0, 0]) <= 1e-10 * Co[0, 0]
