@@ -18,6 +18,8 @@
 
 #include "gp_solver.h"
 
+#include <rocsolver/rocsolver.h>
+
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -1753,6 +1755,10 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
 // the unit frame (sigma' = k_e L0 sigma) the interface rows are scaled by L0:
 // -F(l, c_k, n_k) off the diagonal (scale-free) and
 // 2 pi (eps_out + eps_in) / (eps_out - eps_in) on it.
+// Above this size the one-work-group LU is slower than the host; the
+// dielectric solve then factors with rocSOLVER.
+constexpr int kOneGroupLU = 1024;
+
 void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles, double *C,
                        gp_cap_report *rep) {
   hipStream_t s = ctx->stream;
@@ -1807,12 +1813,7 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
   }
   auto d_v = to_device(hv, s);
   auto d_c = to_device(hc, s);
-  auto d_w = to_device(w, s);
-  auto d_B = to_device(B, s);
-  auto d_coff = to_device(coff, s);
-  auto d_cidx = to_device(cidx, s);
-  DBuf<double> d_P((size_t)n * n), d_work((size_t)n * nc), d_C((size_t)nc * nc);
-  DBuf<int> d_piv(n), d_info(1);
+  DBuf<double> d_P((size_t)n * n);
   launch_dense_assemble(n, d_v.p, d_c.p, d_P.p, s);
   if (!drow.empty()) {
     auto d_dr = to_device(drow, s);
@@ -1823,14 +1824,55 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
   }
   GP_HIP(hipStreamSynchronize(s));
   double t1 = now_s();
-  launch_lu_solve(n, nc, d_P.p, d_piv.p, d_B.p, d_work.p, d_info.p, s);
-  launch_cap_accumulate(nc, nc, nc, d_coff.p, d_cidx.p, d_w.p, d_B.p, d_C.p, s);
-  int info = 0;
-  GP_HIP(hipMemcpyAsync(&info, d_info.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  std::vector<double> hC((size_t)nc * nc);
-  GP_HIP(hipMemcpyAsync(hC.data(), d_C.p, hC.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  GP_HIP(hipStreamSynchronize(s));
-  if (info != 0) throw Error(GP_ERR_SINGULAR, "dielectric system is singular");
+  std::vector<double> hC((size_t)nc * nc, 0.0);
+  if (n <= kOneGroupLU) {
+    // one work-group: the LU of the dense conductor path
+    auto d_w = to_device(w, s);
+    auto d_B = to_device(B, s);
+    auto d_coff = to_device(coff, s);
+    auto d_cidx = to_device(cidx, s);
+    DBuf<double> d_work((size_t)n * nc), d_C((size_t)nc * nc);
+    DBuf<int> d_piv(n), d_info(1);
+    launch_lu_solve(n, nc, d_P.p, d_piv.p, d_B.p, d_work.p, d_info.p, s);
+    launch_cap_accumulate(nc, nc, nc, d_coff.p, d_cidx.p, d_w.p, d_B.p, d_C.p, s);
+    int info = 0;
+    GP_HIP(hipMemcpyAsync(&info, d_info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipMemcpyAsync(hC.data(), d_C.p, hC.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    if (info != 0) throw Error(GP_ERR_SINGULAR, "dielectric system is singular");
+  } else {
+    // larger systems: rocSOLVER's blocked LU over the whole device.  The
+    // row-major matrix is the column-major A^T, so getrs solves with the
+    // transpose of that factorization.
+    std::vector<double> Bc((size_t)n * nc);
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < nc; ++j) Bc[(size_t)j * n + k] = B[(size_t)k * nc + j];
+    auto d_B = to_device(Bc, s);
+    DBuf<int> d_piv(n), d_info(1);
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      throw Error(GP_ERR_INTERNAL, "rocblas_create_handle failed");
+    rocblas_status st = rocblas_set_stream(h, s);
+    if (st == rocblas_status_success) st = rocsolver_dgetrf(h, n, n, d_P.p, n, d_piv.p, d_info.p);
+    if (st == rocblas_status_success)
+      st = rocsolver_dgetrs(h, rocblas_operation_transpose, n, nc, d_P.p, n, d_piv.p, d_B.p, n);
+    int info = 0;
+    if (st == rocblas_status_success) {
+      GP_HIP(hipMemcpyAsync(&info, d_info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      GP_HIP(hipMemcpyAsync(Bc.data(), d_B.p, Bc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      GP_HIP(hipStreamSynchronize(s));
+    }
+    rocblas_destroy_handle(h);
+    if (st != rocblas_status_success)
+      throw Error(GP_ERR_INTERNAL, "rocSOLVER getrf/getrs failed (" + std::to_string((int)st) + ")");
+    if (info != 0) throw Error(GP_ERR_SINGULAR, "dielectric system is singular");
+    // charges of each conductor's panels in ascending panel order
+    for (int i = 0; i < nc; ++i)
+      for (int q = coff[i]; q < coff[i + 1]; ++q) {
+        const int k = cidx[q];
+        for (int j = 0; j < nc; ++j) hC[(size_t)i * nc + j] += Bc[(size_t)j * n + k] * w[k];
+      }
+  }
   const double scale = 4.0 * kPi * kEps0 * L0;
   for (size_t e = 0; e < hC.size(); ++e) C[e] = hC[e] * scale;
   double t2 = now_s();

@@ -35,6 +35,15 @@ def test_coated_sphere_matches_the_oracle(ctx):
     assert abs(Cg[0, 0] - exact) / exact < 0.05
 
 
+def test_large_system_takes_the_blocked_lu(ctx):
+    """2,560 panels: above the one-work-group LU, factored by rocSOLVER."""
+    prob = coated_sphere(3)
+    print(f"coated sphere: {len(prob[0])} panels (blocked LU)", flush=True)
+    Cg, rep = ctx.cap_solve_dielectric(*prob, 1)
+    Co = O.solve_dielectric(*prob, 1)
+    assert abs(Cg[0, 0] - Co[0, 0]) <= 1e-10 * Co[0, 0]
+
+
 def test_two_conductors_in_a_dielectric_box_match_the_oracle(ctx):
     # two spheres inside a dielectric cube (eps 3.9 inside, 1 outside), the
     # cube's faces as interface panels with their reference point inside
