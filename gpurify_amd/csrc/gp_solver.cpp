@@ -206,8 +206,23 @@ void cond_csr(const PanelGeom &g, const uint32_t *perm, std::vector<int> &off,
 }
 
 // ------------------------------------------------------------------ dense
+// Above this size the one-work-group LU is slower than the host: dense
+// systems then factor with rocSOLVER (solve_dense_roles).
+constexpr int kOneGroupLU = 1024;
+
 void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
                  const double *eps_panel) {
+  if (g.n > kOneGroupLU) {
+    std::vector<gp_panel_role> roles(g.n);
+    for (int i = 0; i < g.n; ++i) {
+      roles[i] = gp_panel_role{};
+      roles[i].kind = GP_ROLE_CONDUCTOR;
+      roles[i].conductor = g.cond[i];
+      roles[i].eps_surrounding = eps_panel ? eps_panel[i] : 1.0;
+    }
+    solve_dense_roles(ctx, g, roles.data(), C, rep);
+    return;
+  }
   hipStream_t s = ctx->stream;
   double t0 = now_s();
   const int n = g.n, nc = g.ncond;
@@ -1755,10 +1770,6 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
 // the unit frame (sigma' = k_e L0 sigma) the interface rows are scaled by L0:
 // -F(l, c_k, n_k) off the diagonal (scale-free) and
 // 2 pi (eps_out + eps_in) / (eps_out - eps_in) on it.
-// Above this size the one-work-group LU is slower than the host; the
-// dielectric solve then factors with rocSOLVER.
-constexpr int kOneGroupLU = 1024;
-
 void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles, double *C,
                        gp_cap_report *rep) {
   hipStream_t s = ctx->stream;
