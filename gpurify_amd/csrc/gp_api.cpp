@@ -214,6 +214,8 @@ int gp_cap_solve_dielectric(gp_ctx *ctx, const gp_panels *panels, const gp_panel
   return guarded(ctx, [&] {
     if (!panels || !roles || !C) throw gp::Error(GP_ERR_INVALID_INPUT, "null panels, roles or C");
     if (panels->n_panels == 0) throw gp::Error(GP_ERR_EMPTY, "empty problem");
+    if (panels->n_conductors == 0)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "a dielectric problem needs at least one conductor");
     if (report) *report = gp_cap_report{};
     GP_HIP(hipSetDevice(ctx->device));
     gp::PanelGeom g;
