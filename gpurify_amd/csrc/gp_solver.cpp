@@ -573,7 +573,8 @@ struct FmmSystem {
     phase("panels");
     // Near-field tiles (cap/fmm_solver.rs:52-91 over bbfmm.rs:189-206), per
     // global leaf: its panels against the panels of its near leaves.
-    std::vector<int> hcol_off(n_leaves + 1, 0), hcols, hpl(N);
+    std::vector<int> hcol_off(n_leaves + 1, 0), hpl(N);
+    std::unique_ptr<int[]> hcols;  // every entry written by its leaf's thread
     std::vector<int64_t> hnf(n_leaves + 1, 0);
     std::vector<std::array<int, 3>> trent;  // (S, T, column start of S in T's tile)
     auto leaf_of = [&](int T, int &p, int &lt) {
@@ -601,7 +602,7 @@ struct FmmSystem {
         hnf[T + 1] = hnf[T] + (int64_t)nt_ * ccount[T];
       }
     }
-    hcols.resize(hcol_off[n_leaves]);
+    hcols.reset(new int[std::max(1, hcol_off[n_leaves])]);
     parallel_for(n_leaves, [&](int T) {
       int p, lt;
       leaf_of(T, p, lt);
@@ -652,7 +653,7 @@ struct FmmSystem {
     leaf_lo = to_device(hleaf_lo, s);
     leaf_box = to_device(hleaf_box, s);
     col_off = to_device(hcol_off, s);
-    cols = to_device(hcols, s);
+    cols = to_device(hcols.get(), (size_t)hcol_off[n_leaves], s);
     panel_leaf = to_device(hpl, s);
     nf_off = to_device(hnf, s);
     nf64.alloc((size_t)nnz);
@@ -899,8 +900,13 @@ struct FmmSystem {
         heoff[T + 1] = cnt;
       });
       for (int T = 0; T < n_tiles; ++T) heoff[T + 1] += heoff[T];
-      std::vector<int> heo(heoff[n_tiles]), hes((size_t)heoff[n_tiles] * TS, -1);
+      // (entries filled with -1 by each tile's own thread: no serial fill of
+      // the whole array)
+      std::vector<int> heo(heoff[n_tiles]);
+      const size_t n_es = (size_t)heoff[n_tiles] * TS;
+      std::unique_ptr<int[]> hes(new int[n_es]);
       parallel_for(n_tiles, [&](int T) {
+        std::fill(hes.get() + (size_t)heoff[T] * TS, hes.get() + (size_t)heoff[T + 1] * TS, -1);
         int slot[343];
         std::fill(slot, slot + 343, -1);
         for (int j = 0; j < tcount[T]; ++j) {
@@ -922,7 +928,7 @@ struct FmmSystem {
       tile_tgt = to_device(htt, s);
       tile_eoff = to_device(heoff, s);
       ent_oidx = to_device(heo, s);
-      ent_src = to_device(hes, s);
+      ent_src = to_device(hes.get(), n_es, s);
     }
     phase("m2l lists");
     // 1/half_l in units of each problem's own root half-width.
