@@ -753,7 +753,8 @@ struct FmmSystem {
     phase("tables");
     // Box arrays in the global numbering.
     std::vector<int> hlevel(nb), hparent(nb), hoct(nb), hchild_off(nb + 1, 0), hchild, htargets;
-    std::vector<int> hil_off(nb + 1, 0), hil_src, hil_oidx;
+    std::vector<int> hil_off(nb + 1, 0);
+    std::unique_ptr<int[]> hil_src, hil_oidx;  // every entry written by its box's thread
     for (int l = 0; l <= L; ++l)
       for (int b = lvl_off[l]; b < lvl_off[l + 1]; ++b) hlevel[b] = l;
     parallel_for(nb, [&](int b) {
@@ -768,8 +769,9 @@ struct FmmSystem {
       hil_off[b + 1] += hil_off[b];
     }
     hchild.resize(hchild_off[nb]);
-    hil_src.resize(hil_off[nb]);
-    hil_oidx.resize(hil_off[nb]);
+    const size_t n_il = (size_t)hil_off[nb];
+    hil_src.reset(new int[std::max<size_t>(1, n_il)]);
+    hil_oidx.reset(new int[std::max<size_t>(1, n_il)]);
     parallel_for(nb, [&](int b) {
       const int p = gp_of[b], lb = gb_of[b];
       const Tree &t = trees[p];
@@ -784,7 +786,7 @@ struct FmmSystem {
       }
     });
     phase("boxes");
-    pairs = (long)hil_src.size();
+    pairs = (long)n_il;
     if (need32 && m2l_inner == GP_M2L_BF16 && !want_by_offset) {
       // Sibling pairs for the multi-RHS BF16 M2L: consecutive targets with
       // one parent, the union of their lists ordered by source box.
@@ -840,7 +842,7 @@ struct FmmSystem {
       // FP64 single-RHS M2L beyond the 64-target tiles: the pairs grouped by
       // offset (counting sort; targets stay ascending)
       po_off.assign(344, 0);
-      for (int v : hil_oidx) po_off[v + 1]++;
+      for (size_t q = 0; q < n_il; ++q) po_off[hil_oidx[q] + 1]++;
       for (int o = 0; o < 343; ++o) po_off[o + 1] += po_off[o];
       std::vector<int> ht(pairs), hs(pairs), fillp(po_off.begin(), po_off.end() - 1);
       for (int b = 0; b < nb; ++b)
@@ -940,8 +942,8 @@ struct FmmSystem {
     child_idx = to_device(hchild, s);
     targets = to_device(htargets, s);
     il_off = to_device(hil_off, s);
-    il_src = to_device(hil_src, s);
-    il_oidx = to_device(hil_oidx, s);
+    il_src = to_device(hil_src.get(), n_il, s);
+    il_oidx = to_device(hil_oidx.get(), n_il, s);
     box_level = to_device(hlevel, s);
     inv_half = to_device(hih, s);
     {
