@@ -16,6 +16,7 @@
 // source multipole and its interaction matrix through 16x16x32 BF16 (or
 // 16x16x4 FP64) matrix-core products.
 
+#include <cstdlib>
 #include <type_traits>
 
 #include "gp_common.h"
@@ -1831,6 +1832,123 @@ __global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo
   }
 }
 
+// GPURIFY_FUSE_LEAF=0 keeps the finest L2L and L2P as two launches (the
+// unfused pair computes the same sums; for comparisons and timing).
+static bool fuse_leaf_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("GPURIFY_FUSE_LEAF");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// The finest L2L and L2P fused (FP32 blocks of RHS, every leaf at the
+// finest level): one work-group per leaf, per 32-column chunk the transposed
+// 1-D interpolations of the parent's locals added to the leaf's own (the
+// sums of k_l2l_tiled) into LDS, then the tensor-product sum of each panel
+// over the chunk's locals (the sums of k_l2p_wave) added to y.  The leaf's
+// final locals never go back to memory: k_l2l_tiled wrote them and
+// k_l2p_wave read them again.
+template <int N>
+__global__ __launch_bounds__(256) void k_l2l_l2p_leaf(int leaf0, const int *__restrict__ parent,
+                                                      const int *__restrict__ octant,
+                                                      const float *__restrict__ m2m1d,
+                                                      const int *__restrict__ leaf_lo,
+                                                      const float *__restrict__ w1d, int nr,
+                                                      const float *__restrict__ Lc, int ldm,
+                                                      double *__restrict__ y, int ldy) {
+  constexpr int NN = N * N, N3 = N * N * N, PC = 16, NW = PC * 3 * N;
+  constexpr int ITEMS = (TM_RC * NN + 255) / 256;
+  __shared__ float t2[TM_RC * N * NN];
+  __shared__ float sL[N3 * TM_RC];
+  __shared__ float sW[NW];
+  __shared__ float sS[PC][N3];
+  const int li = blockIdx.x, ch = leaf0 + li, tid = threadIdx.x;
+  const int p = parent[ch], o = octant[ch];
+  const float *mx = m2m1d + (o * 3 + 0) * NN, *my = m2m1d + (o * 3 + 1) * NN,
+              *mz = m2m1d + (o * 3 + 2) * NN;
+  const float *lp = Lc + (size_t)p * N3 * ldm;
+  const float *lc = Lc + (size_t)ch * N3 * ldm;
+  const int pb = leaf_lo[li], pe = leaf_lo[li + 1];
+  for (int r0 = 0; r0 < nr; r0 += TM_RC) {
+    const int rc = min(TM_RC, nr - r0);
+    __syncthreads();  // the previous chunk's t2 and sL are no longer read
+    float cur[ITEMS][N];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int item = tid + 256 * k;
+      const int r = item % rc, bc = item / rc;
+#pragma unroll
+      for (int ap = 0; ap < N; ++ap)
+        cur[k][ap] = item < rc * NN ? lc[(size_t)(ap * NN + bc) * ldm + r0 + r] : 0.f;
+    }
+    if (tid < rc * N) {
+      const int r = tid % rc, a = tid / rc;
+      float in[NN], u[NN];
+#pragma unroll
+      for (int e = 0; e < NN; ++e) in[e] = lp[(size_t)(a * NN + e) * ldm + r0 + r];
+#pragma unroll
+      for (int bb = 0; bb < N; ++bb)
+#pragma unroll
+        for (int cp = 0; cp < N; ++cp) {
+          float acc = 0;
+#pragma unroll
+          for (int c = 0; c < N; ++c) acc += mz[c * N + cp] * in[bb * N + c];
+          u[bb * N + cp] = acc;
+        }
+      float *dst = t2 + ((size_t)r * N + a) * NN;
+#pragma unroll
+      for (int bp = 0; bp < N; ++bp)
+#pragma unroll
+        for (int cp = 0; cp < N; ++cp) {
+          float acc = 0;
+#pragma unroll
+          for (int bb = 0; bb < N; ++bb) acc += my[bb * N + bp] * u[bb * N + cp];
+          dst[bp * N + cp] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int item = tid + 256 * k;
+      if (item < rc * NN) {
+        const int r = item % rc, bc = item / rc;
+        float tv[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) tv[a] = t2[((size_t)r * N + a) * NN + bc];
+#pragma unroll
+        for (int ap = 0; ap < N; ++ap) {
+          float acc = 0;
+#pragma unroll
+          for (int a = 0; a < N; ++a) acc += mx[a * N + ap] * tv[a];
+          sL[(ap * NN + bc) * TM_RC + r] = cur[k][ap] + acc;
+        }
+      }
+    }
+    // L2P of the chunk: 16 panels at a time, thread = (panel, column) pairs
+    for (int p0 = pb; p0 < pe; p0 += PC) {
+      const int np = min(PC, pe - p0);
+      __syncthreads();  // sL complete; the previous panels' weights no longer read
+      for (int e = tid; e < NW; e += 256) sW[e] = e < np * 3 * N ? w1d[(size_t)p0 * 3 * N + e] : 0.f;
+      __syncthreads();
+      for (int e = tid; e < PC * N3; e += 256) {
+        const int pp = e / N3, m = e - pp * N3;
+        const float *w = sW + pp * 3 * N;
+        sS[pp][m] = w[m / (N * N)] * w[N + (m / N) % N] * w[2 * N + m % N];
+      }
+      __syncthreads();
+      for (int e = tid; e < np * TM_RC; e += 256) {
+        const int pp = e / TM_RC, r = e - pp * TM_RC;
+        if (r >= rc) continue;
+        float acc = 0.f;
+#pragma unroll 5
+        for (int m = 0; m < N3; ++m) acc += sS[pp][m] * sL[m * TM_RC + r];
+        y[(size_t)(p0 + pp) * ldy + r0 + r] += (double)acc;
+      }
+    }
+  }
+}
+
 // M2M with the order N fixed at compile time: per parent box and 32-column
 // RHS chunk, for each child the z and y contractions of every (r, a) slice
 // run in registers, one LDS exchange, then the x contraction accumulates the
@@ -2681,10 +2799,17 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       }
     }
   }
+  // The finest L2L fused with L2P (FP32 blocks of RHS, every leaf at the
+  // finest level, no batch leaf map; orders up to 5, whose work-group
+  // needs 41 KB of LDS).
+  const bool fuse_leaf = sizeof(T) == 4 && tiled && !f.leaf_box && f.panel_leaf && ldm % 4 == 0 &&
+                         ldm >= 16 && n >= 3 && n <= 5 && L >= 3 && nr > 1 &&
+                         f.lvl_off[L + 1] - f.lvl_off[L] == nleaf && fuse_leaf_enabled();
   // L2L, level 2 to the leaves.
   for (int l = 2; l < L; ++l) {
     int nb = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     if (nb == 0) continue;
+    if (fuse_leaf && l == L - 1) continue;  // in k_l2l_l2p_leaf
     ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nr * nb, 2.0 * nb * exp_bytes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
       const int tpb = round_up(n3, 64), bpb = 256 / tpb;
@@ -2713,6 +2838,23 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                          f.parent, f.octant, f.m2m1d, nr, Lc, ldm);
     }
     GP_LAUNCH_CHECK();
+  }
+  if (fuse_leaf) {
+    if constexpr (sizeof(T) == 4) {
+      ProfScope pf("l2l_l2p_f32", s, 2.0 * f.n_panels * n3 * nr + 6.0 * n * n3 * nr * nleaf,
+                   (double)f.n_panels * nr * 8.0 + 2.0 * nleaf * exp_bytes);
+      switch (n) {
+#define GP_L2LP(NF)                                                                               \
+  hipLaunchKernelGGL((k_l2l_l2p_leaf<NF>), dim3(nleaf), dim3(256), 0, s, leaf0, f.parent,        \
+                     f.octant, f.m2m1d, f.leaf_lo, f.w1d, nr, Lc, ldm, y, ldy)
+        case 3: GP_L2LP(3); break;
+        case 4: GP_L2LP(4); break;
+        default: GP_L2LP(5); break;
+#undef GP_L2LP
+      }
+      GP_LAUNCH_CHECK();
+    }
+    return;
   }
   ProfScope pp(f32 ? "l2p_f32" : "l2p_f64", s, 2.0 * f.n_panels * n3 * nr,
                (double)f.n_panels * nr * 16.0 + nleaf * exp_bytes);
