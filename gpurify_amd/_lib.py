@@ -60,7 +60,8 @@ class GpCapOpts(C.Structure):
                 ("m2l_outer", C.c_uint32), ("strict", C.c_uint32), ("dense_limit", C.c_uint32),
                 ("rhs_first", C.c_uint32), ("rhs_count", C.c_uint32),
                 ("inner_passes", C.c_uint32), ("inner_basis", C.c_uint32),
-                ("m2l_rank", C.c_uint32), ("step_graphs", C.c_uint32)]
+                ("m2l_rank", C.c_uint32), ("step_graphs", C.c_uint32),
+                ("m2l_compress", C.c_uint32), ("fuse_leaf", C.c_uint32)]
 
 
 class GpParamMap(C.Structure):
@@ -102,6 +103,7 @@ EXPORTS = [
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
     "gp_tune_report", "gp_qui_parse", "gp_solve_resistance", "gp_cap_solve_dielectric",
+    "gp_m2l_operator",
 ]
 
 _lib = None
@@ -168,6 +170,9 @@ def load():
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_double)]
     L.gp_solve_resistance.restype = C.c_int
+    L.gp_m2l_operator.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+    L.gp_m2l_operator.restype = C.c_int
     L.gp_tune_force.argtypes = [vp, C.c_int]
     L.gp_tune_force.restype = C.c_int
     L.gp_tune_report.argtypes = [vp, C.c_char_p, C.c_size_t]
@@ -435,6 +440,20 @@ def parse_qui(text, scale=1.0):
         raise GpError(rc, L.gp_last_error(None).decode())
     out = names.raw[:nl.value].split(b"\0")[:-1] if nl.value else []
     return {"vertices": v, "conductor": c, "n_vertices": nv, "names": [x.decode() for x in out]}
+
+
+def m2l_operator(order, off, compressed=True):
+    """One M2L interaction matrix as the FMM operator applies it
+    (gp_m2l_operator, host only): (G (n^3 x n^3), rank)."""
+    L = load()
+    n3 = int(order) ** 3
+    G = np.zeros((n3, n3))
+    rk = C.c_uint32(0)
+    rc = L.gp_m2l_operator(int(order), int(off[0]), int(off[1]), int(off[2]), int(bool(compressed)),
+                           _dptr(G), C.byref(rk))
+    if rc != GP_OK:
+        raise GpError(rc, L.gp_last_error(None).decode())
+    return G, int(rk.value)
 
 
 class GpResPolygon(C.Structure):

@@ -16,6 +16,7 @@
 #include "gp_common.h"
 #include "gp_fastcap.h"
 #include "gp_resistance.h"
+#include "gp_m2l_cpqr.h"
 #include "gp_solver.h"
 #include "gp_tree.h"
 
@@ -138,8 +139,9 @@ void gp_ctx_destroy(gp_ctx *ctx) {
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->blas.reset();
     if (ctx->stream) {
-      (void)hipStreamSynchronize(ctx->stream);
       (void)hipStreamDestroy(ctx->stream);
     }
   }
@@ -181,6 +183,8 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->inner_basis = GP_BASIS_FP32;
   o->m2l_rank = 32;
   o->step_graphs = 1;
+  o->m2l_compress = 1;  // bbfmm.rs:132-174
+  o->fuse_leaf = 1;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,
@@ -422,6 +426,10 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
     else
       gp_cap_opts_default(&o);
     GP_HIP(hipSetDevice(ctx->device));
+    if (n_groups == 0) {  // bridge.rs:311-313: no volume conductors, no nets
+      if (n_nets) *n_nets = 0;
+      return;
+    }
     gp::Meshed m;
     gp::mesh_groups(groups, n_groups, xy_scale, (int)refine, max_panels, m);
     const int nn = (int)m.nets.size(), nc = nn + 1;
@@ -438,14 +446,19 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
       gp::cap_solve(ctx, g, o, C.data(), report, nullptr);
       for (auto &v : C) v *= first;  // bridge.rs:405-407
     } else {
-      // Conductor-role dielectric solve (bridge.rs:409-430): dense only.
+      // Conductor-role dielectric solve (bridge.rs:409-430).  The reference
+      // solves it dense and refuses 500 panels or more; GP_METHOD_FMM lifts
+      // that limit (the conductor rows are the same potential coefficients,
+      // so the FMM operator applies and only the charge weights change).
       gp_cap_opts od = o;
-      uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
-      if (g.n >= (int)limit)
-        throw gp::Error(GP_ERR_PANEL_CAPACITY, "extruded layout needs " + std::to_string(g.n) +
-                                                   " panels; limit is " +
-                                                   std::to_string(limit - 1));
-      od.method = GP_METHOD_DENSE;
+      if (o.method != GP_METHOD_FMM) {
+        uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
+        if (g.n >= (int)limit)
+          throw gp::Error(GP_ERR_PANEL_CAPACITY, "extruded layout needs " + std::to_string(g.n) +
+                                                     " panels; limit is " +
+                                                     std::to_string(limit - 1));
+        od.method = GP_METHOD_DENSE;
+      }
       gp::cap_solve(ctx, g, od, C.data(), report, m.eps.data());
     }
     if (n_nets) *n_nets = (uint32_t)nn;
@@ -461,6 +474,25 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
   });
 }
 
+
+int gp_m2l_operator(uint32_t order, int32_t o0, int32_t o1, int32_t o2, uint32_t compressed,
+                    double *G, uint32_t *rank) {
+  return guarded(nullptr, [&] {
+    if (!G) throw gp::Error(GP_ERR_INVALID_INPUT, "G is NULL");
+    if (order < 2 || order > 6) throw gp::Error(GP_ERR_INVALID_INPUT, "order must be in [2, 6]");
+    const int a0 = std::abs(o0), a1 = std::abs(o1), a2 = std::abs(o2);
+    if (std::max(a0, std::max(a1, a2)) > 3 || std::max(a0, std::max(a1, a2)) < 2)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "offset must be a far-field offset in [-3, 3]^3");
+    const int n = (int)order, n3 = n * n * n, n3p = (n3 + 31) / 32 * 32;
+    std::vector<int> ranks;
+    const auto tab = gp::m2l_tables(n, n3p, compressed != 0, &ranks);
+    const int oi = (o0 + 3) * 49 + (o1 + 3) * 7 + (o2 + 3);
+    const double *blk = tab->data() + (size_t)oi * n3p * n3p;
+    for (int m = 0; m < n3; ++m)
+      std::memcpy(G + (size_t)m * n3, blk + (size_t)m * n3p, n3 * sizeof(double));
+    if (rank) *rank = (uint32_t)ranks[oi];
+  });
+}
 
 int gp_qui_parse(const char *text, size_t text_len, double scale, uint32_t *n_panels,
                  uint32_t *n_conductors, size_t *names_len, double *vertices,

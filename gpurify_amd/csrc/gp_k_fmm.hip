@@ -1832,16 +1832,6 @@ __global__ __launch_bounds__(64) void k_l2p_wave(const int *__restrict__ leaf_lo
   }
 }
 
-// GPURIFY_FUSE_LEAF=0 keeps the finest L2L and L2P as two launches (the
-// unfused pair computes the same sums; for comparisons and timing).
-static bool fuse_leaf_enabled() {
-  static const bool on = [] {
-    const char *e = std::getenv("GPURIFY_FUSE_LEAF");
-    return !(e && e[0] == '0');
-  }();
-  return on;
-}
-
 // The finest L2L and L2P fused (FP32 blocks of RHS, every leaf at the
 // finest level): one work-group per leaf, per 32-column chunk the transposed
 // 1-D interpolations of the parent's locals added to the leaf's own (the
@@ -2799,12 +2789,11 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       }
     }
   }
-  // The finest L2L fused with L2P (FP32 blocks of RHS, every leaf at the
-  // finest level, no batch leaf map; orders up to 5, whose work-group
-  // needs 41 KB of LDS).
+  // The finest L2L fused with L2P (FP32 blocks of RHS, orders up to 5, whose
+  // work-group needs 41 KB of LDS).  One tree (no batch leaf map) has every
+  // leaf at the finest level, the layout the fused kernel walks.
   const bool fuse_leaf = sizeof(T) == 4 && tiled && !f.leaf_box && f.panel_leaf && ldm % 4 == 0 &&
-                         ldm >= 16 && n >= 3 && n <= 5 && L >= 3 && nr > 1 &&
-                         f.lvl_off[L + 1] - f.lvl_off[L] == nleaf && fuse_leaf_enabled();
+                         ldm >= 16 && n >= 3 && n <= 5 && L >= 3 && nr > 1 && f.fuse_leaf;
   // L2L, level 2 to the leaves.
   for (int l = 2; l < L; ++l) {
     int nb = f.lvl_off[l + 2] - f.lvl_off[l + 1];

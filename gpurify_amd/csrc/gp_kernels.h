@@ -125,6 +125,7 @@ struct FmmDev {
   const double *inv_half = nullptr;  // per level, scaled units
   const double *area = nullptr;      // per sorted panel, scaled units
   int n3p = 0;                       // n^3 rounded up to a multiple of 32
+  int fuse_leaf = 1;                 // FP32 blocks of RHS: finest L2L + L2P as one kernel
   const T *w1d = nullptr;            // per sorted panel 3 * n
   const T *m2m1d = nullptr;          // 24 * n * n
   const T *G = nullptr;              // 343 * n3p * n3p (zero padded)

@@ -38,6 +38,7 @@
 #include <numeric>
 
 #include "gp_kernels.h"
+#include "gp_m2l_cpqr.h"
 #include "gp_m2l_svd.h"
 
 namespace gp {
@@ -362,12 +363,12 @@ void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std:
 struct LowRank {
   std::vector<double> U, Gc;
 };
-std::shared_ptr<const LowRank> low_rank_tables(int n, int n3p, int kr,
+std::shared_ptr<const LowRank> low_rank_tables(int n, int n3p, int kr, bool compressed,
                                                const std::vector<double> &hG) {
   static std::mutex mu;
   static std::map<std::pair<int, int>, std::shared_ptr<const LowRank>> cache;
   std::lock_guard<std::mutex> lock(mu);
-  auto &e = cache[{n, kr}];
+  auto &e = cache[{n * 2 + (compressed ? 1 : 0), kr}];
   if (!e) {
     auto lr = std::make_shared<LowRank>();
     m2l_low_rank(n * n * n, n3p, hG.data(), kr, host_threads(), lr->U, lr->Gc);
@@ -390,6 +391,7 @@ struct FmmSystem {
   int64_t nnz = 0;
   long pairs = 0;
   bool have32 = false;
+  bool m2l_compress = false;  // far-field M2L blocks hold the CPQR truncation
   int n3p = 0;
   int m2l_inner = GP_M2L_BF16, m2l_outer = GP_M2L_FP64;
   DBuf<double> verts, cen, area, inv_diag;
@@ -730,26 +732,14 @@ struct FmmSystem {
               for (int m = 0; m < n; ++m) hm2m[(size_t)(o * 3 + axis) * n * n + m * n + k] = w[m];
             }
         }
-    // Uncompressed M2L matrices on the reference box, bbfmm.rs:156-166.
-    std::vector<double> hG((size_t)343 * n3p * n3p, 0.0);
-    for (int o0 = -3; o0 <= 3; ++o0)
-      for (int o1 = -3; o1 <= 3; ++o1)
-        for (int o2 = -3; o2 <= 3; ++o2) {
-          if (std::max(std::abs(o0), std::max(std::abs(o1), std::abs(o2))) <= 1) continue;
-          int oi = (o0 + 3) * 49 + (o1 + 3) * 7 + (o2 + 3);
-          double *G = &hG[(size_t)oi * n3p * n3p];
-          for (int m = 0; m < n3; ++m) {
-            int ma = m / (n * n), mb = (m / n) % n, mc = m % n;
-            for (int l = 0; l < n3; ++l) {
-              int la = l / (n * n), lb = (l / n) % n, lc = l % n;
-              double dx = nodes[ma] - nodes[la] - 2.0 * o0;
-              double dy = nodes[mb] - nodes[lb] - 2.0 * o1;
-              double dz = nodes[mc] - nodes[lc] - 2.0 * o2;
-              double r = std::sqrt(dx * dx + dy * dy + dz * dz);
-              G[(size_t)m * n3p + l] = r > 0.0 ? 1.0 / r : 0.0;
-            }
-          }
-        }
+    // M2L matrices on the reference box (bbfmm.rs:156-166), each far-field
+    // block truncated by the reference's CPQR (bbfmm.rs:132-174) unless
+    // m2l_compress is 0.  The transposed operator of the adjoint solves
+    // relies on G_-o = G_o^T, which the truncation does not keep, so it
+    // applies the full matrices.
+    m2l_compress = o.m2l_compress != 0 && !want_transpose;
+    const auto m2l_tab = m2l_tables(n, n3p, m2l_compress);
+    const std::vector<double> &hG = *m2l_tab;
     phase("tables");
     // Box arrays in the global numbering.
     std::vector<int> hlevel(nb), hparent(nb), hoct(nb), hchild_off(nb + 1, 0), hchild, htargets;
@@ -971,7 +961,7 @@ struct FmmSystem {
         Gr32 = to_device(hr, s);
       }
       if (kr) {
-        auto lr = low_rank_tables(n, n3p, kr, hG);
+        auto lr = low_rank_tables(n, n3p, kr, m2l_compress, hG);
         U32 = to_device(cast_vec<float>(lr->U), s);
         Gc32 = to_device(cast_vec<float>(lr->Gc), s);
         std::vector<uint16_t> hcb(lr->Gc.size()), hcl(lr->Gc.size());
@@ -999,6 +989,7 @@ struct FmmSystem {
     const bool batch = P > 1;
     auto fill = [&](auto &f) {
       f.n = n;
+      f.fuse_leaf = o.fuse_leaf != 0;
       f.L = L;
       f.n_boxes = nb;
       f.n_targets = (int)htargets.size();
@@ -1586,7 +1577,7 @@ void solve_systems(FmmSystem &sys, const Ops &ops, const gp_cap_opts &o, bool mi
 }
 
 void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
-               gp_cap_report *rep) {
+               gp_cap_report *rep, const double *eps_panel) {
   hipStream_t s = ctx->stream;
   double t0 = now_s();
   const bool mixed = o.precision == GP_PREC_MIXED;
@@ -1610,6 +1601,19 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   const size_t vec = (size_t)sys.N * ld;
   DBuf<double> b(vec), x(vec), dCb((size_t)nc * ld);
   std::vector<double> hC((size_t)nc * nc, 0.0), hblk((size_t)nc * ld);
+  // Charge weights: the panel areas, times the surrounding permittivity of
+  // each panel for the conductor-role dielectric solve (free charge
+  // eps_k * sigma_k * a_k, cap/dielectric.rs:141-149).
+  DBuf<double> wq;
+  const double *w = sys.area.p;
+  if (eps_panel) {
+    std::vector<double> ha(sys.N);
+    GP_HIP(hipMemcpyAsync(ha.data(), sys.area.p, sys.N * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < sys.N; ++k) ha[k] *= eps_panel[sys.t().perm[k]];
+    wq = to_device(ha, s);
+    w = wq.p;
+  }
   int max_iters = 0, refine_steps = 0;
   double worst = 0.0;
   bool conv_all = true;
@@ -1630,7 +1634,7 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
       worst = std::max(worst, st.resid[q]);
       conv_all &= (bool)st.conv[q];
     }
-    launch_cap_accumulate(nc, nr, ld, sys.coff.p, sys.cidx.p, sys.area.p, x.p, dCb.p, s);
+    launch_cap_accumulate(nc, nr, ld, sys.coff.p, sys.cidx.p, w, x.p, dCb.p, s);
     GP_HIP(hipMemcpyAsync(hblk.data(), dCb.p, (size_t)nc * nr * sizeof(double),
                           hipMemcpyDeviceToHost, s));
     GP_HIP(hipStreamSynchronize(s));
@@ -1868,9 +1872,15 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
       for (int j = 0; j < nc; ++j) Bc[(size_t)j * n + k] = B[(size_t)k * nc + j];
     auto d_B = to_device(Bc, s);
     DBuf<int> d_piv(n), d_info(1);
-    rocblas_handle h;
-    if (rocblas_create_handle(&h) != rocblas_status_success)
-      throw Error(GP_ERR_INTERNAL, "rocblas_create_handle failed");
+    if (!ctx->blas) {
+      rocblas_handle h0;
+      if (rocblas_create_handle(&h0) != rocblas_status_success)
+        throw Error(GP_ERR_INTERNAL, "rocblas_create_handle failed");
+      ctx->blas = std::shared_ptr<void>(h0, [](void *p) {
+        (void)rocblas_destroy_handle(static_cast<rocblas_handle>(p));
+      });
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(ctx->blas.get());
     rocblas_status st = rocblas_set_stream(h, s);
     if (st == rocblas_status_success) st = rocsolver_dgetrf(h, n, n, d_P.p, n, d_piv.p, d_info.p);
     if (st == rocblas_status_success)
@@ -1881,7 +1891,6 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
       GP_HIP(hipMemcpyAsync(Bc.data(), d_B.p, Bc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       GP_HIP(hipStreamSynchronize(s));
     }
-    rocblas_destroy_handle(h);
     if (st != rocblas_status_success)
       throw Error(GP_ERR_INTERNAL, "rocSOLVER getrf/getrs failed (" + std::to_string((int)st) + ")");
     if (info != 0) throw Error(GP_ERR_SINGULAR, "dielectric system is singular");
@@ -1914,9 +1923,6 @@ void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   uint32_t method = o.method;
   uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
   if (method == GP_METHOD_AUTO) method = ((uint32_t)g.n < limit) ? GP_METHOD_DENSE : GP_METHOD_FMM;
-  if (eps_panel && method != GP_METHOD_DENSE)
-    throw Error(GP_ERR_PANEL_CAPACITY, "inhomogeneous dielectric needs fewer than " +
-                                           std::to_string(limit) + " panels (dense only)");
   if (rep) {
     std::memset(rep, 0, sizeof(*rep));
     rep->n_panels = (uint32_t)g.n;
@@ -1925,7 +1931,7 @@ void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   if (method == GP_METHOD_DENSE)
     solve_dense(ctx, g, C, rep, eps_panel);
   else if (method == GP_METHOD_FMM)
-    solve_fmm(ctx, g, o, C, rep);
+    solve_fmm(ctx, g, o, C, rep, eps_panel);
   else
     throw Error(GP_ERR_INVALID_INPUT, "unknown method");
   for (size_t e = 0; e < (size_t)g.ncond * g.ncond; ++e)

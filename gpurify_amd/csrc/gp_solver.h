@@ -21,13 +21,17 @@ struct gp_ctx {
   int cu_count = 0;
   gp::Prof prof;
   gp::Tuner tuner;
+  // rocBLAS handle of the large dense dielectric LU (rocSOLVER), created on
+  // first use on this context's stream; released by gp_ctx_destroy.
+  std::shared_ptr<void> blas;
 };
 
 namespace gp {
 
 // Maxwell C (ncond x ncond, farads, vacuum) of g.  eps_panel, when given,
 // weights each panel's charge by its surrounding permittivity (the bridge's
-// inhomogeneous conductor-role solve, dense only).
+// inhomogeneous conductor-role solve; dense or FMM, the operator is the
+// same potential-coefficient matrix).
 // cap::solve_dielectric (cap/dielectric.rs:47-151), dense FP64 on the device.
 void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles, double *C,
                        gp_cap_report *rep);

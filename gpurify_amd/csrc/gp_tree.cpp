@@ -53,10 +53,11 @@ inline uint32_t compact3(uint64_t v) {
 
 }  // namespace
 
-void root_cube(const PanelGeom &g, double center[3], double &half) {
+void root_cube(int n, const double *cx, const double *cy, const double *cz, double center[3],
+               double &half) {
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int i = 0; i < g.n; ++i) {
-    double a[3] = {g.cx[i], g.cy[i], g.cz[i]};
+  for (int i = 0; i < n; ++i) {
+    double a[3] = {cx[i], cy[i], cz[i]};
     for (int d = 0; d < 3; ++d) {
       lo[d] = std::min(lo[d], a[d]);
       hi[d] = std::max(hi[d], a[d]);
@@ -69,6 +70,10 @@ void root_cube(const PanelGeom &g, double center[3], double &half) {
   }
   half *= 1.0 + 1e-6;
   if (half == 0.0) half = 1.0;
+}
+
+void root_cube(const PanelGeom &g, double center[3], double &half) {
+  root_cube(g.n, g.cx.data(), g.cy.data(), g.cz.data(), center, half);
 }
 
 namespace {
@@ -85,6 +90,37 @@ inline void leaf_cell(const double a[3], const double origin[3], double leaf_siz
 
 }  // namespace
 
+bool panel_props(const double *q, double c[3], double &area) {
+  V3 v[4];
+  for (int k = 0; k < 4; ++k) v[k] = {q[3 * k], q[3 * k + 1], q[3 * k + 2]};
+  // panel.rs:29-47 (fan triangulation, area-weighted centroid) and 58-67.
+  double asum = 0.0, cs[3] = {0, 0, 0};
+  for (int k = 1; k < 3; ++k) {
+    double a = norm(cross(sub(v[k], v[0]), sub(v[k + 1], v[0]))) * 0.5;
+    double tc[3] = {(v[0].x + v[k].x + v[k + 1].x) * (1.0 / 3.0),
+                    (v[0].y + v[k].y + v[k + 1].y) * (1.0 / 3.0),
+                    (v[0].z + v[k].z + v[k + 1].z) * (1.0 / 3.0)};
+    cs[0] = cs[0] + tc[0] * a;
+    cs[1] = cs[1] + tc[1] * a;
+    cs[2] = cs[2] + tc[2] * a;
+    asum += a;
+  }
+  if (asum == 0.0) return false;
+  for (int d = 0; d < 3; ++d) c[d] = cs[d] * (1.0 / asum);
+  area = asum;
+  return true;
+}
+
+void check_panel(const gp_panels &in, int i) {
+  if (in.conductor[i] >= in.n_conductors)
+    throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " conductor id " +
+                                          std::to_string(in.conductor[i]) + " out of range");
+  const double *q = in.vertices + (size_t)i * 12;
+  for (int k = 0; k < 12; ++k)
+    if (!std::isfinite(q[k]))
+      throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " has a non-finite vertex");
+}
+
 void panel_geometry(const gp_panels &in, PanelGeom &g) {
   if (in.n_panels == 0) throw Error(GP_ERR_EMPTY, "no panels in geometry");
   if (!in.vertices || !in.conductor) throw Error(GP_ERR_INVALID_INPUT, "null panel arrays");
@@ -98,33 +134,13 @@ void panel_geometry(const gp_panels &in, PanelGeom &g) {
   g.cz.resize(g.n);
   g.area.resize(g.n);
   for (int i = 0; i < g.n; ++i) {
-    if (g.cond[i] >= in.n_conductors)
-      throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " conductor id " +
-                                            std::to_string(g.cond[i]) + " out of range");
-    const double *q = &g.verts[(size_t)i * 12];
-    for (int k = 0; k < 12; ++k)
-      if (!std::isfinite(q[k]))
-        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " has a non-finite vertex");
-    V3 v[4];
-    for (int k = 0; k < 4; ++k) v[k] = {q[3 * k], q[3 * k + 1], q[3 * k + 2]};
-    // panel.rs:29-47 (fan triangulation, area-weighted centroid) and 58-67.
-    double asum = 0.0, c[3] = {0, 0, 0};
-    for (int k = 1; k < 3; ++k) {
-      double a = norm(cross(sub(v[k], v[0]), sub(v[k + 1], v[0]))) * 0.5;
-      double tc[3] = {(v[0].x + v[k].x + v[k + 1].x) * (1.0 / 3.0),
-                      (v[0].y + v[k].y + v[k + 1].y) * (1.0 / 3.0),
-                      (v[0].z + v[k].z + v[k + 1].z) * (1.0 / 3.0)};
-      c[0] = c[0] + tc[0] * a;
-      c[1] = c[1] + tc[1] * a;
-      c[2] = c[2] + tc[2] * a;
-      asum += a;
-    }
-    if (asum == 0.0)
+    check_panel(in, i);
+    double c[3];
+    if (!panel_props(&g.verts[(size_t)i * 12], c, g.area[i]))
       throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " has zero area");
-    g.cx[i] = c[0] * (1.0 / asum);
-    g.cy[i] = c[1] * (1.0 / asum);
-    g.cz[i] = c[2] * (1.0 / asum);
-    g.area[i] = asum;
+    g.cx[i] = c[0];
+    g.cy[i] = c[1];
+    g.cz[i] = c[2];
   }
 }
 

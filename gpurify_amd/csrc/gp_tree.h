@@ -21,6 +21,11 @@ struct PanelGeom {
 // Validates the boundary input and computes centroids/areas with the
 // reference formulas (panel.rs:29-67).  Throws gp::Error on bad input.
 void panel_geometry(const gp_panels &in, PanelGeom &g);
+// The checks of panel_geometry for panel i (conductor id, finite vertices).
+void check_panel(const gp_panels &in, int i);
+// Area-weighted centroid c and area of one panel's 4 vertices (q: 12
+// doubles), panel.rs:29-47 and 58-67; false when the area is zero.
+bool panel_props(const double *q, double c[3], double &area);
 
 // Octree over the panel centroids with all leaves at depth L, empty boxes not
 // stored (fmm/tree.rs:41-106).  Boxes of each level are ordered by Morton code,
@@ -50,6 +55,8 @@ struct Tree {
 
 // Root cube of the centroids (tree.rs:43-62): centre and half-width, metres.
 void root_cube(const PanelGeom &g, double center[3], double &half);
+void root_cube(int n, const double *cx, const double *cy, const double *cz, double center[3],
+               double &half);
 
 // Builds the tree at depth max_level over the centroids of g.
 void build_tree(const PanelGeom &g, int max_level, Tree &t);

@@ -86,8 +86,9 @@ enum { GP_M2L_FP64 = 0, GP_M2L_FP32 = 1, GP_M2L_BF16 = 2, GP_M2L_FP64_VALU = 3,
 
 /* Solver options.  gp_cap_opts_default() fills the reference's own
  * constants: the dense/FMM rule of bridge.rs:397-404 (dense below 500
- * panels, else FMM order 4, depth 4, tol 1e-8), GMRES restart 200 and 50
- * restarts (cap/fmm_solver.rs:164), FP64 throughout. */
+ * panels, else FMM order 4, depth 4, tol 1e-8), the CPQR-truncated M2L
+ * operators of bbfmm.rs:132-174, GMRES restart 200 and 50 restarts
+ * (cap/fmm_solver.rs:164), FP64 throughout. */
 typedef struct {
   uint32_t precision;    /* GP_PREC_FP64 | GP_PREC_MIXED */
   uint32_t method;       /* GP_METHOD_AUTO (reference rule) | DENSE | FMM */
@@ -118,6 +119,14 @@ typedef struct {
                             each inner GMRES step (operator apply, orthogonalisation, Givens
                             update) runs as a HIP graph captured on first use and replayed;
                             0 = kernels launched one by one.  Off while profiling. */
+  uint32_t m2l_compress; /* 1 (default) = the reference's truncated M2L operators: every far-field
+                            interaction matrix compressed by column-pivoted QR, truncated where the
+                            remaining squared column norm falls below eps = max(1e-3 * 0.1^n,
+                            1e-14) of the largest (bbfmm.rs:132-174, linalg.rs:277-381), and
+                            applied as the dense product U V^T; 0 = the full matrices.  The
+                            adjoint solves of gp_cap_jacobian always apply the full matrices. */
+  uint32_t fuse_leaf;    /* 1 (default) = FP32 blocks of RHS at orders <= 5 run the finest L2L and
+                            L2P as one kernel; 0 = two launches (the same sums). */
 } gp_cap_opts;
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 
@@ -197,6 +206,18 @@ int gp_cap_jacobian(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opt
 int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, uint32_t nr,
                  uint32_t precision, const double *X, double *Y);
 
+/* One M2L interaction matrix of the Chebyshev bbFMM as the FMM operator
+ * applies it (host only: no context, no device).  G[m * n^3 + l] =
+ * 1 / |x_m - x_l - 2 o| over the tensor Chebyshev nodes of order `order`
+ * (2..6) on the reference box for the far-field box offset o (components in
+ * [-3, 3], at least one of magnitude 2 or 3), as precompute_m2l builds it
+ * (bbfmm.rs:132-166); with compressed = 1 its truncation U V^T by
+ * LowRankApprox::compress (linalg.rs:277-381) at eps = max(1e-3 * 0.1^n,
+ * 1e-14) (bbfmm.rs:150), whose rank goes to *rank (may be NULL).
+ * G: order^6 doubles. */
+int gp_m2l_operator(uint32_t order, int32_t o0, int32_t o1, int32_t o2, uint32_t compressed,
+                    double *G, uint32_t *rank);
+
 /* Kernel profiler (no reference counterpart; the reference has no timers).
  * Enabled, every device kernel launch of this context is bracketed by HIP
  * events on its stream.  gp_profile_report writes a JSON object
@@ -240,9 +261,12 @@ int gp_mesh_groups(const gp_volume_group *groups, uint32_t n_groups, double xy_s
                    uint32_t *conductor, double *panel_eps);
 
 /* solve_capacitance (bridge.rs:307-443): mesh, solve, scale by eps_r
- * (homogeneous) or by per-panel eps (inhomogeneous, dense only, as the
- * reference's conductor-role dielectric solve), and return cap_af =
- * C[i][i] * 1e18 per net in ascending net order.
+ * (homogeneous) or by per-panel eps (inhomogeneous: the reference's
+ * conductor-role dielectric solve, dense below dense_limit panels and
+ * GP_ERR_PANEL_CAPACITY above, as bridge.rs:409-415; opts->method =
+ * GP_METHOD_FMM solves it with the FMM operator at any size), and return
+ * cap_af = C[i][i] * 1e18 per net in ascending net order.  No groups: GP_OK
+ * with *n_nets = 0 (bridge.rs:311-313).
  * nets_out / cap_af_out: room for the number of distinct nets.
  * C_out (optional): (n_nets+1)^2 farads with eps applied, substrate last. */
 int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_groups,

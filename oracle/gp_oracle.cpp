@@ -1455,6 +1455,38 @@ double or_potential_integral(const double *verts, int nv, const double *p) {
   return potential_integral(v, nv, {p[0], p[1], p[2]});
 }
 
+// panel.rs:213-221: the 7-point table as rows (l0, l1, l2, weight).
+void or_tri7(double *tab) {
+  for (int q = 0; q < 7; ++q)
+    for (int c = 0; c < 4; ++c) tab[q * 4 + c] = TRI7[q][c];
+}
+
+// linalg.rs:277-381 (LowRankApprox::compress).  u: m*min(m,n), vt: min(m,n)*n
+// (the first k columns / rows written); returns k.
+int or_lowrank_compress(const double *a, int m, int n, double eps, double *u, double *vt) {
+  LowRank lr;
+  lr.compress(a, m, n, eps);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < lr.k; ++j) u[(size_t)i * lr.k + j] = lr.u[(size_t)i * lr.k + j];
+  for (size_t e = 0; e < (size_t)lr.k * n; ++e) vt[e] = lr.vt[e];
+  return lr.k;
+}
+
+// bbfmm.rs:150 (eps) and 156-166 (G of one offset on the reference box).
+double or_m2l_eps(int order) { return std::max(1e-3 * std::pow(0.1, order), 1e-14); }
+void or_m2l_matrix(int order, int o0, int o1, int o2, double *G) {
+  const int n = order, n3 = n * n * n;
+  std::vector<double> nd = cheb_nodes(n);
+  for (int m = 0; m < n3; ++m)
+    for (int l = 0; l < n3; ++l) {
+      double dx = nd[m / (n * n)] - nd[l / (n * n)] - 2.0 * o0;
+      double dy = nd[(m / n) % n] - nd[(l / n) % n] - 2.0 * o1;
+      double dz = nd[m % n] - nd[l % n] - 2.0 * o2;
+      double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+      G[(size_t)m * n3 + l] = r > 0.0 ? 1.0 / r : 0.0;
+    }
+}
+
 double or_field_integral(const double *verts, int nv, const double *p, const double *n_obs) {
   V3 v[8];
   for (int i = 0; i < nv && i < 8; ++i) v[i] = {verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};

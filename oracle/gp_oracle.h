@@ -26,6 +26,19 @@ double or_potential_integral(const double *verts, int nv, const double *p);
 /* panel.rs:232-253 (7-point normal-field quadrature). */
 double or_field_integral(const double *verts, int nv, const double *p, const double *n_obs);
 
+/* panel.rs:213-221: the 7-point triangle rule as 7 rows (l0, l1, l2, weight). */
+void or_tri7(double *tab);
+
+/* linalg.rs:277-381 (LowRankApprox::compress): column-pivoted MGS QR of the
+ * m x n row-major a, truncated at squared column norm < eps * max.
+ * u: m*k row-major, vt: k*n row-major (room for k = min(m, n)); returns k. */
+int or_lowrank_compress(const double *a, int m, int n, double eps, double *u, double *vt);
+
+/* bbfmm.rs:150 and 156-166: the truncation eps of order n and the dense M2L
+ * matrix of offset (o0, o1, o2) on the reference box (n^3 x n^3). */
+double or_m2l_eps(int order);
+void or_m2l_matrix(int order, int o0, int o1, int o2, double *G);
+
 /* panel.rs:29-67 + 95-119: centroid (fan, area weighted), area, Newell normal.
  * verts: n*4*3 doubles (quads; a triangle repeats its last vertex). */
 void or_panel_props(int n, const double *verts4, double *cx, double *cy, double *cz,

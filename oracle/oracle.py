@@ -42,6 +42,12 @@ def lib():
         L.or_potential_integral.argtypes = [_d, C.c_int, _d]
         L.or_field_integral.restype = C.c_double
         L.or_field_integral.argtypes = [_d, C.c_int, _d, _d]
+        L.or_tri7.argtypes = [_d]
+        L.or_lowrank_compress.restype = C.c_int
+        L.or_lowrank_compress.argtypes = [_d, C.c_int, C.c_int, C.c_double, _d, _d]
+        L.or_m2l_eps.restype = C.c_double
+        L.or_m2l_eps.argtypes = [C.c_int]
+        L.or_m2l_matrix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _d]
         L.or_panel_props.argtypes = [C.c_int, _d, _d, _d, _d, _d, _d]
         L.or_assemble_p.argtypes = [C.c_int, _d, _d]
         L.or_lu_solve.argtypes = [C.c_int, _d, C.c_int, _d]
@@ -93,6 +99,35 @@ def get_threads() -> int:
 def potential_integral(verts, p) -> float:
     v = _f64(verts).reshape(-1, 3)
     return float(lib().or_potential_integral(_p(v), v.shape[0], _p(_f64(p))))
+
+
+def tri7():
+    """The 7-point triangle rule (panel.rs:213-221): rows (l0, l1, l2, weight)."""
+    t = np.zeros((7, 4))
+    lib().or_tri7(_p(t))
+    return t
+
+
+def lowrank_compress(a, eps):
+    """LowRankApprox::compress (linalg.rs:277-381): (U, Vt, k) with A ~ U @ Vt."""
+    a = _f64(a)
+    m, n = a.shape
+    kk = min(m, n)
+    u = np.zeros(m * kk)
+    vt = np.zeros(kk * n)
+    k = lib().or_lowrank_compress(_p(a), m, n, float(eps), _p(u), _p(vt))
+    return u[:m * k].reshape(m, k), vt[:k * n].reshape(k, n), k
+
+
+def m2l_eps(order) -> float:
+    return float(lib().or_m2l_eps(int(order)))
+
+
+def m2l_matrix(order, off):
+    n3 = order ** 3
+    g = np.zeros((n3, n3))
+    lib().or_m2l_matrix(int(order), int(off[0]), int(off[1]), int(off[2]), _p(g))
+    return g
 
 
 def field_integral(verts, p, n_obs) -> float:
