@@ -103,7 +103,7 @@ EXPORTS = [
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
     "gp_tune_report", "gp_qui_parse", "gp_solve_resistance", "gp_cap_solve_dielectric",
-    "gp_m2l_operator",
+    "gp_m2l_operator", "gp_cap_solve_dense_noalloc",
 ]
 
 _lib = None
@@ -170,6 +170,9 @@ def load():
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_double)]
     L.gp_solve_resistance.restype = C.c_int
+    L.gp_cap_solve_dense_noalloc.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(C.c_double),
+                                             C.POINTER(GpCapReport)]
+    L.gp_cap_solve_dense_noalloc.restype = C.c_int
     L.gp_m2l_operator.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
                                   C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     L.gp_m2l_operator.restype = C.c_int
@@ -280,6 +283,18 @@ class Context:
         rep = GpCapReport()
         o = opts if opts is not None else default_opts()
         self._check(self._L.gp_cap_solve(self._h, C.byref(p), C.byref(o), _dptr(Cm), C.byref(rep)))
+        return Cm, rep.as_dict()
+
+    def cap_solve_dense_noalloc(self, verts4, cond, ncond):
+        """The dense FP64 solve in the library's fixed device workspace
+        (gp_cap_solve_dense_noalloc): C (ncond x ncond, F, vacuum) and report."""
+        v = np.ascontiguousarray(verts4, dtype=np.float64).reshape(-1, 4, 3)
+        c = np.ascontiguousarray(cond, dtype=np.uint32)
+        p = GpPanels(v.shape[0], ncond, _dptr(v), _uptr(c))
+        Cm = np.zeros((ncond, ncond))
+        rep = GpCapReport()
+        self._check(self._L.gp_cap_solve_dense_noalloc(self._h, C.byref(p), _dptr(Cm),
+                                                       C.byref(rep)))
         return Cm, rep.as_dict()
 
     def cap_solve_dielectric(self, verts4, role, cond, eps, eps_out, eps_in, ref, ncond):

@@ -17,6 +17,7 @@
 #include "gp_fastcap.h"
 #include "gp_resistance.h"
 #include "gp_m2l_cpqr.h"
+#include "gp_noalloc.h"
 #include "gp_solver.h"
 #include "gp_tree.h"
 
@@ -130,6 +131,7 @@ int gp_ctx_create(gp_ctx **out, uint32_t device_ordinal) {
       delete c;
       gp::hip_check(e, "hipStreamCreateWithFlags");
     }
+    gp::noalloc_init(dev);  // the fixed workspace's address on this device
     *out = c;
   });
 }
@@ -205,6 +207,21 @@ int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
     gp::PanelGeom g;
     gp::panel_geometry(*panels, g);
     gp::cap_solve(ctx, g, o, C, report, nullptr);
+  });
+}
+
+int gp_cap_solve_dense_noalloc(gp_ctx *ctx, const gp_panels *panels, double *C,
+                               gp_cap_report *report) {
+  if (!ctx) {
+    g_thread_error = "gp_cap_solve_dense_noalloc: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    if (!panels || !C) throw gp::Error(GP_ERR_INVALID_INPUT, "null panels or C");
+    gp::active_prof() = nullptr;  // no profiler events: nothing is created
+    GP_HIP(hipSetDevice(ctx->device));
+    gp::cap_solve_dense_noalloc(ctx, *panels, C, report);
   });
 }
 

@@ -218,6 +218,27 @@ int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
 int gp_m2l_operator(uint32_t order, int32_t o0, int32_t o1, int32_t o2, uint32_t compressed,
                     double *G, uint32_t *rank);
 
+/* ------------------------------------------------------------ fixed workspace
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation): the dense path of gp_cap_solve (opts->method =
+ * GP_METHOD_DENSE in FP64: collocation assembly, one-work-group LU, charge
+ * accumulation) run entirely in a fixed per-device workspace of
+ * GP_NOALLOC_SCRATCH_BYTES, which the library holds from the first
+ * gp_ctx_create on a device until the process ends.  The call makes no
+ * hipMalloc, hipFree or allocating library call.  Inputs and outputs as
+ * gp_cap_solve; C equals its dense result.  At most GP_NOALLOC_MAX_PANELS
+ * panels (the reference's dense rule is below 500) and
+ * GP_NOALLOC_MAX_CONDUCTORS conductors, else GP_ERR_INVALID_INPUT.  Calls on
+ * one device share the workspace, which keeps the last call's contents
+ * (every region a call reads it first writes); the library serializes these
+ * calls within a process, and calls that overlapped on one device would race. */
+#define GP_NOALLOC_MAX_PANELS 512
+#define GP_NOALLOC_MAX_CONDUCTORS 64
+#define GP_NOALLOC_SCRATCH_BYTES 2726656
+int gp_cap_solve_dense_noalloc(gp_ctx *ctx, const gp_panels *panels, double *C,
+                               gp_cap_report *report);
+
 /* Kernel profiler (no reference counterpart; the reference has no timers).
  * Enabled, every device kernel launch of this context is bracketed by HIP
  * events on its stream.  gp_profile_report writes a JSON object
