@@ -84,7 +84,7 @@ class GpCapReport(C.Structure):
                 ("final_rel_residual", C.c_double), ("seconds_total", C.c_double),
                 ("seconds_setup", C.c_double), ("seconds_solve", C.c_double),
                 ("seconds_near", C.c_double), ("seconds_far", C.c_double),
-                ("seconds_orth", C.c_double)]
+                ("seconds_orth", C.c_double), ("seconds_upload", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -103,7 +103,7 @@ EXPORTS = [
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
     "gp_tune_report", "gp_qui_parse", "gp_solve_resistance", "gp_cap_solve_dielectric",
-    "gp_m2l_operator", "gp_cap_solve_dense_noalloc",
+    "gp_m2l_operator", "gp_cap_solve_dense_noalloc", "gp_device_synchronize",
 ]
 
 _lib = None
@@ -126,6 +126,8 @@ def load():
     L.gp_last_error.restype = C.c_char_p
     L.gp_device_ready.argtypes = []
     L.gp_device_ready.restype = C.c_int
+    L.gp_device_synchronize.argtypes = [vp]
+    L.gp_device_synchronize.restype = C.c_int
     L.gp_device_info.argtypes = [vp, C.c_char_p, C.c_size_t]
     L.gp_device_info.restype = C.c_int
     L.gp_version.argtypes = []
@@ -268,6 +270,10 @@ class Context:
         buf = C.create_string_buffer(1 << 16)
         self._check(self._L.gp_tune_report(self._h, buf, 1 << 16))
         return json.loads(buf.value.decode())
+
+    def synchronize(self):
+        """Wait for all work on the context's device (gp_device_synchronize)."""
+        self._check(self._L.gp_device_synchronize(self._h))
 
     def info(self) -> str:
         buf = C.create_string_buffer(256)

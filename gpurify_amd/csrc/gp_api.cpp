@@ -28,6 +28,11 @@ size_t &device_bytes_in_use() {
   return used;
 }
 
+double &upload_seconds() {
+  static thread_local double t = 0.0;
+  return t;
+}
+
 Prof *&active_prof() {
   static thread_local Prof *p = nullptr;
   return p;
@@ -152,6 +157,18 @@ void gp_ctx_destroy(gp_ctx *ctx) {
 
 const char *gp_last_error(const gp_ctx *ctx) {
   return ctx ? ctx->last_error.c_str() : g_thread_error.c_str();
+}
+
+int gp_device_synchronize(gp_ctx *ctx) {
+  if (!ctx) {
+    g_thread_error = "gp_device_synchronize: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    GP_HIP(hipSetDevice(ctx->device));
+    GP_HIP(hipDeviceSynchronize());
+  });
 }
 
 int gp_device_info(gp_ctx *ctx, char *buf, size_t len) {

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <cstdint>
 #include <cstdlib>
 #include <stdexcept>
@@ -93,6 +95,29 @@ struct DBuf {
     if (n) GP_HIP(hipMemsetAsync(p, 0, bytes(), s));
   }
 };
+
+// Wall time of the host-to-device copies of the caller's panel data
+// (vertices, centroids, areas, conductor ids) in the current API call, per
+// thread; cleared by the solve entry points, reported as seconds_upload.
+double &upload_seconds();
+
+template <typename T>
+DBuf<T> to_device(const std::vector<T> &h, hipStream_t s);
+template <typename T>
+DBuf<T> to_device(const T *h, size_t count, hipStream_t s);
+
+// to_device for panel data: the copy's wall time counts as input upload.
+template <typename T>
+DBuf<T> upload_panels(const T *h, size_t count, hipStream_t s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  DBuf<T> d = to_device(h, count, s);  // synchronous: upload() waits for the copy
+  upload_seconds() += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return d;
+}
+template <typename T>
+DBuf<T> upload_panels(const std::vector<T> &h, hipStream_t s) {
+  return upload_panels(h.data(), h.size(), s);
+}
 
 template <typename T>
 DBuf<T> to_device(const std::vector<T> &h, hipStream_t s) {

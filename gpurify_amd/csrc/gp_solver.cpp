@@ -239,9 +239,9 @@ void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
   std::vector<int> coff, cidx;
   cond_csr(g, nullptr, coff, cidx);
 
-  auto d_v = to_device(hv, s);
-  auto d_c = to_device(hc, s);
-  auto d_w = to_device(w, s);
+  auto d_v = upload_panels(hv, s);
+  auto d_c = upload_panels(hc, s);
+  auto d_w = upload_panels(w, s);
   auto d_B = to_device(B, s);
   auto d_coff = to_device(coff, s);
   auto d_cidx = to_device(cidx, s);
@@ -307,8 +307,8 @@ void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std:
   auto d_noff = to_device(hnoff, s);
   auto d_aoff = to_device(haoff, s);
   auto d_boff = to_device(hboff, s);
-  auto d_v = to_device(hv, s);
-  auto d_c = to_device(hc, s);
+  auto d_v = upload_panels(hv, s);
+  auto d_c = upload_panels(hc, s);
   auto d_B = to_device(hB, s);
   DBuf<double> d_A(haoff[B]), d_work(hboff[B]);
   DBuf<int> d_piv(hnoff[B]), d_info(B);
@@ -565,10 +565,10 @@ struct FmmSystem {
                      (uint32_t)cond_off[p]);
         },
         1);
-    verts = to_device(hv.get(), (size_t)N * 12, s);
-    cen = to_device(hc.get(), (size_t)N * 3, s);
-    area = to_device(ha.get(), (size_t)N, s);
-    cond = to_device(hcond.get(), (size_t)N, s);
+    verts = upload_panels(hv.get(), (size_t)N * 12, s);
+    cen = upload_panels(hc.get(), (size_t)N * 3, s);
+    area = upload_panels(ha.get(), (size_t)N, s);
+    cond = upload_panels(hcond.get(), (size_t)N, s);
     hv.reset();
     hc.reset();
 
@@ -1917,6 +1917,7 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
 void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
                gp_cap_report *rep, const double *eps_panel) {
   double t0 = now_s();
+  upload_seconds() = 0.0;
   if (!(o.tol > 0.0) || !std::isfinite(o.tol)) throw Error(GP_ERR_INVALID_INPUT, "tol must be > 0");
   if (o.precision != GP_PREC_FP64 && o.precision != GP_PREC_MIXED)
     throw Error(GP_ERR_INVALID_INPUT, "unknown precision");
@@ -1936,7 +1937,10 @@ void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
     throw Error(GP_ERR_INVALID_INPUT, "unknown method");
   for (size_t e = 0; e < (size_t)g.ncond * g.ncond; ++e)
     if (!std::isfinite(C[e])) throw Error(GP_ERR_NON_FINITE, "solver returned a non-finite capacitance");
-  if (rep) rep->seconds_total = now_s() - t0;
+  if (rep) {
+    rep->seconds_total = now_s() - t0;
+    rep->seconds_upload = upload_seconds();
+  }
 }
 
 void cap_jacobian(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, const ParamMap &pm,
@@ -2178,6 +2182,7 @@ void cap_jacobian(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, const P
 void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_cap_opts &o,
                        const std::vector<double *> &C, gp_cap_report *reps) {
   const double t0 = now_s();
+  upload_seconds() = 0.0;
   if (!(o.tol > 0.0) || !std::isfinite(o.tol)) throw Error(GP_ERR_INVALID_INPUT, "tol must be > 0");
   if (o.precision != GP_PREC_FP64 && o.precision != GP_PREC_MIXED)
     throw Error(GP_ERR_INVALID_INPUT, "unknown precision");
@@ -2219,7 +2224,10 @@ void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_c
       if (!std::isfinite(C[p][e]))
         throw Error(GP_ERR_NON_FINITE, "solver returned a non-finite capacitance for problem " +
                                            std::to_string(p));
-    if (reps) reps[p].seconds_total = t1 - t0;
+    if (reps) {
+      reps[p].seconds_total = t1 - t0;
+      reps[p].seconds_upload = upload_seconds();
+    }
   }
 }
 

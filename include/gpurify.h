@@ -56,6 +56,10 @@ void gp_ctx_destroy(gp_ctx *ctx);
 const char *gp_last_error(const gp_ctx *ctx);
 /* 1 if a gfx950 device is usable now, else 0 (gpu_ready(), backend/src/lib.rs:50-59). */
 int gp_device_ready(void);
+/* Blocks until every queued operation on the context's device has finished
+ * (no reference counterpart; benchmark timing brackets).  Every solve entry
+ * point already returns only after its own work is complete. */
+int gp_device_synchronize(gp_ctx *ctx);
 /* Device name, arch and CU count as text. */
 int gp_device_info(gp_ctx *ctx, char *buf, size_t len);
 /* Library version string (static storage). */
@@ -145,6 +149,8 @@ typedef struct {
   double final_rel_residual; /* max over RHS of ||b - A x|| / ||b|| (FP64 operator) */
   double seconds_total, seconds_setup, seconds_solve;
   double seconds_near, seconds_far, seconds_orth; /* time in near-field, far-field, GMRES kernels */
+  double seconds_upload;  /* host-to-device copies of the panel data (vertices, centroids,
+                             areas, conductor ids), part of seconds_total */
 } gp_cap_report;
 
 /* Maxwell capacitance matrix of a panel set.  Stands in for cap::solve
