@@ -52,12 +52,43 @@ def test_opts_struct_size_matches_header():
     G._lib.load().gp_cap_opts_default(ctypes.cast(ctypes.byref(p), ctypes.POINTER(G._lib.GpCapOpts)))
     assert p.canary == 0x5A5A5A5A5A5A5A5A
     assert p.o.inner_basis == G.GP_BASIS_FP32 and p.o.inner_passes == 1 and p.o.m2l_rank == 32
-    assert p.o.step_graphs == 1
+    assert p.o.step_graphs == 1 and p.o.m2l_compress == 1 and p.o.fuse_leaf == 1
     # the last field is written, so the library's struct is not shorter
     text = open(os.path.join(ROOT, "include", "gpurify.h")).read()
     body = re.search(r"typedef struct \{([^{}]*)\} gp_cap_opts;", text).group(1)
     names = re.findall(r"\b(?:uint32_t|double)\s+(\w+);", body)
     assert names == [f for f, _ in G._lib.GpCapOpts._fields_]
+
+
+def _header_struct_fields(text, name):
+    body = re.search(r"typedef struct \{([^{}]*)\} " + name + ";", text).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        for part in decl.split(","):
+            m = re.search(r"\*?\s*(\w+)(\[\d+\])?\s*$", part.strip())
+            out.append(m.group(1))
+    return out
+
+
+def test_rust_ffi_crate_matches_header():
+    # gpurify-sys/src/lib.rs (source only: no Rust toolchain here) declares
+    # the header's structs in field order and every exported function
+    text = open(os.path.join(ROOT, "include", "gpurify.h")).read()
+    rs = open(os.path.join(ROOT, "gpurify-sys", "src", "lib.rs")).read()
+    for name in ("gp_panels", "gp_cap_opts", "gp_cap_report", "gp_param_map", "gp_volume_group",
+                 "gp_panel_role", "gp_res_polygon"):
+        body = re.search(r"pub struct " + name + r" \{(.*?)\}", rs, flags=re.S).group(1)
+        rust_fields = re.findall(r"pub (\w+):", body)
+        assert rust_fields == _header_struct_fields(text, name), name
+    externs = re.search(r'extern "C" \{(.*?)\n\}', rs, flags=re.S).group(1)
+    assert sorted(re.findall(r"pub fn (gp_\w+)\(", externs)) == header_symbols()
+    for const in ("GP_NOALLOC_MAX_PANELS", "GP_NOALLOC_MAX_CONDUCTORS", "GP_NOALLOC_SCRATCH_BYTES"):
+        c_val = re.search(r"#define " + const + r" (\d+)", text).group(1)
+        assert re.search(r"pub const " + const + r": \w+ = " + c_val + ";", rs), const
 
 
 def test_no_device_is_reported_not_crashed():
