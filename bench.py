@@ -536,9 +536,13 @@ M2L_PIPE = {"m2l_bf16x3_mfma": ("m2l_compress_f32", "m2l_expand_f32")}
 
 def roofline_of(name, prof, work, workload):
     """Roofline entry of one kernel (name) from the profile of the timed
-    steps, with the algorithmic work of its definition:
-      near field (HBM): nnz * s_val + 2 * N * R * s_vec bytes per launch
-        (s = 4 in FP32, 8 in FP64);
+    steps, with the algorithmic work of its definition, counted by the
+    library per launch over the problems the launch computes (a batch skips
+    the problems whose GMRES has converged, and the library counts only the
+    others' share):
+      near field (HBM): nnz * s_val + N * (s_x + 8) bytes per launch (the
+        values once, x read and y written once per row; s = 4 in FP32, 8 in
+        FP64);
       M2L (matrix cores): 2 * pairs * n^6 * R dense-equivalent flops per
         launch over the whole M2L pipeline (for the low-rank inner M2L the
         projection and expansion kernels' time is added: low rank counts as
@@ -555,13 +559,21 @@ def roofline_of(name, prof, work, workload):
     if name.startswith("near_sp"):
         s = 4 if name.endswith("f32") else 8
         bound, peak, unit = "hbm", PEAKS["hbm_GBs"], "GB/s"
-        per = nnz * s + 2.0 * N * R * s
-        basis = f"nnz*{s} + 2*N*R*{s} bytes (nnz={nnz}, N={N}, R={R})"
+        per = v["bytes"] / L
+        full = nnz * s + N * R * (s + 8)
+        basis = (f"nnz*{s} + N*R*({s}+8) bytes over the problems computed: {per:.4g} per launch "
+                 f"on average, {per / full:.3f} of the whole batch's {full:.4g} (nnz={nnz}, N={N}, R={R})")
     elif name.startswith("m2l") and ("mfma" in name):
         bound, unit = "mfma", "TFLOP/s"
         peak = PEAKS["bf16_mfma_TFs"] if "bf16" in name else PEAKS["fp64_TFs"]
-        per = 2.0 * pairs * n3 * n3 * R
-        basis = f"2*pairs*n^6*R dense-equivalent flops (pairs={pairs}, n^3={n3}, R={R})"
+        full = 2.0 * pairs * n3 * n3 * R
+        if name in M2L_PIPE:  # the library counts the three split-BF16 products; count one
+            per = full
+        else:
+            per = v["flops"] / L
+        basis = (f"2*pairs*n^6*R dense-equivalent flops over the problems computed: {per:.4g} per "
+                 f"launch on average, {per / full:.3f} of the whole {full:.4g} (pairs={pairs}, "
+                 f"n^3={n3}, R={R})")
         if name in M2L_PIPE:
             basis += " over " + " + ".join((name,) + M2L_PIPE[name])
     else:

@@ -88,6 +88,7 @@ pub struct gp_cap_opts {
     pub step_graphs: u32,
     pub m2l_compress: u32,
     pub fuse_leaf: u32,
+    pub skip_converged: u32,
 }
 
 #[repr(C)]

@@ -61,7 +61,8 @@ class GpCapOpts(C.Structure):
                 ("rhs_first", C.c_uint32), ("rhs_count", C.c_uint32),
                 ("inner_passes", C.c_uint32), ("inner_basis", C.c_uint32),
                 ("m2l_rank", C.c_uint32), ("step_graphs", C.c_uint32),
-                ("m2l_compress", C.c_uint32), ("fuse_leaf", C.c_uint32)]
+                ("m2l_compress", C.c_uint32), ("fuse_leaf", C.c_uint32),
+                ("skip_converged", C.c_uint32)]
 
 
 class GpParamMap(C.Structure):

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <string>
@@ -17,6 +18,7 @@
 #include "gp_fastcap.h"
 #include "gp_resistance.h"
 #include "gp_m2l_cpqr.h"
+#include "gp_parallel.h"
 #include "gp_noalloc.h"
 #include "gp_solver.h"
 #include "gp_tree.h"
@@ -204,6 +206,7 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->step_graphs = 1;
   o->m2l_compress = 1;  // bbfmm.rs:132-174
   o->fuse_leaf = 1;
+  o->skip_converged = 1;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,
@@ -282,10 +285,24 @@ int gp_cap_solve_batched(gp_ctx *ctx, const gp_panels *problems, uint32_t n_prob
     std::vector<double *> Cp(n_problems);
     size_t off = 0;
     for (uint32_t p = 0; p < n_problems; ++p) {
-      gp::panel_geometry(problems[p], gs[p]);
       Cp[p] = C + off;
-      off += (size_t)gs[p].ncond * gs[p].ncond;
+      off += (size_t)problems[p].n_conductors * problems[p].n_conductors;
     }
+    // one problem a thread; a bad problem's error (the lowest index's, as a
+    // sequential loop reports it) is rethrown
+    std::vector<std::exception_ptr> errs(n_problems);
+    gp::parallel_for(
+        (int)n_problems,
+        [&](int p) {
+          try {
+            gp::panel_geometry(problems[p], gs[p], false);
+          } catch (...) {
+            errs[p] = std::current_exception();
+          }
+        },
+        1);
+    for (auto &e : errs)
+      if (e) std::rethrow_exception(e);
     gp::cap_solve_batched(ctx, gs, o, Cp, reports);
   });
 }

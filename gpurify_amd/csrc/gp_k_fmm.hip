@@ -454,9 +454,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_l2p1w(int n, int nleaf, const int *__restrict__ leaf_lo,
                                                const int *__restrict__ leaf_box, int leaf_box0,
                                                const T *__restrict__ w1d,
-                                               const T *__restrict__ Lc, double *__restrict__ y) {
+                                               const T *__restrict__ Lc, double *__restrict__ y,
+                                               const unsigned char *__restrict__ skip) {
   const int li = blockIdx.x * 4 + (threadIdx.x >> 6), m = threadIdx.x & 63;
-  if (li >= nleaf) return;
+  if (li >= nleaf || (skip && skip[li])) return;  // wavefront-uniform
   const int n3 = n * n * n;
   const int mm = m < n3 ? m : 0;  // idle lanes read node 0
   const int a = mm / (n * n), b = (mm / n) % n, c = mm % n;
@@ -492,11 +493,12 @@ __global__ __launch_bounds__(256) void k_p2m1(int n_rt, int n3p, int tpl, int nl
                                               const T *__restrict__ w1d,
                                               const double *__restrict__ area,
                                               const T *__restrict__ z, T *__restrict__ M,
-                                              __bf16 *__restrict__ Mb) {
+                                              __bf16 *__restrict__ Mb,
+                                              const unsigned char *__restrict__ skip) {
   const int n = NF > 0 ? NF : n_rt;
   const int li = blockIdx.x * (256 / tpl) + threadIdx.x / tpl, m = threadIdx.x % tpl;
   const int n3 = n * n * n;
-  if (li >= nleaf || m >= n3) return;
+  if (li >= nleaf || m >= n3 || (skip && skip[li])) return;
   const int a = m / (n * n), b = (m / n) % n, c = m % n;
   T acc = 0;
 #pragma unroll 4
@@ -520,11 +522,13 @@ __global__ __launch_bounds__(256) void k_p2m1s(int n3p, int nleaf, const int *__
                                                const T *__restrict__ w1d,
                                                const double *__restrict__ area,
                                                const T *__restrict__ z, T *__restrict__ M,
-                                               __bf16 *__restrict__ Mb) {
+                                               __bf16 *__restrict__ Mb,
+                                               const unsigned char *__restrict__ skip) {
   constexpr int n = NF, n3 = n * n * n, W3 = 3 * n, CH = 64;
   __shared__ T sw[CH * W3];
   __shared__ T sq[CH];
   const int li = blockIdx.x, m = threadIdx.x;
+  if (skip && skip[li]) return;  // work-group-uniform, before any barrier
   const int lo = leaf_lo[li], hi = leaf_lo[li + 1];
   const int a = m / (n * n), b = (m / n) % n, c = m % n;
   T acc = 0;
@@ -552,10 +556,12 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_l2p1(int n, int N, const int *__restrict__ panel_leaf,
                                               const int *__restrict__ leaf_box, int leaf_box0,
                                               const T *__restrict__ w1d, const T *__restrict__ Lc,
-                                              double *__restrict__ y) {
+                                              double *__restrict__ y,
+                                              const unsigned char *__restrict__ skip) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const int li = panel_leaf[p], nn = n * n;
+  if (skip && skip[li]) return;
   const T *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * n * nn;
   const T *w = w1d + (size_t)p * 3 * n;
   T acc = 0;
@@ -579,11 +585,13 @@ __global__ __launch_bounds__(256) void k_l2p1v4(int N, const int *__restrict__ p
                                                 const int *__restrict__ leaf_box, int leaf_box0,
                                                 const float *__restrict__ w1d,
                                                 const float *__restrict__ Lc,
-                                                double *__restrict__ y) {
+                                                double *__restrict__ y,
+                                                const unsigned char *__restrict__ skip) {
   typedef float f4 __attribute__((ext_vector_type(4)));
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const int li = panel_leaf[p];
+  if (skip && skip[li]) return;
   const f4 *l4 = reinterpret_cast<const f4 *>(Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * 64);
   const f4 *w4 = reinterpret_cast<const f4 *>(w1d + (size_t)p * 12);
   f4 L[16];
@@ -615,16 +623,20 @@ __global__ __launch_bounds__(256) void k_m2m1(int n_rt, int n3p, int tpb, int bo
                                               const int *__restrict__ child_idx,
                                               const int *__restrict__ octant,
                                               const T *__restrict__ m2m1d, T *__restrict__ M,
-                                              __bf16 *__restrict__ Mb) {
+                                              __bf16 *__restrict__ Mb,
+                                              const unsigned char *__restrict__ skip) {
   const int n = NF > 0 ? NF : n_rt;
   extern __shared__ unsigned char smem_raw[];
   const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
   const int n3 = n * n * n, nn = n * n;
   T *s0 = reinterpret_cast<T *>(smem_raw) + (size_t)slot * 16 * tpb, *s1 = s0 + 8 * tpb;
   const int bi = blockIdx.x * (256 / tpb) + slot;
-  const bool live = bi < nb && m < n3;
   const int b = box0 + min(bi, nb - 1);
-  const int c0 = child_off[b], nch = bi < nb ? child_off[b + 1] - c0 : 0;
+  // a skipped box loads no child and stores nothing, but its threads pass
+  // every barrier with the others
+  const bool on = bi < nb && !(skip && skip[b]);
+  const bool live = on && m < n3;
+  const int c0 = child_off[b], nch = on ? child_off[b + 1] - c0 : 0;
   const int a = m / nn, bq = (m / n) % n, cq = m % n;
   int oc[8];
 #pragma unroll
@@ -677,21 +689,24 @@ template <typename T, int NF = 0>  // NF > 0: the order fixed at compile time (l
 __global__ __launch_bounds__(256) void k_l2l1(int n_rt, int tpb, int box0, int nb,
                                               const int *__restrict__ parent,
                                               const int *__restrict__ octant,
-                                              const T *__restrict__ m2m1d, T *__restrict__ Lc) {
+                                              const T *__restrict__ m2m1d, T *__restrict__ Lc,
+                                              const unsigned char *__restrict__ skip) {
   const int n = NF > 0 ? NF : n_rt;
   extern __shared__ unsigned char smem_raw[];
   const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
   const int n3 = n * n * n, nn = n * n;
   T *s0 = reinterpret_cast<T *>(smem_raw) + slot * 2 * tpb, *s1 = s0 + tpb;
   const int bi = blockIdx.x * (256 / tpb) + slot;
-  const bool live = bi < nb && m < n3;
   const int ch = box0 + min(bi, nb - 1);
+  // a skipped child loads and stores nothing; its threads pass every barrier
+  const bool off = skip && skip[ch];
+  const bool live = bi < nb && m < n3 && !off;
   const int p = parent[ch], o = octant[ch];
   const T *mx = m2m1d + (o * 3 + 0) * nn, *my = m2m1d + (o * 3 + 1) * nn,
           *mz = m2m1d + (o * 3 + 2) * nn;
   const int a = m / nn, bq = (m / n) % n, cq = m % n;
   const T own = live ? Lc[(size_t)ch * n3 + m] : (T)0;  // the child's M2L part, loaded early
-  if (m < n3) s0[m] = Lc[(size_t)p * n3 + m];
+  if (m < n3) s0[m] = off ? (T)0 : Lc[(size_t)p * n3 + m];
   __syncthreads();
   if (m < n3) {  // x^T: s1[ap][b][c] = sum_a mx[a][ap] s0[a][b][c]
     T t = 0;
@@ -1087,13 +1102,18 @@ __global__ __launch_bounds__(256) void k_m2l1t_bf16(int n3, int n_tiles,
                                                     const double *__restrict__ inv_half,
                                                     const __bf16 *__restrict__ G,
                                                     const __bf16 *__restrict__ Mb,
-                                                    float *__restrict__ Lc) {
+                                                    float *__restrict__ Lc,
+                                                    const unsigned char *__restrict__ skip) {
   constexpr int NP = 16 * MT, KS = NP / 32;
   constexpr bool APRE = MT * KS <= 8;  // G_o fragments of the next entry in registers
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tile = blockIdx.x * 4 + wave;
   if (tile >= n_tiles) return;
   const int li = lane & 15, lk = (lane >> 4) * 8;
+  // a skipped target column reads no source and is not written
+  const int t = tile_tgt[tile * 16 + li];
+  const bool off = t < 0 || (skip && skip[t]);
+  if (__all(off)) return;  // wavefront-uniform
   const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
   f32x4 acc[MT];
 #pragma unroll
@@ -1102,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_m2l1t_bf16(int n3, int n_tiles,
   bf16x8 b[KS], bn[KS];
   bf16x8 a[APRE ? MT : 1][APRE ? KS : 1], an[APRE ? MT : 1][APRE ? KS : 1];
   auto load_b = [&](int e, bf16x8 (&B)[KS]) {
-    const int sb = ent_src[(size_t)e * 16 + li];
+    const int sb = off ? -1 : ent_src[(size_t)e * 16 + li];
     const __bf16 *Ms = Mb + (size_t)(sb < 0 ? 0 : sb) * NP;
 #pragma unroll
     for (int k = 0; k < KS; ++k) B[k] = sb >= 0 ? *(const bf16x8 *)(Ms + k * 32 + lk) : zero;
@@ -1146,8 +1166,7 @@ __global__ __launch_bounds__(256) void k_m2l1t_bf16(int n3, int n_tiles,
     }
   }
   // D[i = node][j = target]: lane holds nodes 4 (lane / 16) + v of target lane % 16
-  const int t = tile_tgt[tile * 16 + li];
-  if (t < 0) return;
+  if (off) return;
   const float ih = (float)inv_half[box_level[t]];
   float *out = Lc + (size_t)t * n3;
 #pragma unroll
@@ -1179,7 +1198,8 @@ __global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles, int x
                                                       const double *__restrict__ inv_half,
                                                       const __bf16 *__restrict__ G,
                                                       const __bf16 *__restrict__ Mb,
-                                                      float *__restrict__ Lc) {
+                                                      float *__restrict__ Lc,
+                                                      const unsigned char *__restrict__ skip) {
   constexpr int NP = 16 * MT, KS = NP / 32, LDG = NP + 8;
   constexpr int VPR = NP / 8;                  // 16-byte vectors per G_o row
   constexpr int GPT = (NP * VPR + 255) / 256;  // G_o vectors per thread
@@ -1188,13 +1208,19 @@ __global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles, int x
   const int tile = xcd_block(blockIdx.x, gridDim.x, xrun);
   const int li = lane & 15, lk = (lane >> 4) * 8, col = wave * 16 + li;
   const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
+  // a skipped target column reads no source and is not written; a tile
+  // whose 64 columns are all skipped leaves at once (work-group-uniform,
+  // every thread votes before the first barrier)
+  const int t = tile_tgt[tile * 64 + col];
+  const bool off = t < 0 || (skip && skip[t]);
+  if (!__syncthreads_or(!off)) return;
   f32x4 acc[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) acc[mt] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const bf16x8 zero = {};
   bf16x8 b[KS], bn[KS], gp[GPT];
   auto load_b = [&](int e, bf16x8 (&B)[KS]) {
-    const int sb = ent_src[(size_t)e * 64 + col];
+    const int sb = off ? -1 : ent_src[(size_t)e * 64 + col];
     const __bf16 *Ms = Mb + (size_t)(sb < 0 ? 0 : sb) * NP;
 #pragma unroll
     for (int k = 0; k < KS; ++k) B[k] = sb >= 0 ? *(const bf16x8 *)(Ms + k * 32 + lk) : zero;
@@ -1242,8 +1268,7 @@ __global__ __launch_bounds__(256) void k_m2l1t64_bf16(int n3, int n_tiles, int x
     }
     __syncthreads();
   }
-  const int t = tile_tgt[tile * 64 + col];
-  if (t < 0) return;
+  if (off) return;
   const float ih = (float)inv_half[box_level[t]];
   float *out = Lc + (size_t)t * n3;
 #pragma unroll
@@ -1274,7 +1299,8 @@ __global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles, int xr
                                                      const double *__restrict__ inv_half,
                                                      const double *__restrict__ G,
                                                      const double *__restrict__ M,
-                                                     double *__restrict__ Lc) {
+                                                     double *__restrict__ Lc,
+                                                     const unsigned char *__restrict__ skip) {
   constexpr int NP = 16 * MT, K4 = NP / 4, LDG = NP + 2;
   constexpr int VPR = NP / 2;                  // double2 vectors per G_o row
   constexpr int GPT = (NP * VPR + 255) / 256;  // G_o vectors per thread
@@ -1284,13 +1310,17 @@ __global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles, int xr
   const int tile = xcd_block(blockIdx.x, gridDim.x, xrun);
   const int li = lane & 15, lq = lane >> 4, col = wave * 16 + li;
   const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
+  // skipped target columns as in k_m2l1t64_bf16
+  const int t = tile_tgt[tile * 64 + col];
+  const bool off = t < 0 || (skip && skip[t]);
+  if (!__syncthreads_or(!off)) return;
   f64x4 acc[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) acc[mt] = (f64x4){0.0, 0.0, 0.0, 0.0};
   double b[K4], bn[K4];
   f64x2 gp[GPT];
   auto load_b = [&](int e, double (&B)[K4]) {
-    const int sb = ent_src[(size_t)e * 64 + col];
+    const int sb = off ? -1 : ent_src[(size_t)e * 64 + col];
     const double *Ms = M + (size_t)(sb < 0 ? 0 : sb) * NP;
 #pragma unroll
     for (int k = 0; k < K4; ++k) B[k] = sb >= 0 ? Ms[4 * k + lq] : 0.0;
@@ -1338,8 +1368,7 @@ __global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles, int xr
     __syncthreads();
   }
   // D[i][j]: lane holds nodes lq + 4 v of target column li
-  const int t = tile_tgt[tile * 64 + col];
-  if (t < 0) return;
+  if (off) return;
   const double ih = inv_half[box_level[t]];
   double *out = Lc + (size_t)t * n3;
 #pragma unroll
@@ -1367,19 +1396,22 @@ __global__ __launch_bounds__(256) void k_m2l1_f64(int n3, int npairs, const int 
                                                   const double *__restrict__ inv_half,
                                                   const double *__restrict__ Go,
                                                   const double *__restrict__ M,
-                                                  double *__restrict__ Lc) {
+                                                  double *__restrict__ Lc,
+                                                  const unsigned char *__restrict__ skip) {
   constexpr int NP = 16 * MT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j0 = (blockIdx.x * 4 + wave) * 16;
   if (j0 >= npairs) return;
   const int li = lane & 15, lq = lane >> 4;
   const int jp = min(j0 + li, npairs - 1);
+  // a pair of a skipped target reads zeros and is not written
+  const bool off = skip && skip[tgt[jp]];
   const double *Ms = M + (size_t)src[jp] * NP;
   f64x4 acc[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) acc[mt] = (f64x4){0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < NP; k0 += 4) {
-    const double b = Ms[k0 + lq];
+    const double b = off ? 0.0 : Ms[k0 + lq];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       const double a = Go[(size_t)(mt * 16 + li) * NP + k0 + lq];
@@ -1387,7 +1419,7 @@ __global__ __launch_bounds__(256) void k_m2l1_f64(int n3, int npairs, const int 
     }
   }
   const int j = j0 + li;
-  if (j < npairs) {
+  if (j < npairs && !off) {
     const int t = tgt[j];
     const double ih = inv_half[box_level[t]];
     double *out = Lc + (size_t)t * n3;
@@ -1411,10 +1443,10 @@ void launch_m2l_by_offset(const FmmDev<double> &f, int n3, const double *M, doub
     const int *tg = f.po_tgt + p0, *sr = f.po_src + p0;
     const double *G = f.G + (size_t)o * n3p * n3p;
     switch (MT) {
-      case 2: hipLaunchKernelGGL((k_m2l1_f64<2>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
-      case 4: hipLaunchKernelGGL((k_m2l1_f64<4>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
-      case 8: hipLaunchKernelGGL((k_m2l1_f64<8>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
-      case 14: hipLaunchKernelGGL((k_m2l1_f64<14>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc); break;
+      case 2: hipLaunchKernelGGL((k_m2l1_f64<2>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc, f.skip_box); break;
+      case 4: hipLaunchKernelGGL((k_m2l1_f64<4>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc, f.skip_box); break;
+      case 8: hipLaunchKernelGGL((k_m2l1_f64<8>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc, f.skip_box); break;
+      case 14: hipLaunchKernelGGL((k_m2l1_f64<14>), grid, dim3(256), 0, s, n3, np, tg, sr, f.box_level, f.inv_half, G, M, Lc, f.skip_box); break;
       default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
     }
     GP_LAUNCH_CHECK();
@@ -1427,9 +1459,9 @@ void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *L
   if (f.tile_size == 64) {
     const dim3 grid(f.n_tiles);
     switch (f.n3p / 16) {
-      case 2: hipLaunchKernelGGL((k_m2l1t64_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
-      case 4: hipLaunchKernelGGL((k_m2l1t64_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
-      case 8: hipLaunchKernelGGL((k_m2l1t64_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+      case 2: hipLaunchKernelGGL((k_m2l1t64_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
+      case 4: hipLaunchKernelGGL((k_m2l1t64_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
+      case 8: hipLaunchKernelGGL((k_m2l1t64_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
       default: throw Error(GP_ERR_INTERNAL, "no 64-target M2L tiles for this n3p");
     }
     GP_LAUNCH_CHECK();
@@ -1438,10 +1470,10 @@ void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *L
   if (f.tile_size != 16) throw Error(GP_ERR_INTERNAL, "M2L target tiles of 16 or 64");
   const dim3 grid((f.n_tiles + 3) / 4);
   switch (f.n3p / 16) {
-    case 2: hipLaunchKernelGGL((k_m2l1t_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
-    case 4: hipLaunchKernelGGL((k_m2l1t_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
-    case 8: hipLaunchKernelGGL((k_m2l1t_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
-    case 14: hipLaunchKernelGGL((k_m2l1t_bf16<14>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc); break;
+    case 2: hipLaunchKernelGGL((k_m2l1t_bf16<2>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
+    case 4: hipLaunchKernelGGL((k_m2l1t_bf16<4>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
+    case 8: hipLaunchKernelGGL((k_m2l1t_bf16<8>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
+    case 14: hipLaunchKernelGGL((k_m2l1t_bf16<14>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
     default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
   }
   GP_LAUNCH_CHECK();
@@ -1451,8 +1483,8 @@ void launch_m2l_tiles_f64(const FmmDev<double> &f, int n3, const double *M, doub
                           int xrun, hipStream_t s) {
   const dim3 grid(f.n_tiles);
   switch (f.n3p / 16) {
-    case 2: hipLaunchKernelGGL((k_m2l1t64_f64<2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
-    case 4: hipLaunchKernelGGL((k_m2l1t64_f64<4>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc); break;
+    case 2: hipLaunchKernelGGL((k_m2l1t64_f64<2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc, f.skip_box); break;
+    case 4: hipLaunchKernelGGL((k_m2l1t64_f64<4>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.G, M, Lc, f.skip_box); break;
     default: throw Error(GP_ERR_INTERNAL, "no FP64 M2L tiles for this n3p");
   }
   GP_LAUNCH_CHECK();
@@ -2510,8 +2542,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   const bool tiled = (ldm % 16 == 0) && ldm <= 128 && n >= 2 && n <= 6;
   const int nrc32 = (nr + TM_RC - 1) / TM_RC;
   {
-    ProfScope ps(f32 ? "p2m_f32" : "p2m_f64", s, 2.0 * f.n_panels * n3 * nr,
-                 (double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes);
+    ProfScope ps(f32 ? "p2m_f32" : "p2m_f64", s, 2.0 * f.n_panels * n3 * nr * f.frac_panels,
+                 ((double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes) * f.frac_panels);
     if (ldm == 1 && nr == 1 && ldz == 1) {
       const int tpl = round_up(n3, 64), lpb = 256 / tpl;
       // one thread per (leaf, node), or (variant 1) one work-group per leaf
@@ -2522,10 +2554,11 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
 #define GP_P2M1(NF)                                                                              \
   if (v == 1 && NF > 0)                                                                          \
     hipLaunchKernelGGL((k_p2m1s<T, (NF > 0 ? NF : 3)>), dim3(nleaf), dim3(tpl), 0, s, n3p, nleaf, \
-                       f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb);                    \
+                       f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb, f.skip_leaf);       \
   else                                                                                           \
     hipLaunchKernelGGL((k_p2m1<T, NF>), dim3((nleaf + lpb - 1) / lpb), dim3(256), 0, s, n, n3p,  \
-                       tpl, nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb)
+                       tpl, nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb,         \
+                       f.skip_leaf)
           case 3: GP_P2M1(3); break;
           case 4: GP_P2M1(4); break;
           case 5: GP_P2M1(5); break;
@@ -2560,13 +2593,14 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     int nb = f.npar ? f.npar[l] : f.lvl_off[l + 1] - f.lvl_off[l];
     if (nb == 0) continue;
     int nch = f.lvl_off[l + 2] - f.lvl_off[l + 1];
-    ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch, (nb + nch) * exp_bytes);
+    ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch * f.frac_boxes,
+                 (nb + nch) * exp_bytes * f.frac_boxes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
       const int tpb = round_up(n3, 64), bpb = 256 / tpb;
 #define GP_M2M1(NF)                                                                             \
   hipLaunchKernelGGL((k_m2m1<T, NF>), dim3((nb + bpb - 1) / bpb), dim3(256), 4096 * sizeof(T), s, \
                      n, n3p, tpb, f.lvl_off[l], nb, f.child_off, f.child_idx, f.octant, f.m2m1d,   \
-                     M, mb)
+                     M, mb, f.skip_box)
       switch (n) {
         case 3: GP_M2M1(3); break;
         case 4: GP_M2M1(4); break;
@@ -2686,7 +2720,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     const int MT = m3p / 16, RT = (nr + 15) / 16;
     if (want_bf16 && by_offset && f.tile_tgt) {
       if constexpr (sizeof(T) == 4) {
-        ProfScope pl("m2l1t_bf16_mfma", s, fl, (double)f.n_pairs * m3p * 2 + f.n_targets * mbytes);
+        ProfScope pl("m2l1t_bf16_mfma", s, fl * f.frac_pairs,
+                     ((double)f.n_pairs * m3p * 2 + f.n_targets * mbytes) * f.frac_pairs);
         tuned_launch("m2l1t_bf16", f.n_tiles, m3p, f.tile_size, 2, s,
                      [&](int v) { launch_m2l_tiles(fm, m3, mbo, Lo, v ? 16 : 0, s); });
       }
@@ -2695,14 +2730,16 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset && f.tile_tgt &&
                f.tile_size == 64 && m3p <= 64) {
       if constexpr (sizeof(T) == 8) {
-        ProfScope pl("m2l1t_f64_mfma", s, fl, (double)f.n_pairs * m3p * 8 + f.n_targets * mbytes);
+        ProfScope pl("m2l1t_f64_mfma", s, fl * f.frac_pairs,
+                     ((double)f.n_pairs * m3p * 8 + f.n_targets * mbytes) * f.frac_pairs);
         tuned_launch("m2l1t_f64", f.n_tiles, m3p, f.tile_size, 2, s,
                      [&](int v) { launch_m2l_tiles_f64(fm, m3, Mo, Lo, v ? 16 : 0, s); });
       }
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {
       if (!f.po_off) throw Error(GP_ERR_INTERNAL, "single-RHS M2L pairs by offset not built");
       if constexpr (sizeof(T) == 8) {
-        ProfScope pl("m2l1_f64_mfma", s, fl, (double)f.n_pairs * (m3p * 8 + m3 * 16));
+        ProfScope pl("m2l1_f64_mfma", s, fl * f.frac_pairs,
+                     (double)f.n_pairs * (m3p * 8 + m3 * 16) * f.frac_pairs);
         launch_m2l_by_offset(fm, m3, Mo, Lo, s);
       }
     } else if (split) {
@@ -2799,12 +2836,13 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     int nb = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     if (nb == 0) continue;
     if (fuse_leaf && l == L - 1) continue;  // in k_l2l_l2p_leaf
-    ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nr * nb, 2.0 * nb * exp_bytes);
+    ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nr * nb * f.frac_boxes,
+                 2.0 * nb * exp_bytes * f.frac_boxes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
       const int tpb = round_up(n3, 64), bpb = 256 / tpb;
 #define GP_L2L1(NF)                                                                            \
   hipLaunchKernelGGL((k_l2l1<T, NF>), dim3((nb + bpb - 1) / bpb), dim3(256), 512 * sizeof(T), s, \
-                     n, tpb, f.lvl_off[l + 1], nb, f.parent, f.octant, f.m2m1d, Lc)
+                     n, tpb, f.lvl_off[l + 1], nb, f.parent, f.octant, f.m2m1d, Lc, f.skip_box)
       switch (n) {
         case 3: GP_L2L1(3); break;
         case 4: GP_L2L1(4); break;
@@ -2845,20 +2883,20 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     }
     return;
   }
-  ProfScope pp(f32 ? "l2p_f32" : "l2p_f64", s, 2.0 * f.n_panels * n3 * nr,
-               (double)f.n_panels * nr * 16.0 + nleaf * exp_bytes);
+  ProfScope pp(f32 ? "l2p_f32" : "l2p_f64", s, 2.0 * f.n_panels * n3 * nr * f.frac_panels,
+               ((double)f.n_panels * nr * 16.0 + nleaf * exp_bytes) * f.frac_panels);
   // (L2P accumulates into y, so it is not a tuner candidate: the tuner
   // launches every variant on the same outputs)
   if (ldm == 1 && nr == 1 && ldy == 1 && n == 4 && sizeof(T) == 4 && f.panel_leaf) {
     if constexpr (sizeof(T) == 4)
       hipLaunchKernelGGL(k_l2p1v4, dim3((f.n_panels + 255) / 256), dim3(256), 0, s, f.n_panels,
-                         f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y);
+                         f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf);
   } else if (ldm == 1 && nr == 1 && ldy == 1 && n3 <= 64)
     hipLaunchKernelGGL((k_l2p1w<T>), dim3((nleaf + 3) / 4), dim3(256), 0, s, n, nleaf, f.leaf_lo,
-                       f.leaf_box, leaf0, f.w1d, Lc, y);
+                       f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf);
   else if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf)
     hipLaunchKernelGGL((k_l2p1<T>), dim3((f.n_panels + 255) / 256), dim3(256), 0, s, n, f.n_panels,
-                       f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y);
+                       f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf);
   else if (tiled && sizeof(T) == 4 && n >= 3 && n <= 6 && ldm % 4 == 0 && ldm >= 16) {
     if constexpr (sizeof(T) == 4) {
       // points per lane: 16 / (64 / (ldm / 4)) rounded up

@@ -690,6 +690,29 @@ void kr_rows_axpy_clear(double *y, double *t, const double *w, int N, int ld, in
   GP_LAUNCH_CHECK();
 }
 
+// Skip flags of a batch from its systems' active flags: box b belongs to
+// system box_prob[b], leaf T is box leaf_box[T].  1 = the system has
+// converged (its kernels' work is skipped).
+__global__ __launch_bounds__(256) void k_skip_flags(int nb, int nl, const int *__restrict__ box_prob,
+                                                    const int *__restrict__ leaf_box,
+                                                    const int *__restrict__ active,
+                                                    unsigned char *__restrict__ skip_box,
+                                                    unsigned char *__restrict__ skip_leaf) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
+    if (i < nb)
+      skip_box[i] = active[box_prob[i]] == 0;
+    else
+      skip_leaf[i - nb] = active[box_prob[leaf_box[i - nb]]] == 0;
+  }
+}
+
+void kr_skip_flags(int nb, int nl, const int *box_prob, const int *leaf_box, const int *active,
+                   unsigned char *skip_box, unsigned char *skip_leaf, hipStream_t s) {
+  hipLaunchKernelGGL(k_skip_flags, dim3(grid_for((size_t)nb + nl)), dim3(256), 0, s, nb, nl,
+                     box_prob, leaf_box, active, skip_box, skip_leaf);
+  GP_LAUNCH_CHECK();
+}
+
 void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s) {
   hipLaunchKernelGGL(k_unit_rhs, dim3(grid_for((size_t)N * ld)), dim3(256), 0, s, cond, j0, N, ld,
                      nr, b);

@@ -101,12 +101,13 @@ __global__ __launch_bounds__(256) void k_near_spmv(
     int n, const int *__restrict__ panel_leaf, const int *__restrict__ leaf_lo,
     const int64_t *__restrict__ nf_off, const int *__restrict__ col_off,
     const int *__restrict__ cols, const TV *__restrict__ vals, const TX *__restrict__ x,
-    double *__restrict__ y) {
+    double *__restrict__ y, const unsigned char *__restrict__ skip) {
   using A = typename Acc<TX>::type;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
   const int T = panel_leaf[i];
+  if (skip && skip[T]) return;  // wavefront-uniform
   const int t = i - leaf_lo[T];
   const int c0 = col_off[T], S = col_off[T + 1] - c0;
   const TV *row = vals + nf_off[T] + (int64_t)t * S;
@@ -326,13 +327,14 @@ template <typename TV, typename TX, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf(
     const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
     const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
-    const TX *__restrict__ x, double *__restrict__ y) {
+    const TX *__restrict__ x, double *__restrict__ y, const unsigned char *__restrict__ skip) {
   using A = typename Acc<TX>::type;
   using V = typename Vec16<TV>::type;
   constexpr int VN = Vec16<TV>::n;
   extern __shared__ unsigned char smem_raw[];
   A *xs = reinterpret_cast<A *>(smem_raw);
   const int T = blockIdx.x;
+  if (skip && skip[T]) return;  // work-group-uniform, before the barrier
   const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
   const int c0 = col_off[T], S = col_off[T + 1] - c0;
   const int64_t base = nf_off[T];
@@ -369,13 +371,14 @@ template <typename TV, typename TX, int WPB, int R>
 __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf_pf(
     const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
     const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
-    const TX *__restrict__ x, double *__restrict__ y) {
+    const TX *__restrict__ x, double *__restrict__ y, const unsigned char *__restrict__ skip) {
   using A = typename Acc<TX>::type;
   using V = typename Vec16<TV>::type;
   constexpr int VN = Vec16<TV>::n;
   extern __shared__ unsigned char smem_raw[];
   A *xs = reinterpret_cast<A *>(smem_raw);
   const int T = blockIdx.x;
+  if (skip && skip[T]) return;  // work-group-uniform, before the barrier
   const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
   const int c0 = col_off[T], S = col_off[T + 1] - c0;
   const int64_t base = nf_off[T];
@@ -456,8 +459,10 @@ template <typename TV, typename TX>
 void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int ldx, double *y,
                          int ldy, int nr, hipStream_t s) {
   if (nf.n_leaves == 0) return;
-  const double flops = 2.0 * nf.nnz * nr;
-  const double bytes = (double)nf.nnz * sizeof(TV) + (double)nf.n * nr * (sizeof(TX) + sizeof(double));
+  // the work of the leaves not skipped (frac_nnz, frac_rows: their share)
+  const double flops = 2.0 * nf.nnz * nr * nf.frac_nnz;
+  const double bytes = (double)nf.nnz * sizeof(TV) * nf.frac_nnz +
+                       (double)nf.n * nr * (sizeof(TX) + sizeof(double)) * nf.frac_rows;
   ProfScope ps(sizeof(TV) == 4 ? (nr == 1 ? "near_spmv_f32" : "near_spmm_f32")
                                : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64"),
                s, flops, bytes);
@@ -466,7 +471,8 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
   // kernel variants, fastest per shape picked by the context's autotuner
   auto spmv_row = [&] {
     hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
-                       nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                       nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
+                       nf.skip_leaf);
   };
   auto spmm16 = [&] {
     const dim3 grid(nf.n_leaves * ((nr + NF_RC - 1) / NF_RC));
@@ -495,25 +501,25 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
         spmv_row();
       else if (v == 0)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 4>), dim3(nf.n_leaves), dim3(256), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
       else if (v == 1)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 8>), dim3(nf.n_leaves), dim3(512), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
       else if (v == 3)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 2>), dim3(nf.n_leaves), dim3(128), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
       else if (v == 4)
         hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 1>), dim3(nf.n_leaves), dim3(64), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
       else if (v == 5)
         hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 2>), dim3(nf.n_leaves), dim3(128),
-                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
       else if (v == 6)
         hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 3>), dim3(nf.n_leaves), dim3(128),
-                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
       else
         hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 4, 2>), dim3(nf.n_leaves), dim3(256),
-                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y);
+                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
     });
   } else if (nr > 16) {
     // FP32 matrix cores (64-column chunks, 16-row tiles), register tiles of

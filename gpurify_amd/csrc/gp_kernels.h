@@ -84,6 +84,10 @@ struct NearTiles {
   const int *cols = nullptr;        // source panel per column
   const int *panel_leaf = nullptr;  // n
   int max_cols = 0;                  // widest tile (source columns of one leaf)
+  // Single right-hand side: leaves flagged 1 are skipped (null = none); the
+  // profiler counts frac_nnz of the values and frac_rows of the rows.
+  const unsigned char *skip_leaf = nullptr;
+  double frac_nnz = 1.0, frac_rows = 1.0;
 };
 void launch_near_apply(const NearTiles &nf, const double *vals, const double *x, int ldx,
                        double *y, int ldy, int nr, hipStream_t s);
@@ -126,6 +130,13 @@ struct FmmDev {
   const double *area = nullptr;      // per sorted panel, scaled units
   int n3p = 0;                       // n^3 rounded up to a multiple of 32
   int fuse_leaf = 1;                 // FP32 blocks of RHS: finest L2L + L2P as one kernel
+  // Single right-hand side: work of boxes / leaves flagged 1 is skipped
+  // (skip_box per global box, skip_leaf per leaf index; null = none), for
+  // the problems of a batch whose GMRES has converged.  Problems never share
+  // a box or a near tile, so the others' results are unchanged.
+  const unsigned char *skip_box = nullptr, *skip_leaf = nullptr;
+  // the profiler's share of the work done while boxes are skipped
+  double frac_panels = 1.0, frac_boxes = 1.0, frac_pairs = 1.0;
   const T *w1d = nullptr;            // per sorted panel 3 * n
   const T *m2m1d = nullptr;          // 24 * n * n
   const T *G = nullptr;              // 343 * n3p * n3p (zero padded)
@@ -190,6 +201,10 @@ void kr_scale_cols(const double *in, TO *out, const double *scale, const int *ma
                    int ld, int nr, hipStream_t s);
 void kr_axpy(double *x, const double *y, double alpha, int N, int ld, int nr, hipStream_t s);
 void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s);
+// skip_box[b] = !active[box_prob[b]] over nb boxes, skip_leaf[T] likewise
+// for the nl leaves (leaf T is box leaf_box[T]).
+void kr_skip_flags(int nb, int nl, const int *box_prob, const int *leaf_box, const int *active,
+                   unsigned char *skip_box, unsigned char *skip_leaf, hipStream_t s);
 void kr_weight_rhs(const uint32_t *cond, const double *w, int j0, int N, int ld, int nr, double *b,
                    hipStream_t s);
 void kr_rows_axpy_clear(double *y, double *t, const double *w, int N, int ld, int nr,

@@ -38,6 +38,7 @@
 #include <numeric>
 
 #include "gp_kernels.h"
+#include "gp_parallel.h"
 #include "gp_m2l_cpqr.h"
 #include "gp_m2l_svd.h"
 
@@ -48,45 +49,6 @@ namespace {
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
-}
-
-// Host threads for the set-up loops over problems, leaves and boxes:
-// GPURIFY_HOST_THREADS, else the hardware's, at most 16.
-int host_threads() {
-  static const int nt = [] {
-    const char *e = std::getenv("GPURIFY_HOST_THREADS");
-    int v = e && *e ? std::atoi(e) : (int)std::thread::hardware_concurrency();
-    return std::max(1, std::min(16, v));
-  }();
-  return nt;
-}
-
-// f(i) for i in [0, n) on the set-up threads (dynamic chunks of `grain`:
-// 64 for leaves and boxes, 1 for problems of very different sizes); the
-// first exception is rethrown after all threads end.
-template <typename F>
-void parallel_for(int n, F &&f, int grain = 64) {
-  const int nt = std::min(host_threads(), (n + grain - 1) / grain);
-  if (nt <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::atomic<int> next(0);
-  std::exception_ptr err;
-  std::mutex m;
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; ++t)
-    th.emplace_back([&] {
-      try {
-        for (int i0; (i0 = next.fetch_add(grain)) < n;)
-          for (int i = i0; i < std::min(n, i0 + grain); ++i) f(i);
-      } catch (...) {
-        std::lock_guard<std::mutex> lk(m);
-        if (!err) err = std::current_exception();
-      }
-    });
-  for (auto &t : th) t.join();
-  if (err) std::rethrow_exception(err);
 }
 
 // Progress lines on stderr when GPURIFY_LOG is set (long solves).
@@ -392,6 +354,7 @@ struct FmmSystem {
   long pairs = 0;
   bool have32 = false;
   bool m2l_compress = false;  // far-field M2L blocks hold the CPQR truncation
+  bool skip_converged = true;  // a batch skips its converged problems' work
   int n3p = 0;
   int m2l_inner = GP_M2L_BF16, m2l_outer = GP_M2L_FP64;
   DBuf<double> verts, cen, area, inv_diag;
@@ -433,6 +396,10 @@ struct FmmSystem {
   DBuf<int64_t> tr_base;
   DBuf<double> dinv_area, inv_area, ytmp;
   NearTranspose ntr;
+  // A batch: problem of each global box, skip flags, per-problem work
+  DBuf<int> box_prob;
+  DBuf<unsigned char> skip_box, skip_leaf;
+  std::vector<double> p_panels, p_nnz, p_pairs, p_boxes;
 
   const Tree &t() const { return trees[0]; }
   double L0() const { return L0s[0]; }
@@ -738,6 +705,7 @@ struct FmmSystem {
     // relies on G_-o = G_o^T, which the truncation does not keep, so it
     // applies the full matrices.
     m2l_compress = o.m2l_compress != 0 && !want_transpose;
+    skip_converged = o.skip_converged != 0;
     const auto m2l_tab = m2l_tables(n, n3p, m2l_compress);
     const std::vector<double> &hG = *m2l_tab;
     phase("tables");
@@ -1057,6 +1025,67 @@ struct FmmSystem {
     }
     coff = to_device(hco, s);
     cidx = to_device(hci, s);
+    // A batch: the problem of every box, for skipping converged problems,
+    // and each problem's share of the work, for the profiler's accounting.
+    if (P > 1) {
+      box_prob = to_device(gp_of, s);
+      skip_box.alloc(nb);
+      skip_leaf.alloc(n_leaves);
+      p_panels.assign(P, 0.0);
+      p_nnz.assign(P, 0.0);
+      p_pairs.assign(P, 0.0);
+      p_boxes.assign(P, 0.0);
+      for (int p = 0; p < P; ++p) p_panels[p] = probs[p]->n;
+      for (int T = 0; T < n_leaves; ++T) p_nnz[gp_of[hleaf_box[T]]] += (double)(hnf[T + 1] - hnf[T]);
+      for (int b = 0; b < nb; ++b) {
+        p_pairs[gp_of[b]] += hil_off[b + 1] - hil_off[b];
+        p_boxes[gp_of[b]] += 1.0;
+      }
+    }
+  }
+
+  // The operator applies that follow skip the work of the batch's problems
+  // whose system is inactive (d_active, one flag per problem, read on the
+  // device when the applies run); h_active, when given, is its host copy for
+  // the profiler's accounting.  No effect on one problem.
+  void mask(const int *d_active, const int *h_active) {
+    if (!box_prob.p) return;
+    kr_skip_flags(f64.n_boxes, n_leaves, box_prob.p, leaf_box.p, d_active, skip_box.p,
+                  skip_leaf.p, s);
+    double fr[4] = {1.0, 1.0, 1.0, 1.0};  // panels, nnz, pairs, boxes
+    if (h_active) {
+      double a[4] = {0, 0, 0, 0}, t[4] = {0, 0, 0, 0};
+      for (size_t p = 0; p < p_panels.size(); ++p) {
+        const double *v[4] = {&p_panels[p], &p_nnz[p], &p_pairs[p], &p_boxes[p]};
+        for (int q = 0; q < 4; ++q) {
+          t[q] += *v[q];
+          if (h_active[p]) a[q] += *v[q];
+        }
+      }
+      for (int q = 0; q < 4; ++q) fr[q] = t[q] > 0 ? a[q] / t[q] : 1.0;
+    }
+    auto set = [&](auto &f) {
+      f.skip_box = skip_box.p;
+      f.skip_leaf = skip_leaf.p;
+      f.frac_panels = fr[0];
+      f.frac_pairs = fr[2];
+      f.frac_boxes = fr[3];
+    };
+    set(f32);
+    set(f64);
+    nt.skip_leaf = skip_leaf.p;
+    nt.frac_nnz = fr[1];
+    nt.frac_rows = fr[0];
+  }
+  void unmask() {
+    auto clr = [](auto &f) {
+      f.skip_box = f.skip_leaf = nullptr;
+      f.frac_panels = f.frac_pairs = f.frac_boxes = 1.0;
+    };
+    clr(f32);
+    clr(f64);
+    nt.skip_leaf = nullptr;
+    nt.frac_nnz = nt.frac_rows = 1.0;
   }
 
   void ensure_work(int ld_) {
@@ -1181,6 +1210,8 @@ struct GmresStats {
 // state stride ld.  SegOps: a batch of problems stacked in one column, each
 // system a row segment, per-system state stride nsys.
 struct BlockOps {
+  // a block of RHS of one problem: the operator applies every column
+  static constexpr bool maskable = false;
   int N = 0, ld = 0, nr = 0;
   hipStream_t s = nullptr;
   int nsys() const { return nr; }
@@ -1211,6 +1242,8 @@ struct BlockOps {
 };
 
 struct SegOps {
+  // a batch of problems, one system each: converged problems are skipped
+  static constexpr bool maskable = true;
   SegMap sm;
   hipStream_t s = nullptr;
   int nsys() const { return sm.nsys; }
@@ -1395,7 +1428,12 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
     for (int k = 0; k < m; ++k) {
       ensure_V(k + 2);
       auto step = [&] {
+        // a batch skips the operator work of its inactive systems: their
+        // new basis vectors are zero (inv_h = 0) and never enter x
+        if constexpr (Ops::maskable)
+          if (sys.skip_converged) sys.mask(active.p, graphs ? nullptr : h_active);
         sys.apply(V.p + (size_t)k * sv, true, nc, w.p, prec);
+        if constexpr (Ops::maskable) sys.unmask();
         ops.dots(V.p, sv, k + 1, w.p, partial.p, h1.p, false);
         ops.axpy_multi(w.p, V.p, sv, h1.p, k + 1, -1.0);
         if (passes > 1) {

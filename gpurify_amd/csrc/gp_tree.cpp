@@ -8,6 +8,10 @@
 
 #include "gp_tree.h"
 
+#include <atomic>
+
+#include "gp_parallel.h"
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -121,27 +125,51 @@ void check_panel(const gp_panels &in, int i) {
       throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " has a non-finite vertex");
 }
 
-void panel_geometry(const gp_panels &in, PanelGeom &g) {
+void panel_geometry(const gp_panels &in, PanelGeom &g, bool parallel) {
   if (in.n_panels == 0) throw Error(GP_ERR_EMPTY, "no panels in geometry");
   if (!in.vertices || !in.conductor) throw Error(GP_ERR_INVALID_INPUT, "null panel arrays");
   if (in.n_conductors == 0) throw Error(GP_ERR_INVALID_INPUT, "n_conductors must be > 0");
   g.n = (int)in.n_panels;
   g.ncond = (int)in.n_conductors;
-  g.verts.assign(in.vertices, in.vertices + (size_t)g.n * 12);
-  g.cond.assign(in.conductor, in.conductor + g.n);
+  g.verts = in.vertices;
+  g.cond = in.conductor;
   g.cx.resize(g.n);
   g.cy.resize(g.n);
   g.cz.resize(g.n);
   g.area.resize(g.n);
-  for (int i = 0; i < g.n; ++i) {
-    check_panel(in, i);
-    double c[3];
-    if (!panel_props(&g.verts[(size_t)i * 12], c, g.area[i]))
-      throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " has zero area");
-    g.cx[i] = c[0];
-    g.cy[i] = c[1];
-    g.cz[i] = c[2];
-  }
+  // chunks of panels on the set-up threads; a bad panel only raises a flag,
+  // and a sequential pass then reports the first one
+  constexpr int CH = 4096;
+  std::atomic<bool> bad(false);
+  auto chunk = [&](int q) {
+    const int i1 = std::min(g.n, (q + 1) * CH);
+    for (int i = q * CH; i < i1; ++i) {
+      const double *v = in.vertices + (size_t)i * 12;
+      bool ok = in.conductor[i] < in.n_conductors;
+      for (int k = 0; k < 12; ++k) ok &= std::isfinite(v[k]);
+      double c[3] = {0, 0, 0};
+      ok = ok && panel_props(v, c, g.area[i]);
+      if (!ok) {
+        bad = true;
+        return;
+      }
+      g.cx[i] = c[0];
+      g.cy[i] = c[1];
+      g.cz[i] = c[2];
+    }
+  };
+  const int nch = (g.n + CH - 1) / CH;
+  if (parallel)
+    parallel_for(nch, chunk, 1);
+  else
+    for (int q = 0; q < nch; ++q) chunk(q);
+  if (bad)
+    for (int i = 0; i < g.n; ++i) {
+      check_panel(in, i);
+      double c[3], a;
+      if (!panel_props(in.vertices + (size_t)i * 12, c, a))
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) + " has zero area");
+    }
 }
 
 int choose_depth(const PanelGeom &g, int leaf_target) {

@@ -9,18 +9,20 @@
 namespace gp {
 
 // Solve-time panel set (integrals/panel.rs:78-119): centroids, areas and
-// conductor ids of the caller's panels, in the caller's order.
+// conductor ids of the caller's panels, in the caller's order.  The vertices
+// and conductor ids are the caller's arrays, borrowed for the call.
 struct PanelGeom {
   int n = 0;
   int ncond = 0;
-  std::vector<double> verts;  // n * 12, metres
+  const double *verts = nullptr;   // n * 12, metres
   std::vector<double> cx, cy, cz, area;
-  std::vector<uint32_t> cond;
+  const uint32_t *cond = nullptr;  // n
 };
 
 // Validates the boundary input and computes centroids/areas with the
-// reference formulas (panel.rs:29-67).  Throws gp::Error on bad input.
-void panel_geometry(const gp_panels &in, PanelGeom &g);
+// reference formulas (panel.rs:29-67), on the set-up threads when parallel
+// is set.  Throws gp::Error on bad input (the first bad panel's message).
+void panel_geometry(const gp_panels &in, PanelGeom &g, bool parallel = true);
 // The checks of panel_geometry for panel i (conductor id, finite vertices).
 void check_panel(const gp_panels &in, int i);
 // Area-weighted centroid c and area of one panel's 4 vertices (q: 12

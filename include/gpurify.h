@@ -131,6 +131,10 @@ typedef struct {
                             adjoint solves of gp_cap_jacobian always apply the full matrices. */
   uint32_t fuse_leaf;    /* 1 (default) = FP32 blocks of RHS at orders <= 5 run the finest L2L and
                             L2P as one kernel; 0 = two launches (the same sums). */
+  uint32_t skip_converged; /* gp_cap_solve_batched: 1 (default) = the operator applies of a GMRES
+                            step skip the problems whose system has converged (their basis
+                            vector is zero; the others' results are unchanged bit for bit);
+                            0 = every problem in every step. */
 } gp_cap_opts;
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 

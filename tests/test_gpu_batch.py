@@ -63,3 +63,18 @@ def test_batched_rejects_bad_problem(ctx, probs):
     with pytest.raises(G.GpError) as ei:
         ctx.cap_solve_batched(bad)
     assert ei.value.code == G._lib.GP_ERR_INVALID_INPUT
+
+
+@pytest.mark.parametrize("prec", [G.GP_PREC_FP64, G.GP_PREC_MIXED])
+def test_skipping_converged_problems_changes_nothing(ctx, probs, prec):
+    # the operator applies of a batched GMRES step skip the problems whose
+    # system has converged; problems share no box or tile, so every result
+    # must come out bit for bit as with every problem applied in every step
+    common = dict(method=G.GP_METHOD_FMM, tol=1e-10, rhs_count=1, max_level=0, leaf_target=32,
+                  precision=prec)
+    Cs, rs = ctx.cap_solve_batched(probs, G.default_opts(skip_converged=1, **common))
+    Ca, ra = ctx.cap_solve_batched(probs, G.default_opts(skip_converged=0, **common))
+    assert len({r["iterations"] for r in rs}) > 1  # the systems converge at different steps
+    for a, b, x, y in zip(Cs, Ca, rs, ra):
+        assert x["converged"] == 1 and x["iterations"] == y["iterations"]
+        assert np.array_equal(a, b)
