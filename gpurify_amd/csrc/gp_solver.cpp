@@ -71,6 +71,10 @@ bool log_enabled() {
 
 constexpr double kMinPositive = 2.2250738585072014e-308;
 
+// Arnoldi steps per replayed graph of a batched GMRES (one host check of the
+// convergence flags per chunk).
+constexpr int kBatchGraphSteps = 4;
+
 // Sensitivity parameters evaluated together when they move the same panels.
 #ifndef GP_SENS_GROUP
 #define GP_SENS_GROUP 2
@@ -1275,10 +1279,10 @@ struct SegOps {
 
 // GMRES workspace, kept across the solves of one system (the refinement
 // steps of the mixed solver), with the captured Arnoldi steps that replay on
-// it: graph k holds step k of a cycle (operator apply, Gram-Schmidt passes,
-// norm, Givens update, the scaled new basis vector and the copy of the
-// active flags to the host), valid for the current basis allocation and the
-// scalars it was captured with.
+// it: graph k0 holds the steps k0 .. k0 + g_chunk - 1 of a cycle (each: the
+// operator apply, Gram-Schmidt passes, norm, Givens update and the scaled new
+// basis vector; then the copy of the active flags to the host), valid for
+// the current basis allocation and the scalars it was captured with.
 template <typename TV>
 struct GmresWork {
   int m = 0, S = 0, vcap = 0;
@@ -1289,7 +1293,7 @@ struct GmresWork {
   std::unique_ptr<Pinned> pin;
   std::vector<hipGraphExec_t> graphs;
   double g_tol = -1.0;
-  int g_passes = 0, g_prec = -1, g_nc = 0, g_nr = 0;
+  int g_passes = 0, g_prec = -1, g_nc = 0, g_nr = 0, g_chunk = 0;
   void clear_graphs() {
     for (auto e : graphs)
       if (e) (void)hipGraphExecDestroy(e);
@@ -1425,9 +1429,20 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
     if (nact == 0) break;
     ops.scale(rv.p, V.p, inv_beta.p, active.p);
     int ksteps = 0;
-    for (int k = 0; k < m; ++k) {
-      ensure_V(k + 2);
-      auto step = [&] {
+    // Steps per host check: with graphs, a batch (whose converged systems'
+    // work the kernels skip) replays chunks of kBatchGraphSteps steps and
+    // reads the active flags once per chunk; steps after a system converges
+    // leave its state unchanged (inv_h = 0, kused fixed), so the answer is
+    // the one-step-at-a-time answer.
+    const int chunk = (graphs && Ops::maskable && sys.skip_converged) ? kBatchGraphSteps : 1;
+    if (graphs && W.g_chunk != chunk) {
+      W.clear_graphs();
+      W.g_chunk = chunk;
+    }
+    for (int k0 = 0, len = 0; k0 < m; k0 += len) {
+      len = std::min(chunk, m - k0);
+      ensure_V(k0 + len + 1);
+      auto step = [&](int k) {
         // a batch skips the operator work of its inactive systems: their
         // new basis vectors are zero (inv_h = 0) and never enter x
         if constexpr (Ops::maskable)
@@ -1444,15 +1459,18 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
         kr_gmres_givens(k, m, nr, S, h1.p, passes > 1 ? h2.p : nullptr, hn.p, H.p, cs.p, sn.p,
                         g.p, bnorm.p, tol, active.p, kused.p, est.p, inv_h.p, s);
         ops.scale(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, nullptr);
+      };
+      auto steps = [&] {
+        for (int k = k0; k < k0 + len; ++k) step(k);
         GP_HIP(hipMemcpyAsync(h_active, active.p, nr * sizeof(int), hipMemcpyDeviceToHost, s));
       };
       if (graphs) {
-        if ((int)W.graphs.size() <= k) W.graphs.resize(k + 1, nullptr);
-        if (!W.graphs[k]) {
+        if ((int)W.graphs.size() <= k0) W.graphs.resize(k0 + 1, nullptr);
+        if (!W.graphs[k0]) {
           hipGraph_t gr = nullptr;
           GP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
           try {
-            step();
+            steps();
           } catch (...) {
             (void)hipStreamEndCapture(s, &gr);
             if (gr) (void)hipGraphDestroy(gr);
@@ -1463,18 +1481,18 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
           const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
           (void)hipGraphDestroy(gr);
           GP_HIP(e);
-          W.graphs[k] = ex;
+          W.graphs[k0] = ex;
         }
-        GP_HIP(hipGraphLaunch(W.graphs[k], s));
+        GP_HIP(hipGraphLaunch(W.graphs[k0], s));
       } else {
-        step();
+        steps();
       }
-      ksteps = k + 1;
+      ksteps = k0 + len;
       GP_HIP(hipStreamSynchronize(s));
       int na = 0;
       for (int r = 0; r < nr; ++r) na += h_active[r];
-      if (k % 25 == 24)
-        GP_LOG("gmres prec %d cycle %d iteration %d: %d of %d systems active", prec, cycle, k + 1,
+      if (ksteps / 25 != k0 / 25)
+        GP_LOG("gmres prec %d cycle %d iteration %d: %d of %d systems active", prec, cycle, ksteps,
                na, nr);
       if (!na) break;
     }

@@ -122,6 +122,8 @@ typedef struct {
   uint32_t step_graphs;  /* GP_PREC_MIXED: 1 (default) = from the second refinement step on,
                             each inner GMRES step (operator apply, orthogonalisation, Givens
                             update) runs as a HIP graph captured on first use and replayed;
+                            a batch (gp_cap_solve_batched with skip_converged) replays four
+                            steps per graph and reads the convergence flags once per graph;
                             0 = kernels launched one by one.  Off while profiling. */
   uint32_t m2l_compress; /* 1 (default) = the reference's truncated M2L operators: every far-field
                             interaction matrix compressed by column-pivoted QR, truncated where the
