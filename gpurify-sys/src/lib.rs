@@ -203,6 +203,9 @@ extern "C" {
     pub fn gp_cap_solve_dielectric(ctx: *mut gp_ctx, panels: *const gp_panels,
                                    roles: *const gp_panel_role, c: *mut f64,
                                    report: *mut gp_cap_report) -> c_int;
+    pub fn gp_cap_solve_dielectric_fmm(ctx: *mut gp_ctx, panels: *const gp_panels,
+                                       roles: *const gp_panel_role, opts: *const gp_cap_opts,
+                                       c: *mut f64, report: *mut gp_cap_report) -> c_int;
     pub fn gp_qui_parse(text: *const c_char, text_len: usize, scale: f64, n_panels: *mut u32,
                         n_conductors: *mut u32, names_len: *mut usize, vertices: *mut f64,
                         conductor: *mut u32, n_vertices: *mut u8, names: *mut c_char) -> c_int;

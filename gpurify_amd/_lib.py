@@ -105,6 +105,7 @@ EXPORTS = [
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
     "gp_tune_report", "gp_qui_parse", "gp_solve_resistance", "gp_cap_solve_dielectric",
     "gp_m2l_operator", "gp_cap_solve_dense_noalloc", "gp_device_synchronize",
+    "gp_cap_solve_dielectric_fmm",
 ]
 
 _lib = None
@@ -165,6 +166,10 @@ def load():
     L.gp_cap_solve_dielectric.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(GpPanelRole),
                                           C.POINTER(C.c_double), C.POINTER(GpCapReport)]
     L.gp_cap_solve_dielectric.restype = C.c_int
+    L.gp_cap_solve_dielectric_fmm.argtypes = [vp, C.POINTER(GpPanels), C.POINTER(GpPanelRole),
+                                              C.POINTER(GpCapOpts), C.POINTER(C.c_double),
+                                              C.POINTER(GpCapReport)]
+    L.gp_cap_solve_dielectric_fmm.restype = C.c_int
     L.gp_qui_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_double, C.POINTER(C.c_uint32),
                                C.POINTER(C.c_uint32), C.POINTER(C.c_size_t), C.POINTER(C.c_double),
                                C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_char_p]
@@ -304,11 +309,12 @@ class Context:
                                                        C.byref(rep)))
         return Cm, rep.as_dict()
 
-    def cap_solve_dielectric(self, verts4, role, cond, eps, eps_out, eps_in, ref, ncond):
-        """cap::solve_dielectric on the device (dense FP64): C (ncond x ncond,
-        F, permittivities applied).  role[k] 0 = conductor (cond[k], eps[k]
-        surrounding), 1 = dielectric interface (eps_out[k], eps_in[k], ref[k]
-        a point on the inner side)."""
+    def cap_solve_dielectric(self, verts4, role, cond, eps, eps_out, eps_in, ref, ncond,
+                             opts: GpCapOpts | None = None):
+        """cap::solve_dielectric on the device: C (ncond x ncond, F,
+        permittivities applied), dense FP64, or with opts the FMM form.
+        role[k] 0 = conductor (cond[k], eps[k] surrounding), 1 = dielectric
+        interface (eps_out[k], eps_in[k], ref[k] a point on the inner side)."""
         v = np.ascontiguousarray(verts4, dtype=np.float64).reshape(-1, 4, 3)
         n = v.shape[0]
         c = np.ascontiguousarray(np.where(np.asarray(role) == 0, cond, 0), dtype=np.uint32)
@@ -325,8 +331,13 @@ class Context:
         p = GpPanels(n, ncond, _dptr(v), _uptr(c))
         Cm = np.zeros((ncond, ncond))
         rep = GpCapReport()
-        self._check(self._L.gp_cap_solve_dielectric(self._h, C.byref(p), roles, _dptr(Cm),
-                                                     C.byref(rep)))
+        if opts is None:
+            self._check(self._L.gp_cap_solve_dielectric(self._h, C.byref(p), roles, _dptr(Cm),
+                                                         C.byref(rep)))
+        else:  # the FMM form (gp_cap_solve_dielectric_fmm)
+            self._check(self._L.gp_cap_solve_dielectric_fmm(self._h, C.byref(p), roles,
+                                                             C.byref(opts), _dptr(Cm),
+                                                             C.byref(rep)))
         return Cm, rep.as_dict()
 
     def cap_jacobian(self, verts4, cond, ncond, params, opts: GpCapOpts | None = None):

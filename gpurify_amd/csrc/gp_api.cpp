@@ -265,6 +265,27 @@ int gp_cap_solve_dielectric(gp_ctx *ctx, const gp_panels *panels, const gp_panel
   });
 }
 
+int gp_cap_solve_dielectric_fmm(gp_ctx *ctx, const gp_panels *panels, const gp_panel_role *roles,
+                                const gp_cap_opts *opts, double *C, gp_cap_report *report) {
+  if (!ctx) {
+    g_thread_error = "gp_cap_solve_dielectric_fmm: ctx is NULL";
+    return GP_ERR_INVALID_INPUT;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return guarded(ctx, [&] {
+    if (!panels || !roles || !C) throw gp::Error(GP_ERR_INVALID_INPUT, "null panels, roles or C");
+    gp_cap_opts o;
+    if (opts)
+      o = *opts;
+    else
+      gp_cap_opts_default(&o);
+    GP_HIP(hipSetDevice(ctx->device));
+    gp::PanelGeom g;
+    gp::panel_geometry(*panels, g);
+    gp::solve_dielectric_fmm(ctx, g, roles, o, C, report);
+  });
+}
+
 int gp_cap_solve_batched(gp_ctx *ctx, const gp_panels *problems, uint32_t n_problems,
                          const gp_cap_opts *opts, double *C, gp_cap_report *reports) {
   if (!ctx) {

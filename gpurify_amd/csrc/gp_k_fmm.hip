@@ -2513,6 +2513,97 @@ void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, c
   GP_LAUNCH_CHECK();
 }
 
+// d/dx of each panel's 1-D interpolation weights (the sums of k_interp_w1d
+// differentiated: T_k' = k U_{k-1}), divided by its leaf's half-width: the
+// gradient weights of the local expansion in the unit frame.
+__global__ __launch_bounds__(256) void k_interp_dw1d(int N, int n,
+                                                     const int *__restrict__ panel_leaf,
+                                                     const double *__restrict__ geo,
+                                                     const double *__restrict__ cen,
+                                                     const double *__restrict__ nodes,
+                                                     double *__restrict__ dw) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * 3) return;
+  const int i = e / 3, d = e % 3;
+  const double *g = geo + (size_t)panel_leaf[i] * 4;
+  const double x = (cen[(size_t)i * 3 + d] - g[d]) / g[3];
+  double u[8], dt[8];  // U_k(x), T_k'(x)
+  u[0] = 1.0;
+  u[1] = 2.0 * x;
+  for (int k = 2; k < n; ++k) u[k] = 2.0 * x * u[k - 1] - u[k - 2];
+  dt[0] = 0.0;
+  for (int k = 1; k < n; ++k) dt[k] = k * u[k - 1];
+  for (int m = 0; m < n; ++m) {
+    const double xm = nodes[m];
+    double s = 0.0, tkm1 = 1.0, tk = xm;
+    for (int k = 1; k < n; ++k) {
+      if (k > 1) {
+        const double t = 2.0 * xm * tk - tkm1;
+        tkm1 = tk;
+        tk = t;
+      }
+      s += tk * dt[k];
+    }
+    dw[(size_t)e * n + m] = 2.0 / n * s / g[3];
+  }
+}
+
+void launch_interp_dw1d(int N, int n, const int *panel_leaf, const double *geo, const double *cen,
+                        const double *nodes, double *dw, hipStream_t s) {
+  if (N == 0) return;
+  if (n < 2 || n > 8) throw Error(GP_ERR_INTERNAL, "interpolation order out of range");
+  hipLaunchKernelGGL(k_interp_dw1d, dim3((N * 3 + 255) / 256), dim3(256), 0, s, N, n, panel_leaf,
+                     geo, cen, nodes, dw);
+  GP_LAUNCH_CHECK();
+}
+
+// The far field of the dielectric formulation, one thread per (panel, RHS):
+// a conductor panel takes the potential yfar (from run_far_field), an
+// interface panel minus the normal derivative of its leaf's local expansion
+// at its centroid, sum over nodes of (n . grad S) L (Lc: [box][node][RHS],
+// row stride ldm).
+__global__ __launch_bounds__(256) void k_far_dielectric(
+    int N, int n, int nr, int ldm, int ldy, const int *__restrict__ panel_leaf,
+    const int *__restrict__ leaf_box, int leaf0, const double *__restrict__ w1d,
+    const double *__restrict__ dw1d, const double *__restrict__ nrm,
+    const unsigned char *__restrict__ iface, const double *__restrict__ Lc,
+    const double *__restrict__ yfar, double *__restrict__ y) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * nr) return;
+  const int p = (int)(e / nr), r = (int)(e % nr);
+  const size_t o = (size_t)p * ldy + r;
+  if (!iface[p]) {
+    y[o] += yfar[o];
+    return;
+  }
+  const int li = panel_leaf[p], n3 = n * n * n;
+  const double *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf0 + li) * n3 * ldm;
+  const double *w = w1d + (size_t)p * 3 * n, *dw = dw1d + (size_t)p * 3 * n;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b)
+      for (int c = 0; c < n; ++c) {
+        const double l = ll[(size_t)((a * n + b) * n + c) * ldm + r];
+        gx += dw[a] * w[n + b] * w[2 * n + c] * l;
+        gy += w[a] * dw[n + b] * w[2 * n + c] * l;
+        gz += w[a] * w[n + b] * dw[2 * n + c] * l;
+      }
+  y[o] -= nrm[3 * p] * gx + nrm[3 * p + 1] * gy + nrm[3 * p + 2] * gz;
+}
+
+void launch_far_dielectric(const FmmDev<double> &f, int nr, int ldm, int ldy, const double *dw1d,
+                           const double *nrm, const unsigned char *iface, const double *Lc,
+                           const double *yfar, double *y, hipStream_t s) {
+  const size_t total = (size_t)f.n_panels * nr;
+  if (total == 0) return;
+  const int leaf0 = f.lvl_off[f.L];
+  ProfScope ps("far_dielectric", s, 0.0, 0.0);
+  hipLaunchKernelGGL(k_far_dielectric, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     f.n_panels, f.n, nr, ldm, ldy, f.panel_leaf, f.leaf_box, leaf0, f.w1d, dw1d,
+                     nrm, iface, Lc, yfar, y);
+  GP_LAUNCH_CHECK();
+}
+
 template <typename T>
 void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf16 *Mb, T *Lc,
                      int ldm, int m2l, double *y, int ldy, hipStream_t s, T *Mt, __bf16 *Mtb,

@@ -323,6 +323,55 @@ __global__ __launch_bounds__(256) void k_near_assemble(
   }
 }
 
+// Dielectric-interface rows of the near-field tiles (the rows of
+// cap/dielectric.rs:107-121 restricted to the near leaves): entry (i, l) of
+// an interface panel i becomes -F(l, c_i, n_i), the 7-point normal-field
+// integral, and the diagonal the jump term diag[i] (unit frame: 2 pi
+// (eps_out + eps_in) / (eps_out - eps_in)); conductor rows keep their
+// potential coefficients.  Panels in device order.
+__global__ __launch_bounds__(256) void k_near_dielectric_rows(
+    const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
+    const int *__restrict__ col_off, const int *__restrict__ cols,
+    const unsigned char *__restrict__ iface, const double *__restrict__ nrm,
+    const double *__restrict__ diag, const double *__restrict__ verts,
+    const double *__restrict__ cen, double *__restrict__ v64) {
+  const int T = blockIdx.x;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const long cnt = (long)nt * S;
+  const int64_t base = nf_off[T];
+  for (long e = threadIdx.x; e < cnt; e += blockDim.x) {
+    const int t = (int)(e / S), s = (int)(e % S);
+    const int i = t0 + t;
+    if (!iface[i]) continue;
+    const int j = cols[c0 + s];
+    v64[base + e] = j == i ? diag[i]
+                           : -normal_field(verts + 12L * j, cen[3 * i], cen[3 * i + 1], cen[3 * i + 2],
+                                           nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+  }
+}
+
+// Jacobi preconditioner of the interface rows: 1 / (their jump term).
+__global__ void k_dielectric_inv_diag(int n, const unsigned char *__restrict__ iface,
+                                      const double *__restrict__ diag, double *__restrict__ inv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && iface[i]) inv[i] = 1.0 / diag[i];
+}
+
+void launch_near_dielectric_rows(int n_leaves, int n, const int *leaf_lo, const int64_t *nf_off,
+                                 const int *col_off, const int *cols, const unsigned char *iface,
+                                 const double *nrm, const double *diag, const double *verts,
+                                 const double *cen, double *v64, double *inv_diag, hipStream_t s) {
+  if (n_leaves == 0) return;
+  ProfScope ps("near_dielectric_rows", s, 0.0, 0.0);
+  hipLaunchKernelGGL(k_near_dielectric_rows, dim3(n_leaves), dim3(256), 0, s, leaf_lo, nf_off,
+                     col_off, cols, iface, nrm, diag, verts, cen, v64);
+  GP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dielectric_inv_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, iface, diag,
+                     inv_diag);
+  GP_LAUNCH_CHECK();
+}
+
 // Self term P'_ii for the Jacobi preconditioner (cap/fmm_solver.rs:150-162).
 // inv = 1/d unless |k_e L0 d| <= 1e-30, the reference's threshold in SI units.
 __global__ void k_self_inv_diag(int n, const double *__restrict__ verts,

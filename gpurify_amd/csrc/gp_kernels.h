@@ -19,6 +19,12 @@ void launch_dense_assemble(int n, const double *verts, const double *cen, double
 void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_off,
                           const int *col_off, const int *cols, const double *verts,
                           const double *cen, double *v64, float *v32, hipStream_t s);
+// Dielectric-interface rows of the near tiles (iface, nrm, diag per panel in
+// device order) and 1/diag as their Jacobi preconditioner in inv_diag.
+void launch_near_dielectric_rows(int n_leaves, int n, const int *leaf_lo, const int64_t *nf_off,
+                                 const int *col_off, const int *cols, const unsigned char *iface,
+                                 const double *nrm, const double *diag, const double *verts,
+                                 const double *cen, double *v64, double *inv_diag, hipStream_t s);
 void launch_self_inv_diag(int n, const double *verts, const double *cen, double si_scale,
                           double *inv, hipStream_t s);
 
@@ -172,6 +178,9 @@ struct FmmDev {
 // problem's unit frame; cen: scaled centroids; nodes: the n Chebyshev nodes.
 void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, const double *cen,
                        const double *nodes, double *w64, float *w32, hipStream_t s);
+// d/dx of the 1-D interpolation weights over the leaf half-width (3n per panel).
+void launch_interp_dw1d(int N, int n, const int *panel_leaf, const double *geo, const double *cen,
+                        const double *nodes, double *dw, hipStream_t s);
 // Far field y += P_far z.  Multipoles M: n_boxes x ldm x n3p; locals Lc:
 // n_boxes x n3 x ldm.  m2l selects the M2L kernel (GP_M2L_*): BF16 needs Mb
 // (BF16 multipoles, same layout), FP64 uses matrix cores when ldm allows,
@@ -180,6 +189,11 @@ void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, c
 // projected multipoles Mt (n_boxes x ldm x kr, BF16 copy Mtb for the BF16
 // M2L, with Mtbl its rounding residual for the split-BF16 products of a
 // block of RHS) and locals Lt (n_boxes x kr x ldm); Mb is then unused.
+// y += yfar on conductor panels, y -= n . grad(local expansion) on
+// interface panels (the dielectric formulation's far field; FP64).
+void launch_far_dielectric(const FmmDev<double> &f, int nr, int ldm, int ldy, const double *dw1d,
+                           const double *nrm, const unsigned char *iface, const double *Lc,
+                           const double *yfar, double *y, hipStream_t s);
 void run_far_field(const FmmDev<double> &f, const double *z, int ldz, int nr, double *M,
                    double *Lc, int ldm, int m2l, double *y, int ldy, hipStream_t s);
 void run_far_field(const FmmDev<float> &f, const float *z, int ldz, int nr, float *M, void *Mb,

@@ -324,6 +324,14 @@ void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std:
 }
 
 // ------------------------------------------------------------------ FMM
+// Dielectric-interface rows of an FMM solve (caller order): the flag, the
+// oriented normal and the unit-frame jump term 2 pi (eps_out + eps_in) /
+// (eps_out - eps_in) of every interface panel.
+struct DielRows {
+  std::vector<unsigned char> iface;
+  std::vector<double> nrm, diag;
+};
+
 // Low-rank M2L tables per (order, rank), computed once per process: they
 // depend on the Chebyshev order only (reference box, unit half-width).
 struct LowRank {
@@ -400,6 +408,10 @@ struct FmmSystem {
   DBuf<int64_t> tr_base;
   DBuf<double> dinv_area, inv_area, ytmp;
   NearTranspose ntr;
+  // The dielectric formulation (one problem, FP64): set diel before setup.
+  const DielRows *diel = nullptr;
+  DBuf<unsigned char> d_iface;
+  DBuf<double> d_nrm, d_ddiag, dw1d64, yfar;
   // A batch: problem of each global box, skip flags, per-problem work
   DBuf<int> box_prob;
   DBuf<unsigned char> skip_box, skip_leaf;
@@ -640,6 +652,26 @@ struct FmmSystem {
       const int o0 = panel_off[p];
       launch_self_inv_diag(probs[p]->n, verts.p + (size_t)o0 * 12, cen.p + (size_t)o0 * 3,
                            k_e * L0s[p], inv_diag.p + o0, s);
+    }
+    if (diel) {
+      // interface rows of the near tiles and their Jacobi entries, in the
+      // device (leaf-sorted) order
+      if (P != 1 || need32)
+        throw Error(GP_ERR_INVALID_INPUT, "the FMM dielectric solve is one problem in FP64");
+      const auto &perm = trees[0].perm;
+      std::vector<unsigned char> hi(N);
+      std::vector<double> hn((size_t)N * 3), hd(N);
+      for (int k = 0; k < N; ++k) {
+        const int c = (int)perm[k];
+        hi[k] = diel->iface[c];
+        hd[k] = diel->diag[c];
+        for (int d = 0; d < 3; ++d) hn[(size_t)k * 3 + d] = diel->nrm[(size_t)c * 3 + d];
+      }
+      d_iface = to_device(hi, s);
+      d_nrm = to_device(hn, s);
+      d_ddiag = to_device(hd, s);
+      launch_near_dielectric_rows(n_leaves, N, leaf_lo.p, nf_off.p, col_off.p, cols.p, d_iface.p,
+                                  d_nrm.p, d_ddiag.p, verts.p, cen.p, nf64.p, inv_diag.p, s);
     }
     if (want_transpose) {
       // far-field transpose: P'_far^T x = area * K (x / area), K symmetric
@@ -914,6 +946,10 @@ struct FmmSystem {
       if (need32) w1d32.alloc((size_t)N * 3 * n);
       launch_interp_w1d(N, n, panel_leaf.p, dgeo.p, cen.p, dnodes.p, w1d64.p,
                         need32 ? w1d32.p : nullptr, s);
+      if (diel) {
+        dw1d64.alloc((size_t)N * 3 * n);
+        launch_interp_dw1d(N, n, panel_leaf.p, dgeo.p, cen.p, dnodes.p, dw1d64.p, s);
+      }
       GP_HIP(hipStreamSynchronize(s));  // dgeo and dnodes are freed below
     }
     m2m64 = to_device(hm2m, s);
@@ -1100,6 +1136,7 @@ struct FmmSystem {
     const size_t lexp = (size_t)f64.n_boxes * n * n * n * ld;
     z64.alloc(vec);
     z64.zero(s);
+    if (diel) yfar.alloc(vec);
     if (want_transpose) {
       ytmp.alloc(vec);
       ytmp.zero(s);
@@ -1153,7 +1190,15 @@ struct FmmSystem {
       if constexpr (sizeof(TV) != 8) throw Error(GP_ERR_INTERNAL, "FP64 operator on an FP32 basis");
       else kr_precond_cast(v, dinv, z64.p, N, ld, nr, s);
       launch_near_apply(nt, nf64.p, z64.p, ld, y, ld, nr, s);
-      run_far_field(f64, z64.p, ld, nr, M64.p, Lc64.p, ld, m2l_outer, y, ld, s);
+      if (diel) {
+        // potentials of all panels into yfar; conductor rows take them,
+        // interface rows the normal derivative of the same local expansions
+        yfar.zero(s);
+        run_far_field(f64, z64.p, ld, nr, M64.p, Lc64.p, ld, m2l_outer, yfar.p, ld, s);
+        launch_far_dielectric(f64, nr, ld, ld, dw1d64.p, d_nrm.p, d_iface.p, Lc64.p, yfar.p, y, s);
+      } else {
+        run_far_field(f64, z64.p, ld, nr, M64.p, Lc64.p, ld, m2l_outer, y, ld, s);
+      }
     } else {
       if (!have32) throw Error(GP_ERR_INTERNAL, "FP32 operator not assembled");
       kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
@@ -1633,7 +1678,7 @@ void solve_systems(FmmSystem &sys, const Ops &ops, const gp_cap_opts &o, bool mi
 }
 
 void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
-               gp_cap_report *rep, const double *eps_panel) {
+               gp_cap_report *rep, const double *eps_panel, const DielRows *diel = nullptr) {
   hipStream_t s = ctx->stream;
   double t0 = now_s();
   const bool mixed = o.precision == GP_PREC_MIXED;
@@ -1647,6 +1692,7 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   const int ld = blk == 1 ? 1 : round_up(blk, 16);
   FmmSystem sys;
   sys.want_by_offset = ld == 1;
+  sys.diel = diel;
   sys.setup(g, o, mixed, s);
   sys.ensure_work(ld);
   GP_HIP(hipStreamSynchronize(s));
@@ -1969,6 +2015,81 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
   }
 }
 
+
+void solve_dielectric_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles,
+                          const gp_cap_opts &opts, double *C, gp_cap_report *rep) {
+  const double t0 = now_s();
+  const int n = g.n, nc = g.ncond;
+  if (opts.precision != GP_PREC_FP64)
+    throw Error(GP_ERR_INVALID_INPUT, "the FMM dielectric solve runs in FP64 only");
+  if (!(opts.tol > 0.0) || !std::isfinite(opts.tol))
+    throw Error(GP_ERR_INVALID_INPUT, "tol must be > 0");
+  DielRows dr;
+  dr.iface.assign(n, 0);
+  dr.nrm.assign((size_t)n * 3, 0.0);
+  dr.diag.assign(n, 0.0);
+  // conductor ids for the solve: interface panels form one extra
+  // pseudo-conductor nc whose row of C is dropped; charge weights
+  // eps_surrounding * area on conductor panels only (dielectric.rs:141-149)
+  std::vector<uint32_t> cond(n);
+  std::vector<double> w(n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    const gp_panel_role &r = roles[k];
+    if (r.kind == GP_ROLE_CONDUCTOR) {
+      if (r.conductor >= (uint32_t)nc)
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " conductor out of range");
+      if (!std::isfinite(r.eps_surrounding))
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " has a non-finite eps");
+      cond[k] = r.conductor;
+      w[k] = r.eps_surrounding;
+    } else if (r.kind == GP_ROLE_DIELECTRIC) {
+      if (!(std::isfinite(r.eps_out) && std::isfinite(r.eps_in)) || r.eps_out == r.eps_in)
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) +
+                                              ": interface permittivities must be finite and differ");
+      // Panel::normal (panel.rs:50-55) toward the outer side (dielectric.rs:39-46)
+      const double *q = &g.verts[(size_t)k * 12];
+      const double ux = q[3] - q[0], uy = q[4] - q[1], uz = q[5] - q[2];
+      const double vx = q[6] - q[0], vy = q[7] - q[1], vz = q[8] - q[2];
+      double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+      const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+      if (!(len > 0.0)) throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " has no normal");
+      nx /= len, ny /= len, nz /= len;
+      if (nx * (g.cx[k] - r.reference[0]) + ny * (g.cy[k] - r.reference[1]) +
+              nz * (g.cz[k] - r.reference[2]) < 0.0)
+        nx = -nx, ny = -ny, nz = -nz;
+      dr.iface[k] = 1;
+      dr.nrm[(size_t)k * 3] = nx;
+      dr.nrm[(size_t)k * 3 + 1] = ny;
+      dr.nrm[(size_t)k * 3 + 2] = nz;
+      dr.diag[k] = 2.0 * kPi * (r.eps_out + r.eps_in) / (r.eps_out - r.eps_in);
+      cond[k] = (uint32_t)nc;
+    } else {
+      throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(k) + " has an unknown role");
+    }
+  }
+  PanelGeom ga = g;
+  ga.cond = cond.data();
+  ga.ncond = nc + 1;
+  gp_cap_opts o = opts;
+  o.method = GP_METHOD_FMM;
+  o.rhs_first = 0;
+  o.rhs_count = (uint32_t)nc;
+  if (rep) {
+    std::memset(rep, 0, sizeof(*rep));
+    rep->n_panels = (uint32_t)n;
+    rep->n_conductors = (uint32_t)nc;
+  }
+  std::vector<double> Cf((size_t)(nc + 1) * (nc + 1));
+  solve_fmm(ctx, ga, o, Cf.data(), rep, w.data(), &dr);
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j < nc; ++j) C[(size_t)i * nc + j] = Cf[(size_t)i * (nc + 1) + j];
+  for (int e = 0; e < nc * nc; ++e)
+    if (!std::isfinite(C[e])) throw Error(GP_ERR_NON_FINITE, "solver returned a non-finite capacitance");
+  if (rep) {
+    rep->n_conductors = (uint32_t)nc;
+    rep->seconds_total = now_s() - t0;
+  }
+}
 
 void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
                gp_cap_report *rep, const double *eps_panel) {

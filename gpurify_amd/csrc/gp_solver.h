@@ -38,6 +38,14 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
 void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, double *C,
                gp_cap_report *rep, const double *eps_panel);
 
+// cap::solve_dielectric's problem (cap/dielectric.rs:76-151) on the FMM
+// operator (no reference counterpart; FP64): near tiles with the interface
+// rows' normal-field integrals and jump terms, far field of the centroid
+// charges (potential for conductor rows, minus its normal derivative for
+// interface rows).  C: n_conductors^2 farads, permittivities applied.
+void solve_dielectric_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles,
+                          const gp_cap_opts &opts, double *C, gp_cap_report *rep);
+
 // Maxwell C of each independent problem of a batch, C[p] ncond_p x ncond_p.
 // Problems below the dense limit (method AUTO) or all (DENSE) take the dense
 // path one by one; the FMM problems are solved together (stacked operator,

@@ -332,6 +332,19 @@ typedef struct {
 int gp_cap_solve_dielectric(gp_ctx *ctx, const gp_panels *panels, const gp_panel_role *roles,
                             double *C, gp_cap_report *report);
 
+/* The problem of gp_cap_solve_dielectric on the FMM operator, for sizes the
+ * dense LU cannot reach (no counterpart in the reference, which solves it
+ * dense only; FP64).  The rows are those of cap::solve_dielectric
+ * (cap/dielectric.rs:76-151) over the near field of the FMM (each leaf and
+ * its neighbour leaves); the far field is the centroid-charge field of the
+ * conductor FMM: its potential for conductor rows and minus its normal
+ * derivative (the differentiated local expansion) for interface rows.
+ * Jacobi-preconditioned GMRES per conductor (opts: fmm_order, max_level or
+ * leaf_target, tol, restart, max_restarts, m2l_compress; precision must be
+ * GP_PREC_FP64).  C: n_conductors^2 farads, permittivities applied. */
+int gp_cap_solve_dielectric_fmm(gp_ctx *ctx, const gp_panels *panels, const gp_panel_role *roles,
+                                const gp_cap_opts *opts, double *C, gp_cap_report *report);
+
 /* ------------------------------------------------------------ FastCap input
  * FastCap "quickif" geometry text (.qui), as cap::geometry::parse
  * (crates/pex/src/quasistatic/cap/geometry.rs:56-135) reads it: Q (quad)

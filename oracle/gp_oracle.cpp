@@ -465,6 +465,32 @@ void interp_weights(double x, int n, const double *nodes, double *w) {
   }
 }
 
+// d/dx of the 1-D interpolation weights (no reference counterpart: the
+// gradient of the far-field interpolant, for the FMM form of the dielectric
+// interface rows): w_m'(x) = (2/n) sum_{k=1}^{n-1} T_k(x_m) T_k'(x), with
+// T_k' = k U_{k-1} (Chebyshev polynomials of the second kind).
+void interp_dweights(double x, int n, const double *nodes, double *dw) {
+  double u[64], dt[64];  // U_k(x), T_k'(x)
+  u[0] = 1.0;
+  if (n > 1) u[1] = 2.0 * x;
+  for (int k = 2; k < n; ++k) u[k] = 2.0 * x * u[k - 1] - u[k - 2];
+  dt[0] = 0.0;
+  for (int k = 1; k < n; ++k) dt[k] = k * u[k - 1];
+  for (int m = 0; m < n; ++m) {
+    const double xm = nodes[m];
+    double s = 0.0, tkm1 = 1.0, tk = xm;  // T_0, T_1 at the node
+    for (int k = 1; k < n; ++k) {
+      if (k >= 2) {
+        const double t = 2.0 * xm * tk - tkm1;
+        tkm1 = tk;
+        tk = t;
+      }
+      s += tk * dt[k];
+    }
+    dw[m] = 2.0 / n * s;
+  }
+}
+
 // chebyshev.rs:77-94
 void interp_weights_3d(const double p[3], int n, const double *nodes, double *w) {
   double w1[64], w2[64], w3[64];
@@ -806,7 +832,11 @@ struct Fmm {
   }
 
   // bbfmm.rs:209-448
-  std::vector<double> run(const double *q, bool with_p2p) const {
+  // gnrm/gout (optional): for the points with a nonzero gnrm[pi], gout[pi]
+  // = gnrm[pi] . grad of the far-field interpolant at the point (the local
+  // expansion differentiated; no reference counterpart).
+  std::vector<double> run(const double *q, bool with_p2p, const V3 *gnrm = nullptr,
+                          double *gout = nullptr) const {
     size_t nb = tree.nodes.size();
     std::vector<std::vector<double>> w(nb, std::vector<double>(n3, 0.0)),
         local(nb, std::vector<double>(n3, 0.0));
@@ -930,6 +960,27 @@ struct Fmm {
         const auto &p = pw[pi];
         double acc = 0.0;
         for (int m = 0; m < n3; ++m) acc += p[m] * ll[m];
+        if (gnrm && gout && (gnrm[pi].x != 0.0 || gnrm[pi].y != 0.0 || gnrm[pi].z != 0.0)) {
+          double loc[3] = {(points[pi].x - nd.center[0]) / nd.half,
+                           (points[pi].y - nd.center[1]) / nd.half,
+                           (points[pi].z - nd.center[2]) / nd.half};
+          double w[3][64], dw[3][64];
+          for (int d = 0; d < 3; ++d) {
+            interp_weights(loc[d], n, nodes1d.data(), w[d]);
+            interp_dweights(loc[d], n, nodes1d.data(), dw[d]);
+          }
+          double g[3] = {0.0, 0.0, 0.0};
+          for (int a = 0; a < n; ++a)
+            for (int b = 0; b < n; ++b)
+              for (int c = 0; c < n; ++c) {
+                const double l = ll[(a * n + b) * n + c];
+                g[0] += dw[0][a] * w[1][b] * w[2][c] * l;
+                g[1] += w[0][a] * dw[1][b] * w[2][c] * l;
+                g[2] += w[0][a] * w[1][b] * dw[2][c] * l;
+              }
+          const V3 nn = gnrm[pi];
+          gout[pi] = (nn.x * g[0] + nn.y * g[1] + nn.z * g[2]) / nd.half;
+        }
         if (with_p2p) {
           V3 xi = points[pi];
           auto add_box = [&](long c) {
@@ -1398,6 +1449,84 @@ PanelSet set_from(const std::vector<Pan> &pans, const std::vector<uint32_t> &con
   return s;
 }
 
+// The dielectric formulation of cap/dielectric.rs:76-151 on the FMM
+// operator (no reference counterpart: the reference solves it dense only).
+// Near field (the leaf and its neighbours, the near blocks of the conductor
+// operator): conductor rows k_e * potential_integral, interface rows
+// -k_e * field_integral (l != k) and the jump (eps_out + eps_in) /
+// (2 eps0 (eps_out - eps_in)) on the diagonal, as the dense rows.  Far
+// field: the potential of the panels' centroid charges area * sigma for
+// conductor rows (FmmCapOp), and -k_e n_k . grad of that potential, the
+// differentiated local expansion, for interface rows.
+struct FmmDielOp : Op {
+  Fmm fmm;
+  std::vector<double> areas;
+  std::vector<V3> gnrm;  // interface rows: oriented normal; conductor rows: 0
+  std::vector<long> off;
+  std::vector<uint32_t> idx;
+  std::vector<double> val;
+  long nn = 0;
+  mutable std::vector<double> qbuf, gbuf;
+
+  void init(const PanelSet &s, const int *role, const V3 *nrm, const double *diag, int order,
+            int max_level, int m2l_mode) {
+    nn = s.n();
+    areas = s.area;
+    gnrm.assign(nn, V3{0.0, 0.0, 0.0});
+    for (long i = 0; i < nn; ++i)
+      if (role[i] == 1) gnrm[i] = nrm[i];
+    fmm.init(s.c, order, max_level, m2l_mode);
+    auto blocks = fmm.near_blocks();
+    std::vector<std::vector<uint32_t>> rows_i(nn);
+    std::vector<std::vector<double>> rows_v(nn);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long b = 0; b < (long)blocks.size(); ++b) {
+      const auto &tg = blocks[b].first;
+      const auto &sr = blocks[b].second;
+      for (long i : tg) {
+        rows_i[i].resize(sr.size());
+        rows_v[i].resize(sr.size());
+        for (size_t k = 0; k < sr.size(); ++k) {
+          long j = sr[k];
+          rows_i[i][k] = (uint32_t)j;
+          if (role[i] == 0)
+            rows_v[i][k] = ONE_OVER_4PI_EPS0 * potential_integral(s.pans[j].v, 4, s.c[i]);
+          else
+            rows_v[i][k] = j == i ? diag[i]
+                                  : -ONE_OVER_4PI_EPS0 * field_integral(s.pans[j].v, 4, s.c[i], nrm[i]);
+        }
+      }
+    }
+    off.assign(nn + 1, 0);
+    for (long i = 0; i < nn; ++i) off[i + 1] = off[i] + (long)rows_i[i].size();
+    idx.resize(off[nn]);
+    val.resize(off[nn]);
+    for (long i = 0; i < nn; ++i) {
+      std::copy(rows_i[i].begin(), rows_i[i].end(), idx.begin() + off[i]);
+      std::copy(rows_v[i].begin(), rows_v[i].end(), val.begin() + off[i]);
+    }
+    qbuf.assign(nn, 0.0);
+    gbuf.assign(nn, 0.0);
+  }
+  long dim() const override { return nn; }
+  void apply(const double *sigma, double *y) const override {
+    for (long j = 0; j < nn; ++j) qbuf[j] = areas[j] * sigma[j];
+    std::vector<double> far = fmm.run(qbuf.data(), false, gnrm.data(), gbuf.data());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nn; ++i) {
+      const bool iface = gnrm[i].x != 0.0 || gnrm[i].y != 0.0 || gnrm[i].z != 0.0;
+      double acc = iface ? -ONE_OVER_4PI_EPS0 * gbuf[i] : ONE_OVER_4PI_EPS0 * far[i];
+      for (long k = off[i]; k < off[i + 1]; ++k) acc += val[k] * sigma[idx[k]];
+      y[i] = acc;
+    }
+  }
+  double diagonal(long i) const {
+    for (long k = off[i]; k < off[i + 1]; ++k)
+      if ((long)idx[k] == i) return val[k];
+    return 1.0;
+  }
+};
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -1443,6 +1572,46 @@ int solve_dielectric(int n, const double *verts4, const int *role, const uint32_
     lu.solve(b.data(), sig.data());
     for (int k = 0; k < n; ++k)
       if (role[k] == 0) C[(size_t)cond[k] * ncond + j] += eps[k] * sig[k] * s.area[k];
+  }
+  return 0;
+}
+
+// solve_dielectric's problem on FmmDielOp: Jacobi-preconditioned GMRES
+// (restart 200, 50 restarts) per conductor, charges weighted as the dense
+// solve.  Returns 0, or -6 for no panels.
+int solve_dielectric_fmm(int n, const double *verts4, const int *role, const uint32_t *cond,
+                         const double *eps, const double *eps_out, const double *eps_in,
+                         const double *ref, int ncond, int order, int max_level, double tol,
+                         int m2l_mode, double *C) {
+  if (n == 0) return -6;
+  PanelSet s = build_set(n, verts4, cond);
+  std::vector<V3> nrm(n);
+  std::vector<double> diag(n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    const V3 *v = s.pans[k].v;
+    V3 nk = normalized(cross(v[1] - v[0], v[2] - v[0]));
+    if (role[k] == 1) {
+      V3 r{ref[3 * k], ref[3 * k + 1], ref[3 * k + 2]};
+      if (dot(nk, s.c[k] - r) < 0.0) nk = nk * -1.0;
+      diag[k] = (eps_out[k] + eps_in[k]) / (2.0 * EPS0 * (eps_out[k] - eps_in[k]));
+    }
+    nrm[k] = nk;
+  }
+  FmmDielOp op;
+  op.init(s, role, nrm.data(), diag.data(), order, max_level, m2l_mode);
+  DiagJacobi pre;
+  pre.inv.assign(n, 1.0);
+  for (long i = 0; i < n; ++i) {
+    const double d = op.diagonal(i);
+    pre.inv[i] = std::fabs(d) > 1e-30 ? 1.0 / d : 1.0;
+  }
+  std::fill(C, C + (size_t)ncond * ncond, 0.0);
+  std::vector<double> b(n);
+  for (int j = 0; j < ncond; ++j) {
+    for (int k = 0; k < n; ++k) b[k] = (role[k] == 0 && (int)cond[k] == j) ? 1.0 : 0.0;
+    auto r = gmres(op, b.data(), &pre, tol, 200, 50);
+    for (int k = 0; k < n; ++k)
+      if (role[k] == 0) C[(size_t)cond[k] * ncond + j] += eps[k] * r.x[k] * s.area[k];
   }
   return 0;
 }
@@ -1768,6 +1937,19 @@ int or_solve_dielectric(int n, const double *verts4, const int *role, const uint
                         const double *eps, const double *eps_out, const double *eps_in,
                         const double *ref, int ncond, double *C) {
   return solve_dielectric(n, verts4, role, cond, eps, eps_out, eps_in, ref, ncond, C);
+}
+
+int or_solve_dielectric_fmm(int n, const double *verts4, const int *role, const uint32_t *cond,
+                            const double *eps, const double *eps_out, const double *eps_in,
+                            const double *ref, int ncond, int order, int max_level, double tol,
+                            int m2l_mode, double *C) {
+  return solve_dielectric_fmm(n, verts4, role, cond, eps, eps_out, eps_in, ref, ncond, order,
+                              max_level, tol, m2l_mode, C);
+}
+
+void or_interp_dweights(double x, int n, double *dw) {
+  std::vector<double> nd = cheb_nodes(n);
+  interp_dweights(x, n, nd.data(), dw);
 }
 
 }  // extern "C"

@@ -139,6 +139,20 @@ int or_solve_capacitance(int n_groups, const uint32_t *net, const double *height
 void or_set_threads(int n);
 int or_get_threads(void);
 
+/* No reference counterpart: the dielectric formulation of
+ * cap/dielectric.rs:76-151 on the FMM operator (near rows as the dense
+ * rows over the leaf and its neighbours; far field: the centroid-charge
+ * potential for conductor rows, -k_e n . its gradient for interface rows),
+ * Jacobi-preconditioned GMRES per conductor.  Arguments as
+ * or_solve_dielectric plus the FMM order, depth, tolerance and m2l_mode. */
+int or_solve_dielectric_fmm(int n, const double *verts4, const int *role, const uint32_t *cond,
+                            const double *eps, const double *eps_out, const double *eps_in,
+                            const double *ref, int ncond, int order, int max_level, double tol,
+                            int m2l_mode, double *C);
+
+/* d/dx of the 1-D Chebyshev interpolation weights at x (n values). */
+void or_interp_dweights(double x, int n, double *dw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,9 @@ def lib():
         L.or_field_integral.restype = C.c_double
         L.or_field_integral.argtypes = [_d, C.c_int, _d, _d]
         L.or_tri7.argtypes = [_d]
+        L.or_solve_dielectric_fmm.argtypes = [C.c_int, _d, _int, _u32, _d, _d, _d, _d, C.c_int,
+                                              C.c_int, C.c_int, C.c_double, C.c_int, _d]
+        L.or_interp_dweights.argtypes = [C.c_double, C.c_int, _d]
         L.or_lowrank_compress.restype = C.c_int
         L.or_lowrank_compress.argtypes = [_d, C.c_int, C.c_int, C.c_double, _d, _d]
         L.or_m2l_eps.restype = C.c_double
@@ -180,6 +183,32 @@ def solve_dielectric(verts4, role, cond, eps, eps_out, eps_in, ref, ncond):
     if rc:
         raise RuntimeError(f"oracle dielectric solve failed ({rc})")
     return Cm
+
+
+def solve_dielectric_fmm(verts4, role, cond, eps, eps_out, eps_in, ref, ncond, order=4,
+                         max_level=4, tol=1e-10, m2l_mode=0):
+    """The dielectric problem of solve_dielectric on the FMM operator (no
+    reference counterpart): C in farads."""
+    v = _f64(verts4).reshape(-1, 4, 3)
+    n = v.shape[0]
+    r = np.ascontiguousarray(role, dtype=np.int32)
+    c = np.ascontiguousarray(cond, dtype=np.uint32)
+    e, eo, ei = _f64(eps), _f64(eps_out), _f64(eps_in)
+    rf = _f64(ref).reshape(-1, 3)
+    Cm = np.zeros((ncond, ncond))
+    rc = lib().or_solve_dielectric_fmm(n, _p(v), r.ctypes.data_as(_int), _p(c, _u32), _p(e),
+                                       _p(eo), _p(ei), _p(rf), ncond, order, max_level, tol,
+                                       m2l_mode, _p(Cm))
+    if rc:
+        raise RuntimeError(f"oracle FMM dielectric solve failed ({rc})")
+    return Cm
+
+
+def interp_dweights(x, n):
+    """d/dx of the 1-D Chebyshev interpolation weights at x."""
+    w = np.zeros(n)
+    lib().or_interp_dweights(float(x), int(n), _p(w))
+    return w
 
 
 def cap_fmm(verts4, cond, ncond, order=4, max_level=4, tol=1e-8, m2l_mode=0):

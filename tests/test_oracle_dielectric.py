@@ -51,3 +51,31 @@ def test_interface_orientation_follows_the_reference_point():
     C = O.solve_dielectric(*coated_sphere(2), 1)
     Cf = O.solve_dielectric(*coated_sphere(2, flip=True), 1)
     assert abs(C[0, 0] - Cf[0, 0]) <= 1e-12 * C[0, 0]
+
+
+def test_interpolation_derivative_weights_are_exact_below_the_order():
+    # d/dx of sum_m w_m(x) p(x_m) is p'(x) for every polynomial of degree < n
+    rng = np.random.default_rng(5)
+    for n in (3, 4, 6, 8):
+        xm = O.cheb_nodes(n)
+        c = rng.standard_normal(n)
+        p = np.polynomial.Polynomial(c)
+        for x in (-1.0, -0.37, 0.0, 0.81, 1.0):
+            dw = O.interp_dweights(x, n)
+            assert abs(dw @ p(xm) - p.deriv()(x)) <= 1e-11 * np.abs(c).sum() * n * n
+            # and the derivative of the interpolation weights themselves
+            h = 1e-6
+            fd = (O.interp_weights(x + h, n) - O.interp_weights(x - h, n)) / (2 * h)
+            assert np.abs(dw - fd).max() < 1e-6 * max(1.0, np.abs(dw).max())
+
+
+def test_fmm_form_follows_the_dense_solve():
+    # the interface rows on the FMM operator: near field as the dense rows,
+    # far field the normal derivative of the local expansion
+    p = coated_sphere(2)
+    Cd = O.solve_dielectric(*p, 1)[0, 0]
+    Cf = O.solve_dielectric_fmm(*p, 1, order=4, max_level=2, tol=1e-12)[0, 0]
+    assert abs(Cf - Cd) <= 5e-3 * Cd  # 3.2e-3: centroid lumping of the far field
+    assert abs(Cf - exact()) <= 0.05 * exact()
+    Cff = O.solve_dielectric_fmm(*coated_sphere(2, flip=True), 1, order=4, max_level=2, tol=1e-12)
+    assert abs(Cff[0, 0] - Cf) <= 1e-10 * Cf
