@@ -67,6 +67,8 @@ def parse():
                     help="inner M2L: BF16 matrix cores or the FP32 VALU kernel")
     ap.add_argument("--plain-rhs", type=int, default=2, help="RHS sampled for the 1-RHS FP64 rate")
     ap.add_argument("--fp64-rhs", type=int, default=0, help="RHS of the FP64 comparison (0 = all)")
+    ap.add_argument("--batch-groups", type=int, default=0,
+                    help="groups of a batched solve, set-up overlapping the solve (0 = automatic)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
@@ -276,7 +278,7 @@ def bench_batch(args):
     log(f"workload nets4096: {len(probs)} nets, {n} panels (min {sizes.min()}, max {sizes.max()}), "
         f"{int((sizes < 500).sum())} below the dense limit")
     common = dict(fmm_order=args.order, max_level=0, leaf_target=args.leaf_target, tol=args.tol,
-                  rhs_first=0, rhs_count=1)
+                  rhs_first=0, rhs_count=1, batch_groups=args.batch_groups)
     o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol, **common)
     if args.m2l_rank >= 0:
         o_mix.m2l_rank = args.m2l_rank
@@ -298,16 +300,19 @@ def bench_batch(args):
                    value_incl_upload=n * args.steps / T)
         it = np.array([r["iterations"] for r in fm])
         nfmm = int(sum(r["n_panels"] for r in fm))
+        nnz = int(sum(r["near_nnz"] for r in fm))  # each problem's own share
+        pairs = int(sum(r["m2l_pairs"] for r in fm))
         out["config"] = {"workload": "nets4096_met1met2", "nets": len(probs), "panels": n,
                          "rhs_per_net": 1, "fmm_order": args.order, "m2l": "cpqr",
                          "leaf_target": args.leaf_target, "tol": args.tol,
                          "dense_nets": len(probs) - len(fm), "fmm_nets": len(fm),
                          "fmm_panels": nfmm, "max_depth": max(r["max_level"] for r in fm),
-                         "near_nnz": fm[0]["near_nnz"], "m2l_pairs": fm[0]["m2l_pairs"],
+                         "near_nnz": nnz, "m2l_pairs": pairs,
+                         "batch_groups": o_mix.batch_groups or (1 if nfmm < 2_000_000 else 3),
                          "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),
                          "refine_steps_max": max(r["refine_steps"] for r in fm),
                          "converged": all(r["converged"] for r in reps), "global_batch": len(probs)}
-        work = {"N": nfmm, "R": 1, "nnz": fm[0]["near_nnz"], "pairs": fm[0]["m2l_pairs"],
+        work = {"N": nfmm, "R": 1, "nnz": nnz, "pairs": pairs,
                 "n3": args.order ** 3}
         report_profile(out, prof, work, "nets4096")
         out["autotune"] = autotune_choices(ctx)

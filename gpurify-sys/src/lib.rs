@@ -89,6 +89,7 @@ pub struct gp_cap_opts {
     pub m2l_compress: u32,
     pub fuse_leaf: u32,
     pub skip_converged: u32,
+    pub batch_groups: u32,
 }
 
 #[repr(C)]

@@ -62,7 +62,7 @@ class GpCapOpts(C.Structure):
                 ("inner_passes", C.c_uint32), ("inner_basis", C.c_uint32),
                 ("m2l_rank", C.c_uint32), ("step_graphs", C.c_uint32),
                 ("m2l_compress", C.c_uint32), ("fuse_leaf", C.c_uint32),
-                ("skip_converged", C.c_uint32)]
+                ("skip_converged", C.c_uint32), ("batch_groups", C.c_uint32)]
 
 
 class GpParamMap(C.Structure):

@@ -30,6 +30,11 @@ size_t &device_bytes_in_use() {
   return used;
 }
 
+std::recursive_mutex &device_mem_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 double &upload_seconds() {
   static thread_local double t = 0.0;
   return t;
@@ -134,7 +139,9 @@ int gp_ctx_create(gp_ctx **out, uint32_t device_ordinal) {
     c->device_name = name;
     c->cu_count = cus;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->setup_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
+      if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
       gp::hip_check(e, "hipStreamCreateWithFlags");
     }
@@ -150,9 +157,8 @@ void gp_ctx_destroy(gp_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->blas.reset();
-    if (ctx->stream) {
-      (void)hipStreamDestroy(ctx->stream);
-    }
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->setup_stream) (void)hipStreamDestroy(ctx->setup_stream);
   }
   delete ctx;
 }
@@ -207,6 +213,7 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->m2l_compress = 1;  // bbfmm.rs:132-174
   o->fuse_leaf = 1;
   o->skip_converged = 1;
+  o->batch_groups = 0;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,

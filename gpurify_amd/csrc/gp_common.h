@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdlib>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -35,6 +36,11 @@ inline void hip_check(hipError_t e, const char *what) {
 // path from the host) is enforced in one place.
 size_t &device_bytes_in_use();
 size_t device_bytes_cap();
+// Held by every DBuf allocation, free and upload, and by a thread while it
+// captures a HIP graph: a batched solve sets up its next group of problems
+// on a second thread, whose device allocations and copies must not fall
+// inside the solving thread's capture.
+std::recursive_mutex &device_mem_mutex();
 
 template <typename T>
 struct DBuf {
@@ -60,6 +66,7 @@ struct DBuf {
   }
   ~DBuf() { release(); }
   void alloc(size_t count) {
+    std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
     release();
     if (count == 0) return;
     size_t bytes = count * sizeof(T);
@@ -73,6 +80,7 @@ struct DBuf {
   }
   void release() {
     if (p) {
+      std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
       (void)hipFree(p);
       device_bytes_in_use() -= n * sizeof(T);
     }
@@ -84,6 +92,7 @@ struct DBuf {
   // asynchronous copy from pageable memory may read the host buffer after
   // the call returns.
   void upload(const T *h, size_t count, hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
     if (count != n) alloc(count);
     if (n) {
       GP_HIP(hipMemcpyAsync(p, h, bytes(), hipMemcpyHostToDevice, s));

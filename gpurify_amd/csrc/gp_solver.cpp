@@ -25,6 +25,7 @@
 #include <atomic>
 #include <climits>
 #include <exception>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -1512,6 +1513,8 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
       if (graphs) {
         if ((int)W.graphs.size() <= k0) W.graphs.resize(k0 + 1, nullptr);
         if (!W.graphs[k0]) {
+          // (no device allocation or copy of another thread inside the capture)
+          std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
           hipGraph_t gr = nullptr;
           GP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
           try {
@@ -1773,15 +1776,78 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
 // GMRES per problem running in lockstep with per-problem reductions.  Each
 // right-hand-side column j of [rhs_first, rhs_first + rhs_count) is one pass
 // over the batch (problems with fewer conductors sit it out).
-void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
-                     const gp_cap_opts &o, const std::vector<double *> &C, gp_cap_report *reps,
-                     double t_start) {
-  hipStream_t s = ctx->stream;
-  const bool mixed = o.precision == GP_PREC_MIXED;
+// Splits a batch's FMM problems into k groups, group j holding about
+// 2^j / (2^k - 1) of the panels (largest problems first, each into the group
+// furthest below its share); problems keep their order within a group.
+std::vector<std::vector<int>> batch_groups(const std::vector<const PanelGeom *> &probs, int k) {
   const int P = (int)probs.size();
-  FmmSystem sys;
-  sys.want_by_offset = true;
-  sys.setup(probs, o, mixed, s);
+  k = std::max(1, std::min({k, P, 16}));
+  double total = 0.0;
+  for (auto *g : probs) total += g->n;
+  std::vector<double> share(k), fill(k, 0.0);
+  for (int j = 0; j < k; ++j) share[j] = total * (double)(1 << j) / (double)((1 << k) - 1);
+  std::vector<int> by_size(P);
+  std::iota(by_size.begin(), by_size.end(), 0);
+  std::stable_sort(by_size.begin(), by_size.end(),
+                   [&](int a, int b) { return probs[a]->n > probs[b]->n; });
+  std::vector<std::vector<int>> grp(k);
+  for (int p : by_size) {
+    int best = 0;
+    for (int j = 1; j < k; ++j)
+      if (share[j] - fill[j] > share[best] - fill[best]) best = j;
+    grp[best].push_back(p);
+    fill[best] += probs[p]->n;
+  }
+  std::vector<std::vector<int>> out;
+  for (auto &g : grp)
+    if (!g.empty()) {
+      std::sort(g.begin(), g.end());
+      out.push_back(std::move(g));
+    }
+  return out;
+}
+
+// One group of a batch, set up: its stacked operator and the set-up's
+// timing, profile records and first error.
+struct StagedGroup {
+  std::vector<int> idx;  // positions in the batch
+  std::vector<const PanelGeom *> probs;
+  std::unique_ptr<FmmSystem> sys;
+  double t0 = 0.0, t1 = 0.0, upload = 0.0;
+  Prof prof;
+  std::exception_ptr err;
+};
+
+// FmmSystem::setup of one group on ctx->setup_stream, from a thread of its
+// own (device current, profiling into the group's own records).
+void stage_group(gp_ctx *ctx, const gp_cap_opts &o, bool mixed, bool profile, StagedGroup &st) {
+  st.t0 = now_s();
+  try {
+    GP_HIP(hipSetDevice(ctx->device));
+    st.prof.on = profile;
+    active_prof() = &st.prof;
+    upload_seconds() = 0.0;
+    st.sys.reset(new FmmSystem());
+    st.sys->want_by_offset = true;
+    st.sys->setup(st.probs, o, mixed, ctx->setup_stream);
+    GP_HIP(hipStreamSynchronize(ctx->setup_stream));
+  } catch (...) {
+    st.err = std::current_exception();
+  }
+  active_prof() = nullptr;
+  st.upload = upload_seconds();
+  st.t1 = now_s();
+}
+
+// The GMRES solves of one set-up group, on ctx->stream: every conductor
+// column of every problem, C and the reports by group position.
+void solve_group(gp_ctx *ctx, FmmSystem &sys, const std::vector<const PanelGeom *> &probs,
+                 const int *batch_idx, const gp_cap_opts &o, bool mixed, double t_setup,
+                 double *const *C, gp_cap_report *reps) {
+  hipStream_t s = ctx->stream;
+  sys.s = s;
+  const int P = (int)probs.size();
+  const double t_start = now_s();
   sys.ensure_work(1);
   // Row segments and the reduction block map (at most 2048 rows a block).
   std::vector<int> hseg(sys.N), blo, bhi, sblk{0};
@@ -1806,10 +1872,10 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
   ops.sm.sys_blk = dsblk.p;
   ops.s = s;
   GP_HIP(hipStreamSynchronize(s));
-  const double t1 = now_s();
-  GP_LOG("batch setup %.3f s: %d problems, %d panels, max depth %d, %d boxes, near nnz %lld, "
-         "%lld M2L pairs", t1 - t_start, P, sys.N, sys.L, sys.f64.n_boxes, (long long)sys.nnz,
+  GP_LOG("group set-up %.3f s: %d problems, %d panels, max depth %d, %d boxes, near nnz %lld, "
+         "%lld M2L pairs", t_setup, P, sys.N, sys.L, sys.f64.n_boxes, (long long)sys.nnz,
          (long long)sys.pairs);
+  const double t1 = now_s();
 
   int maxc = 0;
   for (auto *g : probs) maxc = std::max(maxc, g->ncond);
@@ -1861,18 +1927,97 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
       r.n_leaves = (uint32_t)sys.trees[p].n_leaves();
       r.n_boxes = (uint32_t)sys.trees[p].n_boxes();
       r.fmm_order = (uint32_t)sys.n;
-      r.near_nnz = (uint64_t)sys.nnz;
-      r.m2l_pairs = (uint64_t)sys.pairs;
+      // the problem's own share of the stacked operator
+      r.near_nnz = P > 1 ? (uint64_t)sys.p_nnz[p] : (uint64_t)sys.nnz;
+      r.m2l_pairs = P > 1 ? (uint64_t)sys.p_pairs[p] : (uint64_t)sys.pairs;
       r.final_rel_residual = resid[p];
-      r.seconds_setup = t1 - t_start;
+      r.seconds_setup = t_setup + (t1 - t_start);
       r.seconds_solve = t2 - t1;
     }
   if (o.strict)
     for (int p = 0; p < P; ++p)
       if (!conv[p])
         throw Error(GP_ERR_NOT_CONVERGED, "GMRES did not reach tol for problem " +
-                                              std::to_string(p) + " (relative residual " +
+                                              std::to_string(batch_idx[p]) + " (relative residual " +
                                               std::to_string(resid[p]) + ")");
+}
+
+// The FMM problems of a batched call, in groups (gp_cap_opts::batch_groups):
+// a second host thread sets up group k + 1 on ctx->setup_stream while this
+// thread solves group k on ctx->stream, and `before` (the dense problems'
+// solve) runs while group 0 is set up.  Only group 0's uploads count in
+// upload_seconds(): the later groups' copies overlap a solve.
+void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
+                     const gp_cap_opts &o, const std::vector<double *> &C, gp_cap_report *reps,
+                     const std::function<void()> &before) {
+  const bool mixed = o.precision == GP_PREC_MIXED;
+  double total = 0.0;
+  for (auto *g : probs) total += g->n;
+  const int k = o.batch_groups ? (int)o.batch_groups : (total < 2.0e6 ? 1 : 3);
+  const auto groups = batch_groups(probs, k);
+  const bool profile = active_prof() && active_prof()->on;
+  auto make = [&](int j) {
+    auto st = std::make_unique<StagedGroup>();
+    st->idx = groups[j];
+    for (int p : st->idx) st->probs.push_back(probs[p]);
+    return st;
+  };
+  // the set-up thread's profile records move to this thread's profile
+  auto absorb = [&](StagedGroup &st) {
+    Prof *mine = active_prof();
+    if (mine) {
+      mine->recs.insert(mine->recs.end(), st.prof.recs.begin(), st.prof.recs.end());
+      mine->pool.insert(mine->pool.end(), st.prof.pool.begin(), st.prof.pool.end());
+      st.prof.recs.clear();
+      st.prof.pool.clear();
+    }
+  };
+  std::unique_ptr<StagedGroup> cur = make(0), next;
+  {
+    StagedGroup *c = cur.get();
+    std::thread th([&, c] { stage_group(ctx, o, mixed, profile, *c); });
+    try {
+      if (before) before();
+    } catch (...) {
+      th.join();
+      throw;
+    }
+    th.join();
+  }
+  absorb(*cur);
+  if (cur->err) std::rethrow_exception(cur->err);
+  upload_seconds() += cur->upload;
+  for (size_t j = 0; j < groups.size(); ++j) {
+    std::thread th;
+    if (j + 1 < groups.size()) {
+      next = make((int)j + 1);
+      StagedGroup *nx = next.get();
+      th = std::thread([&, nx] { stage_group(ctx, o, mixed, profile, *nx); });
+    }
+    try {
+      const int G = (int)cur->idx.size();
+      std::vector<double *> Cg(G);
+      std::vector<gp_cap_report> rg(G);
+      for (int q = 0; q < G; ++q) {
+        Cg[q] = C[cur->idx[q]];
+        if (reps) rg[q] = reps[cur->idx[q]];
+      }
+      solve_group(ctx, *cur->sys, cur->probs, cur->idx.data(), o, mixed, cur->t1 - cur->t0,
+                  Cg.data(), reps ? rg.data() : nullptr);
+      if (reps)
+        for (int q = 0; q < G; ++q) reps[cur->idx[q]] = rg[q];
+      cur->sys.reset();  // this group's device memory
+    } catch (...) {
+      if (th.joinable()) th.join();
+      throw;
+    }
+    if (th.joinable()) {
+      th.join();
+      absorb(*next);
+      if (next->err) std::rethrow_exception(next->err);
+      cur = std::move(next);
+    }
+  }
 }
 
 }  // namespace
@@ -2387,11 +2532,16 @@ void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_c
       fidx.push_back(p);
     }
   }
-  if (!dprobs.empty()) solve_dense_batch(ctx, gs, dprobs, C, reps);
-  if (!fmm.empty()) {
+  auto dense = [&] {
+    if (!dprobs.empty()) solve_dense_batch(ctx, gs, dprobs, C, reps);
+  };
+  if (fmm.empty()) {
+    dense();
+  } else {
+    // (the dense problems solve while the first group of FMM problems is set up)
     std::vector<gp_cap_report> fr(fmm.size());
     for (size_t q = 0; q < fmm.size(); ++q) fr[q] = reps ? reps[fidx[q]] : gp_cap_report{};
-    solve_fmm_batch(ctx, fmm, o, Cf, fr.data(), now_s());
+    solve_fmm_batch(ctx, fmm, o, Cf, fr.data(), dense);
     if (reps)
       for (size_t q = 0; q < fmm.size(); ++q) reps[fidx[q]] = fr[q];
   }

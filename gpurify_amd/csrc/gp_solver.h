@@ -15,6 +15,9 @@
 struct gp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // uploads and set-up kernels of a batch's next group of problems, issued
+  // by a second host thread while `stream` solves the current group
+  hipStream_t setup_stream = nullptr;
   std::mutex mu;
   std::string last_error;
   std::string device_name;

@@ -137,6 +137,15 @@ typedef struct {
                             step skip the problems whose system has converged (their basis
                             vector is zero; the others' results are unchanged bit for bit);
                             0 = every problem in every step. */
+  uint32_t batch_groups; /* gp_cap_solve_batched: the FMM problems are solved in this many groups
+                            (each its own stacked operator and GMRES), and the host set-up of a
+                            group (octrees, near-field and interaction lists) runs on a second
+                            thread while the device solves the group before it; the first
+                            group's set-up overlaps the dense problems' solve.  Group k takes
+                            about 2^k / (2^groups - 1) of the FMM panels, so each set-up is about
+                            twice the one it hides behind.  0 (default) = 1 below 2 million FMM
+                            panels, else 3; results agree with one group to the solve
+                            tolerance. */
 } gp_cap_opts;
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 

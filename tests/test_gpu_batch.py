@@ -78,3 +78,43 @@ def test_skipping_converged_problems_changes_nothing(ctx, probs, prec):
     for a, b, x, y in zip(Cs, Ca, rs, ra):
         assert x["converged"] == 1 and x["iterations"] == y["iterations"]
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [G.GP_PREC_FP64, G.GP_PREC_MIXED])
+def test_groups_set_up_beside_the_solve_agree_with_one_group(ctx, probs, prec):
+    # batch_groups = 3: three stacked operators, each set up on a second host
+    # thread while the one before solves, the dense problems solving during
+    # the first set-up; every problem keeps its own tree and operator, so
+    # the solutions agree to the solve tolerance
+    common = dict(tol=1e-11, rhs_count=1, max_level=0, leaf_target=32, precision=prec)
+    C1, r1 = ctx.cap_solve_batched(probs, G.default_opts(batch_groups=1, **common))
+    C3, r3 = ctx.cap_solve_batched(probs, G.default_opts(batch_groups=3, **common))
+    for a, b, x, y, (v, _, _) in zip(C3, C1, r3, r1, probs):
+        assert x["converged"] == 1 and x["method_used"] == y["method_used"]
+        assert x["n_boxes"] == y["n_boxes"] and x["near_nnz"] == y["near_nnz"]
+        assert x["m2l_pairs"] == y["m2l_pairs"]
+        np.testing.assert_allclose(a[:, 0], b[:, 0], rtol=1e-8 if prec == G.GP_PREC_FP64 else 1e-6)
+    # each problem reports its own share of the operator
+    fm = [r for r in r1 if r["method_used"] == G.GP_METHOD_FMM]
+    assert len({r["near_nnz"] for r in fm}) == len(fm)
+
+
+def test_groups_under_the_profiler(ctx, probs):
+    # the set-up thread's kernel records join the calling thread's profile
+    ctx.profile(True)
+    try:
+        ctx.cap_solve_batched(probs, G.default_opts(batch_groups=3, tol=1e-10, rhs_count=1,
+                                                    max_level=0, leaf_target=32))
+        prof = ctx.profile_report()
+    finally:
+        ctx.profile(False)
+    assert prof["near_assemble"]["launches"] == 3
+
+
+def test_groups_report_the_failing_problem_by_its_batch_position(ctx, probs):
+    o = G.default_opts(method=G.GP_METHOD_FMM, batch_groups=3, tol=1e-14, restart=5,
+                       max_restarts=1, strict=1, rhs_count=1, max_level=0, leaf_target=32)
+    with pytest.raises(G.GpError) as ei:
+        ctx.cap_solve_batched(probs, o)
+    assert ei.value.code == G._lib.GP_ERR_NOT_CONVERGED
+    assert "for problem" in str(ei.value)
