@@ -206,7 +206,8 @@ def test_c5_routing_full_mixed_vs_fp64(ctx):
     v, c, _ = G.mesh_groups(groups, refine=3, max_panels=10 ** 9)
     nc = int(c.max()) + 1
     say(f"C5 refine 3: {len(v)} panels, {nc} conductors")
-    common = dict(method=G.GP_METHOD_FMM, fmm_order=6, max_level=0, tol=1e-10, rhs_first=0)
+    # tol 1e-14: the couplings among the 32 nets reach 5e-11 of the diagonal
+    common = dict(method=G.GP_METHOD_FMM, fmm_order=6, max_level=0, tol=1e-14, rhs_first=0)
     Cm, rm = ctx.cap_solve(v, c, nc, G.default_opts(precision=G.GP_PREC_MIXED, rhs_count=32,
                                                    **common))
     assert rm["converged"] == 1
@@ -215,14 +216,14 @@ def test_c5_routing_full_mixed_vs_fp64(ctx):
     C64, r64 = ctx.cap_solve(v, c, nc, G.default_opts(rhs_count=1, rhs_block=1, **common))
     assert r64["converged"] == 1
     col_m, col_64 = Cm[:, 0], C64[:, 0]
-    # the diagonal and every coupling of at least 1e-3 of the diagonal: the
-    # weaker couplings to distant nets carry the solve tolerance itself
-    strong = np.abs(col_64) >= 1e-3 * col_64[0]
-    rel = np.abs(col_m[strong] - col_64[strong]) / np.abs(col_64[strong])
-    say(f"  column 0: {int(strong.sum())} entries >= 1e-3 C00, mixed vs FP64 max {rel.max():.2e}; "
-        f"all rows: abs max {np.abs(col_m - col_64).max() / col_64[0]:.2e} of C00")
+    # every entry of the reported block's column 0 (the 32 nets), relative;
+    # the rows of all 6,260 conductors to the solve tolerance of C00
+    rel = np.abs(col_m[:32] - col_64[:32]) / np.abs(col_64[:32])
+    say(f"  column 0: the 32 reported rows, mixed vs FP64 max {rel.max():.2e} (smallest "
+        f"{np.abs(col_64[:32]).min() / col_64[0]:.1e} of C00); all rows: abs max "
+        f"{np.abs(col_m - col_64).max() / col_64[0]:.2e} of C00")
     assert rel.max() < 1e-6
-    assert np.abs(col_m - col_64).max() < 1e-8 * col_64[0]
+    assert np.abs(col_m - col_64).max() < 1e-12 * col_64[0]
     # signs, up to the far-field truncation noise of the distant couplings
     # (the reference's CPQR M2L keeps ~1e-4 of each interaction matrix)
     d = np.diag(B)
