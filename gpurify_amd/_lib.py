@@ -13,7 +13,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgpurify.so")
+# GPURIFY_LIB_VARIANT=san loads the host-sanitizer build (libgpurify_san.so,
+# `make -C gpurify_amd/csrc sanitize`), for the host-code tests under ASan
+LIB_PATH = os.path.join(_HERE, "libgpurify_san.so" if os.environ.get("GPURIFY_LIB_VARIANT") == "san"
+                        else "libgpurify.so")
 
 GP_OK = 0
 GP_ERR_INVALID_INPUT = 1

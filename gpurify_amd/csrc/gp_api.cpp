@@ -586,9 +586,9 @@ int gp_qui_parse(const char *text, size_t text_len, double scale, uint32_t *n_pa
     if (n_panels) *n_panels = (uint32_t)n;
     if (n_conductors) *n_conductors = (uint32_t)geo.names.size();
     if (names_len) *names_len = nl;
-    if (vertices) std::memcpy(vertices, geo.verts.data(), n * 12 * sizeof(double));
-    if (conductor) std::memcpy(conductor, geo.cond.data(), n * sizeof(uint32_t));
-    if (n_vertices) std::memcpy(n_vertices, geo.n_vertices.data(), n);
+    if (vertices && n) std::memcpy(vertices, geo.verts.data(), n * 12 * sizeof(double));
+    if (conductor && n) std::memcpy(conductor, geo.cond.data(), n * sizeof(uint32_t));
+    if (n_vertices && n) std::memcpy(n_vertices, geo.n_vertices.data(), n);
     if (names)
       for (const auto &s : geo.names) {
         std::memcpy(names, s.c_str(), s.size() + 1);
