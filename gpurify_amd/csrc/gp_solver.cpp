@@ -1645,6 +1645,20 @@ void solve_systems(FmmSystem &sys, const Ops &ops, const gp_cap_opts &o, bool mi
   // the autotuned kernel variants)
   GmresWork<float> work32;
   GmresWork<double> work64;
+  // A batch recomputes the outer residual only for the problems still being
+  // refined: a finished problem's x no longer changes, so its rows of A x
+  // (kept in ax from the step that finished it; zero for problems with no
+  // column here) and of r stay what a new apply would give, bit for bit.
+  bool mask_outer = false;
+  DBuf<int> d_lv;
+  std::vector<int> h_lv(nr);  // (the source of d_lv's copy outlives it)
+  if constexpr (Ops::maskable) {
+    mask_outer = sys.skip_converged && sys.box_prob.p;
+    if (mask_outer) {
+      ax.zero(s);
+      d_lv.alloc(nr);
+    }
+  }
   for (;;) {
     ops.dots(r.p, 0, 1, r.p, part.p, rn.p, true);
     GP_HIP(hipMemcpyAsync(hrn.data(), rn.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1668,7 +1682,13 @@ void solve_systems(FmmSystem &sys, const Ops &ops, const gp_cap_opts &o, bool mi
                        capture);
     for (int q = 0; q < nr; ++q) out.iters[q] += st.iters[q];
     ops.axpy(x, dx.p, 1.0);
+    if (mask_outer) {
+      h_lv = lv;
+      GP_HIP(hipMemcpyAsync(d_lv.p, h_lv.data(), nr * sizeof(int), hipMemcpyHostToDevice, s));
+      sys.mask(d_lv.p, h_lv.data());
+    }
     sys.apply(x, false, nc, ax.p, 0);
+    if (mask_outer) sys.unmask();
     ops.sub(b, ax.p, r.p);
     ++out.refine_steps;
     GP_LOG("refinement step %d: inner iterations up to %d", out.refine_steps,
