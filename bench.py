@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--fp64-rhs", type=int, default=0, help="RHS of the FP64 comparison (0 = all)")
     ap.add_argument("--batch-groups", type=int, default=0,
                     help="groups of a batched solve, set-up overlapping the solve (0 = automatic)")
+    ap.add_argument("--near-inner", choices=["fp16", "fp32"], default="fp32",
+                    help="near-field values of the single-RHS inner operator (nets4096)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
@@ -283,13 +285,16 @@ def bench_batch(args):
         f"{int((sizes < 500).sum())} below the dense limit")
     common = dict(fmm_order=args.order, max_level=0, leaf_target=args.leaf_target, tol=args.tol,
                   rhs_first=0, rhs_count=1, batch_groups=args.batch_groups)
-    o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol, **common)
+    o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol,
+                           near_inner=G.GP_NEAR_FP16 if args.near_inner == "fp16" else G.GP_NEAR_FP32,
+                           **common)
     if args.m2l_rank >= 0:
         o_mix.m2l_rank = args.m2l_rank
     o_64 = G.default_opts(**common)
     out = {"metric": METRIC, "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "bf16+fp32/fp64", "data": "synthetic (seeded sky130 layout nets4096)"}
+           "dtype": "fp16+bf16+fp32/fp64" if o_mix.near_inner == G.GP_NEAR_FP16 else "bf16+fp32/fp64",
+           "data": "synthetic (seeded sky130 layout nets4096)"}
     with G.Context(0) as ctx:
         log("device:", ctx.info())
 
@@ -511,7 +516,8 @@ def calib_copy(mib):
 PROFILE_SYMBOLS = {"m2l_bf16_mfma": "k_m2l_bf16_pipe", "m2l_bf16x3_mfma": "k_m2l_bf16x3",
                    "m2l1t_bf16_mfma": "k_m2l1t64_bf16",
                    "m2l_f64_mfma": "k_m2l_f64_pipe", "m2l1t_f64_mfma": "k_m2l1t64_f64",
-                   "near_spmv_f32": "k_near_spmv_leaf", "near_spmm_f32": "k_near_spmm_mfma"}
+                   "near_spmv_f32": "k_near_spmv_leaf", "near_spmv_f16": "k_near_spmv_leaf",
+                   "near_spmm_f32": "k_near_spmm_mfma"}
 
 
 def measured_traffic(name, workload):
@@ -566,11 +572,12 @@ def roofline_of(name, prof, work, workload):
     avg = t / L
     N, R, nnz, pairs, n3 = work["N"], work["R"], work["nnz"], work["pairs"], work["n3"]
     if name.startswith("near_sp"):
-        s = 4 if name.endswith("f32") else 8
+        s = {"f16": 2, "f32": 4}.get(name[-3:], 8)
+        row = 16 if name.endswith("f16") else s + 8  # (FP16: the row scale,) x read, y written
         bound, peak, unit = "hbm", PEAKS["hbm_GBs"], "GB/s"
         per = v["bytes"] / L
-        full = nnz * s + N * R * (s + 8)
-        basis = (f"nnz*{s} + N*R*({s}+8) bytes over the problems computed: {per:.4g} per launch "
+        full = nnz * s + N * R * row
+        basis = (f"nnz*{s} + N*R*{row} bytes over the problems computed: {per:.4g} per launch "
                  f"on average, {per / full:.3f} of the whole batch's {full:.4g} (nnz={nnz}, N={N}, R={R})")
     elif name.startswith("m2l") and ("mfma" in name):
         bound, unit = "mfma", "TFLOP/s"

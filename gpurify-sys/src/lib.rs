@@ -41,6 +41,8 @@ pub const GP_M2L_FP64_VALU: u32 = 3;
 pub const GP_M2L_BF16_VALU: u32 = 4;
 pub const GP_BASIS_FP32: u32 = 0;
 pub const GP_BASIS_FP64: u32 = 1;
+pub const GP_NEAR_FP32: u32 = 0;
+pub const GP_NEAR_FP16: u32 = 1;
 pub const GP_ROLE_CONDUCTOR: u32 = 0;
 pub const GP_ROLE_DIELECTRIC: u32 = 1;
 pub const GP_NOALLOC_MAX_PANELS: u32 = 512;
@@ -90,6 +92,7 @@ pub struct gp_cap_opts {
     pub fuse_leaf: u32,
     pub skip_converged: u32,
     pub batch_groups: u32,
+    pub near_inner: u32,
 }
 
 #[repr(C)]

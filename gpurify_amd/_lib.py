@@ -48,6 +48,8 @@ GP_M2L_FP64_VALU = 3
 GP_M2L_BF16_VALU = 4
 GP_BASIS_FP32 = 0
 GP_BASIS_FP64 = 1
+GP_NEAR_FP32 = 0
+GP_NEAR_FP16 = 1
 
 
 class GpPanels(C.Structure):
@@ -65,7 +67,8 @@ class GpCapOpts(C.Structure):
                 ("inner_passes", C.c_uint32), ("inner_basis", C.c_uint32),
                 ("m2l_rank", C.c_uint32), ("step_graphs", C.c_uint32),
                 ("m2l_compress", C.c_uint32), ("fuse_leaf", C.c_uint32),
-                ("skip_converged", C.c_uint32), ("batch_groups", C.c_uint32)]
+                ("skip_converged", C.c_uint32), ("batch_groups", C.c_uint32),
+                ("near_inner", C.c_uint32)]
 
 
 class GpParamMap(C.Structure):

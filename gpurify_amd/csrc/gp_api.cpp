@@ -214,6 +214,7 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->fuse_leaf = 1;
   o->skip_converged = 1;
   o->batch_groups = 0;
+  o->near_inner = GP_NEAR_FP32;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,

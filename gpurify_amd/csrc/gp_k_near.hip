@@ -101,7 +101,8 @@ __global__ __launch_bounds__(256) void k_near_spmv(
     int n, const int *__restrict__ panel_leaf, const int *__restrict__ leaf_lo,
     const int64_t *__restrict__ nf_off, const int *__restrict__ col_off,
     const int *__restrict__ cols, const TV *__restrict__ vals, const TX *__restrict__ x,
-    double *__restrict__ y, const unsigned char *__restrict__ skip) {
+    double *__restrict__ y, const unsigned char *__restrict__ skip,
+    const float *__restrict__ rscale) {
   using A = typename Acc<TX>::type;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256) void k_near_spmv(
   A acc = 0;
   for (int s = lane; s < S; s += 64) acc += (A)row[s] * (A)x[cols[c0 + s]];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-  if (lane == 0) y[i] = (double)acc;
+  if (lane == 0) y[i] = rscale ? (double)acc * (double)rscale[i] : (double)acc;
 }
 
 // Multi-RHS, register-tiled: one work-group per (target leaf, RHS chunk of
@@ -306,7 +307,9 @@ __global__ __launch_bounds__(256) void k_near_spmm_mfma(
   }
 }
 
-// 16-byte vector of values
+// One lane's vector load of tile values: 16 bytes of FP32 or FP64, 8 bytes
+// of FP16 (as many values per lane as FP32, so a row keeps as many lanes
+// busy, at half the bytes)
 template <typename TV>
 struct Vec16;
 template <>
@@ -319,6 +322,11 @@ struct Vec16<double> {
   typedef double type __attribute__((ext_vector_type(2)));
   static constexpr int n = 2;
 };
+template <>
+struct Vec16<_Float16> {
+  typedef _Float16 type __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
 
 // One work-group (WPB wavefronts) per target leaf, single RHS: xs = x[cols]
 // in LDS, then wavefront w takes rows w, w + WPB, ...; a row is an unaligned
@@ -327,7 +335,8 @@ template <typename TV, typename TX, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf(
     const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
     const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
-    const TX *__restrict__ x, double *__restrict__ y, const unsigned char *__restrict__ skip) {
+    const TX *__restrict__ x, double *__restrict__ y, const unsigned char *__restrict__ skip,
+    const float *__restrict__ rscale) {
   using A = typename Acc<TX>::type;
   using V = typename Vec16<TV>::type;
   constexpr int VN = Vec16<TV>::n;
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf(
     const int tl = head + nv * VN + lane;
     if (tl < S) acc += (A)row[tl] * xs[tl];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-    if (lane == 0) y[t0 + t] = (double)acc;
+    if (lane == 0) y[t0 + t] = rscale ? (double)acc * (double)rscale[t0 + t] : (double)acc;
   }
 }
 
@@ -371,7 +380,8 @@ template <typename TV, typename TX, int WPB, int R>
 __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf_pf(
     const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
     const int *__restrict__ col_off, const int *__restrict__ cols, const TV *__restrict__ vals,
-    const TX *__restrict__ x, double *__restrict__ y, const unsigned char *__restrict__ skip) {
+    const TX *__restrict__ x, double *__restrict__ y, const unsigned char *__restrict__ skip,
+    const float *__restrict__ rscale) {
   using A = typename Acc<TX>::type;
   using V = typename Vec16<TV>::type;
   constexpr int VN = Vec16<TV>::n;
@@ -430,7 +440,7 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf_pf(
       const int tl = head + nv * VN + lane;
       if (tl < S) acc += (A)pt[r] * xs[tl];
       for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-      if (lane == 0) y[t0 + t] = (double)acc;
+      if (lane == 0) y[t0 + t] = rscale ? (double)acc * (double)rscale[t0 + t] : (double)acc;
     }
   }
   for (int t = w + R * WPB; t < nt; t += WPB) {
@@ -451,8 +461,98 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf_pf(
     const int tl = head + nv * VN + lane;
     if (tl < S) acc += (A)row[tl] * xs[tl];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-    if (lane == 0) y[t0 + t] = (double)acc;
+    if (lane == 0) y[t0 + t] = rscale ? (double)acc * (double)rscale[t0 + t] : (double)acc;
   }
+}
+
+// Single RHS: the leaf-staged kernels with 4, 8, 2 or 1 wavefronts a leaf
+// (fewer: more leaves in flight per CU for leaves of a few panels), or one
+// wavefront per row; 5-7: leaf-staged with the first rows' values issued
+// before the gather.  rscale (FP16 values) multiplies each row's sum.
+template <typename TV, typename TX>
+void launch_spmv1(const char *tname, const NearTiles &nf, const TV *vals, const float *rscale,
+                  const TX *x, double *y, hipStream_t s) {
+  using A = typename Acc<TX>::type;
+  const size_t xs_bytes = (size_t)nf.max_cols * sizeof(A);
+  const bool leaf_ok = xs_bytes <= 48 * 1024;
+  tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 8 : 1, s, [&](int v) {
+    if (!leaf_ok || v == 2)
+      hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
+                         nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
+                         nf.skip_leaf, rscale);
+    else if (v == 0)
+      hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 4>), dim3(nf.n_leaves), dim3(256), xs_bytes, s,
+                         nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf, rscale);
+    else if (v == 1)
+      hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 8>), dim3(nf.n_leaves), dim3(512), xs_bytes, s,
+                         nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf, rscale);
+    else if (v == 3)
+      hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 2>), dim3(nf.n_leaves), dim3(128), xs_bytes, s,
+                         nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf, rscale);
+    else if (v == 4)
+      hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 1>), dim3(nf.n_leaves), dim3(64), xs_bytes, s,
+                         nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf, rscale);
+    else if (v == 5)
+      hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 2>), dim3(nf.n_leaves), dim3(128),
+                         xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
+                         nf.skip_leaf, rscale);
+    else if (v == 6)
+      hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 3>), dim3(nf.n_leaves), dim3(128),
+                         xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
+                         nf.skip_leaf, rscale);
+    else
+      hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 4, 2>), dim3(nf.n_leaves), dim3(256),
+                         xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
+                         nf.skip_leaf, rscale);
+  });
+}
+
+// FP16 copy of the near tiles for the single-RHS inner operator: per row
+// the scale is the largest magnitude (1 for a zero row) and each value is
+// stored as fp16(v / scale), one rounding from FP64; a work-group per leaf,
+// a wavefront per row.
+__global__ __launch_bounds__(256) void k_near_to_f16(const int *__restrict__ leaf_lo,
+                                                      const int64_t *__restrict__ nf_off,
+                                                      const int *__restrict__ col_off,
+                                                      const double *__restrict__ v64,
+                                                      _Float16 *__restrict__ v16,
+                                                      float *__restrict__ rscale) {
+  const int T = blockIdx.x;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int S = col_off[T + 1] - col_off[T];
+  const int64_t base = nf_off[T];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int t = w; t < nt; t += 4) {
+    const double *row = v64 + base + (int64_t)t * S;
+    double m = 0.0;
+    for (int c = lane; c < S; c += 64) m = fmax(m, fabs(row[c]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    const float sc = m > 0.0 ? (float)m : 1.0f;
+    const double inv = 1.0 / (double)sc;
+    _Float16 *h = v16 + base + (int64_t)t * S;
+    for (int c = lane; c < S; c += 64) h[c] = (_Float16)(row[c] * inv);
+    if (lane == 0) rscale[t0 + t] = sc;
+  }
+}
+
+void launch_near_to_f16(const NearTiles &nf, const double *v64, _Float16 *v16, float *rscale,
+                        hipStream_t s) {
+  if (nf.n_leaves == 0) return;
+  hipLaunchKernelGGL(k_near_to_f16, dim3(nf.n_leaves), dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
+                     nf.col_off, v64, v16, rscale);
+  GP_LAUNCH_CHECK();
+}
+
+void launch_near_apply_f16(const NearTiles &nf, const _Float16 *vals, const float *rscale,
+                           const float *x, double *y, hipStream_t s) {
+  if (nf.n_leaves == 0) return;
+  // the values, and per row its scale, x and y (the rows not skipped)
+  const double flops = 2.0 * nf.nnz * nf.frac_nnz;
+  const double bytes = (double)nf.nnz * sizeof(_Float16) * nf.frac_nnz +
+                       (double)nf.n * (sizeof(float) + sizeof(float) + sizeof(double)) * nf.frac_rows;
+  ProfScope ps("near_spmv_f16", s, flops, bytes);
+  launch_spmv1<_Float16, float>("near_spmv_f16", nf, vals, rscale, x, y, s);
+  GP_LAUNCH_CHECK();
 }
 
 template <typename TV, typename TX>
@@ -466,14 +566,7 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
   ProfScope ps(sizeof(TV) == 4 ? (nr == 1 ? "near_spmv_f32" : "near_spmm_f32")
                                : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64"),
                s, flops, bytes);
-  using A = typename Acc<TX>::type;
-  const size_t xs_bytes = (size_t)nf.max_cols * sizeof(A);
   // kernel variants, fastest per shape picked by the context's autotuner
-  auto spmv_row = [&] {
-    hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
-                       nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
-                       nf.skip_leaf);
-  };
   auto spmm16 = [&] {
     const dim3 grid(nf.n_leaves * ((nr + NF_RC - 1) / NF_RC));
     hipLaunchKernelGGL((k_near_spmm<TV, TX>), grid, dim3(256), 0, s, nf.leaf_lo, nf.nf_off,
@@ -492,35 +585,7 @@ void launch_near_apply_t(const NearTiles &nf, const TV *vals, const TX *x, int l
   const char *tname = sizeof(TV) == 4 ? (nr == 1 ? "near_spmv_f32" : "near_spmm_f32")
                                       : (nr == 1 ? "near_spmv_f64" : "near_spmm_f64");
   if (nr == 1 && ldx == 1 && ldy == 1) {
-    // leaf-staged with 4, 8, 2 or 1 wavefronts a leaf (fewer: more leaves in
-    // flight per CU for leaves of a few panels), or one wavefront per row;
-    // 5-7: leaf-staged with the first rows' values issued before the gather
-    const bool leaf_ok = xs_bytes <= 48 * 1024;
-    tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 8 : 1, s, [&](int v) {
-      if (!leaf_ok || v == 2)
-        spmv_row();
-      else if (v == 0)
-        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 4>), dim3(nf.n_leaves), dim3(256), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
-      else if (v == 1)
-        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 8>), dim3(nf.n_leaves), dim3(512), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
-      else if (v == 3)
-        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 2>), dim3(nf.n_leaves), dim3(128), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
-      else if (v == 4)
-        hipLaunchKernelGGL((k_near_spmv_leaf<TV, TX, 1>), dim3(nf.n_leaves), dim3(64), xs_bytes,
-                           s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
-      else if (v == 5)
-        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 2>), dim3(nf.n_leaves), dim3(128),
-                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
-      else if (v == 6)
-        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 2, 3>), dim3(nf.n_leaves), dim3(128),
-                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
-      else
-        hipLaunchKernelGGL((k_near_spmv_leaf_pf<TV, TX, 4, 2>), dim3(nf.n_leaves), dim3(256),
-                           xs_bytes, s, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y, nf.skip_leaf);
-    });
+    launch_spmv1<TV, TX>(tname, nf, vals, nullptr, x, y, s);
   } else if (nr > 16) {
     // FP32 matrix cores (64-column chunks, 16-row tiles), register tiles of
     // 64 or 32 RHS columns, or 16-column work-groups

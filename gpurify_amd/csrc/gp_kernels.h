@@ -99,6 +99,12 @@ void launch_near_apply(const NearTiles &nf, const double *vals, const double *x,
                        double *y, int ldy, int nr, hipStream_t s);
 void launch_near_apply(const NearTiles &nf, const float *vals, const float *x, int ldx,
                        double *y, int ldy, int nr, hipStream_t s);
+// Single RHS on FP16 tiles scaled per row (rscale: one FP32 scale per panel).
+void launch_near_apply_f16(const NearTiles &nf, const _Float16 *vals, const float *rscale,
+                           const float *x, double *y, hipStream_t s);
+// The FP16 tiles and row scales from the FP64 values.
+void launch_near_to_f16(const NearTiles &nf, const double *v64, _Float16 *v16, float *rscale,
+                        hipStream_t s);
 // Transposed tiles: per leaf S (CSR tr.off), its near leaves T with the
 // value offset and row stride of block (T, S) inside T's tile.
 struct NearTranspose {

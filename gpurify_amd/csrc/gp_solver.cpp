@@ -376,6 +376,10 @@ struct FmmSystem {
   DBuf<int64_t> nf_off;
   DBuf<double> nf64;
   DBuf<float> nf32;
+  // single-RHS inner operator on FP16 tiles (near_inner): values and row scales
+  bool near16 = false;
+  DBuf<_Float16> nf16;
+  DBuf<float> nscale;
   DBuf<int> parent, octant, child_off, child_idx, targets, il_off, il_src, il_oidx, box_level;
   DBuf<double> inv_half;
   DBuf<double> w1d64, m2m64, G64;
@@ -644,9 +648,12 @@ struct FmmSystem {
     nf_off = to_device(hnf, s);
     nf64.alloc((size_t)nnz);
     have32 = need32;
-    if (need32) nf32.alloc((size_t)nnz);
+    if (o.near_inner != GP_NEAR_FP32 && o.near_inner != GP_NEAR_FP16)
+      throw Error(GP_ERR_INVALID_INPUT, "near_inner must be GP_NEAR_FP32 or GP_NEAR_FP16");
+    near16 = need32 && want_by_offset && o.near_inner == GP_NEAR_FP16 && !want_transpose && !diel;
+    if (need32 && !near16) nf32.alloc((size_t)nnz);
     launch_near_assemble(n_leaves, leaf_lo.p, nf_off.p, col_off.p, cols.p, verts.p, cen.p, nf64.p,
-                         need32 ? nf32.p : nullptr, s);
+                         (need32 && !near16) ? nf32.p : nullptr, s);
     inv_diag.alloc(N);
     const double k_e = 1.0 / (4.0 * kPi * kEps0);
     for (int p = 0; p < P; ++p) {
@@ -698,6 +705,11 @@ struct FmmSystem {
     nt.col_off = col_off.p;
     nt.cols = cols.p;
     nt.panel_leaf = panel_leaf.p;
+    if (near16) {
+      nf16.alloc((size_t)nnz);
+      nscale.alloc(N);
+      launch_near_to_f16(nt, nf64.p, nf16.p, nscale.p, s);
+    }
 
     phase("near");
     // FMM tables.
@@ -1203,7 +1215,12 @@ struct FmmSystem {
     } else {
       if (!have32) throw Error(GP_ERR_INTERNAL, "FP32 operator not assembled");
       kr_precond_cast(v, dinv, z32.p, N, ld, nr, s);
-      launch_near_apply(nt, nf32.p, z32.p, ld, y, ld, nr, s);
+      if (near16) {
+        if (ld != 1 || nr != 1) throw Error(GP_ERR_INTERNAL, "FP16 near tiles take one RHS");
+        launch_near_apply_f16(nt, nf16.p, nscale.p, z32.p, y, s);
+      } else {
+        launch_near_apply(nt, nf32.p, z32.p, ld, y, ld, nr, s);
+      }
       run_far_field(f32, z32.p, ld, nr, M32.p, Mbf.p, Lc32.p, ld, inner_m2l(), y, ld, s, Mt32.p,
                     Mtbf.p, Lt32.p, Mtbf_lo.p);
     }

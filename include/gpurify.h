@@ -87,6 +87,9 @@ enum { GP_METHOD_AUTO = 0, GP_METHOD_DENSE = 1, GP_METHOD_FMM = 2 };
  * a cross-check of the matrix-core path; inner operator only). */
 enum { GP_M2L_FP64 = 0, GP_M2L_FP32 = 1, GP_M2L_BF16 = 2, GP_M2L_FP64_VALU = 3,
        GP_M2L_BF16_VALU = 4 };
+/* Near-field values of the inner operator of single-RHS mixed solves: FP32,
+ * or FP16 scaled per row by the row's largest magnitude (FP32 scale). */
+enum { GP_NEAR_FP32 = 0, GP_NEAR_FP16 = 1 };
 
 /* Solver options.  gp_cap_opts_default() fills the reference's own
  * constants: the dense/FMM rule of bridge.rs:397-404 (dense below 500
@@ -146,6 +149,14 @@ typedef struct {
                             twice the one it hides behind.  0 (default) = 1 below 2 million FMM
                             panels, else 3; results agree with one group to the solve
                             tolerance. */
+  uint32_t near_inner;   /* GP_PREC_MIXED with one RHS per problem (gp_cap_solve_batched, or
+                            rhs_block 1): the inner operator's near-field values, GP_NEAR_FP32
+                            (default) or GP_NEAR_FP16 (each row scaled by its largest magnitude
+                            and stored as FP16, 11 significant bits per entry: half the memory
+                            of FP32, the same answer; the single-RHS product is bound by its
+                            per-leaf load latency, not by the value bytes, and measured slower
+                            with FP16 values).  Blocks of RHS always keep FP32 values; the outer
+                            residual is FP64 either way. */
 } gp_cap_opts;
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 

@@ -118,3 +118,24 @@ def test_groups_report_the_failing_problem_by_its_batch_position(ctx, probs):
         ctx.cap_solve_batched(probs, o)
     assert ei.value.code == G._lib.GP_ERR_NOT_CONVERGED
     assert "for problem" in str(ei.value)
+
+
+def test_fp16_near_values_reach_the_fp64_answer(ctx, probs):
+    # the inner operator of one-RHS batches on FP16 near tiles or FP32 (the
+    # default): both refine to the same FP64 operator's solution
+    common = dict(method=G.GP_METHOD_FMM, tol=1e-10, rhs_count=1, max_level=0, leaf_target=32)
+    C64, _ = ctx.cap_solve_batched(probs, G.default_opts(**common))
+    mixed = dict(common, precision=G.GP_PREC_MIXED)
+    C16, r16 = ctx.cap_solve_batched(probs, G.default_opts(near_inner=G.GP_NEAR_FP16, **mixed))
+    C32, r32 = ctx.cap_solve_batched(probs, G.default_opts(near_inner=G.GP_NEAR_FP32, **mixed))
+    for a, b, c, x, y in zip(C16, C32, C64, r16, r32):
+        assert x["converged"] == 1 and y["converged"] == 1
+        assert x["refine_steps"] <= y["refine_steps"] + 1
+        assert np.max(np.abs(a[:, 0] - c[:, 0]) / np.abs(c[:, 0])) < 1e-6
+        assert np.max(np.abs(b[:, 0] - c[:, 0]) / np.abs(c[:, 0])) < 1e-6
+
+
+def test_near_inner_is_validated(ctx, probs):
+    with pytest.raises(G.GpError, match="INVALID_INPUT"):
+        ctx.cap_solve_batched(probs[:2], G.default_opts(precision=G.GP_PREC_MIXED, near_inner=7,
+                                                        method=G.GP_METHOD_FMM))

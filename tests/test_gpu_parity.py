@@ -110,6 +110,29 @@ def test_single_rhs_bf16_m2l_matches_valu(ctx, order):
     assert np.linalg.norm(Yb - Yv) / nrm < min(2e-5, 0.05 * eb)
 
 
+@pytest.mark.parametrize("order", [3, 4, 5])
+def test_single_rhs_fp16_near_values(ctx, order):
+    # one RHS, mixed: the inner near field as FP16 scaled per row (11
+    # significant bits per entry) against FP32 values, with the FP32 M2L so
+    # only the near field differs.  Rounding each entry to 2^-12 relative
+    # moves the operator by ~1e-4; a wrong scale, a lost row or a misread
+    # vector moves it by the near field itself (~1)
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    x = np.random.default_rng(order).normal(size=(len(v), 1))
+    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, m2l_inner=G.GP_M2L_FP32)
+    Y64 = ctx.fmm_apply(v, x, o)
+    o.near_inner = G.GP_NEAR_FP32
+    Y32 = ctx.fmm_apply(v, x, o, precision=G.GP_PREC_MIXED)
+    o.near_inner = G.GP_NEAR_FP16
+    Y16 = ctx.fmm_apply(v, x, o, precision=G.GP_PREC_MIXED)
+    nrm = np.linalg.norm(Y64)
+    e32 = np.linalg.norm(Y32 - Y64) / nrm
+    e16 = np.linalg.norm(Y16 - Y64) / nrm
+    assert e32 < 1e-5 and 1e-7 < e16 < 1e-3
+    # entry by entry: each output row within the FP16 rounding of its own row
+    assert np.max(np.abs(Y16 - Y32)) < 2e-3 * np.max(np.abs(Y64))
+
+
 @pytest.mark.parametrize("order,ncols", [(5, 97), (6, 32), (5, 1), (6, 1), (5, 16)])
 def test_fmm_low_rank_operator(ctx, order, ncols):
     # rank-64 projection of the M2L matrices (inner operator of a block of
@@ -117,8 +140,9 @@ def test_fmm_low_rank_operator(ctx, order, ncols):
     # (orders 5 and 6); one RHS keeps the full rank
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     X = np.random.default_rng(order + ncols).normal(size=(len(v), ncols))
+    # (FP32 near values for one RHS as for a block: the bounds are FP32's)
     o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=order, m2l_inner=G.GP_M2L_FP32,
-                       m2l_rank=0)
+                       m2l_rank=0, near_inner=G.GP_NEAR_FP32)
     Y64 = ctx.fmm_apply(v, X, o)
     nrm = np.linalg.norm(Y64, axis=0)
     Yf = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
@@ -196,9 +220,10 @@ def test_fmm_single_and_block_rhs_agree(ctx, order, ncols):
     # 64-target tiles at orders <= 4, grouped by offset above), FP64 and FP32
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     X = np.random.default_rng(ncols).normal(size=(len(v), ncols))
-    # (full rank: one RHS never takes the projected operator)
+    # (full rank: one RHS never takes the projected operator; FP32 near values
+    # for one RHS as for the block)
     o = G.default_opts(method=G.GP_METHOD_FMM, m2l_inner=G.GP_M2L_FP32, fmm_order=order,
-                       m2l_rank=0)
+                       m2l_rank=0, near_inner=G.GP_NEAR_FP32)
     Yb = ctx.fmm_apply(v, X, o)
     Yb32 = ctx.fmm_apply(v, X, o, precision=G.GP_PREC_MIXED)
     for j in sorted({0, ncols // 2, ncols - 1}):
