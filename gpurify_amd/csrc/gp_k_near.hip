@@ -465,6 +465,65 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_leaf_pf(
   }
 }
 
+// Single RHS over packs of consecutive leaves of one problem (pack[P] ..
+// pack[P + 1]: one leaf with a wide tile, or up to 32 small leaves whose
+// columns add up to at most 1024): one gather of the pack's x[cols] into LDS
+// (the pack's column lists are contiguous), then 16 lanes per row, four rows
+// of the pack at a time per wavefront, each row against its own leaf's
+// columns.  Most leaves of a one-RHS batch hold one to a few panels, so a
+// work-group per leaf with a wavefront per row spent its time on load chains
+// (offsets -> columns -> x -> LDS, then offsets -> values) rather than on the
+// tile; here the chains of a whole pack overlap.  Per row the same products
+// as the leaf kernels, summed in another order.
+template <typename TV, typename TX, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_near_spmv_pack(
+    const int *__restrict__ pack, const int *__restrict__ leaf_lo,
+    const int64_t *__restrict__ nf_off, const int *__restrict__ col_off,
+    const int *__restrict__ cols, const int *__restrict__ panel_leaf,
+    const TV *__restrict__ vals, const TX *__restrict__ x, double *__restrict__ y,
+    const unsigned char *__restrict__ skip, const float *__restrict__ rscale) {
+  using A = typename Acc<TX>::type;
+  using V = typename Vec16<TV>::type;
+  constexpr int VN = Vec16<TV>::n;
+  extern __shared__ unsigned char smem_raw[];
+  A *xs = reinterpret_cast<A *>(smem_raw);
+  const int P = blockIdx.x;
+  const int T0 = pack[P], T1 = pack[P + 1];
+  if (skip && skip[T0]) return;  // one problem per pack: work-group-uniform
+  const int c0 = col_off[T0], SC = col_off[T1] - c0;
+  const int r0 = leaf_lo[T0], r1 = leaf_lo[T1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int gl = lane & 15, slot = lane >> 4;
+  for (int c = threadIdx.x; c < SC; c += 64 * WPB) xs[c] = (A)x[cols[c0 + c]];
+  __syncthreads();
+  for (int rb = r0 + 4 * w; rb < r1; rb += 4 * WPB) {
+    const int r = rb + slot;
+    A acc = 0;
+    if (r < r1) {
+      const int T = panel_leaf[r];
+      const int cT = col_off[T], S = col_off[T + 1] - cT;
+      const int64_t st = nf_off[T] + (int64_t)(r - leaf_lo[T]) * S;
+      const TV *row = vals + st;
+      const A *xr = xs + (cT - c0);
+      const int head = min((int)((VN - (st & (VN - 1))) & (VN - 1)), S);
+      const int nv = (S - head) / VN;
+      if (gl < head) acc += (A)row[gl] * xr[gl];
+      const V *rv = reinterpret_cast<const V *>(row + head);
+#pragma unroll 4
+      for (int v = gl; v < nv; v += 16) {
+        const V vv = rv[v];
+        const A *xq = xr + head + v * VN;
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc += (A)vv[k] * xq[k];
+      }
+      const int tl = head + nv * VN + gl;
+      if (tl < S) acc += (A)row[tl] * xr[tl];
+    }
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_down(acc, off, 16);
+    if (gl == 0 && r < r1) y[r] = rscale ? (double)acc * (double)rscale[r] : (double)acc;
+  }
+}
+
 // Single RHS: the leaf-staged kernels with 4, 8, 2 or 1 wavefronts a leaf
 // (fewer: more leaves in flight per CU for leaves of a few panels), or one
 // wavefront per row; 5-7: leaf-staged with the first rows' values issued
@@ -475,8 +534,18 @@ void launch_spmv1(const char *tname, const NearTiles &nf, const TV *vals, const 
   using A = typename Acc<TX>::type;
   const size_t xs_bytes = (size_t)nf.max_cols * sizeof(A);
   const bool leaf_ok = xs_bytes <= 48 * 1024;
-  tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, leaf_ok ? 8 : 1, s, [&](int v) {
-    if (!leaf_ok || v == 2)
+  const int nvar = !leaf_ok ? 1 : nf.n_packs > 0 ? 10 : 8;
+  tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, nvar, s, [&](int v) {
+    if (v == 8 || v == 9) {  // packs of small leaves, 2 or 4 wavefronts, 16 lanes a row
+      if (v == 8)
+        hipLaunchKernelGGL((k_near_spmv_pack<TV, TX, 2>), dim3(nf.n_packs), dim3(128), xs_bytes, s,
+                           nf.pack, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, nf.panel_leaf, vals,
+                           x, y, nf.skip_leaf, rscale);
+      else
+        hipLaunchKernelGGL((k_near_spmv_pack<TV, TX, 4>), dim3(nf.n_packs), dim3(256), xs_bytes, s,
+                           nf.pack, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, nf.panel_leaf, vals,
+                           x, y, nf.skip_leaf, rscale);
+    } else if (!leaf_ok || v == 2)
       hipLaunchKernelGGL((k_near_spmv<TV, TX>), dim3((nf.n + 3) / 4), dim3(256), 0, s, nf.n,
                          nf.panel_leaf, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
                          nf.skip_leaf, rscale);

@@ -90,6 +90,10 @@ struct NearTiles {
   const int *cols = nullptr;        // source panel per column
   const int *panel_leaf = nullptr;  // n
   int max_cols = 0;                  // widest tile (source columns of one leaf)
+  // Single RHS: packs of consecutive leaves of one problem (n_packs + 1
+  // first leaves; each pack's columns fit max_cols), or none.
+  int n_packs = 0;
+  const int *pack = nullptr;
   // Single right-hand side: leaves flagged 1 are skipped (null = none); the
   // profiler counts frac_nnz of the values and frac_rows of the rows.
   const unsigned char *skip_leaf = nullptr;

@@ -376,6 +376,7 @@ struct FmmSystem {
   DBuf<int64_t> nf_off;
   DBuf<double> nf64;
   DBuf<float> nf32;
+  DBuf<int> packs;  // single-RHS near-field packs of leaves (NearTiles::pack)
   // single-RHS inner operator on FP16 tiles (near_inner): values and row scales
   bool near16 = false;
   DBuf<_Float16> nf16;
@@ -700,6 +701,25 @@ struct FmmSystem {
     nt.max_cols = 0;
     for (int T = 0; T < n_leaves; ++T)
       nt.max_cols = std::max(nt.max_cols, hcol_off[T + 1] - hcol_off[T]);
+    if (want_by_offset) {
+      // packs of consecutive leaves of one problem for the single-RHS
+      // product: up to 32 leaves and 1024 columns (a wider leaf alone)
+      const int lim = std::min(1024, nt.max_cols);
+      std::vector<int> hp;
+      for (int T = 0; T < n_leaves;) {
+        hp.push_back(T);
+        const int p = gp_of[hleaf_box[T]];
+        int c = hcol_off[T + 1] - hcol_off[T], k = 1;
+        for (++T; T < n_leaves && k < 32 && gp_of[hleaf_box[T]] == p &&
+                  c + (hcol_off[T + 1] - hcol_off[T]) <= lim;
+             ++T, ++k)
+          c += hcol_off[T + 1] - hcol_off[T];
+      }
+      hp.push_back(n_leaves);
+      packs = to_device(hp, s);
+      nt.n_packs = (int)hp.size() - 1;
+      nt.pack = packs.p;
+    }
     nt.leaf_lo = leaf_lo.p;
     nt.nf_off = nf_off.p;
     nt.col_off = col_off.p;
