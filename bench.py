@@ -317,7 +317,7 @@ def bench_batch(args):
                          "dense_nets": len(probs) - len(fm), "fmm_nets": len(fm),
                          "fmm_panels": nfmm, "max_depth": max(r["max_level"] for r in fm),
                          "near_nnz": nnz, "m2l_pairs": pairs,
-                         "batch_groups": o_mix.batch_groups or (1 if nfmm < 2_000_000 else 3),
+                         "batch_groups": o_mix.batch_groups or (1 if nfmm < 2_000_000 else 4),
                          "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),
                          "refine_steps_max": max(r["refine_steps"] for r in fm),
                          "converged": all(r["converged"] for r in reps), "global_batch": len(probs)}
