@@ -309,6 +309,7 @@ def bench_batch(args):
                    value_incl_upload=n * args.steps / T)
         it = np.array([r["iterations"] for r in fm])
         nfmm = int(sum(r["n_panels"] for r in fm))
+        ngroups = o_mix.batch_groups or (1 if nfmm < 2_000_000 else 4)  # the library's rule
         nnz = int(sum(r["near_nnz"] for r in fm))  # each problem's own share
         pairs = int(sum(r["m2l_pairs"] for r in fm))
         out["config"] = {"workload": "nets4096_met1met2", "nets": len(probs), "panels": n,
@@ -317,13 +318,14 @@ def bench_batch(args):
                          "dense_nets": len(probs) - len(fm), "fmm_nets": len(fm),
                          "fmm_panels": nfmm, "max_depth": max(r["max_level"] for r in fm),
                          "near_nnz": nnz, "m2l_pairs": pairs,
-                         "batch_groups": o_mix.batch_groups or (1 if nfmm < 2_000_000 else 4),
+                         "batch_groups": ngroups,
                          "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),
                          "refine_steps_max": max(r["refine_steps"] for r in fm),
                          "converged": all(r["converged"] for r in reps), "global_batch": len(probs)}
         work = {"N": nfmm, "R": 1, "nnz": nnz, "pairs": pairs,
                 "n3": args.order ** 3}
-        report_profile(out, prof, work, "nets4096")
+        # (counter summaries are per group count: the launches differ)
+        report_profile(out, prof, work, f"nets4096_g{ngroups}")
         out["autotune"] = autotune_choices(ctx)
         if not args.no_fp64:
             t = time.perf_counter()

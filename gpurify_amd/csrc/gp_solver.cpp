@@ -1880,10 +1880,14 @@ struct StagedGroup {
   std::exception_ptr err;
 };
 
-// FmmSystem::setup of one group on ctx->setup_stream, from a thread of its
-// own (device current, profiling into the group's own records).
+// FmmSystem::setup of one group on ctx->setup_stream (device current,
+// profiling into the group's own records, uploads timed apart), from a
+// thread of its own or from the calling thread, whose per-thread profiler
+// and upload time are restored.
 void stage_group(gp_ctx *ctx, const gp_cap_opts &o, bool mixed, bool profile, StagedGroup &st) {
   st.t0 = now_s();
+  Prof *const prev_prof = active_prof();
+  const double prev_upload = upload_seconds();
   try {
     GP_HIP(hipSetDevice(ctx->device));
     st.prof.on = profile;
@@ -1896,8 +1900,9 @@ void stage_group(gp_ctx *ctx, const gp_cap_opts &o, bool mixed, bool profile, St
   } catch (...) {
     st.err = std::current_exception();
   }
-  active_prof() = nullptr;
+  active_prof() = prev_prof;
   st.upload = upload_seconds();
+  upload_seconds() = prev_upload;
   st.t1 = now_s();
 }
 
@@ -2035,7 +2040,12 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
     }
   };
   std::unique_ptr<StagedGroup> cur = make(0), next;
-  {
+  if (groups.size() == 1) {
+    // one group: nothing to overlap but the dense problems; every launch
+    // from this thread (the set-up runs here, on the set-up stream)
+    if (before) before();
+    stage_group(ctx, o, mixed, profile, *cur);
+  } else {
     StagedGroup *c = cur.get();
     std::thread th([&, c] { stage_group(ctx, o, mixed, profile, *c); });
     try {

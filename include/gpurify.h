@@ -148,8 +148,9 @@ typedef struct {
                             a share of the FMM panels proportional to 1.5^k (a set-up costs
                             about 0.6 of the solve of the same panels, so each set-up hides
                             behind the previous group's solve).  0 (default) = 1 below 2
-                            million FMM panels, else 4; results agree with one group to the
-                            solve tolerance. */
+                            million FMM panels, else 4; 1 = one group, every launch from the
+                            calling thread; results agree with one group to the solve
+                            tolerance. */
   uint32_t near_inner;   /* GP_PREC_MIXED with one RHS per problem (gp_cap_solve_batched, or
                             rhs_block 1): the inner operator's near-field values, GP_NEAR_FP32
                             (default) or GP_NEAR_FP16 (each row scaled by its largest magnitude
