@@ -93,6 +93,9 @@ pub struct gp_cap_opts {
     pub skip_converged: u32,
     pub batch_groups: u32,
     pub near_inner: u32,
+    pub inner_order: u32,
+    pub ground: u32,
+    pub ground_z: f64,
 }
 
 #[repr(C)]

@@ -10,6 +10,7 @@ from ._lib import (  # noqa: F401
     solve_resistance, version,
     GP_PREC_FP64, GP_PREC_MIXED, GP_METHOD_AUTO, GP_METHOD_DENSE, GP_METHOD_FMM,
     GP_M2L_FP64, GP_M2L_FP32, GP_M2L_BF16, GP_M2L_FP64_VALU, GP_M2L_BF16_VALU, GP_BASIS_FP32, GP_BASIS_FP64, GP_NEAR_FP32, GP_NEAR_FP16,
+    GP_GROUND_NONE, GP_GROUND_IMAGE,
     LIB_PATH,
 )
 from . import workloads  # noqa: F401

@@ -50,6 +50,8 @@ GP_BASIS_FP32 = 0
 GP_BASIS_FP64 = 1
 GP_NEAR_FP32 = 0
 GP_NEAR_FP16 = 1
+GP_GROUND_NONE = 0
+GP_GROUND_IMAGE = 1
 
 
 class GpPanels(C.Structure):
@@ -68,7 +70,8 @@ class GpCapOpts(C.Structure):
                 ("m2l_rank", C.c_uint32), ("step_graphs", C.c_uint32),
                 ("m2l_compress", C.c_uint32), ("fuse_leaf", C.c_uint32),
                 ("skip_converged", C.c_uint32), ("batch_groups", C.c_uint32),
-                ("near_inner", C.c_uint32)]
+                ("near_inner", C.c_uint32), ("inner_order", C.c_uint32),
+                ("ground", C.c_uint32), ("ground_z", C.c_double)]
 
 
 class GpParamMap(C.Structure):

@@ -215,6 +215,9 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->skip_converged = 1;
   o->batch_groups = 0;
   o->near_inner = GP_NEAR_FP32;
+  o->inner_order = 0;
+  o->ground = GP_GROUND_NONE;
+  o->ground_z = 0.0;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,
@@ -288,6 +291,8 @@ int gp_cap_solve_dielectric_fmm(gp_ctx *ctx, const gp_panels *panels, const gp_p
     else
       gp_cap_opts_default(&o);
     GP_HIP(hipSetDevice(ctx->device));
+    if (o.ground != GP_GROUND_NONE)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "the dielectric solves take no ground plane");
     gp::PanelGeom g;
     gp::panel_geometry(*panels, g);
     gp::solve_dielectric_fmm(ctx, g, roles, o, C, report);
@@ -356,6 +361,8 @@ int gp_cap_jacobian(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opt
     else
       gp_cap_opts_default(&o);
     GP_HIP(hipSetDevice(ctx->device));
+    if (o.ground != GP_GROUND_NONE)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "gp_cap_jacobian takes no ground plane");
     gp::PanelGeom g;
     gp::panel_geometry(*panels, g);
     gp::ParamMap pm;
@@ -383,6 +390,8 @@ int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
     else
       gp_cap_opts_default(&o);
     GP_HIP(hipSetDevice(ctx->device));
+    if (o.ground != GP_GROUND_NONE)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "gp_fmm_apply takes no ground plane");
     gp::PanelGeom g;
     gp::panel_geometry(*panels, g);
     gp::fmm_apply_host(ctx, g, o, (int)nr, X, Y, precision == GP_PREC_MIXED ? 1 : 0);
@@ -513,7 +522,18 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
     gp::Meshed m;
     gp::mesh_groups(groups, n_groups, xy_scale, (int)refine, max_panels, m);
     const int nn = (int)m.nets.size(), nc = nn + 1;
-    gp_panels p{(uint32_t)m.cond.size(), (uint32_t)nc, m.verts.data(), m.cond.data()};
+    // A ground plane by images takes the place of the substrate plate, at
+    // the plate's height: the conductors alone, the plane as the reference.
+    const bool image = o.ground == GP_GROUND_IMAGE;
+    if (image) {
+      const size_t keep = m.cond.size() - m.n_plate;
+      m.cond.resize(keep);
+      m.eps.resize(keep);
+      m.verts.resize(keep * 12);
+      o.ground_z = m.ground_z;
+    }
+    const int ncs = image ? nn : nc;  // conductors of the solve
+    gp_panels p{(uint32_t)m.cond.size(), (uint32_t)ncs, m.verts.data(), m.cond.data()};
     gp::PanelGeom g;
     gp::panel_geometry(p, g);
     // Homogeneity test of bridge.rs:393-396.
@@ -521,7 +541,7 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
     bool homog = true;
     for (double e : m.eps)
       if (std::fabs(e - first) > std::max(std::fabs(first), 1.0) * 1.0e-12) homog = false;
-    std::vector<double> C((size_t)nc * nc);
+    std::vector<double> C((size_t)ncs * ncs);
     if (homog) {
       gp::cap_solve(ctx, g, o, C.data(), report, nullptr);
       for (auto &v : C) v *= first;  // bridge.rs:405-407
@@ -540,6 +560,22 @@ int gp_solve_capacitance(gp_ctx *ctx, const gp_volume_group *groups, uint32_t n_
         od.method = GP_METHOD_DENSE;
       }
       gp::cap_solve(ctx, g, od, C.data(), report, m.eps.data());
+    }
+    if (image) {
+      // the plane's row and column by charge neutrality: it holds minus the
+      // conductors' total charge
+      std::vector<double> Cf((size_t)nc * nc, 0.0);
+      double all = 0.0;
+      for (int i = 0; i < nn; ++i)
+        for (int j = 0; j < nn; ++j) {
+          const double v = C[(size_t)i * nn + j];
+          Cf[(size_t)i * nc + j] = v;
+          Cf[(size_t)i * nc + nn] -= v;
+          Cf[(size_t)nn * nc + j] -= v;
+          all += v;
+        }
+      Cf[(size_t)nn * nc + nn] = all;
+      C.swap(Cf);
     }
     if (n_nets) *n_nets = (uint32_t)nn;
     for (int i = 0; i < nn; ++i) {

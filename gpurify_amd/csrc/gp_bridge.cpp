@@ -272,6 +272,8 @@ void mesh_groups(const gp_volume_group *groups, uint32_t n_groups, double xy_sca
   plate(quads, 0.5 * (b[0] + b[2]), 0.5 * (b[1] + b[3]), gz, lx, ly, gnx, gny);
   m.cond.insert(m.cond.end(), quads.size() - before, (uint32_t)m.nets.size());
   m.eps.insert(m.eps.end(), quads.size() - before, geps);
+  m.ground_z = gz;
+  m.n_plate = quads.size() - before;
   if (quads.size() > max_panels)
     throw Error(GP_ERR_PANEL_CAPACITY, "extruded layout needs " + std::to_string(quads.size()) +
                                            " panels; limit is " + std::to_string(max_panels));

@@ -14,6 +14,8 @@ struct Meshed {
   std::vector<double> verts;   // n * 12, metres
   std::vector<uint32_t> cond;
   std::vector<double> eps;     // surrounding dielectric per panel
+  double ground_z = 0.0;       // height of the substrate plate (bridge.rs:353), metres
+  size_t n_plate = 0;          // the plate's panels, the last ones
 };
 
 // Throws gp::Error with the BridgeError class of the failure.

@@ -577,6 +577,47 @@ __global__ __launch_bounds__(256) void k_l2p1(int n, int N, const int *__restric
   y[p] += (double)acc;
 }
 
+// Single right-hand side L2P with the order fixed at compile time: one
+// thread per panel (neighbouring threads are the panels of one leaf and share
+// its locals' cache lines), the weights in registers, the same expression and
+// order as k_l2p1.  A leaf of a few panels keeps most lanes of the
+// wavefront-per-leaf form idle; this form keeps every lane on a panel.
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void k_l2p1r(int N, const int *__restrict__ panel_leaf,
+                                               const int *__restrict__ leaf_box, int leaf_box0,
+                                               const T *__restrict__ w1d, const T *__restrict__ Lc,
+                                               double *__restrict__ y,
+                                               const unsigned char *__restrict__ skip) {
+  constexpr int n = NF, nn = n * n;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int li = panel_leaf[p];
+  if (skip && skip[li]) return;
+  const T *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * n * nn;
+  const T *w = w1d + (size_t)p * 3 * n;
+  T wx[n], wy[n], wz[n];
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    wx[i] = w[i];
+    wy[i] = w[n + i];
+    wz[i] = w[2 * n + i];
+  }
+  T acc = 0;
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    T sa = 0;
+#pragma unroll
+    for (int bb = 0; bb < n; ++bb) {
+      T sb = 0;
+#pragma unroll
+      for (int c = 0; c < n; ++c) sb += wz[c] * ll[a * nn + bb * n + c];
+      sa += wy[bb] * sb;
+    }
+    acc += wx[a] * sa;
+  }
+  y[p] += (double)acc;
+}
+
 // Single right-hand side L2P, order 4, FP32: one thread per panel, its
 // leaf's 64 local coefficients as 16 aligned 16-byte loads (the panels of a
 // leaf are neighbouring threads and share those lines), the tensor-product
@@ -2499,7 +2540,7 @@ __global__ __launch_bounds__(256) void k_interp_w1d(int N, int n,
     }
     const double w = 2.0 * inv_n * sum;
     const size_t o = (size_t)e * n + m;
-    w64[o] = w;
+    if (w64) w64[o] = w;
     if (w32) w32[o] = (float)w;
   }
 }
@@ -2982,6 +3023,19 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     if constexpr (sizeof(T) == 4)
       hipLaunchKernelGGL(k_l2p1v4, dim3((f.n_panels + 255) / 256), dim3(256), 0, s, f.n_panels,
                          f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf);
+  } else if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf && n >= 2 && n <= 6) {
+    const dim3 grid((f.n_panels + 255) / 256);
+#define GP_L2P1R(NF)                                                                              \
+  hipLaunchKernelGGL((k_l2p1r<T, NF>), grid, dim3(256), 0, s, f.n_panels, f.panel_leaf,          \
+                     f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf)
+    switch (n) {
+      case 2: GP_L2P1R(2); break;
+      case 3: GP_L2P1R(3); break;
+      case 4: GP_L2P1R(4); break;
+      case 5: GP_L2P1R(5); break;
+      default: GP_L2P1R(6); break;
+    }
+#undef GP_L2P1R
   } else if (ldm == 1 && nr == 1 && ldy == 1 && n3 <= 64)
     hipLaunchKernelGGL((k_l2p1w<T>), dim3((nleaf + 3) / 4), dim3(256), 0, s, n, nleaf, f.leaf_lo,
                        f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf);

@@ -258,12 +258,13 @@ __global__ __launch_bounds__(256) void k_axpy(double *__restrict__ x, const doub
 // Unit-conductor right-hand sides (rhs_unit_conductor, cap/solver.rs:55-63):
 // b[p][r] = 1 where panel p belongs to conductor j0 + r, else 0 (all ld columns).
 __global__ __launch_bounds__(256) void k_unit_rhs(const uint32_t *__restrict__ cond, int j0, int N,
-                                                  int ld, int nr, double *__restrict__ b) {
+                                                  int ld, int nr, const double *__restrict__ sign,
+                                                  double *__restrict__ b) {
   size_t total = (size_t)N * ld;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (size_t)gridDim.x * blockDim.x) {
     int row = (int)(e / ld), r = (int)(e % ld);
-    b[e] = (r < nr && (int)cond[row] == j0 + r) ? 1.0 : 0.0;
+    b[e] = (r < nr && (int)cond[row] == j0 + r) ? (sign ? sign[row] : 1.0) : 0.0;
   }
 }
 
@@ -509,14 +510,15 @@ __global__ __launch_bounds__(256) void k_seg_update_x(double *__restrict__ x,
   }
 }
 
-// b[row] = 1 where the panel's (batch-global) conductor is system p's
-// right-hand-side conductor, else 0.
+// b[row] = 1 (sign[row] when given) where the panel's (batch-global)
+// conductor is system p's right-hand-side conductor, else 0.
 __global__ __launch_bounds__(256) void k_seg_unit_rhs(const uint32_t *__restrict__ cond, int N,
                                                       const int *__restrict__ seg,
                                                       const int *__restrict__ rhs_cond,
+                                                      const double *__restrict__ sign,
                                                       double *__restrict__ b) {
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < N; row += gridDim.x * blockDim.x)
-    b[row] = ((int)cond[row] == rhs_cond[seg[row]]) ? 1.0 : 0.0;
+    b[row] = ((int)cond[row] == rhs_cond[seg[row]]) ? (sign ? sign[row] : 1.0) : 0.0;
 }
 
 namespace {
@@ -563,9 +565,9 @@ void seg_update_x(const SegMap &sm, double *x, const TV *V, size_t strideV, cons
   GP_LAUNCH_CHECK();
 }
 void seg_unit_rhs(const SegMap &sm, const uint32_t *cond, const int *rhs_cond, double *b,
-                  hipStream_t s) {
+                  hipStream_t s, const double *sign) {
   hipLaunchKernelGGL(k_seg_unit_rhs, dim3(grid_for(sm.N)), dim3(256), 0, s, cond, sm.N, sm.seg,
-                     rhs_cond, b);
+                     rhs_cond, sign, b);
   GP_LAUNCH_CHECK();
 }
 #define GP_SEG_BASIS(TV)                                                                        \
@@ -713,9 +715,10 @@ void kr_skip_flags(int nb, int nl, const int *box_prob, const int *leaf_box, con
   GP_LAUNCH_CHECK();
 }
 
-void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s) {
+void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s,
+                 const double *sign) {
   hipLaunchKernelGGL(k_unit_rhs, dim3(grid_for((size_t)N * ld)), dim3(256), 0, s, cond, j0, N, ld,
-                     nr, b);
+                     nr, sign, b);
   GP_LAUNCH_CHECK();
 }
 

@@ -235,15 +235,37 @@ __device__ void gauss4_t(const T *v, T xg[4][3], T wg[4]) {
   }
 }
 
-// P'[k][l] = I_l(c_k), cap/solver.rs:81-97 (row k = collocation at centroid k).
+// Integral over the mirror image of panel v in the plane z = zm2 / 2 (its
+// vertices' z reflected to zm2 - z; the integral does not depend on the
+// vertex orientation the reflection reverses).
+__device__ double wilton_image(const double *__restrict__ v, double zm2, double px, double py,
+                               double pz) {
+  double q[12];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    q[3 * k] = v[3 * k];
+    q[3 * k + 1] = v[3 * k + 1];
+    q[3 * k + 2] = zm2 - v[3 * k + 2];
+  }
+  return wilton(q, px, py, pz);
+}
+
+// P'[k][l] = I_l(c_k), cap/solver.rs:81-97 (row k = collocation at centroid
+// k); with a ground plane (image != 0, the plane z = zm2 / 2 in the unit
+// frame) P'[k][l] = I_l(c_k) - I_{l*}(c_k), l* the mirror image of panel l.
 __global__ __launch_bounds__(256) void k_dense_assemble(int n, const double *__restrict__ verts,
                                                         const double *__restrict__ cen,
-                                                        double *__restrict__ P) {
+                                                        double *__restrict__ P, int image,
+                                                        double zm2) {
   long total = (long)n * n;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
     int k = (int)(e / n), l = (int)(e % n);
-    P[e] = wilton(verts + 12L * l, cen[3 * k], cen[3 * k + 1], cen[3 * k + 2]);
+    const double *v = verts + 12L * l;
+    const double px = cen[3 * k], py = cen[3 * k + 1], pz = cen[3 * k + 2];
+    double pv = wilton(v, px, py, pz);
+    if (image) pv -= wilton_image(v, zm2, px, py, pz);
+    P[e] = pv;
   }
 }
 
@@ -390,7 +412,8 @@ __global__ __launch_bounds__(256) void k_dense_assemble_batched(const int *__res
                                                                 const long *__restrict__ a_off,
                                                                 const double *__restrict__ verts,
                                                                 const double *__restrict__ cen,
-                                                                double *__restrict__ P) {
+                                                                double *__restrict__ P,
+                                                                const double *__restrict__ zm2) {
   const int p = blockIdx.y, np = n[p];
   const double *v = verts + 12L * n_off[p], *c = cen + 3L * n_off[p];
   double *Pp = P + a_off[p];
@@ -398,7 +421,10 @@ __global__ __launch_bounds__(256) void k_dense_assemble_batched(const int *__res
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
     const int k = (int)(e / np), l = (int)(e % np);
-    Pp[e] = wilton(v + 12L * l, c[3 * k], c[3 * k + 1], c[3 * k + 2]);
+    const double px = c[3 * k], py = c[3 * k + 1], pz = c[3 * k + 2];
+    double pv = wilton(v + 12L * l, px, py, pz);
+    if (zm2) pv -= wilton_image(v + 12L * l, zm2[p], px, py, pz);
+    Pp[e] = pv;
   }
 }
 
@@ -425,16 +451,17 @@ void launch_transpose(int n, const double *A, double *AT, hipStream_t s) {
 void launch_dense_assemble_batched(const DenseBatch &b, hipStream_t s) {
   ProfScope ps("dense_assemble", s, 0.0, 0.0);
   hipLaunchKernelGGL(k_dense_assemble_batched, dim3(64, b.count), dim3(256), 0, s, b.n, b.n_off,
-                     b.a_off, b.verts, b.cen, b.A);
+                     b.a_off, b.verts, b.cen, b.A, b.zm2);
   GP_LAUNCH_CHECK();
 }
 
 void launch_dense_assemble(int n, const double *verts, const double *cen, double *P,
-                           hipStream_t s) {
+                           hipStream_t s, bool image, double zm2) {
   long total = (long)n * n;
   int blocks = (int)std::min<long>((total + 255) / 256, 65536);
   ProfScope ps("dense_assemble", s, 0.0, 8.0 * total);
-  hipLaunchKernelGGL(k_dense_assemble, dim3(blocks), dim3(256), 0, s, n, verts, cen, P);
+  hipLaunchKernelGGL(k_dense_assemble, dim3(blocks), dim3(256), 0, s, n, verts, cen, P,
+                     image ? 1 : 0, zm2);
   GP_LAUNCH_CHECK();
 }
 

@@ -14,8 +14,10 @@ namespace gp {
 // wrote.
 void launch_dielectric_rows(int n, int nd, const int *drow, const double *nrm, const double *diag,
                             const double *verts, const double *cen, double *P, hipStream_t s);
+// P'[k][l] = I_l(c_k) over n panels; image: minus the integral over panel
+// l's mirror image in the plane z = zm2 / 2 (a grounded plane by images).
 void launch_dense_assemble(int n, const double *verts, const double *cen, double *P,
-                           hipStream_t s);
+                           hipStream_t s, bool image = false, double zm2 = 0.0);
 void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_off,
                           const int *col_off, const int *cols, const double *verts,
                           const double *cen, double *v64, float *v32, hipStream_t s);
@@ -73,6 +75,7 @@ struct DenseBatch {
   const double *verts = nullptr, *cen = nullptr;
   double *A = nullptr, *B = nullptr, *work = nullptr;
   int *piv = nullptr, *info = nullptr;
+  const double *zm2 = nullptr;  // per problem 2 z of its ground plane (unit frame); null: none
 };
 void launch_dense_assemble_batched(const DenseBatch &b, hipStream_t s);
 void launch_dense_batch(const DenseBatch &b, hipStream_t s);
@@ -224,7 +227,10 @@ template <typename TO>
 void kr_scale_cols(const double *in, TO *out, const double *scale, const int *mask, int N,
                    int ld, int nr, hipStream_t s);
 void kr_axpy(double *x, const double *y, double alpha, int N, int ld, int nr, hipStream_t s);
-void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s);
+// b = unit right-hand sides of conductors j0 .. j0 + nr - 1 (sign[row] in
+// place of 1 when given: a signed panel set).
+void kr_unit_rhs(const uint32_t *cond, int j0, int N, int ld, int nr, double *b, hipStream_t s,
+                 const double *sign = nullptr);
 // skip_box[b] = !active[box_prob[b]] over nb boxes, skip_leaf[T] likewise
 // for the nl leaves (leaf T is box leaf_box[T]).
 void kr_skip_flags(int nb, int nl, const int *box_prob, const int *leaf_box, const int *active,
@@ -269,7 +275,7 @@ template <typename TV>
 void seg_update_x(const SegMap &sm, double *x, const TV *V, size_t strideV, const double *Y,
                   const int *kused, int kmax, const double *dinv, hipStream_t s);
 void seg_unit_rhs(const SegMap &sm, const uint32_t *cond, const int *rhs_cond, double *b,
-                  hipStream_t s);
+                  hipStream_t s, const double *sign = nullptr);
 
 template <typename TV>
 void kr_gmres_update_x(double *x, const TV *V, size_t strideV, const double *Y,

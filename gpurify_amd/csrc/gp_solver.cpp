@@ -173,13 +173,62 @@ void cond_csr(const PanelGeom &g, const uint32_t *perm, std::vector<int> &off,
   for (int i = 0; i < g.n; ++i) idx[fill[g.cond[i]]++] = pos[i];
 }
 
+// ------------------------------------------------------------------ ground
+// A grounded plane z = zg by the method of images (GP_GROUND_IMAGE; the
+// reference meshes a finite substrate plate instead, bridge.rs:346-379):
+// every vertex must lie above the plane.
+void check_ground(const PanelGeom &g, double zg) {
+  if (!std::isfinite(zg)) throw Error(GP_ERR_INVALID_INPUT, "ground_z must be finite");
+  for (int i = 0; i < g.n; ++i)
+    for (int k = 0; k < 4; ++k)
+      if (!(g.verts[(size_t)i * 12 + 3 * k + 2] > zg))
+        throw Error(GP_ERR_INVALID_INPUT, "panel " + std::to_string(i) +
+                                              " has a vertex on or below the ground plane");
+}
+
+// The panels followed by their mirror images in z = zg (same conductors,
+// sign -1): the image system as a signed panel set of 2n panels.
+struct Mirrored {
+  std::vector<double> v, sign;
+  std::vector<uint32_t> c;
+  PanelGeom g;
+};
+void mirror_geom(const PanelGeom &g, double zg, Mirrored &m) {
+  const int n = g.n;
+  m.v.resize((size_t)n * 24);
+  std::copy(g.verts, g.verts + (size_t)n * 12, m.v.begin());
+  m.c.assign(g.cond, g.cond + n);
+  m.c.insert(m.c.end(), g.cond, g.cond + n);
+  m.sign.assign((size_t)2 * n, 1.0);
+  std::fill(m.sign.begin() + n, m.sign.end(), -1.0);
+  m.g = PanelGeom();
+  m.g.n = 2 * n;
+  m.g.ncond = g.ncond;
+  m.g.cx = g.cx, m.g.cy = g.cy, m.g.cz = g.cz, m.g.area = g.area;
+  m.g.cx.resize(2 * n), m.g.cy.resize(2 * n), m.g.cz.resize(2 * n), m.g.area.resize(2 * n);
+  parallel_for(n, [&](int i) {
+    double *q = &m.v[(size_t)(n + i) * 12];
+    for (int k = 0; k < 4; ++k) {
+      q[3 * k] = g.verts[(size_t)i * 12 + 3 * k];
+      q[3 * k + 1] = g.verts[(size_t)i * 12 + 3 * k + 1];
+      q[3 * k + 2] = 2.0 * zg - g.verts[(size_t)i * 12 + 3 * k + 2];
+    }
+    double c[3], a;
+    panel_props(q, c, a);
+    m.g.cx[n + i] = c[0], m.g.cy[n + i] = c[1], m.g.cz[n + i] = c[2], m.g.area[n + i] = a;
+  });
+  m.g.verts = m.v.data();
+  m.g.cond = m.c.data();
+  m.g.sign = m.sign.data();
+}
+
 // ------------------------------------------------------------------ dense
 // Above this size the one-work-group LU is slower than the host: dense
 // systems then factor with rocSOLVER (solve_dense_roles).
 constexpr int kOneGroupLU = 1024;
 
 void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
-                 const double *eps_panel) {
+                 const double *eps_panel, const double *ground_z = nullptr) {
   if (g.n > kOneGroupLU) {
     std::vector<gp_panel_role> roles(g.n);
     for (int i = 0; i < g.n; ++i) {
@@ -188,7 +237,7 @@ void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
       roles[i].conductor = g.cond[i];
       roles[i].eps_surrounding = eps_panel ? eps_panel[i] : 1.0;
     }
-    solve_dense_roles(ctx, g, roles.data(), C, rep);
+    solve_dense_roles(ctx, g, roles.data(), C, rep, ground_z);
     return;
   }
   hipStream_t s = ctx->stream;
@@ -214,7 +263,8 @@ void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
   auto d_cidx = to_device(cidx, s);
   DBuf<double> d_P((size_t)n * n), d_work((size_t)n * nc), d_C((size_t)nc * nc);
   DBuf<int> d_piv(n), d_info(1);
-  launch_dense_assemble(n, d_v.p, d_c.p, d_P.p, s);
+  launch_dense_assemble(n, d_v.p, d_c.p, d_P.p, s, ground_z != nullptr,
+                        ground_z ? 2.0 * (*ground_z - c[2]) / L0 : 0.0);
   GP_HIP(hipStreamSynchronize(s));
   double t1 = now_s();
   launch_lu_solve(n, nc, d_P.p, d_piv.p, d_B.p, d_work.p, d_info.p, s);
@@ -241,7 +291,8 @@ void solve_dense(gp_ctx *ctx, const PanelGeom &g, double *C, gp_cap_report *rep,
 // one work-group per problem), then C per problem on the host in the
 // reference's summation order.
 void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std::vector<int> &idx,
-                       const std::vector<double *> &C, gp_cap_report *reps) {
+                       const std::vector<double *> &C, gp_cap_report *reps,
+                       const double *ground_z = nullptr) {
   hipStream_t s = ctx->stream;
   const double t0 = now_s();
   const int B = (int)idx.size();
@@ -258,10 +309,12 @@ void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std:
   }
   std::vector<double> hv((size_t)hnoff[B] * 12), hc((size_t)hnoff[B] * 3), hB(hboff[B], 0.0);
   std::vector<std::vector<double>> area(B);
+  std::vector<double> hzm2(B, 0.0);
   for (int q = 0; q < B; ++q) {
     const PanelGeom &g = gs[idx[q]];
     double c[3];
     root_cube(g, c, L0[q]);
+    if (ground_z) hzm2[q] = 2.0 * (*ground_z - c[2]) / L0[q];
     std::vector<double> v, ce;
     std::vector<uint32_t> cd;
     scaled_arrays(g, nullptr, c, L0[q], v, ce, area[q], cd);
@@ -277,6 +330,8 @@ void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std:
   auto d_v = upload_panels(hv, s);
   auto d_c = upload_panels(hc, s);
   auto d_B = to_device(hB, s);
+  DBuf<double> d_zm2;
+  if (ground_z) d_zm2 = to_device(hzm2, s);
   DBuf<double> d_A(haoff[B]), d_work(hboff[B]);
   DBuf<int> d_piv(hnoff[B]), d_info(B);
   DenseBatch db;
@@ -293,6 +348,7 @@ void solve_dense_batch(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const std:
   db.work = d_work.p;
   db.piv = d_piv.p;
   db.info = d_info.p;
+  db.zm2 = ground_z ? d_zm2.p : nullptr;
   launch_dense_batch(db, s);
   std::vector<int> info(B);
   GP_HIP(hipMemcpyAsync(info.data(), d_info.p, B * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -369,8 +425,13 @@ struct FmmSystem {
   bool m2l_compress = false;  // far-field M2L blocks hold the CPQR truncation
   bool skip_converged = true;  // a batch skips its converged problems' work
   int n3p = 0;
+  // Chebyshev order of the FP32 / BF16 inner operator's far field (n_in <= n)
+  int n_in = 0, n3p_in = 0;
   int m2l_inner = GP_M2L_BF16, m2l_outer = GP_M2L_FP64;
   DBuf<double> verts, cen, area, inv_diag;
+  // Signed panel sets (PanelGeom::sign): the sign per sorted panel and the
+  // charge weights 1/2 sign area; null otherwise (weights = area).
+  DBuf<double> sgn, qw;
   DBuf<uint32_t> cond;
   DBuf<int> leaf_lo, leaf_box, col_off, cols, panel_leaf;
   DBuf<int64_t> nf_off;
@@ -438,6 +499,9 @@ struct FmmSystem {
     if (n < 2 || n > 6) throw Error(GP_ERR_INVALID_INPUT, "fmm_order must be in [2, 6]");
     m2l_inner = (int)o.m2l_inner;
     m2l_outer = (int)o.m2l_outer;
+    if (o.inner_order != 0 && (o.inner_order < 2 || (int)o.inner_order > n))
+      throw Error(GP_ERR_INVALID_INPUT, "inner_order must be 0 or in [2, fmm_order]");
+    n_in = o.inner_order ? (int)o.inner_order : n;
     if (m2l_inner != GP_M2L_BF16 && m2l_inner != GP_M2L_FP32 && m2l_inner != GP_M2L_BF16_VALU)
       throw Error(GP_ERR_INVALID_INPUT, "m2l_inner must be GP_M2L_BF16, _FP32 or _BF16_VALU");
     if (m2l_outer != GP_M2L_FP64 && m2l_outer != GP_M2L_FP64_VALU)
@@ -558,6 +622,21 @@ struct FmmSystem {
     cen = upload_panels(hc.get(), (size_t)N * 3, s);
     area = upload_panels(ha.get(), (size_t)N, s);
     cond = upload_panels(hcond.get(), (size_t)N, s);
+    bool any_sign = false;
+    for (int p = 0; p < P; ++p) any_sign |= probs[p]->sign != nullptr;
+    if (any_sign) {
+      std::vector<double> hs(N, 1.0), hq(N);
+      for (int p = 0; p < P; ++p) {
+        const double *sg = probs[p]->sign;
+        for (int k = 0; k < probs[p]->n; ++k) {
+          const size_t q = (size_t)panel_off[p] + k;
+          hs[q] = sg ? sg[trees[p].perm[k]] : 1.0;
+          hq[q] = sg ? 0.5 * hs[q] * ha[q] : ha[q];
+        }
+      }
+      sgn = to_device(hs, s);
+      qw = to_device(hq, s);
+    }
     hv.reset();
     hc.reset();
 
@@ -735,11 +814,12 @@ struct FmmSystem {
     // FMM tables.
     const int n3 = n * n * n;
     n3p = round_up(n3, 32);
+    n3p_in = round_up(n_in * n_in * n_in, 32);
     kr = 0;  // low-rank inner M2L: rank 32, 64 or 128 below n3p (blocks of RHS only:
              // single-RHS solves run plain BF16 target tiles, too coarse for it)
     if (need32 && o.m2l_rank > 0 && !want_by_offset) {
       kr = o.m2l_rank <= 32 ? 32 : o.m2l_rank <= 64 ? 64 : 128;
-      if (kr >= n3p) kr = 0;
+      if (kr >= n3p_in) kr = 0;
     }
     auto nodes = cheb_nodes(n);
     // Leaf boxes in each problem's unit frame (centre and half-width, exact
@@ -755,19 +835,24 @@ struct FmmSystem {
       hgeo[(size_t)T * 4 + 3] = hs;
     });
     // M2M 1-D matrices, bbfmm.rs:89-112: mat[m*n + k] = S_n(cc + x_k / 2, x_m).
-    std::vector<double> hm2m((size_t)24 * n * n);
-    for (int oz = 0; oz < 2; ++oz)
-      for (int oy = 0; oy < 2; ++oy)
-        for (int ox = 0; ox < 2; ++ox) {
-          int o = ox + 2 * oy + 4 * oz;
-          double cc[3] = {(2.0 * ox - 1.0) * 0.5, (2.0 * oy - 1.0) * 0.5, (2.0 * oz - 1.0) * 0.5};
-          for (int axis = 0; axis < 3; ++axis)
-            for (int k = 0; k < n; ++k) {
-              double w[32];
-              interp_weights(cc[axis] + 0.5 * nodes[k], n, nodes.data(), w);
-              for (int m = 0; m < n; ++m) hm2m[(size_t)(o * 3 + axis) * n * n + m * n + k] = w[m];
-            }
-        }
+    auto m2m_table = [](int nn, const std::vector<double> &nd) {
+      std::vector<double> t((size_t)24 * nn * nn);
+      for (int oz = 0; oz < 2; ++oz)
+        for (int oy = 0; oy < 2; ++oy)
+          for (int ox = 0; ox < 2; ++ox) {
+            int o = ox + 2 * oy + 4 * oz;
+            double cc[3] = {(2.0 * ox - 1.0) * 0.5, (2.0 * oy - 1.0) * 0.5,
+                            (2.0 * oz - 1.0) * 0.5};
+            for (int axis = 0; axis < 3; ++axis)
+              for (int k = 0; k < nn; ++k) {
+                double w[32];
+                interp_weights(cc[axis] + 0.5 * nd[k], nn, nd.data(), w);
+                for (int m = 0; m < nn; ++m) t[(size_t)(o * 3 + axis) * nn * nn + m * nn + k] = w[m];
+              }
+          }
+      return t;
+    };
+    const std::vector<double> hm2m = m2m_table(n, nodes);
     // M2L matrices on the reference box (bbfmm.rs:156-166), each far-field
     // block truncated by the reference's CPQR (bbfmm.rs:132-174) unless
     // m2l_compress is 0.  The transposed operator of the adjoint solves
@@ -777,6 +862,13 @@ struct FmmSystem {
     skip_converged = o.skip_converged != 0;
     const auto m2l_tab = m2l_tables(n, n3p, m2l_compress);
     const std::vector<double> &hG = *m2l_tab;
+    // the inner operator's tables at its own order (the same tables when
+    // n_in == n)
+    const auto nodes_in = n_in == n ? nodes : cheb_nodes(n_in);
+    const std::vector<double> hm2m_in = n_in == n ? hm2m : m2m_table(n_in, nodes_in);
+    const auto m2l_tab_in =
+        (need32 && n_in != n) ? m2l_tables(n_in, n3p_in, m2l_compress) : m2l_tab;
+    const std::vector<double> &hGi = *m2l_tab_in;
     phase("tables");
     // Box arrays in the global numbering.
     std::vector<int> hlevel(nb), hparent(nb), hoct(nb), hchild_off(nb + 1, 0), hchild, htargets;
@@ -976,9 +1068,14 @@ struct FmmSystem {
     {
       DBuf<double> dgeo = to_device(hgeo, s), dnodes = to_device(nodes, s);
       w1d64.alloc((size_t)N * 3 * n);
-      if (need32) w1d32.alloc((size_t)N * 3 * n);
+      if (need32) w1d32.alloc((size_t)N * 3 * n_in);
       launch_interp_w1d(N, n, panel_leaf.p, dgeo.p, cen.p, dnodes.p, w1d64.p,
-                        need32 ? w1d32.p : nullptr, s);
+                        (need32 && n_in == n) ? w1d32.p : nullptr, s);
+      DBuf<double> dnodes_in;
+      if (need32 && n_in != n) {
+        dnodes_in = to_device(nodes_in, s);
+        launch_interp_w1d(N, n_in, panel_leaf.p, dgeo.p, cen.p, dnodes_in.p, nullptr, w1d32.p, s);
+      }
       if (diel) {
         dw1d64.alloc((size_t)N * 3 * n);
         launch_interp_dw1d(N, n, panel_leaf.p, dgeo.p, cen.p, dnodes.p, dw1d64.p, s);
@@ -988,21 +1085,21 @@ struct FmmSystem {
     m2m64 = to_device(hm2m, s);
     G64 = to_device(hG, s);
     if (need32) {
-      m2m32 = to_device(cast_vec<float>(hm2m), s);
-      G32 = to_device(cast_vec<float>(hG), s);
-      std::vector<uint16_t> hb(hG.size());
-      for (size_t e = 0; e < hG.size(); ++e) hb[e] = bf16_bits((float)hG[e]);
+      m2m32 = to_device(cast_vec<float>(hm2m_in), s);
+      G32 = to_device(cast_vec<float>(hGi), s);
+      std::vector<uint16_t> hb(hGi.size());
+      for (size_t e = 0; e < hGi.size(); ++e) hb[e] = bf16_bits((float)hGi[e]);
       Gbf = to_device(hb, s);
       if (m2l_inner == GP_M2L_BF16_VALU) {
-        std::vector<float> hr(hG.size());
-        for (size_t e = 0; e < hG.size(); ++e) {
+        std::vector<float> hr(hGi.size());
+        for (size_t e = 0; e < hGi.size(); ++e) {
           const uint32_t u = (uint32_t)hb[e] << 16;
           std::memcpy(&hr[e], &u, 4);
         }
         Gr32 = to_device(hr, s);
       }
       if (kr) {
-        auto lr = low_rank_tables(n, n3p, kr, m2l_compress, hG);
+        auto lr = low_rank_tables(n_in, n3p_in, kr, m2l_compress, hGi);
         U32 = to_device(cast_vec<float>(lr->U), s);
         Gc32 = to_device(cast_vec<float>(lr->Gc), s);
         std::vector<uint16_t> hcb(lr->Gc.size()), hcl(lr->Gc.size());
@@ -1075,6 +1172,8 @@ struct FmmSystem {
     f64.m2m1d = m2m64.p;
     f64.G = G64.p;
     fill(f32);
+    f32.n = n_in;
+    f32.n3p = n3p_in;
     f32.w1d = w1d32.p;
     f32.m2m1d = m2m32.p;
     f32.G = G32.p;
@@ -1767,10 +1866,10 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   // each panel for the conductor-role dielectric solve (free charge
   // eps_k * sigma_k * a_k, cap/dielectric.rs:141-149).
   DBuf<double> wq;
-  const double *w = sys.area.p;
+  const double *w = sys.qw.p ? sys.qw.p : sys.area.p;
   if (eps_panel) {
     std::vector<double> ha(sys.N);
-    GP_HIP(hipMemcpyAsync(ha.data(), sys.area.p, sys.N * sizeof(double), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipMemcpyAsync(ha.data(), w, sys.N * sizeof(double), hipMemcpyDeviceToHost, s));
     GP_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < sys.N; ++k) ha[k] *= eps_panel[sys.t().perm[k]];
     wq = to_device(ha, s);
@@ -1781,7 +1880,7 @@ void solve_fmm(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
   bool conv_all = true;
   for (int j0 = jlo; j0 < jhi; j0 += blk) {
     const int nr = std::min(blk, jhi - j0);
-    kr_unit_rhs(sys.cond.p, j0, sys.N, ld, nr, b.p, s);
+    kr_unit_rhs(sys.cond.p, j0, sys.N, ld, nr, b.p, s, sys.sgn.p);
     x.zero(s);
     BlockOps ops;
     ops.N = sys.N;
@@ -1962,11 +2061,12 @@ void solve_group(gp_ctx *ctx, FmmSystem &sys, const std::vector<const PanelGeom 
       hr[p] = live[p] ? sys.cond_off[p] + j : -1;
     }
     GP_HIP(hipMemcpyAsync(drhs.p, hr.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
-    seg_unit_rhs(ops.sm, sys.cond.p, drhs.p, b.p, s);
+    seg_unit_rhs(ops.sm, sys.cond.p, drhs.p, b.p, s, sys.sgn.p);
     x.zero(s);
     SolveStats st;
     solve_systems(sys, ops, o, mixed, b.p, x.p, live, st);
-    launch_cap_accumulate(sys.ncond, 1, 1, sys.coff.p, sys.cidx.p, sys.area.p, x.p, dC.p, s);
+    launch_cap_accumulate(sys.ncond, 1, 1, sys.coff.p, sys.cidx.p,
+                          sys.qw.p ? sys.qw.p : sys.area.p, x.p, dC.p, s);
     GP_HIP(hipMemcpyAsync(hdC.data(), dC.p, sys.ncond * sizeof(double), hipMemcpyDeviceToHost, s));
     GP_HIP(hipStreamSynchronize(s));
     for (int p = 0; p < P; ++p) {
@@ -2102,7 +2202,7 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
 // -F(l, c_k, n_k) off the diagonal (scale-free) and
 // 2 pi (eps_out + eps_in) / (eps_out - eps_in) on it.
 void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles, double *C,
-                       gp_cap_report *rep) {
+                       gp_cap_report *rep, const double *ground_z) {
   hipStream_t s = ctx->stream;
   double t0 = now_s();
   const int n = g.n, nc = g.ncond;
@@ -2153,10 +2253,13 @@ void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *rol
     for (int k = 0; k < n; ++k)
       if (roles[k].kind == GP_ROLE_CONDUCTOR) cidx[fill[roles[k].conductor]++] = k;
   }
+  if (ground_z && !drow.empty())
+    throw Error(GP_ERR_INVALID_INPUT, "the ground plane by images takes conductor panels only");
   auto d_v = to_device(hv, s);
   auto d_c = to_device(hc, s);
   DBuf<double> d_P((size_t)n * n);
-  launch_dense_assemble(n, d_v.p, d_c.p, d_P.p, s);
+  launch_dense_assemble(n, d_v.p, d_c.p, d_P.p, s, ground_z != nullptr,
+                        ground_z ? 2.0 * (*ground_z - c[2]) / L0 : 0.0);
   if (!drow.empty()) {
     auto d_dr = to_device(drow, s);
     auto d_n = to_device(hn, s);
@@ -2323,12 +2426,29 @@ void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &o, double *C,
     rep->n_panels = (uint32_t)g.n;
     rep->n_conductors = (uint32_t)g.ncond;
   }
-  if (method == GP_METHOD_DENSE)
-    solve_dense(ctx, g, C, rep, eps_panel);
-  else if (method == GP_METHOD_FMM)
-    solve_fmm(ctx, g, o, C, rep, eps_panel);
-  else
+  if (o.ground != GP_GROUND_NONE && o.ground != GP_GROUND_IMAGE)
+    throw Error(GP_ERR_INVALID_INPUT, "unknown ground model");
+  const bool image = o.ground == GP_GROUND_IMAGE;
+  if (image) check_ground(g, o.ground_z);
+  if (method == GP_METHOD_DENSE) {
+    solve_dense(ctx, g, C, rep, eps_panel, image ? &o.ground_z : nullptr);
+  } else if (method == GP_METHOD_FMM) {
+    if (image) {
+      // the panels and their mirrors in one octree, antisymmetric RHS
+      Mirrored m;
+      mirror_geom(g, o.ground_z, m);
+      std::vector<double> eps2;
+      if (eps_panel) {
+        eps2.assign(eps_panel, eps_panel + g.n);
+        eps2.insert(eps2.end(), eps_panel, eps_panel + g.n);
+      }
+      solve_fmm(ctx, m.g, o, C, rep, eps_panel ? eps2.data() : nullptr);
+    } else {
+      solve_fmm(ctx, g, o, C, rep, eps_panel);
+    }
+  } else {
     throw Error(GP_ERR_INVALID_INPUT, "unknown method");
+  }
   for (size_t e = 0; e < (size_t)g.ncond * g.ncond; ++e)
     if (!std::isfinite(C[e])) throw Error(GP_ERR_NON_FINITE, "solver returned a non-finite capacitance");
   if (rep) {
@@ -2582,11 +2702,19 @@ void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_c
     throw Error(GP_ERR_INVALID_INPUT, "unknown precision");
   if (o.method != GP_METHOD_AUTO && o.method != GP_METHOD_DENSE && o.method != GP_METHOD_FMM)
     throw Error(GP_ERR_INVALID_INPUT, "unknown method");
+  if (o.ground != GP_GROUND_NONE && o.ground != GP_GROUND_IMAGE)
+    throw Error(GP_ERR_INVALID_INPUT, "unknown ground model");
+  const bool image = o.ground == GP_GROUND_IMAGE;
   const uint32_t limit = o.dense_limit ? o.dense_limit : 500u;
   std::vector<const PanelGeom *> fmm;
   std::vector<double *> Cf;
   std::vector<int> fidx, dprobs;
   const int P = (int)gs.size();
+  if (image)
+    for (const PanelGeom &g : gs) check_ground(g, o.ground_z);
+  // with a ground plane the FMM problems are their signed mirrored sets
+  std::vector<Mirrored> mir;
+  mir.reserve(P);
   if (reps) std::memset(reps, 0, sizeof(gp_cap_report) * P);
   for (int p = 0; p < P; ++p) {
     const PanelGeom &g = gs[p];
@@ -2598,6 +2726,12 @@ void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_c
                        (o.method == GP_METHOD_AUTO && (uint32_t)g.n < limit);
     if (dense) {
       dprobs.push_back(p);
+    } else if (image) {
+      mir.emplace_back();
+      mirror_geom(g, o.ground_z, mir.back());
+      fmm.push_back(&mir.back().g);
+      Cf.push_back(C[p]);
+      fidx.push_back(p);
     } else {
       fmm.push_back(&g);
       Cf.push_back(C[p]);
@@ -2605,7 +2739,7 @@ void cap_solve_batched(gp_ctx *ctx, const std::vector<PanelGeom> &gs, const gp_c
     }
   }
   auto dense = [&] {
-    if (!dprobs.empty()) solve_dense_batch(ctx, gs, dprobs, C, reps);
+    if (!dprobs.empty()) solve_dense_batch(ctx, gs, dprobs, C, reps, image ? &o.ground_z : nullptr);
   };
   if (fmm.empty()) {
     dense();

@@ -37,7 +37,7 @@ namespace gp {
 // same potential-coefficient matrix).
 // cap::solve_dielectric (cap/dielectric.rs:47-151), dense FP64 on the device.
 void solve_dense_roles(gp_ctx *ctx, const PanelGeom &g, const gp_panel_role *roles, double *C,
-                       gp_cap_report *rep);
+                       gp_cap_report *rep, const double *ground_z = nullptr);
 void cap_solve(gp_ctx *ctx, const PanelGeom &g, const gp_cap_opts &opts, double *C,
                gp_cap_report *rep, const double *eps_panel);
 

@@ -17,6 +17,10 @@ struct PanelGeom {
   const double *verts = nullptr;   // n * 12, metres
   std::vector<double> cx, cy, cz, area;
   const uint32_t *cond = nullptr;  // n
+  // Optional +1 / -1 per panel (caller order): the right-hand side of
+  // conductor j is sign on its panels and its charge 1/2 sum sign sigma a
+  // (a panel set extended by its mirror images, the mirrors at -1).
+  const double *sign = nullptr;
 };
 
 // Validates the boundary input and computes centroids/areas with the

@@ -159,7 +159,32 @@ typedef struct {
                             per-leaf load latency, not by the value bytes, and measured slower
                             with FP16 values).  Blocks of RHS always keep FP32 values; the outer
                             residual is FP64 either way. */
+  uint32_t inner_order;  /* GP_PREC_MIXED: Chebyshev order of the inner operator's far field
+                            (P2M, M2M, M2L, L2L, L2P of the FP32 / BF16 operator, on the same
+                            tree, near field and interaction lists as the FP64 operator, its
+                            M2L matrices truncated by the same CPQR rule at that order); 0
+                            (default) = fmm_order.  Must be in [2, fmm_order].  The FP64 outer
+                            residual keeps fmm_order, so the converged answer is that of the
+                            fmm_order operator whatever the inner order: a lower order only
+                            changes how far each inner solve moves the iterate (the refinement
+                            contraction is bounded by inner_tol plus the inner operator's
+                            relative error). */
+  uint32_t ground;       /* GP_GROUND_NONE (default): the panels as given in free space, the
+                            reference's model (its substrate is a meshed finite plate among the
+                            panels, bridge.rs:346-379).  GP_GROUND_IMAGE: a perfectly conducting
+                            infinite plane z = ground_z by the method of images, the Green's
+                            function 1/|x - y| - 1/|x - y*| with y* = (y.x, y.y, 2 ground_z -
+                            y.z); every vertex must lie above the plane; C is the conductors'
+                            Maxwell matrix with the plane as the grounded reference.  Dense
+                            path: each entry adds the mirror panel's integral; FMM path: the
+                            mirror panels join the octree as explicit panels (2N in all) with
+                            antisymmetric right-hand sides.  gp_solve_capacitance replaces its
+                            substrate plate by the plane at the plate's height (ground_z is then
+                            ignored).  Not available for the dielectric solves and
+                            gp_cap_jacobian. */
+  double ground_z;       /* GP_GROUND_IMAGE: the plane's height, metres */
 } gp_cap_opts;
+enum { GP_GROUND_NONE = 0, GP_GROUND_IMAGE = 1 };
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
 
 void gp_cap_opts_default(gp_cap_opts *opts);

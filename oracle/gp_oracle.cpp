@@ -1149,6 +1149,101 @@ int cap_fmm(const PanelSet &s, int ncond, int order, int max_level, double tol, 
   return 0;
 }
 
+// ---------------------------------------------------------------- ground plane
+// (No reference counterpart: the reference models the substrate as a meshed
+// finite plate, bridge.rs:346-379, cap/mesh.rs:12-43.)  A perfectly
+// conducting infinite plane z = zg by the method of images: the Green's
+// function 1/|x - y| - 1/|x - y*| with y* = (y.x, y.y, 2 zg - y.z), so a panel
+// l contributes I_l(c_k) - I_{l*}(c_k), l* its mirror image (the integral is
+// independent of the vertex orientation the mirror reverses).
+Pan mirror_panel(const Pan &p, double zg) {
+  Pan m = p;
+  for (int v = 0; v < 4; ++v) m.v[v].z = 2.0 * zg - p.v[v].z;
+  return m;
+}
+
+bool above_plane(const PanelSet &s, double zg) {
+  for (const Pan &p : s.pans)
+    for (int v = 0; v < 4; ++v)
+      if (!(p.v[v].z > zg)) return false;
+  return true;
+}
+
+void assemble_p_image(const PanelSet &s, double zg, double *P) {
+  long n = s.n();
+  std::vector<Pan> mir(n);
+  for (long l = 0; l < n; ++l) mir[l] = mirror_panel(s.pans[l], zg);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long k = 0; k < n; ++k) {
+    V3 rk = s.c[k];
+    for (long l = 0; l < n; ++l)
+      P[k * n + l] = ONE_OVER_4PI_EPS0 * (potential_integral(s.pans[l].v, 4, rk) -
+                                          potential_integral(mir[l].v, 4, rk));
+  }
+}
+
+// Dense LU of the image system, one solve per conductor (the conductors'
+// Maxwell matrix with the plane as the grounded reference).
+int cap_dense_image(const PanelSet &s, int ncond, double zg, double *C) {
+  long n = s.n();
+  if (n == 0) return -6;
+  if (!above_plane(s, zg)) return -7;
+  std::vector<double> P((size_t)n * n);
+  assemble_p_image(s, zg, P.data());
+  std::fill(C, C + (size_t)ncond * ncond, 0.0);
+  LU lu;
+  if (!lu.factor(P.data(), n)) return -1;
+  std::vector<double> sig(n);
+  for (int j = 0; j < ncond; ++j) {
+    auto b = rhs_unit(s, j);
+    lu.solve(b.data(), sig.data());
+    accumulate_c(s, sig.data(), j, ncond, C, nullptr);
+  }
+  return 0;
+}
+
+// The FMM form: the mirror panels join the panel set as explicit sources and
+// targets (2N panels, mirrors after the originals, same conductor ids); the
+// right-hand side of conductor j is +1 on its panels and -1 on their mirrors,
+// whose solution is antisymmetric (sigma* = -sigma), so the first N rows are
+// the image system.  C_ij = 1/2 sum_{k in i} s_k sigma_k a_k over both halves
+// (s = +1 / -1).  Same Jacobi preconditioner and GMRES as cap_fmm.
+int cap_fmm_image(const PanelSet &s, int ncond, double zg, int order, int max_level, double tol,
+                  int m2l_mode, double *C, int *iters, double *resid) {
+  long n = s.n();
+  if (n == 0) return -6;
+  if (!above_plane(s, zg)) return -7;
+  PanelSet d;
+  d.pans = s.pans;
+  d.cond = s.cond;
+  for (long l = 0; l < n; ++l) d.pans.push_back(mirror_panel(s.pans[l], zg));
+  d.cond.insert(d.cond.end(), s.cond.begin(), s.cond.end());
+  for (const Pan &p : d.pans) {
+    d.c.push_back(centroid(p.v, 4));
+    d.area.push_back(area(p.v, 4));
+  }
+  FmmCapOp op;
+  op.init(d, order, max_level, m2l_mode);
+  DiagJacobi pre;
+  pre.inv.assign(2 * n, 1.0);
+  for (long i = 0; i < 2 * n; ++i) {
+    double dg = 1.0;
+    op.diagonal(i, dg);
+    pre.inv[i] = std::fabs(dg) > 1e-30 ? 1.0 / dg : 1.0;
+  }
+  std::fill(C, C + (size_t)ncond * ncond, 0.0);
+  for (int j = 0; j < ncond; ++j) {
+    auto b = rhs_unit(d, j);
+    for (long k = n; k < 2 * n; ++k) b[k] = -b[k];
+    auto r = gmres(op, b.data(), &pre, tol, 200, 50);
+    if (iters) iters[j] = r.iterations;
+    if (resid) resid[j] = r.residual;
+    for (long k = 0; k < 2 * n; ++k)
+      C[(size_t)d.cond[k] * ncond + j] += 0.5 * (k < n ? 1.0 : -1.0) * r.x[k] * d.area[k];
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------- meshers
 // cap/mesh.rs:12-43
 void plate(double cx, double cy, double z0, double lx, double ly, int nx, int ny,
@@ -1707,6 +1802,19 @@ int or_cap_solve_fmm(int n, const double *verts4, const uint32_t *cond, int ncon
                      double *resid_out) {
   PanelSet s = build_set(n, verts4, cond);
   return cap_fmm(s, ncond, order, max_level, tol, m2l_mode, C, iters_out, resid_out);
+}
+
+int or_cap_solve_dense_image(int n, const double *verts4, const uint32_t *cond, int ncond,
+                             double zg, double *C) {
+  PanelSet s = build_set(n, verts4, cond);
+  return cap_dense_image(s, ncond, zg, C);
+}
+
+int or_cap_solve_fmm_image(int n, const double *verts4, const uint32_t *cond, int ncond, double zg,
+                           int order, int max_level, double tol, int m2l_mode, double *C,
+                           int *iters_out, double *resid_out) {
+  PanelSet s = build_set(n, verts4, cond);
+  return cap_fmm_image(s, ncond, zg, order, max_level, tol, m2l_mode, C, iters_out, resid_out);
 }
 
 int or_cap_solve_fmm_cols(int n, const double *verts4, const uint32_t *cond, int ncond,

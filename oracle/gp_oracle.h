@@ -65,6 +65,16 @@ int or_cap_solve_fmm(int n, const double *verts4, const uint32_t *cond, int ncon
                      int order, int max_level, double tol, int m2l_mode,
                      double *C, int *iters_out, double *resid_out);
 
+/* Ground plane z = zg by images (no reference counterpart): dense LU
+ * (or_cap_solve_dense_image) and the FMM form over the panels and their
+ * mirrors with an antisymmetric right-hand side (or_cap_solve_fmm_image).
+ * C: ncond^2, farads, the conductors' Maxwell matrix with the plane as the
+ * grounded reference.  -7 when a vertex is not above the plane. */
+int or_cap_solve_dense_image(int n, const double *verts4, const uint32_t *cond, int ncond,
+                             double zg, double *C);
+int or_cap_solve_fmm_image(int n, const double *verts4, const uint32_t *cond, int ncond, double zg,
+                           int order, int max_level, double tol, int m2l_mode, double *C,
+                           int *iters_out, double *resid_out);
 /* As or_cap_solve_fmm for the conductor columns [j0, j0 + nj) only (the
  * bounded CPU sample).  seconds_out[0] = operator setup, [1] = GMRES. */
 int or_cap_solve_fmm_cols(int n, const double *verts4, const uint32_t *cond, int ncond,

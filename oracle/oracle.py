@@ -58,6 +58,9 @@ def lib():
         L.or_solve_dielectric.argtypes = [C.c_int, _d, _int, _u32, _d, _d, _d, _d, C.c_int, _d]
         L.or_cap_solve_fmm.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, C.c_int,
                                        C.c_double, C.c_int, _d, _int, _d]
+        L.or_cap_solve_dense_image.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_double, _d]
+        L.or_cap_solve_fmm_image.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_double, C.c_int,
+                                             C.c_int, C.c_double, C.c_int, _d, _int, _d]
         L.or_cap_solve_fmm_cols.argtypes = [C.c_int, _d, _u32, C.c_int, C.c_int, C.c_int,
                                             C.c_double, C.c_int, C.c_int, C.c_int, _d, _int, _d]
         L.or_fmm_cap_apply.argtypes = [C.c_int, _d, C.c_int, C.c_int, C.c_int, _d, _d]
@@ -222,6 +225,34 @@ def cap_fmm(verts4, cond, ncond, order=4, max_level=4, tol=1e-8, m2l_mode=0):
                                 _p(Cm), it.ctypes.data_as(_int), _p(rs))
     if rc:
         raise RuntimeError(f"oracle FMM solve failed ({rc})")
+    return Cm, it, rs
+
+
+def cap_dense_image(verts4, cond, ncond, zg):
+    """Conductors over a grounded plane z = zg by images, dense LU (no
+    reference counterpart): C in farads, the plane as the reference."""
+    v = _f64(verts4).reshape(-1, 4, 3)
+    c = np.ascontiguousarray(cond, dtype=np.uint32)
+    Cm = np.zeros((ncond, ncond))
+    rc = lib().or_cap_solve_dense_image(v.shape[0], _p(v), _p(c, _u32), ncond, float(zg), _p(Cm))
+    if rc:
+        raise RuntimeError(f"oracle image dense solve failed ({rc})")
+    return Cm
+
+
+def cap_fmm_image(verts4, cond, ncond, zg, order=4, max_level=4, tol=1e-8, m2l_mode=0):
+    """The image problem of cap_dense_image on the FMM operator of the panels
+    and their mirrors (antisymmetric right-hand sides): (C, iters, resid)."""
+    v = _f64(verts4).reshape(-1, 4, 3)
+    c = np.ascontiguousarray(cond, dtype=np.uint32)
+    Cm = np.zeros((ncond, ncond))
+    it = np.zeros(ncond, dtype=np.int32)
+    rs = np.zeros(ncond)
+    rc = lib().or_cap_solve_fmm_image(v.shape[0], _p(v), _p(c, _u32), ncond, float(zg), order,
+                                      max_level, tol, m2l_mode, _p(Cm), it.ctypes.data_as(_int),
+                                      _p(rs))
+    if rc:
+        raise RuntimeError(f"oracle image FMM solve failed ({rc})")
     return Cm, it, rs
 
 
