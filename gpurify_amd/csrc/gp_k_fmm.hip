@@ -2677,7 +2677,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     ProfScope ps(f32 ? "p2m_f32" : "p2m_f64", s, 2.0 * f.n_panels * n3 * nr * f.frac_panels,
                  ((double)f.n_panels * nr * sizeof(T) + nleaf * exp_bytes) * f.frac_panels);
     if (ldm == 1 && nr == 1 && ldz == 1) {
-      const int tpl = round_up(n3, 64), lpb = 256 / tpl;
+      // (orders <= 3: half-wavefront slots, two leaves to a wavefront)
+      const int tpl = round_up(n3, n3 <= 32 ? 32 : 64), lpb = 256 / tpl;
       // one thread per (leaf, node), or (variant 1) one work-group per leaf
       // with its panels staged in LDS
       tuned_launch(f32 ? "p2m1_f32" : "p2m1_f64", nleaf, n, 1, n >= 3 && n <= 6 ? 2 : 1, s,
@@ -2728,7 +2729,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nr * nch * f.frac_boxes,
                  (nb + nch) * exp_bytes * f.frac_boxes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
-      const int tpb = round_up(n3, 64), bpb = 256 / tpb;
+      // (orders <= 3: half-wavefront slots, two boxes to a wavefront)
+      const int tpb = round_up(n3, n3 <= 32 ? 32 : 64), bpb = 256 / tpb;
 #define GP_M2M1(NF)                                                                             \
   hipLaunchKernelGGL((k_m2m1<T, NF>), dim3((nb + bpb - 1) / bpb), dim3(256), 4096 * sizeof(T), s, \
                      n, n3p, tpb, f.lvl_off[l], nb, f.child_off, f.child_idx, f.octant, f.m2m1d,   \
@@ -2971,7 +2973,8 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nr * nb * f.frac_boxes,
                  2.0 * nb * exp_bytes * f.frac_boxes);
     if (ldm == 1 && nr == 1 && n3 <= 256) {  // orders <= 6: one slot fits a work-group
-      const int tpb = round_up(n3, 64), bpb = 256 / tpb;
+      // (orders <= 3: half-wavefront slots, two boxes to a wavefront)
+      const int tpb = round_up(n3, n3 <= 32 ? 32 : 64), bpb = 256 / tpb;
 #define GP_L2L1(NF)                                                                            \
   hipLaunchKernelGGL((k_l2l1<T, NF>), dim3((nb + bpb - 1) / bpb), dim3(256), 512 * sizeof(T), s, \
                      n, tpb, f.lvl_off[l + 1], nb, f.parent, f.octant, f.m2m1d, Lc, f.skip_box)

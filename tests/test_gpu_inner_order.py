@@ -66,6 +66,24 @@ def test_batched_inner_order_matches_fp64(ctx):
             assert np.max(np.abs(a[:, 0] - b[:, 0]) / np.abs(b[:, 0])) < 1e-6
 
 
+@pytest.mark.parametrize("inner,ncols", [(2, 1), (3, 1), (3, 20), (4, 1)])
+def test_inner_operator_is_the_lower_order_operator(ctx, inner, ncols):
+    """The FP32 inner operator of an order-5 system with inner_order k is
+    the order-k operator on the same tree (its tables, weights and CPQR
+    truncation at order k): it equals the FP64 order-k operator to FP32
+    rounding, one RHS (target tiles, packed tree passes) and blocks alike."""
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    X = np.random.default_rng(inner).normal(size=(len(v), ncols))
+    common = dict(method=G.GP_METHOD_FMM, max_level=4, m2l_inner=G.GP_M2L_FP32, m2l_rank=0)
+    Yk = ctx.fmm_apply(v, X, G.default_opts(fmm_order=inner, **common))
+    Yi = ctx.fmm_apply(v, X, G.default_opts(fmm_order=5, inner_order=inner, **common),
+                       precision=G.GP_PREC_MIXED)
+    Y5 = ctx.fmm_apply(v, X, G.default_opts(fmm_order=5, **common))
+    nrm = np.linalg.norm(Yk)
+    assert np.linalg.norm(Yi - Yk) / nrm < 3e-6
+    assert np.linalg.norm(Yi - Y5) / nrm > 1e-6  # (it is not the order-5 operator)
+
+
 def test_inner_order_is_validated(ctx):
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     for bad in (1, 5):

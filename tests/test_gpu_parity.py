@@ -213,7 +213,7 @@ def test_m2l_matrix_cores_match_valu(ctx):
 
 
 @pytest.mark.parametrize("order,ncols", [(4, 3), (4, 16), (4, 20), (4, 40), (4, 97), (3, 3),
-                                         (5, 3), (6, 3)])
+                                         (5, 3), (6, 3), (2, 3)])
 def test_fmm_single_and_block_rhs_agree(ctx, order, ncols):
     # block kernels (near-field tiles of 16 / 32 / 64 RHS columns, far field
     # over RHS blocks) against the single-column kernels (FP64 M2L over
