@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--refine", type=int, default=0, help="0 = the workload's own")
     ap.add_argument("--order", type=int, default=0, help="0 = the workload's own")
-    ap.add_argument("--leaf-target", type=int, default=64)
+    ap.add_argument("--leaf-target", type=int, default=0, help="0 = the workload's own (64)")
     ap.add_argument("--tol", type=float, default=0.0, help="0 = the workload's own")
     ap.add_argument("--inner-tol", type=float, default=1e-2)
     ap.add_argument("--inner-order", type=int, default=-1,
@@ -90,6 +90,7 @@ def parse():
     a.tol = a.tol or w.get("tol", 1e-10)
     if a.inner_order < 0:
         a.inner_order = w.get("inner_order", 0)
+    a.leaf_target = a.leaf_target or w.get("leaf_target", 64)
     return a
 
 
@@ -120,9 +121,11 @@ WORKLOADS = {
     # batch: 4096 independent nets, each with its own substrate plate, 1 RHS
     # the inner operator's far field at order 3 (the FP64 outer residual
     # keeps order 4): 8 % faster than an order-4 inner operator, the same
-    # converged answer (gate on every entry)
+    # converged answer (gate on every entry); leaves of ~96 panels: fewer
+    # M2L pairs for a longer near field (mixed 6.76M -> 7.51M unknowns/s
+    # against leaves of 64; 7.46M at 128, 7.03M at 160)
     "nets4096": {"groups": None, "refine": 1, "order": 4, "rhs": 1, "name": "nets4096_met1met2",
-                 "cpu": None, "inner_order": 3},
+                 "cpu": None, "inner_order": 3, "leaf_target": 96},
     # adjoint sensitivities of the 16 x 16 crossing bundle (W = S = 140 nm)
     "bundle16": {"groups": None, "refine": 5, "order": 5, "rhs": None, "name": "bundle16x16_met1met2",
                  "cpu": None},
@@ -334,7 +337,7 @@ def bench_batch(args):
                          "refine_steps_max": max(r["refine_steps"] for r in fm),
                          "converged": all(r["converged"] for r in reps), "global_batch": len(probs)}
         work = {"N": nfmm, "R": 1, "nnz": nnz, "pairs": pairs,
-                "n3": args.order ** 3}
+                "n3": args.order ** 3, "n3_inner": (args.inner_order or args.order) ** 3}
         # (counter summaries are per group count: the launches differ)
         report_profile(out, prof, work, f"nets4096_g{ngroups}")
         out["autotune"] = autotune_choices(ctx)
@@ -584,6 +587,8 @@ def roofline_of(name, prof, work, workload):
             t += prof[extra]["seconds"] * L / prof[extra]["launches"]
     avg = t / L
     N, R, nnz, pairs, n3 = work["N"], work["R"], work["nnz"], work["pairs"], work["n3"]
+    if "f64" not in name and "n3_inner" in work:  # the inner operator's M2L at its own order
+        n3 = work["n3_inner"]
     if name.startswith("near_sp"):
         s = {"f16": 2, "f32": 4}.get(name[-3:], 8)
         row = 16 if name.endswith("f16") else s + 8  # (FP16: the row scale,) x read, y written

@@ -325,23 +325,178 @@ __global__ __launch_bounds__(256) void k_dielectric_rows(int n, const int *__res
 // Near-field tile of target leaf T: rows = T's panels, columns = the panels of
 // T and its neighbour leaves (cap/fmm_solver.rs:52-91), row-major at nf_off[T].
 // Writes the FP64 values and, when v32 != nullptr, their FP32 rounding.
+// The plane frame of a source panel for the near-field tiles: the Newell
+// normal (panel.rs:256-269), e1 along the first non-degenerate edge,
+// e2 = n x e1, the vertices' in-plane coordinates (a_k, b_k) from vertex 0,
+// and whether the panel is flat (every vertex within rounding of the plane,
+// non-zero normal).  Both near-field assembly kernels call it, so they agree
+// on which panels each one takes.
+__device__ __forceinline__ bool panel_plane(const double *__restrict__ v, double n[3],
+                                            double e1[3], double e2[3], double av[4],
+                                            double bv[4]) {
+  double nx = 0.0, ny = 0.0, nz = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int j = (i + 1) & 3;
+    nx += (v[3 * i + 1] - v[3 * j + 1]) * (v[3 * i + 2] + v[3 * j + 2]);
+    ny += (v[3 * i + 2] - v[3 * j + 2]) * (v[3 * i] + v[3 * j]);
+    nz += (v[3 * i] - v[3 * j]) * (v[3 * i + 1] + v[3 * j + 1]);
+  }
+  const double nn = sqrt(nx * nx + ny * ny + nz * nz);
+  double e1x = 0.0, e1y = 0.0, e1z = 0.0, emax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int j = (i + 1) & 3;
+    const double ex = v[3 * j] - v[3 * i], ey = v[3 * j + 1] - v[3 * i + 1],
+                 ez = v[3 * j + 2] - v[3 * i + 2];
+    const double el = sqrt(ex * ex + ey * ey + ez * ez);
+    if (el > 0.0 && emax == 0.0) e1x = ex / el, e1y = ey / el, e1z = ez / el;
+    emax = fmax(emax, el);
+  }
+  bool flat = nn > 0.0 && emax > 0.0;
+  if (flat) nx /= nn, ny /= nn, nz /= nn;
+  n[0] = nx, n[1] = ny, n[2] = nz;
+  e1[0] = e1x, e1[1] = e1y, e1[2] = e1z;
+  e2[0] = ny * e1z - nz * e1y, e2[1] = nz * e1x - nx * e1z, e2[2] = nx * e1y - ny * e1x;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double qx = v[3 * k] - v[0], qy = v[3 * k + 1] - v[1], qz = v[3 * k + 2] - v[2];
+    av[k] = qx * e1[0] + qy * e1[1] + qz * e1[2];
+    bv[k] = qx * e2[0] + qy * e2[1] + qz * e2[2];
+    flat = flat && fabs(qx * nx + qy * ny + qz * nz) <= 1e-14 * emax;
+  }
+  return flat;
+}
+
+// Near-field tiles, one work-group per target leaf: each thread owns
+// columns of the leaf's tile (source panels), builds a flat source panel's
+// plane frame once (panel_plane) and evaluates the closed-form integral of
+// wilton() (panel.rs:147-208) for every target row of the leaf, so the
+// per-panel half of wilton() (the Newell normal, edge lengths and
+// directions: five square roots and fifteen divisions) is paid once per
+// (leaf, column), not once per entry.  In the plane, edge i has the unit
+// direction (la_i, lb_i) and the in-plane normal u = l x n = (lb_i, -la_i); a
+// target at in-plane (a, b) and height d gives perp = (a_i - a) lb_i -
+// (b_i - b) la_i, s- = (a_i - a) la_i + (b_i - b) lb_i and s+ likewise from
+// vertex j: the projections of wilton() on the plane, up to rounding; then the
+// log term with its asinh fallback and the solid-angle term.  The panel's
+// in-plane data sit in this thread's LDS slot, so the edge loop runs rolled
+// (fewer registers, four waves a SIMD).  Columns of panels that are not flat
+// are left to k_near_assemble_skew.  A leaf narrower than the work-group
+// splits its rows over groups of S threads.
 __global__ __launch_bounds__(256) void k_near_assemble(
     const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
     const int *__restrict__ col_off, const int *__restrict__ cols,
     const double *__restrict__ verts, const double *__restrict__ cen, double *__restrict__ v64,
     float *__restrict__ v32) {
-  int T = blockIdx.x;
-  int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
-  int c0 = col_off[T], S = col_off[T + 1] - c0;
-  long cnt = (long)nt * S;
-  int64_t base = nf_off[T];
-  for (long e = threadIdx.x; e < cnt; e += blockDim.x) {
-    int t = (int)(e / S), s = (int)(e % S);
-    int j = cols[c0 + s];
-    int i = t0 + t;
-    double w = wilton(verts + 12L * j, cen[3 * i], cen[3 * i + 1], cen[3 * i + 2]);
-    v64[base + e] = w;
-    if (v32) v32[base + e] = (float)w;
+  __shared__ double sgeo[256 * 16];
+  const int T = blockIdx.x;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const int64_t base = nf_off[T];
+  const int tid = threadIdx.x, nth = blockDim.x;  // nth == 256 (the LDS slots)
+  if (S == 0) return;
+  int G, g, s0, sstep;
+  if (S >= nth) {
+    G = 1, g = 0, s0 = tid, sstep = nth;
+  } else {
+    G = nth / S, g = tid / S, s0 = tid % S, sstep = S;
+    if (g >= G) return;
+  }
+  double *st = sgeo + (size_t)tid * 16;  // a_k, b_k, la_i, lb_i
+  for (int s = s0; s < S; s += sstep) {
+    const double *v = verts + 12L * cols[c0 + s];
+    double n[3], e1[3], e2[3], av[4], bv[4];
+    if (!panel_plane(v, n, e1, e2, av, bv)) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int j = (i + 1) & 3;
+      const double ea = av[j] - av[i], eb = bv[j] - bv[i];
+      const double el = sqrt(ea * ea + eb * eb);
+      // a zero-length edge: l = 0 makes perp, s- and s+ zero and its terms vanish
+      const double inv = el > 0.0 ? 1.0 / el : 0.0;
+      st[i] = av[i];
+      st[4 + i] = bv[i];
+      st[8 + i] = ea * inv;
+      st[12 + i] = eb * inv;
+    }
+    const double v0x = v[0], v0y = v[1], v0z = v[2];
+    for (int t = g; t < nt; t += G) {
+      const int it = t0 + t;
+      const double qx = cen[3 * it] - v0x, qy = cen[3 * it + 1] - v0y, qz = cen[3 * it + 2] - v0z;
+      const double d = qx * n[0] + qy * n[1] + qz * n[2];
+      const double a = qx * e1[0] + qy * e1[1] + qz * e1[2];
+      const double b = qx * e2[0] + qy * e2[1] + qz * e2[2];
+      const double absd = fabs(d), dd = d * d;
+      double total = 0.0;
+#pragma unroll 1
+      for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) & 3;
+        const double lai = st[8 + i], lbi = st[12 + i];
+        const double ai = st[i] - a, bi = st[4 + i] - b, aj = st[j] - a, bj = st[4 + j] - b;
+        const double perp = ai * lbi - bi * lai;
+        const double sm = ai * lai + bi * lbi;
+        const double sp = aj * lai + bj * lbi;
+        const double r0sq = perp * perp + dd;
+        const double rp = sqrt(sp * sp + r0sq);
+        const double rm = sqrt(sm * sm + r0sq);
+        const double num = rp + sp, den = rm + sm;
+        double logt;
+        if (num > 0.0 && den > 0.0) {
+          logt = perp * log(num / den);
+        } else if (fabs(den) > 0.0) {
+          const double r0 = sqrt(r0sq);
+          logt = r0 == 0.0 ? 0.0 : perp * (asinh(sp / r0) - asinh(sm / r0));
+        } else {
+          logt = 0.0;
+        }
+        double att = 0.0;
+        if (absd > 0.0 && perp != 0.0) {
+          const double tp = (perp * sp) / (r0sq + absd * rp);
+          const double tm = (perp * sm) / (r0sq + absd * rm);
+          att = absd * (atan(tp) - atan(tm));
+        }
+        total += logt - att;
+      }
+      const int64_t e = base + (int64_t)t * S + s;
+      v64[e] = total;
+      if (v32) v32[e] = (float)total;
+    }
+  }
+}
+
+// The columns k_near_assemble leaves: source panels that are not flat (a
+// skew quadrilateral of a panel file), wilton() per entry.  Same thread
+// mapping; a flat column costs one panel_plane() call.
+__global__ __launch_bounds__(256) void k_near_assemble_skew(
+    const int *__restrict__ leaf_lo, const int64_t *__restrict__ nf_off,
+    const int *__restrict__ col_off, const int *__restrict__ cols,
+    const double *__restrict__ verts, const double *__restrict__ cen, double *__restrict__ v64,
+    float *__restrict__ v32) {
+  const int T = blockIdx.x;
+  const int t0 = leaf_lo[T], nt = leaf_lo[T + 1] - t0;
+  const int c0 = col_off[T], S = col_off[T + 1] - c0;
+  const int64_t base = nf_off[T];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  if (S == 0) return;
+  int G, g, s0, sstep;
+  if (S >= nth) {
+    G = 1, g = 0, s0 = tid, sstep = nth;
+  } else {
+    G = nth / S, g = tid / S, s0 = tid % S, sstep = S;
+    if (g >= G) return;
+  }
+  for (int s = s0; s < S; s += sstep) {
+    const double *v = verts + 12L * cols[c0 + s];
+    double n[3], e1[3], e2[3], av[4], bv[4];
+    if (panel_plane(v, n, e1, e2, av, bv)) continue;
+    for (int t = g; t < nt; t += G) {
+      const int it = t0 + t;
+      const double w = wilton(v, cen[3 * it], cen[3 * it + 1], cen[3 * it + 2]);
+      const int64_t e = base + (int64_t)t * S + s;
+      v64[e] = w;
+      if (v32) v32[e] = (float)w;
+    }
   }
 }
 
@@ -481,6 +636,9 @@ void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_of
   ProfScope ps("near_assemble", s, 0.0, 0.0);
   hipLaunchKernelGGL(k_near_assemble, dim3(n_leaves), dim3(256), 0, s, leaf_lo, nf_off, col_off,
                      cols, verts, cen, v64, v32);
+  GP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_near_assemble_skew, dim3(n_leaves), dim3(256), 0, s, leaf_lo, nf_off,
+                     col_off, cols, verts, cen, v64, v32);
   GP_LAUNCH_CHECK();
 }
 

@@ -65,6 +65,25 @@ def test_fmm_operator_matches_oracle(ctx, order, level):
         assert np.linalg.norm(Yu[:, j] - yo) / np.linalg.norm(yo) < 1e-12
 
 
+def test_fmm_operator_with_skew_panels_matches_oracle(ctx):
+    """Quadrilaterals whose vertices leave their plane (a panel file's skew
+    quads) take the per-entry closed form in the near field; flat ones the
+    plane-frame form: both equal the oracle's wilton() to 1e-12."""
+    v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
+    v = v.copy()
+    rng = np.random.default_rng(5)
+    skew = rng.random(len(v)) < 0.3
+    # lift one vertex of 30 % of the panels off the plane by 1 % of the panel size
+    size = np.linalg.norm(v[:, 2] - v[:, 0], axis=1)
+    v[skew, 2, :] += 0.01 * size[skew, None] * rng.normal(size=(skew.sum(), 3))
+    x = rng.normal(size=(len(v), 2))
+    o = G.default_opts(method=G.GP_METHOD_FMM, fmm_order=4, max_level=4)
+    Y = ctx.fmm_apply(v, x, o)
+    for j in range(2):
+        yr = O.fmm_cap_apply(v, x[:, j], 4, 4, m2l_mode=0)
+        assert np.linalg.norm(Y[:, j] - yr) / np.linalg.norm(yr) < 1e-12
+
+
 @pytest.mark.parametrize("order,ncols", [(4, 16), (4, 40), (5, 97), (3, 20), (6, 32), (6, 97)])
 def test_fmm_operator_low_precision_close(ctx, order, ncols):
     # ncols > 16 spreads the columns over several 16-wide MFMA row tiles
