@@ -524,6 +524,77 @@ __global__ __launch_bounds__(64 * WPB) void k_near_spmv_pack(
   }
 }
 
+// Packs of small leaves with the pack's leaf table in LDS: before the
+// barrier, besides gathering x[cols], the work-group stores for each leaf of
+// the pack (at most 32) its first row, column window and value offset; a row
+// then finds its leaf by a binary search of that table, so its value loads
+// issue right after the barrier with no dependent global loads
+// (k_near_spmv_pack reads panel_leaf, col_off, nf_off and leaf_lo per row).
+// The same sums in the same order as k_near_spmv_pack.
+template <typename TV, typename TX, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_near_spmv_pack_lt(
+    const int *__restrict__ pack, const int *__restrict__ leaf_lo,
+    const int64_t *__restrict__ nf_off, const int *__restrict__ col_off,
+    const int *__restrict__ cols, const TV *__restrict__ vals, const TX *__restrict__ x,
+    double *__restrict__ y, const unsigned char *__restrict__ skip,
+    const float *__restrict__ rscale) {
+  using A = typename Acc<TX>::type;
+  using V = typename Vec16<TV>::type;
+  constexpr int VN = Vec16<TV>::n;
+  extern __shared__ unsigned char smem_raw[];
+  A *xs = reinterpret_cast<A *>(smem_raw);
+  __shared__ int64_t lt_off[33];
+  __shared__ int lt_row[33], lt_col[33];
+  const int P = blockIdx.x;
+  const int T0 = pack[P], T1 = pack[P + 1];
+  if (skip && skip[T0]) return;  // one problem per pack: work-group-uniform
+  const int nl = T1 - T0;         // 1 .. 32
+  const int c0 = col_off[T0], SC = col_off[T1] - c0;
+  const int r0 = leaf_lo[T0], R = leaf_lo[T1] - r0;
+  if ((int)threadIdx.x <= nl) {
+    const int T = T0 + threadIdx.x;
+    lt_row[threadIdx.x] = leaf_lo[T] - r0;
+    lt_col[threadIdx.x] = col_off[T] - c0;
+    lt_off[threadIdx.x] = nf_off[T];
+  }
+  for (int c = threadIdx.x; c < SC; c += 64 * WPB) xs[c] = (A)x[cols[c0 + c]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int gl = lane & 15, slot = lane >> 4;
+  for (int rb = 4 * w; rb < R; rb += 4 * WPB) {
+    const int r = rb + slot;
+    A acc = 0;
+    if (r < R) {
+      int lo = 0, hi = nl - 1;  // the last leaf whose first row is <= r
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (lt_row[mid] <= r) lo = mid;
+        else hi = mid - 1;
+      }
+      const int S = lt_col[lo + 1] - lt_col[lo];
+      const int64_t st = lt_off[lo] + (int64_t)(r - lt_row[lo]) * S;
+      const TV *row = vals + st;
+      const A *xr = xs + lt_col[lo];
+      const int head = min((int)((VN - (st & (VN - 1))) & (VN - 1)), S);
+      const int nv = (S - head) / VN;
+      if (gl < head) acc += (A)row[gl] * xr[gl];
+      const V *rv = reinterpret_cast<const V *>(row + head);
+#pragma unroll 4
+      for (int v = gl; v < nv; v += 16) {
+        const V vv = rv[v];
+        const A *xq = xr + head + v * VN;
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc += (A)vv[k] * xq[k];
+      }
+      const int tl = head + nv * VN + gl;
+      if (tl < S) acc += (A)row[tl] * xr[tl];
+    }
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_down(acc, off, 16);
+    if (gl == 0 && r < R)
+      y[r0 + r] = rscale ? (double)acc * (double)rscale[r0 + r] : (double)acc;
+  }
+}
+
 // Single RHS: the leaf-staged kernels with 4, 8, 2 or 1 wavefronts a leaf
 // (fewer: more leaves in flight per CU for leaves of a few panels), or one
 // wavefront per row; 5-7: leaf-staged with the first rows' values issued
@@ -534,9 +605,18 @@ void launch_spmv1(const char *tname, const NearTiles &nf, const TV *vals, const 
   using A = typename Acc<TX>::type;
   const size_t xs_bytes = (size_t)nf.max_cols * sizeof(A);
   const bool leaf_ok = xs_bytes <= 48 * 1024;
-  const int nvar = !leaf_ok ? 1 : nf.n_packs > 0 ? 10 : 8;
+  const int nvar = !leaf_ok ? 1 : nf.n_packs > 0 ? 12 : 8;
   tuned_launch(tname, nf.n_leaves, nf.max_cols, 1, nvar, s, [&](int v) {
-    if (v == 8 || v == 9) {  // packs of small leaves, 2 or 4 wavefronts, 16 lanes a row
+    if (v == 10 || v == 11) {  // packs with their leaf table in LDS, 4 or 2 wavefronts
+      if (v == 10)
+        hipLaunchKernelGGL((k_near_spmv_pack_lt<TV, TX, 4>), dim3(nf.n_packs), dim3(256), xs_bytes,
+                           s, nf.pack, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
+                           nf.skip_leaf, rscale);
+      else
+        hipLaunchKernelGGL((k_near_spmv_pack_lt<TV, TX, 2>), dim3(nf.n_packs), dim3(128), xs_bytes,
+                           s, nf.pack, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, vals, x, y,
+                           nf.skip_leaf, rscale);
+    } else if (v == 8 || v == 9) {  // packs of small leaves, 2 or 4 wavefronts, 16 lanes a row
       if (v == 8)
         hipLaunchKernelGGL((k_near_spmv_pack<TV, TX, 2>), dim3(nf.n_packs), dim3(128), xs_bytes, s,
                            nf.pack, nf.leaf_lo, nf.nf_off, nf.col_off, nf.cols, nf.panel_leaf, vals,
