@@ -533,6 +533,7 @@ PROFILE_SYMBOLS = {"m2l_bf16_mfma": "k_m2l_bf16_pipe", "m2l_bf16x3_mfma": "k_m2l
                    "m2l1t_bf16_mfma": "k_m2l1t64_bf16",
                    "m2l_f64_mfma": "k_m2l_f64_pipe", "m2l1t_f64_mfma": "k_m2l1t64_f64",
                    "near_spmv_f32": "k_near_spmv", "near_spmv_f16": "k_near_spmv",
+                   "near_spmv_f64": "k_near_spmv_f64",
                    "near_spmm_f32": "k_near_spmm_mfma"}
 
 

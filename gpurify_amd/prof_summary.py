@@ -11,7 +11,7 @@ counter reads for a streaming copy of known size (each counter reports KiB;
 the factor turns the reading into bytes).
 
     python -m gpurify_amd.prof_summary --pmc <fetch.csv> <write.csv> <copy symbol>
-        <copy MiB> <kernel symbol> <workload> "<command>" > out.md
+        <copy MiB> <kernel symbol>[=<name match>] <workload> "<command>" > out.md
 
 The last line, `traffic_bytes_per_launch: <symbol> <workload> <bytes>`, is
 what bench.py reports as the kernel's HBM traffic.
@@ -53,6 +53,10 @@ def counter_means(path, counter, symbol):
 
 def pmc_main(argv):
     fetch_csv, write_csv, copy_sym, copy_mib, sym, workload, cmd = argv
+    # "key=match": the traffic line is keyed `key`, the counters are those of
+    # the kernels whose name contains `match` (one instantiation of a family)
+    key, _, match = sym.partition("=")
+    sym = match or key
     copy_bytes = float(copy_mib) * 2 ** 20
     rows = []
     total = 0.0
@@ -69,7 +73,7 @@ def pmc_main(argv):
     print("|---|---:|---:|---:|---:|---:|")
     for counter, c_kib, factor, nd, k_kib, b in rows:
         print(f"| {counter} | {c_kib:.0f} | {factor:.1f} | {nd} | {k_kib:.0f} | {b:.4g} |")
-    print(f"\ntraffic_bytes_per_launch: {sym} {workload} {total:.6g}")
+    print(f"\ntraffic_bytes_per_launch: {key} {workload} {total:.6g}")
 
 
 def main():
