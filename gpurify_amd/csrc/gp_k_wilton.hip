@@ -631,18 +631,15 @@ void launch_dielectric_rows(int n, int nd, const int *drow, const double *nrm, c
 
 void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_off,
                           const int *col_off, const int *cols, const double *verts,
-                          const double *cen, double *v64, float *v32, hipStream_t s,
-                          bool skew_possible) {
+                          const double *cen, double *v64, float *v32, hipStream_t s) {
   if (n_leaves == 0) return;
   ProfScope ps("near_assemble", s, 0.0, 0.0);
   hipLaunchKernelGGL(k_near_assemble, dim3(n_leaves), dim3(256), 0, s, leaf_lo, nf_off, col_off,
                      cols, verts, cen, v64, v32);
   GP_LAUNCH_CHECK();
-  if (skew_possible) {
-    hipLaunchKernelGGL(k_near_assemble_skew, dim3(n_leaves), dim3(256), 0, s, leaf_lo, nf_off,
-                       col_off, cols, verts, cen, v64, v32);
-    GP_LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL(k_near_assemble_skew, dim3(n_leaves), dim3(256), 0, s, leaf_lo, nf_off,
+                     col_off, cols, verts, cen, v64, v32);
+  GP_LAUNCH_CHECK();
 }
 
 void launch_self_inv_diag(int n, const double *verts, const double *cen, double si_scale,

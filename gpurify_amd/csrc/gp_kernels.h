@@ -18,15 +18,9 @@ void launch_dielectric_rows(int n, int nd, const int *drow, const double *nrm, c
 // l's mirror image in the plane z = zm2 / 2 (a grounded plane by images).
 void launch_dense_assemble(int n, const double *verts, const double *cen, double *P,
                            hipStream_t s, bool image = false, double zm2 = 0.0);
-// Near-field tiles of every leaf (FP64, and FP32 when v32 is set).  Flat
-// panels take the plane-frame kernel; skew_possible (some panel may leave
-// its plane by more than rounding: 1e-15 of its longest edge on the host,
-// ten times tighter than the device's own test) adds the per-entry pass
-// over the columns of skew panels.
 void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_off,
                           const int *col_off, const int *cols, const double *verts,
-                          const double *cen, double *v64, float *v32, hipStream_t s,
-                          bool skew_possible = true);
+                          const double *cen, double *v64, float *v32, hipStream_t s);
 // Dielectric-interface rows of the near tiles (iface, nrm, diag per panel in
 // device order) and 1/diag as their Jacobi preconditioner in inv_diag.
 void launch_near_dielectric_rows(int n_leaves, int n, const int *leaf_lo, const int64_t *nf_off,

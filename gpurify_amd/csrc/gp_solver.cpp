@@ -408,39 +408,6 @@ std::shared_ptr<const LowRank> low_rank_tables(int n, int n3p, int kr, bool comp
   return e;
 }
 
-// Whether any panel of the problems may be skew for the near-field
-// assembly: a vertex off the Newell plane through vertex 0 by more than
-// 1e-15 of the panel's longest edge (ten times tighter than the device
-// kernel's 1e-14, so a panel the device finds skew is always flagged here).
-bool any_skew(const std::vector<const PanelGeom *> &probs) {
-  std::atomic<bool> skew{false};
-  for (const PanelGeom *g : probs)
-    parallel_for(g->n, [&](int i) {
-      const double *v = g->verts + (size_t)i * 12;
-      double nx = 0.0, ny = 0.0, nz = 0.0, emax = 0.0;
-      for (int a = 0; a < 4; ++a) {
-        const int b = (a + 1) & 3;
-        nx += (v[3 * a + 1] - v[3 * b + 1]) * (v[3 * a + 2] + v[3 * b + 2]);
-        ny += (v[3 * a + 2] - v[3 * b + 2]) * (v[3 * a] + v[3 * b]);
-        nz += (v[3 * a] - v[3 * b]) * (v[3 * a + 1] + v[3 * b + 1]);
-        const double ex = v[3 * b] - v[3 * a], ey = v[3 * b + 1] - v[3 * a + 1],
-                     ez = v[3 * b + 2] - v[3 * a + 2];
-        emax = std::max(emax, std::sqrt(ex * ex + ey * ey + ez * ez));
-      }
-      const double nn = std::sqrt(nx * nx + ny * ny + nz * nz);
-      if (!(nn > 0.0) || !(emax > 0.0)) {
-        skew = true;
-        return;
-      }
-      for (int k = 1; k < 4; ++k) {
-        const double q = ((v[3 * k] - v[0]) * nx + (v[3 * k + 1] - v[1]) * ny +
-                          (v[3 * k + 2] - v[2]) * nz) / nn;
-        if (!(std::fabs(q) <= 1e-15 * emax)) skew = true;
-      }
-    });
-  return skew;
-}
-
 struct FmmSystem {
   hipStream_t s = nullptr;
   int N = 0, n = 0, L = 0, ncond = 0;
@@ -766,8 +733,7 @@ struct FmmSystem {
     near16 = need32 && want_by_offset && o.near_inner == GP_NEAR_FP16 && !want_transpose && !diel;
     if (need32 && !near16) nf32.alloc((size_t)nnz);
     launch_near_assemble(n_leaves, leaf_lo.p, nf_off.p, col_off.p, cols.p, verts.p, cen.p, nf64.p,
-                         (need32 && !near16) ? nf32.p : nullptr, s,
-                         any_skew(probs));
+                         (need32 && !near16) ? nf32.p : nullptr, s);
     inv_diag.alloc(N);
     const double k_e = 1.0 / (4.0 * kPi * kEps0);
     for (int p = 0; p < P; ++p) {
