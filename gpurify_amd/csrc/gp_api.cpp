@@ -9,6 +9,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <unordered_map>
 #include <mutex>
 #include <string>
 
@@ -33,6 +34,84 @@ size_t &device_bytes_in_use() {
 std::recursive_mutex &device_mem_mutex() {
   static std::recursive_mutex m;
   return m;
+}
+
+namespace {
+struct DevCache {
+  std::multimap<size_t, void *> avail;           // cached blocks by size
+  std::vector<void *> pending;                   // released during the current calls
+  std::unordered_map<void *, size_t> size_of;    // every block of this device
+};
+std::map<int, DevCache> &dev_caches() {
+  static std::map<int, DevCache> c;
+  return c;
+}
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) d = 0;
+  return d;
+}
+void trim_one(DevCache &c) {
+  for (auto &kv : c.avail) {
+    (void)hipFree(kv.second);
+    c.size_of.erase(kv.second);
+  }
+  c.avail.clear();
+}
+}  // namespace
+
+void *device_alloc(size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
+  DevCache &c = dev_caches()[current_device()];
+  auto it = c.avail.lower_bound(bytes);
+  if (it != c.avail.end() && it->first <= bytes + std::max(bytes / 4, (size_t)1 << 20)) {
+    void *p = it->second;
+    c.avail.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim_one(c);
+    e = hipMalloc(&p, bytes);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(GP_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes: " +
+                                hipGetErrorString(e));
+  }
+  c.size_of[p] = bytes;
+  return p;
+}
+
+void device_release(void *p) {
+  std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
+  dev_caches()[current_device()].pending.push_back(p);
+}
+
+void device_recycle() {
+  std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
+  const int cur = current_device();
+  for (auto &kv : dev_caches()) {
+    DevCache &c = kv.second;
+    if (c.pending.empty()) continue;
+    // every kernel that used a pending block was queued before its release:
+    // once the device is idle none is left
+    (void)hipSetDevice(kv.first);
+    if (hipDeviceSynchronize() != hipSuccess) {
+      (void)hipGetLastError();
+      continue;  // keep them pending rather than reuse memory of a failed device
+    }
+    for (void *p : c.pending) c.avail.emplace(c.size_of[p], p);
+    c.pending.clear();
+  }
+  (void)hipSetDevice(cur);
+}
+
+void device_trim() {
+  std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
+  trim_one(dev_caches()[current_device()]);
 }
 
 double &upload_seconds() {
@@ -92,8 +171,20 @@ struct ActiveProf {
   }
 };
 
+// Recycles the device memory released during the call once it has ended
+// (gp::device_recycle), whatever its outcome.
+struct RecycleAtEnd {
+  ~RecycleAtEnd() {
+    try {
+      gp::device_recycle();
+    } catch (...) {
+    }
+  }
+};
+
 template <typename F>
 int guarded(gp_ctx *ctx, F &&f) {
+  RecycleAtEnd rec;
   ActiveProf ap(ctx);
   try {
     f();
@@ -157,6 +248,8 @@ void gp_ctx_destroy(gp_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->blas.reset();
+    gp::device_recycle();
+    gp::device_trim();  // the cached device memory goes back with the context
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->setup_stream) (void)hipStreamDestroy(ctx->setup_stream);
   }

@@ -42,6 +42,21 @@ size_t device_bytes_cap();
 // inside the solving thread's capture.
 std::recursive_mutex &device_mem_mutex();
 
+// Device memory of the library's buffers, cached per device: a released
+// block waits in a pending list until the API call that released it has
+// ended with the device idle (device_recycle), then serves later requests
+// of its size (a block up to a quarter larger, or 1 MiB, than asked is
+// reused); nothing is returned to the driver until device_trim (context
+// destruction, or a failed hipMalloc, which trims and retries once).  A
+// solve that rebuilds buffers of the same sizes call after call (a batch of
+// nets, a benchmark step) therefore maps its device memory once, and no
+// release inside a call synchronises the device as hipFree does.  All four
+// under device_mem_mutex; device_alloc throws GP_ERR_OOM.
+void *device_alloc(size_t bytes);
+void device_release(void *p);
+void device_recycle();
+void device_trim();
+
 template <typename T>
 struct DBuf {
   T *p = nullptr;
@@ -74,14 +89,14 @@ struct DBuf {
     if (cap && device_bytes_in_use() + bytes > cap)
       throw Error(GP_ERR_OOM, "device allocation of " + std::to_string(bytes) +
                                   " bytes exceeds GPURIFY_MAX_DEVICE_BYTES");
-    GP_HIP(hipMalloc((void **)&p, bytes));
+    p = static_cast<T *>(device_alloc(bytes));
     n = count;
     device_bytes_in_use() += bytes;
   }
   void release() {
     if (p) {
       std::lock_guard<std::recursive_mutex> lk(device_mem_mutex());
-      (void)hipFree(p);
+      device_release(p);
       device_bytes_in_use() -= n * sizeof(T);
     }
     p = nullptr;

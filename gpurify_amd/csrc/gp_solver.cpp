@@ -2166,6 +2166,7 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
       StagedGroup *nx = next.get();
       th = std::thread([&, nx] { stage_group(ctx, o, mixed, profile, *nx); });
     }
+    const double ts0 = now_s();
     try {
       const int G = (int)cur->idx.size();
       std::vector<double *> Cg(G);
@@ -2183,8 +2184,11 @@ void solve_fmm_batch(gp_ctx *ctx, const std::vector<const PanelGeom *> &probs,
       if (th.joinable()) th.join();
       throw;
     }
+    const double ts1 = now_s();
     if (th.joinable()) {
       th.join();
+      GP_LOG("group %zu: solve %.3f s; then waited %.3f s for group %zu's set-up (%.3f s)", j,
+             ts1 - ts0, now_s() - ts1, j + 1, next->t1 - next->t0);
       absorb(*next);
       if (next->err) std::rethrow_exception(next->err);
       cur = std::move(next);
