@@ -75,6 +75,8 @@ def parse():
                     help="groups of a batched solve, set-up overlapping the solve (0 = automatic)")
     ap.add_argument("--near-inner", choices=["fp16", "fp32"], default="fp32",
                     help="near-field values of the single-RHS inner operator (nets4096)")
+    ap.add_argument("--m2l-tile-order", type=int, default=0,
+                    help="single-RHS M2L tiles: 0 longest lists first, 1 box order")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
@@ -297,7 +299,8 @@ def bench_batch(args):
     log(f"workload nets4096: {len(probs)} nets, {n} panels (min {sizes.min()}, max {sizes.max()}), "
         f"{int((sizes < 500).sum())} below the dense limit")
     common = dict(fmm_order=args.order, max_level=0, leaf_target=args.leaf_target, tol=args.tol,
-                  rhs_first=0, rhs_count=1, batch_groups=args.batch_groups)
+                  rhs_first=0, rhs_count=1, batch_groups=args.batch_groups,
+                  m2l_tile_order=args.m2l_tile_order)
     o_mix = G.default_opts(precision=G.GP_PREC_MIXED, inner_tol=args.inner_tol,
                            near_inner=G.GP_NEAR_FP16 if args.near_inner == "fp16" else G.GP_NEAR_FP32,
                            inner_order=args.inner_order, **common)

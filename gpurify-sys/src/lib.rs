@@ -96,6 +96,7 @@ pub struct gp_cap_opts {
     pub inner_order: u32,
     pub ground: u32,
     pub ground_z: f64,
+    pub m2l_tile_order: u32,
 }
 
 #[repr(C)]

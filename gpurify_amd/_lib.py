@@ -71,7 +71,8 @@ class GpCapOpts(C.Structure):
                 ("m2l_compress", C.c_uint32), ("fuse_leaf", C.c_uint32),
                 ("skip_converged", C.c_uint32), ("batch_groups", C.c_uint32),
                 ("near_inner", C.c_uint32), ("inner_order", C.c_uint32),
-                ("ground", C.c_uint32), ("ground_z", C.c_double)]
+                ("ground", C.c_uint32), ("ground_z", C.c_double),
+                ("m2l_tile_order", C.c_uint32)]
 
 
 class GpParamMap(C.Structure):

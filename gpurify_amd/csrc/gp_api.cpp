@@ -311,6 +311,7 @@ void gp_cap_opts_default(gp_cap_opts *o) {
   o->inner_order = 0;
   o->ground = GP_GROUND_NONE;
   o->ground_z = 0.0;
+  o->m2l_tile_order = 0;
 }
 
 int gp_cap_solve(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, double *C,

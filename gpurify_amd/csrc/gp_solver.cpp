@@ -499,6 +499,8 @@ struct FmmSystem {
     if (n < 2 || n > 6) throw Error(GP_ERR_INVALID_INPUT, "fmm_order must be in [2, 6]");
     m2l_inner = (int)o.m2l_inner;
     m2l_outer = (int)o.m2l_outer;
+    if (o.m2l_tile_order > 1)
+      throw Error(GP_ERR_INVALID_INPUT, "m2l_tile_order must be 0 or 1");
     if (o.inner_order != 0 && (o.inner_order < 2 || (int)o.inner_order > n))
       throw Error(GP_ERR_INVALID_INPUT, "inner_order must be 0 or in [2, fmm_order]");
     n_in = o.inner_order ? (int)o.inner_order : n;
@@ -988,7 +990,12 @@ struct FmmSystem {
         auto len = [&](int b) { return hil_off[b + 1] - hil_off[b]; };
         int maxlen = 0;
         for (int b : htargets) maxlen = std::max(maxlen, len(b));
-        auto key = [&](int b) { return hoct[b] * (maxlen + 1) + (maxlen - len(b)); };
+        // m2l_tile_order 1: box order within the octant (neighbouring
+        // targets, shared sources) instead of longest lists first
+        const bool by_box = o.m2l_tile_order == 1;
+        auto key = [&](int b) {
+          return by_box ? hoct[b] * (maxlen + 1) : hoct[b] * (maxlen + 1) + (maxlen - len(b));
+        };
         std::vector<int> start((size_t)8 * (maxlen + 1) + 1, 0);
         for (int b : htargets) start[key(b) + 1]++;
         for (size_t k = 0; k + 1 < start.size(); ++k) start[k + 1] += start[k];

@@ -183,6 +183,10 @@ typedef struct {
                             ignored).  Not available for the dielectric solves and
                             gp_cap_jacobian. */
   double ground_z;       /* GP_GROUND_IMAGE: the plane's height, metres */
+  uint32_t m2l_tile_order; /* single-RHS M2L target tiles (64 targets of one octant): 0 (default)
+                            = longest interaction lists first, 1 = box (Morton) order within
+                            the octant; the same sums either way, only which targets share a
+                            tile and which tiles run together change. */
 } gp_cap_opts;
 enum { GP_GROUND_NONE = 0, GP_GROUND_IMAGE = 1 };
 enum { GP_BASIS_FP32 = 0, GP_BASIS_FP64 = 1 };
