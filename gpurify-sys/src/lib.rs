@@ -43,6 +43,8 @@ pub const GP_BASIS_FP32: u32 = 0;
 pub const GP_BASIS_FP64: u32 = 1;
 pub const GP_NEAR_FP32: u32 = 0;
 pub const GP_NEAR_FP16: u32 = 1;
+pub const GP_GROUND_NONE: u32 = 0;
+pub const GP_GROUND_IMAGE: u32 = 1;
 pub const GP_ROLE_CONDUCTOR: u32 = 0;
 pub const GP_ROLE_DIELECTRIC: u32 = 1;
 pub const GP_NOALLOC_MAX_PANELS: u32 = 512;
