@@ -338,6 +338,7 @@ def bench_batch(args):
                          "batch_groups": ngroups,
                          "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),
                          "refine_steps_max": max(r["refine_steps"] for r in fm),
+                         "refine_steps_mean": float(np.mean([r["refine_steps"] for r in fm])),
                          "converged": all(r["converged"] for r in reps), "global_batch": len(probs)}
         work = {"N": nfmm, "R": 1, "nnz": nnz, "pairs": pairs,
                 "n3": args.order ** 3, "n3_inner": (args.inner_order or args.order) ** 3}

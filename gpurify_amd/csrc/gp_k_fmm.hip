@@ -550,6 +550,54 @@ __global__ __launch_bounds__(256) void k_p2m1s(int n3p, int nleaf, const int *__
   }
 }
 
+// Single-RHS P2M over several consecutive leaves per work-group (LPB = 256 /
+// tpl leaf slots of tpl threads): the 1-D weights and charges of the
+// group's panels (contiguous: its leaves are consecutive) are staged in LDS
+// 256 panels at a time by coalesced loads of the whole block, then thread
+// (slot, m) sums the panels of its leaf in that chunk for node m (the
+// expression and order of k_p2m1).  With leaves of a few panels k_p2m1
+// waits on one small load per panel and k_p2m1s runs a work-group per leaf;
+// here every load instruction moves a full block.  Panels of skipped leaves
+// are staged but not summed.
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void k_p2m1m(int n3p, int nleaf, const int *__restrict__ leaf_lo,
+                                               const int *__restrict__ leaf_box, int leaf_box0,
+                                               const T *__restrict__ w1d,
+                                               const double *__restrict__ area,
+                                               const T *__restrict__ z, T *__restrict__ M,
+                                               __bf16 *__restrict__ Mb,
+                                               const unsigned char *__restrict__ skip) {
+  constexpr int n = NF, n3 = n * n * n, W3 = 3 * n, CH = 256;
+  constexpr int TPL = n3 <= 32 ? 32 : n3 <= 64 ? 64 : n3 <= 128 ? 128 : 256, LPB = 256 / TPL;
+  __shared__ T sw[CH * W3];
+  __shared__ T sq[CH];
+  const int l0 = blockIdx.x * LPB, l1 = min(l0 + LPB, nleaf);
+  const int slot = threadIdx.x / TPL, m = threadIdx.x % TPL, li = l0 + slot;
+  const bool mine = li < l1 && m < n3 && !(skip && skip[li]);
+  const int P0 = leaf_lo[l0], P1 = leaf_lo[l1];
+  const int lo = li < l1 ? leaf_lo[li] : P1, hi = li < l1 ? leaf_lo[li + 1] : P1;
+  const int a = m / (n * n), b = (m / n) % n, c = m % n;
+  T acc = 0;
+  for (int c0 = P0; c0 < P1; c0 += CH) {
+    const int cnt = min(CH, P1 - c0);
+    __syncthreads();  // the previous chunk has been read
+    for (int e = threadIdx.x; e < cnt * W3; e += 256) sw[e] = w1d[(size_t)c0 * W3 + e];
+    if ((int)threadIdx.x < cnt) sq[threadIdx.x] = (T)area[c0 + threadIdx.x] * z[c0 + threadIdx.x];
+    __syncthreads();
+    if (mine) {
+      const int j1 = min(hi, c0 + cnt) - c0;
+      for (int j = max(lo, c0) - c0; j < j1; ++j) {
+        const T *w = sw + j * W3;
+        acc += w[a] * w[n + b] * w[2 * n + c] * sq[j];
+      }
+    }
+  }
+  if (mine) {
+    const int box = leaf_box ? leaf_box[li] : leaf_box0 + li;
+    store_mult(M, Mb, (size_t)box * n3p + m, acc);
+  }
+}
+
 // Single right-hand side L2P: one thread per panel, y[p] += sum over nodes of
 // S(p, node) Lc[leaf][node] (the same expression and order as k_l2p).
 template <typename T>
@@ -2681,11 +2729,15 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       const int tpl = round_up(n3, n3 <= 32 ? 32 : 64), lpb = 256 / tpl;
       // one thread per (leaf, node), or (variant 1) one work-group per leaf
       // with its panels staged in LDS
-      tuned_launch(f32 ? "p2m1_f32" : "p2m1_f64", nleaf, n, 1, n >= 3 && n <= 6 ? 2 : 1, s,
+      tuned_launch(f32 ? "p2m1_f32" : "p2m1_f64", nleaf, n, 1, n >= 3 && n <= 6 ? 3 : 1, s,
                    [&](int v) {
         switch (n) {
 #define GP_P2M1(NF)                                                                              \
-  if (v == 1 && NF > 0)                                                                          \
+  if (v == 2 && NF > 0)                                                                          \
+    hipLaunchKernelGGL((k_p2m1m<T, (NF > 0 ? NF : 3)>), dim3((nleaf + 256 / tpl - 1) / (256 / tpl)), \
+                       dim3(256), 0, s, n3p, nleaf, f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, \
+                       M, mb, f.skip_leaf);                                                       \
+  else if (v == 1 && NF > 0)                                                                     \
     hipLaunchKernelGGL((k_p2m1s<T, (NF > 0 ? NF : 3)>), dim3(nleaf), dim3(tpl), 0, s, n3p, nleaf, \
                        f.leaf_lo, f.leaf_box, leaf0, f.w1d, f.area, z, M, mb, f.skip_leaf);       \
   else                                                                                           \

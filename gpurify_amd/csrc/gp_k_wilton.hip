@@ -368,6 +368,12 @@ __device__ __forceinline__ bool panel_plane(const double *__restrict__ v, double
   return flat;
 }
 
+// The arctangent and logarithm of k_near_assemble's edge loop as calls, not
+// inlined: 94 VGPRs (five waves a SIMD, as many as its LDS allows); with
+// both inlined the kernel takes 148 (three waves a SIMD).
+__device__ __noinline__ double atan_call(double x) { return ::atan(x); }
+__device__ __noinline__ double log_call(double x) { return ::log(x); }
+
 // Near-field tiles, one work-group per target leaf: each thread owns
 // columns of the leaf's tile (source panels), builds a flat source panel's
 // plane frame once (panel_plane) and evaluates the closed-form integral of
@@ -443,7 +449,7 @@ __global__ __launch_bounds__(256) void k_near_assemble(
         const double num = rp + sp, den = rm + sm;
         double logt;
         if (num > 0.0 && den > 0.0) {
-          logt = perp * log(num / den);
+          logt = perp * log_call(num / den);
         } else if (fabs(den) > 0.0) {
           const double r0 = sqrt(r0sq);
           logt = r0 == 0.0 ? 0.0 : perp * (asinh(sp / r0) - asinh(sm / r0));
@@ -452,9 +458,15 @@ __global__ __launch_bounds__(256) void k_near_assemble(
         }
         double att = 0.0;
         if (absd > 0.0 && perp != 0.0) {
-          const double tp = (perp * sp) / (r0sq + absd * rp);
-          const double tm = (perp * sm) / (r0sq + absd * rm);
-          att = absd * (atan(tp) - atan(tm));
+          // atan(perp sp / Q) - atan(perp sm / W) with Q, W > 0, as one angle:
+          // atan2(perp (sp W - sm Q), Q W + perp^2 sp sm) (one arctangent
+          // and one division instead of two of each; the same value up to
+          // rounding, the angle's error a few ulps of pi either way)
+          const double Q = r0sq + absd * rp, W = r0sq + absd * rm;
+          const double ay = perp * (sp * W - sm * Q), ax = Q * W + perp * perp * sp * sm;
+          double ang = atan_call(ay / ax);  // ax == +0: +-pi/2 from ay / ax = +-inf
+          if (ax < 0.0) ang += copysign(3.141592653589793, ay);
+          att = absd * ang;
         }
         total += logt - att;
       }

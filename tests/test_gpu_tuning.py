@@ -14,7 +14,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("order", [4, 5])
+@pytest.mark.parametrize("order", [3, 4, 5])
 def test_every_variant_applies_the_same_operator(ctx, order):
     v, c, _ = G.mesh_groups(W.two_wires(), refine=3)
     rng = np.random.default_rng(order)
