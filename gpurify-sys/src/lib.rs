@@ -196,6 +196,8 @@ extern "C" {
                         nr: u32, precision: u32, x: *const f64, y: *mut f64) -> c_int;
     pub fn gp_m2l_operator(order: u32, o0: i32, o1: i32, o2: i32, compressed: u32, g: *mut f64,
                            rank: *mut u32) -> c_int;
+    pub fn gp_m2l_factors(order: u32, o0: i32, o1: i32, o2: i32, u: *mut f64, vt: *mut f64,
+                          rank: *mut u32) -> c_int;
     pub fn gp_cap_solve_dense_noalloc(ctx: *mut gp_ctx, panels: *const gp_panels, c: *mut f64,
                                       report: *mut gp_cap_report) -> c_int;
     pub fn gp_profile_enable(ctx: *mut gp_ctx, enable: c_int) -> c_int;

@@ -114,7 +114,7 @@ EXPORTS = [
     "gp_fmm_apply", "gp_mesh_groups",
     "gp_solve_capacitance", "gp_profile_enable", "gp_profile_report", "gp_tune_force",
     "gp_tune_report", "gp_qui_parse", "gp_solve_resistance", "gp_cap_solve_dielectric",
-    "gp_m2l_operator", "gp_cap_solve_dense_noalloc", "gp_device_synchronize",
+    "gp_m2l_operator", "gp_m2l_factors", "gp_cap_solve_dense_noalloc", "gp_device_synchronize",
     "gp_cap_solve_dielectric_fmm",
 ]
 
@@ -194,6 +194,10 @@ def load():
     L.gp_m2l_operator.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
                                   C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     L.gp_m2l_operator.restype = C.c_int
+    L.gp_m2l_factors.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_int32,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_uint32)]
+    L.gp_m2l_factors.restype = C.c_int
     L.gp_tune_force.argtypes = [vp, C.c_int]
     L.gp_tune_force.restype = C.c_int
     L.gp_tune_report.argtypes = [vp, C.c_char_p, C.c_size_t]
@@ -497,6 +501,22 @@ def m2l_operator(order, off, compressed=True):
     if rc != GP_OK:
         raise GpError(rc, L.gp_last_error(None).decode())
     return G, int(rk.value)
+
+
+def m2l_factors(order, off):
+    """The factors of the truncated M2L block (gp_m2l_factors, host only):
+    (U (n^3 x rank), Vt (rank x n^3), rank)."""
+    L = load()
+    n3 = int(order) ** 3
+    U = np.zeros((n3, n3))
+    Vt = np.zeros((n3, n3))
+    rk = C.c_uint32(0)
+    rc = L.gp_m2l_factors(int(order), int(off[0]), int(off[1]), int(off[2]), _dptr(U), _dptr(Vt),
+                          C.byref(rk))
+    if rc != GP_OK:
+        raise GpError(rc, L.gp_last_error(None).decode())
+    k = int(rk.value)
+    return U[:, :k].copy(), Vt[:k].copy(), k
 
 
 class GpResPolygon(C.Structure):

@@ -704,6 +704,28 @@ int gp_m2l_operator(uint32_t order, int32_t o0, int32_t o1, int32_t o2, uint32_t
   });
 }
 
+int gp_m2l_factors(uint32_t order, int32_t o0, int32_t o1, int32_t o2, double *U, double *Vt,
+                   uint32_t *rank) {
+  return guarded(nullptr, [&] {
+    if (!U || !Vt) throw gp::Error(GP_ERR_INVALID_INPUT, "U or Vt is NULL");
+    if (order < 2 || order > 6) throw gp::Error(GP_ERR_INVALID_INPUT, "order must be in [2, 6]");
+    const int a0 = std::abs(o0), a1 = std::abs(o1), a2 = std::abs(o2);
+    if (std::max(a0, std::max(a1, a2)) > 3 || std::max(a0, std::max(a1, a2)) < 2)
+      throw gp::Error(GP_ERR_INVALID_INPUT, "offset must be a far-field offset in [-3, 3]^3");
+    const int n = (int)order, n3 = n * n * n, n3p = (n3 + 31) / 32 * 32;
+    const auto f = gp::m2l_factors(n, n3p);
+    const int oi = (o0 + 3) * 49 + (o1 + 3) * 7 + (o2 + 3), k = f->rank[oi], kp = f->kp;
+    std::fill(U, U + (size_t)n3 * n3, 0.0);
+    std::fill(Vt, Vt + (size_t)n3 * n3, 0.0);
+    for (int s = 0; s < k; ++s) {
+      for (int l = 0; l < n3; ++l) Vt[(size_t)s * n3 + l] = f->Vt[((size_t)oi * kp + s) * n3p + l];
+      for (int m = 0; m < n3; ++m)
+        U[(size_t)m * n3 + s] = f->Up[(((size_t)oi * n3p + m) * 4 + s % 4) * (kp / 4) + s / 4];
+    }
+    if (rank) *rank = (uint32_t)k;
+  });
+}
+
 int gp_qui_parse(const char *text, size_t text_len, double scale, uint32_t *n_panels,
                  uint32_t *n_conductors, size_t *names_len, double *vertices,
                  uint32_t *conductor, uint8_t *n_vertices, char *names) {

@@ -1470,6 +1470,246 @@ __global__ __launch_bounds__(256) void k_m2l1t64_f64(int n3, int n_tiles, int xr
 }
 
 
+// The 64-target tiles of the FP64 operator with truncated M2L blocks,
+// applied in factored form as the reference applies them (y += U (V^T x),
+// bbfmm.rs:132-174): per entry (offset o of rank k), T = V^T_o [M_s ...]
+// (k x 16 per wavefront, in ceil(k / 16) blocks of 16 rows, 16 K-steps of 4
+// over the n^3 nodes) and then acc += U_o T over ceil(k / 4) K-steps of 4:
+// at k = 8 that is 24 matrix-core products of 16x16x4 a wavefront instead of
+// the 64 of the dense block.  The D layout of the first product (row 4 r + q
+// of a block in lane group q, register r) is the B layout of the second
+// (K-step r, row q), so T never leaves its registers.  The nodes are summed
+// in the order lane group q takes nodes q K4 .. q K4 + K4 - 1, so each lane
+// reads contiguous runs of V^T_o and of its source's multipole; U_o rows are
+// stored with the rank index split in four (M2lFactors::Up) for the same
+// reason.  Wavefronts are independent (no LDS, no barrier): the factors come
+// through the caches, and a wavefront whose 16 targets are all skipped
+// leaves at once.
+template <int MT, int KB>
+__global__ __launch_bounds__(256) void k_m2l1t64_f64lr(int n3, int n_tiles, int xrun,
+                                                       const int *__restrict__ tile_tgt,
+                                                       const int *__restrict__ tile_eoff,
+                                                       const int *__restrict__ ent_oidx,
+                                                       const int *__restrict__ ent_src,
+                                                       const int *__restrict__ box_level,
+                                                       const double *__restrict__ inv_half,
+                                                       const double *__restrict__ Vt,
+                                                       const double *__restrict__ Up,
+                                                       const int *__restrict__ rank, int kp,
+                                                       const double *__restrict__ M,
+                                                       double *__restrict__ Lc,
+                                                       const unsigned char *__restrict__ skip) {
+  constexpr int NP = 16 * MT, K4 = NP / 4;
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tile = xcd_block(blockIdx.x, gridDim.x, xrun);
+  const int li = lane & 15, lq = lane >> 4, col = wave * 16 + li;
+  const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
+  const int t = tile_tgt[tile * 64 + col];
+  const bool off = t < 0 || (skip && skip[t]);
+  if (__all(off)) return;  // wavefront-uniform; no barrier follows
+  const int k4 = kp / 4;
+  f64x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  double b[K4], bn[K4];
+  // nodes lq K4 .. lq K4 + K4 - 1 of the target's source (zeros: none)
+  auto load_b = [&](int e, double (&B)[K4]) {
+    const int sb = off ? -1 : ent_src[(size_t)e * 64 + col];
+    const f64x2 *Ms = reinterpret_cast<const f64x2 *>(M + (size_t)(sb < 0 ? 0 : sb) * NP + lq * K4);
+#pragma unroll
+    for (int c = 0; c < K4 / 2; ++c) {
+      const f64x2 v = sb >= 0 ? Ms[c] : (f64x2){0.0, 0.0};
+      B[2 * c] = v[0];
+      B[2 * c + 1] = v[1];
+    }
+  };
+  if (e0 < e1) load_b(e0, b);
+  for (int e = e0; e < e1; ++e) {
+    if (e + 1 < e1) load_b(e + 1, bn);
+    const int o = ent_oidx[e], k = rank[o];
+    f64x4 tt[KB];
+#pragma unroll
+    for (int tb = 0; tb < KB; ++tb) {
+      tt[tb] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      if (tb * 16 < k) {  // wavefront-uniform
+        const f64x2 *va =
+            reinterpret_cast<const f64x2 *>(Vt + ((size_t)o * kp + tb * 16 + li) * NP + lq * K4);
+#pragma unroll
+        for (int c = 0; c < K4 / 2; ++c) {
+          const f64x2 a = va[c];
+          tt[tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[2 * c], tt[tb], 0, 0, 0);
+          tt[tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[2 * c + 1], tt[tb], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const double *ua = Up + (((size_t)o * NP + mt * 16 + li) * 4 + lq) * k4;
+#pragma unroll
+      for (int st = 0; st < 4 * KB; ++st)
+        if (st * 4 < k)  // wavefront-uniform
+          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[st], tt[st / 4][st % 4], acc[mt], 0,
+                                                         0, 0);
+    }
+    if (e + 1 < e1) {
+#pragma unroll
+      for (int c = 0; c < K4; ++c) b[c] = bn[c];
+    }
+  }
+  // D[i][j]: lane holds nodes lq + 4 v of target column li
+  if (off) return;
+  const double ih = inv_half[box_level[t]];
+  double *out = Lc + (size_t)t * n3;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = mt * 16 + lq + 4 * v;
+      if (m < n3) out[m] = acc[mt][v] * ih;
+    }
+}
+
+// The factored FP64 tiles with the factors staged in LDS by the work-group,
+// as k_m2l1t64_f64 stages G_o: per entry only the k rows of V^T_o and the k
+// columns of U_o (2 k n3 doubles: 17 KB at order 4, k <= 17, against 32 KB
+// for G_o), loaded into registers one entry ahead and stored into the other
+// of two LDS buffers (dynamic LDS of 2 (kr (n3p + 2) + n3p (kr + 1))
+// doubles, kr the largest rank), one barrier per entry.  The products and
+// their order are those of k_m2l1t64_f64lr; rows and columns beyond the
+// entry's rank read as zero.  PB: the next entry's source multipoles are
+// loaded before this entry's products (a second register set); otherwise
+// into the same registers once the first product has read them, so the
+// loads overlap the second product (fewer registers: three waves a SIMD at
+// order 4).
+template <int MT, int KRB, bool PB>  // KRB: a bound on the largest rank (kr <= KRB)
+__global__ __launch_bounds__(256) void k_m2l1t64_f64lrs(int n3, int n_tiles, int xrun,
+                                                        const int *__restrict__ tile_tgt,
+                                                        const int *__restrict__ tile_eoff,
+                                                        const int *__restrict__ ent_oidx,
+                                                        const int *__restrict__ ent_src,
+                                                        const int *__restrict__ box_level,
+                                                        const double *__restrict__ inv_half,
+                                                        const double *__restrict__ Vt,
+                                                        const double *__restrict__ U,
+                                                        const int *__restrict__ rank, int kp,
+                                                        int kr, const double *__restrict__ M,
+                                                        double *__restrict__ Lc,
+                                                        const unsigned char *__restrict__ skip) {
+  constexpr int NP = 16 * MT, K4 = NP / 4, LDV = NP + 2, KB = (KRB + 15) / 16;
+  constexpr int GPT = (2 * KRB * NP + 255) / 256;  // staged doubles per thread, at most
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  extern __shared__ double lds_f[];
+  const int LDU = kr + 1, bufsz = kr * LDV + NP * LDU;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tile = xcd_block(blockIdx.x, gridDim.x, xrun);
+  const int li = lane & 15, lq = lane >> 4, col = wave * 16 + li;
+  const int e0 = tile_eoff[tile], e1 = tile_eoff[tile + 1];
+  const int t = tile_tgt[tile * 64 + col];
+  const bool off = t < 0 || (skip && skip[t]);
+  if (!__syncthreads_or(!off)) return;
+  f64x4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  double b[K4], bn[PB ? K4 : 1], gp[GPT];
+  auto load_b = [&](int e, double *B) {
+    const int sb = off ? -1 : ent_src[(size_t)e * 64 + col];
+    const f64x2 *Ms = reinterpret_cast<const f64x2 *>(M + (size_t)(sb < 0 ? 0 : sb) * NP + lq * K4);
+#pragma unroll
+    for (int c = 0; c < K4 / 2; ++c) {
+      const f64x2 v = sb >= 0 ? Ms[c] : (f64x2){0.0, 0.0};
+      B[2 * c] = v[0];
+      B[2 * c + 1] = v[1];
+    }
+  };
+  // V^T_o rows 0 .. k-1 (k NP contiguous doubles), then U_o columns 0 .. k-1
+  auto load_g = [&](int o, int k) {
+    const double *Vo = Vt + (size_t)o * kp * NP, *Uo = U + (size_t)o * NP * kp;
+    const int nv = k * NP;
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < nv) gp[i] = Vo[v];
+      else if (v < 2 * nv) gp[i] = Uo[(size_t)((v - nv) / k) * kp + (v - nv) % k];
+    }
+  };
+  auto store_g = [&](int buf, int k) {
+    double *sV = lds_f + (size_t)buf * bufsz, *sU = sV + kr * LDV;
+    const int nv = k * NP;
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < nv) sV[(v / NP) * LDV + v % NP] = gp[i];
+      else if (v < 2 * nv) sU[((v - nv) / k) * LDU + (v - nv) % k] = gp[i];
+    }
+  };
+  int kc = 0;
+  if (e0 < e1) {
+    kc = rank[ent_oidx[e0]];
+    load_g(ent_oidx[e0], kc);
+    load_b(e0, b);
+    store_g(0, kc);
+  }
+  __syncthreads();
+  for (int e = e0; e < e1; ++e) {
+    const int buf = (e - e0) & 1;
+    const bool more = e + 1 < e1;
+    int kn = 0;
+    if (more) {
+      const int on = ent_oidx[e + 1];
+      kn = rank[on];
+      load_g(on, kn);
+      if constexpr (PB) load_b(e + 1, bn);
+    }
+    const double *sV = lds_f + (size_t)buf * bufsz, *sU = sV + kr * LDV;
+    f64x4 tt[KB];
+#pragma unroll
+    for (int tb = 0; tb < KB; ++tb) {
+      tt[tb] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      if (tb * 16 < kc) {  // work-group-uniform
+        const int r = tb * 16 + li;
+        const bool rv = r < kc;
+        const double *va = sV + (rv ? r : 0) * LDV + lq * K4;
+#pragma unroll
+        for (int j = 0; j < K4; ++j)
+          tt[tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(rv ? va[j] : 0.0, b[j], tt[tb], 0, 0, 0);
+      }
+    }
+    if constexpr (!PB)
+      if (more) load_b(e + 1, b);  // b is read; the loads overlap U_o T
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const double *ua = sU + (mt * 16 + li) * LDU;
+#pragma unroll
+      for (int st = 0; st < 4 * KB; ++st)
+        if (st * 4 < kc) {  // work-group-uniform
+          const int si = 4 * st + lq;
+          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(si < kc ? ua[si] : 0.0, tt[st / 4][st % 4],
+                                                         acc[mt], 0, 0, 0);
+        }
+    }
+    if (more) {
+      store_g(buf ^ 1, kn);
+      if constexpr (PB) {
+#pragma unroll
+        for (int c = 0; c < K4; ++c) b[c] = bn[c];
+      }
+    }
+    kc = kn;
+    __syncthreads();
+  }
+  if (off) return;
+  const double ih = inv_half[box_level[t]];
+  double *out = Lc + (size_t)t * n3;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = mt * 16 + lq + 4 * v;
+      if (m < n3) out[m] = acc[mt][v] * ih;
+    }
+}
+
 // Single right-hand side (ldm = 1), FP64 beyond the 64-target tiles: M2L
 // grouped by offset.  One launch per M2L offset o covers every (target,
 // source) pair with that offset (at most one per target, so the updates of one
@@ -1565,6 +1805,56 @@ void launch_m2l_tiles(const FmmDev<float> &f, int n3, const __bf16 *Mb, float *L
     case 14: hipLaunchKernelGGL((k_m2l1t_bf16<14>), grid, dim3(256), 0, s, n3, f.n_tiles, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, G, Mb, Lc, f.skip_box); break;
     default: throw Error(GP_ERR_INTERNAL, "no single-RHS M2L for this n3p");
   }
+  GP_LAUNCH_CHECK();
+}
+
+void launch_m2l_tiles_f64lr(const FmmDev<double> &f, int n3, const double *M, double *Lc,
+                            int xrun, hipStream_t s) {
+  const dim3 grid(f.n_tiles);
+  if (f.lr_kp != 16 && f.lr_kp != 32)
+    throw Error(GP_ERR_INTERNAL, "factored FP64 M2L tiles take ranks up to 32");
+  const bool k2 = f.lr_kp == 32;
+  switch (f.n3p / 16) {
+    case 2:
+      if (k2) hipLaunchKernelGGL((k_m2l1t64_f64lr<2, 2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.lr_vt, f.lr_up, f.lr_rank, f.lr_kp, M, Lc, f.skip_box);
+      else hipLaunchKernelGGL((k_m2l1t64_f64lr<2, 1>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.lr_vt, f.lr_up, f.lr_rank, f.lr_kp, M, Lc, f.skip_box);
+      break;
+    case 4:
+      if (k2) hipLaunchKernelGGL((k_m2l1t64_f64lr<4, 2>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.lr_vt, f.lr_up, f.lr_rank, f.lr_kp, M, Lc, f.skip_box);
+      else hipLaunchKernelGGL((k_m2l1t64_f64lr<4, 1>), grid, dim3(256), 0, s, n3, f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half, f.lr_vt, f.lr_up, f.lr_rank, f.lr_kp, M, Lc, f.skip_box);
+      break;
+    default: throw Error(GP_ERR_INTERNAL, "no FP64 M2L tiles for this n3p");
+  }
+  GP_LAUNCH_CHECK();
+}
+
+void launch_m2l_tiles_f64lrs(const FmmDev<double> &f, int n3, const double *M, double *Lc,
+                             int xrun, bool pb, hipStream_t s) {
+  const dim3 grid(f.n_tiles);
+  const int kr = f.lr_kmax, NP = f.n3p;
+  if (kr < 1 || kr > 32 || kr > f.lr_kp)
+    throw Error(GP_ERR_INTERNAL, "staged factored FP64 M2L tiles take ranks up to 32");
+  const size_t lds = 2 * ((size_t)kr * (NP + 2) + (size_t)NP * (kr + 1)) * sizeof(double);
+#define GP_LRS(MT_, KRB_)                                                                        \
+  if (pb) hipLaunchKernelGGL((k_m2l1t64_f64lrs<MT_, KRB_, true>), grid, dim3(256), lds, s, n3,     \
+                     f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src,             \
+                     f.box_level, f.inv_half, f.lr_vt, f.lr_u, f.lr_rank, f.lr_kp, kr, M, Lc,     \
+                     f.skip_box);                                                                 \
+  else hipLaunchKernelGGL((k_m2l1t64_f64lrs<MT_, KRB_, false>), grid, dim3(256), lds, s, n3, f.n_tiles, xrun, \
+                     f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src, f.box_level, f.inv_half,     \
+                     f.lr_vt, f.lr_u, f.lr_rank, f.lr_kp, kr, M, Lc, f.skip_box)
+  if (NP == 32) {
+    if (kr <= 16) GP_LRS(2, 16);
+    else if (kr <= 20) GP_LRS(2, 20);
+    else GP_LRS(2, 32);
+  } else if (NP == 64) {
+    if (kr <= 16) GP_LRS(4, 16);
+    else if (kr <= 20) GP_LRS(4, 20);
+    else GP_LRS(4, 32);
+  } else {
+    throw Error(GP_ERR_INTERNAL, "no FP64 M2L tiles for this n3p");
+  }
+#undef GP_LRS
   GP_LAUNCH_CHECK();
 }
 
@@ -2918,8 +3208,14 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       if constexpr (sizeof(T) == 8) {
         ProfScope pl("m2l1t_f64_mfma", s, fl * f.frac_pairs,
                      ((double)f.n_pairs * m3p * 8 + f.n_targets * mbytes) * f.frac_pairs);
-        tuned_launch("m2l1t_f64", f.n_tiles, m3p, f.tile_size, 2, s,
-                     [&](int v) { launch_m2l_tiles_f64(fm, m3, Mo, Lo, v ? 16 : 0, s); });
+        // 2, 3: the truncated blocks applied in factored form, by independent
+        // wavefronts; 4-7: the same with the factors staged in LDS (4, 5
+        // with the sources one entry ahead in registers, 6, 7 without)
+        tuned_launch("m2l1t_f64", f.n_tiles, m3p, f.tile_size, f.lr_vt ? 8 : 2, s, [&](int v) {
+          if (v >= 4) launch_m2l_tiles_f64lrs(fm, m3, Mo, Lo, v % 2 ? 16 : 0, v < 6, s);
+          else if (v >= 2) launch_m2l_tiles_f64lr(fm, m3, Mo, Lo, v == 3 ? 16 : 0, s);
+          else launch_m2l_tiles_f64(fm, m3, Mo, Lo, v ? 16 : 0, s);
+        });
       }
     } else if (m2l == GP_M2L_FP64 && sizeof(T) == 8 && by_offset) {
       if (!f.po_off) throw Error(GP_ERR_INTERNAL, "single-RHS M2L pairs by offset not built");

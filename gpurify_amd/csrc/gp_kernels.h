@@ -182,6 +182,13 @@ struct FmmDev {
   int n_pairs2 = 0;
   const int *pair_tgt = nullptr, *pair_off = nullptr, *pe_src = nullptr, *pe_o1 = nullptr,
             *pe_o2 = nullptr;
+  // FP64 operator with truncated M2L blocks: their factors (M2lFactors of
+  // gp_m2l_cpqr.h: lr_vt, lr_up, lr_u, per-offset lr_rank, padded and
+  // largest rank lr_kp, lr_kmax), for the factored single-RHS tile kernels;
+  // null when not built.
+  const double *lr_vt = nullptr, *lr_up = nullptr, *lr_u = nullptr;
+  const int *lr_rank = nullptr;
+  int lr_kp = 0, lr_kmax = 0;
   int n_tiles = 0, tile_size = 16;  // targets per tile: 16, or 64 (n3p <= 128)
   const int *tile_tgt = nullptr, *tile_eoff = nullptr, *ent_oidx = nullptr, *ent_src = nullptr;
 };

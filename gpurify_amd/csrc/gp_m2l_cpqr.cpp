@@ -22,7 +22,8 @@ namespace gp {
 
 double m2l_cpqr_eps(int n) { return std::max(1e-3 * std::pow(0.1, n), 1e-14); }
 
-int cpqr_truncate(int m, int n, double *a, int lda, double eps) {
+int cpqr_truncate(int m, int n, double *a, int lda, double eps, double *U, int ldu, double *Vt,
+                  int ldv) {
   std::vector<double> q((size_t)m * n);      // column j at q[j * m]
   std::vector<double> r((size_t)n * n, 0.0);  // R row-major, rows = pivot steps
   std::vector<int> perm(n);
@@ -75,6 +76,11 @@ int cpqr_truncate(int m, int n, double *a, int lda, double eps) {
     k = step + 1;
   }
   // a = U V^T with U = Q[:, :k] and V^T[s][perm[j]] = R[s][j].
+  if (U && Vt)
+    for (int s = 0; s < k; ++s) {
+      for (int i = 0; i < m; ++i) U[(size_t)i * ldu + s] = q[(size_t)s * m + i];
+      for (int j = 0; j < n; ++j) Vt[(size_t)s * ldv + perm[j]] = r[(size_t)s * n + j];
+    }
   for (int i = 0; i < m; ++i)
     for (int j = 0; j < n; ++j) {
       double v = 0.0;
@@ -145,6 +151,62 @@ std::shared_ptr<const std::vector<double>> m2l_tables(int n, int n3p, bool compr
   }
   if (ranks) *ranks = e.ranks;
   return e.G;
+}
+
+std::shared_ptr<const M2lFactors> m2l_factors(int n, int n3p) {
+  static std::mutex mu;
+  static std::map<int, std::shared_ptr<const M2lFactors>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto &slot = cache[n * 1024 + n3p];
+  if (slot) return slot;
+  const int n3 = n * n * n;
+  std::vector<double> nodes(n);
+  for (int m = 0; m < n; ++m) nodes[m] = std::cos((2.0 * m + 1.0) * M_PI / (2.0 * n));
+  const double eps = m2l_cpqr_eps(n);
+  // factors at full width first (rank <= n3), then packed to kp
+  std::vector<int> rank(343, 0);
+  std::vector<double> U((size_t)343 * n3 * n3, 0.0), V((size_t)343 * n3 * n3, 0.0);
+  std::vector<int> far;
+  for (int oi = 0; oi < 343; ++oi) {
+    const int o0 = oi / 49 - 3, o1 = (oi / 7) % 7 - 3, o2 = oi % 7 - 3;
+    if (std::max(std::abs(o0), std::max(std::abs(o1), std::abs(o2))) > 1) far.push_back(oi);
+  }
+  std::atomic<int> next(0);
+  auto work = [&] {
+    std::vector<double> blk((size_t)n3 * n3);
+    for (int t; (t = next.fetch_add(1)) < (int)far.size();) {
+      const int oi = far[t];
+      m2l_matrix(n, oi / 49 - 3, (oi / 7) % 7 - 3, oi % 7 - 3, nodes.data(), blk.data(), n3);
+      rank[oi] = cpqr_truncate(n3, n3, blk.data(), n3, eps, &U[(size_t)oi * n3 * n3], n3,
+                               &V[(size_t)oi * n3 * n3], n3);
+    }
+  };
+  const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+  auto f = std::make_shared<M2lFactors>();
+  const int kmax = *std::max_element(rank.begin(), rank.end());
+  f->kp = std::max(16, (kmax + 15) / 16 * 16);
+  const int kp = f->kp, k4 = kp / 4;
+  f->kmax = kmax;
+  f->rank = rank;
+  f->Vt.assign((size_t)343 * kp * n3p, 0.0);
+  f->Up.assign((size_t)343 * n3p * kp, 0.0);
+  f->U.assign((size_t)343 * n3p * kp, 0.0);
+  for (int oi : far)
+    for (int s = 0; s < rank[oi]; ++s) {
+      for (int l = 0; l < n3; ++l)
+        f->Vt[((size_t)oi * kp + s) * n3p + l] = V[(size_t)oi * n3 * n3 + (size_t)s * n3 + l];
+      for (int m = 0; m < n3; ++m) {
+        const double u = U[(size_t)oi * n3 * n3 + (size_t)m * n3 + s];
+        f->Up[(((size_t)oi * n3p + m) * 4 + s % 4) * k4 + s / 4] = u;
+        f->U[((size_t)oi * n3p + m) * kp + s] = u;
+      }
+    }
+  slot = f;
+  return slot;
 }
 
 }  // namespace gp

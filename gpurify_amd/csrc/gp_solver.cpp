@@ -444,7 +444,8 @@ struct FmmSystem {
   DBuf<float> nscale;
   DBuf<int> parent, octant, child_off, child_idx, targets, il_off, il_src, il_oidx, box_level;
   DBuf<double> inv_half;
-  DBuf<double> w1d64, m2m64, G64;
+  DBuf<double> w1d64, m2m64, G64, lr_vt64, lr_up64, lr_u64;
+  DBuf<int> lr_rank;
   DBuf<float> w1d32, m2m32, G32, Gr32;
   DBuf<uint16_t> Gbf;
   // Low-rank inner M2L: rank kr (0 = off), projection U and kr x kr tables.
@@ -1091,6 +1092,17 @@ struct FmmSystem {
     }
     m2m64 = to_device(hm2m, s);
     G64 = to_device(hG, s);
+    // the truncated blocks' factors for the factored single-RHS tile kernel
+    // (64-target tiles of the FP64 operator)
+    if (m2l_compress && want_by_offset && tile_size == 64 && n3p <= 64) {
+      const auto lr = m2l_factors(n, n3p);
+      lr_vt64 = to_device(lr->Vt, s);
+      lr_up64 = to_device(lr->Up, s);
+      lr_u64 = to_device(lr->U, s);
+      lr_rank = to_device(lr->rank, s);
+      f64.lr_kp = lr->kp;
+      f64.lr_kmax = lr->kmax;
+    }
     if (need32) {
       m2m32 = to_device(cast_vec<float>(hm2m_in), s);
       G32 = to_device(cast_vec<float>(hGi), s);
@@ -1178,6 +1190,10 @@ struct FmmSystem {
     f64.w1d = w1d64.p;
     f64.m2m1d = m2m64.p;
     f64.G = G64.p;
+    f64.lr_vt = lr_vt64.p;
+    f64.lr_up = lr_up64.p;
+    f64.lr_u = lr_u64.p;
+    f64.lr_rank = lr_rank.p;
     fill(f32);
     f32.n = n_in;
     f32.n3p = n3p_in;

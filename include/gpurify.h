@@ -281,6 +281,14 @@ int gp_fmm_apply(gp_ctx *ctx, const gp_panels *panels, const gp_cap_opts *opts, 
 int gp_m2l_operator(uint32_t order, int32_t o0, int32_t o1, int32_t o2, uint32_t compressed,
                     double *G, uint32_t *rank);
 
+/* The factors of that truncation, as the reference keeps and applies them
+ * (LowRankApprox u and vt, linalg.rs:277-381; y += u (vt x), bbfmm.rs:132-174)
+ * and the FP64 FMM operator's factored M2L kernel reads them (host only):
+ * U[m * n^3 + s] and Vt[s * n^3 + l] for s < *rank, zero beyond.  U, Vt:
+ * order^6 doubles each. */
+int gp_m2l_factors(uint32_t order, int32_t o0, int32_t o1, int32_t o2, double *U, double *Vt,
+                   uint32_t *rank);
+
 /* ------------------------------------------------------------ fixed workspace
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

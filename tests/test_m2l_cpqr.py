@@ -56,3 +56,19 @@ def test_rejects_near_and_out_of_range_offsets():
             G.m2l_operator(4, off)
     with pytest.raises(G.GpError):
         G.m2l_operator(7, (2, 0, 0))
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_factors_are_the_references(order):
+    # gp_m2l_factors: the u and vt of LowRankApprox::compress themselves (the
+    # FP64 operator's factored M2L kernel applies u (vt x), as bbfmm.rs does)
+    eps = O.m2l_eps(order)
+    for off in FAR:
+        u, vt, k = O.lowrank_compress(O.m2l_matrix(order, off), eps)
+        fu, fvt, fk = G.m2l_factors(order, off)
+        assert fk == k, off
+        assert np.abs(fu - u).max() <= 1e-15 and np.abs(fvt - vt).max() <= 1e-14 * np.abs(vt).max()
+        gc, _ = G.m2l_operator(order, off, compressed=True)
+        assert np.abs(fu @ fvt - gc).max() <= 1e-14 * np.abs(gc).max(), off
+    with pytest.raises(G.GpError):
+        G.m2l_factors(order, (1, 1, 0))
