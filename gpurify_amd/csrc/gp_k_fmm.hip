@@ -557,8 +557,9 @@ __global__ __launch_bounds__(256) void k_p2m1s(int n3p, int nleaf, const int *__
 // (slot, m) sums the panels of its leaf in that chunk for node m (the
 // expression and order of k_p2m1).  With leaves of a few panels k_p2m1
 // waits on one small load per panel and k_p2m1s runs a work-group per leaf;
-// here every load instruction moves a full block.  Panels of skipped leaves
-// are staged but not summed.
+// here every load instruction moves a full block.  A work-group whose leaves
+// are all skipped leaves at once; otherwise panels of skipped leaves are
+// staged but not summed.
 template <typename T, int NF>
 __global__ __launch_bounds__(256) void k_p2m1m(int n3p, int nleaf, const int *__restrict__ leaf_lo,
                                                const int *__restrict__ leaf_box, int leaf_box0,
@@ -574,6 +575,7 @@ __global__ __launch_bounds__(256) void k_p2m1m(int n3p, int nleaf, const int *__
   const int l0 = blockIdx.x * LPB, l1 = min(l0 + LPB, nleaf);
   const int slot = threadIdx.x / TPL, m = threadIdx.x % TPL, li = l0 + slot;
   const bool mine = li < l1 && m < n3 && !(skip && skip[li]);
+  if (!__syncthreads_or(mine)) return;  // every leaf of the group skipped
   const int P0 = leaf_lo[l0], P1 = leaf_lo[l1];
   const int lo = li < l1 ? leaf_lo[li] : P1, hi = li < l1 ? leaf_lo[li + 1] : P1;
   const int a = m / (n * n), b = (m / n) % n, c = m % n;
