@@ -123,11 +123,12 @@ WORKLOADS = {
     # batch: 4096 independent nets, each with its own substrate plate, 1 RHS
     # the inner operator's far field at order 3 (the FP64 outer residual
     # keeps order 4): 8 % faster than an order-4 inner operator, the same
-    # converged answer (gate on every entry); leaves of ~96 panels: fewer
-    # M2L pairs for a longer near field (mixed 6.76M -> 7.51M unknowns/s
-    # against leaves of 64; 7.46M at 128, 7.03M at 160)
+    # converged answer (gate on every entry); leaves of ~112 panels: fewer
+    # M2L pairs for a longer near field (mixed unknowns/s over two steps with
+    # the factored FP64 M2L: 7.19M at 64, 7.88M at 80, 8.17M at 96, 8.40M at
+    # 112, 8.31M at 128, 8.29M at 144, 8.13M at 160, 7.35M at 192)
     "nets4096": {"groups": None, "refine": 1, "order": 4, "rhs": 1, "name": "nets4096_met1met2",
-                 "cpu": None, "inner_order": 3, "leaf_target": 96},
+                 "cpu": None, "inner_order": 3, "leaf_target": 112},
     # adjoint sensitivities of the 16 x 16 crossing bundle (W = S = 140 nm)
     "bundle16": {"groups": None, "refine": 5, "order": 5, "rhs": None, "name": "bundle16x16_met1met2",
                  "cpu": None},
@@ -535,7 +536,7 @@ def calib_copy(mib):
 # device-profiler names of the dominant kernels -> their HIP symbols
 PROFILE_SYMBOLS = {"m2l_bf16_mfma": "k_m2l_bf16_pipe", "m2l_bf16x3_mfma": "k_m2l_bf16x3",
                    "m2l1t_bf16_mfma": "k_m2l1t64_bf16",
-                   "m2l_f64_mfma": "k_m2l_f64_pipe", "m2l1t_f64_mfma": "k_m2l1t64_f64",
+                   "m2l_f64_mfma": "k_m2l_f64_pipe", "m2l1t_f64_mfma": "k_m2l1t64_f64lrs",
                    "near_spmv_f32": "k_near_spmv", "near_spmv_f16": "k_near_spmv",
                    "near_spmv_f64": "k_near_spmv_f64",
                    "near_spmm_f32": "k_near_spmm_mfma"}
