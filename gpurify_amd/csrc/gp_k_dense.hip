@@ -85,26 +85,29 @@ __device__ void lu_solve_block(int n, int nrhs, double *__restrict__ A, int *__r
     const double pivot = s_piv;
     for (int i = k + 1 + tid; i < n; i += blockDim.x) A[(long)i * n + k] = A[(long)i * n + k] / pivot;
     __syncthreads();
-    const int m = n - k - 1;
-    const long cnt = (long)m * m;
-    for (long e = tid; e < cnt; e += blockDim.x) {
-      int i = k + 1 + (int)(e / m), j = k + 1 + (int)(e % m);
-      A[(long)i * n + j] = A[(long)i * n + j] - A[(long)i * n + k] * A[(long)k * n + j];
+    // rank-1 update, a wavefront per row, lanes along the row (no index
+    // division; every element the same expression as before)
+    const int nw = blockDim.x >> 6;
+    const double *Ak = A + (long)k * n;
+    for (int i = k + 1 + wid; i < n; i += nw) {
+      double *Ai = A + (long)i * n;
+      const double l = Ai[k];
+      for (int j = k + 1 + lane; j < n; j += 64) Ai[j] = Ai[j] - l * Ak[j];
     }
     __syncthreads();
   }
   // Permute: work[i][r] = B[piv[i]][r]  (linalg.rs:231-234).
-  for (long e = tid; e < (long)n * nrhs; e += blockDim.x) {
-    int i = (int)(e / nrhs), r = (int)(e % nrhs);
-    work[e] = B[(long)piv[i] * nrhs + r];
-  }
+  for (int i = tid; i < n; i += blockDim.x)
+    for (int r = 0; r < nrhs; ++r) work[(long)i * nrhs + r] = B[(long)piv[i] * nrhs + r];
   __syncthreads();
-  // Forward substitution, unit-diagonal L (column-oriented).
+  // Forward substitution, unit-diagonal L (column-oriented; a thread per
+  // row, every element the same expression as before).
   for (int j = 0; j < n; ++j) {
-    const long cnt = (long)(n - j - 1) * nrhs;
-    for (long e = tid; e < cnt; e += blockDim.x) {
-      int i = j + 1 + (int)(e / nrhs), r = (int)(e % nrhs);
-      work[(long)i * nrhs + r] -= A[(long)i * n + j] * work[(long)j * nrhs + r];
+    const double *wj = work + (long)j * nrhs;
+    for (int i = j + 1 + tid; i < n; i += blockDim.x) {
+      const double a = A[(long)i * n + j];
+      double *wi = work + (long)i * nrhs;
+      for (int r = 0; r < nrhs; ++r) wi[r] -= a * wj[r];
     }
     __syncthreads();
   }
@@ -112,10 +115,11 @@ __device__ void lu_solve_block(int n, int nrhs, double *__restrict__ A, int *__r
   for (int j = n - 1; j >= 0; --j) {
     for (int r = tid; r < nrhs; r += blockDim.x) work[(long)j * nrhs + r] /= A[(long)j * n + j];
     __syncthreads();
-    const long cnt = (long)j * nrhs;
-    for (long e = tid; e < cnt; e += blockDim.x) {
-      int i = (int)(e / nrhs), r = (int)(e % nrhs);
-      work[(long)i * nrhs + r] -= A[(long)i * n + j] * work[(long)j * nrhs + r];
+    const double *wj = work + (long)j * nrhs;
+    for (int i = tid; i < j; i += blockDim.x) {
+      const double a = A[(long)i * n + j];
+      double *wi = work + (long)i * nrhs;
+      for (int r = 0; r < nrhs; ++r) wi[r] -= a * wj[r];
     }
     __syncthreads();
   }

@@ -420,17 +420,25 @@ __global__ __launch_bounds__(256) void k_gmres_update_x(double *__restrict__ x,
 // covers rows [blk_lo[b], blk_hi[b]) of system blk_sys[b], and the blocks of
 // system p are [sys_blk[p], sys_blk[p + 1]) (fixed order, same bits every run).
 
-// partial[v * nblk + b] = sum over the rows of block b of X_v[row] * Y[row]
+// partial[v * nblk + b] = sum over the rows of block b of X_v[row] * Y[row];
+// with active given, 0 for the blocks of inactive systems (none of their
+// rows read)
 template <typename TX>
 __global__ __launch_bounds__(256) void k_seg_dot_partial(const TX *__restrict__ X, size_t strideX,
                                                          int nvec, const double *__restrict__ Y,
                                                          const int *__restrict__ blk_lo,
                                                          const int *__restrict__ blk_hi,
+                                                         const int *__restrict__ seg,
+                                                         const int *__restrict__ active,
                                                          double *__restrict__ partial) {
   __shared__ double red[KR_VB][256 / 64];
   const int b = blockIdx.x, nblk = gridDim.x, v0 = blockIdx.y * KR_VB;
   const int nv = min(KR_VB, nvec - v0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (active && (blk_lo[b] >= blk_hi[b] || !active[seg[blk_lo[b]]])) {  // work-group-uniform
+    if (tid < nv) partial[(size_t)(v0 + tid) * nblk + b] = 0.0;
+    return;
+  }
   double acc[KR_VB];
 #pragma unroll
   for (int v = 0; v < KR_VB; ++v) acc[v] = 0.0;
@@ -466,15 +474,18 @@ __global__ void k_seg_reduce(const double *__restrict__ partial, int nblk, int n
   out[e] = take_sqrt ? sqrt(t) : t;
 }
 
-// w[row] += alpha * sum_{v < nvec} coef[v][seg[row]] * V_v[row]
+// w[row] += alpha * sum_{v < nvec} coef[v][seg[row]] * V_v[row] (rows of
+// inactive systems untouched when active is given)
 template <typename TV>
 __global__ __launch_bounds__(256) void k_seg_axpy_multi(double *__restrict__ w,
                                                         const TV *__restrict__ V, size_t strideV,
                                                         const double *__restrict__ coef, int nvec,
                                                         int N, const int *__restrict__ seg, int nsys,
-                                                        double alpha) {
+                                                        double alpha,
+                                                        const int *__restrict__ active) {
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < N; row += gridDim.x * blockDim.x) {
     const int p = seg[row];
+    if (active && !active[p]) continue;
     double t = 0.0;
     for (int v = 0; v < nvec; ++v) t += coef[(size_t)v * nsys + p] * (double)V[(size_t)v * strideV + row];
     w[row] += alpha * t;
@@ -527,11 +538,11 @@ int grid_for(size_t total) { return (int)std::min<size_t>((total + 255) / 256, 8
 
 template <typename TX>
 void seg_dots(const SegMap &sm, const TX *X, size_t strideX, int nvec, const double *Y,
-              double *partial, double *out, bool take_sqrt, hipStream_t s) {
+              double *partial, double *out, bool take_sqrt, hipStream_t s, const int *active) {
   if (nvec <= 0) return;
   ProfScope ps("gmres_dots", s, 2.0 * nvec * sm.N, (sizeof(TX) * nvec + 8.0) * sm.N);
   hipLaunchKernelGGL((k_seg_dot_partial<TX>), dim3(sm.nblk, (nvec + KR_VB - 1) / KR_VB), dim3(256), 0,
-                     s, X, strideX, nvec, Y, sm.blk_lo, sm.blk_hi, partial);
+                     s, X, strideX, nvec, Y, sm.blk_lo, sm.blk_hi, sm.seg, active, partial);
   GP_LAUNCH_CHECK();
   const int tot = nvec * sm.nsys;
   hipLaunchKernelGGL(k_seg_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, partial, sm.nblk, nvec,
@@ -543,11 +554,11 @@ size_t seg_partial_size(const SegMap &sm, int nvec_max) {
 }
 template <typename TV>
 void seg_axpy_multi(const SegMap &sm, double *w, const TV *V, size_t strideV, const double *coef,
-                    int nvec, double alpha, hipStream_t s) {
+                    int nvec, double alpha, hipStream_t s, const int *active) {
   if (nvec <= 0) return;
   ProfScope ps("gmres_axpy", s, 2.0 * nvec * sm.N, (sizeof(TV) * nvec + 16.0) * sm.N);
   hipLaunchKernelGGL((k_seg_axpy_multi<TV>), dim3(grid_for(sm.N)), dim3(256), 0, s, w, V, strideV,
-                     coef, nvec, sm.N, sm.seg, sm.nsys, alpha);
+                     coef, nvec, sm.N, sm.seg, sm.nsys, alpha, active);
   GP_LAUNCH_CHECK();
 }
 template <typename TO>
@@ -572,9 +583,9 @@ void seg_unit_rhs(const SegMap &sm, const uint32_t *cond, const int *rhs_cond, d
 }
 #define GP_SEG_BASIS(TV)                                                                        \
   template void seg_dots<TV>(const SegMap &, const TV *, size_t, int, const double *, double *,  \
-                             double *, bool, hipStream_t);                                      \
+                             double *, bool, hipStream_t, const int *);                         \
   template void seg_axpy_multi<TV>(const SegMap &, double *, const TV *, size_t, const double *, \
-                                   int, double, hipStream_t);                                   \
+                                   int, double, hipStream_t, const int *);                      \
   template void seg_scale<TV>(const SegMap &, const double *, TV *, const double *, const int *, \
                               hipStream_t);                                                     \
   template void seg_update_x<TV>(const SegMap &, double *, const TV *, size_t, const double *,   \

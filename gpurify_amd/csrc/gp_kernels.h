@@ -269,12 +269,15 @@ struct SegMap {
   const int *sys_blk = nullptr;  // nsys + 1: blocks of each system
 };
 size_t seg_partial_size(const SegMap &sm, int nvec_max);
+// active (optional, per system): systems flagged 0 are skipped (dots 0,
+// rows untouched)
 template <typename TX>
 void seg_dots(const SegMap &sm, const TX *X, size_t strideX, int nvec, const double *Y,
-              double *partial, double *out, bool take_sqrt, hipStream_t s);
+              double *partial, double *out, bool take_sqrt, hipStream_t s,
+              const int *active = nullptr);
 template <typename TV>
 void seg_axpy_multi(const SegMap &sm, double *w, const TV *V, size_t strideV, const double *coef,
-                    int nvec, double alpha, hipStream_t s);
+                    int nvec, double alpha, hipStream_t s, const int *active = nullptr);
 template <typename TO>
 void seg_scale(const SegMap &sm, const double *in, TO *out, const double *scale, const int *mask,
                hipStream_t s);

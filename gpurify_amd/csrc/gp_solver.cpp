@@ -1430,11 +1430,12 @@ struct BlockOps {
   size_t partial_size(int nv) const { return kr_partial_size(nv, ld); }
   template <typename TX>
   void dots(const TX *X, size_t sx, int nv, const double *Y, double *part, double *out,
-            bool sq) const {
+            bool sq, const int * = nullptr) const {
     kr_dots(X, sx, nv, Y, N, ld, nr, part, out, sq, s);
   }
   template <typename TV>
-  void axpy_multi(double *w, const TV *V, size_t sv, const double *coef, int nv, double a) const {
+  void axpy_multi(double *w, const TV *V, size_t sv, const double *coef, int nv, double a,
+                  const int * = nullptr) const {
     kr_axpy_multi(w, V, sv, coef, nv, N, ld, nr, a, s);
   }
   template <typename TO>
@@ -1462,12 +1463,13 @@ struct SegOps {
   size_t partial_size(int nv) const { return seg_partial_size(sm, nv); }
   template <typename TX>
   void dots(const TX *X, size_t sx, int nv, const double *Y, double *part, double *out,
-            bool sq) const {
-    seg_dots(sm, X, sx, nv, Y, part, out, sq, s);
+            bool sq, const int *act = nullptr) const {
+    seg_dots(sm, X, sx, nv, Y, part, out, sq, s, act);
   }
   template <typename TV>
-  void axpy_multi(double *w, const TV *V, size_t sv, const double *coef, int nv, double a) const {
-    seg_axpy_multi(sm, w, V, sv, coef, nv, a, s);
+  void axpy_multi(double *w, const TV *V, size_t sv, const double *coef, int nv, double a,
+                  const int *act = nullptr) const {
+    seg_axpy_multi(sm, w, V, sv, coef, nv, a, s, act);
   }
   template <typename TO>
   void scale(const double *in, TO *out, const double *sc, const int *mask) const {
@@ -1647,23 +1649,26 @@ void gmres_run(FmmSystem &sys, const Ops &ops, int prec, const double *b, double
     for (int k0 = 0, len = 0; k0 < m; k0 += len) {
       len = std::min(chunk, m - k0);
       ensure_V(k0 + len + 1);
+      // a batch skips the operator and Gram-Schmidt work of its inactive
+      // systems (read from the device flags, so graph replays see the
+      // current ones): their new basis vectors are zero (inv_h = 0) and
+      // never enter x, and their coefficients are not used
+      const int *act = (Ops::maskable && sys.skip_converged) ? active.p : nullptr;
       auto step = [&](int k) {
-        // a batch skips the operator work of its inactive systems: their
-        // new basis vectors are zero (inv_h = 0) and never enter x
         if constexpr (Ops::maskable)
           if (sys.skip_converged) sys.mask(active.p, graphs ? nullptr : h_active);
         sys.apply(V.p + (size_t)k * sv, true, nc, w.p, prec);
         if constexpr (Ops::maskable) sys.unmask();
-        ops.dots(V.p, sv, k + 1, w.p, partial.p, h1.p, false);
-        ops.axpy_multi(w.p, V.p, sv, h1.p, k + 1, -1.0);
+        ops.dots(V.p, sv, k + 1, w.p, partial.p, h1.p, false, act);
+        ops.axpy_multi(w.p, V.p, sv, h1.p, k + 1, -1.0, act);
         if (passes > 1) {
-          ops.dots(V.p, sv, k + 1, w.p, partial.p, h2.p, false);
-          ops.axpy_multi(w.p, V.p, sv, h2.p, k + 1, -1.0);
+          ops.dots(V.p, sv, k + 1, w.p, partial.p, h2.p, false, act);
+          ops.axpy_multi(w.p, V.p, sv, h2.p, k + 1, -1.0, act);
         }
-        ops.dots(w.p, 0, 1, w.p, partial.p, hn.p, true);
+        ops.dots(w.p, 0, 1, w.p, partial.p, hn.p, true, act);
         kr_gmres_givens(k, m, nr, S, h1.p, passes > 1 ? h2.p : nullptr, hn.p, H.p, cs.p, sn.p,
                         g.p, bnorm.p, tol, active.p, kused.p, est.p, inv_h.p, s);
-        ops.scale(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, nullptr);
+        ops.scale(w.p, V.p + (size_t)(k + 1) * sv, inv_h.p, act);
       };
       auto steps = [&] {
         for (int k = k0; k < k0 + len; ++k) step(k);
