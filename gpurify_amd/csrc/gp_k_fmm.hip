@@ -703,29 +703,23 @@ __global__ __launch_bounds__(256) void k_l2p1v4(int N, const int *__restrict__ p
   y[p] += (double)acc;
 }
 
-// Single right-hand side M2M: 256 / tpb parent boxes per work-group, one
-// thread per (parent, node).  All children of a parent are loaded into its
-// LDS slot at once (8 loads in flight), then the three 1-D contractions (z,
-// then y, then x, as k_m2m) run for every child between three barriers, the
-// x pass summing over the children in order.
-template <typename T, int NF = 0>  // NF > 0: the order fixed at compile time (loops unrolled)
-__global__ __launch_bounds__(256) void k_m2m1(int n_rt, int n3p, int tpb, int box0, int nb,
-                                              const int *__restrict__ child_off,
-                                              const int *__restrict__ child_idx,
-                                              const int *__restrict__ octant,
-                                              const T *__restrict__ m2m1d, T *__restrict__ M,
-                                              __bf16 *__restrict__ Mb,
-                                              const unsigned char *__restrict__ skip) {
+// Single right-hand side M2M of parent box b by the tpb threads of one LDS
+// slot of the work-group (thread m: node m; on = false: loads and stores
+// nothing but passes the barriers).  All children are loaded into the slot
+// at once (8 loads in flight), then the three 1-D contractions (z, then y,
+// then x, as k_m2m) run for every child between three barriers, the x pass
+// summing over the children in order.
+template <typename T, int NF>
+__device__ __forceinline__ void m2m1_box(int n_rt, int n3p, int tpb, int b, bool on, int slot,
+                                         int m, unsigned char *smem_raw,
+                                         const int *__restrict__ child_off,
+                                         const int *__restrict__ child_idx,
+                                         const int *__restrict__ octant,
+                                         const T *__restrict__ m2m1d, T *__restrict__ M,
+                                         __bf16 *__restrict__ Mb) {
   const int n = NF > 0 ? NF : n_rt;
-  extern __shared__ unsigned char smem_raw[];
-  const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
   const int n3 = n * n * n, nn = n * n;
   T *s0 = reinterpret_cast<T *>(smem_raw) + (size_t)slot * 16 * tpb, *s1 = s0 + 8 * tpb;
-  const int bi = blockIdx.x * (256 / tpb) + slot;
-  const int b = box0 + min(bi, nb - 1);
-  // a skipped box loads no child and stores nothing, but its threads pass
-  // every barrier with the others
-  const bool on = bi < nb && !(skip && skip[b]);
   const bool live = on && m < n3;
   const int c0 = child_off[b], nch = on ? child_off[b + 1] - c0 : 0;
   const int a = m / nn, bq = (m / n) % n, cq = m % n;
@@ -774,30 +768,84 @@ __global__ __launch_bounds__(256) void k_m2m1(int n_rt, int n3p, int tpb, int bo
   if (live) store_mult(M, Mb, (size_t)b * n3p + m, acc);
 }
 
-// Single right-hand side L2L: one thread per (child, node), the transposed
-// contractions x^T, y^T, z^T of k_l2l through a per-child LDS slot.
+// Single right-hand side M2M of one level: 256 / tpb parent boxes per
+// work-group, one thread per (parent, node).
 template <typename T, int NF = 0>  // NF > 0: the order fixed at compile time (loops unrolled)
-__global__ __launch_bounds__(256) void k_l2l1(int n_rt, int tpb, int box0, int nb,
-                                              const int *__restrict__ parent,
+__global__ __launch_bounds__(256) void k_m2m1(int n_rt, int n3p, int tpb, int box0, int nb,
+                                              const int *__restrict__ child_off,
+                                              const int *__restrict__ child_idx,
                                               const int *__restrict__ octant,
-                                              const T *__restrict__ m2m1d, T *__restrict__ Lc,
+                                              const T *__restrict__ m2m1d, T *__restrict__ M,
+                                              __bf16 *__restrict__ Mb,
                                               const unsigned char *__restrict__ skip) {
-  const int n = NF > 0 ? NF : n_rt;
   extern __shared__ unsigned char smem_raw[];
   const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
+  const int bi = blockIdx.x * (256 / tpb) + slot;
+  const int b = box0 + min(bi, nb - 1);
+  // a skipped box loads no child and stores nothing, but its threads pass
+  // every barrier with the others
+  const bool on = bi < nb && !(skip && skip[b]);
+  m2m1_box<T, NF>(n_rt, n3p, tpb, b, on, slot, m, smem_raw, child_off, child_idx, octant, m2m1d,
+                  M, Mb);
+}
+
+// The whole single-RHS upward pass (M2M from the finest parents up to level
+// 2) of one subtree per work-group: the descendants of level-2 box
+// sub_root[r] at level l are the global boxes [rng[2 l], rng[2 l + 1]),
+// rng = sub_rng + 2 (L + 1) r (contiguous: boxes are Morton-ordered within a
+// problem's level, and a problem's boxes of one level are contiguous), and
+// sub_leaf[r] is the level of the subtree's leaves.  Level by level, deepest
+// parents first, 256 / tpb parents at a time, with a barrier after each
+// batch (the slots' LDS is reused) and between levels (a level reads the
+// multipoles the level below wrote).  Subtrees are independent, so one
+// launch replaces a launch per level, most of which held a few hundred boxes.
+// Per box the arithmetic of k_m2m1.
+template <typename T, int NF = 0>
+__global__ __launch_bounds__(256) void k_m2m1_sub(int n_rt, int n3p, int tpb, int L,
+                                                  const int *__restrict__ sub_root,
+                                                  const int *__restrict__ sub_leaf,
+                                                  const int *__restrict__ sub_rng,
+                                                  const int *__restrict__ child_off,
+                                                  const int *__restrict__ child_idx,
+                                                  const int *__restrict__ octant,
+                                                  const T *__restrict__ m2m1d, T *__restrict__ M,
+                                                  __bf16 *__restrict__ Mb,
+                                                  const unsigned char *__restrict__ skip) {
+  extern __shared__ unsigned char smem_raw[];
+  const int r = blockIdx.x;
+  if (skip && skip[sub_root[r]]) return;  // every box of a problem shares its flag
+  const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb, bpb = 256 / tpb;
+  const int *rng = sub_rng + (size_t)r * 2 * (L + 1);
+  for (int l = sub_leaf[r] - 1; l >= 2; --l) {
+    const int lo = rng[2 * l], hi = rng[2 * l + 1];
+    for (int b0 = lo; b0 < hi; b0 += bpb) {
+      const int b = b0 + slot;
+      m2m1_box<T, NF>(n_rt, n3p, tpb, b < hi ? b : lo, b < hi, slot, m, smem_raw, child_off,
+                      child_idx, octant, m2m1d, M, Mb);
+      __syncthreads();
+    }
+  }
+}
+
+// Single right-hand side L2L of child box ch by the tpb threads of one LDS
+// slot (live = false: loads and stores nothing, passes the barriers): the
+// transposed contractions x^T, y^T, z^T of k_l2l through the slot.
+template <typename T, int NF>
+__device__ __forceinline__ void l2l1_box(int n_rt, int tpb, int ch, bool on, int slot, int m,
+                                         unsigned char *smem_raw,
+                                         const int *__restrict__ parent,
+                                         const int *__restrict__ octant,
+                                         const T *__restrict__ m2m1d, T *__restrict__ Lc) {
+  const int n = NF > 0 ? NF : n_rt;
   const int n3 = n * n * n, nn = n * n;
   T *s0 = reinterpret_cast<T *>(smem_raw) + slot * 2 * tpb, *s1 = s0 + tpb;
-  const int bi = blockIdx.x * (256 / tpb) + slot;
-  const int ch = box0 + min(bi, nb - 1);
-  // a skipped child loads and stores nothing; its threads pass every barrier
-  const bool off = skip && skip[ch];
-  const bool live = bi < nb && m < n3 && !off;
+  const bool live = on && m < n3;
   const int p = parent[ch], o = octant[ch];
   const T *mx = m2m1d + (o * 3 + 0) * nn, *my = m2m1d + (o * 3 + 1) * nn,
           *mz = m2m1d + (o * 3 + 2) * nn;
   const int a = m / nn, bq = (m / n) % n, cq = m % n;
   const T own = live ? Lc[(size_t)ch * n3 + m] : (T)0;  // the child's M2L part, loaded early
-  if (m < n3) s0[m] = off ? (T)0 : Lc[(size_t)p * n3 + m];
+  if (m < n3) s0[m] = on ? Lc[(size_t)p * n3 + m] : (T)0;
   __syncthreads();
   if (m < n3) {  // x^T: s1[ap][b][c] = sum_a mx[a][ap] s0[a][b][c]
     T t = 0;
@@ -815,6 +863,52 @@ __global__ __launch_bounds__(256) void k_l2l1(int n_rt, int tpb, int box0, int n
     T t = 0;
     for (int c = 0; c < n; ++c) t += mz[c * n + cq] * s0[(a * n + bq) * n + c];
     Lc[(size_t)ch * n3 + m] = own + t;
+  }
+}
+
+// Single right-hand side L2L of one level: one thread per (child, node).
+template <typename T, int NF = 0>  // NF > 0: the order fixed at compile time (loops unrolled)
+__global__ __launch_bounds__(256) void k_l2l1(int n_rt, int tpb, int box0, int nb,
+                                              const int *__restrict__ parent,
+                                              const int *__restrict__ octant,
+                                              const T *__restrict__ m2m1d, T *__restrict__ Lc,
+                                              const unsigned char *__restrict__ skip) {
+  extern __shared__ unsigned char smem_raw[];
+  const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb;
+  const int bi = blockIdx.x * (256 / tpb) + slot;
+  const int ch = box0 + min(bi, nb - 1);
+  // a skipped child loads and stores nothing; its threads pass every barrier
+  const bool on = bi < nb && !(skip && skip[ch]);
+  l2l1_box<T, NF>(n_rt, tpb, ch, on, slot, m, smem_raw, parent, octant, m2m1d, Lc);
+}
+
+// The whole single-RHS downward pass (L2L from level 2 to the leaves) of one
+// subtree per work-group, the subtrees and ranges of k_m2m1_sub: level by
+// level from the children of the root down, a barrier after each batch and
+// between levels (a level reads the locals the level above completed).  Per
+// box the arithmetic of k_l2l1.
+template <typename T, int NF = 0>
+__global__ __launch_bounds__(256) void k_l2l1_sub(int n_rt, int tpb, int L,
+                                                  const int *__restrict__ sub_root,
+                                                  const int *__restrict__ sub_leaf,
+                                                  const int *__restrict__ sub_rng,
+                                                  const int *__restrict__ parent,
+                                                  const int *__restrict__ octant,
+                                                  const T *__restrict__ m2m1d, T *__restrict__ Lc,
+                                                  const unsigned char *__restrict__ skip) {
+  extern __shared__ unsigned char smem_raw[];
+  const int r = blockIdx.x;
+  if (skip && skip[sub_root[r]]) return;  // every box of a problem shares its flag
+  const int slot = threadIdx.x / tpb, m = threadIdx.x % tpb, bpb = 256 / tpb;
+  const int *rng = sub_rng + (size_t)r * 2 * (L + 1);
+  for (int l = 3; l <= sub_leaf[r]; ++l) {
+    const int lo = rng[2 * l], hi = rng[2 * l + 1];
+    for (int b0 = lo; b0 < hi; b0 += bpb) {
+      const int ch = b0 + slot;
+      l2l1_box<T, NF>(n_rt, tpb, ch < hi ? ch : lo, ch < hi, slot, m, smem_raw, parent, octant,
+                      m2m1d, Lc);
+      __syncthreads();
+    }
   }
 }
 
@@ -3065,8 +3159,33 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
     GP_LAUNCH_CHECK();
   }
   // M2M, finest to level 2 (multipoles above level 2 are never used), over
-  // the boxes of each level that have children (listed first in the level).
-  for (int l = L - 1; l >= 2; --l) {
+  // the boxes of each level that have children (listed first in the level);
+  // one RHS with subtrees built: one launch, a work-group per subtree.
+  const bool by_sub = ldm == 1 && nr == 1 && n3 <= 256 && f.n_sub > 0 && f.sub_root;
+  if (by_sub) {
+    int npar_all = 0, nch_all = 0;
+    for (int l = 2; l < L; ++l) {
+      npar_all += f.npar ? f.npar[l] : f.lvl_off[l + 1] - f.lvl_off[l];
+      nch_all += f.lvl_off[l + 2] - f.lvl_off[l + 1];
+    }
+    ProfScope pm(f32 ? "m2m_f32" : "m2m_f64", s, 6.0 * n * n3 * nch_all * f.frac_boxes,
+                 (npar_all + nch_all) * exp_bytes * f.frac_boxes);
+    const int tpb = round_up(n3, n3 <= 32 ? 32 : 64);
+#define GP_M2M1S(NF)                                                                            \
+  hipLaunchKernelGGL((k_m2m1_sub<T, NF>), dim3(f.n_sub), dim3(256), 4096 * sizeof(T), s, n, n3p, \
+                     tpb, L, f.sub_root, f.sub_leaf, f.sub_rng, f.child_off, f.child_idx,        \
+                     f.octant, f.m2m1d, M, mb, f.skip_box)
+    switch (n) {
+      case 3: GP_M2M1S(3); break;
+      case 4: GP_M2M1S(4); break;
+      case 5: GP_M2M1S(5); break;
+      case 6: GP_M2M1S(6); break;
+      default: GP_M2M1S(0); break;
+    }
+#undef GP_M2M1S
+    GP_LAUNCH_CHECK();
+  }
+  for (int l = L - 1; l >= 2 && !by_sub; --l) {
     int nb = f.npar ? f.npar[l] : f.lvl_off[l + 1] - f.lvl_off[l];
     if (nb == 0) continue;
     int nch = f.lvl_off[l + 2] - f.lvl_off[l + 1];
@@ -3315,8 +3434,29 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
   // leaf at the finest level, the layout the fused kernel walks.
   const bool fuse_leaf = sizeof(T) == 4 && tiled && !f.leaf_box && f.panel_leaf && ldm % 4 == 0 &&
                          ldm >= 16 && n >= 3 && n <= 5 && L >= 3 && nr > 1 && f.fuse_leaf;
-  // L2L, level 2 to the leaves.
-  for (int l = 2; l < L; ++l) {
+  // L2L, level 2 to the leaves (one RHS with subtrees built: one launch, a
+  // work-group per subtree).
+  if (by_sub) {
+    int nch_all = 0;
+    for (int l = 2; l < L; ++l) nch_all += f.lvl_off[l + 2] - f.lvl_off[l + 1];
+    ProfScope pm(f32 ? "l2l_f32" : "l2l_f64", s, 6.0 * n * n3 * nch_all * f.frac_boxes,
+                 2.0 * nch_all * exp_bytes * f.frac_boxes);
+    const int tpb = round_up(n3, n3 <= 32 ? 32 : 64);
+#define GP_L2L1S(NF)                                                                            \
+  hipLaunchKernelGGL((k_l2l1_sub<T, NF>), dim3(f.n_sub), dim3(256), 512 * sizeof(T), s, n, tpb, \
+                     L, f.sub_root, f.sub_leaf, f.sub_rng, f.parent, f.octant, f.m2m1d, Lc,      \
+                     f.skip_box)
+    switch (n) {
+      case 3: GP_L2L1S(3); break;
+      case 4: GP_L2L1S(4); break;
+      case 5: GP_L2L1S(5); break;
+      case 6: GP_L2L1S(6); break;
+      default: GP_L2L1S(0); break;
+    }
+#undef GP_L2L1S
+    GP_LAUNCH_CHECK();
+  }
+  for (int l = 2; l < L && !by_sub; ++l) {
     int nb = f.lvl_off[l + 2] - f.lvl_off[l + 1];
     if (nb == 0) continue;
     if (fuse_leaf && l == L - 1) continue;  // in k_l2l_l2p_leaf

@@ -189,6 +189,12 @@ struct FmmDev {
   const double *lr_vt = nullptr, *lr_up = nullptr, *lr_u = nullptr;
   const int *lr_rank = nullptr;
   int lr_kp = 0, lr_kmax = 0;
+  // Single-RHS tree passes by subtree (k_m2m1_sub, k_l2l1_sub): the level-2
+  // boxes sub_root, the level of their leaves sub_leaf, and per level l <= L
+  // the global range of each one's descendants, sub_rng[(r (L + 1) + l) 2 ..
+  // + 1]; n_sub = 0: a launch per level.
+  int n_sub = 0;
+  const int *sub_root = nullptr, *sub_leaf = nullptr, *sub_rng = nullptr;
   int n_tiles = 0, tile_size = 16;  // targets per tile: 16, or 64 (n3p <= 128)
   const int *tile_tgt = nullptr, *tile_eoff = nullptr, *ent_oidx = nullptr, *ent_src = nullptr;
 };

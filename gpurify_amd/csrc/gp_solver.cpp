@@ -443,6 +443,9 @@ struct FmmSystem {
   DBuf<_Float16> nf16;
   DBuf<float> nscale;
   DBuf<int> parent, octant, child_off, child_idx, targets, il_off, il_src, il_oidx, box_level;
+  // subtrees of the single-RHS tree passes (FmmDev::sub_root ..)
+  int n_sub = 0;
+  DBuf<int> sub_root, sub_leaf, sub_rng;
   DBuf<double> inv_half;
   DBuf<double> w1d64, m2m64, G64, lr_vt64, lr_up64, lr_u64;
   DBuf<int> lr_rank;
@@ -1073,6 +1076,68 @@ struct FmmSystem {
     il_oidx = to_device(hil_oidx.get(), n_il, s);
     box_level = to_device(hlevel, s);
     inv_half = to_device(hih, s);
+    // Subtrees of the single-RHS tree passes: each level-2 box of a problem
+    // with boxes below level 2, the level of its leaves, and per level the
+    // global range of its descendants (Morton order keeps them contiguous;
+    // checked here, else the passes keep their launch per level).  Largest
+    // subtrees first.
+    n_sub = 0;
+    if (want_by_offset && L >= 3) {
+      std::vector<int> sr, sl, sg, cnt;
+      bool contiguous = true;
+      for (int p : order) {
+        const Tree &t = trees[p];
+        if (t.L < 3) continue;
+        std::vector<int> anc(t.n_boxes(), -1);
+        for (int b = t.lvl_off[2]; b < t.lvl_off[3]; ++b) anc[b] = b;
+        for (int l = 3; l <= t.L; ++l)
+          for (int b = t.lvl_off[l]; b < t.lvl_off[l + 1]; ++b) anc[b] = anc[t.parent[b]];
+        const int r0 = (int)sr.size();
+        for (int b = t.lvl_off[2]; b < t.lvl_off[3]; ++b) {
+          sr.push_back(gid[p][b]);
+          sl.push_back(t.L);
+          cnt.push_back(0);
+        }
+        sg.resize(sr.size() * 2 * (L + 1), 0);
+        std::vector<int> lo(sr.size() - r0), hi(sr.size() - r0), nb(sr.size() - r0);
+        for (int l = 2; l <= t.L; ++l) {
+          std::fill(lo.begin(), lo.end(), INT_MAX);
+          std::fill(hi.begin(), hi.end(), -1);
+          std::fill(nb.begin(), nb.end(), 0);
+          for (int b = t.lvl_off[l]; b < t.lvl_off[l + 1]; ++b) {
+            const int k = anc[b] - t.lvl_off[2], g = gid[p][b];
+            lo[k] = std::min(lo[k], g);
+            hi[k] = std::max(hi[k], g + 1);
+            ++nb[k];
+          }
+          for (int k = 0; k < (int)lo.size(); ++k) {
+            int *rg = &sg[(size_t)(r0 + k) * 2 * (L + 1) + 2 * l];
+            if (nb[k] == 0) continue;
+            contiguous &= hi[k] - lo[k] == nb[k];
+            rg[0] = lo[k];
+            rg[1] = hi[k];
+            cnt[r0 + k] += nb[k];
+          }
+        }
+      }
+      if (!contiguous) {
+        GP_LOG("tree passes by level: a subtree's boxes are not contiguous");
+      } else if (!sr.empty()) {
+        std::vector<int> ord(sr.size());
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+        std::vector<int> r2(sr.size()), l2(sr.size()), g2(sg.size());
+        for (size_t i = 0; i < ord.size(); ++i) {
+          r2[i] = sr[ord[i]];
+          l2[i] = sl[ord[i]];
+          std::copy_n(&sg[(size_t)ord[i] * 2 * (L + 1)], 2 * (L + 1), &g2[i * 2 * (L + 1)]);
+        }
+        sub_root = to_device(r2, s);
+        sub_leaf = to_device(l2, s);
+        sub_rng = to_device(g2, s);
+        n_sub = (int)r2.size();
+      }
+    }
     {
       DBuf<double> dgeo = to_device(hgeo, s), dnodes = to_device(nodes, s);
       w1d64.alloc((size_t)N * 3 * n);
@@ -1185,6 +1250,10 @@ struct FmmSystem {
       f.tile_eoff = tile_eoff.p;
       f.ent_oidx = ent_oidx.p;
       f.ent_src = ent_src.p;
+      f.n_sub = n_sub;
+      f.sub_root = sub_root.p;
+      f.sub_leaf = sub_leaf.p;
+      f.sub_rng = sub_rng.p;
     };
     fill(f64);
     f64.w1d = w1d64.p;
