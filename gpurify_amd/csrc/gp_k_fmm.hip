@@ -600,6 +600,173 @@ __global__ __launch_bounds__(256) void k_p2m1m(int n3p, int nleaf, const int *__
   }
 }
 
+// ---------------------------------------------------------------- weights on the fly
+// The FP32 single-RHS operator's P2M and L2P can form each panel's 1-D
+// interpolation weights from its local coordinates (k_interp_w1d's sums,
+// w_m(x) = sum_k wc[m n + k] T_k(x) with wc = (2 / n) T_k(x_m), the k = 0
+// term halved), reading 16 bytes a panel (x, y, z in the leaf's box frame
+// and the area, FP32) instead of the 3 n stored weights and the FP64 area
+// (48 bytes at order 3).  The weights then differ from the stored ones (FP64
+// sums rounded to FP32) by FP32 rounding.
+
+// w_m(x) for one node row c of wc (n coefficients), by the Chebyshev
+// recurrence.
+template <int n>
+__device__ __forceinline__ float cheb_w(float x, const float (&c)[n]) {
+  float tkm1 = 1.0f, tk = x, s = c[0];
+  if (n > 1) s += c[1] * x;
+#pragma unroll
+  for (int k = 2; k < n; ++k) {
+    const float t = 2.0f * x * tk - tkm1;
+    tkm1 = tk;
+    tk = t;
+    s += c[k] * t;
+  }
+  return s;
+}
+
+// xl[p] = (x, y, z, area): panel p's centroid in its leaf box's frame
+// ((cen - centre) / half width, as k_interp_w1d) and its area, FP32.
+__global__ __launch_bounds__(256) void k_local_coords(int N, const int *__restrict__ panel_leaf,
+                                                      const double *__restrict__ geo,
+                                                      const double *__restrict__ cen,
+                                                      const double *__restrict__ area,
+                                                      float4 *__restrict__ xl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *g = geo + (size_t)panel_leaf[i] * 4;
+  xl[i] = make_float4((float)((cen[(size_t)i * 3] - g[0]) / g[3]),
+                      (float)((cen[(size_t)i * 3 + 1] - g[1]) / g[3]),
+                      (float)((cen[(size_t)i * 3 + 2] - g[2]) / g[3]), (float)area[i]);
+}
+
+void launch_local_coords(int N, const int *panel_leaf, const double *geo, const double *cen,
+                         const double *area, float *xl, hipStream_t s) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_local_coords, dim3((N + 255) / 256), dim3(256), 0, s, N, panel_leaf, geo, cen,
+                     area, reinterpret_cast<float4 *>(xl));
+  GP_LAUNCH_CHECK();
+}
+
+// k_p2m1 with the weights formed from xl: thread (leaf, node m = (a, b, c))
+// evaluates w_a(x) w_b(y) w_c(z) per panel.
+template <int NF>
+__global__ __launch_bounds__(256) void k_p2m1x(int n3p, int tpl, int nleaf,
+                                               const int *__restrict__ leaf_lo,
+                                               const int *__restrict__ leaf_box, int leaf_box0,
+                                               const float4 *__restrict__ xl,
+                                               const float *__restrict__ wc,
+                                               const float *__restrict__ z, float *__restrict__ M,
+                                               __bf16 *__restrict__ Mb,
+                                               const unsigned char *__restrict__ skip) {
+  constexpr int n = NF, n3 = n * n * n;
+  const int li = blockIdx.x * (256 / tpl) + threadIdx.x / tpl, m = threadIdx.x % tpl;
+  if (li >= nleaf || m >= n3 || (skip && skip[li])) return;
+  const int a = m / (n * n), b = (m / n) % n, c = m % n;
+  float ca[n], cb[n], cc[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) ca[k] = wc[a * n + k], cb[k] = wc[b * n + k], cc[k] = wc[c * n + k];
+  float acc = 0;
+#pragma unroll 4
+  for (int p = leaf_lo[li]; p < leaf_lo[li + 1]; ++p) {
+    const float4 v = xl[p];
+    const float q = v.w * z[p];
+    acc += cheb_w<n>(v.x, ca) * cheb_w<n>(v.y, cb) * cheb_w<n>(v.z, cc) * q;
+  }
+  const int box = leaf_box ? leaf_box[li] : leaf_box0 + li;
+  store_mult(M, Mb, (size_t)box * n3p + m, acc);
+}
+
+// k_p2m1m with the weights formed from xl: the group's panels' (x, y, z,
+// area z) staged in LDS 256 at a time, thread (slot, m) evaluating its
+// node's three weights per panel.
+template <int NF>
+__global__ __launch_bounds__(256) void k_p2m1mx(int n3p, int nleaf, const int *__restrict__ leaf_lo,
+                                                const int *__restrict__ leaf_box, int leaf_box0,
+                                                const float4 *__restrict__ xl,
+                                                const float *__restrict__ wc,
+                                                const float *__restrict__ z, float *__restrict__ M,
+                                                __bf16 *__restrict__ Mb,
+                                                const unsigned char *__restrict__ skip) {
+  constexpr int n = NF, n3 = n * n * n, CH = 256;
+  constexpr int TPL = n3 <= 32 ? 32 : n3 <= 64 ? 64 : n3 <= 128 ? 128 : 256, LPB = 256 / TPL;
+  __shared__ float4 sx[CH];
+  const int l0 = blockIdx.x * LPB, l1 = min(l0 + LPB, nleaf);
+  const int slot = threadIdx.x / TPL, m = threadIdx.x % TPL, li = l0 + slot;
+  const bool mine = li < l1 && m < n3 && !(skip && skip[li]);
+  if (!__syncthreads_or(mine)) return;  // every leaf of the group skipped
+  const int P0 = leaf_lo[l0], P1 = leaf_lo[l1];
+  const int lo = li < l1 ? leaf_lo[li] : P1, hi = li < l1 ? leaf_lo[li + 1] : P1;
+  const int mm = min(m, n3 - 1), a = mm / (n * n), b = (mm / n) % n, c = mm % n;
+  float ca[n], cb[n], cc[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) ca[k] = wc[a * n + k], cb[k] = wc[b * n + k], cc[k] = wc[c * n + k];
+  float acc = 0;
+  for (int c0 = P0; c0 < P1; c0 += CH) {
+    const int cnt = min(CH, P1 - c0);
+    __syncthreads();  // the previous chunk has been read
+    if ((int)threadIdx.x < cnt) {
+      float4 v = xl[c0 + threadIdx.x];
+      v.w *= z[c0 + threadIdx.x];  // area * z, the charge
+      sx[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (mine) {
+      const int j1 = min(hi, c0 + cnt) - c0;
+      for (int j = max(lo, c0) - c0; j < j1; ++j) {
+        const float4 v = sx[j];
+        acc += cheb_w<n>(v.x, ca) * cheb_w<n>(v.y, cb) * cheb_w<n>(v.z, cc) * v.w;
+      }
+    }
+  }
+  if (mine) {
+    const int box = leaf_box ? leaf_box[li] : leaf_box0 + li;
+    store_mult(M, Mb, (size_t)box * n3p + m, acc);
+  }
+}
+
+// k_l2p1r with the weights formed from xl.
+template <int NF>
+__global__ __launch_bounds__(256) void k_l2p1rx(int N, const int *__restrict__ panel_leaf,
+                                                const int *__restrict__ leaf_box, int leaf_box0,
+                                                const float4 *__restrict__ xl,
+                                                const float *__restrict__ wc,
+                                                const float *__restrict__ Lc,
+                                                double *__restrict__ y,
+                                                const unsigned char *__restrict__ skip) {
+  constexpr int n = NF, nn = n * n;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int li = panel_leaf[p];
+  if (skip && skip[li]) return;
+  const float *ll = Lc + (size_t)(leaf_box ? leaf_box[li] : leaf_box0 + li) * n * nn;
+  const float4 v = xl[p];
+  float wx[n], wy[n], wz[n];
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    float c[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) c[k] = wc[i * n + k];
+    wx[i] = cheb_w<n>(v.x, c);
+    wy[i] = cheb_w<n>(v.y, c);
+    wz[i] = cheb_w<n>(v.z, c);
+  }
+  float acc = 0;
+#pragma unroll
+  for (int a = 0; a < n; ++a) {
+    float sa = 0;
+#pragma unroll
+    for (int bb = 0; bb < n; ++bb) {
+      float sb = 0;
+#pragma unroll
+      for (int c = 0; c < n; ++c) sb += wz[c] * ll[a * nn + bb * n + c];
+      sa += wy[bb] * sb;
+    }
+    acc += wx[a] * sa;
+  }
+  y[p] += (double)acc;
+}
+
 // Single right-hand side L2P: one thread per panel, y[p] += sum over nodes of
 // S(p, node) Lc[leaf][node] (the same expression and order as k_l2p).
 template <typename T>
@@ -3115,8 +3282,31 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
       const int tpl = round_up(n3, n3 <= 32 ? 32 : 64), lpb = 256 / tpl;
       // one thread per (leaf, node), or (variant 1) one work-group per leaf
       // with its panels staged in LDS
-      tuned_launch(f32 ? "p2m1_f32" : "p2m1_f64", nleaf, n, 1, n >= 3 && n <= 6 ? 3 : 1, s,
-                   [&](int v) {
+      // 3, 4: k_p2m1x, k_p2m1mx (FP32, weights from the box coordinates)
+      const bool wx = sizeof(T) == 4 && f.xl && f.wc && n >= 3 && n <= 6;
+      tuned_launch(f32 ? "p2m1_f32" : "p2m1_f64", nleaf, n, 1,
+                   n >= 3 && n <= 6 ? (wx ? 5 : 3) : 1, s, [&](int v) {
+        if constexpr (sizeof(T) == 4) {
+          if (wx && v >= 3) {
+            const float4 *xl = reinterpret_cast<const float4 *>(f.xl);
+#define GP_P2M1X(NF)                                                                             \
+  if (v == 4)                                                                                    \
+    hipLaunchKernelGGL((k_p2m1mx<NF>), dim3((nleaf + 256 / tpl - 1) / (256 / tpl)), dim3(256), 0, \
+                       s, n3p, nleaf, f.leaf_lo, f.leaf_box, leaf0, xl, f.wc, z, M, mb,           \
+                       f.skip_leaf);                                                              \
+  else                                                                                           \
+    hipLaunchKernelGGL((k_p2m1x<NF>), dim3((nleaf + lpb - 1) / lpb), dim3(256), 0, s, n3p, tpl,   \
+                       nleaf, f.leaf_lo, f.leaf_box, leaf0, xl, f.wc, z, M, mb, f.skip_leaf)
+            switch (n) {
+              case 3: GP_P2M1X(3); break;
+              case 4: GP_P2M1X(4); break;
+              case 5: GP_P2M1X(5); break;
+              default: GP_P2M1X(6); break;
+            }
+#undef GP_P2M1X
+            return;
+          }
+        }
         switch (n) {
 #define GP_P2M1(NF)                                                                              \
   if (v == 2 && NF > 0)                                                                          \
@@ -3518,10 +3708,28 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                          f.panel_leaf, f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf);
   } else if (ldm == 1 && nr == 1 && ldy == 1 && f.panel_leaf && n >= 2 && n <= 6) {
     const dim3 grid((f.n_panels + 255) / 256);
+    // (FP32 with box coordinates: the weights formed on the fly)
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+      if (f.xl && f.wc && n >= 3) {
+        const float4 *xl = reinterpret_cast<const float4 *>(f.xl);
+#define GP_L2P1RX(NF)                                                                             \
+  hipLaunchKernelGGL((k_l2p1rx<NF>), grid, dim3(256), 0, s, f.n_panels, f.panel_leaf, f.leaf_box, \
+                     leaf0, xl, f.wc, Lc, y, f.skip_leaf)
+        switch (n) {
+          case 3: GP_L2P1RX(3); break;
+          case 4: GP_L2P1RX(4); break;
+          case 5: GP_L2P1RX(5); break;
+          default: GP_L2P1RX(6); break;
+        }
+#undef GP_L2P1RX
+        done = true;
+      }
+    }
 #define GP_L2P1R(NF)                                                                              \
   hipLaunchKernelGGL((k_l2p1r<T, NF>), grid, dim3(256), 0, s, f.n_panels, f.panel_leaf,          \
                      f.leaf_box, leaf0, f.w1d, Lc, y, f.skip_leaf)
-    switch (n) {
+    if (!done) switch (n) {
       case 2: GP_L2P1R(2); break;
       case 3: GP_L2P1R(3); break;
       case 4: GP_L2P1R(4); break;

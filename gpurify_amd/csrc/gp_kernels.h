@@ -157,6 +157,10 @@ struct FmmDev {
   // the profiler's share of the work done while boxes are skipped
   double frac_panels = 1.0, frac_boxes = 1.0, frac_pairs = 1.0;
   const T *w1d = nullptr;            // per sorted panel 3 * n
+  // FP32 single-RHS operator: per panel (x, y, z, area) in its leaf box's
+  // frame and the n x n weight coefficients, for P2M / L2P forming the
+  // weights on the fly (null: they read w1d)
+  const float *xl = nullptr, *wc = nullptr;
   const T *m2m1d = nullptr;          // 24 * n * n
   const T *G = nullptr;              // 343 * n3p * n3p (zero padded)
   const void *Gbf = nullptr;         // same, BF16 (FP32 operator only)
@@ -202,6 +206,9 @@ struct FmmDev {
 // leaf box: w1d[i][d][m] for axis d and node m (3 n per panel), FP64 and, when
 // w32 is set, FP32.  geo: per leaf the box centre (3) and half-width in the
 // problem's unit frame; cen: scaled centroids; nodes: the n Chebyshev nodes.
+// xl[4 p ..] = panel p's centroid in its leaf box's frame and its area (FP32).
+void launch_local_coords(int N, const int *panel_leaf, const double *geo, const double *cen,
+                         const double *area, float *xl, hipStream_t s);
 void launch_interp_w1d(int N, int n, const int *panel_leaf, const double *geo, const double *cen,
                        const double *nodes, double *w64, float *w32, hipStream_t s);
 // d/dx of the 1-D interpolation weights over the leaf half-width (3n per panel).

@@ -449,7 +449,7 @@ struct FmmSystem {
   DBuf<double> inv_half;
   DBuf<double> w1d64, m2m64, G64, lr_vt64, lr_up64, lr_u64;
   DBuf<int> lr_rank;
-  DBuf<float> w1d32, m2m32, G32, Gr32;
+  DBuf<float> w1d32, m2m32, G32, Gr32, xl32, wc32;
   DBuf<uint16_t> Gbf;
   // Low-rank inner M2L: rank kr (0 = off), projection U and kr x kr tables.
   int kr = 0;
@@ -1153,6 +1153,24 @@ struct FmmSystem {
         dw1d64.alloc((size_t)N * 3 * n);
         launch_interp_dw1d(N, n, panel_leaf.p, dgeo.p, cen.p, dnodes.p, dw1d64.p, s);
       }
+      // single RHS: the FP32 operator's P2M and L2P form the weights from
+      // the panels' box coordinates (16 bytes a panel instead of 3 n weights
+      // and the FP64 area)
+      if (need32 && want_by_offset && n_in >= 2 && n_in <= 6) {
+        xl32.alloc((size_t)N * 4);
+        launch_local_coords(N, panel_leaf.p, dgeo.p, cen.p, area.p, xl32.p, s);
+        std::vector<float> hwc((size_t)n_in * n_in);
+        for (int m = 0; m < n_in; ++m) {
+          const double xm = nodes_in[m];
+          double tkm1 = 1.0, tk = xm;
+          for (int k = 0; k < n_in; ++k) {
+            const double t = k == 0 ? 1.0 : k == 1 ? xm : 2.0 * xm * tk - tkm1;
+            if (k >= 2) tkm1 = tk, tk = t;
+            hwc[(size_t)m * n_in + k] = (float)(2.0 / n_in * (k == 0 ? 0.5 : 1.0) * t);
+          }
+        }
+        wc32 = to_device(hwc, s);
+      }
       GP_HIP(hipStreamSynchronize(s));  // dgeo and dnodes are freed below
     }
     m2m64 = to_device(hm2m, s);
@@ -1267,6 +1285,8 @@ struct FmmSystem {
     f32.n = n_in;
     f32.n3p = n3p_in;
     f32.w1d = w1d32.p;
+    f32.xl = xl32.p;
+    f32.wc = wc32.p;
     f32.m2m1d = m2m32.p;
     f32.G = G32.p;
     f32.Gbf = Gbf.p;
