@@ -2095,9 +2095,9 @@ void launch_m2l_tiles_f64lrs(const FmmDev<double> &f, int n3, const double *M, d
                              int xrun, bool pb, hipStream_t s) {
   const dim3 grid(f.n_tiles);
   const int kr = f.lr_kmax, NP = f.n3p;
-  if (kr < 1 || kr > 32 || kr > f.lr_kp)
-    throw Error(GP_ERR_INTERNAL, "staged factored FP64 M2L tiles take ranks up to 32");
   const size_t lds = 2 * ((size_t)kr * (NP + 2) + (size_t)NP * (kr + 1)) * sizeof(double);
+  if (kr < 1 || kr > 32 || kr > f.lr_kp || lds > 65536)
+    throw Error(GP_ERR_INTERNAL, "staged factored FP64 M2L tiles take ranks up to 32 and 64 KB");
 #define GP_LRS(MT_, KRB_)                                                                        \
   if (pb) hipLaunchKernelGGL((k_m2l1t64_f64lrs<MT_, KRB_, true>), grid, dim3(256), lds, s, n3,     \
                      f.n_tiles, xrun, f.tile_tgt, f.tile_eoff, f.ent_oidx, f.ent_src,             \
@@ -3521,8 +3521,14 @@ void run_far_field_t(const FmmDev<T> &f, const T *z, int ldz, int nr, T *M, __bf
                      ((double)f.n_pairs * m3p * 8 + f.n_targets * mbytes) * f.frac_pairs);
         // 2, 3: the truncated blocks applied in factored form, by independent
         // wavefronts; 4-7: the same with the factors staged in LDS (4, 5
-        // with the sources one entry ahead in registers, 6, 7 without)
-        tuned_launch("m2l1t_f64", f.n_tiles, m3p, f.tile_size, f.lr_vt ? 8 : 2, s, [&](int v) {
+        // with the sources one entry ahead in registers, 6, 7 without), where
+        // the ranks and the staging buffers fit
+        const bool lr_ok = f.lr_vt && (f.lr_kp == 16 || f.lr_kp == 32);
+        const size_t lrs_lds =
+            2 * ((size_t)f.lr_kmax * (m3p + 2) + (size_t)m3p * (f.lr_kmax + 1)) * sizeof(double);
+        const bool lrs_ok = lr_ok && f.lr_kmax >= 1 && f.lr_kmax <= 32 && lrs_lds <= 65536;
+        tuned_launch("m2l1t_f64", f.n_tiles, m3p, f.tile_size, lrs_ok ? 8 : lr_ok ? 4 : 2, s,
+                     [&](int v) {
           if (v >= 4) launch_m2l_tiles_f64lrs(fm, m3, Mo, Lo, v % 2 ? 16 : 0, v < 6, s);
           else if (v >= 2) launch_m2l_tiles_f64lr(fm, m3, Mo, Lo, v == 3 ? 16 : 0, s);
           else launch_m2l_tiles_f64(fm, m3, Mo, Lo, v ? 16 : 0, s);
