@@ -132,7 +132,9 @@ typedef struct {
                             interaction matrix compressed by column-pivoted QR, truncated where the
                             remaining squared column norm falls below eps = max(1e-3 * 0.1^n,
                             1e-14) of the largest (bbfmm.rs:132-174, linalg.rs:277-381), and
-                            applied as the dense product U V^T; 0 = the full matrices.  The
+                            applied in factored form, U (V^T x), as the reference applies it, by
+                            the single-RHS FP64 tiles (the dense product U V^T for blocks of RHS
+                            and where the autotuner prefers it); 0 = the full matrices.  The
                             adjoint solves of gp_cap_jacobian always apply the full matrices. */
   uint32_t fuse_leaf;    /* 1 (default) = FP32 blocks of RHS at orders <= 5 run the finest L2L and
                             L2P as one kernel; 0 = two launches (the same sums). */
