@@ -231,8 +231,10 @@ int gp_ctx_create(gp_ctx **out, uint32_t device_ordinal) {
     c->cu_count = cus;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->setup_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->asm_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       if (c->stream) (void)hipStreamDestroy(c->stream);
+      if (c->setup_stream) (void)hipStreamDestroy(c->setup_stream);
       delete c;
       gp::hip_check(e, "hipStreamCreateWithFlags");
     }
@@ -252,6 +254,7 @@ void gp_ctx_destroy(gp_ctx *ctx) {
     gp::device_trim();  // the cached device memory goes back with the context
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->setup_stream) (void)hipStreamDestroy(ctx->setup_stream);
+    if (ctx->asm_stream) (void)hipStreamDestroy(ctx->asm_stream);
   }
   delete ctx;
 }

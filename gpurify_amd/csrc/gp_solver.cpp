@@ -496,9 +496,12 @@ struct FmmSystem {
     setup(one, o, need32, stream);
   }
 
+  // sa: the stream of the near-field assembly and the kernels that read its
+  // values (null: stream); the caller synchronises it before the solve
   void setup(const std::vector<const PanelGeom *> &probs, const gp_cap_opts &o, bool need32,
-             hipStream_t stream) {
+             hipStream_t stream, hipStream_t sa = nullptr) {
     s = stream;
+    if (!sa) sa = stream;
     n = (int)o.fmm_order;
     if (n < 2 || n > 6) throw Error(GP_ERR_INVALID_INPUT, "fmm_order must be in [2, 6]");
     m2l_inner = (int)o.m2l_inner;
@@ -678,6 +681,7 @@ struct FmmSystem {
         hnf[T + 1] = hnf[T] + (int64_t)nt_ * ccount[T];
       }
     }
+    phase("near count");
     hcols.reset(new int[std::max(1, hcol_off[n_leaves])]);
     parallel_for(n_leaves, [&](int T) {
       int p, lt;
@@ -726,26 +730,34 @@ struct FmmSystem {
       ntr.base = tr_base.p;
       ntr.stride = tr_stride.p;
     }
+    phase("near cols");
     leaf_lo = to_device(hleaf_lo, s);
     leaf_box = to_device(hleaf_box, s);
     col_off = to_device(hcol_off, s);
     cols = to_device(hcols.get(), (size_t)hcol_off[n_leaves], s);
     panel_leaf = to_device(hpl, s);
     nf_off = to_device(hnf, s);
+    phase("near upload");
     nf64.alloc((size_t)nnz);
     have32 = need32;
     if (o.near_inner != GP_NEAR_FP32 && o.near_inner != GP_NEAR_FP16)
       throw Error(GP_ERR_INVALID_INPUT, "near_inner must be GP_NEAR_FP32 or GP_NEAR_FP16");
     near16 = need32 && want_by_offset && o.near_inner == GP_NEAR_FP16 && !want_transpose && !diel;
     if (need32 && !near16) nf32.alloc((size_t)nnz);
+    phase("near alloc");
+    // (on sa: every input above is on the device, setup_stream synchronised
+    // by its uploads; nothing below reads the values but on sa)
+    if ((diel || want_transpose) && sa != s)
+      throw Error(GP_ERR_INTERNAL, "dielectric and adjoint set-ups assemble on their own stream");
     launch_near_assemble(n_leaves, leaf_lo.p, nf_off.p, col_off.p, cols.p, verts.p, cen.p, nf64.p,
-                         (need32 && !near16) ? nf32.p : nullptr, s);
+                         (need32 && !near16) ? nf32.p : nullptr, sa);
+    phase("near launch");
     inv_diag.alloc(N);
     const double k_e = 1.0 / (4.0 * kPi * kEps0);
     for (int p = 0; p < P; ++p) {
       const int o0 = panel_off[p];
       launch_self_inv_diag(probs[p]->n, verts.p + (size_t)o0 * 12, cen.p + (size_t)o0 * 3,
-                           k_e * L0s[p], inv_diag.p + o0, s);
+                           k_e * L0s[p], inv_diag.p + o0, sa);
     }
     if (diel) {
       // interface rows of the near tiles and their Jacobi entries, in the
@@ -780,6 +792,7 @@ struct FmmSystem {
       dinv_area = to_device(hdi, s);
       inv_area = to_device(hia, s);
     }
+    phase("near diag");
     nt.n = N;
     nt.nnz = (long)nnz;
     nt.n_leaves = n_leaves;
@@ -813,7 +826,7 @@ struct FmmSystem {
     if (near16) {
       nf16.alloc((size_t)nnz);
       nscale.alloc(N);
-      launch_near_to_f16(nt, nf64.p, nf16.p, nscale.p, s);
+      launch_near_to_f16(nt, nf64.p, nf16.p, nscale.p, sa);
     }
 
     phase("near");
@@ -2111,8 +2124,9 @@ void stage_group(gp_ctx *ctx, const gp_cap_opts &o, bool mixed, bool profile, St
     upload_seconds() = 0.0;
     st.sys.reset(new FmmSystem());
     st.sys->want_by_offset = true;
-    st.sys->setup(st.probs, o, mixed, ctx->setup_stream);
+    st.sys->setup(st.probs, o, mixed, ctx->setup_stream, ctx->asm_stream);
     GP_HIP(hipStreamSynchronize(ctx->setup_stream));
+    GP_HIP(hipStreamSynchronize(ctx->asm_stream));
   } catch (...) {
     st.err = std::current_exception();
   }

@@ -18,6 +18,9 @@ struct gp_ctx {
   // uploads and set-up kernels of a batch's next group of problems, issued
   // by a second host thread while `stream` solves the current group
   hipStream_t setup_stream = nullptr;
+  // the near-field assembly of that next group, so the set-up's later
+  // uploads (each synchronising setup_stream) need not wait for it
+  hipStream_t asm_stream = nullptr;
   std::mutex mu;
   std::string last_error;
   std::string device_name;
