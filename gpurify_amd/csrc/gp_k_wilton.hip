@@ -562,14 +562,23 @@ void launch_near_dielectric_rows(int n_leaves, int n, const int *leaf_lo, const 
 }
 
 // Self term P'_ii for the Jacobi preconditioner (cap/fmm_solver.rs:150-162).
-// inv = 1/d unless |k_e L0 d| <= 1e-30, the reference's threshold in SI units.
-__global__ void k_self_inv_diag(int n, const double *__restrict__ verts,
-                                const double *__restrict__ cen, double si_scale,
-                                double *__restrict__ inv) {
+// inv = 1/d unless |k_e L0 d| <= 1e-30, the reference's threshold in SI units;
+// panel i belongs to problem p with off[p] <= i < off[p + 1] (off[0] = 0,
+// off[np] = n), whose threshold scale k_e L0 is si_scale[p].
+__global__ void k_self_inv_diag(int n, int np, const int *__restrict__ off,
+                                const double *__restrict__ si_scale,
+                                const double *__restrict__ verts,
+                                const double *__restrict__ cen, double *__restrict__ inv) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  int lo = 0, hi = np;  // off[lo] <= i < off[hi]
+  while (hi - lo > 1) {
+    const int m = (lo + hi) >> 1;
+    if (off[m] <= i) lo = m;
+    else hi = m;
+  }
   double d = wilton(verts + 12L * i, cen[3 * i], cen[3 * i + 1], cen[3 * i + 2]);
-  inv[i] = fabs(d * si_scale) > 1e-30 ? 1.0 / d : 1.0;
+  inv[i] = fabs(d * si_scale[lo]) > 1e-30 ? 1.0 / d : 1.0;
 }
 
 // P'_p[k][l] = I_l(c_k) for every problem p of a batch (blockIdx.y), its
@@ -654,10 +663,11 @@ void launch_near_assemble(int n_leaves, const int *leaf_lo, const int64_t *nf_of
   GP_LAUNCH_CHECK();
 }
 
-void launch_self_inv_diag(int n, const double *verts, const double *cen, double si_scale,
-                          double *inv, hipStream_t s) {
-  hipLaunchKernelGGL(k_self_inv_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, verts, cen,
-                     si_scale, inv);
+void launch_self_inv_diag(int n, int np, const int *off, const double *si_scale,
+                          const double *verts, const double *cen, double *inv, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_self_inv_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, np, off,
+                     si_scale, verts, cen, inv);
   GP_LAUNCH_CHECK();
 }
 

@@ -27,8 +27,10 @@ void launch_near_dielectric_rows(int n_leaves, int n, const int *leaf_lo, const 
                                  const int *col_off, const int *cols, const unsigned char *iface,
                                  const double *nrm, const double *diag, const double *verts,
                                  const double *cen, double *v64, double *inv_diag, hipStream_t s);
-void launch_self_inv_diag(int n, const double *verts, const double *cen, double si_scale,
-                          double *inv, hipStream_t s);
+// Jacobi entries of every problem of a batch in one launch: off (np + 1
+// panel offsets) and si_scale (np threshold scales) are device arrays.
+void launch_self_inv_diag(int n, int np, const int *off, const double *si_scale,
+                          const double *verts, const double *cen, double *inv, hipStream_t s);
 
 // Geometric sensitivities of C' for a group of nd (1 or 2) parameters that
 // move the same panels (device arrays; scaled units).  dv: direction t at

@@ -434,6 +434,8 @@ struct FmmSystem {
   DBuf<double> sgn, qw;
   DBuf<uint32_t> cond;
   DBuf<int> leaf_lo, leaf_box, col_off, cols, panel_leaf;
+  DBuf<int> d_panel_off;    // problems' first panels in ascending order, then N
+  DBuf<double> d_si_scale;  // k_e L0 of the same problems: the Jacobi threshold's SI scale
   DBuf<int64_t> nf_off;
   DBuf<double> nf64;
   DBuf<float> nf32;
@@ -731,6 +733,22 @@ struct FmmSystem {
       ntr.stride = tr_stride.p;
     }
     phase("near cols");
+    {
+      // the problems' panel ranges in device order (ascending offsets) and
+      // their threshold scales, for the one Jacobi launch below
+      std::vector<int> ord(P), ho(P + 1);
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](int a, int b) { return panel_off[a] < panel_off[b]; });
+      std::vector<double> hsc(P);
+      for (int k = 0; k < P; ++k) {
+        ho[k] = panel_off[ord[k]];
+        hsc[k] = L0s[ord[k]] / (4.0 * kPi * kEps0);
+      }
+      ho[P] = N;
+      d_panel_off = to_device(ho, s);
+      d_si_scale = to_device(hsc, s);
+    }
     leaf_lo = to_device(hleaf_lo, s);
     leaf_box = to_device(hleaf_box, s);
     col_off = to_device(hcol_off, s);
@@ -753,12 +771,7 @@ struct FmmSystem {
                          (need32 && !near16) ? nf32.p : nullptr, sa);
     phase("near launch");
     inv_diag.alloc(N);
-    const double k_e = 1.0 / (4.0 * kPi * kEps0);
-    for (int p = 0; p < P; ++p) {
-      const int o0 = panel_off[p];
-      launch_self_inv_diag(probs[p]->n, verts.p + (size_t)o0 * 12, cen.p + (size_t)o0 * 3,
-                           k_e * L0s[p], inv_diag.p + o0, sa);
-    }
+    launch_self_inv_diag(N, P, d_panel_off.p, d_si_scale.p, verts.p, cen.p, inv_diag.p, sa);
     if (diel) {
       // interface rows of the near tiles and their Jacobi entries, in the
       // device (leaf-sorted) order
