@@ -562,6 +562,33 @@ def measured_traffic(name, workload):
     return None
 
 
+def one_group_traffic(name, workload):
+    """For a line whose launches no counter run covers (the default groups
+    dispatch from two host threads and are not profiled), the same kernel's
+    counter figures on the workload in one group (`<workload>_g1`): bytes per
+    launch and their ratio to the launch's algorithmic bytes (a line
+    `traffic_over_algorithmic: <symbol> <workload> <ratio>`), or None."""
+    sym = PROFILE_SYMBOLS.get(name)
+    pdir = os.path.join(ROOT, "profiles")
+    g1 = workload + "_g1"
+    if not sym or workload.endswith("_g1") or not os.path.isdir(pdir):
+        return None
+    got = {}
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, fn)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[1] == sym and parts[2] == g1 and parts[0] in (
+                        "traffic_bytes_per_launch:", "traffic_over_algorithmic:"):
+                    got[parts[0]] = float(parts[3])
+    if "traffic_bytes_per_launch:" not in got:
+        return None
+    return {"workload": g1, "bytes_per_launch": got["traffic_bytes_per_launch:"],
+            "over_algorithmic": got.get("traffic_over_algorithmic:")}
+
+
 def autotune_choices(ctx):
     """The context's autotuned kernel variants: kernel/key -> [variant, ms of each]."""
     return {f"{e['kernel']}/{'x'.join(str(k) for k in e['key'])}": [e["variant"], e["ms"]]
@@ -630,7 +657,8 @@ def roofline_of(name, prof, work, workload):
     return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "avg_launch_ms": 1e3 * avg, "launches": L,
             "work_per_launch": per, "work_basis": basis,
-            "traffic": measured_traffic(name, workload)}
+            "traffic": measured_traffic(name, workload),
+            "traffic_one_group": one_group_traffic(name, workload)}
 
 
 def report_profile(out, prof, work, workload):
