@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--order", type=int, default=0, help="0 = the workload's own")
     ap.add_argument("--leaf-target", type=int, default=0, help="0 = the workload's own (64)")
     ap.add_argument("--tol", type=float, default=0.0, help="0 = the workload's own")
-    ap.add_argument("--inner-tol", type=float, default=1e-2)
+    ap.add_argument("--inner-tol", type=float, default=0.0, help="0 = the workload's own (1e-2)")
     ap.add_argument("--inner-order", type=int, default=-1,
                     help="Chebyshev order of the inner operator's far field (0 = --order; "
                          "-1 = the workload's own)")
@@ -93,6 +93,7 @@ def parse():
     if a.inner_order < 0:
         a.inner_order = w.get("inner_order", 0)
     a.leaf_target = a.leaf_target or w.get("leaf_target", 64)
+    a.inner_tol = a.inner_tol or w.get("inner_tol", 1e-2)
     return a
 
 
@@ -126,9 +127,13 @@ WORKLOADS = {
     # converged answer (gate on every entry); leaves of ~112 panels: fewer
     # M2L pairs for a longer near field (mixed unknowns/s over two steps with
     # the factored FP64 M2L: 7.19M at 64, 7.88M at 80, 8.17M at 96, 8.40M at
-    # 112, 8.31M at 128, 8.29M at 144, 8.13M at 160, 7.35M at 192)
+    # 112, 8.31M at 128, 8.29M at 144, 8.13M at 160, 7.35M at 192); inner
+    # GMRES to 3e-3: 5.2 refinement steps instead of 6 for 6 % more inner
+    # iterations (mixed unknowns/s over five steps, three runs each: 9.06 /
+    # 9.22 / 9.35M at 1e-2, 9.43 / 9.28 / 9.39M at 3e-3, 9.35M at 5e-3, 8.59M
+    # at 1e-3, 9.17M at 3e-2, 8.49M at 1e-1)
     "nets4096": {"groups": None, "refine": 1, "order": 4, "rhs": 1, "name": "nets4096_met1met2",
-                 "cpu": None, "inner_order": 3, "leaf_target": 112},
+                 "cpu": None, "inner_order": 3, "leaf_target": 112, "inner_tol": 3e-3},
     # adjoint sensitivities of the 16 x 16 crossing bundle (W = S = 140 nm)
     "bundle16": {"groups": None, "refine": 5, "order": 5, "rhs": None, "name": "bundle16x16_met1met2",
                  "cpu": None},
@@ -229,7 +234,7 @@ def bench_layout(args):
                          "octree_depth": last["max_level"], "m2l": "cpqr",
                          "leaf_target": args.leaf_target, "tol": args.tol,
                          "m2l_rank": int(o_mix.m2l_rank),
-                         "inner_order": args.inner_order or args.order,
+                         "inner_order": args.inner_order or args.order, "inner_tol": args.inner_tol,
                          "near_nnz": last["near_nnz"], "m2l_pairs": last["m2l_pairs"],
                          "iterations": last["iterations"], "refine_steps": last["refine_steps"],
                          "converged": bool(last["converged"]), "global_batch": nrhs}
@@ -331,7 +336,7 @@ def bench_batch(args):
         pairs = int(sum(r["m2l_pairs"] for r in fm))
         out["config"] = {"workload": "nets4096_met1met2", "nets": len(probs), "panels": n,
                          "rhs_per_net": 1, "fmm_order": args.order, "m2l": "cpqr",
-                         "inner_order": args.inner_order or args.order,
+                         "inner_order": args.inner_order or args.order, "inner_tol": args.inner_tol,
                          "leaf_target": args.leaf_target, "tol": args.tol,
                          "dense_nets": len(probs) - len(fm), "fmm_nets": len(fm),
                          "fmm_panels": nfmm, "max_depth": max(r["max_level"] for r in fm),
