@@ -35,6 +35,7 @@ the same workload on the host cores.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -570,13 +571,13 @@ def measured_traffic(name, workload):
 def one_group_traffic(name, workload):
     """For a line whose launches no counter run covers (the default groups
     dispatch from two host threads and are not profiled), the same kernel's
-    counter figures on the workload in one group (`<workload>_g1`): bytes per
+    counter figures on the workload in one group (`<name>_g1`): bytes per
     launch and their ratio to the launch's algorithmic bytes (a line
     `traffic_over_algorithmic: <symbol> <workload> <ratio>`), or None."""
     sym = PROFILE_SYMBOLS.get(name)
     pdir = os.path.join(ROOT, "profiles")
-    g1 = workload + "_g1"
-    if not sym or workload.endswith("_g1") or not os.path.isdir(pdir):
+    g1 = re.sub(r"_g\d+$", "", workload) + "_g1"
+    if not sym or workload == g1 or not os.path.isdir(pdir):
         return None
     got = {}
     for fn in sorted(os.listdir(pdir)):
