@@ -132,7 +132,9 @@ WORKLOADS = {
     # GMRES to 3e-3: 5.2 refinement steps instead of 6 for 6 % more inner
     # iterations (mixed unknowns/s over five steps, three runs each: 9.06 /
     # 9.22 / 9.35M at 1e-2, 9.43 / 9.28 / 9.39M at 3e-3, 9.35M at 5e-3, 8.59M
-    # at 1e-3, 9.17M at 3e-2, 8.49M at 1e-1)
+    # at 1e-3, 9.17M at 3e-2, 8.49M at 1e-1); at 3e-3, leaves of 96 / 112 /
+    # 128 give 9.32M / 9.29M / 9.23M (means of five, five and two runs of
+    # five steps: equal within the spread)
     "nets4096": {"groups": None, "refine": 1, "order": 4, "rhs": 1, "name": "nets4096_met1met2",
                  "cpu": None, "inner_order": 3, "leaf_target": 112, "inner_tol": 3e-3},
     # adjoint sensitivities of the 16 x 16 crossing bundle (W = S = 140 nm)
